@@ -1,0 +1,155 @@
+/* phg_hip.h -- C ABI of the MI355X (gfx950) finite-element hot path.
+ *
+ * The library libphg_hip.so implements, as hand-written HIP kernels, the part
+ * of PHG that the element loop and the Krylov solve run through:
+ *   element integration  quad.c:5260 phgQuadGradBasAGradBas,
+ *                        quad.c:5582 phgQuadBasABas,
+ *                        quad.c:5481 phgQuadDofTimesBas,
+ *                        geom.c:103  compute_jacobian,
+ *   row insertion        matvec.c:5348 phgMatAddEntries_ / :3045
+ *                        add_columns_to_row (insertion order, exact zeros
+ *                        dropped), matvec.c:5661 phgVecAddEntries_,
+ *   Dirichlet handling   matvec.c:4430 save_bdry_columns, solver.c:508
+ *                        update_rhs,
+ *   Krylov               matvec.c:552 phgMatVec, matvec.c:1193 phgVecDot,
+ *                        matvec.c:51 phgVecNorm2, matvec.c:1237 phgVecAXPBY,
+ *                        solver-phg.c:485 solver_pcg with
+ *                        solver-phg.c:207 jacobi_pc.
+ * PHG's own element loop (examples/poisson.c:101-170) calls the quad
+ * functions once per basis pair; here the loop is lowered to one batched
+ * call (phg_hip_assemble) over all elements, which produces the same CSR
+ * storage order and the same values as PHG's row lists.
+ *
+ * Conventions: every function returns PHG_HIP_OK (0) or a negative error
+ * code; phg_hip_last_error() gives the message.  Host pointers are plain
+ * arrays; the library owns all device memory behind the opaque handles.
+ * All work is ordered on one HIP stream per process (phg_hip_init).
+ */
+#ifndef PHG_HIP_H
+#define PHG_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PHG_HIP_OK              0
+#define PHG_HIP_E_HIP         (-1)   /* a HIP runtime call failed */
+#define PHG_HIP_E_ARG         (-2)   /* invalid argument / size */
+#define PHG_HIP_E_NOMEM       (-3)   /* device allocation failed */
+#define PHG_HIP_E_ZERO_DIAG   (-4)   /* solver-phg.c:233 zero diagonal */
+#define PHG_HIP_E_STATE       (-5)   /* library not initialised */
+
+typedef struct phg_hip_space phg_hip_space;   /* mesh + FE space on device */
+typedef struct phg_hip_mat   phg_hip_mat;     /* assembled CSR system     */
+typedef struct phg_hip_vec   phg_hip_vec;     /* FP64 device vector       */
+
+/* Problem selector for the element integrals (the integrand a driver such
+ * as examples/poisson.c, simplest.c or elastic.c builds). */
+typedef struct {
+    int32_t dim;          /* 1: scalar (poisson.c), 3: elasticity (elastic.c) */
+    int32_t use_kappa;    /* scalar: phgQuadGradBasAGradBas with a P0 kappa */
+    double  mass_coef;    /* scalar: + mass_coef * phgQuadBasDotBas (simplest.c:70) */
+    int32_t rhs_order;    /* quadrature order of the load, <0: 2*order */
+    double  E, nu;        /* elasticity moduli (elastic.c:39-40) */
+} phg_hip_problem;
+
+/*--------------------------------------------------------------- runtime */
+
+/* Replaces the GPU side of utils.c:555 phgInit: selects the device and
+ * creates the library stream. */
+int phg_hip_init(int32_t device);
+/* utils.c:681 phgFinalize counterpart: releases the stream. */
+int phg_hip_finalize(void);
+const char *phg_hip_last_error(void);
+int phg_hip_synchronize(void);
+
+/*---------------------------------------------------------- mesh / space */
+
+/* Uploads the mesh and the Lagrange Pk space (order 1..3):
+ * verts[nvert][3] (utils.c:5470 phgImport_ result), elem_verts[nelem][4],
+ * map[nelem][nbas] = dof.c:4821 phgDofMapE2D_ of every local basis,
+ * bdry[ndof] = 1 for Dirichlet DOFs (dof-utils.c:1269 nodal path). */
+int phg_hip_space_create(int32_t order, int64_t nvert, const double *verts,
+                         int64_t nelem, const int32_t *elem_verts,
+                         const int32_t *map, int64_t ndof, const uint8_t *bdry,
+                         phg_hip_space **sp);
+int phg_hip_space_destroy(phg_hip_space *sp);
+
+/*--------------------------------------------------------------- assembly */
+
+/* Element matrices only, copied to the host: K[nelem][M][M], b[nelem][M]
+ * with M = nbas*dim, exactly as the drivers hold them before rows are
+ * inserted (replaces the per-pair calls of quad.c:5260 / :5582 / :5481 in
+ * examples/poisson.c:124-150 and elastic.c:160-196). kappa[nelem] may be
+ * NULL; f[ndof*dim] is the load DOF (nodal interpolant). */
+int phg_hip_element_matrices(phg_hip_space *sp, const phg_hip_problem *pb,
+                             const double *kappa, const double *f,
+                             double *K, double *b);
+
+/* Full assembly: element integrals, PHG row insertion (matvec.c:5348),
+ * Dirichlet rows e_i with right-hand side g (dof-utils.c:1369) and RHS
+ * accumulation (matvec.c:5661).  g[ndof*dim] holds the Dirichlet values.
+ * The result is the matrix before phgMatAssemble's boundary treatment. */
+int phg_hip_assemble(phg_hip_space *sp, const phg_hip_problem *pb,
+                     const double *kappa, const double *f, const double *g,
+                     phg_hip_mat **A);
+
+/* phgSolverAssemble + phgSolverUpdateRHS on one process
+ * (matvec.c:4602 phgMatAssemble -> :4430 save_bdry_columns(TRUE),
+ * solver.c:734 -> :508 update_rhs). */
+int phg_hip_mat_eliminate(phg_hip_mat *A);
+
+int phg_hip_mat_info(const phg_hip_mat *A, int64_t *nrows, int64_t *nnz);
+/* matvec.c:6134 phgMatDumpCSR equivalent: copies the CSR (0-based, row
+ * pointers widened to int64) and the right-hand side to the host. */
+int phg_hip_mat_download(const phg_hip_mat *A, int64_t *rowptr, int32_t *cols,
+                         double *vals, double *rhs);
+int phg_hip_mat_destroy(phg_hip_mat *A);
+
+/*---------------------------------------------------------------- vectors */
+
+/* matvec.c:2690 phgMapCreateVec counterpart (one FP64 vector of n). */
+int phg_hip_vec_create(int64_t n, phg_hip_vec **v);
+int phg_hip_vec_destroy(phg_hip_vec *v);
+int phg_hip_vec_upload(phg_hip_vec *v, const double *host);
+int phg_hip_vec_download(const phg_hip_vec *v, double *host);
+/* The assembled right-hand side of A as a vector (copy). */
+int phg_hip_mat_rhs(const phg_hip_mat *A, phg_hip_vec *b);
+
+/* matvec.c:552 phgMatVec(MAT_OP_N, alpha, A, x, beta, &y). */
+int phg_hip_matvec(double alpha, const phg_hip_mat *A, const phg_hip_vec *x,
+                   double beta, phg_hip_vec *y);
+/* matvec.c:1193 phgVecDot, :51 phgVecNorm2, :1237 phgVecAXPBY, :157 phgVecCopy */
+int phg_hip_vec_dot(const phg_hip_vec *x, const phg_hip_vec *y, double *res);
+int phg_hip_vec_norm2(const phg_hip_vec *x, double *res);
+int phg_hip_vec_axpby(double a, const phg_hip_vec *x, double b, phg_hip_vec *y);
+int phg_hip_vec_copy(const phg_hip_vec *src, phg_hip_vec *dst);
+
+/*---------------------------------------------------------------- solvers */
+
+/* solver-phg.c:2430 PCG_Solve / :485 solver_pcg with the Jacobi PC of
+ * solver-phg.c:207, FP64 throughout: x is the initial guess on input and
+ * the solution on output, b the right-hand side.  Stops when
+ * |r| <= max(rtol*|r0|, btol*|b|, atol) or after maxit iterations.
+ * Returns the iteration count (>= 0) or an error code. */
+int phg_hip_pcg(const phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x,
+                int32_t maxit, double rtol, double btol, double atol,
+                double *residual);
+
+/* Mixed precision (no reference counterpart): FP64 iterative refinement
+ * r = b - A x in FP64, correction from Jacobi-PCG in FP32 arithmetic on an
+ * FP32 copy of A with FP64 dot products, x += d in FP64.  Stops on the
+ * FP64 residual with the same tolerance rule as phg_hip_pcg.  inner_rtol is
+ * the relative reduction asked of each FP32 solve.  *inner_its receives the
+ * total FP32 iterations.  Returns the number of refinement steps. */
+int phg_hip_pcg_mixed(const phg_hip_mat *A, const phg_hip_vec *b,
+                      phg_hip_vec *x, int32_t maxit, double rtol, double btol,
+                      double atol, double inner_rtol, int32_t *inner_its,
+                      double *residual);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,734 @@
+/* Oracle: CPU restatement of PHG's finite-element hot path (test
+ * infrastructure only; see phg_oracle.h).  Built with -ffp-contract=off. */
+#include "phg_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+static void *xmalloc(size_t n)
+{
+    void *p = malloc(n ? n : 1);
+    if (p == NULL) {
+        fprintf(stderr, "oracle: out of memory\n");
+        abort();
+    }
+    return p;
+}
+
+/*------------------------------ quadrature --------------------------------*/
+/* quad.c:1799-2000 static 3D rules, permutations of
+ * include/phg/quad-permu.h:83-120 in the same point order. */
+
+#define NQMAX 24
+typedef struct {
+    int np;
+    double pts[NQMAX][4];
+    double wts[NQMAX];
+} qrule;
+
+static qrule rules[7];
+static int rules_ready = 0;
+
+static void add31(qrule *q, double a, double w)
+{
+    double b = 1.0 - 3.0 * a;
+    double P[4][4] = {{b, a, a, a}, {a, b, a, a}, {a, a, b, a}, {a, a, a, b}};
+    for (int i = 0; i < 4; i++) {
+        memcpy(q->pts[q->np], P[i], sizeof(P[i]));
+        q->wts[q->np++] = w;
+    }
+}
+
+static void add22(qrule *q, double a, double w)
+{
+    double h = 0.5 - a;
+    double P[6][4] = {{a, a, h, h}, {a, h, a, h}, {a, h, h, a},
+                      {h, a, h, a}, {h, a, a, h}, {h, h, a, a}};
+    for (int i = 0; i < 6; i++) {
+        memcpy(q->pts[q->np], P[i], sizeof(P[i]));
+        q->wts[q->np++] = w;
+    }
+}
+
+static void add211(qrule *q, double a, double b, double w)
+{
+    double c = 1.0 - a - a - b;
+    double P[12][4] = {{a, a, b, c}, {a, a, c, b}, {a, b, a, c}, {a, b, c, a},
+                       {a, c, a, b}, {a, c, b, a}, {b, a, a, c}, {b, a, c, a},
+                       {b, c, a, a}, {c, a, a, b}, {c, a, b, a}, {c, b, a, a}};
+    for (int i = 0; i < 12; i++) {
+        memcpy(q->pts[q->np], P[i], sizeof(P[i]));
+        q->wts[q->np++] = w;
+    }
+}
+
+static void init_rules(void)
+{
+    qrule *q;
+    if (rules_ready)
+        return;
+    memset(rules, 0, sizeof(rules));
+    /* order 1 (quad.c:1809): one point, weight 1 */
+    q = &rules[1];
+    q->pts[0][0] = q->pts[0][1] = q->pts[0][2] = q->pts[0][3] = .25;
+    q->wts[0] = 1.;
+    q->np = 1;
+    /* order 2 (quad.c:1827) */
+    add31(&rules[2], .13819660112501051517954131656343619, .25);
+    /* order 3 (quad.c:1858) */
+    add31(&rules[3], .32805469671142664733580581998119743,
+          .13852796651186214232361769837564129);
+    add31(&rules[3], .10695227393293068277170204157061650,
+          .11147203348813785767638230162435871);
+    /* order 4 (quad.c:1904) */
+    add31(&rules[4], .09273525031089122628655892066032137,
+          .07349304311636194934358694586367885);
+    add31(&rules[4], .31088591926330060975814749494040332,
+          .11268792571801585036501492847638892);
+    add22(&rules[4], .04550370412564965000000000000000000,
+          .04254602077708146686093208377328816);
+    /* order 5 (quad.c:1955) */
+    add31(&rules[5], .31088591926330060979734573376345783,
+          .11268792571801585079918565233328633);
+    add31(&rules[5], .09273525031089122640232391373703061,
+          .07349304311636194954371020548632750);
+    add22(&rules[5], .04550370412564964949188052627933943,
+          .04254602077708146643806942812025744);
+    /* order 6 (quad.c:1983) */
+    add31(&rules[6], .21460287125915202928883921938628499,
+          .03992275025816749209969062755747998);
+    add31(&rules[6], .04067395853461135311557944895641006,
+          .01007721105532064294801323744593686);
+    add31(&rules[6], .32233789014227551034399447076249213,
+          .05535718154365472209515327785372602);
+    add211(&rules[6], .06366100187501752529923552760572698,
+           .60300566479164914136743113906093969, 27. / 560.);
+    /* order 0 falls to the order-1 rule (quad.c:2807 init_table) */
+    rules[0] = rules[1];
+    rules_ready = 1;
+}
+
+int orc_quad3d(int order, const double **pts, const double **wts)
+{
+    init_rules();
+    if (order < 0)
+        order = 0;
+    if (order > 6)
+        return -1;
+    *pts = &rules[order].pts[0][0];
+    *wts = rules[order].wts;
+    return rules[order].np;
+}
+
+/*------------------------------- bases ------------------------------------*/
+
+static const int EV[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+static const int FV[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
+
+void orc_bas(int p, const double *lam, double *val)
+{
+    double L0 = lam[0], L1 = lam[1], L2 = lam[2], L3 = lam[3];
+    if (p == 1) {                         /* lagrange.c:80 */
+        memcpy(val, lam, 4 * sizeof(double));
+    }
+    else if (p == 2) {                    /* lagrange.c:160 full block */
+        double *q = val;
+        *(q++) = L0 * (L0 + L0 - 1.);
+        *(q++) = L1 * (L1 + L1 - 1.);
+        *(q++) = L2 * (L2 + L2 - 1.);
+        *(q++) = L3 * (L3 + L3 - 1.);
+        *(q++) = 4. * L0 * L1;
+        *(q++) = 4. * L0 * L2;
+        *(q++) = 4. * L0 * L3;
+        *(q++) = 4. * L1 * L2;
+        *(q++) = 4. * L1 * L3;
+        *q = 4. * L2 * L3;
+    }
+    else {                                /* lagrange.c:392 full block */
+        double L03 = 3. * L0, L13 = 3. * L1, L23 = 3. * L2, L33 = 3. * L3, d;
+        double *q = val;
+        *(q++) = .5 * (L03 - 1.) * (L03 - 2.) * L0;
+        *(q++) = .5 * (L13 - 1.) * (L13 - 2.) * L1;
+        *(q++) = .5 * (L23 - 1.) * (L23 - 2.) * L2;
+        *(q++) = .5 * (L33 - 1.) * (L33 - 2.) * L3;
+        *(q++) = 4.5 * (L03 - 1.) * (d = L0 * L1);
+        *(q++) = 4.5 * (L13 - 1.) * d;
+        *(q++) = 4.5 * (L03 - 1.) * (d = L0 * L2);
+        *(q++) = 4.5 * (L23 - 1.) * d;
+        *(q++) = 4.5 * (L03 - 1.) * (d = L0 * L3);
+        *(q++) = 4.5 * (L33 - 1.) * d;
+        *(q++) = 4.5 * (L13 - 1.) * (d = L1 * L2);
+        *(q++) = 4.5 * (L23 - 1.) * d;
+        *(q++) = 4.5 * (L13 - 1.) * (d = L1 * L3);
+        *(q++) = 4.5 * (L33 - 1.) * d;
+        *(q++) = 4.5 * (L23 - 1.) * (d = L2 * L3);
+        *(q++) = 4.5 * (L33 - 1.) * d;
+        *(q++) = 27. * L1 * L2 * L3;
+        *(q++) = 27. * L0 * L2 * L3;
+        *(q++) = 27. * L0 * L1 * L3;
+        *q = 27. * L0 * L1 * L2;
+    }
+}
+
+void orc_grad_lambda(int p, const double *lam, double *grad)
+{
+    int nb = p == 1 ? 4 : (p == 2 ? 10 : 20);
+    memset(grad, 0, (size_t)nb * 4 * sizeof(double));
+    if (p == 1) {                         /* lagrange.c:111 */
+        for (int k = 0; k < 4; k++)
+            grad[4 * k + k] = 1.0;
+        return;
+    }
+    if (p == 2) {                         /* lagrange.c:248 */
+        for (int k = 0; k < 4; k++)
+            grad[4 * k + k] = 4. * lam[k] - 1.;
+        for (int k = 0; k < 6; k++) {
+            int a = EV[k][0], b = EV[k][1];
+            grad[4 * (4 + k) + a] = 4. * lam[b];
+            grad[4 * (4 + k) + b] = 4. * lam[a];
+        }
+        return;
+    }
+    /* lagrange.c:538 */
+    for (int k = 0; k < 4; k++)
+        grad[4 * k + k] = (13.5 * lam[k] - 9.) * lam[k] + 1.;
+    for (int k = 0; k < 6; k++) {
+        int a = EV[k][0], b = EV[k][1];
+        double *g0 = grad + 4 * (4 + 2 * k), *g1 = g0 + 4;
+        g0[a] = (27. * lam[a] - 4.5) * lam[b];
+        g0[b] = 4.5 * lam[a] * (3. * lam[a] - 1.);
+        g1[a] = 4.5 * lam[b] * (3. * lam[b] - 1.);
+        g1[b] = (27. * lam[b] - 4.5) * lam[a];
+    }
+    for (int k = 0; k < 4; k++) {
+        int a = FV[k][0], b = FV[k][1], c = FV[k][2];
+        double *g = grad + 4 * (16 + k);
+        g[a] = 27. * lam[b] * lam[c];
+        g[b] = 27. * lam[a] * lam[c];
+        g[c] = 27. * lam[a] * lam[b];
+    }
+}
+
+/*------------------------------- geometry ---------------------------------*/
+
+double orc_volume(const double x[4][3])
+{
+    /* geom.c:46 _phg_compute_elem_data */
+    static const double oned6 = 1.0 / (double)6.0;
+    double x0 = x[0][0], y0 = x[0][1], z0 = x[0][2];
+    double x1 = x[1][0] - x0, y1 = x[1][1] - y0, z1 = x[1][2] - z0;
+    double x2 = x[2][0] - x0, y2 = x[2][1] - y0, z2 = x[2][2] - z0;
+    double x3 = x[3][0] - x0, y3 = x[3][1] - y0, z3 = x[3][2] - z0;
+    return fabs(x1 * y2 * z3 + x2 * y3 * z1 + y1 * z2 * x3 -
+                (z1 * y2 * x3 + y1 * x2 * z3 + z2 * y3 * x1)) * oned6;
+}
+
+void orc_jacobian(const double x[4][3], double *J)
+{
+    /* geom.c:103 compute_jacobian: Gauss-Jordan on [x y z 1 | I] */
+    const int n = 4;
+    double d, t, a[4][8];
+    int i, j, k;
+    for (j = 0; j < n; j++)
+        for (i = 0; i < n; i++)
+            a[j][n + i] = (i == j) ? 1.0 : 0.0;
+    for (j = 0; j < n; j++) {
+        for (i = 0; i < 3; i++)
+            a[j][i] = x[j][i];
+        a[j][3] = 1.0;
+    }
+    for (j = 0; j < n; j++) {
+        d = fabs(a[j][j]);
+        k = j;
+        for (i = j + 1; i < n; i++)
+            if (d < (t = fabs(a[i][j]))) {
+                d = t;
+                k = i;
+            }
+        if (k != j)
+            for (i = j; i < n + n; i++) {
+                t = a[j][i];
+                a[j][i] = a[k][i];
+                a[k][i] = t;
+            }
+        d = 1.0 / ((t = a[j][j]) == 0. ? 1. : t);
+        for (i = j + 1; i < n + n; i++)
+            a[j][i] *= d;
+        for (i = j + 1; i < n; i++) {
+            d = a[i][j];
+            for (k = j + 1; k < n + n; k++)
+                a[i][k] -= d * a[j][k];
+        }
+    }
+    for (j = n - 2; j >= 0; j--)
+        for (i = j + 1; i < n; i++) {
+            d = a[j][i];
+            for (k = n; k < n + n; k++)
+                a[j][k] -= d * a[i][k];
+        }
+    for (j = 0; j < n; j++)
+        for (i = 0; i < n; i++)
+            *(J++) = a[i][j + n];
+}
+
+/*--------------------------- element integrals ----------------------------*/
+
+/* quad.c:3314 phgQuadGetBasisGradient for basis k at point q:
+ * P1 copies Jacobian row k; vertex bases of P2/P3 scale row k by the
+ * diagonal lambda derivative; other bases take the 4-term product. */
+static void xgrad(int p, int k, const double *G /* [4] lambda grad of k */,
+                  const double *J, double *out)
+{
+    if (p == 1) {
+        out[0] = J[4 * k + 0];
+        out[1] = J[4 * k + 1];
+        out[2] = J[4 * k + 2];
+        return;
+    }
+    if (k < 4) {
+        double d = G[k];
+        out[0] = d * J[4 * k + 0];
+        out[1] = d * J[4 * k + 1];
+        out[2] = d * J[4 * k + 2];
+        return;
+    }
+    for (int c = 0; c < 3; c++)
+        out[c] = G[0] * J[0 * 4 + c] + G[1] * J[1 * 4 + c] +
+                 G[2] * J[2 * 4 + c] + G[3] * J[3 * 4 + c];
+}
+
+#define NBMAX 20
+
+void orc_element(const orc_problem *pb, const double x[4][3], double kappa,
+                 const double *f_loc, double *K, double *b)
+{
+    int p = pb->p, N = p == 1 ? 4 : (p == 2 ? 10 : 20), dim = pb->dim;
+    double J[16], vol;
+    const double *qp, *qw;
+    int nq;
+    static double gx[NQMAX][NBMAX][3];   /* x-gradients per point/basis */
+    static double bv[NQMAX][NBMAX];      /* basis values per point */
+    double glam[NBMAX * 4];
+
+    orc_jacobian(x, J);
+    vol = orc_volume(x);
+
+    if (dim == 1) {
+        /* stiffness: order (p-1)+(p-1) [+0 for a P0 coefficient]
+         * (quad.c:5271-5279) */
+        nq = orc_quad3d(2 * (p - 1), &qp, &qw);
+        for (int q = 0; q < nq; q++) {
+            orc_grad_lambda(p, qp + 4 * q, glam);
+            for (int k = 0; k < N; k++)
+                xgrad(p, k, glam + 4 * k, J, gx[q][k]);
+        }
+        for (int i = 0; i < N; i++)
+            for (int j = 0; j <= i; j++) {
+                /* phgQuadGradBasAGradBas(e, u, j, A, u, i): g1 = grad j */
+                double d = 0., d0;
+                for (int q = 0; q < nq; q++) {
+                    const double *g1 = gx[q][j], *g2 = gx[q][i];
+                    d0 = 0.;
+                    d0 += g1[0] * g2[0] + g1[1] * g2[1] + g1[2] * g2[2];
+                    if (pb->use_kappa)
+                        d += d0 * qw[q] * kappa;
+                    else
+                        d += d0 * qw[q];
+                }
+                K[i * N + j] = K[j * N + i] = d * vol;
+            }
+        if (pb->mass_coef != 0.0) {
+            /* simplest.c:70 a * phgQuadBasDotBas, order p+p (quad.c:5582) */
+            nq = orc_quad3d(2 * p, &qp, &qw);
+            for (int q = 0; q < nq; q++)
+                orc_bas(p, qp + 4 * q, bv[q]);
+            for (int i = 0; i < N; i++)
+                for (int j = 0; j <= i; j++) {
+                    double d = 0., d0;
+                    for (int q = 0; q < nq; q++) {
+                        d0 = 0.;
+                        d0 += bv[q][j] * bv[q][i];
+                        d += d0 * qw[q];
+                    }
+                    K[i * N + j] = K[j * N + i] =
+                        K[i * N + j] + pb->mass_coef * (d * vol);
+                }
+        }
+        /* load: quad.c:5481 phgQuadDofTimesBas with f_h at the points
+         * evaluated by dof-utils.c:59 phgDofEval_ (sum in basis order) */
+        nq = orc_quad3d(pb->rhs_order < 0 ? 2 * p : pb->rhs_order, &qp, &qw);
+        {
+            double fq[NQMAX];
+            for (int q = 0; q < nq; q++) {
+                orc_bas(p, qp + 4 * q, bv[q]);
+                fq[q] = 0.0;
+                for (int k = 0; k < N; k++)
+                    fq[q] += bv[q][k] * f_loc[k];
+            }
+            for (int i = 0; i < N; i++) {
+                double d = 0.0;
+                for (int q = 0; q < nq; q++)
+                    d += bv[q][i] * qw[q] * fq[q];
+                b[i] = d * vol;
+            }
+        }
+        return;
+    }
+
+    /* linear elasticity, elastic.c:90 stiffness_matrix with
+     * quad = phgQuadGetQuad3D((p-1)*2) (elastic.c:141) */
+    {
+        int M = 3 * N;
+        double E = pb->E, nu = pb->nu;
+        double a = nu / (1.0 - nu);
+        double bb = (1. - 2. * nu) / (2. * (1. - nu));
+        double c = E * (1. - nu) / ((1. + nu) * (1. - 2. * nu)) * vol;
+        nq = orc_quad3d((p - 1) * 2, &qp, &qw);
+        for (int q = 0; q < nq; q++) {
+            orc_grad_lambda(p, qp + 4 * q, glam);
+            for (int k = 0; k < N; k++)
+                xgrad(p, k, glam + 4 * k, J, gx[q][k]);
+        }
+        for (int i = 0; i < N; i++)
+            for (int j = 0; j <= i; j++) {
+                double a00, a01, a02, a10, a11, a12, a20, a21, a22;
+                a00 = a01 = a02 = a10 = a11 = a12 = a20 = a21 = a22 = 0.;
+                for (int q = 0; q < nq; q++) {
+                    const double *pm = gx[q][i], *pn = gx[q][j];
+                    double w = qw[q];
+                    a00 += (pm[0] * pn[0] + bb * (pm[1] * pn[1] + pm[2] * pn[2])) * w;
+                    a01 += (a * pm[0] * pn[1] + bb * pm[1] * pn[0]) * w;
+                    a02 += (a * pm[0] * pn[2] + bb * pm[2] * pn[0]) * w;
+                    a10 += (a * pm[1] * pn[0] + bb * pm[0] * pn[1]) * w;
+                    a11 += (pm[1] * pn[1] + bb * (pm[0] * pn[0] + pm[2] * pn[2])) * w;
+                    a12 += (a * pm[1] * pn[2] + bb * pm[2] * pn[1]) * w;
+                    a20 += (a * pm[2] * pn[0] + bb * pm[0] * pn[2]) * w;
+                    a21 += (a * pm[2] * pn[1] + bb * pm[1] * pn[2]) * w;
+                    a22 += (pm[2] * pn[2] + bb * (pm[0] * pn[0] + pm[1] * pn[1])) * w;
+                }
+                double A3[3][3] = {{a00 * c, a01 * c, a02 * c},
+                                   {a10 * c, a11 * c, a12 * c},
+                                   {a20 * c, a21 * c, a22 * c}};
+                for (int k = 0; k < 3; k++)
+                    for (int l = 0; l < 3; l++) {
+                        K[(i * 3 + k) * M + j * 3 + l] = A3[k][l];
+                        if (j != i)
+                            K[(j * 3 + k) * M + i * 3 + l] = A3[l][k];
+                    }
+            }
+        /* load: phgQuadDofTimesBas, vector path, then negated
+         * (elastic.c:208-212) */
+        nq = orc_quad3d(pb->rhs_order < 0 ? 2 * p : pb->rhs_order, &qp, &qw);
+        {
+            double fq[NQMAX][3];
+            for (int q = 0; q < nq; q++) {
+                orc_bas(p, qp + 4 * q, bv[q]);
+                fq[q][0] = fq[q][1] = fq[q][2] = 0.0;
+                for (int k = 0; k < N; k++)
+                    for (int cc = 0; cc < 3; cc++)
+                        fq[q][cc] += bv[q][k] * f_loc[3 * k + cc];
+            }
+            for (int i = 0; i < N; i++) {
+                double r[3] = {0, 0, 0}, d;
+                for (int q = 0; q < nq; q++) {
+                    d = bv[q][i] * qw[q];
+                    for (int cc = 0; cc < 3; cc++)
+                        r[cc] += d * fq[q][cc];
+                }
+                for (int cc = 0; cc < 3; cc++) {
+                    r[cc] *= vol;
+                    b[3 * i + cc] = -r[cc];
+                }
+            }
+        }
+    }
+}
+
+/*------------------------- PHG row-list insertion -------------------------*/
+
+typedef struct {
+    int32_t ncols, alloc;
+    int32_t *cols;
+    double *data;
+} mrow;
+
+#define TRUNK 8
+
+/* matvec.c:3045 add_columns_to_row, SORT_COLUMNS == 0: linear search,
+ * add to an existing column or append */
+static void row_add(mrow *r, int32_t col, double v)
+{
+    int32_t i;
+    for (i = 0; i < r->ncols; i++)
+        if (r->cols[i] == col)
+            break;
+    if (i < r->ncols) {
+        r->data[i] += v;
+        return;
+    }
+    if (r->ncols >= r->alloc) {
+        r->alloc += TRUNK;
+        r->cols = realloc(r->cols, (size_t)r->alloc * sizeof(int32_t));
+        r->data = realloc(r->data, (size_t)r->alloc * sizeof(double));
+    }
+    r->cols[r->ncols] = col;
+    r->data[r->ncols++] = v;
+}
+
+orc_csr *orc_assemble(const orc_problem *pb, int64_t nelem, const double *verts,
+                      const int32_t *elem_verts, const int32_t *map,
+                      int64_t ndof, const uint8_t *bdry, const double *g,
+                      const double *f, const double *kappa)
+{
+    int p = pb->p, N = p == 1 ? 4 : (p == 2 ? 10 : 20), dim = pb->dim;
+    int M = N * dim;
+    int64_t n = ndof * dim;
+    mrow *rows = calloc((size_t)n, sizeof(mrow));
+    double *rhs = calloc((size_t)n, sizeof(double));
+    double *K = xmalloc((size_t)M * M * sizeof(double));
+    double *B = xmalloc((size_t)M * sizeof(double));
+    double *floc = xmalloc((size_t)M * sizeof(double));
+    int32_t *I = xmalloc((size_t)M * sizeof(int32_t));
+
+    for (int64_t e = 0; e < nelem; e++) {
+        double x[4][3];
+        const int32_t *m = map + (int64_t)N * e;
+        for (int a = 0; a < 4; a++)
+            for (int c = 0; c < 3; c++)
+                x[a][c] = verts[3 * (int64_t)elem_verts[4 * e + a] + c];
+        for (int i = 0; i < N; i++)
+            for (int c = 0; c < dim; c++) {
+                I[i * dim + c] = (int32_t)((int64_t)m[i] * dim + c);
+                floc[i * dim + c] = f[(int64_t)m[i] * dim + c];
+            }
+        orc_element(pb, x, kappa ? kappa[e] : 0.0, floc, K, B);
+        for (int i = 0; i < N; i++) {
+            int is_b = bdry[m[i]] != 0;
+            for (int c = 0; c < dim; c++) {
+                int a = i * dim + c;
+                mrow *r = rows + I[a];
+                if (is_b) {
+                    /* Dirichlet row e_i, rhs g (dof-utils.c:1369-1392);
+                     * zeros dropped (matvec.c:5460) */
+                    row_add(r, I[a], 1.0);
+                    B[a] = g[(int64_t)m[i] * dim + c];
+                }
+                else {
+                    for (int b2 = 0; b2 < M; b2++) {
+                        double v = K[a * M + b2];
+                        if (v == 0.0)
+                            continue;
+                        row_add(r, I[b2], v);
+                    }
+                }
+            }
+        }
+        /* phgVecAddEntries (matvec.c:5661): zeros skipped */
+        for (int a = 0; a < M; a++)
+            if (B[a] != 0.0)
+                rhs[I[a]] += B[a];
+    }
+
+    orc_csr *A = xmalloc(sizeof(*A));
+    A->n = n;
+    A->rowptr = xmalloc((size_t)(n + 1) * sizeof(int64_t));
+    A->rowptr[0] = 0;
+    for (int64_t i = 0; i < n; i++)
+        A->rowptr[i + 1] = A->rowptr[i] + rows[i].ncols;
+    A->nnz = A->rowptr[n];
+    A->cols = xmalloc((size_t)A->nnz * sizeof(int32_t));
+    A->vals = xmalloc((size_t)A->nnz * sizeof(double));
+    for (int64_t i = 0; i < n; i++) {
+        memcpy(A->cols + A->rowptr[i], rows[i].cols, (size_t)rows[i].ncols * sizeof(int32_t));
+        memcpy(A->vals + A->rowptr[i], rows[i].data, (size_t)rows[i].ncols * sizeof(double));
+        free(rows[i].cols);
+        free(rows[i].data);
+    }
+    A->rhs = rhs;
+    free(rows);
+    free(K);
+    free(B);
+    free(floc);
+    free(I);
+    return A;
+}
+
+void orc_eliminate(orc_csr *A, const uint8_t *bdry)
+{
+    int64_t n = A->n, w = 0;
+    int64_t *bstart = xmalloc((size_t)(n + 1) * sizeof(int64_t));
+    int32_t *bcols = xmalloc((size_t)A->nnz * sizeof(int32_t));
+    double *bvals = xmalloc((size_t)A->nnz * sizeof(double));
+    int64_t nb = 0;
+
+    /* save_bdry_columns(mat, TRUE): keep interior columns and the diagonal
+     * in the row, move every boundary column to bdry_rows */
+    for (int64_t i = 0; i < n; i++) {
+        int64_t s = A->rowptr[i], t = A->rowptr[i + 1];
+        bstart[i] = nb;
+        A->rowptr[i] = w;
+        for (int64_t j = s; j < t; j++) {
+            int32_t c = A->cols[j];
+            if (c == i || !bdry[c]) {
+                A->cols[w] = c;
+                A->vals[w++] = A->vals[j];
+            }
+            if (bdry[c]) {
+                bcols[nb] = c;
+                bvals[nb++] = A->vals[j];
+            }
+        }
+    }
+    A->rowptr[n] = w;
+    bstart[n] = nb;
+    A->nnz = w;
+
+    /* update_rhs (solver.c:714-724): rhs_i -= coeff*rhs_k*a_ik / d_k */
+    for (int64_t i = 0; i < n; i++)
+        for (int64_t j = bstart[i]; j < bstart[i + 1]; j++) {
+            int32_t k = bcols[j];
+            if (k == i)
+                continue;
+            double dk = bvals[bstart[k]];
+            A->rhs[i] -= 1.0 * A->rhs[k] * bvals[j] / dk;
+        }
+    free(bstart);
+    free(bcols);
+    free(bvals);
+}
+
+void orc_matvec(const orc_csr *A, const double *x, double *y)
+{
+    for (int64_t i = 0; i < A->n; i++) {
+        int64_t s = A->rowptr[i], t = A->rowptr[i + 1];
+        double beta;
+        y[i] = 0.0;
+        if (t == s)
+            continue;
+        beta = A->vals[s] * x[A->cols[s]];
+        for (int64_t j = s + 1; j < t; j++)
+            beta += A->vals[j] * x[A->cols[j]];
+        y[i] += beta;
+    }
+}
+
+static double norm2(int64_t n, const double *v)
+{
+    double s = 0.0;
+    for (int64_t i = 0; i < n; i++)
+        s += v[i] * v[i];
+    return sqrt(s);
+}
+
+static double dot(int64_t n, const double *x, const double *y)
+{
+    double s = 0.0;
+    for (int64_t i = 0; i < n; i++)
+        s += x[i] * y[i];
+    return s;
+}
+
+int orc_pcg(const orc_csr *A, const double *b, double *x, int maxit,
+            double rtol, double btol, double atol, double *residual)
+{
+    int64_t n = A->n;
+    double *diag = xmalloc((size_t)n * sizeof(double));
+    double *r = xmalloc((size_t)n * sizeof(double));
+    double *z = xmalloc((size_t)n * sizeof(double));
+    double *p = xmalloc((size_t)n * sizeof(double));
+    double *Ap = xmalloc((size_t)n * sizeof(double));
+    double rb, r0, res, tol, rz1 = 0.0, rz2, alpha, x1 = 0.0;
+    int nits = 0;
+
+    /* phgMatSetupDiagonal (matvec.c:500): first diagonal entry in the row */
+    for (int64_t i = 0; i < n; i++) {
+        diag[i] = 0.0;
+        for (int64_t j = A->rowptr[i]; j < A->rowptr[i + 1]; j++)
+            if (A->cols[j] == i) {
+                diag[i] = A->vals[j];
+                break;
+            }
+    }
+    rb = norm2(n, b);
+    /* IsZero (utils.h:451): |a| < 10 * FLOAT_MIN */
+    if (fabs(rb) < 10.0 * DBL_MIN) {
+        memset(x, 0, (size_t)n * sizeof(double));
+        res = 0.0;
+        goto done;
+    }
+    for (int64_t i = 0; i < n; i++)
+        x1 += fabs(x[i]);
+    if (!(fabs(x1 / (double)n) < 10.0 * DBL_MIN)) {
+        orc_matvec(A, x, Ap);
+        for (int64_t i = 0; i < n; i++)
+            r[i] = -Ap[i] + b[i];
+    }
+    else {
+        memcpy(r, b, (size_t)n * sizeof(double));
+    }
+    r0 = res = norm2(n, r);
+    tol = r0 * rtol;
+    if (tol < rb * btol)
+        tol = rb * btol;
+    if (tol < atol)
+        tol = atol;
+
+    while (res > tol && nits < maxit) {
+        for (int64_t i = 0; i < n; i++) {
+            if (diag[i] == 0.0) {
+                nits = -1;
+                goto done;
+            }
+            z[i] = r[i] / diag[i];
+        }
+        nits++;
+        if (nits == 1) {
+            memcpy(p, z, (size_t)n * sizeof(double));
+            rz1 = dot(n, r, z);
+        }
+        else {
+            rz2 = rz1;
+            rz1 = dot(n, r, z);
+            if (rz2 == 0.0)
+                break;
+            double beta = rz1 / rz2;
+            for (int64_t i = 0; i < n; i++)
+                p[i] = beta * p[i];
+            for (int64_t i = 0; i < n; i++)
+                p[i] += z[i];
+        }
+        orc_matvec(A, p, Ap);
+        if ((alpha = dot(n, p, Ap)) == 0.0)
+            break;
+        alpha = rz1 / alpha;
+        for (int64_t i = 0; i < n; i++)
+            x[i] += alpha * p[i];
+        double ma = -1.0 * alpha;
+        for (int64_t i = 0; i < n; i++)
+            r[i] += ma * Ap[i];
+        res = norm2(n, r);
+    }
+done:
+    if (residual)
+        *residual = res;
+    free(diag);
+    free(r);
+    free(z);
+    free(p);
+    free(Ap);
+    return nits;
+}
+
+void orc_csr_free(orc_csr *A)
+{
+    if (!A)
+        return;
+    free(A->rowptr);
+    free(A->cols);
+    free(A->vals);
+    free(A->rhs);
+    free(A);
+}

@@ -1,0 +1,388 @@
+"""phg_amd -- MI355X (gfx950) finite-element hot path of PHG.
+
+Python bindings over the two in-tree native libraries:
+
+* ``lib/libphg_grid.so`` -- host grid layer with PHG's import numbering
+  (``phg_amd/host/phg_grid.h``);
+* ``lib/libphg_hip.so``  -- HIP kernels behind the C ABI of
+  ``include/phg_hip.h``.
+
+The bindings are plumbing for drivers, tests and the benchmark: all numerical
+work runs in the HIP library.  Importing the package without the built HIP
+library raises ``ImportError`` -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "lib")
+
+__all__ = [
+    "Grid", "Space", "Mat", "Vec", "Problem", "init", "finalize",
+    "assemble", "element_matrices", "pcg", "pcg_mixed", "PhgHipError",
+    "hip_lib", "grid_lib", "HIP_LIB_PATH", "GRID_LIB_PATH",
+]
+
+HIP_LIB_PATH = os.path.join(_LIB, "libphg_hip.so")
+GRID_LIB_PATH = os.path.join(_LIB, "libphg_grid.so")
+
+
+def _load(path: str) -> C.CDLL:
+    if not os.path.exists(path):
+        raise ImportError(
+            f"phg_amd native library missing: {path}. Build it with "
+            "`python -m phg_amd.build` (or __graft_entry__.build()).")
+    return C.CDLL(path)
+
+
+grid_lib = _load(GRID_LIB_PATH)
+hip_lib = _load(HIP_LIB_PATH)
+
+_i32, _i64, _dbl, _vp, _cp = C.c_int32, C.c_int64, C.c_double, C.c_void_p, C.c_char_p
+_P = C.POINTER
+
+
+class _GridStruct(C.Structure):
+    _fields_ = [
+        ("nvert", _i32), ("nelem", _i32), ("nedge", _i32), ("nface", _i32),
+        ("verts", _P(_dbl)), ("elem_verts", _P(_i32)), ("elem_edges", _P(_i32)),
+        ("elem_faces", _P(_i32)), ("elem_neigh", _P(_i32)),
+        ("elem_bound", _P(C.c_uint8)), ("types_vert", _P(C.c_uint8)),
+        ("types_edge", _P(C.c_uint8)), ("types_face", _P(C.c_uint8)),
+    ]
+
+
+class Problem(C.Structure):
+    """Integrand selector (include/phg_hip.h phg_hip_problem)."""
+    _fields_ = [("dim", _i32), ("use_kappa", _i32), ("mass_coef", _dbl),
+                ("rhs_order", _i32), ("E", _dbl), ("nu", _dbl)]
+
+    @classmethod
+    def poisson(cls, use_kappa=False, mass_coef=0.0, rhs_order=-1):
+        return cls(1, int(use_kappa), float(mass_coef), int(rhs_order), 0.0, 0.0)
+
+    @classmethod
+    def elasticity(cls, E=2.5, nu=0.25, rhs_order=-1):
+        return cls(3, 0, 0.0, int(rhs_order), float(E), float(nu))
+
+
+def _sig(lib, name, res, args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = args
+    return f
+
+
+# grid library
+_sig(grid_lib, "phg_grid_new_kuhn", _P(_GridStruct), [_i32, _P(C.c_int)])
+_sig(grid_lib, "phg_grid_read_albert", _P(_GridStruct), [_cp])
+_sig(grid_lib, "phg_grid_write_albert", C.c_int, [_P(_GridStruct), _cp])
+_sig(grid_lib, "phg_grid_free", None, [_P(_GridStruct)])
+_sig(grid_lib, "phg_dof_nbas", C.c_int, [C.c_int])
+_sig(grid_lib, "phg_dof_count", _i64, [_P(_GridStruct), C.c_int])
+_sig(grid_lib, "phg_dof_map", None, [_P(_GridStruct), C.c_int, _P(_i32)])
+_sig(grid_lib, "phg_dof_coords", None, [_P(_GridStruct), C.c_int, _P(_dbl)])
+_sig(grid_lib, "phg_dof_bdry", None, [_P(_GridStruct), C.c_int, C.c_uint,
+                                      _P(C.c_uint8)])
+
+# HIP library
+_sig(hip_lib, "phg_hip_init", C.c_int, [_i32])
+_sig(hip_lib, "phg_hip_finalize", C.c_int, [])
+_sig(hip_lib, "phg_hip_last_error", _cp, [])
+_sig(hip_lib, "phg_hip_synchronize", C.c_int, [])
+_sig(hip_lib, "phg_hip_space_create", C.c_int,
+     [_i32, _i64, _P(_dbl), _i64, _P(_i32), _P(_i32), _i64, _P(C.c_uint8), _P(_vp)])
+_sig(hip_lib, "phg_hip_space_destroy", C.c_int, [_vp])
+_sig(hip_lib, "phg_hip_element_matrices", C.c_int,
+     [_vp, _P(Problem), _P(_dbl), _P(_dbl), _P(_dbl), _P(_dbl)])
+_sig(hip_lib, "phg_hip_assemble", C.c_int,
+     [_vp, _P(Problem), _P(_dbl), _P(_dbl), _P(_dbl), _P(_vp)])
+_sig(hip_lib, "phg_hip_mat_eliminate", C.c_int, [_vp])
+_sig(hip_lib, "phg_hip_mat_info", C.c_int, [_vp, _P(_i64), _P(_i64)])
+_sig(hip_lib, "phg_hip_mat_download", C.c_int,
+     [_vp, _P(_i64), _P(_i32), _P(_dbl), _P(_dbl)])
+_sig(hip_lib, "phg_hip_mat_destroy", C.c_int, [_vp])
+_sig(hip_lib, "phg_hip_vec_create", C.c_int, [_i64, _P(_vp)])
+_sig(hip_lib, "phg_hip_vec_destroy", C.c_int, [_vp])
+_sig(hip_lib, "phg_hip_vec_upload", C.c_int, [_vp, _P(_dbl)])
+_sig(hip_lib, "phg_hip_vec_download", C.c_int, [_vp, _P(_dbl)])
+_sig(hip_lib, "phg_hip_mat_rhs", C.c_int, [_vp, _vp])
+_sig(hip_lib, "phg_hip_matvec", C.c_int, [_dbl, _vp, _vp, _dbl, _vp])
+_sig(hip_lib, "phg_hip_vec_dot", C.c_int, [_vp, _vp, _P(_dbl)])
+_sig(hip_lib, "phg_hip_vec_norm2", C.c_int, [_vp, _P(_dbl)])
+_sig(hip_lib, "phg_hip_vec_axpby", C.c_int, [_dbl, _vp, _dbl, _vp])
+_sig(hip_lib, "phg_hip_vec_copy", C.c_int, [_vp, _vp])
+_sig(hip_lib, "phg_hip_pcg", C.c_int,
+     [_vp, _vp, _vp, _i32, _dbl, _dbl, _dbl, _P(_dbl)])
+_sig(hip_lib, "phg_hip_pcg_mixed", C.c_int,
+     [_vp, _vp, _vp, _i32, _dbl, _dbl, _dbl, _dbl, _P(_i32), _P(_dbl)])
+
+
+class PhgHipError(RuntimeError):
+    def __init__(self, code, where):
+        msg = hip_lib.phg_hip_last_error().decode(errors="replace")
+        super().__init__(f"{where}: error {code}: {msg}")
+        self.code = code
+
+
+def _check(rc, where):
+    if rc < 0:
+        raise PhgHipError(rc, where)
+    return rc
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(_P(ctype)) if a is not None else None
+
+
+def init(device: int = 0) -> None:
+    _check(hip_lib.phg_hip_init(device), "phg_hip_init")
+
+
+def finalize() -> None:
+    _check(hip_lib.phg_hip_finalize(), "phg_hip_finalize")
+
+
+def synchronize() -> None:
+    _check(hip_lib.phg_hip_synchronize(), "phg_hip_synchronize")
+
+
+class Grid:
+    """Host grid with PHG numbering (phg_amd/host/phg_grid.h)."""
+
+    def __init__(self, ptr):
+        if not ptr:
+            raise RuntimeError("grid construction failed")
+        self._p = ptr
+
+    @classmethod
+    def kuhn(cls, n: int, bcode=(1, 1, 1, 1, 1, 1)):
+        arr = (C.c_int * 6)(*bcode)
+        return cls(grid_lib.phg_grid_new_kuhn(n, arr))
+
+    @classmethod
+    def read_albert(cls, path: str):
+        return cls(grid_lib.phg_grid_read_albert(path.encode()))
+
+    def write_albert(self, path: str) -> None:
+        if grid_lib.phg_grid_write_albert(self._p, path.encode()) != 0:
+            raise OSError(f"cannot write {path}")
+
+    def __del__(self):
+        if getattr(self, "_p", None):
+            grid_lib.phg_grid_free(self._p)
+            self._p = None
+
+    @property
+    def s(self):
+        return self._p.contents
+
+    nvert = property(lambda self: self.s.nvert)
+    nelem = property(lambda self: self.s.nelem)
+    nedge = property(lambda self: self.s.nedge)
+    nface = property(lambda self: self.s.nface)
+
+    def verts(self):
+        return np.ctypeslib.as_array(self.s.verts, (self.nvert, 3)).copy()
+
+    def elem_verts(self):
+        return np.ctypeslib.as_array(self.s.elem_verts, (self.nelem, 4)).copy()
+
+    def elem_edges(self):
+        return np.ctypeslib.as_array(self.s.elem_edges, (self.nelem, 6)).copy()
+
+    def elem_faces(self):
+        return np.ctypeslib.as_array(self.s.elem_faces, (self.nelem, 4)).copy()
+
+    def elem_neigh(self):
+        return np.ctypeslib.as_array(self.s.elem_neigh, (self.nelem, 4)).copy()
+
+    def types_vert(self):
+        return np.ctypeslib.as_array(self.s.types_vert, (self.nvert,)).copy()
+
+    def ndof(self, order: int) -> int:
+        return int(grid_lib.phg_dof_count(self._p, order))
+
+    def dof_map(self, order: int) -> np.ndarray:
+        nb = grid_lib.phg_dof_nbas(order)
+        m = np.empty((self.nelem, nb), dtype=np.int32)
+        grid_lib.phg_dof_map(self._p, order, _ptr(m, _i32))
+        return m
+
+    def dof_coords(self, order: int) -> np.ndarray:
+        x = np.empty((self.ndof(order), 3), dtype=np.float64)
+        grid_lib.phg_dof_coords(self._p, order, _ptr(x, _dbl))
+        return x
+
+    def dof_bdry(self, order: int, mask: int = 1) -> np.ndarray:
+        b = np.empty(self.ndof(order), dtype=np.uint8)
+        grid_lib.phg_dof_bdry(self._p, order, mask, _ptr(b, C.c_uint8))
+        return b
+
+
+class Space:
+    """Mesh + Lagrange Pk space uploaded to the device."""
+
+    def __init__(self, grid: Grid, order: int, bdry_mask: int = 1,
+                 bdry: np.ndarray | None = None):
+        self.order = order
+        self.nelem = grid.nelem
+        self.verts = np.ascontiguousarray(grid.verts(), dtype=np.float64)
+        self.elem_verts = np.ascontiguousarray(grid.elem_verts(), dtype=np.int32)
+        self.map = grid.dof_map(order)
+        self.ndof = grid.ndof(order)
+        self.bdry = grid.dof_bdry(order, bdry_mask) if bdry is None else \
+            np.ascontiguousarray(bdry, dtype=np.uint8)
+        self.coords = grid.dof_coords(order)
+        h = _vp()
+        _check(hip_lib.phg_hip_space_create(
+            order, grid.nvert, _ptr(self.verts, _dbl), self.nelem,
+            _ptr(self.elem_verts, _i32), _ptr(self.map, _i32), self.ndof,
+            _ptr(self.bdry, C.c_uint8), C.byref(h)), "phg_hip_space_create")
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            hip_lib.phg_hip_space_destroy(self._h)
+            self._h = None
+
+
+class Vec:
+    """FP64 device vector."""
+
+    def __init__(self, n: int, data: np.ndarray | None = None):
+        self.n = int(n)
+        h = _vp()
+        _check(hip_lib.phg_hip_vec_create(self.n, C.byref(h)), "phg_hip_vec_create")
+        self._h = h
+        if data is not None:
+            self.upload(data)
+
+    def upload(self, data):
+        a = np.ascontiguousarray(data, dtype=np.float64)
+        assert a.size == self.n
+        _check(hip_lib.phg_hip_vec_upload(self._h, _ptr(a, _dbl)), "vec_upload")
+
+    def numpy(self) -> np.ndarray:
+        a = np.empty(self.n, dtype=np.float64)
+        _check(hip_lib.phg_hip_vec_download(self._h, _ptr(a, _dbl)), "vec_download")
+        return a
+
+    def dot(self, other: "Vec") -> float:
+        r = _dbl()
+        _check(hip_lib.phg_hip_vec_dot(self._h, other._h, C.byref(r)), "vec_dot")
+        return r.value
+
+    def norm2(self) -> float:
+        r = _dbl()
+        _check(hip_lib.phg_hip_vec_norm2(self._h, C.byref(r)), "vec_norm2")
+        return r.value
+
+    def axpby(self, a: float, x: "Vec", b: float) -> None:
+        _check(hip_lib.phg_hip_vec_axpby(a, x._h, b, self._h), "vec_axpby")
+
+    def copy_from(self, src: "Vec") -> None:
+        _check(hip_lib.phg_hip_vec_copy(src._h, self._h), "vec_copy")
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            hip_lib.phg_hip_vec_destroy(self._h)
+            self._h = None
+
+
+class Mat:
+    """Assembled CSR system on the device."""
+
+    def __init__(self, h):
+        self._h = h
+
+    def info(self):
+        n, nnz = _i64(), _i64()
+        _check(hip_lib.phg_hip_mat_info(self._h, C.byref(n), C.byref(nnz)), "mat_info")
+        return n.value, nnz.value
+
+    @property
+    def n(self):
+        return self.info()[0]
+
+    @property
+    def nnz(self):
+        return self.info()[1]
+
+    def eliminate(self) -> None:
+        _check(hip_lib.phg_hip_mat_eliminate(self._h), "mat_eliminate")
+
+    def download(self):
+        n, nnz = self.info()
+        rp = np.empty(n + 1, dtype=np.int64)
+        ci = np.empty(nnz, dtype=np.int32)
+        va = np.empty(nnz, dtype=np.float64)
+        rhs = np.empty(n, dtype=np.float64)
+        _check(hip_lib.phg_hip_mat_download(self._h, _ptr(rp, _i64), _ptr(ci, _i32),
+                                            _ptr(va, _dbl), _ptr(rhs, _dbl)),
+               "mat_download")
+        return rp, ci, va, rhs
+
+    def rhs(self) -> Vec:
+        v = Vec(self.n)
+        _check(hip_lib.phg_hip_mat_rhs(self._h, v._h), "mat_rhs")
+        return v
+
+    def matvec(self, x: Vec, y: Vec, alpha=1.0, beta=0.0) -> None:
+        _check(hip_lib.phg_hip_matvec(alpha, self._h, x._h, beta, y._h), "matvec")
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            hip_lib.phg_hip_mat_destroy(self._h)
+            self._h = None
+
+
+def _opt(a, dtype=np.float64):
+    return None if a is None else np.ascontiguousarray(a, dtype=dtype)
+
+
+def element_matrices(sp: Space, pb: Problem, kappa=None, f=None):
+    M = {1: 4, 2: 10, 3: 20}[sp.order] * pb.dim
+    kap = _opt(kappa)
+    f = _opt(f) if f is not None else np.zeros(sp.ndof * pb.dim)
+    K = np.empty((sp.nelem, M, M), dtype=np.float64)
+    B = np.empty((sp.nelem, M), dtype=np.float64)
+    _check(hip_lib.phg_hip_element_matrices(sp._h, C.byref(pb), _ptr(kap, _dbl),
+                                            _ptr(f, _dbl), _ptr(K, _dbl), _ptr(B, _dbl)),
+           "element_matrices")
+    return K, B
+
+
+def assemble(sp: Space, pb: Problem, kappa=None, f=None, g=None) -> Mat:
+    n = sp.ndof * pb.dim
+    kap = _opt(kappa)
+    f = _opt(f) if f is not None else np.zeros(n)
+    g = _opt(g) if g is not None else np.zeros(n)
+    h = _vp()
+    _check(hip_lib.phg_hip_assemble(sp._h, C.byref(pb), _ptr(kap, _dbl), _ptr(f, _dbl),
+                                    _ptr(g, _dbl), C.byref(h)), "assemble")
+    return Mat(h)
+
+
+def pcg(A: Mat, b: Vec, x: Vec, maxit=10000, rtol=1e-10, btol=1e-10, atol=0.0):
+    """FP64 Jacobi-PCG (solver-phg.c:485). Returns (nits, residual)."""
+    res = _dbl()
+    nits = _check(hip_lib.phg_hip_pcg(A._h, b._h, x._h, maxit, rtol, btol, atol,
+                                      C.byref(res)), "pcg")
+    return nits, res.value
+
+
+def pcg_mixed(A: Mat, b: Vec, x: Vec, maxit=100, rtol=1e-10, btol=1e-10, atol=0.0,
+              inner_rtol=1e-4):
+    """FP32 PCG under FP64 iterative refinement.
+    Returns (outer_steps, inner_iterations, residual)."""
+    res = _dbl()
+    inner = _i32()
+    outer = _check(hip_lib.phg_hip_pcg_mixed(A._h, b._h, x._h, maxit, rtol, btol, atol,
+                                             inner_rtol, C.byref(inner), C.byref(res)),
+                   "pcg_mixed")
+    return outer, inner.value, res.value

@@ -1,0 +1,1288 @@
+// Element integration and PHG-order CSR assembly for gfx950.
+//
+// Compiled with -ffp-contract=off: every floating-point expression below is
+// evaluated in the order of the reference source it cites, so the element
+// matrices are bit-identical to PHG's and the exact-zero drop of
+// matvec.c:5460 removes the same entries.
+//
+// Kernels
+//   k_element<P,DIM>   one work-item per tetrahedron: Jacobian (geom.c:103)
+//                      and volume (geom.c:46) in registers, reference-basis
+//                      tables (lagrange.c) and quadrature rules (quad.c) staged
+//                      in LDS once per work-group, element matrix and load
+//                      written to HBM.
+//   k_inc_*            element-row incidence (row -> (element, local index)
+//                      sorted by element index), built with a counting sort.
+//   k_row_walk         one work-item per row replays PHG's row-list insertion
+//                      (matvec.c:3045 add_columns_to_row: linear search,
+//                      add or append, zeros skipped) over the row's incident
+//                      elements in element order, plus the RHS accumulation of
+//                      matvec.c:5661.
+//   k_elim_*           serial path of phgMatAssemble (matvec.c:4430
+//                      save_bdry_columns(TRUE)) and solver.c:508 update_rhs.
+#include "phg_hip_internal.h"
+
+#include <stdio.h>
+#include <string.h>
+#include <vector>
+
+#pragma clang fp contract(off)
+
+namespace {
+
+constexpr int NQ = 24;   // largest rule used (order 6)
+constexpr int NB = 20;   // P3
+
+//--------------------------------------------------------------- quadrature
+// quad.c:1799-2000 static 3D rules with the orbit order of
+// include/phg/quad-permu.h:83-120; order 0 uses the order-1 rule
+// (quad.c:2807 init_table).
+struct Rule {
+    int np;
+    double pts[NQ][4];
+    double w[NQ];
+};
+
+__host__ __device__ inline void r31(Rule &q, double a, double w)
+{
+    double b = 1.0 - 3.0 * a;
+    double P[4][4] = {{b, a, a, a}, {a, b, a, a}, {a, a, b, a}, {a, a, a, b}};
+    for (int i = 0; i < 4; i++) {
+        for (int c = 0; c < 4; c++)
+            q.pts[q.np][c] = P[i][c];
+        q.w[q.np++] = w;
+    }
+}
+
+__host__ __device__ inline void r22(Rule &q, double a, double w)
+{
+    double h = 0.5 - a;
+    double P[6][4] = {{a, a, h, h}, {a, h, a, h}, {a, h, h, a},
+                      {h, a, h, a}, {h, a, a, h}, {h, h, a, a}};
+    for (int i = 0; i < 6; i++) {
+        for (int c = 0; c < 4; c++)
+            q.pts[q.np][c] = P[i][c];
+        q.w[q.np++] = w;
+    }
+}
+
+__host__ __device__ inline void r211(Rule &q, double a, double b, double w)
+{
+    double c = 1.0 - a - a - b;
+    double P[12][4] = {{a, a, b, c}, {a, a, c, b}, {a, b, a, c}, {a, b, c, a},
+                       {a, c, a, b}, {a, c, b, a}, {b, a, a, c}, {b, a, c, a},
+                       {b, c, a, a}, {c, a, a, b}, {c, a, b, a}, {c, b, a, a}};
+    for (int i = 0; i < 12; i++) {
+        for (int k = 0; k < 4; k++)
+            q.pts[q.np][k] = P[i][k];
+        q.w[q.np++] = w;
+    }
+}
+
+__host__ __device__ inline void make_rule(int order, Rule &q)
+{
+    q.np = 0;
+    if (order <= 1) {
+        q.pts[0][0] = q.pts[0][1] = q.pts[0][2] = q.pts[0][3] = .25;
+        q.w[0] = 1.;
+        q.np = 1;
+    }
+    else if (order == 2) {
+        r31(q, .13819660112501051517954131656343619, .25);
+    }
+    else if (order == 3) {
+        r31(q, .32805469671142664733580581998119743,
+            .13852796651186214232361769837564129);
+        r31(q, .10695227393293068277170204157061650,
+            .11147203348813785767638230162435871);
+    }
+    else if (order == 4) {
+        r31(q, .09273525031089122628655892066032137,
+            .07349304311636194934358694586367885);
+        r31(q, .31088591926330060975814749494040332,
+            .11268792571801585036501492847638892);
+        r22(q, .04550370412564965000000000000000000,
+            .04254602077708146686093208377328816);
+    }
+    else if (order == 5) {
+        r31(q, .31088591926330060979734573376345783,
+            .11268792571801585079918565233328633);
+        r31(q, .09273525031089122640232391373703061,
+            .07349304311636194954371020548632750);
+        r22(q, .04550370412564964949188052627933943,
+            .04254602077708146643806942812025744);
+    }
+    else {
+        r31(q, .21460287125915202928883921938628499,
+            .03992275025816749209969062755747998);
+        r31(q, .04067395853461135311557944895641006,
+            .01007721105532064294801323744593686);
+        r31(q, .32233789014227551034399447076249213,
+            .05535718154365472209515327785372602);
+        r211(q, .06366100187501752529923552760572698,
+             .60300566479164914136743113906093969, 27. / 560.);
+    }
+}
+
+//------------------------------------------------------------------ bases
+__device__ __constant__ int c_ev[6][2] = {{0, 1}, {0, 2}, {0, 3},
+                                          {1, 2}, {1, 3}, {2, 3}};
+__device__ __constant__ int c_fv[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3},
+                                          {0, 1, 2}};
+
+// lagrange.c:80 / :160 / :392 (full-block evaluation)
+__device__ inline void bas_values(int p, const double *lam, double *v)
+{
+    double L0 = lam[0], L1 = lam[1], L2 = lam[2], L3 = lam[3];
+    if (p == 1) {
+        v[0] = L0; v[1] = L1; v[2] = L2; v[3] = L3;
+    }
+    else if (p == 2) {
+        v[0] = L0 * (L0 + L0 - 1.);
+        v[1] = L1 * (L1 + L1 - 1.);
+        v[2] = L2 * (L2 + L2 - 1.);
+        v[3] = L3 * (L3 + L3 - 1.);
+        v[4] = 4. * L0 * L1;
+        v[5] = 4. * L0 * L2;
+        v[6] = 4. * L0 * L3;
+        v[7] = 4. * L1 * L2;
+        v[8] = 4. * L1 * L3;
+        v[9] = 4. * L2 * L3;
+    }
+    else {
+        double L03 = 3. * L0, L13 = 3. * L1, L23 = 3. * L2, L33 = 3. * L3, d;
+        v[0] = .5 * (L03 - 1.) * (L03 - 2.) * L0;
+        v[1] = .5 * (L13 - 1.) * (L13 - 2.) * L1;
+        v[2] = .5 * (L23 - 1.) * (L23 - 2.) * L2;
+        v[3] = .5 * (L33 - 1.) * (L33 - 2.) * L3;
+        d = L0 * L1;
+        v[4] = 4.5 * (L03 - 1.) * d;
+        v[5] = 4.5 * (L13 - 1.) * d;
+        d = L0 * L2;
+        v[6] = 4.5 * (L03 - 1.) * d;
+        v[7] = 4.5 * (L23 - 1.) * d;
+        d = L0 * L3;
+        v[8] = 4.5 * (L03 - 1.) * d;
+        v[9] = 4.5 * (L33 - 1.) * d;
+        d = L1 * L2;
+        v[10] = 4.5 * (L13 - 1.) * d;
+        v[11] = 4.5 * (L23 - 1.) * d;
+        d = L1 * L3;
+        v[12] = 4.5 * (L13 - 1.) * d;
+        v[13] = 4.5 * (L33 - 1.) * d;
+        d = L2 * L3;
+        v[14] = 4.5 * (L23 - 1.) * d;
+        v[15] = 4.5 * (L33 - 1.) * d;
+        v[16] = 27. * L1 * L2 * L3;
+        v[17] = 27. * L0 * L2 * L3;
+        v[18] = 27. * L0 * L1 * L3;
+        v[19] = 27. * L0 * L1 * L2;
+    }
+}
+
+// lagrange.c:111 / :248 / :538: gradients with respect to lambda, [nb][4]
+__device__ inline void bas_grad_lambda(int p, const double *lam, double *g)
+{
+    int nb = p == 1 ? 4 : (p == 2 ? 10 : 20);
+    for (int i = 0; i < 4 * nb; i++)
+        g[i] = 0.;
+    if (p == 1) {
+        for (int k = 0; k < 4; k++)
+            g[4 * k + k] = 1.0;
+        return;
+    }
+    if (p == 2) {
+        for (int k = 0; k < 4; k++)
+            g[4 * k + k] = 4. * lam[k] - 1.;
+        for (int k = 0; k < 6; k++) {
+            int a = c_ev[k][0], b = c_ev[k][1];
+            g[4 * (4 + k) + a] = 4. * lam[b];
+            g[4 * (4 + k) + b] = 4. * lam[a];
+        }
+        return;
+    }
+    for (int k = 0; k < 4; k++)
+        g[4 * k + k] = (13.5 * lam[k] - 9.) * lam[k] + 1.;
+    for (int k = 0; k < 6; k++) {
+        int a = c_ev[k][0], b = c_ev[k][1];
+        double *g0 = g + 4 * (4 + 2 * k), *g1 = g0 + 4;
+        g0[a] = (27. * lam[a] - 4.5) * lam[b];
+        g0[b] = 4.5 * lam[a] * (3. * lam[a] - 1.);
+        g1[a] = 4.5 * lam[b] * (3. * lam[b] - 1.);
+        g1[b] = (27. * lam[b] - 4.5) * lam[a];
+    }
+    for (int k = 0; k < 4; k++) {
+        int a = c_fv[k][0], b = c_fv[k][1], c = c_fv[k][2];
+        double *gg = g + 4 * (16 + k);
+        gg[a] = 27. * lam[b] * lam[c];
+        gg[b] = 27. * lam[a] * lam[c];
+        gg[c] = 27. * lam[a] * lam[b];
+    }
+}
+
+//--------------------------------------------------------------- geometry
+// geom.c:103 compute_jacobian, Gauss-Jordan with partial pivoting
+__device__ inline void jacobian(const double x[4][3], double *J)
+{
+    const int n = 4;
+    double d, t, a[4][8];
+#pragma unroll
+    for (int j = 0; j < n; j++)
+#pragma unroll
+        for (int i = 0; i < n; i++)
+            a[j][n + i] = (i == j) ? 1.0 : 0.0;
+#pragma unroll
+    for (int j = 0; j < n; j++) {
+        a[j][0] = x[j][0];
+        a[j][1] = x[j][1];
+        a[j][2] = x[j][2];
+        a[j][3] = 1.0;
+    }
+#pragma unroll
+    for (int j = 0; j < n; j++) {
+        d = fabs(a[j][j]);
+        int k = j;
+#pragma unroll
+        for (int i = j + 1; i < n; i++)
+            if (d < (t = fabs(a[i][j]))) {
+                d = t;
+                k = i;
+            }
+        // row swap written as selects so the matrix stays in registers
+#pragma unroll
+        for (int i = j + 1; i < n; i++) {
+            if (i == k) {
+#pragma unroll
+                for (int c = 0; c < 2 * n; c++) {
+                    if (c < j)
+                        continue;
+                    t = a[j][c];
+                    a[j][c] = a[i][c];
+                    a[i][c] = t;
+                }
+            }
+        }
+        d = 1.0 / ((t = a[j][j]) == 0. ? 1. : t);
+#pragma unroll
+        for (int i = j + 1; i < 2 * n; i++)
+            a[j][i] *= d;
+#pragma unroll
+        for (int i = j + 1; i < n; i++) {
+            d = a[i][j];
+#pragma unroll
+            for (int c = j + 1; c < 2 * n; c++)
+                a[i][c] -= d * a[j][c];
+        }
+    }
+#pragma unroll
+    for (int j = n - 2; j >= 0; j--)
+#pragma unroll
+        for (int i = j + 1; i < n; i++) {
+            d = a[j][i];
+#pragma unroll
+            for (int c = n; c < 2 * n; c++)
+                a[j][c] -= d * a[i][c];
+        }
+#pragma unroll
+    for (int j = 0; j < n; j++)
+#pragma unroll
+        for (int i = 0; i < n; i++)
+            J[4 * j + i] = a[i][j + n];
+}
+
+// geom.c:46 _phg_compute_elem_data (volume)
+__device__ inline double volume(const double x[4][3])
+{
+    const double oned6 = 1.0 / (double)6.0;
+    double x0 = x[0][0], y0 = x[0][1], z0 = x[0][2];
+    double x1 = x[1][0] - x0, y1 = x[1][1] - y0, z1 = x[1][2] - z0;
+    double x2 = x[2][0] - x0, y2 = x[2][1] - y0, z2 = x[2][2] - z0;
+    double x3 = x[3][0] - x0, y3 = x[3][1] - y0, z3 = x[3][2] - z0;
+    return fabs(x1 * y2 * z3 + x2 * y3 * z1 + y1 * z2 * x3 -
+                (z1 * y2 * x3 + y1 * x2 * z3 + z2 * y3 * x1)) * oned6;
+}
+
+// quad.c:3314 phgQuadGetBasisGradient for basis k with lambda gradient G
+template <int P>
+__device__ inline void xgrad(int k, const double *G, const double *J,
+                             double *o)
+{
+    if (P == 1) {
+        o[0] = J[4 * k + 0];
+        o[1] = J[4 * k + 1];
+        o[2] = J[4 * k + 2];
+    }
+    else if (k < 4) {
+        double d = G[k];
+        o[0] = d * J[4 * k + 0];
+        o[1] = d * J[4 * k + 1];
+        o[2] = d * J[4 * k + 2];
+    }
+    else {
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            o[c] = G[0] * J[0 * 4 + c] + G[1] * J[1 * 4 + c] +
+                   G[2] * J[2 * 4 + c] + G[3] * J[3 * 4 + c];
+    }
+}
+
+//--------------------------------------------------------- element kernel
+struct ElemArgs {
+    int64_t nelem;
+    const double *verts;
+    const int32_t *elem_verts;
+    const int32_t *map;      // [nelem][N]
+    const double *kappa;     // [nelem] or null
+    const double *f;         // [ndof*dim]
+    int use_kappa;
+    double mass_coef;
+    int rhs_order;
+    double E, nu;
+    double *K;               // [nelem][M][M]
+    double *B;               // [nelem][M]
+};
+
+struct Tables {
+    int nq_s, nq_m, nq_r;
+    double ws[NQ], wm[NQ], wr[NQ];
+    double gs[NQ][NB][4];   // lambda gradients at the stiffness points
+    double bm[NQ][NB];      // basis values at the mass points
+    double br[NQ][NB];      // basis values at the load points
+};
+
+template <int P>
+__device__ void build_tables(Tables &T, int dim, int rhs_order, int mass)
+{
+    constexpr int N = P == 1 ? 4 : (P == 2 ? 10 : 20);
+    // all work-items build the same rules; thread 0 publishes them
+    if (threadIdx.x == 0) {
+        Rule q;
+        make_rule(2 * (P - 1), q);   // stiffness (quad.c:5271; elastic.c:141)
+        T.nq_s = q.np;
+        for (int i = 0; i < q.np; i++) {
+            T.ws[i] = q.w[i];
+            double g[NB * 4];
+            bas_grad_lambda(P, q.pts[i], g);
+            for (int k = 0; k < N; k++)
+                for (int c = 0; c < 4; c++)
+                    T.gs[i][k][c] = g[4 * k + c];
+        }
+        T.nq_m = 0;
+        if (mass) {
+            make_rule(2 * P, q);     // quad.c:5582 order p+p
+            T.nq_m = q.np;
+            for (int i = 0; i < q.np; i++) {
+                T.wm[i] = q.w[i];
+                bas_values(P, q.pts[i], T.bm[i]);
+            }
+        }
+        make_rule(rhs_order < 0 ? 2 * P : rhs_order, q);   // quad.c:5495
+        T.nq_r = q.np;
+        for (int i = 0; i < q.np; i++) {
+            T.wr[i] = q.w[i];
+            bas_values(P, q.pts[i], T.br[i]);
+        }
+    }
+    (void)dim;
+    __syncthreads();
+}
+
+template <int P, int DIM>
+__global__ void __launch_bounds__(128) k_element(ElemArgs a)
+{
+    constexpr int N = P == 1 ? 4 : (P == 2 ? 10 : 20);
+    constexpr int M = N * DIM;
+    __shared__ Tables T;
+    build_tables<P>(T, DIM, a.rhs_order, a.mass_coef != 0.0);
+
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.nelem)
+        return;
+
+    double x[4][3];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        int64_t iv = a.elem_verts[4 * e + v];
+        x[v][0] = a.verts[3 * iv + 0];
+        x[v][1] = a.verts[3 * iv + 1];
+        x[v][2] = a.verts[3 * iv + 2];
+    }
+    double J[16];
+    jacobian(x, J);
+    double vol = volume(x);
+    double *K = a.K + e * (int64_t)(M * M);
+    double *B = a.B + e * (int64_t)M;
+    const int32_t *m = a.map + e * (int64_t)N;
+
+    constexpr int NQS = P == 1 ? 1 : (P == 2 ? 4 : 14);   // rule 2(P-1)
+    if (DIM == 1) {
+        const double kap = a.use_kappa ? a.kappa[e] : 0.0;
+        const int nq = NQS;
+        for (int i = 0; i < N; i++) {
+            double gi[NQS][3];
+            for (int q = 0; q < nq; q++)
+                xgrad<P>(i, T.gs[q][i], J, gi[q]);
+            for (int j = 0; j <= i; j++) {
+                // phgQuadGradBasAGradBas(e, u, j, A, u, i): g1 = grad j
+                double d = 0., d0;
+                for (int q = 0; q < nq; q++) {
+                    double g1[3];
+                    xgrad<P>(j, T.gs[q][j], J, g1);
+                    d0 = 0.;
+                    d0 += g1[0] * gi[q][0] + g1[1] * gi[q][1] +
+                          g1[2] * gi[q][2];
+                    if (a.use_kappa)
+                        d += d0 * T.ws[q] * kap;
+                    else
+                        d += d0 * T.ws[q];
+                }
+                double v = d * vol;
+                K[i * N + j] = v;
+                K[j * N + i] = v;
+            }
+        }
+        if (a.mass_coef != 0.0) {
+            // simplest.c:70 + a * phgQuadBasDotBas (quad.c:5582)
+            for (int i = 0; i < N; i++)
+                for (int j = 0; j <= i; j++) {
+                    double d = 0., d0;
+                    for (int q = 0; q < T.nq_m; q++) {
+                        d0 = 0.;
+                        d0 += T.bm[q][j] * T.bm[q][i];
+                        d += d0 * T.wm[q];
+                    }
+                    double v = K[i * N + j] + a.mass_coef * (d * vol);
+                    K[i * N + j] = v;
+                    K[j * N + i] = v;
+                }
+        }
+        // quad.c:5481 phgQuadDofTimesBas with f_h at the points from
+        // dof-utils.c:59 phgDofEval_ (basis order)
+        double fl[N];
+#pragma unroll
+        for (int k = 0; k < N; k++)
+            fl[k] = a.f[m[k]];
+        double fq[NQ];
+        for (int q = 0; q < T.nq_r; q++) {
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < N; k++)
+                s += T.br[q][k] * fl[k];
+            fq[q] = s;
+        }
+        for (int i = 0; i < N; i++) {
+            double d = 0.0;
+            for (int q = 0; q < T.nq_r; q++)
+                d += T.br[q][i] * T.wr[q] * fq[q];
+            B[i] = d * vol;
+        }
+        return;
+    }
+
+    // linear elasticity, examples/elastic.c:90 stiffness_matrix
+    {
+        const double E = a.E, nu = a.nu;
+        const double aa = nu / (1.0 - nu);
+        const double bb = (1. - 2. * nu) / (2. * (1. - nu));
+        const double c = E * (1. - nu) / ((1. + nu) * (1. - 2. * nu)) * vol;
+        const int nq = NQS;
+        for (int i = 0; i < N; i++) {
+            double gi[NQS][3];
+            for (int q = 0; q < nq; q++)
+                xgrad<P>(i, T.gs[q][i], J, gi[q]);
+            for (int j = 0; j <= i; j++) {
+                double a00, a01, a02, a10, a11, a12, a20, a21, a22;
+                a00 = a01 = a02 = a10 = a11 = a12 = a20 = a21 = a22 = 0.;
+                for (int q = 0; q < nq; q++) {
+                    double pn[3];
+                    const double *pm = gi[q];
+                    xgrad<P>(j, T.gs[q][j], J, pn);
+                    double w = T.ws[q];
+                    a00 += (pm[0] * pn[0] + bb * (pm[1] * pn[1] + pm[2] * pn[2])) * w;
+                    a01 += (aa * pm[0] * pn[1] + bb * pm[1] * pn[0]) * w;
+                    a02 += (aa * pm[0] * pn[2] + bb * pm[2] * pn[0]) * w;
+                    a10 += (aa * pm[1] * pn[0] + bb * pm[0] * pn[1]) * w;
+                    a11 += (pm[1] * pn[1] + bb * (pm[0] * pn[0] + pm[2] * pn[2])) * w;
+                    a12 += (aa * pm[1] * pn[2] + bb * pm[2] * pn[1]) * w;
+                    a20 += (aa * pm[2] * pn[0] + bb * pm[0] * pn[2]) * w;
+                    a21 += (aa * pm[2] * pn[1] + bb * pm[1] * pn[2]) * w;
+                    a22 += (pm[2] * pn[2] + bb * (pm[0] * pn[0] + pm[1] * pn[1])) * w;
+                }
+                double A3[3][3] = {{a00 * c, a01 * c, a02 * c},
+                                   {a10 * c, a11 * c, a12 * c},
+                                   {a20 * c, a21 * c, a22 * c}};
+                for (int k = 0; k < 3; k++)
+                    for (int l = 0; l < 3; l++) {
+                        K[(i * 3 + k) * M + j * 3 + l] = A3[k][l];
+                        if (j != i)
+                            K[(j * 3 + k) * M + i * 3 + l] = A3[l][k];
+                    }
+            }
+        }
+        // load: vector path of phgQuadDofTimesBas, negated (elastic.c:212)
+        double fq[NQ][3];
+        for (int q = 0; q < T.nq_r; q++) {
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+            for (int k = 0; k < N; k++) {
+                int64_t d = (int64_t)m[k] * 3;
+                double bv = T.br[q][k];
+                s0 += bv * a.f[d + 0];
+                s1 += bv * a.f[d + 1];
+                s2 += bv * a.f[d + 2];
+            }
+            fq[q][0] = s0;
+            fq[q][1] = s1;
+            fq[q][2] = s2;
+        }
+        for (int i = 0; i < N; i++) {
+            double r0 = 0, r1 = 0, r2 = 0, d;
+            for (int q = 0; q < T.nq_r; q++) {
+                d = T.br[q][i] * T.wr[q];
+                r0 += d * fq[q][0];
+                r1 += d * fq[q][1];
+                r2 += d * fq[q][2];
+            }
+            r0 *= vol;
+            r1 *= vol;
+            r2 *= vol;
+            B[3 * i + 0] = -r0;
+            B[3 * i + 1] = -r1;
+            B[3 * i + 2] = -r2;
+        }
+    }
+}
+
+//-------------------------------------------------------- incidence build
+// row of local index a of element e
+__device__ inline int64_t row_of(const int32_t *map, int64_t e, int a, int N,
+                                 int dim)
+{
+    return (int64_t)map[e * N + a / dim] * dim + a % dim;
+}
+
+__global__ void k_inc_count(int64_t nelem, int N, int dim,
+                            const int32_t *map, int32_t *cnt)
+{
+    int M = N * dim;
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nelem * M)
+        return;
+    int64_t e = t / M;
+    int a = (int)(t % M);
+    atomicAdd(&cnt[row_of(map, e, a, N, dim)], 1);
+}
+
+__global__ void k_inc_fill(int64_t nelem, int N, int dim, const int32_t *map,
+                           const int64_t *off, int32_t *cursor, int32_t *inc)
+{
+    int M = N * dim;
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nelem * M)
+        return;
+    int64_t e = t / M;
+    int a = (int)(t % M);
+    int64_t r = row_of(map, e, a, N, dim);
+    int pos = atomicAdd(&cursor[r], 1);
+    inc[off[r] + pos] = (int32_t)t;
+}
+
+// insertion sort of each row's incidence list (lists are short)
+__global__ void k_inc_sort(int64_t n, const int64_t *off, int32_t *inc)
+{
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n)
+        return;
+    int64_t s = off[r], t = off[r + 1];
+    for (int64_t i = s + 1; i < t; i++) {
+        int32_t v = inc[i];
+        int64_t j = i - 1;
+        while (j >= s && inc[j] > v) {
+            inc[j + 1] = inc[j];
+            j--;
+        }
+        inc[j + 1] = v;
+    }
+}
+
+//---------------------------------------------------------------- row walk
+struct WalkArgs {
+    int64_t n;
+    int N, dim;
+    const int32_t *map;
+    const int64_t *inc_off;
+    const int32_t *inc;
+    const double *K;        // [nelem][M][M]
+    const double *B;        // [nelem][M]
+    const uint8_t *bdry;    // per row
+    const double *g;        // per row
+    const int64_t *scr_off; // scratch segment of each row
+    int32_t *scols;
+    double *svals;
+    int32_t *count;         // [n] entries of each row
+    double *rhs;            // [n]
+};
+
+__global__ void __launch_bounds__(256) k_row_walk(WalkArgs w)
+{
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= w.n)
+        return;
+    const int M = w.N * w.dim;
+    int32_t *cols = w.scols + w.scr_off[r];
+    double *vals = w.svals + w.scr_off[r];
+    int32_t nc = 0;
+    double rhs = 0.0;
+    const bool isb = w.bdry[r] != 0;
+    for (int64_t t = w.inc_off[r]; t < w.inc_off[r + 1]; t++) {
+        int32_t code = w.inc[t];
+        int64_t e = code / M;
+        int a = code % M;
+        double bval;
+        if (isb) {
+            // Dirichlet row e_i (dof-utils.c:1369): only the diagonal 1.0
+            int32_t c = (int32_t)r;
+            int32_t i;
+            for (i = 0; i < nc; i++)
+                if (cols[i] == c)
+                    break;
+            if (i < nc)
+                vals[i] += 1.0;
+            else {
+                cols[nc] = c;
+                vals[nc++] = 1.0;
+            }
+            bval = w.g[r];
+        }
+        else {
+            const double *Krow = w.K + (e * M + a) * (int64_t)M;
+            for (int b = 0; b < M; b++) {
+                double v = Krow[b];
+                if (v == 0.0)
+                    continue;   // matvec.c:5460
+                int32_t c = (int32_t)row_of(w.map, e, b, w.N, w.dim);
+                int32_t i;
+                for (i = 0; i < nc; i++)
+                    if (cols[i] == c)
+                        break;
+                if (i < nc)
+                    vals[i] += v;
+                else {
+                    cols[nc] = c;
+                    vals[nc++] = v;
+                }
+            }
+            bval = w.B[e * M + a];
+        }
+        if (bval != 0.0)        // matvec.c:5708
+            rhs += bval;
+    }
+    w.count[r] = nc;
+    w.rhs[r] = rhs;
+}
+
+__global__ void k_compact(int64_t n, const int64_t *scr_off,
+                          const int32_t *scols, const double *svals,
+                          const int64_t *rowptr, int32_t *cols, double *vals)
+{
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n)
+        return;
+    int64_t s = scr_off[r], d = rowptr[r], len = rowptr[r + 1] - rowptr[r];
+    for (int64_t i = 0; i < len; i++) {
+        cols[d + i] = scols[s + i];
+        vals[d + i] = svals[s + i];
+    }
+}
+
+//-------------------------------------------------------------- elimination
+__global__ void k_elim_count(int64_t n, const int32_t *rowptr,
+                             const int32_t *cols, const uint8_t *bdry,
+                             int32_t *count)
+{
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n)
+        return;
+    int32_t k = 0;
+    for (int32_t j = rowptr[r]; j < rowptr[r + 1]; j++) {
+        int32_t c = cols[j];
+        if (c == r || !bdry[c])
+            k++;
+    }
+    count[r] = k;
+}
+
+__global__ void k_elim_fill(int64_t n, const int32_t *rowptr,
+                            const int32_t *cols, const double *vals,
+                            const double *rhs, const uint8_t *bdry,
+                            const int64_t *nrowptr, int32_t *ncols,
+                            double *nvals, double *nrhs, double *diag)
+{
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n)
+        return;
+    int64_t w = nrowptr[r];
+    double b = rhs[r], dg = 0.0;
+    bool have_diag = false;
+    for (int32_t j = rowptr[r]; j < rowptr[r + 1]; j++) {
+        int32_t c = cols[j];
+        double v = vals[j];
+        if (c == r || !bdry[c]) {
+            ncols[w] = c;
+            nvals[w++] = v;
+            if (c == r && !have_diag) {
+                dg = v;
+                have_diag = true;
+            }
+        }
+        if (bdry[c] && c != r) {
+            // solver.c:722, d_k is the diagonal of boundary row k
+            double dk = 0.0;
+            for (int32_t jj = rowptr[c]; jj < rowptr[c + 1]; jj++)
+                if (cols[jj] == c) {
+                    dk = vals[jj];
+                    break;
+                }
+            b -= 1.0 * rhs[c] * v / dk;
+        }
+    }
+    nrhs[r] = b;
+    diag[r] = dg;
+}
+
+//------------------------------------------------------------------ scan
+// exclusive prefix sum of int32 counts into int64 offsets (n+1 entries)
+__global__ void k_scan_blocks(int64_t n, const int32_t *in, int64_t *out,
+                              int64_t *bsum)
+{
+    __shared__ int64_t s[1024];
+    int64_t i = (int64_t)blockIdx.x * 1024 + threadIdx.x;
+    int64_t v = i < n ? in[i] : 0;
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        int64_t t = threadIdx.x >= (unsigned)o ? s[threadIdx.x - o] : 0;
+        __syncthreads();
+        s[threadIdx.x] += t;
+        __syncthreads();
+    }
+    if (i < n)
+        out[i] = s[threadIdx.x] - v;
+    if (threadIdx.x == 1023)
+        bsum[blockIdx.x] = s[1023];
+}
+
+__global__ void k_scan_sums(int64_t nb, int64_t *bsum, int64_t *total)
+{
+    // single work-item: the number of blocks is n/1024
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        int64_t acc = 0;
+        for (int64_t i = 0; i < nb; i++) {
+            int64_t v = bsum[i];
+            bsum[i] = acc;
+            acc += v;
+        }
+        *total = acc;
+    }
+}
+
+__global__ void k_scan_add(int64_t n, int64_t *out, const int64_t *bsum,
+                           const int64_t *total)
+{
+    int64_t i = (int64_t)blockIdx.x * 1024 + threadIdx.x;
+    if (i < n)
+        out[i] += bsum[blockIdx.x];
+    if (i == n)
+        out[n] = *total;
+}
+
+__global__ void k_to_int32(int64_t n, const int64_t *in, int32_t *out)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= n)
+        out[i] = (int32_t)in[i];
+}
+
+__global__ void k_scale_i32(int64_t n, const int32_t *in, int32_t mul,
+                            int32_t *out)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        out[i] = in[i] * mul;
+}
+
+inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// out[0..n] = exclusive scan of in[0..n-1]; returns the total in *total_h
+int scan_i32(int64_t n, const int32_t *d_in, int64_t *d_out, int64_t *total_h,
+             hipStream_t s)
+{
+    int64_t nb = (n + 1 + 1023) / 1024;
+    int64_t *bsum = nullptr, *tot = nullptr;
+    int rc = phg::dmalloc((void **)&bsum, (size_t)(nb + 1) * sizeof(int64_t));
+    if (rc)
+        return rc;
+    tot = bsum + nb;
+    hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(1024), 0, s, n,
+                       d_in, d_out, bsum);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1), 0, s, nb, bsum, tot);
+    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(1024), 0, s, n,
+                       d_out, bsum, tot);
+    PHG_HIP_CHECK(hipGetLastError());
+    if (total_h) {
+        PHG_HIP_CHECK(hipMemcpyAsync(total_h, tot, sizeof(int64_t),
+                                     hipMemcpyDeviceToHost, s));
+    }
+    PHG_HIP_CHECK(hipStreamSynchronize(s));
+    phg::dfree(bsum);
+    return PHG_HIP_OK;
+}
+
+// launches k_element for the space's order and the problem's dim
+int launch_element(const phg_hip_space *sp, const phg_hip_problem *pb,
+                   const ElemArgs &a, hipStream_t s)
+{
+    dim3 grid(nblk(sp->nelem, 128)), block(128);
+    if (sp->nelem == 0)
+        return PHG_HIP_OK;
+    if (pb->dim == 1) {
+        if (sp->order == 1)
+            hipLaunchKernelGGL((k_element<1, 1>), grid, block, 0, s, a);
+        else if (sp->order == 2)
+            hipLaunchKernelGGL((k_element<2, 1>), grid, block, 0, s, a);
+        else
+            hipLaunchKernelGGL((k_element<3, 1>), grid, block, 0, s, a);
+    }
+    else {
+        if (sp->order == 1)
+            hipLaunchKernelGGL((k_element<1, 3>), grid, block, 0, s, a);
+        else if (sp->order == 2)
+            hipLaunchKernelGGL((k_element<2, 3>), grid, block, 0, s, a);
+        else
+            hipLaunchKernelGGL((k_element<3, 3>), grid, block, 0, s, a);
+    }
+    PHG_HIP_CHECK(hipGetLastError());
+    return PHG_HIP_OK;
+}
+
+int check_problem(const phg_hip_space *sp, const phg_hip_problem *pb)
+{
+    if (sp == nullptr || pb == nullptr)
+        return phg::set_error(PHG_HIP_E_ARG, "null space or problem");
+    if (pb->dim != 1 && pb->dim != 3)
+        return phg::set_error(PHG_HIP_E_ARG, "dim must be 1 or 3 (got %d)",
+                              pb->dim);
+    if (pb->dim == 3 && (pb->use_kappa || pb->mass_coef != 0.0))
+        return phg::set_error(PHG_HIP_E_ARG,
+                              "kappa/mass terms are scalar-only");
+    if (pb->rhs_order > 6)
+        return phg::set_error(PHG_HIP_E_ARG,
+                              "load quadrature order %d > 6 unsupported",
+                              pb->rhs_order);
+    if (pb->mass_coef != 0.0 && 2 * sp->order > 6)
+        return phg::set_error(PHG_HIP_E_ARG, "mass order too high");
+    return PHG_HIP_OK;
+}
+
+// element matrices into device buffers K (nelem*M*M) and B (nelem*M)
+int element_pass(phg_hip_space *sp, const phg_hip_problem *pb,
+                 const double *kappa_h, const double *f_h, double **dK,
+                 double **dB, double **dkap_out)
+{
+    hipStream_t s = phg::ctx()->stream;
+    int N = sp->nbas, M = N * pb->dim, rc;
+    double *dk = nullptr, *df = nullptr;
+    *dK = *dB = nullptr;
+    if (pb->use_kappa && kappa_h == nullptr)
+        return phg::set_error(PHG_HIP_E_ARG, "use_kappa set but kappa is NULL");
+    if (f_h == nullptr)
+        return phg::set_error(PHG_HIP_E_ARG, "load vector f is NULL");
+    size_t ksz = (size_t)sp->nelem * M * M * sizeof(double);
+    if ((rc = phg::dmalloc((void **)dK, ksz)) ||
+        (rc = phg::dmalloc((void **)dB, (size_t)sp->nelem * M * sizeof(double))) ||
+        (rc = phg::dmalloc((void **)&df, (size_t)sp->ndof * pb->dim * sizeof(double))))
+        goto fail;
+    if ((rc = phg::upload(df, f_h, (size_t)sp->ndof * pb->dim * sizeof(double))))
+        goto fail;
+    if (pb->use_kappa) {
+        if ((rc = phg::dmalloc((void **)&dk, (size_t)sp->nelem * sizeof(double))) ||
+            (rc = phg::upload(dk, kappa_h, (size_t)sp->nelem * sizeof(double))))
+            goto fail;
+    }
+    {
+        ElemArgs a;
+        a.nelem = sp->nelem;
+        a.verts = sp->d_verts;
+        a.elem_verts = sp->d_elem_verts;
+        a.map = sp->d_map;
+        a.kappa = dk;
+        a.f = df;
+        a.use_kappa = pb->use_kappa;
+        a.mass_coef = pb->mass_coef;
+        a.rhs_order = pb->rhs_order;
+        a.E = pb->E;
+        a.nu = pb->nu;
+        a.K = *dK;
+        a.B = *dB;
+        if ((rc = launch_element(sp, pb, a, s)))
+            goto fail;
+    }
+    {
+        hipError_t e = hipStreamSynchronize(s);
+        if (e != hipSuccess) {
+            rc = phg::hip_fail(e, "element kernel", __FILE__, __LINE__);
+            goto fail;
+        }
+    }
+    phg::dfree(df);
+    if (dkap_out)
+        *dkap_out = dk;
+    else
+        phg::dfree(dk);
+    return PHG_HIP_OK;
+fail:
+    phg::dfree(*dK);
+    phg::dfree(*dB);
+    phg::dfree(df);
+    phg::dfree(dk);
+    *dK = *dB = nullptr;
+    return rc;
+}
+
+}  // namespace
+
+//============================================================== C ABI ====
+
+extern "C" int phg_hip_space_create(int32_t order, int64_t nvert,
+                                    const double *verts, int64_t nelem,
+                                    const int32_t *elem_verts,
+                                    const int32_t *map, int64_t ndof,
+                                    const uint8_t *bdry, phg_hip_space **out)
+{
+    int rc;
+    if ((rc = phg::require_ctx()))
+        return rc;
+    if (out == nullptr || order < 1 || order > 3 || nvert <= 0 || nelem < 0 ||
+        ndof <= 0 || verts == nullptr || elem_verts == nullptr ||
+        map == nullptr || bdry == nullptr)
+        return phg::set_error(PHG_HIP_E_ARG, "phg_hip_space_create: bad args");
+    int nb = order == 1 ? 4 : (order == 2 ? 10 : 20);
+    // host-side bounds checks so that no kernel indexes out of range
+    for (int64_t i = 0; i < nelem * 4; i++)
+        if (elem_verts[i] < 0 || elem_verts[i] >= nvert)
+            return phg::set_error(PHG_HIP_E_ARG, "element vertex out of range");
+    for (int64_t i = 0; i < nelem * nb; i++)
+        if (map[i] < 0 || map[i] >= ndof)
+            return phg::set_error(PHG_HIP_E_ARG, "DOF index out of range");
+    if (nelem * (int64_t)nb * 3 >= (int64_t)INT32_MAX)
+        return phg::set_error(PHG_HIP_E_ARG, "mesh too large for int32 codes");
+    phg_hip_space *sp = new phg_hip_space();
+    sp->order = order;
+    sp->nbas = nb;
+    sp->nvert = nvert;
+    sp->nelem = nelem;
+    sp->ndof = ndof;
+    if ((rc = phg::dmalloc((void **)&sp->d_verts, (size_t)nvert * 3 * sizeof(double))) ||
+        (rc = phg::dmalloc((void **)&sp->d_elem_verts, (size_t)nelem * 4 * sizeof(int32_t))) ||
+        (rc = phg::dmalloc((void **)&sp->d_map, (size_t)nelem * nb * sizeof(int32_t))) ||
+        (rc = phg::dmalloc((void **)&sp->d_bdry, (size_t)ndof)) ||
+        (rc = phg::upload(sp->d_verts, verts, (size_t)nvert * 3 * sizeof(double))) ||
+        (rc = phg::upload(sp->d_elem_verts, elem_verts, (size_t)nelem * 4 * sizeof(int32_t))) ||
+        (rc = phg::upload(sp->d_map, map, (size_t)nelem * nb * sizeof(int32_t))) ||
+        (rc = phg::upload(sp->d_bdry, bdry, (size_t)ndof))) {
+        phg_hip_space_destroy(sp);
+        return rc;
+    }
+    *out = sp;
+    return PHG_HIP_OK;
+}
+
+extern "C" int phg_hip_space_destroy(phg_hip_space *sp)
+{
+    if (sp == nullptr)
+        return PHG_HIP_OK;
+    phg::dfree(sp->d_verts);
+    phg::dfree(sp->d_elem_verts);
+    phg::dfree(sp->d_map);
+    phg::dfree(sp->d_bdry);
+    delete sp;
+    return PHG_HIP_OK;
+}
+
+extern "C" int phg_hip_element_matrices(phg_hip_space *sp,
+                                        const phg_hip_problem *pb,
+                                        const double *kappa, const double *f,
+                                        double *K, double *b)
+{
+    int rc;
+    if ((rc = phg::require_ctx()) || (rc = check_problem(sp, pb)))
+        return rc;
+    double *dK, *dB;
+    if ((rc = element_pass(sp, pb, kappa, f, &dK, &dB, nullptr)))
+        return rc;
+    int M = sp->nbas * pb->dim;
+    if (K)
+        rc = phg::download(K, dK, (size_t)sp->nelem * M * M * sizeof(double));
+    if (!rc && b)
+        rc = phg::download(b, dB, (size_t)sp->nelem * M * sizeof(double));
+    phg::dfree(dK);
+    phg::dfree(dB);
+    return rc;
+}
+
+extern "C" int phg_hip_assemble(phg_hip_space *sp, const phg_hip_problem *pb,
+                                const double *kappa, const double *f,
+                                const double *g, phg_hip_mat **Aout)
+{
+    int rc;
+    if ((rc = phg::require_ctx()) || (rc = check_problem(sp, pb)))
+        return rc;
+    if (Aout == nullptr || g == nullptr)
+        return phg::set_error(PHG_HIP_E_ARG, "phg_hip_assemble: bad args");
+    hipStream_t s = phg::ctx()->stream;
+    const int N = sp->nbas, dim = pb->dim, M = N * dim;
+    const int64_t n = sp->ndof * dim, ne = sp->nelem;
+    double *dK = nullptr, *dB = nullptr, *dg = nullptr;
+    int32_t *cnt = nullptr, *inc = nullptr, *count = nullptr;
+    int64_t *inc_off = nullptr, *scr_off = nullptr, *rowptr64 = nullptr;
+    int32_t *scols = nullptr;
+    double *svals = nullptr;
+    uint8_t *bdry_row = nullptr;
+    int64_t ninc = 0, nscr = 0, nnz = 0;
+    phg_hip_mat *A = nullptr;
+    std::vector<uint8_t> brow;
+
+    if ((rc = element_pass(sp, pb, kappa, f, &dK, &dB, nullptr)))
+        return rc;
+
+    // per-row Dirichlet flags and values
+    {
+        std::vector<uint8_t> b1((size_t)sp->ndof);
+        if ((rc = phg::download(b1.data(), sp->d_bdry, (size_t)sp->ndof)))
+            goto done;
+        brow.resize((size_t)n);
+        for (int64_t i = 0; i < sp->ndof; i++)
+            for (int c = 0; c < dim; c++)
+                brow[(size_t)(i * dim + c)] = b1[(size_t)i];
+    }
+    if ((rc = phg::dmalloc((void **)&bdry_row, (size_t)n)) ||
+        (rc = phg::upload(bdry_row, brow.data(), (size_t)n)) ||
+        (rc = phg::dmalloc((void **)&dg, (size_t)n * sizeof(double))) ||
+        (rc = phg::upload(dg, g, (size_t)n * sizeof(double))))
+        goto done;
+
+    // incidence
+    if ((rc = phg::dmalloc((void **)&cnt, (size_t)(n + 1) * sizeof(int32_t))) ||
+        (rc = phg::dmalloc((void **)&inc_off, (size_t)(n + 1) * sizeof(int64_t))) ||
+        (rc = phg::dmalloc((void **)&inc, (size_t)(ne * M + 1) * sizeof(int32_t))))
+        goto done;
+    PHG_HIP_CHECK(hipMemsetAsync(cnt, 0, (size_t)(n + 1) * sizeof(int32_t), s));
+    hipLaunchKernelGGL(k_inc_count, dim3(nblk(ne * M, 256)), dim3(256), 0, s,
+                       ne, N, dim, (const int32_t *)sp->d_map, cnt);
+    if ((rc = scan_i32(n, cnt, inc_off, &ninc, s)))
+        goto done;
+    PHG_HIP_CHECK(hipMemsetAsync(cnt, 0, (size_t)(n + 1) * sizeof(int32_t), s));
+    hipLaunchKernelGGL(k_inc_fill, dim3(nblk(ne * M, 256)), dim3(256), 0, s,
+                       ne, N, dim, (const int32_t *)sp->d_map,
+                       (const int64_t *)inc_off, cnt, inc);
+    hipLaunchKernelGGL(k_inc_sort, dim3(nblk(n, 256)), dim3(256), 0, s, n,
+                       (const int64_t *)inc_off, inc);
+    // scratch segments: count * M entries per row (upper bound)
+    if ((rc = phg::dmalloc((void **)&count, (size_t)(n + 1) * sizeof(int32_t))) ||
+        (rc = phg::dmalloc((void **)&scr_off, (size_t)(n + 1) * sizeof(int64_t))))
+        goto done;
+    // count[r] = cnt[r] * M (cnt holds the incidence sizes after the fill)
+    hipLaunchKernelGGL(k_scale_i32, dim3(nblk(n, 256)), dim3(256), 0, s, n,
+                       (const int32_t *)cnt, (int32_t)M, count);
+    if ((rc = scan_i32(n, count, scr_off, &nscr, s)))
+        goto done;
+    if ((rc = phg::dmalloc((void **)&scols, (size_t)(nscr + 1) * sizeof(int32_t))) ||
+        (rc = phg::dmalloc((void **)&svals, (size_t)(nscr + 1) * sizeof(double))))
+        goto done;
+
+    A = new phg_hip_mat();
+    A->n = n;
+    if ((rc = phg::dmalloc((void **)&A->d_rhs, (size_t)n * sizeof(double))))
+        goto done;
+    {
+        WalkArgs w;
+        w.n = n;
+        w.N = N;
+        w.dim = dim;
+        w.map = sp->d_map;
+        w.inc_off = inc_off;
+        w.inc = inc;
+        w.K = dK;
+        w.B = dB;
+        w.bdry = bdry_row;
+        w.g = dg;
+        w.scr_off = scr_off;
+        w.scols = scols;
+        w.svals = svals;
+        w.count = count;
+        w.rhs = A->d_rhs;
+        hipLaunchKernelGGL(k_row_walk, dim3(nblk(n, 256)), dim3(256), 0, s, w);
+        PHG_HIP_CHECK(hipGetLastError());
+    }
+    if ((rc = phg::dmalloc((void **)&rowptr64, (size_t)(n + 1) * sizeof(int64_t))) ||
+        (rc = scan_i32(n, count, rowptr64, &nnz, s)))
+        goto done;
+    if (nnz >= (int64_t)INT32_MAX) {
+        rc = phg::set_error(PHG_HIP_E_ARG, "nnz %lld exceeds int32 CSR",
+                            (long long)nnz);
+        goto done;
+    }
+    A->nnz = nnz;
+    if ((rc = phg::dmalloc((void **)&A->d_rowptr, (size_t)(n + 1) * sizeof(int32_t))) ||
+        (rc = phg::dmalloc((void **)&A->d_cols, (size_t)(nnz + 1) * sizeof(int32_t))) ||
+        (rc = phg::dmalloc((void **)&A->d_vals, (size_t)(nnz + 1) * sizeof(double))))
+        goto done;
+    hipLaunchKernelGGL(k_compact, dim3(nblk(n, 256)), dim3(256), 0, s, n,
+                       (const int64_t *)scr_off, (const int32_t *)scols,
+                       (const double *)svals, (const int64_t *)rowptr64,
+                       A->d_cols, A->d_vals);
+    hipLaunchKernelGGL(k_to_int32, dim3(nblk(n + 1, 256)), dim3(256), 0, s, n,
+                       (const int64_t *)rowptr64, A->d_rowptr);
+    PHG_HIP_CHECK(hipGetLastError());
+    PHG_HIP_CHECK(hipStreamSynchronize(s));
+    A->d_bdry_row = bdry_row;
+    bdry_row = nullptr;
+    *Aout = A;
+    A = nullptr;
+    rc = PHG_HIP_OK;
+done:
+    if (A)
+        phg_hip_mat_destroy(A);
+    phg::dfree(dK);
+    phg::dfree(dB);
+    phg::dfree(dg);
+    phg::dfree(cnt);
+    phg::dfree(inc);
+    phg::dfree(inc_off);
+    phg::dfree(count);
+    phg::dfree(scr_off);
+    phg::dfree(scols);
+    phg::dfree(svals);
+    phg::dfree(rowptr64);
+    phg::dfree(bdry_row);
+    return rc;
+}
+
+extern "C" int phg_hip_mat_eliminate(phg_hip_mat *A)
+{
+    int rc;
+    if ((rc = phg::require_ctx()))
+        return rc;
+    if (A == nullptr)
+        return phg::set_error(PHG_HIP_E_ARG, "null matrix");
+    if (A->eliminated)
+        return PHG_HIP_OK;
+    hipStream_t s = phg::ctx()->stream;
+    int64_t n = A->n, nnz = 0;
+    int32_t *count = nullptr, *ncols = nullptr, *nrow32 = nullptr;
+    int64_t *nrow = nullptr;
+    double *nvals = nullptr, *nrhs = nullptr, *diag = nullptr;
+    if ((rc = phg::dmalloc((void **)&count, (size_t)(n + 1) * sizeof(int32_t))) ||
+        (rc = phg::dmalloc((void **)&nrow, (size_t)(n + 1) * sizeof(int64_t))))
+        goto fail;
+    hipLaunchKernelGGL(k_elim_count, dim3(nblk(n, 256)), dim3(256), 0, s, n,
+                       (const int32_t *)A->d_rowptr, (const int32_t *)A->d_cols,
+                       (const uint8_t *)A->d_bdry_row, count);
+    if ((rc = scan_i32(n, count, nrow, &nnz, s)))
+        goto fail;
+    if ((rc = phg::dmalloc((void **)&ncols, (size_t)(nnz + 1) * sizeof(int32_t))) ||
+        (rc = phg::dmalloc((void **)&nvals, (size_t)(nnz + 1) * sizeof(double))) ||
+        (rc = phg::dmalloc((void **)&nrhs, (size_t)n * sizeof(double))) ||
+        (rc = phg::dmalloc((void **)&diag, (size_t)n * sizeof(double))) ||
+        (rc = phg::dmalloc((void **)&nrow32, (size_t)(n + 1) * sizeof(int32_t))))
+        goto fail;
+    hipLaunchKernelGGL(k_elim_fill, dim3(nblk(n, 256)), dim3(256), 0, s, n,
+                       (const int32_t *)A->d_rowptr, (const int32_t *)A->d_cols,
+                       (const double *)A->d_vals, (const double *)A->d_rhs,
+                       (const uint8_t *)A->d_bdry_row, (const int64_t *)nrow,
+                       ncols, nvals, nrhs, diag);
+    hipLaunchKernelGGL(k_to_int32, dim3(nblk(n + 1, 256)), dim3(256), 0, s, n,
+                       (const int64_t *)nrow, nrow32);
+    PHG_HIP_CHECK(hipGetLastError());
+    PHG_HIP_CHECK(hipStreamSynchronize(s));
+    phg::dfree(A->d_rowptr);
+    phg::dfree(A->d_cols);
+    phg::dfree(A->d_vals);
+    phg::dfree(A->d_rhs);
+    phg::dfree(A->d_diag);
+    phg::dfree(A->d_vals32);
+    phg::dfree(A->d_diag32);
+    A->d_vals32 = nullptr;
+    A->d_diag32 = nullptr;
+    A->d_rowptr = nrow32;
+    A->d_cols = ncols;
+    A->d_vals = nvals;
+    A->d_rhs = nrhs;
+    A->d_diag = diag;
+    A->nnz = nnz;
+    A->eliminated = 1;
+    phg::dfree(count);
+    phg::dfree(nrow);
+    return phg::choose_spmv_shape(A);
+fail:
+    phg::dfree(count);
+    phg::dfree(nrow);
+    phg::dfree(ncols);
+    phg::dfree(nvals);
+    phg::dfree(nrhs);
+    phg::dfree(diag);
+    phg::dfree(nrow32);
+    return rc;
+}
+
+extern "C" int phg_hip_mat_info(const phg_hip_mat *A, int64_t *nrows,
+                                int64_t *nnz)
+{
+    if (A == nullptr)
+        return phg::set_error(PHG_HIP_E_ARG, "null matrix");
+    if (nrows)
+        *nrows = A->n;
+    if (nnz)
+        *nnz = A->nnz;
+    return PHG_HIP_OK;
+}
+
+extern "C" int phg_hip_mat_download(const phg_hip_mat *A, int64_t *rowptr,
+                                    int32_t *cols, double *vals, double *rhs)
+{
+    int rc;
+    if ((rc = phg::require_ctx()))
+        return rc;
+    if (A == nullptr)
+        return phg::set_error(PHG_HIP_E_ARG, "null matrix");
+    if (rowptr) {
+        std::vector<int32_t> r32((size_t)A->n + 1);
+        if ((rc = phg::download(r32.data(), A->d_rowptr,
+                                (size_t)(A->n + 1) * sizeof(int32_t))))
+            return rc;
+        for (int64_t i = 0; i <= A->n; i++)
+            rowptr[i] = r32[(size_t)i];
+    }
+    if (cols && (rc = phg::download(cols, A->d_cols, (size_t)A->nnz * sizeof(int32_t))))
+        return rc;
+    if (vals && (rc = phg::download(vals, A->d_vals, (size_t)A->nnz * sizeof(double))))
+        return rc;
+    if (rhs && (rc = phg::download(rhs, A->d_rhs, (size_t)A->n * sizeof(double))))
+        return rc;
+    return PHG_HIP_OK;
+}
+
+extern "C" int phg_hip_mat_destroy(phg_hip_mat *A)
+{
+    if (A == nullptr)
+        return PHG_HIP_OK;
+    phg::dfree(A->d_rowptr);
+    phg::dfree(A->d_cols);
+    phg::dfree(A->d_vals);
+    phg::dfree(A->d_rhs);
+    phg::dfree(A->d_bdry_row);
+    phg::dfree(A->d_diag);
+    phg::dfree(A->d_vals32);
+    phg::dfree(A->d_diag32);
+    delete A;
+    return PHG_HIP_OK;
+}

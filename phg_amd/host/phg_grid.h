@@ -1,0 +1,85 @@
+/* Host-side tetrahedral grid and Lagrange DOF layout with PHG's numbering.
+ *
+ * This is the host prerequisite of the GPU path: it produces the element ->
+ * DOF maps, nodal coordinates and Dirichlet flags that the HIP assembly
+ * consumes.  The numbering restates PHG's import pipeline for a flat mesh:
+ *   - vertices keep file order (albert.c:351 phgImportALBERT),
+ *   - edges are numbered on first encounter while elements are traversed in
+ *     index order and local edges 0..5 (utils.c:4282 update_edges_callback),
+ *   - a face shared by two elements is numbered when the element whose
+ *     opposite vertex has the smaller global index is visited; boundary faces
+ *     when their element is visited (utils.c:4499 update_faces_callback1),
+ *   - boundary types of faces are OR-ed onto their vertices and edges
+ *     (utils.c:962 types_callback),
+ *   - DOF data are laid out vertex | edge | face | element with P3 edge pairs
+ *     ordered by global vertex index (dof.c:4821 phgDofMapE2D_,
+ *     dof.c:978 phgDofMapEdgeData).
+ * Meshes read from ALBERT files that carry an "element type" section are
+ * imported without the vertex permutation of refine.c:1916 phgRefineInit,
+ * which is what PHG does for such files.
+ */
+#ifndef PHG_GRID_H
+#define PHG_GRID_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* boundary type bits (PHG grid.h semantics for the bits used here) */
+#define PHG_BT_INTERIOR   0u
+#define PHG_BT_DIRICHLET  1u
+#define PHG_BT_NEUMANN    2u
+
+typedef struct phg_grid {
+    int32_t nvert, nelem, nedge, nface;
+    double *verts;          /* [nvert][3] */
+    int32_t *elem_verts;    /* [nelem][4] */
+    int32_t *elem_edges;    /* [nelem][6] */
+    int32_t *elem_faces;    /* [nelem][4] */
+    int32_t *elem_neigh;    /* [nelem][4], -1 on the boundary */
+    uint8_t *elem_bound;    /* [nelem][4] boundary type of face i (opposite vertex i) */
+    uint8_t *types_vert;    /* [nvert] */
+    uint8_t *types_edge;    /* [nedge] */
+    uint8_t *types_face;    /* [nface] */
+} phg_grid;
+
+/* Local edge -> vertex table (elem-info.c:62 phgTetEdgeVertex). */
+extern const int phg_tet_edge_vertex[6][2];
+
+/* Structured Kuhn mesh of the unit cube: n^3 cubes, 6 tetrahedra each, every
+ * tetrahedron listing its cube's (0,0,0) and (1,1,1) corners first, in the
+ * pattern of test/cube.dat.  bcode[6] gives the ALBERT boundary code of the
+ * faces x=0, x=1, y=0, y=1, z=0, z=1 (1 Dirichlet, -1 Neumann, 0 interior).
+ * Numbering is applied (phg_grid_number) before returning. */
+phg_grid *phg_grid_new_kuhn(int n, const int bcode[6]);
+
+/* ALBERT text format (albert.c:351).  The writer always emits an
+ * "element type" section so the reader imports vertex order unchanged.
+ * Return 0 on success. */
+int phg_grid_write_albert(const phg_grid *g, const char *path);
+phg_grid *phg_grid_read_albert(const char *path);
+
+/* Computes neighbours, edge and face numbering and boundary types. */
+void phg_grid_number(phg_grid *g);
+void phg_grid_free(phg_grid *g);
+
+/* Lagrange Pk (k = 1, 2, 3) scalar DOF layout. */
+int phg_dof_nbas(int order);
+int64_t phg_dof_count(const phg_grid *g, int order);
+/* map[nelem][nbas]: element-local basis -> DOF index (global ordering). */
+void phg_dof_map(const phg_grid *g, int order, int32_t *map);
+/* xyz[ndof][3]: nodal coordinates, the node of each DOF evaluated from the
+ * first element (in index order) that holds it, with PHG's barycentric
+ * combination in that element's local vertex order (dof.c:5109,
+ * dof.c:2210). */
+void phg_dof_coords(const phg_grid *g, int order, double *xyz);
+/* flags[ndof] = 1 if the DOF's geometric entity carries a boundary bit in
+ * `mask` (dof-utils.c:1269 phgDofDirichletBC__, nodal path). */
+void phg_dof_bdry(const phg_grid *g, int order, unsigned mask, uint8_t *flags);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

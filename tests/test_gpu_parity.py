@@ -1,0 +1,201 @@
+"""Parity of the HIP path with the oracle, through the C ABI.
+
+Bar: element matrices, CSR storage order, CSR values and right-hand sides are
+bit-identical to the oracle (the oracle keeps PHG's floating-point operation
+order); solutions agree with the oracle's FP64 PCG to 1e-9 relative.
+"""
+import numpy as np
+import pytest
+
+import oracle as orc
+from tests._cases import elasticity_case, poisson_case
+
+pytestmark = pytest.mark.gpu
+
+
+def _space(pa, c, dim=1):
+    return pa.Space(c["grid"], c["p"], bdry=c["bdry"])
+
+
+def _oracle_problem(c, pb):
+    return orc.problem(c["p"], dim=pb.dim, use_kappa=pb.use_kappa,
+                       mass_coef=pb.mass_coef, rhs_order=pb.rhs_order, E=pb.E, nu=pb.nu)
+
+
+def _bits_equal(a, b):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    return a.shape == b.shape and np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+CASES = [
+    ("sin", 1, dict()), ("sin", 2, dict()), ("sin", 3, dict()),
+    ("kappa", 1, dict(use_kappa=True)), ("kappa", 2, dict(use_kappa=True)),
+    ("kappa", 3, dict(use_kappa=True)),
+    ("sin", 2, dict(mass_coef=1.0)), ("sin", 3, dict(mass_coef=2.5)),
+]
+
+
+@pytest.mark.parametrize("kind,p,opts", CASES)
+def test_element_matrices_bitexact(hip, kind, p, opts):
+    pa = hip
+    c = poisson_case(3, p, kind)
+    sp = _space(pa, c)
+    pb = pa.Problem.poisson(**opts)
+    K, B = pa.element_matrices(sp, pb, kappa=c["kappa"], f=c["f"])
+    opb = _oracle_problem(c, pb)
+    verts, ev, m = c["verts"], c["elem_verts"], c["map"]
+    for e in range(c["grid"].nelem):
+        Ko, Bo = orc.element(opb, verts[ev[e]], 0.0 if c["kappa"] is None else c["kappa"][e],
+                             c["f"][m[e]])
+        assert _bits_equal(K[e], Ko), f"element {e} stiffness differs"
+        assert _bits_equal(B[e], Bo), f"element {e} load differs"
+
+
+def test_elasticity_element_bitexact(hip):
+    pa = hip
+    c = elasticity_case(2)
+    sp = _space(pa, c)
+    pb = pa.Problem.elasticity()
+    K, B = pa.element_matrices(sp, pb, f=c["f"])
+    opb = _oracle_problem(c, pb)
+    for e in range(c["grid"].nelem):
+        floc = c["f"].reshape(-1, 3)[c["map"][e]].ravel()
+        Ko, Bo = orc.element(opb, c["verts"][c["elem_verts"][e]], 0.0, floc)
+        assert _bits_equal(K[e], Ko)
+        assert _bits_equal(B[e], Bo)
+
+
+ASM_CASES = [("sin", 1, 4), ("sin", 2, 3), ("sin", 3, 2), ("kappa", 2, 3),
+             ("kappa", 3, 2), ("linear", 1, 5)]
+
+
+def _assemble_both(pa, c, pb):
+    sp = _space(pa, c)
+    A = pa.assemble(sp, pb, kappa=c["kappa"], f=c["f"], g=c["g"])
+    Ao = orc.assemble(_oracle_problem(c, pb), c["verts"], c["elem_verts"], c["map"],
+                      c["ndof"], c["bdry"], c["g"], c["f"], c["kappa"])
+    return sp, A, Ao
+
+
+@pytest.mark.parametrize("kind,p,n", ASM_CASES)
+def test_csr_bitexact(hip, kind, p, n):
+    pa = hip
+    c = poisson_case(n, p, kind)
+    pb = pa.Problem.poisson(use_kappa=(kind == "kappa"))
+    sp, A, Ao = _assemble_both(pa, c, pb)
+    rp, ci, va, rhs = A.download()
+    assert np.array_equal(rp, Ao.rowptr)
+    assert np.array_equal(ci, Ao.cols), "column storage order differs"
+    assert _bits_equal(va, Ao.vals)
+    assert _bits_equal(rhs, Ao.rhs)
+    # after Dirichlet elimination and RHS lifting
+    A.eliminate()
+    orc.eliminate(Ao, c["bdry"])
+    rp, ci, va, rhs = A.download()
+    assert np.array_equal(rp, Ao.rowptr)
+    assert np.array_equal(ci, Ao.cols)
+    assert _bits_equal(va, Ao.vals)
+    assert _bits_equal(rhs, Ao.rhs)
+
+
+def test_elasticity_csr_bitexact(hip):
+    pa = hip
+    c = elasticity_case(2)
+    pb = pa.Problem.elasticity()
+    bd3 = np.repeat(c["bdry"], 3)
+    sp, A, Ao = _assemble_both(pa, c, pb)
+    rp, ci, va, rhs = A.download()
+    assert np.array_equal(rp, Ao.rowptr)
+    assert np.array_equal(ci, Ao.cols)
+    assert _bits_equal(va, Ao.vals)
+    assert _bits_equal(rhs, Ao.rhs)
+    A.eliminate()
+    orc.eliminate(Ao, bd3)
+    rp, ci, va, rhs = A.download()
+    assert np.array_equal(ci, Ao.cols)
+    assert _bits_equal(va, Ao.vals)
+    assert _bits_equal(rhs, Ao.rhs)
+
+
+@pytest.mark.parametrize("kind,p,n", [("sin", 1, 8), ("sin", 2, 4), ("kappa", 2, 4),
+                                      ("kappa", 3, 3), ("linear", 2, 3)])
+def test_pcg_matches_oracle(hip, kind, p, n):
+    pa = hip
+    c = poisson_case(n, p, kind)
+    pb = pa.Problem.poisson(use_kappa=(kind == "kappa"))
+    sp, A, Ao = _assemble_both(pa, c, pb)
+    A.eliminate()
+    orc.eliminate(Ao, c["bdry"])
+    xo, nits_o, res_o = orc.pcg(Ao, Ao.rhs, rtol=1e-12, btol=1e-12)
+    b = A.rhs()
+    x = pa.Vec(A.n)
+    nits, res = pa.pcg(A, b, x, rtol=1e-12, btol=1e-12)
+    xg = x.numpy()
+    rel = np.linalg.norm(xg - xo) / np.linalg.norm(xo)
+    # reductions run in a different order, so trajectories agree to rounding
+    assert abs(nits - nits_o) <= 2, (nits, nits_o)
+    assert rel < 1e-9, rel
+
+
+def test_pcg_nonzero_initial_guess(hip):
+    pa = hip
+    c = poisson_case(4, 2, "kappa")
+    pb = pa.Problem.poisson(use_kappa=True)
+    sp, A, Ao = _assemble_both(pa, c, pb)
+    A.eliminate()
+    orc.eliminate(Ao, c["bdry"])
+    x0 = np.random.default_rng(3).standard_normal(Ao.n)
+    xo, nits_o, _ = orc.pcg(Ao, Ao.rhs, x0=x0, rtol=1e-12, btol=1e-14)
+    x = pa.Vec(A.n, x0)
+    nits, _ = pa.pcg(A, A.rhs(), x, rtol=1e-12, btol=1e-14)
+    assert abs(nits - nits_o) <= 2
+    assert np.linalg.norm(x.numpy() - xo) / np.linalg.norm(xo) < 1e-9
+
+
+def test_pcg_zero_rhs_returns_zero(hip):
+    pa = hip
+    c = poisson_case(3, 1, "linear")
+    pb = pa.Problem.poisson()
+    sp = _space(pa, c)
+    A = pa.assemble(sp, pb, f=np.zeros(c["ndof"]), g=np.zeros(c["ndof"]))
+    A.eliminate()
+    x = pa.Vec(A.n, np.ones(A.n))
+    nits, res = pa.pcg(A, pa.Vec(A.n), x)
+    assert nits == 0 and np.all(x.numpy() == 0.0)
+
+
+def test_matvec_and_blas1(hip):
+    pa = hip
+    c = poisson_case(4, 2, "kappa")
+    pb = pa.Problem.poisson(use_kappa=True)
+    sp, A, Ao = _assemble_both(pa, c, pb)
+    A.eliminate()
+    orc.eliminate(Ao, c["bdry"])
+    rng = np.random.default_rng(5)
+    xv, yv = rng.standard_normal(A.n), rng.standard_normal(A.n)
+    x, y = pa.Vec(A.n, xv), pa.Vec(A.n, yv)
+    A.matvec(x, y, alpha=2.0, beta=-0.5)
+    ref = 2.0 * orc.matvec(Ao, xv) - 0.5 * yv
+    assert np.allclose(y.numpy(), ref, rtol=1e-13, atol=1e-13 * np.abs(ref).max())
+    assert abs(x.dot(pa.Vec(A.n, yv)) - xv @ yv) < 1e-12 * np.abs(xv) @ np.abs(yv)
+    assert abs(x.norm2() - np.linalg.norm(xv)) < 1e-13 * np.linalg.norm(xv)
+    y2 = pa.Vec(A.n, yv)
+    y2.axpby(3.0, x, 0.25)
+    assert np.allclose(y2.numpy(), 3.0 * xv + 0.25 * yv, rtol=1e-15, atol=1e-15)
+
+
+def test_mixed_precision_refinement(hip):
+    pa = hip
+    c = poisson_case(6, 2, "kappa", sigma=0.5)
+    pb = pa.Problem.poisson(use_kappa=True)
+    sp, A, Ao = _assemble_both(pa, c, pb)
+    A.eliminate()
+    orc.eliminate(Ao, c["bdry"])
+    xo, _, _ = orc.pcg(Ao, Ao.rhs, rtol=1e-14, btol=1e-14)
+    x = pa.Vec(A.n)
+    outer, inner, res = pa.pcg_mixed(A, A.rhs(), x, rtol=1e-11, btol=1e-11,
+                                     inner_rtol=1e-4)
+    rel = np.linalg.norm(x.numpy() - xo) / np.linalg.norm(xo)
+    assert outer >= 2 and inner > 0
+    assert rel < 1e-8, rel   # 1e-8 relative to the FP64 oracle
