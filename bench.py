@@ -1,0 +1,233 @@
+"""Benchmark of the MI355X finite-element hot path.
+
+Workload (default): P2 variable-coefficient Poisson  -div(kappa grad u) = 1,
+u = 0 on the boundary of the unit cube, on a 64^3 x 6 Kuhn mesh (1,572,864
+tetrahedra, 2,146,689 DOFs), kappa_e = exp(g_e) with g_e ~ N(0, 0.5^2) i.i.d.
+per element (seeded).  FP64 assembly on the GPU, then FP32 Jacobi-PCG under
+FP64 iterative refinement.
+
+A step is one solve from x0 = 0 to the stopping rule with the assembled system
+already resident in HBM.  value = DOFs / (seconds per solve).  The same rule
+is timed for the plain FP64 GPU PCG; every solution is checked against a
+tightly converged FP64 reference solution (relative 2-norm error <= 1e-8).
+
+Prints one JSON line (last line of stdout).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "DOFs solved/s to 1e-8 rel err: mixed-prec vs FP64 PCG; SpMV HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def spmv_bytes(n, nnz, s_val, s_vec):
+    """Algorithmic CSR SpMV bytes: values + int32 columns + int32 row pointers
+    + one read of x + one write of y."""
+    return nnz * (s_val + 4) + (n + 1) * 4 + n * s_vec + n * s_vec
+
+
+def build_problem(n, p, sigma, seed):
+    import phg_amd as pa
+    t0 = time.perf_counter()
+    g = pa.Grid.kuhn(n)
+    nd = g.ndof(p)
+    bd = g.dof_bdry(p)
+    rng = np.random.default_rng(seed)
+    kappa = np.exp(rng.normal(0.0, sigma, size=g.nelem))
+    f = np.ones(nd)
+    gv = np.zeros(nd)
+    t1 = time.perf_counter()
+    return g, nd, bd, kappa, f, gv, t1 - t0
+
+
+def cpu_baseline(n_cpu, p, sigma, seed, rtol):
+    """Oracle (sequential C restatement of PHG's assembly + PCG) on a bounded
+    sample of the same workload family, one thread."""
+    import phg_amd as pa
+    import oracle as orc
+    g = pa.Grid.kuhn(n_cpu)
+    nd = g.ndof(p)
+    bd = g.dof_bdry(p)
+    rng = np.random.default_rng(seed)
+    kappa = np.exp(rng.normal(0.0, sigma, size=g.nelem))
+    f, gv = np.ones(nd), np.zeros(nd)
+    t0 = time.perf_counter()
+    A = orc.assemble(orc.problem(p, use_kappa=True), g.verts(), g.elem_verts(),
+                     g.dof_map(p), nd, bd, gv, f, kappa)
+    orc.eliminate(A, bd)
+    t1 = time.perf_counter()
+    x, nits, res = orc.pcg(A, A.rhs, rtol=rtol, btol=rtol)
+    t2 = time.perf_counter()
+    return dict(ndof=nd, nelem=g.nelem, assemble_s=t1 - t0, solve_s=t2 - t1, nits=nits)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=64, help="cubes per side")
+    ap.add_argument("--order", type=int, default=2)
+    ap.add_argument("--sigma", type=float, default=0.5)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--rtol", type=float, default=1e-10,
+                    help="relative residual stopping rule of both solvers")
+    ap.add_argument("--inner-rtol", type=float, default=1e-4)
+    ap.add_argument("--cpu-n", type=int, default=40,
+                    help="cubes per side of the CPU baseline sample (0: skip)")
+    ap.add_argument("--no-profile", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} "
+              "requested); multi-GPU sharding is design-only.", file=sys.stderr)
+        sys.exit(2)
+
+    import phg_amd as pa
+    pa.init(0)
+
+    g, nd, bd, kappa, f, gv, t_mesh = build_problem(args.n, args.order, args.sigma,
+                                                    args.seed)
+    log(f"mesh {args.n}^3x6: {g.nelem} tets, {nd} DOFs (host setup {t_mesh:.2f}s)")
+    sp = pa.Space(g, args.order, bdry=bd)
+    pb = pa.Problem.poisson(use_kappa=True)
+    pa.synchronize()
+    t0 = time.perf_counter()
+    A = pa.assemble(sp, pb, kappa=kappa, f=f, g=gv)
+    A.eliminate()
+    pa.synchronize()
+    t_asm = time.perf_counter() - t0
+    n, nnz = A.info()
+    log(f"assembled: n={n} nnz={nnz} ({nnz / n:.1f}/row) in {t_asm:.3f}s")
+
+    b = A.rhs()
+    # FP64 reference solution (untimed, converged far below the target)
+    xref = pa.Vec(n)
+    nits_ref, res_ref = pa.pcg(A, b, xref, maxit=100000, rtol=1e-13, btol=1e-13)
+    xr = xref.numpy()
+    nxr = np.linalg.norm(xr)
+    log(f"reference FP64 PCG: {nits_ref} its, residual {res_ref:.3e}")
+
+    x = pa.Vec(n)
+
+    def run_mixed():
+        x.axpby(0.0, x, 0.0)
+        return pa.pcg_mixed(A, b, x, maxit=50, rtol=args.rtol, btol=args.rtol,
+                            inner_rtol=args.inner_rtol)
+
+    def run_fp64():
+        x.axpby(0.0, x, 0.0)
+        return pa.pcg(A, b, x, maxit=100000, rtol=args.rtol, btol=args.rtol)
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        pa.synchronize()
+        t = time.perf_counter()
+        out = None
+        for _ in range(steps):
+            out = fn()
+        pa.synchronize()
+        return (time.perf_counter() - t) / steps, out
+
+    # mixed precision (the headline)
+    sec_mp, (outer, inner, res_mp) = timed(run_mixed, args.steps, args.warmup)
+    err_mp = np.linalg.norm(x.numpy() - xr) / nxr
+    log(f"mixed: {sec_mp * 1e3:.2f} ms/solve, {outer} refinements, {inner} FP32 its, "
+        f"rel.err {err_mp:.2e}")
+    # plain FP64 GPU path with the same stopping rule
+    sec_64, (nits64, res64) = timed(run_fp64, max(1, args.steps), 1)
+    err_64 = np.linalg.norm(x.numpy() - xr) / nxr
+    log(f"fp64:  {sec_64 * 1e3:.2f} ms/solve, {nits64} its, rel.err {err_64:.2e}")
+
+    roof = None
+    spmv64 = None
+    if not args.no_profile:
+        pa.profile(True)
+        run_mixed()
+        l32, ms32 = pa.profile_read(pa.K_SPMV32)
+        run_fp64()
+        l64, ms64 = pa.profile_read(pa.K_SPMV64)
+        pa.profile(False)
+        if l32 > 0:
+            t32 = ms32 / l32 * 1e-3
+            b32 = spmv_bytes(n, nnz, 4, 4)
+            ach = b32 / t32 / 1e9
+            roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                    "traffic": None,
+                    "kernel": "k_pcg_spmv<float> (FP32 CSR SpMV + (p,Ap))",
+                    "bytes_per_launch": b32, "avg_us": round(t32 * 1e6, 2),
+                    "launches": l32}
+        if l64 > 0:
+            t64 = ms64 / l64 * 1e-3
+            b64 = spmv_bytes(n, nnz, 8, 8)
+            spmv64 = {"achieved_GBps": round(b64 / t64 / 1e9, 1),
+                      "frac": round(b64 / t64 / 1e9 / HBM_PEAK_GBS, 4),
+                      "avg_us": round(t64 * 1e6, 2), "bytes_per_launch": b64,
+                      "launches": l64}
+
+    cpu = None
+    if args.cpu_n > 0:
+        c = cpu_baseline(args.cpu_n, args.order, args.sigma, args.seed, args.rtol)
+        cpu = {"value": round(c["ndof"] / c["solve_s"], 1), "unit": "DOFs/s",
+               "cores": 1, "kind": "port",
+               "sample": (f"oracle FP64 Jacobi-PCG (sequential C restatement of "
+                          f"solver-phg.c), P2 kappa-Poisson {args.cpu_n}^3x6 "
+                          f"({c['ndof']} DOFs), rtol {args.rtol}: {c['nits']} its in "
+                          f"{c['solve_s']:.2f}s; oracle assembly {c['assemble_s']:.2f}s")}
+        log("cpu baseline:", cpu["sample"])
+
+    value = nd / sec_mp
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "DOFs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(sec_mp * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"P2 -div(kappa grad u)=1 on {args.n}^3x6 Kuhn mesh "
+                         f"({g.nelem} tets, {nd} DOFs), kappa=exp(N(0,{args.sigma}^2)) "
+                         f"seed {args.seed}; FP32 Jacobi-PCG (inner rtol "
+                         f"{args.inner_rtol}) under FP64 refinement to relative "
+                         f"residual {args.rtol}"),
+            "nnz": nnz, "parallelism": "single GPU",
+        },
+        "rel_err_vs_fp64": float(f"{err_mp:.3e}"),
+        "fp64_pcg": {"value": round(nd / sec_64, 1), "ms_per_solve": round(sec_64 * 1e3, 3),
+                     "its": nits64, "rel_err": float(f"{err_64:.3e}")},
+        "mixed_vs_fp64": round(sec_64 / sec_mp, 3),
+        "mixed_detail": {"refinements": outer, "fp32_its": inner},
+        "assemble_s": round(t_asm, 4),
+        "roofline": roof,
+        "spmv_fp64": spmv64,
+        "cpu_baseline": cpu,
+    }
+    if err_mp > 1e-8:
+        log(f"WARNING: mixed-precision error {err_mp:.2e} above 1e-8")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -149,6 +149,21 @@ int phg_hip_pcg_mixed(const phg_hip_mat *A, const phg_hip_vec *b,
                       double atol, double inner_rtol, int32_t *inner_its,
                       double *residual);
 
+/*------------------------------------------------------------- profiling */
+
+/* Per-kernel timing, the role of perf.c's counters (perf.c:490
+ * phgPerfGetMflops) for the GPU path: with profiling on, every PCG kernel
+ * launch is bracketed by HIP events on the library stream and solves run
+ * one iteration per host round trip.  phg_hip_profile resets the totals. */
+#define PHG_HIP_K_PCG_P64     0   /* p = z + beta p, FP64            */
+#define PHG_HIP_K_SPMV64      1   /* Ap = A p and (p, Ap), FP64       */
+#define PHG_HIP_K_UPDATE64    2   /* x, r updates and norms, FP64     */
+#define PHG_HIP_K_PCG_P32     3
+#define PHG_HIP_K_SPMV32      4   /* FP32 inner solve of pcg_mixed    */
+#define PHG_HIP_K_UPDATE32    5
+int phg_hip_profile(int32_t on);
+int phg_hip_profile_read(int32_t kind, int64_t *launches, double *ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -120,6 +120,11 @@ _sig(hip_lib, "phg_hip_pcg", C.c_int,
      [_vp, _vp, _vp, _i32, _dbl, _dbl, _dbl, _P(_dbl)])
 _sig(hip_lib, "phg_hip_pcg_mixed", C.c_int,
      [_vp, _vp, _vp, _i32, _dbl, _dbl, _dbl, _dbl, _P(_i32), _P(_dbl)])
+_sig(hip_lib, "phg_hip_profile", C.c_int, [_i32])
+_sig(hip_lib, "phg_hip_profile_read", C.c_int, [_i32, _P(_i64), _P(_dbl)])
+
+# kernel classes of phg_hip_profile_read (include/phg_hip.h)
+K_PCG_P64, K_SPMV64, K_UPDATE64, K_PCG_P32, K_SPMV32, K_UPDATE32 = range(6)
 
 
 class PhgHipError(RuntimeError):
@@ -366,6 +371,18 @@ def assemble(sp: Space, pb: Problem, kappa=None, f=None, g=None) -> Mat:
     _check(hip_lib.phg_hip_assemble(sp._h, C.byref(pb), _ptr(kap, _dbl), _ptr(f, _dbl),
                                     _ptr(g, _dbl), C.byref(h)), "assemble")
     return Mat(h)
+
+
+def profile(on: bool) -> None:
+    """Turns per-kernel HIP-event timing on/off (resets the totals)."""
+    _check(hip_lib.phg_hip_profile(int(on)), "phg_hip_profile")
+
+
+def profile_read(kind: int):
+    """(launches, total milliseconds) recorded for a kernel class."""
+    n, ms = _i64(), _dbl()
+    _check(hip_lib.phg_hip_profile_read(kind, C.byref(n), C.byref(ms)), "profile_read")
+    return n.value, ms.value
 
 
 def pcg(A: Mat, b: Vec, x: Vec, maxit=10000, rtol=1e-10, btol=1e-10, atol=0.0):

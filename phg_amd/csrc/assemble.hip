@@ -1222,7 +1222,7 @@ extern "C" int phg_hip_mat_eliminate(phg_hip_mat *A)
     A->eliminated = 1;
     phg::dfree(count);
     phg::dfree(nrow);
-    return phg::choose_spmv_shape(A);
+    return phg::build_row_blocks(A);
 fail:
     phg::dfree(count);
     phg::dfree(nrow);
@@ -1283,6 +1283,7 @@ extern "C" int phg_hip_mat_destroy(phg_hip_mat *A)
     phg::dfree(A->d_diag);
     phg::dfree(A->d_vals32);
     phg::dfree(A->d_diag32);
+    phg::dfree(A->d_rblk);
     delete A;
     return PHG_HIP_OK;
 }

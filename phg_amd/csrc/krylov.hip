@@ -1,32 +1,43 @@
 // Krylov kernels for gfx950: CSR SpMV, BLAS-1 and the Jacobi-PCG of
 // solver-phg.c:485 restructured into three fused kernels per iteration.
 //
-// Per iteration (vector streams in brackets):
-//   k_pcg_p      p = z + beta*p with z = r/diag computed on the fly    [r,d,p,p]
-//   k_pcg_spmv   Ap = A p and the partial sums of (p, Ap)              [A,p,Ap]
-//   k_pcg_update x += alpha p, r -= alpha Ap, partial sums of (r,r) and
-//                (r, r/diag) for the next iteration                    [x,p,r,Ap,d,x,r]
-// Scalars (alpha, beta, residual, iteration count) never leave the device:
-// the last work-group to finish a kernel reduces the per-group partial sums
-// in a fixed order (deterministic) and updates the solver state, and every
-// kernel returns at once when the state says the solve is over.  The host
-// launches iterations in chunks and reads the state from a pinned mailbox
-// once per chunk.
+// Per iteration k (vector streams in brackets):
+//   k_pcg_p      convergence test on the residual of iteration k-1, then
+//                p = z + beta p with z = r/diag computed on the fly [r,d,p,p]
+//   k_pcg_spmv   Ap = A p and per-group partial sums of (p, Ap)    [A,p,Ap]
+//   k_pcg_update x += alpha p, r -= alpha Ap and per-group partial sums of
+//                (r, r) and (r, r/diag)                  [x,p,r,Ap,d,x,r]
 //
-// Work-groups are remapped so that each of the 8 XCDs (own L2) processes a
-// contiguous slice of rows, keeping the gathered x entries of neighbouring
-// rows in the same L2.
+// Reductions: every work-group writes its partial sums with plain stores and
+// the next kernel on the stream -- which needs the total anyway -- sums the
+// partials in every work-group, in a fixed order, so all work-groups derive
+// bit-identical scalars (alpha, beta, the residual) without device-scope
+// fences or atomics.  (A release fence per work-group makes each XCD write
+// back its L2, which costs more than the whole SpMV.)
+//
+// Solver state lives in device memory as two copies: the p-kernel of
+// iteration k reads copy k&1 and writes copy (k+1)&1 from work-group 0 only,
+// so no kernel reads a field another work-group of the same launch writes.
+// The host launches iterations in chunks and reads the current copy from a
+// pinned mailbox once per chunk; kernels of iterations after convergence
+// return at once.
+//
+// Work-groups of the row kernels are remapped so that each of the 8 XCDs
+// (own L2) processes a contiguous slice of rows, keeping the gathered x
+// entries of neighbouring rows in the same L2.
 #include "phg_hip_internal.h"
 
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
 #include <algorithm>
+#include <vector>
 
 namespace {
 
 constexpr int BS = 256;      // work-group size (4 wavefronts of 64)
 constexpr int NXCD = 8;
+constexpr int MAX_GRID = 2048;   // partial-sum slots per reduced value
 
 // dispatch order places work-group b on XCD b % 8; return the logical index
 // that gives XCD x the x-th contiguous share of [0, gridDim.x)
@@ -46,15 +57,11 @@ __device__ inline double wave_sum(double v)
     return v;
 }
 
-// Block reduction of NV values; the partials of every work-group go to
-// partials[k * gridDim.x + block]; the last work-group to arrive sums them
-// in a fixed order and returns true with the totals in out[] (all threads).
+// work-group sums of NV values, written to partials[k * gridDim.x + block]
 template <int NV>
-__device__ bool reduce_last(double (&v)[NV], double *partials,
-                            unsigned *counter, double (&out)[NV])
+__device__ inline void write_partials(double (&v)[NV], double *partials)
 {
     __shared__ double sh[NV][BS / 64];
-    __shared__ int last;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < NV; k++) {
@@ -63,34 +70,31 @@ __device__ bool reduce_last(double (&v)[NV], double *partials,
             sh[k][wid] = s;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < NV) {
+        double s = 0.0;
 #pragma unroll
-        for (int k = 0; k < NV; k++) {
-            double s = 0.0;
-#pragma unroll
-            for (int w = 0; w < BS / 64; w++)
-                s += sh[k][w];
-            partials[(int64_t)k * gridDim.x + blockIdx.x] = s;
-        }
-        __threadfence();
-        unsigned t = atomicAdd(counter, 1u);
-        last = (t == gridDim.x - 1);
+        for (int w = 0; w < BS / 64; w++)
+            s += sh[threadIdx.x][w];
+        partials[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = s;
     }
-    __syncthreads();
-    if (!last)
-        return false;
-    __threadfence();
+}
+
+// totals of the np partial sums written by a previous kernel; the same fixed
+// summation order in every work-group (result in every thread)
+template <int NV>
+__device__ inline void sum_partials(const double *partials, int np,
+                                    double (&out)[NV])
+{
+    __shared__ double sh[NV][BS / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     double acc[NV];
 #pragma unroll
     for (int k = 0; k < NV; k++)
         acc[k] = 0.0;
-    for (int64_t i = threadIdx.x; i < gridDim.x; i += BS)
+    for (int i = threadIdx.x; i < np; i += BS)
 #pragma unroll
         for (int k = 0; k < NV; k++)
-            acc[k] += __hip_atomic_load(&partials[(int64_t)k * gridDim.x + i],
-                                        __ATOMIC_RELAXED,
-                                        __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
+            acc[k] += partials[(int64_t)k * np + i];
 #pragma unroll
     for (int k = 0; k < NV; k++) {
         double s = wave_sum(acc[k]);
@@ -106,25 +110,22 @@ __device__ bool reduce_last(double (&v)[NV], double *partials,
             s += sh[k][w];
         out[k] = s;
     }
-    if (threadIdx.x == 0)
-        *counter = 0u;
-    return true;
+    __syncthreads();
 }
 
-// solver state kept in device memory
+// solver state (two copies, see above)
 struct State {
-    double rz, rz_prev, pAp, alpha;
-    double res, tol, rb, r0;
-    double rtol, btol, atol, scale;
+    double rz, rz_prev, res, tol, rb, r0;
+    double rtol, btol, atol;
     int32_t nits, done, maxit, xnz;
-    int32_t zero_diag, outer, pad0, pad1;
-    unsigned counter[4];
 };
-// done codes
 constexpr int RUNNING = 0, CONVERGED = 1, BREAKDOWN = 2, ZERO_DIAG = 3;
 
 // IsZero (utils.h:451)
-__device__ inline bool is_zero(double a) { return fabs(a) < 10.0 * 2.2250738585072014e-308; }
+__device__ inline bool is_zero(double a)
+{
+    return fabs(a) < 10.0 * 2.2250738585072014e-308;
+}
 
 // contiguous row range of this work-group
 __device__ inline void row_range(int64_t n, int64_t &r0, int64_t &r1)
@@ -135,54 +136,110 @@ __device__ inline void row_range(int64_t n, int64_t &r0, int64_t &r1)
     r1 = r0 + per < n ? r0 + per : n;
 }
 
-// sum over a row with L lanes (result valid in every lane of the group)
-template <int L, typename VT, typename XT, typename AT>
-__device__ inline AT row_dot(const int32_t *__restrict__ rp,
-                             const int32_t *__restrict__ ci,
-                             const VT *__restrict__ va,
-                             const XT *__restrict__ x, int64_t row, int lane)
+// Streaming CSR SpMV over precomputed row blocks.  A work-group takes a
+// contiguous range of row blocks; for each block it loads the block's
+// entries (contiguous in CSR, so every load is coalesced) and stores the
+// products a_ij * x_j in LDS, then one work-item per row sums its products in
+// storage order -- the order of matvec.c:887-930 -- and hands (row, sum) to
+// the epilogue.  A row longer than the LDS stage is a block of its own,
+// summed by the whole work-group.
+constexpr int NNZ_CAP = 2048;   // entries staged per row block
+constexpr int ROW_CAP = BS;     // rows per row block
+
+template <typename T>
+__device__ inline T block_sum(T v)
 {
-    AT s = 0;
-    int32_t j0 = rp[row], j1 = rp[row + 1];
-    for (int32_t j = j0 + lane; j < j1; j += L)
-        s += (AT)va[j] * (AT)x[ci[j]];
+    __shared__ T sh[BS / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
-    for (int o = L / 2; o > 0; o >>= 1)
-        s += __shfl_xor(s, o, L);
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o, 64);
+    if (lane == 0)
+        sh[wid] = v;
+    __syncthreads();
+    T s = 0;
+#pragma unroll
+    for (int w = 0; w < BS / 64; w++)
+        s += sh[w];
+    __syncthreads();
     return s;
 }
 
-//------------------------------------------------------------- plain SpMV
-template <int L>
-__global__ void __launch_bounds__(BS) k_spmv64(int64_t n, const int32_t *rp,
-                                               const int32_t *ci,
-                                               const double *va,
-                                               const double *x, double *y,
-                                               double alpha, double beta)
+template <typename T, typename Epi>
+__device__ inline void stream_spmv(const int32_t *__restrict__ rp,
+                                   const int32_t *__restrict__ ci,
+                                   const T *__restrict__ va,
+                                   const T *__restrict__ x,
+                                   const int32_t *__restrict__ rblk,
+                                   int64_t nrblk, Epi epi)
 {
-    int64_t r0, r1;
-    row_range(n, r0, r1);
-    const int lane = threadIdx.x % L, grp = threadIdx.x / L;
-    for (int64_t row = r0 + grp; row < r1; row += BS / L) {
-        double s = row_dot<L, double, double, double>(rp, ci, va, x, row, lane);
-        if (lane == 0) {
-            // matvec.c:608-633 beta first, then alpha * A x
-            double yo = beta == 0.0 ? 0.0 : beta * y[row];
-            y[row] = yo + alpha * s;
+    __shared__ T prod[NNZ_CAP];
+    int64_t b0, b1;
+    row_range(nrblk, b0, b1);
+    const int tid = threadIdx.x;
+    for (int64_t b = b0; b < b1; b++) {
+        const int32_t r0 = rblk[b], r1 = rblk[b + 1];
+        const int32_t j0 = rp[r0], j1 = rp[r1];
+        if (j1 - j0 <= NNZ_CAP) {
+#pragma unroll 4
+            for (int32_t j = j0 + tid; j < j1; j += BS)
+                prod[j - j0] = va[j] * x[ci[j]];
+            __syncthreads();
+            if (tid < r1 - r0) {
+                const int32_t r = r0 + tid;
+                const int32_t s0 = rp[r] - j0, s1 = rp[r + 1] - j0;
+                T sum = 0;
+                for (int32_t j = s0; j < s1; j++)
+                    sum += prod[j];
+                epi(r, sum);
+            }
+            __syncthreads();
+        }
+        else {
+            T sum = 0;
+            for (int32_t j = j0 + tid; j < j1; j += BS)
+                sum += va[j] * x[ci[j]];
+            sum = block_sum<T>(sum);
+            if (tid == 0)
+                epi(r0, sum);
         }
     }
 }
 
+//------------------------------------------------------------- plain SpMV
+__global__ void __launch_bounds__(BS) k_spmv64(const int32_t *rp,
+                                               const int32_t *ci,
+                                               const double *va,
+                                               const int32_t *rblk,
+                                               int64_t nrblk, const double *x,
+                                               double *y, double alpha,
+                                               double beta)
+{
+    stream_spmv<double>(rp, ci, va, x, rblk, nrblk,
+                        [&](int32_t row, double s) {
+                            // matvec.c:608-633 beta first, then alpha * A x
+                            double yo = beta == 0.0 ? 0.0 : beta * y[row];
+                            y[row] = yo + alpha * s;
+                        });
+}
+
 //------------------------------------------------------------------ BLAS-1
 __global__ void __launch_bounds__(BS) k_dot(int64_t n, const double *x,
-                                            const double *y, double *partials,
-                                            unsigned *counter, double *out)
+                                            const double *y, double *partials)
 {
-    double v[1] = {0.0}, o[1];
+    double v[1] = {0.0};
     for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * BS)
         v[0] += x[i] * y[i];
-    if (reduce_last<1>(v, partials, counter, o) && threadIdx.x == 0)
+    write_partials<1>(v, partials);
+}
+
+__global__ void __launch_bounds__(BS) k_final_sum(const double *partials,
+                                                  int np, double *out)
+{
+    double o[1];
+    sum_partials<1>(partials, np, o);
+    if (threadIdx.x == 0)
         *out = o[0];
 }
 
@@ -198,138 +255,177 @@ __global__ void __launch_bounds__(BS) k_axpby(int64_t n, double a,
 }
 
 //--------------------------------------------------------- PCG kernels
-// Prologue (solver-phg.c:505-533): |b|, initial residual, tolerance, and
-// (r, z) for the first iteration.  r = b - A x when |x|_1/n is not zero.
-template <int L>
+// Prologue (solver-phg.c:505-533): |b| and |x|_1 partials ...
 __global__ void __launch_bounds__(BS) k_pcg_init_norms(int64_t n,
                                                        const double *b,
                                                        const double *x,
-                                                       double *partials,
-                                                       State *st)
+                                                       double *part_out)
 {
-    double v[2] = {0.0, 0.0}, o[2];
+    double v[2] = {0.0, 0.0};
     for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * BS) {
         v[0] += b[i] * b[i];
         v[1] += fabs(x[i]);
     }
-    if (reduce_last<2>(v, partials, &st->counter[0], o) && threadIdx.x == 0) {
-        st->rb = sqrt(o[0]);
-        st->xnz = !is_zero(o[1] / (double)n);
-    }
+    write_partials<2>(v, part_out);
 }
 
-template <int L>
+// ... then r = b - A x when |x|_1/n is not zero (else r = b; x = 0 when b
+// is zero) and the partials (r,r), (r,z), #zero diagonals for iteration 0.
 __global__ void __launch_bounds__(BS) k_pcg_init_res(
     int64_t n, const int32_t *rp, const int32_t *ci, const double *va,
-    const double *diag, const double *b, double *x, double *r,
-    double *partials, State *st)
+    const int32_t *rblk, int64_t nrblk, const double *diag, const double *b,
+    double *x, double *r, const double *part_in, int np_in, double *part_out,
+    State *st0)
 {
-    int64_t r0, r1;
-    row_range(n, r0, r1);
-    const int lane = threadIdx.x % L, grp = threadIdx.x / L;
-    const bool bzero = is_zero(st->rb);
-    const bool xnz = st->xnz != 0;
-    double v[3] = {0.0, 0.0, 0.0}, o[3];
-    for (int64_t row = r0 + grp; row < r1; row += BS / L) {
-        double s = 0.0;
-        if (xnz && !bzero)
-            s = row_dot<L, double, double, double>(rp, ci, va, x, row, lane);
-        if (lane == 0) {
-            double ri;
-            if (bzero) {
-                x[row] = 0.0;
-                ri = 0.0;
-            }
-            else
-                ri = xnz ? -s + b[row] : b[row];
-            r[row] = ri;
-            double d = diag[row];
-            v[0] += ri * ri;
-            if (d == 0.0)
-                v[2] += 1.0;
-            else
-                v[1] += ri * (ri / d);
+    double nb[2];
+    sum_partials<2>(part_in, np_in, nb);
+    const double rb = sqrt(nb[0]);
+    const bool bzero = is_zero(rb);
+    const bool xnz = !is_zero(nb[1] / (double)n);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st0->rb = rb;
+        st0->xnz = xnz;
+    }
+    double v[3] = {0.0, 0.0, 0.0};
+    auto acc = [&](int64_t row, double ri) {
+        r[row] = ri;
+        double d = diag[row];
+        v[0] += ri * ri;
+        if (d == 0.0)
+            v[2] += 1.0;
+        else
+            v[1] += ri * (ri / d);
+    };
+    if (xnz && !bzero) {
+        stream_spmv<double>(rp, ci, va, x, rblk, nrblk,
+                            [&](int32_t row, double s) { acc(row, -s + b[row]); });
+    }
+    else {
+        for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
+             i += (int64_t)gridDim.x * BS) {
+            if (bzero)
+                x[i] = 0.0;
+            acc(i, bzero ? 0.0 : b[i]);
         }
     }
-    if (reduce_last<3>(v, partials, &st->counter[0], o) && threadIdx.x == 0) {
-        st->nits = 0;
-        if (bzero) {
-            st->res = 0.0;
-            st->done = CONVERGED;
-            return;
+    write_partials<3>(v, part_out);
+}
+
+// p-update with the loop control of solver-phg.c:535-559
+template <typename T>
+__global__ void __launch_bounds__(BS) k_pcg_p(int64_t n, const T *r,
+                                              const T *diag, T *p,
+                                              const double *part_in, int np_in,
+                                              const State *sin, State *sout)
+{
+    const State s = *sin;
+    const bool w0 = blockIdx.x == 0 && threadIdx.x == 0;
+    if (s.done) {
+        if (w0)
+            *sout = s;
+        return;
+    }
+    double t[3];
+    sum_partials<3>(part_in, np_in, t);    // (r,r), (r,z), #zero diagonals
+    const double res = sqrt(t[0]), rz = t[1];
+    State o = s;
+    o.res = res;
+    bool stop = false;
+    double beta = 0.0;
+    if (s.nits == 0) {
+        if (is_zero(s.rb)) {
+            o.res = 0.0;
+            o.done = CONVERGED;
+            stop = true;
         }
-        double res = sqrt(o[0]);
-        st->r0 = st->res = res;
-        double tol = res * st->rtol;   // get_tol (solver-phg.c:104)
-        if (tol < st->rb * st->btol)
-            tol = st->rb * st->btol;
-        if (tol < st->atol)
-            tol = st->atol;
-        st->tol = tol;
-        st->rz = o[1];
-        st->rz_prev = 0.0;
-        bool run = res > tol && st->maxit > 0;
-        st->zero_diag = o[2] != 0.0;
-        st->done = !run ? CONVERGED : (o[2] != 0.0 ? ZERO_DIAG : RUNNING);
+        else {
+            double tol = res * s.rtol;       // get_tol (solver-phg.c:104)
+            if (tol < s.rb * s.btol)
+                tol = s.rb * s.btol;
+            if (tol < s.atol)
+                tol = s.atol;
+            o.tol = tol;
+            o.r0 = res;
+            if (!(res > tol && s.maxit > 0)) {
+                o.done = CONVERGED;
+                stop = true;
+            }
+            else if (t[2] != 0.0) {          // solver-phg.c:233
+                o.done = ZERO_DIAG;
+                stop = true;
+            }
+            else {
+                o.nits = 1;
+                o.rz = rz;
+                o.rz_prev = 0.0;
+            }
+        }
+    }
+    else {
+        if (!(res > s.tol && s.nits < s.maxit)) {
+            o.done = CONVERGED;
+            stop = true;
+        }
+        else if (s.rz == 0.0) {              // solver-phg.c:547
+            o.nits = s.nits + 1;
+            o.done = BREAKDOWN;
+            stop = true;
+        }
+        else {
+            o.nits = s.nits + 1;
+            o.rz_prev = s.rz;
+            o.rz = rz;
+            beta = rz / s.rz;
+        }
+    }
+    if (w0)
+        *sout = o;
+    if (stop)
+        return;
+    const bool first = s.nits == 0;
+    const T tb = (T)beta;
+    for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * BS) {
+        T z = r[i] / diag[i];
+        p[i] = first ? z : tb * p[i] + z;
     }
 }
 
 template <typename T>
-__global__ void __launch_bounds__(BS) k_pcg_p(int64_t n, const T *r,
-                                              const T *diag, T *p,
-                                              const State *st)
-{
-    if (st->done)
-        return;
-    const bool first = st->nits == 0;
-    const T beta = first ? (T)0 : (T)(st->rz / st->rz_prev);
-    for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * BS) {
-        T z = r[i] / diag[i];
-        p[i] = first ? z : beta * p[i] + z;
-    }
-}
-
-template <int L, typename T>
 __global__ void __launch_bounds__(BS) k_pcg_spmv(
-    int64_t n, const int32_t *rp, const int32_t *ci, const T *va, const T *p,
-    T *Ap, double *partials, State *st)
+    const int32_t *rp, const int32_t *ci, const T *va, const int32_t *rblk,
+    int64_t nrblk, const T *p, T *Ap, double *part_out, const State *st)
 {
     if (st->done)
         return;
-    int64_t r0, r1;
-    row_range(n, r0, r1);
-    const int lane = threadIdx.x % L, grp = threadIdx.x / L;
-    double v[1] = {0.0}, o[1];
-    for (int64_t row = r0 + grp; row < r1; row += BS / L) {
-        T s = row_dot<L, T, T, T>(rp, ci, va, p, row, lane);
-        if (lane == 0) {
-            Ap[row] = s;
-            v[0] += (double)p[row] * (double)s;
-        }
-    }
-    if (reduce_last<1>(v, partials, &st->counter[1], o) && threadIdx.x == 0) {
-        st->nits += 1;
-        st->pAp = o[0];
-        if (o[0] == 0.0)
-            st->done = BREAKDOWN;      // solver-phg.c:561
-        else
-            st->alpha = st->rz / o[0];
-    }
+    double v[1] = {0.0};
+    stream_spmv<T>(rp, ci, va, p, rblk, nrblk, [&](int32_t row, T s) {
+        Ap[row] = s;
+        v[0] += (double)p[row] * (double)s;
+    });
+    write_partials<1>(v, part_out);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(BS) k_pcg_update(int64_t n, T *x,
                                                    const T *p, T *r,
                                                    const T *Ap, const T *diag,
-                                                   double *partials,
+                                                   const double *part_in,
+                                                   int np_in, double *part_out,
                                                    State *st)
 {
     if (st->done)
         return;
-    const T alpha = (T)st->alpha;
-    double v[2] = {0.0, 0.0}, o[2];
+    double pap[1];
+    sum_partials<1>(part_in, np_in, pap);
+    if (pap[0] == 0.0) {                     // solver-phg.c:561
+        if (blockIdx.x == 0 && threadIdx.x == 0)
+            st->done = BREAKDOWN;
+        return;
+    }
+    const T alpha = (T)(st->rz / pap[0]);
+    double v[3] = {0.0, 0.0, 0.0};
     for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * BS) {
         x[i] += alpha * p[i];
@@ -339,69 +435,47 @@ __global__ void __launch_bounds__(BS) k_pcg_update(int64_t n, T *x,
         v[0] += (double)ri * (double)ri;
         v[1] += (double)ri * (double)z;
     }
-    if (reduce_last<2>(v, partials, &st->counter[2], o) && threadIdx.x == 0) {
-        double res = sqrt(o[0]);
-        st->res = res;
-        st->rz_prev = st->rz;
-        st->rz = o[1];
-        if (!(res > st->tol && st->nits < st->maxit))
-            st->done = CONVERGED;
-        else if (st->rz_prev == 0.0) {  // solver-phg.c:547, next iteration
-            st->nits += 1;
-            st->done = BREAKDOWN;
-        }
-    }
+    write_partials<3>(v, part_out);
 }
 
 //----------------------------------------------- mixed-precision refinement
-// r = b - A x in FP64 with |r|^2 and |b|^2; sets the outer state.
-template <int L>
+// r = b - A x in FP64; partials of |r|^2 and |b|^2
 __global__ void __launch_bounds__(BS) k_ir_residual(
-    int64_t n, const int32_t *rp, const int32_t *ci, const double *va,
-    const double *b, const double *x, double *r, double *partials, State *so)
+    const int32_t *rp, const int32_t *ci, const double *va,
+    const int32_t *rblk, int64_t nrblk, const double *b, const double *x,
+    double *r, double *part_out)
 {
-    int64_t r0, r1;
-    row_range(n, r0, r1);
-    const int lane = threadIdx.x % L, grp = threadIdx.x / L;
-    double v[2] = {0.0, 0.0}, o[2];
-    for (int64_t row = r0 + grp; row < r1; row += BS / L) {
-        double s = row_dot<L, double, double, double>(rp, ci, va, x, row, lane);
-        if (lane == 0) {
-            double bi = b[row], ri = bi - s;
-            r[row] = ri;
-            v[0] += ri * ri;
-            v[1] += bi * bi;
-        }
-    }
-    if (reduce_last<2>(v, partials, &so->counter[0], o) && threadIdx.x == 0) {
-        double res = sqrt(o[0]);
-        so->res = res;
-        if (so->outer == 0) {
-            so->rb = sqrt(o[1]);
-            so->r0 = res;
-            double tol = res * so->rtol;
-            if (tol < so->rb * so->btol)
-                tol = so->rb * so->btol;
-            if (tol < so->atol)
-                tol = so->atol;
-            so->tol = tol;
-        }
-        so->done = (res <= so->tol || is_zero(so->rb)) ? CONVERGED : RUNNING;
+    double v[2] = {0.0, 0.0};
+    stream_spmv<double>(rp, ci, va, x, rblk, nrblk, [&](int32_t row, double s) {
+        double bi = b[row], ri = bi - s;
+        r[row] = ri;
+        v[0] += ri * ri;
+        v[1] += bi * bi;
+    });
+    write_partials<2>(v, part_out);
+}
+
+// one work-group: totals into out[0..1] = (|r|, |b|)
+__global__ void __launch_bounds__(BS) k_ir_norms(const double *partials,
+                                                 int np, double *out)
+{
+    double t[2];
+    sum_partials<2>(partials, np, t);
+    if (threadIdx.x == 0) {
+        out[0] = sqrt(t[0]);
+        out[1] = sqrt(t[1]);
     }
 }
 
-// inner right-hand side r32 = r / |r|, x32 = 0, inner state for the FP32 PCG
+// inner right-hand side r32 = r * scale, x32 = 0; partials as k_pcg_init_res
 __global__ void __launch_bounds__(BS) k_ir_inner_init(
-    int64_t n, const double *r, const float *diag32, float *r32, float *x32,
-    double *partials, const State *so, State *si)
+    int64_t n, const double *r, double scale, const float *diag32, float *r32,
+    float *x32, double *part_out)
 {
-    if (so->done)
-        return;
-    const double inv = 1.0 / so->res;
-    double v[3] = {0.0, 0.0, 0.0}, o[3];
+    double v[3] = {0.0, 0.0, 0.0};
     for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * BS) {
-        float ri = (float)(r[i] * inv);
+        float ri = (float)(r[i] * scale);
         r32[i] = ri;
         x32[i] = 0.0f;
         float d = diag32[i];
@@ -411,31 +485,16 @@ __global__ void __launch_bounds__(BS) k_ir_inner_init(
         else
             v[1] += (double)ri * (double)(ri / d);
     }
-    if (reduce_last<3>(v, partials, &si->counter[0], o) && threadIdx.x == 0) {
-        double res = sqrt(o[0]);
-        si->nits = 0;
-        si->res = si->r0 = res;
-        si->tol = res * si->rtol;
-        si->rz = o[1];
-        si->rz_prev = 0.0;
-        si->done = res > si->tol ? (o[2] != 0.0 ? ZERO_DIAG : RUNNING)
-                                 : CONVERGED;
-    }
+    write_partials<3>(v, part_out);
 }
 
-// x += |r| * d  (FP64 correction)
+// x += s * d  (FP64 correction)
 __global__ void __launch_bounds__(BS) k_ir_correct(int64_t n, double *x,
-                                                   const float *d,
-                                                   State *so)
+                                                   const float *d, double s)
 {
-    if (so->done)
-        return;
-    const double s = so->res;
     for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * BS)
         x[i] += s * (double)d[i];
-    if (blockIdx.x == 0 && threadIdx.x == 0)
-        so->outer += 1;   // visible to the next kernel on the stream
 }
 
 __global__ void k_to_f32(int64_t n, const double *a, float *b)
@@ -446,30 +505,32 @@ __global__ void k_to_f32(int64_t n, const double *a, float *b)
 }
 
 //------------------------------------------------------------- host side
+// partial-sum segments (each MAX_GRID * 4 doubles)
+enum { SEG_NORMS = 0, SEG_RES, SEG_S, SEG_U, SEG_IR, SEG_N };
+
 struct Workspace {
     int64_t n = 0;
     double *r = nullptr, *p = nullptr, *Ap = nullptr, *partials = nullptr;
+    double *scal = nullptr;               // small device scalars
     float *r32 = nullptr, *p32 = nullptr, *Ap32 = nullptr, *x32 = nullptr;
-    State *st = nullptr;      // [2]: outer/FP64, inner/FP32
+    State *st = nullptr;                  // [2] ping-pong copies
+    double *seg(int k) { return partials + (size_t)k * 4 * MAX_GRID; }
 };
-
-constexpr int MAX_GRID = 256 * 8;   // partial-sum slots per reduced value
 
 int grid_for(int64_t n)
 {
     int cus = phg::ctx()->num_cus > 0 ? phg::ctx()->num_cus : 256;
     int64_t g = (n + BS - 1) / BS;
-    int64_t cap = std::min<int64_t>((int64_t)cus * 8, MAX_GRID);
+    int64_t cap = std::min<int64_t>((int64_t)cus * 4, MAX_GRID);
     return (int)std::max<int64_t>(1, std::min(g, cap));
 }
 
-int grid_rows(int64_t n, int L)
+int grid_rows(const phg_hip_mat *A)
 {
-    // each work-group covers a contiguous range of several row groups
+    // each work-group covers a contiguous range of row blocks
     int cus = phg::ctx()->num_cus > 0 ? phg::ctx()->num_cus : 256;
-    int64_t groups = (n + BS / L - 1) / (BS / L);
     int64_t cap = std::min<int64_t>((int64_t)cus * 8, MAX_GRID);
-    return (int)std::max<int64_t>(1, std::min(groups, cap));
+    return (int)std::max<int64_t>(1, std::min<int64_t>(A->nrblk, cap));
 }
 
 Workspace *get_ws(const phg_hip_mat *A, int need32)
@@ -477,17 +538,17 @@ Workspace *get_ws(const phg_hip_mat *A, int need32)
     static Workspace ws;     // one solve at a time per process (one stream)
     if (ws.n != A->n) {
         phg::dfree(ws.r); phg::dfree(ws.p); phg::dfree(ws.Ap);
-        phg::dfree(ws.partials); phg::dfree(ws.st);
+        phg::dfree(ws.partials); phg::dfree(ws.st); phg::dfree(ws.scal);
         phg::dfree(ws.r32); phg::dfree(ws.p32); phg::dfree(ws.Ap32);
         phg::dfree(ws.x32);
         ws = Workspace();
         size_t n8 = (size_t)A->n * sizeof(double);
         if (phg::dmalloc((void **)&ws.r, n8) || phg::dmalloc((void **)&ws.p, n8) ||
             phg::dmalloc((void **)&ws.Ap, n8) ||
-            phg::dmalloc((void **)&ws.partials, 4 * MAX_GRID * sizeof(double)) ||
+            phg::dmalloc((void **)&ws.partials,
+                         (size_t)SEG_N * 4 * MAX_GRID * sizeof(double)) ||
+            phg::dmalloc((void **)&ws.scal, 16 * sizeof(double)) ||
             phg::dmalloc((void **)&ws.st, 2 * sizeof(State)))
-            return nullptr;
-        if (hipMemsetAsync(ws.st, 0, 2 * sizeof(State), phg::ctx()->stream) != hipSuccess)
             return nullptr;
         ws.n = A->n;
     }
@@ -500,43 +561,82 @@ Workspace *get_ws(const phg_hip_mat *A, int need32)
     return &ws;
 }
 
-#define LANES_SWITCH(L, CALL)                                                 \
-    switch (L) {                                                              \
-    case 4: { constexpr int LL = 4; CALL; } break;                            \
-    case 16: { constexpr int LL = 16; CALL; } break;                          \
-    case 32: { constexpr int LL = 32; CALL; } break;                          \
-    default: { constexpr int LL = 8; CALL; } break;                           \
-    }
 
-// one PCG iteration (3 kernels) on the given precision
+// Optional per-kernel timing with HIP events recorded on the library stream
+// around each launch of the PCG kernels (phg_hip_profile).
+enum { KP_P64 = 0, KP_SPMV64, KP_UPD64, KP_P32, KP_SPMV32, KP_UPD32, KP_N };
+struct KernelProfile {
+    bool on = false;
+    std::vector<hipEvent_t> ev[KP_N];   // start/stop pairs
+    size_t used[KP_N] = {};
+    int64_t launches[KP_N] = {};
+    double ms[KP_N] = {};
+};
+KernelProfile g_prof;
+
+inline void prof_mark(int kind, hipStream_t s)
+{
+    std::vector<hipEvent_t> &v = g_prof.ev[kind];
+    size_t &u = g_prof.used[kind];
+    if (u >= v.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess)
+            return;
+        v.push_back(e);
+    }
+    (void)hipEventRecord(v[u++], s);
+}
+
+// folds the recorded pairs into the totals (the stream must be idle)
+void prof_collect()
+{
+    for (int k = 0; k < KP_N; k++) {
+        std::vector<hipEvent_t> &v = g_prof.ev[k];
+        for (size_t i = 0; i + 1 < g_prof.used[k]; i += 2) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, v[i], v[i + 1]) == hipSuccess) {
+                g_prof.ms[k] += ms;
+                g_prof.launches[k] += 1;
+            }
+        }
+        g_prof.used[k] = 0;
+    }
+}
+
+// one PCG iteration (3 kernels) with launch index k on the given precision;
+// part_in holds the residual partials of the previous iteration (or of the
+// prologue when k == 0)
 template <typename T>
-void launch_iteration(const phg_hip_mat *A, int L, const T *va, const T *diag,
-                      T *x, T *r, T *p, T *Ap, double *partials, State *st,
-                      hipStream_t s)
+void launch_iteration(const phg_hip_mat *A, const T *va, const T *diag,
+                      T *x, T *r, T *p, T *Ap, Workspace *ws, State *st,
+                      int64_t k, int np_first, hipStream_t s)
 {
     int64_t n = A->n;
-    int g = grid_for(n), gr = grid_rows(n, L);
+    int g = grid_for(n), gr = grid_rows(A);
+    const int np_in = k == 0 ? np_first : g;
+    State *sin = st + (k & 1), *sout = st + ((k + 1) & 1);
+    double *pres = ws->seg(SEG_RES), *ps = ws->seg(SEG_S);
+    const int base = sizeof(T) == 8 ? KP_P64 : KP_P32;
+    const bool prof = g_prof.on;
+    if (prof) prof_mark(base + 0, s);
     hipLaunchKernelGGL(k_pcg_p<T>, dim3(g), dim3(BS), 0, s, n, (const T *)r,
-                       diag, p, (const State *)st);
-    LANES_SWITCH(L, hipLaunchKernelGGL((k_pcg_spmv<LL, T>), dim3(gr), dim3(BS),
-                                       0, s, n, (const int32_t *)A->d_rowptr,
-                                       (const int32_t *)A->d_cols, va,
-                                       (const T *)p, Ap, partials, st));
+                       diag, p, (const double *)pres, np_in,
+                       (const State *)sin, sout);
+    if (prof) { prof_mark(base + 0, s); prof_mark(base + 1, s); }
+    hipLaunchKernelGGL(k_pcg_spmv<T>, dim3(gr), dim3(BS), 0, s,
+                       (const int32_t *)A->d_rowptr, (const int32_t *)A->d_cols,
+                       va, (const int32_t *)A->d_rblk, A->nrblk, (const T *)p,
+                       Ap, ps, (const State *)sout);
+    if (prof) { prof_mark(base + 1, s); prof_mark(base + 2, s); }
     hipLaunchKernelGGL(k_pcg_update<T>, dim3(g), dim3(BS), 0, s, n, x,
-                       (const T *)p, r, (const T *)Ap, diag, partials, st);
+                       (const T *)p, r, (const T *)Ap, diag,
+                       (const double *)ps, gr, pres, sout);
+    if (prof) prof_mark(base + 2, s);
 }
 
-int read_state(State *d, State *h, int count, hipStream_t s)
+int read_state(State *d, State *h, hipStream_t s)
 {
-    PHG_HIP_CHECK(hipMemcpyAsync(h, d, count * sizeof(State),
-                                 hipMemcpyDeviceToHost, s));
-    PHG_HIP_CHECK(hipStreamSynchronize(s));
-    return PHG_HIP_OK;
-}
-
-int upload_state(State *d, const State &h, hipStream_t s)
-{
-    PHG_HIP_CHECK(hipMemcpyAsync(d, &h, sizeof(State), hipMemcpyHostToDevice, s));
+    PHG_HIP_CHECK(hipMemcpyAsync(h, d, sizeof(State), hipMemcpyDeviceToHost, s));
     PHG_HIP_CHECK(hipStreamSynchronize(s));
     return PHG_HIP_OK;
 }
@@ -559,17 +659,79 @@ int check_solve_args(const phg_hip_mat *A, const phg_hip_vec *b,
     return PHG_HIP_OK;
 }
 
+// Runs PCG iterations (launch indices from 0) until the state copy reports
+// completion; *out receives the final state.
+template <typename T>
+int run_pcg_loop(const phg_hip_mat *A, const T *va, const T *diag,
+                 T *x, T *r, T *p, T *Ap, Workspace *ws, int maxit,
+                 int np_first, int first_chunk, State *out)
+{
+    hipStream_t s = phg::ctx()->stream;
+    State *hm = (State *)phg::ctx()->host_mailbox;
+    int64_t k = 0;
+    int chunk = g_prof.on ? 1 : first_chunk;
+    int rc;
+    for (;;) {
+        for (int i = 0; i < chunk; i++, k++)
+            launch_iteration<T>(A, va, diag, x, r, p, Ap, ws, ws->st, k,
+                                np_first, s);
+        PHG_HIP_CHECK(hipGetLastError());
+        if ((rc = read_state(ws->st + (k & 1), hm, s)))
+            return rc;
+        if (hm->done != RUNNING)
+            break;
+        // the device stops at maxit; a state that never finishes means the
+        // kernels did not run
+        if (k > (int64_t)maxit + 256)
+            return phg::set_error(PHG_HIP_E_HIP, "PCG state did not advance");
+        if (!g_prof.on)
+            chunk = std::min(chunk * 2, 64);
+    }
+    if (g_prof.on)
+        prof_collect();
+    *out = *hm;
+    return PHG_HIP_OK;
+}
+
 }  // namespace
 
 namespace phg {
 
-int choose_spmv_shape(phg_hip_mat *A)
+int build_row_blocks(phg_hip_mat *A)
 {
-    // lanes per row from the mean row length: one or two passes per lane
-    double mean = A->n > 0 ? (double)A->nnz / (double)A->n : 0.0;
-    int L = mean <= 6 ? 4 : (mean <= 20 ? 8 : (mean <= 48 ? 16 : 32));
-    A->spmv_lanes = L;
-    A->spmv_lanes32 = L;
+    // greedy split into runs of consecutive rows holding at most NNZ_CAP
+    // entries and ROW_CAP rows; a longer row forms a block of its own
+    std::vector<int32_t> rp((size_t)A->n + 1), blk;
+    int rc;
+    if ((rc = download(rp.data(), A->d_rowptr, rp.size() * sizeof(int32_t))))
+        return rc;
+    blk.reserve((size_t)(A->nnz / NNZ_CAP + A->n / ROW_CAP + 2));
+    blk.push_back(0);
+    int64_t cur_nnz = 0, cur_rows = 0;
+    for (int64_t r = 0; r < A->n; r++) {
+        int64_t len = rp[(size_t)r + 1] - rp[(size_t)r];
+        if (len > NNZ_CAP) {
+            if (cur_rows > 0)
+                blk.push_back((int32_t)r);
+            blk.push_back((int32_t)(r + 1));
+            cur_nnz = cur_rows = 0;
+            continue;
+        }
+        if (cur_rows > 0 && (cur_nnz + len > NNZ_CAP || cur_rows == ROW_CAP)) {
+            blk.push_back((int32_t)r);
+            cur_nnz = cur_rows = 0;
+        }
+        cur_nnz += len;
+        cur_rows++;
+    }
+    if (cur_rows > 0)
+        blk.push_back((int32_t)A->n);
+    dfree(A->d_rblk);
+    A->d_rblk = nullptr;
+    A->nrblk = (int64_t)blk.size() - 1;
+    if ((rc = dmalloc((void **)&A->d_rblk, blk.size() * sizeof(int32_t))) ||
+        (rc = upload(A->d_rblk, blk.data(), blk.size() * sizeof(int32_t))))
+        return rc;
     return PHG_HIP_OK;
 }
 
@@ -594,13 +756,12 @@ int ensure_fp32_copy(phg_hip_mat *A)
 int spmv64(const phg_hip_mat *A, const double *x, double *y, double alpha,
            double beta, hipStream_t s)
 {
-    int L = A->spmv_lanes;
-    int gr = grid_rows(A->n, L);
-    LANES_SWITCH(L, hipLaunchKernelGGL((k_spmv64<LL>), dim3(gr), dim3(BS), 0,
-                                       s, A->n, (const int32_t *)A->d_rowptr,
-                                       (const int32_t *)A->d_cols,
-                                       (const double *)A->d_vals, x, y, alpha,
-                                       beta));
+    if (A->nrblk == 0)
+        return PHG_HIP_OK;
+    hipLaunchKernelGGL(k_spmv64, dim3(grid_rows(A)), dim3(BS), 0, s,
+                       (const int32_t *)A->d_rowptr, (const int32_t *)A->d_cols,
+                       (const double *)A->d_vals, (const int32_t *)A->d_rblk,
+                       A->nrblk, x, y, alpha, beta);
     PHG_HIP_CHECK(hipGetLastError());
     return PHG_HIP_OK;
 }
@@ -634,19 +795,15 @@ extern "C" int phg_hip_vec_dot(const phg_hip_vec *x, const phg_hip_vec *y,
         return phg::set_error(PHG_HIP_E_ARG, "phg_hip_vec_dot: bad args");
     hipStream_t s = phg::ctx()->stream;
     double *part = nullptr;
-    unsigned *cnt = nullptr;
-    double *out = nullptr;
-    if ((rc = phg::dmalloc((void **)&part, (MAX_GRID + 2) * sizeof(double))))
+    if ((rc = phg::dmalloc((void **)&part, (MAX_GRID + 1) * sizeof(double))))
         return rc;
-    cnt = (unsigned *)(part + MAX_GRID);
-    out = part + MAX_GRID + 1;
-    PHG_HIP_CHECK(hipMemsetAsync(cnt, 0, sizeof(unsigned), s));
     int g = grid_for(x->n);
     hipLaunchKernelGGL(k_dot, dim3(g), dim3(BS), 0, s, x->n,
-                       (const double *)x->d, (const double *)y->d, part, cnt,
-                       out);
+                       (const double *)x->d, (const double *)y->d, part);
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(BS), 0, s,
+                       (const double *)part, g, part + MAX_GRID);
     PHG_HIP_CHECK(hipGetLastError());
-    rc = phg::download(res, out, sizeof(double));
+    rc = phg::download(res, part + MAX_GRID, sizeof(double));
     phg::dfree(part);
     return rc;
 }
@@ -703,48 +860,32 @@ extern "C" int phg_hip_pcg(const phg_hip_mat *A, const phg_hip_vec *b,
     h.btol = btol;
     h.atol = atol;
     h.maxit = maxit;
-    if ((rc = upload_state(ws->st, h, s)))
-        return rc;
+    PHG_HIP_CHECK(hipMemcpyAsync(ws->st, &h, sizeof(State),
+                                 hipMemcpyHostToDevice, s));
     const int64_t n = A->n;
-    const int L = A->spmv_lanes;
-    int g = grid_for(n), gr = grid_rows(n, L);
-    hipLaunchKernelGGL(k_pcg_init_norms<8>, dim3(g), dim3(BS), 0, s, n,
-                       (const double *)b->d, (const double *)x->d, ws->partials,
+    int g = grid_for(n), gr = grid_rows(A);
+    hipLaunchKernelGGL(k_pcg_init_norms, dim3(g), dim3(BS), 0, s, n,
+                       (const double *)b->d, (const double *)x->d,
+                       ws->seg(SEG_NORMS));
+    hipLaunchKernelGGL(k_pcg_init_res, dim3(gr), dim3(BS), 0, s, n,
+                       (const int32_t *)A->d_rowptr, (const int32_t *)A->d_cols,
+                       (const double *)A->d_vals, (const int32_t *)A->d_rblk,
+                       A->nrblk, (const double *)A->d_diag,
+                       (const double *)b->d, x->d, ws->r,
+                       (const double *)ws->seg(SEG_NORMS), g, ws->seg(SEG_RES),
                        ws->st);
-    LANES_SWITCH(L, hipLaunchKernelGGL((k_pcg_init_res<LL>), dim3(gr),
-                                       dim3(BS), 0, s, n,
-                                       (const int32_t *)A->d_rowptr,
-                                       (const int32_t *)A->d_cols,
-                                       (const double *)A->d_vals,
-                                       (const double *)A->d_diag,
-                                       (const double *)b->d, x->d, ws->r,
-                                       ws->partials, ws->st));
     PHG_HIP_CHECK(hipGetLastError());
-    State *hm = (State *)phg::ctx()->host_mailbox;
-    int chunk = 8;
-    int64_t launched = 0;
-    for (;;) {
-        if ((rc = read_state(ws->st, hm, 1, s)))
-            return rc;
-        if (hm->done != RUNNING)
-            break;
-        // the device stops at maxit; a state that never finishes means the
-        // kernels did not run
-        if (launched > (int64_t)maxit + 256)
-            return phg::set_error(PHG_HIP_E_HIP, "PCG state did not advance");
-        launched += chunk;
-        for (int k = 0; k < chunk; k++)
-            launch_iteration<double>(A, L, A->d_vals, A->d_diag, x->d, ws->r,
-                                     ws->p, ws->Ap, ws->partials, ws->st, s);
-        PHG_HIP_CHECK(hipGetLastError());
-        chunk = std::min(chunk * 2, 64);
-    }
+    State fin;
+    // the prologue ran on the row grid: its partials count is gr
+    if ((rc = run_pcg_loop<double>(A, A->d_vals, A->d_diag, x->d, ws->r,
+                                   ws->p, ws->Ap, ws, maxit, gr, 8, &fin)))
+        return rc;
     if (residual)
-        *residual = hm->res;
-    if (hm->done == ZERO_DIAG)
+        *residual = fin.res;
+    if (fin.done == ZERO_DIAG)
         return phg::set_error(PHG_HIP_E_ZERO_DIAG,
                               "zero diagonal in Jacobi preconditioner");
-    return hm->nits;
+    return fin.nits;
 }
 
 extern "C" int phg_hip_pcg_mixed(const phg_hip_mat *A_, const phg_hip_vec *b,
@@ -762,70 +903,91 @@ extern "C" int phg_hip_pcg_mixed(const phg_hip_mat *A_, const phg_hip_vec *b,
     Workspace *ws = get_ws(A, 1);
     if (ws == nullptr)
         return phg::set_error(PHG_HIP_E_NOMEM, "workspace allocation failed");
-    State h[2];
-    memset(h, 0, sizeof(h));
-    h[0].rtol = rtol;
-    h[0].btol = btol;
-    h[0].atol = atol;
-    h[0].maxit = maxit;
-    h[1].rtol = inner_rtol;
-    h[1].maxit = 100000;   // each FP32 solve stops on inner_rtol
-    PHG_HIP_CHECK(hipMemcpyAsync(ws->st, h, 2 * sizeof(State),
-                                 hipMemcpyHostToDevice, s));
     const int64_t n = A->n;
-    const int L = A->spmv_lanes, L32 = A->spmv_lanes32;
-    int g = grid_for(n), gr = grid_rows(n, L);
-    State *so = ws->st, *si = ws->st + 1;
-    State *hm = (State *)phg::ctx()->host_mailbox;
+    int g = grid_for(n), gr = grid_rows(A);
+    double *hm = (double *)phg::ctx()->host_mailbox;
+    double tol = 0.0, res = 0.0;
     int total_inner = 0, outer = 0;
     for (;;) {
-        LANES_SWITCH(L, hipLaunchKernelGGL((k_ir_residual<LL>), dim3(gr),
-                                           dim3(BS), 0, s, n,
-                                           (const int32_t *)A->d_rowptr,
-                                           (const int32_t *)A->d_cols,
-                                           (const double *)A->d_vals,
-                                           (const double *)b->d,
-                                           (const double *)x->d, ws->r,
-                                           ws->partials, so));
-        hipLaunchKernelGGL(k_ir_inner_init, dim3(g), dim3(BS), 0, s, n,
-                           (const double *)ws->r, (const float *)A->d_diag32,
-                           ws->r32, ws->x32, ws->partials, (const State *)so,
-                           si);
+        hipLaunchKernelGGL(k_ir_residual, dim3(gr), dim3(BS), 0, s,
+                           (const int32_t *)A->d_rowptr,
+                           (const int32_t *)A->d_cols, (const double *)A->d_vals,
+                           (const int32_t *)A->d_rblk, A->nrblk,
+                           (const double *)b->d, (const double *)x->d, ws->r,
+                           ws->seg(SEG_IR));
+        hipLaunchKernelGGL(k_ir_norms, dim3(1), dim3(BS), 0, s,
+                           (const double *)ws->seg(SEG_IR), gr, ws->scal);
         PHG_HIP_CHECK(hipGetLastError());
-        if ((rc = read_state(ws->st, hm, 2, s)))
-            return rc;
-        if (hm[0].done != RUNNING || outer >= maxit)
+        PHG_HIP_CHECK(hipMemcpyAsync(hm, ws->scal, 2 * sizeof(double),
+                                     hipMemcpyDeviceToHost, s));
+        PHG_HIP_CHECK(hipStreamSynchronize(s));
+        res = hm[0];
+        const double rb = hm[1];
+        if (outer == 0) {
+            // get_tol on the FP64 residual (solver-phg.c:104)
+            tol = res * rtol;
+            if (tol < rb * btol)
+                tol = rb * btol;
+            if (tol < atol)
+                tol = atol;
+        }
+        if (res <= tol || rb == 0.0 || outer >= maxit)
             break;
-        if (hm[1].done == ZERO_DIAG)
+        State h;
+        memset(&h, 0, sizeof(h));
+        h.rtol = inner_rtol;
+        h.rb = 1.0;
+        h.maxit = 100000;     // each FP32 solve stops on inner_rtol
+        PHG_HIP_CHECK(hipMemcpyAsync(ws->st, &h, sizeof(State),
+                                     hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_ir_inner_init, dim3(g), dim3(BS), 0, s, n,
+                           (const double *)ws->r, 1.0 / res,
+                           (const float *)A->d_diag32, ws->r32, ws->x32,
+                           ws->seg(SEG_RES));
+        PHG_HIP_CHECK(hipGetLastError());
+        State fin;
+        if ((rc = run_pcg_loop<float>(A, A->d_vals32, A->d_diag32, ws->x32,
+                                      ws->r32, ws->p32, ws->Ap32, ws, h.maxit,
+                                      g, 16, &fin)))
+            return rc;
+        if (fin.done == ZERO_DIAG)
             return phg::set_error(PHG_HIP_E_ZERO_DIAG,
                                   "zero diagonal in Jacobi preconditioner");
-        int chunk = 16;
-        int64_t launched = 0;
-        for (;;) {
-            if (launched > (int64_t)h[1].maxit + 256)
-                return phg::set_error(PHG_HIP_E_HIP,
-                                      "inner PCG state did not advance");
-            launched += chunk;
-            for (int k = 0; k < chunk; k++)
-                launch_iteration<float>(A, L32, A->d_vals32, A->d_diag32,
-                                        ws->x32, ws->r32, ws->p32, ws->Ap32,
-                                        ws->partials, si, s);
-            PHG_HIP_CHECK(hipGetLastError());
-            if ((rc = read_state(si, hm + 1, 1, s)))
-                return rc;
-            if (hm[1].done != RUNNING)
-                break;
-            chunk = std::min(chunk * 2, 64);
-        }
-        total_inner += hm[1].nits;
+        total_inner += fin.nits;
         hipLaunchKernelGGL(k_ir_correct, dim3(g), dim3(BS), 0, s, n, x->d,
-                           (const float *)ws->x32, so);
+                           (const float *)ws->x32, res);
         PHG_HIP_CHECK(hipGetLastError());
         outer++;
     }
     if (inner_its)
         *inner_its = total_inner;
     if (residual)
-        *residual = hm[0].res;
+        *residual = res;
     return outer;
+}
+
+extern "C" int phg_hip_profile(int32_t on)
+{
+    int rc;
+    if ((rc = phg::require_ctx()))
+        return rc;
+    PHG_HIP_CHECK(hipStreamSynchronize(phg::ctx()->stream));
+    prof_collect();
+    for (int k = 0; k < KP_N; k++) {
+        g_prof.launches[k] = 0;
+        g_prof.ms[k] = 0.0;
+    }
+    g_prof.on = on != 0;
+    return PHG_HIP_OK;
+}
+
+extern "C" int phg_hip_profile_read(int32_t kind, int64_t *launches, double *ms)
+{
+    if (kind < 0 || kind >= KP_N)
+        return phg::set_error(PHG_HIP_E_ARG, "profile kind %d out of range", kind);
+    if (launches)
+        *launches = g_prof.launches[kind];
+    if (ms)
+        *ms = g_prof.ms[kind];
+    return PHG_HIP_OK;
 }

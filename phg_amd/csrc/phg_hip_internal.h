@@ -67,16 +67,17 @@ struct phg_hip_mat {
     // FP32 copy of the values for the mixed-precision inner solve
     float *d_vals32 = nullptr;
     float *d_diag32 = nullptr;
-    // SpMV launch shape chosen for this matrix (lanes per row)
-    int spmv_lanes = 8;
-    int spmv_lanes32 = 8;
+    // SpMV row blocks: consecutive rows whose entries fit the LDS stage of
+    // the streaming SpMV (or one long row); boundaries [nrblk + 1]
+    int32_t *d_rblk = nullptr;
+    int64_t nrblk = 0;
 };
 
 namespace phg {
 // Krylov building blocks (krylov.hip) used by the C ABI layer.
 int spmv64(const phg_hip_mat *A, const double *x, double *y, double alpha,
            double beta, hipStream_t s);
-int choose_spmv_shape(phg_hip_mat *A);
+int build_row_blocks(phg_hip_mat *A);
 int ensure_fp32_copy(phg_hip_mat *A);
 int setup_diagonal(phg_hip_mat *A);
 }  // namespace phg

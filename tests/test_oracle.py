@@ -1,0 +1,363 @@
+"""Pins the oracle: closed forms, invariants, brute force and the reference's
+own known-answer programs.  Each check is chosen so that a plausible mistake
+(a dropped term, a wrong sign or index, a transposed operand) fails it.
+CPU only."""
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from tests._cases import elasticity_case, poisson_case
+
+# ------------------------------------------------------------ quadrature
+
+
+def _simplex_moment(alpha):
+    # volume-normalised integral of prod lambda_i^a_i over a tetrahedron
+    return 6.0 * np.prod([math.factorial(a) for a in alpha]) / math.factorial(sum(alpha) + 3)
+
+
+@pytest.mark.parametrize("order", range(0, 7))
+def test_quad_weights_and_exactness(order):
+    pts, w = orc.quad3d(order)
+    assert abs(w.sum() - 1.0) < 1e-14
+    assert np.allclose(pts.sum(axis=1), 1.0, atol=1e-15)
+    deg = max(order, 1)
+    for alpha in itertools.product(range(deg + 1), repeat=4):
+        if sum(alpha) > deg:
+            continue
+        val = np.sum(w * np.prod(pts ** np.array(alpha), axis=1))
+        assert abs(val - _simplex_moment(alpha)) < 1e-13, (order, alpha)
+
+
+@pytest.mark.parametrize("order", range(0, 7))
+def test_quad_test_c_closed_form(order):
+    # test/quad_test.c:129-153: sum_i lambda_i^p -> 4*6/((p+1)(p+2)(p+3))
+    pts, w = orc.quad3d(order)
+    b = np.sum(w * (pts ** order).sum(axis=1))
+    a = 4.0 / ((order + 1) * (order + 2) * (order + 3)) * 6.0
+    assert abs((a - b) / b) < 1e4 * np.finfo(float).eps
+
+
+def test_quad_point_counts():
+    assert [orc.quad3d(k)[0].shape[0] for k in range(7)] == [1, 1, 4, 8, 14, 14, 24]
+
+# ----------------------------------------------------------------- bases
+
+
+def _nodes(p):
+    """Barycentric nodes of local bases (lagrange.c:1646 points tables)."""
+    E = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
+    F = [(1, 2, 3), (0, 2, 3), (0, 1, 3), (0, 1, 2)]
+    nodes = [np.eye(4)[k] for k in range(4)]
+    if p == 2:
+        for a, b in E:
+            l = np.zeros(4); l[a] = l[b] = .5; nodes.append(l)
+    if p == 3:
+        for a, b in E:
+            for wa, wb in ((2 / 3, 1 / 3), (1 / 3, 2 / 3)):
+                l = np.zeros(4); l[a], l[b] = wa, wb; nodes.append(l)
+        for f in F:
+            l = np.zeros(4); l[list(f)] = 1 / 3; nodes.append(l)
+    return np.array(nodes)
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_bases_nodal_and_partition_of_unity(p):
+    N = _nodes(p)
+    V = np.array([orc.bas(p, l) for l in N])
+    assert np.allclose(V, np.eye(len(N)), atol=1e-14)
+    rng = np.random.default_rng(0)
+    for _ in range(20):
+        l = rng.dirichlet(np.ones(4))
+        assert abs(orc.bas(p, l).sum() - 1.0) < 1e-14
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_lambda_gradients_finite_difference(p):
+    rng = np.random.default_rng(1)
+    h = 1e-6
+    for _ in range(5):
+        l = rng.dirichlet(np.ones(4))
+        G = orc.grad_lambda(p, l)
+        for a in range(4):
+            e = np.zeros(4); e[a] = h
+            fd = (orc.bas(p, l + e) - orc.bas(p, l - e)) / (2 * h)
+            assert np.allclose(G[:, a], fd, atol=1e-8), (p, a)
+
+# -------------------------------------------------------------- geometry
+
+
+def test_jacobian_inverts_vertex_matrix():
+    rng = np.random.default_rng(2)
+    for _ in range(20):
+        x = rng.standard_normal((4, 3))
+        J = orc.jacobian(x)
+        X = np.hstack([x, np.ones((4, 1))])          # rows [x y z 1]
+        assert np.allclose(J @ X.T, np.eye(4), atol=1e-10)
+        vol = abs(np.linalg.det(X[1:, :3] - X[0, :3])) / 6.0
+        assert abs(orc.volume(x) - vol) < 1e-13 * max(1.0, vol)
+
+# ------------------------------------------------------- element matrices
+
+
+def _grad_lambda_x(x):
+    X = np.hstack([x, np.ones((4, 1))])
+    return np.linalg.inv(X)[:3, :].T      # row a: grad of lambda_a
+
+
+def _brute_stiffness(p, x, kappa=1.0):
+    pts, w = orc.quad3d(6)
+    GL = _grad_lambda_x(x)
+    vol = abs(np.linalg.det(x[1:] - x[0])) / 6
+    nb = orc.nbas(p)
+    K = np.zeros((nb, nb))
+    for q in range(len(w)):
+        gx = orc.grad_lambda(p, pts[q]) @ GL      # [nb][3]
+        K += w[q] * gx @ gx.T
+    return kappa * vol * K
+
+
+def _brute_mass(p, x):
+    pts, w = orc.quad3d(6)
+    vol = abs(np.linalg.det(x[1:] - x[0])) / 6
+    V = np.array([orc.bas(p, pt) for pt in pts])
+    return vol * (V.T * w) @ V
+
+
+def _rand_tet(rng):
+    while True:
+        x = rng.standard_normal((4, 3))
+        if abs(np.linalg.det(x[1:] - x[0])) > 0.1:
+            return x
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_stiffness_and_mass_brute_force(p):
+    rng = np.random.default_rng(3)
+    for _ in range(4):
+        x = _rand_tet(rng)
+        K, _ = orc.element(orc.problem(p), x)
+        Kb = _brute_stiffness(p, x)
+        assert np.allclose(K, Kb, rtol=1e-11, atol=1e-11 * np.abs(Kb).max())
+        Kk, _ = orc.element(orc.problem(p, use_kappa=True), x, kappa=2.75)
+        assert np.allclose(Kk, 2.75 * Kb, rtol=1e-11, atol=1e-11 * np.abs(Kb).max())
+        Km, _ = orc.element(orc.problem(p, mass_coef=1.5), x)
+        Mb = _brute_mass(p, x)
+        assert np.allclose(Km - K, 1.5 * Mb, rtol=1e-10, atol=1e-12 * np.abs(Mb).max())
+        # constants lie in the kernel of the stiffness matrix
+        assert np.abs(K.sum(axis=1)).max() < 1e-11 * np.abs(K).max()
+
+
+def test_kuhn_p1_closed_form():
+    # Kuhn tetrahedron (000, 111, 110, 100) * h: path 000-100-110-111 gives
+    # K = h/6 * tridiag(-1, [1,2,2,1], -1) in path order
+    h = 0.25
+    x = h * np.array([[0, 0, 0], [1, 1, 1], [1, 1, 0], [1, 0, 0]], float)
+    K, _ = orc.element(orc.problem(1), x)
+    path = [0, 3, 2, 1]
+    T = np.array([[1, -1, 0, 0], [-1, 2, -1, 0], [0, -1, 2, -1], [0, 0, -1, 1]]) * h / 6
+    assert np.allclose(K[np.ix_(path, path)], T, atol=1e-15)
+    Mb = _brute_mass(1, x)
+    vol = h ** 3 / 6
+    assert np.allclose(Mb, vol / 20 * (np.ones((4, 4)) + np.eye(4)), atol=1e-15)
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_load_f_one_is_basis_integral(p):
+    rng = np.random.default_rng(4)
+    x = _rand_tet(rng)
+    nb = orc.nbas(p)
+    _, b = orc.element(orc.problem(p), x, f_loc=np.ones(nb))
+    pts, w = orc.quad3d(6)
+    vol = abs(np.linalg.det(x[1:] - x[0])) / 6
+    ref = vol * sum(w[q] * orc.bas(p, pts[q]) for q in range(len(w)))
+    assert np.allclose(b, ref, rtol=1e-12, atol=1e-14)
+    if p == 1:
+        assert np.allclose(b, vol / 4)
+    if p == 2:
+        assert np.allclose(b[:4], -vol / 20) and np.allclose(b[4:], vol / 5)
+
+
+def test_elasticity_element_vs_strain_energy():
+    rng = np.random.default_rng(5)
+    x = _rand_tet(rng)
+    E, nu = 2.5, 0.25
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+    mu = E / (2 * (1 + nu))
+    K, _ = orc.element(orc.problem(2, dim=3, E=E, nu=nu), x)
+    pts, w = orc.quad3d(6)
+    GL = _grad_lambda_x(x)
+    vol = abs(np.linalg.det(x[1:] - x[0])) / 6
+    Kb = np.zeros_like(K)
+    for q in range(len(w)):
+        g = orc.grad_lambda(2, pts[q]) @ GL          # [10][3]
+        for i in range(10):
+            for k in range(3):
+                Gu = np.zeros((3, 3)); Gu[k] = g[i]   # grad of phi_i e_k
+                for j in range(10):
+                    for l in range(3):
+                        Gv = np.zeros((3, 3)); Gv[l] = g[j]
+                        eu, ev = (Gu + Gu.T) / 2, (Gv + Gv.T) / 2
+                        Kb[3 * i + k, 3 * j + l] += w[q] * (
+                            lam * np.trace(Gu) * np.trace(Gv) + 2 * mu * np.sum(eu * ev))
+    Kb *= vol
+    assert np.allclose(K, Kb, rtol=1e-11, atol=1e-12 * np.abs(Kb).max())
+    # rigid translations are in the kernel
+    for k in range(3):
+        t = np.zeros(30); t[k::3] = 1.0
+        assert np.abs(K @ t).max() < 1e-11 * np.abs(K).max()
+
+# ------------------------------------------------------- assembly / solve
+
+
+def _dense_reference(c, pb):
+    """Brute-force dense assembly from oracle element matrices with the
+    driver's Dirichlet rows (examples/poisson.c:142-158)."""
+    p, dim = pb.p, pb.dim
+    n = c["ndof"] * dim
+    A = np.zeros((n, n))
+    b = np.zeros(n)
+    M = orc.nbas(p) * dim
+    for e in range(c["grid"].nelem):
+        m = c["map"][e]
+        I = (m[:, None] * dim + np.arange(dim)).ravel()
+        floc = c["f"].reshape(-1, dim)[m].ravel()
+        kap = 0.0 if c["kappa"] is None else c["kappa"][e]
+        K, B = orc.element(pb, c["verts"][c["elem_verts"][e]], kap, floc)
+        for a in range(M):
+            if c["bdry"][m[a // dim]]:
+                A[I[a], I[a]] += 1.0
+                b[I[a]] += c["g"][I[a]]
+            else:
+                A[I[a], I] += K[a]
+                b[I[a]] += B[a]
+    return A, b
+
+
+def _insertion_order(c, pb):
+    """Column order of matvec.c:3045 add_columns_to_row for every row."""
+    dim = pb.dim
+    M = orc.nbas(pb.p) * dim
+    rows = [[] for _ in range(c["ndof"] * dim)]
+    for e in range(c["grid"].nelem):
+        m = c["map"][e]
+        I = (m[:, None] * dim + np.arange(dim)).ravel()
+        floc = c["f"].reshape(-1, dim)[m].ravel()
+        kap = 0.0 if c["kappa"] is None else c["kappa"][e]
+        K, _ = orc.element(pb, c["verts"][c["elem_verts"][e]], kap, floc)
+        for a in range(M):
+            r = rows[I[a]]
+            cols = [I[a]] if c["bdry"][m[a // dim]] else \
+                [I[bb] for bb in range(M) if K[a, bb] != 0.0]
+            for col in cols:
+                if col not in r:
+                    r.append(col)
+    return rows
+
+
+@pytest.mark.parametrize("kind,p,n,dim", [("sin", 1, 3, 1), ("kappa", 2, 2, 1),
+                                          ("sin", 3, 1, 1), ("elast", 2, 1, 3)])
+def test_assembly_dense_and_order(kind, p, n, dim):
+    if dim == 3:
+        c = elasticity_case(n)
+        pb = orc.problem(2, dim=3)
+    else:
+        c = poisson_case(n, p, kind)
+        pb = orc.problem(p, use_kappa=c["kappa"] is not None)
+    A = orc.assemble(pb, c["verts"], c["elem_verts"], c["map"], c["ndof"], c["bdry"],
+                     c["g"], c["f"], c["kappa"])
+    Ad, bd = _dense_reference(c, pb)
+    assert np.allclose(A.to_scipy().toarray(), Ad, rtol=1e-13, atol=1e-15)
+    assert np.allclose(A.rhs, bd, rtol=1e-13, atol=1e-15)
+    order = _insertion_order(c, pb)
+    for i in range(A.n):
+        assert list(A.cols[A.rowptr[i]:A.rowptr[i + 1]]) == order[i]
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_linear_solution_reproduced(p):
+    # examples/functions.h TEST_CASE 1: u = x+y+z is in every Pk space
+    c = poisson_case(3, p, "linear")
+    A = orc.assemble(orc.problem(p), c["verts"], c["elem_verts"], c["map"], c["ndof"],
+                     c["bdry"], c["g"], c["f"])
+    orc.eliminate(A, c["bdry"])
+    x, nits, res = orc.pcg(A, A.rhs, rtol=1e-14, btol=1e-14)
+    assert np.abs(x - c["g"]).max() < 1e-11
+
+
+@pytest.mark.parametrize("p", [2, 3])
+def test_quadratic_solution_reproduced(p):
+    # u = x^2 + 2y^2 - z^2, -Lap u = -4: exact in P2 and P3 (nodal interpolant of
+    # a constant f is exact, so the discrete solution is the interpolant)
+    c = poisson_case(2, p, "linear")
+    xyz = c["coords"]
+    u = xyz[:, 0] ** 2 + 2 * xyz[:, 1] ** 2 - xyz[:, 2] ** 2
+    f = np.full(c["ndof"], -4.0)
+    A = orc.assemble(orc.problem(p), c["verts"], c["elem_verts"], c["map"], c["ndof"],
+                     c["bdry"], u, f)
+    orc.eliminate(A, c["bdry"])
+    x, nits, res = orc.pcg(A, A.rhs, rtol=1e-14, btol=1e-14)
+    assert np.abs(x - u).max() < 1e-11
+
+
+def test_elimination_matches_dense_schur():
+    c = poisson_case(3, 2, "kappa")
+    pb = orc.problem(2, use_kappa=True)
+    A = orc.assemble(pb, c["verts"], c["elem_verts"], c["map"], c["ndof"], c["bdry"],
+                     c["g"], c["f"], c["kappa"])
+    D = A.to_scipy().toarray()
+    rhs = A.rhs.copy()
+    bd = c["bdry"].astype(bool)
+    orc.eliminate(A, c["bdry"])
+    E = A.to_scipy().toarray()
+    # interior block unchanged, boundary columns gone, boundary rows diagonal
+    assert np.array_equal(E[np.ix_(~bd, ~bd)], D[np.ix_(~bd, ~bd)])
+    assert np.all(E[np.ix_(~bd, bd)] == 0)
+    assert np.array_equal(np.diag(E)[bd], np.diag(D)[bd])
+    # lifting: b_i -= sum_k A_ik g_k with g_k = rhs_k / d_k
+    gk = rhs[bd] / np.diag(D)[bd]
+    lifted = rhs[~bd] - D[np.ix_(~bd, bd)] @ gk
+    assert np.allclose(A.rhs[~bd], lifted, rtol=1e-12, atol=1e-14)
+
+
+def test_mat_test1_product():
+    # test/mat_test1.c: A(I,J) = 1 + I + J, x_J = 1 + J  ->  closed form
+    n = 7
+    I, J = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    A = orc.csr_from_arrays(np.arange(0, n * n + 1, n), J.ravel(), (1.0 + I + J).ravel())
+    x = 1.0 + np.arange(n, dtype=float)
+    y = np.empty(n)
+    orc.lib.orc_matvec(orc.C.byref(A._s), x.ctypes.data_as(orc._P(orc._dbl)),
+                       y.ctypes.data_as(orc._P(orc._dbl)))
+    ref = np.array([sum((1 + i + j) * (1 + j) for j in range(n)) for i in range(n)])
+    assert np.array_equal(y, ref.astype(float))
+
+
+def test_pcg_diagonal_system_one_iteration():
+    # test/solver_test.c: diagonal systems are solved exactly in one step
+    d = np.arange(1.0, 13.0)
+    x, nits, res = orc.pcg_arrays(np.arange(13), np.arange(12), d, 2.0 * d)
+    assert nits == 1 and np.allclose(x, 2.0, atol=0)
+
+
+def test_pcg_against_scipy_cg():
+    import scipy.sparse.linalg as spla
+    c = poisson_case(4, 2, "kappa")
+    A = orc.assemble(orc.problem(2, use_kappa=True), c["verts"], c["elem_verts"],
+                     c["map"], c["ndof"], c["bdry"], c["g"], c["f"], c["kappa"])
+    orc.eliminate(A, c["bdry"])
+    x, nits, res = orc.pcg(A, A.rhs, rtol=1e-12, btol=1e-12)
+    S = A.to_scipy()
+    xs = np.linalg.solve(S.toarray(), A.rhs)
+    assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-10
+    d = S.diagonal()
+    Mi = spla.LinearOperator(S.shape, matvec=lambda v: v / d)
+    its = []
+    spla.cg(S, A.rhs, rtol=1e-12, atol=0.0, M=Mi, maxiter=10000,
+            callback=lambda xk: its.append(1))
+    assert abs(len(its) - nits) <= 2
+    # the reported residual is |b - A x|
+    assert abs(res - np.linalg.norm(A.rhs - S @ x)) < 1e-8 * np.linalg.norm(A.rhs)
