@@ -113,6 +113,11 @@ def main():
     t_asm = time.perf_counter() - t0
     n, nnz = A.info()
     log(f"assembled: n={n} nnz={nnz} ({nnz / n:.1f}/row) in {t_asm:.3f}s")
+    t0 = time.perf_counter()
+    tune = A.tune(reps=20)
+    log(f"SpMV layouts (us fp64/fp32): " + ", ".join(
+        f"{k} {v[0]:.1f}/{v[1]:.1f}" for k, v in tune.items() if k != "chosen")
+        + f"; chosen {tune['chosen']} ({time.perf_counter() - t0:.2f}s)")
 
     b = A.rhs()
     # FP64 reference solution (untimed, converged far below the target)
@@ -170,13 +175,14 @@ def main():
             roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                     "traffic": None,
-                    "kernel": "k_pcg_spmv<float> (FP32 CSR SpMV + (p,Ap))",
+                    "kernel": f"k_pcg_spmv<float,{tune['chosen'][1]}> (FP32 SpMV + (p,Ap))",
                     "bytes_per_launch": b32, "avg_us": round(t32 * 1e6, 2),
                     "launches": l32}
         if l64 > 0:
             t64 = ms64 / l64 * 1e-3
             b64 = spmv_bytes(n, nnz, 8, 8)
-            spmv64 = {"achieved_GBps": round(b64 / t64 / 1e9, 1),
+            spmv64 = {"layout": tune["chosen"][0],
+                      "achieved_GBps": round(b64 / t64 / 1e9, 1),
                       "frac": round(b64 / t64 / 1e9 / HBM_PEAK_GBS, 4),
                       "avg_us": round(t64 * 1e6, 2), "bytes_per_launch": b64,
                       "launches": l64}
@@ -220,6 +226,7 @@ def main():
         "mixed_vs_fp64": round(sec_64 / sec_mp, 3),
         "mixed_detail": {"refinements": outer, "fp32_its": inner},
         "assemble_s": round(t_asm, 4),
+        "spmv_layout_us": {k: v for k, v in tune.items() if k != "chosen"},
         "roofline": roof,
         "spmv_fp64": spmv64,
         "cpu_baseline": cpu,

@@ -149,6 +149,29 @@ int phg_hip_pcg_mixed(const phg_hip_mat *A, const phg_hip_vec *b,
                       double atol, double inner_rtol, int32_t *inner_its,
                       double *residual);
 
+/*--------------------------------------------------- SpMV format autotuner */
+
+/* No reference counterpart (PHG has one CSR kernel, matvec.c:887).  The
+ * solvers can run their SpMV on three layouts of the eliminated matrix:
+ *   CSR_BLOCK  work-group CSR stream: a row block's entries are loaded
+ *              coalesced into LDS as products, one work-item sums a row;
+ *   CSR_WAVE   the same per wavefront, with wavefront-private LDS and no
+ *              work-group barriers (default);
+ *   SELL       SELL-64: slices of 64 rows sorted by length inside windows of
+ *              4096 rows, stored column-major, rows renumbered symmetrically.
+ * phg_hip_mat_tune times each layout on the matrix (HIP events, `reps`
+ * launches) in FP64 and FP32 and keeps the fastest for phg_hip_pcg and
+ * phg_hip_pcg_mixed respectively. */
+#define PHG_HIP_FMT_CSR_BLOCK 0
+#define PHG_HIP_FMT_CSR_WAVE  1
+#define PHG_HIP_FMT_SELL      2
+int phg_hip_mat_tune(phg_hip_mat *A, int32_t reps);
+int phg_hip_mat_tune_time(const phg_hip_mat *A, int32_t fmt, int32_t fp32,
+                          double *us);
+int phg_hip_mat_set_format(phg_hip_mat *A, int32_t fmt64, int32_t fmt32);
+int phg_hip_mat_get_format(const phg_hip_mat *A, int32_t *fmt64,
+                           int32_t *fmt32);
+
 /*------------------------------------------------------------- profiling */
 
 /* Per-kernel timing, the role of perf.c's counters (perf.c:490

@@ -120,8 +120,16 @@ _sig(hip_lib, "phg_hip_pcg", C.c_int,
      [_vp, _vp, _vp, _i32, _dbl, _dbl, _dbl, _P(_dbl)])
 _sig(hip_lib, "phg_hip_pcg_mixed", C.c_int,
      [_vp, _vp, _vp, _i32, _dbl, _dbl, _dbl, _dbl, _P(_i32), _P(_dbl)])
+_sig(hip_lib, "phg_hip_mat_tune", C.c_int, [_vp, _i32])
+_sig(hip_lib, "phg_hip_mat_tune_time", C.c_int, [_vp, _i32, _i32, _P(_dbl)])
+_sig(hip_lib, "phg_hip_mat_set_format", C.c_int, [_vp, _i32, _i32])
+_sig(hip_lib, "phg_hip_mat_get_format", C.c_int, [_vp, _P(_i32), _P(_i32)])
 _sig(hip_lib, "phg_hip_profile", C.c_int, [_i32])
 _sig(hip_lib, "phg_hip_profile_read", C.c_int, [_i32, _P(_i64), _P(_dbl)])
+
+# SpMV layouts of the solvers (include/phg_hip.h)
+FMT_CSR_BLOCK, FMT_CSR_WAVE, FMT_SELL = 0, 1, 2
+FMT_NAMES = {0: "csr_block", 1: "csr_wave", 2: "sell64"}
 
 # kernel classes of phg_hip_profile_read (include/phg_hip.h)
 K_PCG_P64, K_SPMV64, K_UPDATE64, K_PCG_P32, K_SPMV32, K_UPDATE32 = range(6)
@@ -336,6 +344,31 @@ class Mat:
         v = Vec(self.n)
         _check(hip_lib.phg_hip_mat_rhs(self._h, v._h), "mat_rhs")
         return v
+
+    def tune(self, reps: int = 10) -> dict:
+        """Times the SpMV layouts and keeps the fastest per precision.
+        Returns {layout: (fp64_us, fp32_us)} and the choice."""
+        _check(hip_lib.phg_hip_mat_tune(self._h, reps), "mat_tune")
+        out = {}
+        for f, name in FMT_NAMES.items():
+            t = []
+            for fp32 in (0, 1):
+                us = _dbl()
+                _check(hip_lib.phg_hip_mat_tune_time(self._h, f, fp32, C.byref(us)),
+                       "mat_tune_time")
+                t.append(us.value)
+            out[name] = tuple(t)
+        out["chosen"] = self.format()
+        return out
+
+    def format(self):
+        a, b = _i32(), _i32()
+        _check(hip_lib.phg_hip_mat_get_format(self._h, C.byref(a), C.byref(b)),
+               "mat_get_format")
+        return FMT_NAMES[a.value], FMT_NAMES[b.value]
+
+    def set_format(self, fmt64: int, fmt32: int) -> None:
+        _check(hip_lib.phg_hip_mat_set_format(self._h, fmt64, fmt32), "mat_set_format")
 
     def matvec(self, x: Vec, y: Vec, alpha=1.0, beta=0.0) -> None:
         _check(hip_lib.phg_hip_matvec(alpha, self._h, x._h, beta, y._h), "matvec")

@@ -1209,6 +1209,7 @@ extern "C" int phg_hip_mat_eliminate(phg_hip_mat *A)
     phg::dfree(A->d_vals);
     phg::dfree(A->d_rhs);
     phg::dfree(A->d_diag);
+    phg::free_sell(A);
     phg::dfree(A->d_vals32);
     phg::dfree(A->d_diag32);
     A->d_vals32 = nullptr;
@@ -1281,9 +1282,11 @@ extern "C" int phg_hip_mat_destroy(phg_hip_mat *A)
     phg::dfree(A->d_rhs);
     phg::dfree(A->d_bdry_row);
     phg::dfree(A->d_diag);
+    phg::dfree(A->d_rblk);
+    phg::dfree(A->d_wblk);
     phg::dfree(A->d_vals32);
     phg::dfree(A->d_diag32);
-    phg::dfree(A->d_rblk);
+    phg::free_sell(A);
     delete A;
     return PHG_HIP_OK;
 }

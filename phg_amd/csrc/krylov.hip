@@ -31,6 +31,7 @@
 #include <stdio.h>
 #include <string.h>
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 namespace {
@@ -206,6 +207,199 @@ __device__ inline void stream_spmv(const int32_t *__restrict__ rp,
     }
 }
 
+// Matrix views for the SpMV-shaped kernels.  CsrView streams PHG's CSR over
+// row blocks (vectors in the original numbering); SellView walks the SELL-64
+// copy (vectors in the solver's permuted numbering).
+struct CsrView {
+    const int32_t *rp, *ci, *rblk;
+    int64_t nrblk;
+};
+
+struct SellView {
+    const int64_t *sptr;
+    const int32_t *swidth, *scols;
+    int64_t nslice, n;
+};
+
+template <typename T, typename Epi>
+__device__ inline void spmv_apply(const CsrView &m, const T *va, const T *x,
+                                  Epi epi)
+{
+    stream_spmv<T>(m.rp, m.ci, va, x, m.rblk, m.nrblk, epi);
+}
+
+// Wavefront-level CSR stream: each wavefront owns a private LDS slice and
+// takes row blocks of at most WNNZ entries and 64 rows.  All loads of a block
+// (8 entries per lane, coalesced, plus the lane's row bounds) are issued
+// together, the products go to the wavefront's LDS slice, and lane l sums row
+// r0 + l in storage order.  No work-group barrier is involved, so the four
+// wavefronts of a work-group progress independently.
+constexpr int WNNZ = 512;       // entries per wavefront row block (8 per lane)
+constexpr int WROWS = 64;
+
+struct CsrWaveView {
+    const int32_t *rp, *ci, *wblk;
+    int64_t nwblk;
+};
+
+template <typename T, typename Epi>
+__device__ inline void spmv_apply(const CsrWaveView &m,
+                                  const T *__restrict__ va,
+                                  const T *__restrict__ x, Epi epi)
+{
+#pragma clang fp contract(off)
+    __shared__ T prod_all[BS / 64][WNNZ];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    T *prod = prod_all[wv];
+    const int32_t *__restrict__ rp = m.rp;
+    const int32_t *__restrict__ ci = m.ci;
+    int64_t g0, g1;
+    row_range(m.nwblk, g0, g1);
+    for (int64_t b = g0 + wv; b < g1; b += BS / 64) {
+        const int32_t r0 = m.wblk[b], r1 = m.wblk[b + 1];
+        const int32_t j0 = rp[r0], j1 = rp[r1];
+        const int nrows = r1 - r0;
+        if (j1 - j0 <= WNNZ) {
+            int32_t c[WNNZ / 64];
+            T v[WNNZ / 64];
+            int32_t rs = 0, re = 0;
+            if (lane < nrows) {
+                rs = rp[r0 + lane];
+                re = rp[r0 + lane + 1];
+            }
+#pragma unroll
+            for (int k = 0; k < WNNZ / 64; k++) {
+                const int32_t j = j0 + lane + 64 * k;
+                c[k] = j < j1 ? ci[j] : 0;
+                v[k] = j < j1 ? va[j] : (T)0;
+            }
+#pragma unroll
+            for (int k = 0; k < WNNZ / 64; k++) {
+                const int32_t j = j0 + lane + 64 * k;
+                if (j < j1)
+                    prod[lane + 64 * k] = v[k] * x[c[k]];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (lane < nrows) {
+                T sum = 0;
+                for (int32_t j = rs - j0; j < re - j0; j++)
+                    sum += prod[j];
+                epi(r0 + lane, sum);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        else {
+            // a row longer than the stage: lanes stride over it
+            T sum = 0;
+            for (int32_t j = j0 + lane; j < j1; j += 64)
+                sum += va[j] * x[ci[j]];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1)
+                sum += __shfl_xor(sum, o, 64);
+            if (lane == 0)
+                epi(r0, sum);
+        }
+    }
+}
+
+// SELL-64: one wavefront per slice, one row per lane; entry k of the lane's
+// row sits at sptr[s] + 64 k + lane, so each load of the wavefront reads 64
+// consecutive words.  The row sum runs over k in CSR storage order without
+// contraction, which reproduces matvec.c:920-928 bit for bit in FP64.
+template <typename T, typename Epi>
+__device__ inline void spmv_apply(const SellView &m, const T *__restrict__ va,
+                                  const T *__restrict__ x, Epi epi)
+{
+#pragma clang fp contract(off)
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int64_t g0, g1;
+    row_range((m.nslice + BS / 64 - 1) / (BS / 64), g0, g1);
+    for (int64_t gi = g0; gi < g1; gi++) {
+        const int64_t sl = gi * (BS / 64) + wv;
+        if (sl >= m.nslice)
+            break;
+        const int64_t base = m.sptr[sl] + lane;
+        const int w = m.swidth[sl];
+        const int32_t *__restrict__ cp = m.scols + base;
+        const T *__restrict__ vp = va + base;
+        T acc = 0;
+        int k = 0;
+        for (; k + 4 <= w; k += 4) {
+            const int32_t c0 = cp[0], c1 = cp[64], c2 = cp[128], c3 = cp[192];
+            const T v0 = vp[0], v1 = vp[64], v2 = vp[128], v3 = vp[192];
+            const T x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+            acc += v0 * x0;
+            acc += v1 * x1;
+            acc += v2 * x2;
+            acc += v3 * x3;
+            cp += 256;
+            vp += 256;
+        }
+        for (; k < w; k++) {
+            acc += vp[0] * x[cp[0]];
+            cp += 64;
+            vp += 64;
+        }
+        const int64_t i = sl * 64 + lane;
+        if (i < m.n)
+            epi((int32_t)i, acc);
+    }
+}
+
+// SELL construction: entries of permuted row i (original row perm[i]) with
+// column indices renumbered through iperm; padding repeats the row's own
+// index with value 0
+__global__ void k_sell_fill(int64_t n, int64_t nslice, const int32_t *rp,
+                            const int32_t *ci, const double *va,
+                            const double *diag, const int32_t *perm,
+                            const int32_t *iperm, const int64_t *sptr,
+                            const int32_t *swidth, int32_t *scols,
+                            double *svals, double *sdiag)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nslice * 64)
+        return;
+    const int64_t sl = i / 64, lane = i % 64;
+    const int64_t base = sptr[sl] + lane;
+    const int w = swidth[sl];
+    int k = 0;
+    int32_t self = 0;
+    if (i < n) {
+        const int32_t r = perm[i];
+        const int32_t j0 = rp[r], len = rp[r + 1] - j0;
+        for (; k < len; k++) {
+            scols[base + 64 * (int64_t)k] = iperm[ci[j0 + k]];
+            svals[base + 64 * (int64_t)k] = va[j0 + k];
+        }
+        self = (int32_t)i;
+        sdiag[i] = diag[r];
+    }
+    for (; k < w; k++) {
+        scols[base + 64 * (int64_t)k] = self;
+        svals[base + 64 * (int64_t)k] = 0.0;
+    }
+}
+
+__global__ void k_gather(int64_t n, const int32_t *perm, const double *src,
+                         double *dst)
+{
+    for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * BS)
+        dst[i] = src[perm[i]];
+}
+
+__global__ void k_scatter(int64_t n, const int32_t *perm, const double *src,
+                          double *dst)
+{
+    for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * BS)
+        dst[perm[i]] = src[i];
+}
+
 //------------------------------------------------------------- plain SpMV
 __global__ void __launch_bounds__(BS) k_spmv64(const int32_t *rp,
                                                const int32_t *ci,
@@ -272,9 +466,9 @@ __global__ void __launch_bounds__(BS) k_pcg_init_norms(int64_t n,
 
 // ... then r = b - A x when |x|_1/n is not zero (else r = b; x = 0 when b
 // is zero) and the partials (r,r), (r,z), #zero diagonals for iteration 0.
+template <typename View>
 __global__ void __launch_bounds__(BS) k_pcg_init_res(
-    int64_t n, const int32_t *rp, const int32_t *ci, const double *va,
-    const int32_t *rblk, int64_t nrblk, const double *diag, const double *b,
+    int64_t n, View mv, const double *va, const double *diag, const double *b,
     double *x, double *r, const double *part_in, int np_in, double *part_out,
     State *st0)
 {
@@ -298,8 +492,8 @@ __global__ void __launch_bounds__(BS) k_pcg_init_res(
             v[1] += ri * (ri / d);
     };
     if (xnz && !bzero) {
-        stream_spmv<double>(rp, ci, va, x, rblk, nrblk,
-                            [&](int32_t row, double s) { acc(row, -s + b[row]); });
+        spmv_apply<double>(mv, va, x,
+                           [&](int32_t row, double s) { acc(row, -s + b[row]); });
     }
     else {
         for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
@@ -392,15 +586,14 @@ __global__ void __launch_bounds__(BS) k_pcg_p(int64_t n, const T *r,
     }
 }
 
-template <typename T>
+template <typename T, typename View>
 __global__ void __launch_bounds__(BS) k_pcg_spmv(
-    const int32_t *rp, const int32_t *ci, const T *va, const int32_t *rblk,
-    int64_t nrblk, const T *p, T *Ap, double *part_out, const State *st)
+    View mv, const T *va, const T *p, T *Ap, double *part_out, const State *st)
 {
     if (st->done)
         return;
     double v[1] = {0.0};
-    stream_spmv<T>(rp, ci, va, p, rblk, nrblk, [&](int32_t row, T s) {
+    spmv_apply<T>(mv, va, p, [&](int32_t row, T s) {
         Ap[row] = s;
         v[0] += (double)p[row] * (double)s;
     });
@@ -440,13 +633,13 @@ __global__ void __launch_bounds__(BS) k_pcg_update(int64_t n, T *x,
 
 //----------------------------------------------- mixed-precision refinement
 // r = b - A x in FP64; partials of |r|^2 and |b|^2
+template <typename View>
 __global__ void __launch_bounds__(BS) k_ir_residual(
-    const int32_t *rp, const int32_t *ci, const double *va,
-    const int32_t *rblk, int64_t nrblk, const double *b, const double *x,
-    double *r, double *part_out)
+    View mv, const double *va, const double *b, const double *x, double *r,
+    double *part_out)
 {
     double v[2] = {0.0, 0.0};
-    stream_spmv<double>(rp, ci, va, x, rblk, nrblk, [&](int32_t row, double s) {
+    spmv_apply<double>(mv, va, x, [&](int32_t row, double s) {
         double bi = b[row], ri = bi - s;
         r[row] = ri;
         v[0] += ri * ri;
@@ -507,15 +700,24 @@ __global__ void k_to_f32(int64_t n, const double *a, float *b)
 //------------------------------------------------------------- host side
 // partial-sum segments (each MAX_GRID * 4 doubles)
 enum { SEG_NORMS = 0, SEG_RES, SEG_S, SEG_U, SEG_IR, SEG_N };
+// storage formats of the solver SpMV
+enum { FMT_CSR_BLOCK = 0, FMT_CSR_WAVE = 1, FMT_SELL = 2, FMT_N };
 
 struct Workspace {
     int64_t n = 0;
     double *r = nullptr, *p = nullptr, *Ap = nullptr, *partials = nullptr;
+    double *xp = nullptr, *bp = nullptr;  // x and b in the SELL numbering
     double *scal = nullptr;               // small device scalars
     float *r32 = nullptr, *p32 = nullptr, *Ap32 = nullptr, *x32 = nullptr;
-    State *st = nullptr;                  // [2] ping-pong copies
+    State *st = nullptr;                  // [2] ping-pong copies + 1 spare
     double *seg(int k) { return partials + (size_t)k * 4 * MAX_GRID; }
 };
+
+int cap_grid()
+{
+    int cus = phg::ctx()->num_cus > 0 ? phg::ctx()->num_cus : 256;
+    return std::min(cus * 8, MAX_GRID);
+}
 
 int grid_for(int64_t n)
 {
@@ -525,30 +727,75 @@ int grid_for(int64_t n)
     return (int)std::max<int64_t>(1, std::min(g, cap));
 }
 
-int grid_rows(const phg_hip_mat *A)
+// views and launch grids of the three formats
+CsrView view_of(const phg_hip_mat *A, CsrView *)
 {
-    // each work-group covers a contiguous range of row blocks
-    int cus = phg::ctx()->num_cus > 0 ? phg::ctx()->num_cus : 256;
-    int64_t cap = std::min<int64_t>((int64_t)cus * 8, MAX_GRID);
-    return (int)std::max<int64_t>(1, std::min<int64_t>(A->nrblk, cap));
+    return CsrView{A->d_rowptr, A->d_cols, A->d_rblk, A->nrblk};
 }
+CsrWaveView view_of(const phg_hip_mat *A, CsrWaveView *)
+{
+    return CsrWaveView{A->d_rowptr, A->d_cols, A->d_wblk, A->nwblk};
+}
+SellView view_of(const phg_hip_mat *A, SellView *)
+{
+    return SellView{A->d_sptr, A->d_swidth, A->d_scols, A->nslice, A->n};
+}
+int64_t units_of(const phg_hip_mat *A, CsrView *) { return A->nrblk; }
+int64_t units_of(const phg_hip_mat *A, CsrWaveView *)
+{
+    return (A->nwblk + BS / 64 - 1) / (BS / 64) * (BS / 64);
+}
+int64_t units_of(const phg_hip_mat *A, SellView *)
+{
+    return (A->nslice + BS / 64 - 1) / (BS / 64);
+}
+
+// For the wavefront stream the work-group range is taken over wavefront
+// blocks (waves interleave inside it), so the grid is sized by groups of 4.
+template <typename View>
+int grid_spmv(const phg_hip_mat *A)
+{
+    int64_t u = units_of(A, (View *)nullptr);
+    if (std::is_same<View, CsrWaveView>::value)
+        u /= BS / 64;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(u, cap_grid()));
+}
+
+template <typename View> struct FmtData;
+template <> struct FmtData<CsrView> {
+    static const double *v64(const phg_hip_mat *A) { return A->d_vals; }
+    static const float *v32(const phg_hip_mat *A) { return A->d_vals32; }
+    static const double *d64(const phg_hip_mat *A) { return A->d_diag; }
+    static const float *d32(const phg_hip_mat *A) { return A->d_diag32; }
+    static constexpr bool permuted = false;
+};
+template <> struct FmtData<CsrWaveView> : FmtData<CsrView> {};
+template <> struct FmtData<SellView> {
+    static const double *v64(const phg_hip_mat *A) { return A->d_svals; }
+    static const float *v32(const phg_hip_mat *A) { return A->d_svals32; }
+    static const double *d64(const phg_hip_mat *A) { return A->d_sdiag; }
+    static const float *d32(const phg_hip_mat *A) { return A->d_sdiag32; }
+    static constexpr bool permuted = true;
+};
 
 Workspace *get_ws(const phg_hip_mat *A, int need32)
 {
     static Workspace ws;     // one solve at a time per process (one stream)
     if (ws.n != A->n) {
         phg::dfree(ws.r); phg::dfree(ws.p); phg::dfree(ws.Ap);
+        phg::dfree(ws.xp); phg::dfree(ws.bp);
         phg::dfree(ws.partials); phg::dfree(ws.st); phg::dfree(ws.scal);
         phg::dfree(ws.r32); phg::dfree(ws.p32); phg::dfree(ws.Ap32);
         phg::dfree(ws.x32);
         ws = Workspace();
         size_t n8 = (size_t)A->n * sizeof(double);
         if (phg::dmalloc((void **)&ws.r, n8) || phg::dmalloc((void **)&ws.p, n8) ||
-            phg::dmalloc((void **)&ws.Ap, n8) ||
+            phg::dmalloc((void **)&ws.Ap, n8) || phg::dmalloc((void **)&ws.xp, n8) ||
+            phg::dmalloc((void **)&ws.bp, n8) ||
             phg::dmalloc((void **)&ws.partials,
                          (size_t)SEG_N * 4 * MAX_GRID * sizeof(double)) ||
             phg::dmalloc((void **)&ws.scal, 16 * sizeof(double)) ||
-            phg::dmalloc((void **)&ws.st, 2 * sizeof(State)))
+            phg::dmalloc((void **)&ws.st, 3 * sizeof(State)))
             return nullptr;
         ws.n = A->n;
     }
@@ -560,7 +807,6 @@ Workspace *get_ws(const phg_hip_mat *A, int need32)
     }
     return &ws;
 }
-
 
 // Optional per-kernel timing with HIP events recorded on the library stream
 // around each launch of the PCG kernels (phg_hip_profile).
@@ -605,14 +851,14 @@ void prof_collect()
 
 // one PCG iteration (3 kernels) with launch index k on the given precision;
 // part_in holds the residual partials of the previous iteration (or of the
-// prologue when k == 0)
-template <typename T>
+// prologue, np_first entries, when k == 0)
+template <typename T, typename View>
 void launch_iteration(const phg_hip_mat *A, const T *va, const T *diag,
                       T *x, T *r, T *p, T *Ap, Workspace *ws, State *st,
                       int64_t k, int np_first, hipStream_t s)
 {
     int64_t n = A->n;
-    int g = grid_for(n), gr = grid_rows(A);
+    int g = grid_for(n), gs = grid_spmv<View>(A);
     const int np_in = k == 0 ? np_first : g;
     State *sin = st + (k & 1), *sout = st + ((k + 1) & 1);
     double *pres = ws->seg(SEG_RES), *ps = ws->seg(SEG_S);
@@ -623,14 +869,13 @@ void launch_iteration(const phg_hip_mat *A, const T *va, const T *diag,
                        diag, p, (const double *)pres, np_in,
                        (const State *)sin, sout);
     if (prof) { prof_mark(base + 0, s); prof_mark(base + 1, s); }
-    hipLaunchKernelGGL(k_pcg_spmv<T>, dim3(gr), dim3(BS), 0, s,
-                       (const int32_t *)A->d_rowptr, (const int32_t *)A->d_cols,
-                       va, (const int32_t *)A->d_rblk, A->nrblk, (const T *)p,
-                       Ap, ps, (const State *)sout);
+    hipLaunchKernelGGL((k_pcg_spmv<T, View>), dim3(gs), dim3(BS), 0, s,
+                       view_of(A, (View *)nullptr), va, (const T *)p, Ap, ps,
+                       (const State *)sout);
     if (prof) { prof_mark(base + 1, s); prof_mark(base + 2, s); }
     hipLaunchKernelGGL(k_pcg_update<T>, dim3(g), dim3(BS), 0, s, n, x,
                        (const T *)p, r, (const T *)Ap, diag,
-                       (const double *)ps, gr, pres, sout);
+                       (const double *)ps, gs, pres, sout);
     if (prof) prof_mark(base + 2, s);
 }
 
@@ -661,10 +906,10 @@ int check_solve_args(const phg_hip_mat *A, const phg_hip_vec *b,
 
 // Runs PCG iterations (launch indices from 0) until the state copy reports
 // completion; *out receives the final state.
-template <typename T>
-int run_pcg_loop(const phg_hip_mat *A, const T *va, const T *diag,
-                 T *x, T *r, T *p, T *Ap, Workspace *ws, int maxit,
-                 int np_first, int first_chunk, State *out)
+template <typename T, typename View>
+int run_pcg_loop(const phg_hip_mat *A, const T *va, const T *diag, T *x, T *r,
+                 T *p, T *Ap, Workspace *ws, int maxit, int np_first,
+                 int first_chunk, State *out)
 {
     hipStream_t s = phg::ctx()->stream;
     State *hm = (State *)phg::ctx()->host_mailbox;
@@ -673,8 +918,8 @@ int run_pcg_loop(const phg_hip_mat *A, const T *va, const T *diag,
     int rc;
     for (;;) {
         for (int i = 0; i < chunk; i++, k++)
-            launch_iteration<T>(A, va, diag, x, r, p, Ap, ws, ws->st, k,
-                                np_first, s);
+            launch_iteration<T, View>(A, va, diag, x, r, p, Ap, ws, ws->st, k,
+                                      np_first, s);
         PHG_HIP_CHECK(hipGetLastError());
         if ((rc = read_state(ws->st + (k & 1), hm, s)))
             return rc;
@@ -693,61 +938,346 @@ int run_pcg_loop(const phg_hip_mat *A, const T *va, const T *diag,
     return PHG_HIP_OK;
 }
 
+int ensure_format(phg_hip_mat *A, int fmt, int need32);
+
+// FP64 PCG in the numbering of the format
+template <typename View>
+int pcg64_impl(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x,
+               int32_t maxit, double rtol, double btol, double atol,
+               double *residual)
+{
+    using F = FmtData<View>;
+    hipStream_t s = phg::ctx()->stream;
+    Workspace *ws = get_ws(A, 0);
+    if (ws == nullptr)
+        return phg::set_error(PHG_HIP_E_NOMEM, "PCG workspace allocation failed");
+    State h;
+    memset(&h, 0, sizeof(h));
+    h.rtol = rtol;
+    h.btol = btol;
+    h.atol = atol;
+    h.maxit = maxit;
+    PHG_HIP_CHECK(hipMemcpyAsync(ws->st, &h, sizeof(State),
+                                 hipMemcpyHostToDevice, s));
+    const int64_t n = A->n;
+    int g = grid_for(n), gs = grid_spmv<View>(A);
+    const double *bb = b->d;
+    double *xx = x->d;
+    if (F::permuted) {
+        hipLaunchKernelGGL(k_gather, dim3(g), dim3(BS), 0, s, n,
+                           (const int32_t *)A->d_perm, (const double *)b->d,
+                           ws->bp);
+        hipLaunchKernelGGL(k_gather, dim3(g), dim3(BS), 0, s, n,
+                           (const int32_t *)A->d_perm, (const double *)x->d,
+                           ws->xp);
+        bb = ws->bp;
+        xx = ws->xp;
+    }
+    hipLaunchKernelGGL(k_pcg_init_norms, dim3(g), dim3(BS), 0, s, n, bb,
+                       (const double *)xx, ws->seg(SEG_NORMS));
+    hipLaunchKernelGGL((k_pcg_init_res<View>), dim3(gs), dim3(BS), 0, s, n,
+                       view_of(A, (View *)nullptr), F::v64(A), F::d64(A), bb,
+                       xx, ws->r, (const double *)ws->seg(SEG_NORMS), g,
+                       ws->seg(SEG_RES), ws->st);
+    PHG_HIP_CHECK(hipGetLastError());
+    State fin;
+    int rc;
+    // the prologue ran on the SpMV grid: its partials count is gs
+    if ((rc = run_pcg_loop<double, View>(A, F::v64(A), F::d64(A), xx, ws->r,
+                                         ws->p, ws->Ap, ws, maxit, gs, 8, &fin)))
+        return rc;
+    if (F::permuted)
+        hipLaunchKernelGGL(k_scatter, dim3(g), dim3(BS), 0, s, n,
+                           (const int32_t *)A->d_perm, (const double *)ws->xp,
+                           x->d);
+    PHG_HIP_CHECK(hipGetLastError());
+    PHG_HIP_CHECK(hipStreamSynchronize(s));
+    if (residual)
+        *residual = fin.res;
+    if (fin.done == ZERO_DIAG)
+        return phg::set_error(PHG_HIP_E_ZERO_DIAG,
+                              "zero diagonal in Jacobi preconditioner");
+    return fin.nits;
+}
+
+// FP32 PCG under FP64 refinement in the numbering of the format
+template <typename View>
+int mixed_impl(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x,
+               int32_t maxit, double rtol, double btol, double atol,
+               double inner_rtol, int32_t *inner_its, double *residual)
+{
+    using F = FmtData<View>;
+    hipStream_t s = phg::ctx()->stream;
+    Workspace *ws = get_ws(A, 1);
+    if (ws == nullptr)
+        return phg::set_error(PHG_HIP_E_NOMEM, "workspace allocation failed");
+    const int64_t n = A->n;
+    int g = grid_for(n), gs = grid_spmv<View>(A);
+    double *hm = (double *)phg::ctx()->host_mailbox;
+    double tol = 0.0, res = 0.0;
+    int total_inner = 0, outer = 0, rc;
+    const double *bb = b->d;
+    double *xx = x->d;
+    if (F::permuted) {
+        hipLaunchKernelGGL(k_gather, dim3(g), dim3(BS), 0, s, n,
+                           (const int32_t *)A->d_perm, (const double *)b->d,
+                           ws->bp);
+        hipLaunchKernelGGL(k_gather, dim3(g), dim3(BS), 0, s, n,
+                           (const int32_t *)A->d_perm, (const double *)x->d,
+                           ws->xp);
+        bb = ws->bp;
+        xx = ws->xp;
+    }
+    for (;;) {
+        hipLaunchKernelGGL((k_ir_residual<View>), dim3(gs), dim3(BS), 0, s,
+                           view_of(A, (View *)nullptr), F::v64(A), bb,
+                           (const double *)xx, ws->r, ws->seg(SEG_IR));
+        hipLaunchKernelGGL(k_ir_norms, dim3(1), dim3(BS), 0, s,
+                           (const double *)ws->seg(SEG_IR), gs, ws->scal);
+        PHG_HIP_CHECK(hipGetLastError());
+        PHG_HIP_CHECK(hipMemcpyAsync(hm, ws->scal, 2 * sizeof(double),
+                                     hipMemcpyDeviceToHost, s));
+        PHG_HIP_CHECK(hipStreamSynchronize(s));
+        res = hm[0];
+        const double rb = hm[1];
+        if (outer == 0) {
+            // get_tol on the FP64 residual (solver-phg.c:104)
+            tol = res * rtol;
+            if (tol < rb * btol)
+                tol = rb * btol;
+            if (tol < atol)
+                tol = atol;
+        }
+        if (res <= tol || rb == 0.0 || outer >= maxit)
+            break;
+        State h;
+        memset(&h, 0, sizeof(h));
+        h.rtol = inner_rtol;
+        h.rb = 1.0;
+        h.maxit = 100000;     // each FP32 solve stops on inner_rtol
+        PHG_HIP_CHECK(hipMemcpyAsync(ws->st, &h, sizeof(State),
+                                     hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_ir_inner_init, dim3(g), dim3(BS), 0, s, n,
+                           (const double *)ws->r, 1.0 / res, F::d32(A),
+                           ws->r32, ws->x32, ws->seg(SEG_RES));
+        PHG_HIP_CHECK(hipGetLastError());
+        State fin;
+        if ((rc = run_pcg_loop<float, View>(A, F::v32(A), F::d32(A), ws->x32,
+                                            ws->r32, ws->p32, ws->Ap32, ws,
+                                            h.maxit, g, 16, &fin)))
+            return rc;
+        if (fin.done == ZERO_DIAG)
+            return phg::set_error(PHG_HIP_E_ZERO_DIAG,
+                                  "zero diagonal in Jacobi preconditioner");
+        total_inner += fin.nits;
+        hipLaunchKernelGGL(k_ir_correct, dim3(g), dim3(BS), 0, s, n, xx,
+                           (const float *)ws->x32, res);
+        PHG_HIP_CHECK(hipGetLastError());
+        outer++;
+    }
+    if (F::permuted)
+        hipLaunchKernelGGL(k_scatter, dim3(g), dim3(BS), 0, s, n,
+                           (const int32_t *)A->d_perm, (const double *)ws->xp,
+                           x->d);
+    PHG_HIP_CHECK(hipGetLastError());
+    PHG_HIP_CHECK(hipStreamSynchronize(s));
+    if (inner_its)
+        *inner_its = total_inner;
+    if (residual)
+        *residual = res;
+    return outer;
+}
+
+// time `reps` launches of the PCG SpMV kernel of one format and precision
+template <typename T, typename View>
+int time_spmv(phg_hip_mat *A, const T *va, int reps, double *us)
+{
+    hipStream_t s = phg::ctx()->stream;
+    Workspace *ws = get_ws(A, 1);
+    if (ws == nullptr)
+        return phg::set_error(PHG_HIP_E_NOMEM, "workspace allocation failed");
+    State h;
+    memset(&h, 0, sizeof(h));
+    PHG_HIP_CHECK(hipMemcpyAsync(ws->st + 2, &h, sizeof(State),
+                                 hipMemcpyHostToDevice, s));
+    T *p = (T *)(sizeof(T) == 8 ? (void *)ws->p : (void *)ws->p32);
+    T *Ap = (T *)(sizeof(T) == 8 ? (void *)ws->Ap : (void *)ws->Ap32);
+    PHG_HIP_CHECK(hipMemsetAsync(p, 0, (size_t)A->n * sizeof(T), s));
+    int gs = grid_spmv<View>(A);
+    hipEvent_t e0, e1;
+    PHG_HIP_CHECK(hipEventCreate(&e0));
+    PHG_HIP_CHECK(hipEventCreate(&e1));
+    for (int i = 0; i < reps + 2; i++) {
+        if (i == 2)
+            PHG_HIP_CHECK(hipEventRecord(e0, s));
+        hipLaunchKernelGGL((k_pcg_spmv<T, View>), dim3(gs), dim3(BS), 0, s,
+                           view_of(A, (View *)nullptr), va, (const T *)p, Ap,
+                           ws->seg(SEG_S), (const State *)(ws->st + 2));
+    }
+    PHG_HIP_CHECK(hipEventRecord(e1, s));
+    PHG_HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    PHG_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *us = 1e3 * ms / reps;
+    return PHG_HIP_OK;
+}
+
 }  // namespace
 
 namespace phg {
 
 int build_row_blocks(phg_hip_mat *A)
 {
-    // greedy split into runs of consecutive rows holding at most NNZ_CAP
-    // entries and ROW_CAP rows; a longer row forms a block of its own
-    std::vector<int32_t> rp((size_t)A->n + 1), blk;
+    // greedy split into runs of consecutive rows holding at most cap entries
+    // and rcap rows; a longer row forms a block of its own
+    std::vector<int32_t> rp((size_t)A->n + 1);
     int rc;
     if ((rc = download(rp.data(), A->d_rowptr, rp.size() * sizeof(int32_t))))
         return rc;
-    blk.reserve((size_t)(A->nnz / NNZ_CAP + A->n / ROW_CAP + 2));
-    blk.push_back(0);
-    int64_t cur_nnz = 0, cur_rows = 0;
-    for (int64_t r = 0; r < A->n; r++) {
-        int64_t len = rp[(size_t)r + 1] - rp[(size_t)r];
-        if (len > NNZ_CAP) {
-            if (cur_rows > 0)
+    auto split = [&](int64_t cap, int64_t rcap, int32_t **d, int64_t *nb) {
+        std::vector<int32_t> blk;
+        blk.reserve((size_t)(A->nnz / cap + A->n / rcap + 2));
+        blk.push_back(0);
+        int64_t cur_nnz = 0, cur_rows = 0;
+        for (int64_t r = 0; r < A->n; r++) {
+            int64_t len = rp[(size_t)r + 1] - rp[(size_t)r];
+            if (len > cap) {
+                if (cur_rows > 0)
+                    blk.push_back((int32_t)r);
+                blk.push_back((int32_t)(r + 1));
+                cur_nnz = cur_rows = 0;
+                continue;
+            }
+            if (cur_rows > 0 && (cur_nnz + len > cap || cur_rows == rcap)) {
                 blk.push_back((int32_t)r);
-            blk.push_back((int32_t)(r + 1));
-            cur_nnz = cur_rows = 0;
-            continue;
+                cur_nnz = cur_rows = 0;
+            }
+            cur_nnz += len;
+            cur_rows++;
         }
-        if (cur_rows > 0 && (cur_nnz + len > NNZ_CAP || cur_rows == ROW_CAP)) {
-            blk.push_back((int32_t)r);
-            cur_nnz = cur_rows = 0;
-        }
-        cur_nnz += len;
-        cur_rows++;
-    }
-    if (cur_rows > 0)
-        blk.push_back((int32_t)A->n);
-    dfree(A->d_rblk);
-    A->d_rblk = nullptr;
-    A->nrblk = (int64_t)blk.size() - 1;
-    if ((rc = dmalloc((void **)&A->d_rblk, blk.size() * sizeof(int32_t))) ||
-        (rc = upload(A->d_rblk, blk.data(), blk.size() * sizeof(int32_t))))
+        if (cur_rows > 0)
+            blk.push_back((int32_t)A->n);
+        dfree(*d);
+        *d = nullptr;
+        *nb = (int64_t)blk.size() - 1;
+        int e;
+        if ((e = dmalloc((void **)d, blk.size() * sizeof(int32_t))) ||
+            (e = upload(*d, blk.data(), blk.size() * sizeof(int32_t))))
+            return e;
+        return PHG_HIP_OK;
+    };
+    if ((rc = split(NNZ_CAP, ROW_CAP, &A->d_rblk, &A->nrblk)) ||
+        (rc = split(WNNZ, WROWS, &A->d_wblk, &A->nwblk)))
         return rc;
+    return PHG_HIP_OK;
+}
+
+void free_sell(phg_hip_mat *A)
+{
+    dfree(A->d_perm); dfree(A->d_iperm); dfree(A->d_sptr); dfree(A->d_swidth);
+    dfree(A->d_scols); dfree(A->d_svals); dfree(A->d_svals32);
+    dfree(A->d_sdiag); dfree(A->d_sdiag32);
+    A->d_perm = A->d_iperm = A->d_swidth = A->d_scols = nullptr;
+    A->d_sptr = nullptr;
+    A->d_svals = A->d_sdiag = nullptr;
+    A->d_svals32 = A->d_sdiag32 = nullptr;
+    A->nslice = A->sell_nnz = 0;
+}
+
+constexpr int64_t SELL_WINDOW = 4096;   // rows sorted together (64 slices)
+
+int build_sell(phg_hip_mat *A)
+{
+    free_sell(A);
+    const int64_t n = A->n;
+    std::vector<int32_t> rp((size_t)n + 1);
+    int rc;
+    if ((rc = download(rp.data(), A->d_rowptr, rp.size() * sizeof(int32_t))))
+        return rc;
+    // stable sort by decreasing length inside each window: rows of equal
+    // length keep their order, so neighbouring rows stay neighbours
+    std::vector<int32_t> perm((size_t)n), iperm((size_t)n);
+    std::vector<int32_t> order;
+    for (int64_t w0 = 0; w0 < n; w0 += SELL_WINDOW) {
+        int64_t w1 = std::min<int64_t>(w0 + SELL_WINDOW, n);
+        order.resize((size_t)(w1 - w0));
+        for (int64_t i = w0; i < w1; i++)
+            order[(size_t)(i - w0)] = (int32_t)i;
+        std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+            return rp[(size_t)a + 1] - rp[(size_t)a] > rp[(size_t)b + 1] - rp[(size_t)b];
+        });
+        for (int64_t i = w0; i < w1; i++)
+            perm[(size_t)i] = order[(size_t)(i - w0)];
+    }
+    for (int64_t i = 0; i < n; i++)
+        iperm[(size_t)perm[(size_t)i]] = (int32_t)i;
+    const int64_t nslice = (n + 63) / 64;
+    std::vector<int64_t> sptr((size_t)nslice + 1);
+    std::vector<int32_t> width((size_t)nslice);
+    sptr[0] = 0;
+    for (int64_t sl = 0; sl < nslice; sl++) {
+        int32_t w = 0;
+        for (int64_t i = sl * 64; i < std::min<int64_t>(sl * 64 + 64, n); i++) {
+            int32_t r = perm[(size_t)i];
+            w = std::max(w, rp[(size_t)r + 1] - rp[(size_t)r]);
+        }
+        width[(size_t)sl] = w;
+        sptr[(size_t)sl + 1] = sptr[(size_t)sl] + 64 * (int64_t)w;
+    }
+    const int64_t snnz = sptr[(size_t)nslice];
+    A->nslice = nslice;
+    A->sell_nnz = snnz;
+    if ((rc = dmalloc((void **)&A->d_perm, (size_t)n * sizeof(int32_t))) ||
+        (rc = dmalloc((void **)&A->d_iperm, (size_t)n * sizeof(int32_t))) ||
+        (rc = dmalloc((void **)&A->d_sptr, sptr.size() * sizeof(int64_t))) ||
+        (rc = dmalloc((void **)&A->d_swidth, width.size() * sizeof(int32_t))) ||
+        (rc = dmalloc((void **)&A->d_scols, (size_t)(snnz + 1) * sizeof(int32_t))) ||
+        (rc = dmalloc((void **)&A->d_svals, (size_t)(snnz + 1) * sizeof(double))) ||
+        (rc = dmalloc((void **)&A->d_sdiag, (size_t)n * sizeof(double))) ||
+        (rc = upload(A->d_perm, perm.data(), perm.size() * sizeof(int32_t))) ||
+        (rc = upload(A->d_iperm, iperm.data(), iperm.size() * sizeof(int32_t))) ||
+        (rc = upload(A->d_sptr, sptr.data(), sptr.size() * sizeof(int64_t))) ||
+        (rc = upload(A->d_swidth, width.data(), width.size() * sizeof(int32_t)))) {
+        free_sell(A);
+        return rc;
+    }
+    hipStream_t s = ctx()->stream;
+    hipLaunchKernelGGL(k_sell_fill, dim3((unsigned)((nslice * 64 + 255) / 256)),
+                       dim3(256), 0, s, n, nslice, (const int32_t *)A->d_rowptr,
+                       (const int32_t *)A->d_cols, (const double *)A->d_vals,
+                       (const double *)A->d_diag, (const int32_t *)A->d_perm,
+                       (const int32_t *)A->d_iperm, (const int64_t *)A->d_sptr,
+                       (const int32_t *)A->d_swidth, A->d_scols, A->d_svals,
+                       A->d_sdiag);
+    PHG_HIP_CHECK(hipGetLastError());
+    PHG_HIP_CHECK(hipStreamSynchronize(s));
     return PHG_HIP_OK;
 }
 
 int ensure_fp32_copy(phg_hip_mat *A)
 {
-    if (A->d_vals32 != nullptr)
-        return PHG_HIP_OK;
     hipStream_t s = ctx()->stream;
     int rc;
-    if ((rc = dmalloc((void **)&A->d_vals32, (size_t)(A->nnz + 1) * sizeof(float))) ||
-        (rc = dmalloc((void **)&A->d_diag32, (size_t)A->n * sizeof(float))))
-        return rc;
-    hipLaunchKernelGGL(k_to_f32, dim3(grid_for(A->nnz)), dim3(BS), 0, s, A->nnz,
-                       (const double *)A->d_vals, A->d_vals32);
-    hipLaunchKernelGGL(k_to_f32, dim3(grid_for(A->n)), dim3(BS), 0, s, A->n,
-                       (const double *)A->d_diag, A->d_diag32);
+    if (A->d_vals32 == nullptr) {
+        if ((rc = dmalloc((void **)&A->d_vals32, (size_t)(A->nnz + 1) * sizeof(float))) ||
+            (rc = dmalloc((void **)&A->d_diag32, (size_t)A->n * sizeof(float))))
+            return rc;
+        hipLaunchKernelGGL(k_to_f32, dim3(grid_for(A->nnz)), dim3(BS), 0, s,
+                           A->nnz, (const double *)A->d_vals, A->d_vals32);
+        hipLaunchKernelGGL(k_to_f32, dim3(grid_for(A->n)), dim3(BS), 0, s, A->n,
+                           (const double *)A->d_diag, A->d_diag32);
+    }
+    if (A->d_svals != nullptr && A->d_svals32 == nullptr) {
+        if ((rc = dmalloc((void **)&A->d_svals32, (size_t)(A->sell_nnz + 1) * sizeof(float))) ||
+            (rc = dmalloc((void **)&A->d_sdiag32, (size_t)A->n * sizeof(float))))
+            return rc;
+        hipLaunchKernelGGL(k_to_f32, dim3(grid_for(A->sell_nnz)), dim3(BS), 0, s,
+                           A->sell_nnz, (const double *)A->d_svals, A->d_svals32);
+        hipLaunchKernelGGL(k_to_f32, dim3(grid_for(A->n)), dim3(BS), 0, s, A->n,
+                           (const double *)A->d_sdiag, A->d_sdiag32);
+    }
     PHG_HIP_CHECK(hipGetLastError());
     PHG_HIP_CHECK(hipStreamSynchronize(s));
     return PHG_HIP_OK;
@@ -758,7 +1288,7 @@ int spmv64(const phg_hip_mat *A, const double *x, double *y, double alpha,
 {
     if (A->nrblk == 0)
         return PHG_HIP_OK;
-    hipLaunchKernelGGL(k_spmv64, dim3(grid_rows(A)), dim3(BS), 0, s,
+    hipLaunchKernelGGL(k_spmv64, dim3(grid_spmv<CsrView>(A)), dim3(BS), 0, s,
                        (const int32_t *)A->d_rowptr, (const int32_t *)A->d_cols,
                        (const double *)A->d_vals, (const int32_t *)A->d_rblk,
                        A->nrblk, x, y, alpha, beta);
@@ -767,6 +1297,18 @@ int spmv64(const phg_hip_mat *A, const double *x, double *y, double alpha,
 }
 
 }  // namespace phg
+
+namespace {
+int ensure_format(phg_hip_mat *A, int fmt, int need32)
+{
+    int rc;
+    if (fmt == FMT_SELL && A->d_svals == nullptr && (rc = phg::build_sell(A)))
+        return rc;
+    if (need32 && (rc = phg::ensure_fp32_copy(A)))
+        return rc;
+    return PHG_HIP_OK;
+}
+}  // namespace
 
 //================================================================= C ABI
 
@@ -843,49 +1385,24 @@ extern "C" int phg_hip_vec_copy(const phg_hip_vec *src, phg_hip_vec *dst)
     return PHG_HIP_OK;
 }
 
-extern "C" int phg_hip_pcg(const phg_hip_mat *A, const phg_hip_vec *b,
+extern "C" int phg_hip_pcg(const phg_hip_mat *A_, const phg_hip_vec *b,
                            phg_hip_vec *x, int32_t maxit, double rtol,
                            double btol, double atol, double *residual)
 {
     int rc;
-    if ((rc = check_solve_args(A, b, x)))
+    if ((rc = check_solve_args(A_, b, x)))
         return rc;
-    hipStream_t s = phg::ctx()->stream;
-    Workspace *ws = get_ws(A, 0);
-    if (ws == nullptr)
-        return phg::set_error(PHG_HIP_E_NOMEM, "PCG workspace allocation failed");
-    State h;
-    memset(&h, 0, sizeof(h));
-    h.rtol = rtol;
-    h.btol = btol;
-    h.atol = atol;
-    h.maxit = maxit;
-    PHG_HIP_CHECK(hipMemcpyAsync(ws->st, &h, sizeof(State),
-                                 hipMemcpyHostToDevice, s));
-    const int64_t n = A->n;
-    int g = grid_for(n), gr = grid_rows(A);
-    hipLaunchKernelGGL(k_pcg_init_norms, dim3(g), dim3(BS), 0, s, n,
-                       (const double *)b->d, (const double *)x->d,
-                       ws->seg(SEG_NORMS));
-    hipLaunchKernelGGL(k_pcg_init_res, dim3(gr), dim3(BS), 0, s, n,
-                       (const int32_t *)A->d_rowptr, (const int32_t *)A->d_cols,
-                       (const double *)A->d_vals, (const int32_t *)A->d_rblk,
-                       A->nrblk, (const double *)A->d_diag,
-                       (const double *)b->d, x->d, ws->r,
-                       (const double *)ws->seg(SEG_NORMS), g, ws->seg(SEG_RES),
-                       ws->st);
-    PHG_HIP_CHECK(hipGetLastError());
-    State fin;
-    // the prologue ran on the row grid: its partials count is gr
-    if ((rc = run_pcg_loop<double>(A, A->d_vals, A->d_diag, x->d, ws->r,
-                                   ws->p, ws->Ap, ws, maxit, gr, 8, &fin)))
+    phg_hip_mat *A = const_cast<phg_hip_mat *>(A_);
+    if ((rc = ensure_format(A, A->fmt64, 0)))
         return rc;
-    if (residual)
-        *residual = fin.res;
-    if (fin.done == ZERO_DIAG)
-        return phg::set_error(PHG_HIP_E_ZERO_DIAG,
-                              "zero diagonal in Jacobi preconditioner");
-    return fin.nits;
+    switch (A->fmt64) {
+    case FMT_CSR_BLOCK:
+        return pcg64_impl<CsrView>(A, b, x, maxit, rtol, btol, atol, residual);
+    case FMT_SELL:
+        return pcg64_impl<SellView>(A, b, x, maxit, rtol, btol, atol, residual);
+    default:
+        return pcg64_impl<CsrWaveView>(A, b, x, maxit, rtol, btol, atol, residual);
+    }
 }
 
 extern "C" int phg_hip_pcg_mixed(const phg_hip_mat *A_, const phg_hip_vec *b,
@@ -897,73 +1414,101 @@ extern "C" int phg_hip_pcg_mixed(const phg_hip_mat *A_, const phg_hip_vec *b,
     if ((rc = check_solve_args(A_, b, x)))
         return rc;
     phg_hip_mat *A = const_cast<phg_hip_mat *>(A_);
-    if ((rc = phg::ensure_fp32_copy(A)))
+    if ((rc = ensure_format(A, A->fmt32, 1)))
         return rc;
-    hipStream_t s = phg::ctx()->stream;
-    Workspace *ws = get_ws(A, 1);
-    if (ws == nullptr)
-        return phg::set_error(PHG_HIP_E_NOMEM, "workspace allocation failed");
-    const int64_t n = A->n;
-    int g = grid_for(n), gr = grid_rows(A);
-    double *hm = (double *)phg::ctx()->host_mailbox;
-    double tol = 0.0, res = 0.0;
-    int total_inner = 0, outer = 0;
-    for (;;) {
-        hipLaunchKernelGGL(k_ir_residual, dim3(gr), dim3(BS), 0, s,
-                           (const int32_t *)A->d_rowptr,
-                           (const int32_t *)A->d_cols, (const double *)A->d_vals,
-                           (const int32_t *)A->d_rblk, A->nrblk,
-                           (const double *)b->d, (const double *)x->d, ws->r,
-                           ws->seg(SEG_IR));
-        hipLaunchKernelGGL(k_ir_norms, dim3(1), dim3(BS), 0, s,
-                           (const double *)ws->seg(SEG_IR), gr, ws->scal);
-        PHG_HIP_CHECK(hipGetLastError());
-        PHG_HIP_CHECK(hipMemcpyAsync(hm, ws->scal, 2 * sizeof(double),
-                                     hipMemcpyDeviceToHost, s));
-        PHG_HIP_CHECK(hipStreamSynchronize(s));
-        res = hm[0];
-        const double rb = hm[1];
-        if (outer == 0) {
-            // get_tol on the FP64 residual (solver-phg.c:104)
-            tol = res * rtol;
-            if (tol < rb * btol)
-                tol = rb * btol;
-            if (tol < atol)
-                tol = atol;
-        }
-        if (res <= tol || rb == 0.0 || outer >= maxit)
-            break;
-        State h;
-        memset(&h, 0, sizeof(h));
-        h.rtol = inner_rtol;
-        h.rb = 1.0;
-        h.maxit = 100000;     // each FP32 solve stops on inner_rtol
-        PHG_HIP_CHECK(hipMemcpyAsync(ws->st, &h, sizeof(State),
-                                     hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(k_ir_inner_init, dim3(g), dim3(BS), 0, s, n,
-                           (const double *)ws->r, 1.0 / res,
-                           (const float *)A->d_diag32, ws->r32, ws->x32,
-                           ws->seg(SEG_RES));
-        PHG_HIP_CHECK(hipGetLastError());
-        State fin;
-        if ((rc = run_pcg_loop<float>(A, A->d_vals32, A->d_diag32, ws->x32,
-                                      ws->r32, ws->p32, ws->Ap32, ws, h.maxit,
-                                      g, 16, &fin)))
-            return rc;
-        if (fin.done == ZERO_DIAG)
-            return phg::set_error(PHG_HIP_E_ZERO_DIAG,
-                                  "zero diagonal in Jacobi preconditioner");
-        total_inner += fin.nits;
-        hipLaunchKernelGGL(k_ir_correct, dim3(g), dim3(BS), 0, s, n, x->d,
-                           (const float *)ws->x32, res);
-        PHG_HIP_CHECK(hipGetLastError());
-        outer++;
+    switch (A->fmt32) {
+    case FMT_CSR_BLOCK:
+        return mixed_impl<CsrView>(A, b, x, maxit, rtol, btol, atol, inner_rtol,
+                                   inner_its, residual);
+    case FMT_SELL:
+        return mixed_impl<SellView>(A, b, x, maxit, rtol, btol, atol, inner_rtol,
+                                    inner_its, residual);
+    default:
+        return mixed_impl<CsrWaveView>(A, b, x, maxit, rtol, btol, atol,
+                                       inner_rtol, inner_its, residual);
     }
-    if (inner_its)
-        *inner_its = total_inner;
-    if (residual)
-        *residual = res;
-    return outer;
+}
+
+extern "C" int phg_hip_mat_set_format(phg_hip_mat *A, int32_t fmt64,
+                                      int32_t fmt32)
+{
+    int rc;
+    if ((rc = phg::require_ctx()))
+        return rc;
+    if (A == nullptr || !A->eliminated || fmt64 < 0 || fmt64 >= FMT_N ||
+        fmt32 < 0 || fmt32 >= FMT_N)
+        return phg::set_error(PHG_HIP_E_ARG, "phg_hip_mat_set_format: bad args");
+    A->fmt64 = fmt64;
+    A->fmt32 = fmt32;
+    return PHG_HIP_OK;
+}
+
+extern "C" int phg_hip_mat_get_format(const phg_hip_mat *A, int32_t *fmt64,
+                                      int32_t *fmt32)
+{
+    if (A == nullptr)
+        return phg::set_error(PHG_HIP_E_ARG, "null matrix");
+    if (fmt64)
+        *fmt64 = A->fmt64;
+    if (fmt32)
+        *fmt32 = A->fmt32;
+    return PHG_HIP_OK;
+}
+
+extern "C" int phg_hip_mat_tune(phg_hip_mat *A, int32_t reps)
+{
+    int rc;
+    if ((rc = phg::require_ctx()))
+        return rc;
+    if (A == nullptr || !A->eliminated)
+        return phg::set_error(PHG_HIP_E_ARG, "phg_hip_mat_tune: matrix not assembled");
+    if (reps <= 0)
+        reps = 10;
+    if ((rc = ensure_format(A, FMT_SELL, 1)))
+        return rc;
+    for (int f = 0; f < FMT_N; f++) {
+        double t64 = 0, t32 = 0;
+        switch (f) {
+        case FMT_CSR_BLOCK:
+            if ((rc = time_spmv<double, CsrView>(A, A->d_vals, reps, &t64)) ||
+                (rc = time_spmv<float, CsrView>(A, A->d_vals32, reps, &t32)))
+                return rc;
+            break;
+        case FMT_CSR_WAVE:
+            if ((rc = time_spmv<double, CsrWaveView>(A, A->d_vals, reps, &t64)) ||
+                (rc = time_spmv<float, CsrWaveView>(A, A->d_vals32, reps, &t32)))
+                return rc;
+            break;
+        default:
+            if ((rc = time_spmv<double, SellView>(A, A->d_svals, reps, &t64)) ||
+                (rc = time_spmv<float, SellView>(A, A->d_svals32, reps, &t32)))
+                return rc;
+            break;
+        }
+        A->tune_us[f][0] = t64;
+        A->tune_us[f][1] = t32;
+    }
+    int b64 = 0, b32 = 0;
+    for (int f = 1; f < FMT_N; f++) {
+        if (A->tune_us[f][0] < A->tune_us[b64][0])
+            b64 = f;
+        if (A->tune_us[f][1] < A->tune_us[b32][1])
+            b32 = f;
+    }
+    A->fmt64 = b64;
+    A->fmt32 = b32;
+    if (b64 != FMT_SELL && b32 != FMT_SELL)
+        phg::free_sell(A);    // the permuted copy is not needed
+    return PHG_HIP_OK;
+}
+
+extern "C" int phg_hip_mat_tune_time(const phg_hip_mat *A, int32_t fmt,
+                                     int32_t fp32, double *us)
+{
+    if (A == nullptr || fmt < 0 || fmt >= FMT_N || us == nullptr)
+        return phg::set_error(PHG_HIP_E_ARG, "phg_hip_mat_tune_time: bad args");
+    *us = A->tune_us[fmt][fp32 ? 1 : 0];
+    return PHG_HIP_OK;
 }
 
 extern "C" int phg_hip_profile(int32_t on)

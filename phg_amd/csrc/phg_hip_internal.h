@@ -64,13 +64,33 @@ struct phg_hip_mat {
     uint8_t *d_bdry_row = nullptr;   // [n]
     double *d_diag = nullptr;        // [n], set after elimination
     int eliminated = 0;
-    // FP32 copy of the values for the mixed-precision inner solve
-    float *d_vals32 = nullptr;
-    float *d_diag32 = nullptr;
     // SpMV row blocks: consecutive rows whose entries fit the LDS stage of
     // the streaming SpMV (or one long row); boundaries [nrblk + 1]
     int32_t *d_rblk = nullptr;
     int64_t nrblk = 0;
+    // the same for the wavefront-level stream (smaller blocks)
+    int32_t *d_wblk = nullptr;
+    int64_t nwblk = 0;
+    // FP32 copies of the CSR values and diagonal (mixed precision)
+    float *d_vals32 = nullptr;
+    float *d_diag32 = nullptr;
+    // SpMV storage format used by the solvers, per precision, and the times
+    // the autotuner measured for each format [fmt][0: FP64, 1: FP32] (us)
+    int fmt64 = 1, fmt32 = 1;
+    double tune_us[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+    // SELL-64 copy used by the solvers: rows sorted by length inside windows
+    // of SELL_WINDOW rows and renumbered symmetrically (perm: new -> old),
+    // slices of 64 rows stored column-major, padded with zero entries
+    int64_t nslice = 0;
+    int32_t *d_perm = nullptr, *d_iperm = nullptr;
+    int64_t *d_sptr = nullptr;       // [nslice + 1] first entry of each slice
+    int32_t *d_swidth = nullptr;     // [nslice] entries per row in the slice
+    int32_t *d_scols = nullptr;
+    double *d_svals = nullptr;
+    float *d_svals32 = nullptr;
+    double *d_sdiag = nullptr;       // permuted diagonal
+    float *d_sdiag32 = nullptr;
+    int64_t sell_nnz = 0;            // stored entries including padding
 };
 
 namespace phg {
@@ -78,6 +98,8 @@ namespace phg {
 int spmv64(const phg_hip_mat *A, const double *x, double *y, double alpha,
            double beta, hipStream_t s);
 int build_row_blocks(phg_hip_mat *A);
+int build_sell(phg_hip_mat *A);
+void free_sell(phg_hip_mat *A);
 int ensure_fp32_copy(phg_hip_mat *A);
 int setup_diagonal(phg_hip_mat *A);
 }  // namespace phg

@@ -138,6 +138,59 @@ def test_pcg_matches_oracle(hip, kind, p, n):
     assert rel < 1e-9, rel
 
 
+@pytest.mark.parametrize("fmt", [0, 1, 2])
+def test_pcg_every_spmv_layout(hip, fmt):
+    """FP64 PCG and the mixed solver agree with the oracle on each layout."""
+    pa = hip
+    c = poisson_case(5, 2, "kappa")
+    pb = pa.Problem.poisson(use_kappa=True)
+    sp, A, Ao = _assemble_both(pa, c, pb)
+    A.eliminate()
+    A.set_format(fmt, fmt)
+    orc.eliminate(Ao, c["bdry"])
+    xo, nits_o, _ = orc.pcg(Ao, Ao.rhs, rtol=1e-12, btol=1e-12)
+    x = pa.Vec(A.n)
+    nits, _ = pa.pcg(A, A.rhs(), x, rtol=1e-12, btol=1e-12)
+    assert abs(nits - nits_o) <= 2
+    assert np.linalg.norm(x.numpy() - xo) / np.linalg.norm(xo) < 1e-9
+    x2 = pa.Vec(A.n)
+    pa.pcg_mixed(A, A.rhs(), x2, rtol=1e-11, btol=1e-11, inner_rtol=1e-4)
+    assert np.linalg.norm(x2.numpy() - xo) / np.linalg.norm(xo) < 1e-8
+
+
+def test_tuner_picks_measured_layout(hip):
+    pa = hip
+    c = poisson_case(6, 2, "kappa")
+    pb = pa.Problem.poisson(use_kappa=True)
+    sp = _space(pa, c)
+    A = pa.assemble(sp, pb, kappa=c["kappa"], f=c["f"], g=c["g"])
+    A.eliminate()
+    t = A.tune(reps=5)
+    names = ("csr_block", "csr_wave", "sell64")
+    assert all(t[k][0] > 0 and t[k][1] > 0 for k in names)
+    f64, f32 = t["chosen"]
+    assert t[f64][0] == min(t[k][0] for k in names)
+    assert t[f32][1] == min(t[k][1] for k in names)
+    x = pa.Vec(A.n)
+    nits, res = pa.pcg(A, A.rhs(), x, rtol=1e-10, btol=1e-10)
+    assert nits > 0
+
+
+def test_matvec_bitexact_with_phg_row_sums(hip):
+    """phgMatVec sums each row in storage order (matvec.c:920-928); the
+    streaming SpMV stages products in LDS and sums them in the same order."""
+    pa = hip
+    c = poisson_case(5, 3, "kappa")
+    pb = pa.Problem.poisson(use_kappa=True)
+    sp, A, Ao = _assemble_both(pa, c, pb)
+    A.eliminate()
+    orc.eliminate(Ao, c["bdry"])
+    xv = np.random.default_rng(9).standard_normal(A.n)
+    x, y = pa.Vec(A.n, xv), pa.Vec(A.n)
+    A.matvec(x, y)
+    assert _bits_equal(y.numpy(), orc.matvec(Ao, xv))
+
+
 def test_pcg_nonzero_initial_guess(hip):
     pa = hip
     c = poisson_case(4, 2, "kappa")
