@@ -152,19 +152,27 @@ int phg_hip_pcg_mixed(const phg_hip_mat *A, const phg_hip_vec *b,
 /*--------------------------------------------------- SpMV format autotuner */
 
 /* No reference counterpart (PHG has one CSR kernel, matvec.c:887).  The
- * solvers can run their SpMV on three layouts of the eliminated matrix:
- *   CSR_BLOCK  work-group CSR stream: a row block's entries are loaded
- *              coalesced into LDS as products, one work-item sums a row;
- *   CSR_WAVE   the same per wavefront, with wavefront-private LDS and no
- *              work-group barriers (default);
+ * solvers run their SpMV on one of these layouts of the eliminated matrix,
+ * all in the solver numbering (DOFs along a Morton curve of their nodes):
+ *   B2K, B4K   work-group CSR stream: row blocks of at most 2048 / 4096
+ *              entries are loaded coalesced and staged in LDS as products;
+ *              one work-item sums a row in storage order;
+ *   B2K_PAR, B4K_PAR
+ *              the same with several lanes per row for the sums;
+ *   WAVE       the same per wavefront (512-entry blocks, wavefront-private
+ *              LDS, no work-group barriers);
  *   SELL       SELL-64: slices of 64 rows sorted by length inside windows of
- *              4096 rows, stored column-major, rows renumbered symmetrically.
+ *              4096 rows, stored column-major.
  * phg_hip_mat_tune times each layout on the matrix (HIP events, `reps`
  * launches) in FP64 and FP32 and keeps the fastest for phg_hip_pcg and
  * phg_hip_pcg_mixed respectively. */
-#define PHG_HIP_FMT_CSR_BLOCK 0
-#define PHG_HIP_FMT_CSR_WAVE  1
-#define PHG_HIP_FMT_SELL      2
+#define PHG_HIP_FMT_B2K        0
+#define PHG_HIP_FMT_B4K        1
+#define PHG_HIP_FMT_B2K_PAR    2
+#define PHG_HIP_FMT_B4K_PAR    3
+#define PHG_HIP_FMT_WAVE       4
+#define PHG_HIP_FMT_SELL       5
+#define PHG_HIP_FMT_COUNT      6
 int phg_hip_mat_tune(phg_hip_mat *A, int32_t reps);
 int phg_hip_mat_tune_time(const phg_hip_mat *A, int32_t fmt, int32_t fp32,
                           double *us);
@@ -185,6 +193,10 @@ int phg_hip_mat_get_format(const phg_hip_mat *A, int32_t *fmt64,
 #define PHG_HIP_K_SPMV32      4   /* FP32 inner solve of pcg_mixed    */
 #define PHG_HIP_K_UPDATE32    5
 int phg_hip_profile(int32_t on);
+/* Bandwidth reference: `reps` launches of a plain copy of n doubles
+ * (16 n bytes each); *us = average launch time.  Used to scale HBM
+ * counters and to state the attainable copy bandwidth. */
+int phg_hip_stream_copy(int64_t n, int32_t reps, double *us);
 int phg_hip_profile_read(int32_t kind, int64_t *launches, double *ms);
 
 #ifdef __cplusplus

@@ -35,7 +35,7 @@ def _load(path: str) -> C.CDLL:
     if not os.path.exists(path):
         raise ImportError(
             f"phg_amd native library missing: {path}. Build it with "
-            "`python -m phg_amd.build` (or __graft_entry__.build()).")
+            "`python phg_amd/build.py` (or __graft_entry__.build()).")
     return C.CDLL(path)
 
 
@@ -125,11 +125,12 @@ _sig(hip_lib, "phg_hip_mat_tune_time", C.c_int, [_vp, _i32, _i32, _P(_dbl)])
 _sig(hip_lib, "phg_hip_mat_set_format", C.c_int, [_vp, _i32, _i32])
 _sig(hip_lib, "phg_hip_mat_get_format", C.c_int, [_vp, _P(_i32), _P(_i32)])
 _sig(hip_lib, "phg_hip_profile", C.c_int, [_i32])
+_sig(hip_lib, "phg_hip_stream_copy", C.c_int, [_i64, _i32, _P(_dbl)])
 _sig(hip_lib, "phg_hip_profile_read", C.c_int, [_i32, _P(_i64), _P(_dbl)])
 
 # SpMV layouts of the solvers (include/phg_hip.h)
-FMT_CSR_BLOCK, FMT_CSR_WAVE, FMT_SELL = 0, 1, 2
-FMT_NAMES = {0: "csr_block", 1: "csr_wave", 2: "sell64"}
+FMT_NAMES = {0: "csr_b2k", 1: "csr_b4k", 2: "csr_b2k_par", 3: "csr_b4k_par",
+             4: "csr_wave", 5: "sell64"}
 
 # kernel classes of phg_hip_profile_read (include/phg_hip.h)
 K_PCG_P64, K_SPMV64, K_UPDATE64, K_PCG_P32, K_SPMV32, K_UPDATE32 = range(6)
@@ -404,6 +405,13 @@ def assemble(sp: Space, pb: Problem, kappa=None, f=None, g=None) -> Mat:
     _check(hip_lib.phg_hip_assemble(sp._h, C.byref(pb), _ptr(kap, _dbl), _ptr(f, _dbl),
                                     _ptr(g, _dbl), C.byref(h)), "assemble")
     return Mat(h)
+
+
+def stream_copy(n: int, reps: int = 20) -> float:
+    """Average microseconds of a plain copy of n doubles (16 n bytes)."""
+    us = _dbl()
+    _check(hip_lib.phg_hip_stream_copy(n, reps, C.byref(us)), "stream_copy")
+    return us.value
 
 
 def profile(on: bool) -> None:

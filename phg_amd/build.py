@@ -3,7 +3,7 @@
     libphg_hip.so   HIP kernels + C ABI (include/phg_hip.h), gfx950 only
     libphg_grid.so  host grid / PHG numbering layer (C)
 
-Run ``python -m phg_amd.build`` or call :func:`build`.  Outputs go to
+Run ``python phg_amd/build.py`` or call :func:`build`.  Outputs go to
 ``phg_amd/lib``.  Every HIP translation unit is compiled with packed-FP32
 instructions disabled; the assembly unit additionally with
 ``-ffp-contract=off`` so its floating-point operations follow the reference
@@ -37,6 +37,7 @@ HIP_UNITS = {
     "runtime.hip": [],
     "assemble.hip": ["-ffp-contract=off"],
     "krylov.hip": [],
+    "reorder.hip": [],
 }
 
 

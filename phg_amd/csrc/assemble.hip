@@ -552,6 +552,47 @@ __global__ void __launch_bounds__(128) k_element(ElemArgs a)
     }
 }
 
+//------------------------------------------------------ nodal coordinates
+// barycentric node of local basis b (dof.c:4925), combined in the element's
+// local vertex order as geom.c:1259 phgGeomLambda2XYZ does
+__global__ void k_dof_coords(int64_t nelem, int order, int N,
+                             const double *verts, const int32_t *ev,
+                             const int32_t *map, double *xyz)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nelem * N)
+        return;
+    int64_t e = t / N;
+    int b = (int)(t % N);
+    double lam[4] = {0., 0., 0., 0.};
+    if (b < 4)
+        lam[b] = 1.0;
+    else if (order == 2) {
+        lam[c_ev[b - 4][0]] = 0.5;
+        lam[c_ev[b - 4][1]] = 0.5;
+    }
+    else if (b < 16) {
+        int k = (b - 4) / 2, l = (b - 4) % 2;
+        lam[c_ev[k][0]] = l == 0 ? 2. / 3. : 1. / 3.;
+        lam[c_ev[k][1]] = l == 0 ? 1. / 3. : 2. / 3.;
+    }
+    else {
+        int f = b - 16;
+        lam[c_fv[f][0]] = lam[c_fv[f][1]] = lam[c_fv[f][2]] = 1. / 3.;
+    }
+    double x = 0., y = 0., z = 0.;
+    for (int i = 0; i < 4; i++) {
+        const double *c = verts + 3 * (int64_t)ev[4 * e + i];
+        x += c[0] * lam[i];
+        y += c[1] * lam[i];
+        z += c[2] * lam[i];
+    }
+    int64_t d = map[e * N + b];
+    xyz[3 * d] = x;
+    xyz[3 * d + 1] = y;
+    xyz[3 * d + 2] = z;
+}
+
 //-------------------------------------------------------- incidence build
 // row of local index a of element e
 __device__ inline int64_t row_of(const int32_t *map, int64_t e, int a, int N,
@@ -1101,8 +1142,18 @@ extern "C" int phg_hip_assemble(phg_hip_space *sp, const phg_hip_problem *pb,
 
     A = new phg_hip_mat();
     A->n = n;
-    if ((rc = phg::dmalloc((void **)&A->d_rhs, (size_t)n * sizeof(double))))
+    A->dim = dim;
+    A->ndof = sp->ndof;
+    if ((rc = phg::dmalloc((void **)&A->d_rhs, (size_t)n * sizeof(double))) ||
+        (rc = phg::dmalloc((void **)&A->d_xyz, (size_t)sp->ndof * 3 * sizeof(double))))
         goto done;
+    // nodal coordinates (identical values whichever element writes a node
+    // for P1/P2 nodes; P3 face nodes may differ in the last bit, which only
+    // matters for the ordering key)
+    hipLaunchKernelGGL(k_dof_coords, dim3(nblk(ne * N, 256)), dim3(256), 0, s,
+                       ne, (int)sp->order, N, (const double *)sp->d_verts,
+                       (const int32_t *)sp->d_elem_verts,
+                       (const int32_t *)sp->d_map, A->d_xyz);
     {
         WalkArgs w;
         w.n = n;
@@ -1210,10 +1261,7 @@ extern "C" int phg_hip_mat_eliminate(phg_hip_mat *A)
     phg::dfree(A->d_rhs);
     phg::dfree(A->d_diag);
     phg::free_sell(A);
-    phg::dfree(A->d_vals32);
-    phg::dfree(A->d_diag32);
-    A->d_vals32 = nullptr;
-    A->d_diag32 = nullptr;
+    phg::free_solver_copy(A);
     A->d_rowptr = nrow32;
     A->d_cols = ncols;
     A->d_vals = nvals;
@@ -1223,7 +1271,9 @@ extern "C" int phg_hip_mat_eliminate(phg_hip_mat *A)
     A->eliminated = 1;
     phg::dfree(count);
     phg::dfree(nrow);
-    return phg::build_row_blocks(A);
+    if ((rc = phg::build_row_blocks(A)))
+        return rc;
+    return phg::build_solver_copy(A);
 fail:
     phg::dfree(count);
     phg::dfree(nrow);
@@ -1283,10 +1333,9 @@ extern "C" int phg_hip_mat_destroy(phg_hip_mat *A)
     phg::dfree(A->d_bdry_row);
     phg::dfree(A->d_diag);
     phg::dfree(A->d_rblk);
-    phg::dfree(A->d_wblk);
-    phg::dfree(A->d_vals32);
-    phg::dfree(A->d_diag32);
+    phg::dfree(A->d_xyz);
     phg::free_sell(A);
+    phg::free_solver_copy(A);
     delete A;
     return PHG_HIP_OK;
 }

@@ -144,8 +144,18 @@ __device__ inline void row_range(int64_t n, int64_t &r0, int64_t &r1)
 // storage order -- the order of matvec.c:887-930 -- and hands (row, sum) to
 // the epilogue.  A row longer than the LDS stage is a block of its own,
 // summed by the whole work-group.
-constexpr int NNZ_CAP = 2048;   // entries staged per row block
-constexpr int ROW_CAP = BS;     // rows per row block
+constexpr int NNZ_CAP = phg::SPMV_BLOCK_NNZ;   // entries staged per row block
+constexpr int ROW_CAP = phg::SPMV_BLOCK_ROWS;  // rows per row block
+static_assert(ROW_CAP <= BS, "one work-item per row");
+
+// work-group barrier that orders LDS only: global loads issued before it
+// (prefetches) stay in flight
+__device__ inline void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
 
 template <typename T>
 __device__ inline T block_sum(T v)
@@ -157,16 +167,21 @@ __device__ inline T block_sum(T v)
         v += __shfl_xor(v, o, 64);
     if (lane == 0)
         sh[wid] = v;
-    __syncthreads();
+    lds_barrier();
     T s = 0;
 #pragma unroll
     for (int w = 0; w < BS / 64; w++)
         s += sh[w];
-    __syncthreads();
+    lds_barrier();
     return s;
 }
 
-template <typename T, typename Epi>
+// Row-block streaming SpMV, a family for the autotuner: CAP entries staged
+// per block; MODE 0 loads them in a loop unrolled by four, MODE 1 in one
+// batch of CAP / BS per work-item; in both one work-item sums a row in
+// storage order (matvec.c:920-928).  MODE 2 loads like MODE 0 and sums each
+// row with several lanes (different summation order, solvers only).
+template <typename T, int CAP, int MODE, typename Epi>
 __device__ inline void stream_spmv(const int32_t *__restrict__ rp,
                                    const int32_t *__restrict__ ci,
                                    const T *__restrict__ va,
@@ -174,29 +189,72 @@ __device__ inline void stream_spmv(const int32_t *__restrict__ rp,
                                    const int32_t *__restrict__ rblk,
                                    int64_t nrblk, Epi epi)
 {
-    __shared__ T prod[NNZ_CAP];
+    constexpr int PER = CAP / BS;
+    static_assert(CAP % BS == 0, "whole entries per work-item");
+    __shared__ T prod[CAP];
     int64_t b0, b1;
     row_range(nrblk, b0, b1);
     const int tid = threadIdx.x;
     for (int64_t b = b0; b < b1; b++) {
         const int32_t r0 = rblk[b], r1 = rblk[b + 1];
         const int32_t j0 = rp[r0], j1 = rp[r1];
-        if (j1 - j0 <= NNZ_CAP) {
+        if (j1 - j0 <= CAP) {
+            if (MODE == 0) {
 #pragma unroll 4
-            for (int32_t j = j0 + tid; j < j1; j += BS)
-                prod[j - j0] = va[j] * x[ci[j]];
-            __syncthreads();
-            if (tid < r1 - r0) {
-                const int32_t r = r0 + tid;
-                const int32_t s0 = rp[r] - j0, s1 = rp[r + 1] - j0;
-                T sum = 0;
-                for (int32_t j = s0; j < s1; j++)
-                    sum += prod[j];
-                epi(r, sum);
+                for (int32_t j = j0 + tid; j < j1; j += BS)
+                    prod[j - j0] = va[j] * x[ci[j]];
             }
-            __syncthreads();
+            else {
+                int32_t c[PER];
+                T v[PER];
+#pragma unroll
+                for (int k = 0; k < PER; k++) {
+                    const int32_t j = j0 + tid + BS * k;
+                    c[k] = j < j1 ? ci[j] : 0;
+                    v[k] = j < j1 ? va[j] : (T)0;
+                }
+#pragma unroll
+                for (int k = 0; k < PER; k++) {
+                    const int32_t j = j0 + tid + BS * k;
+                    if (j < j1)
+                        prod[j - j0] = v[k] * x[c[k]];
+                }
+            }
+            lds_barrier();
+            const int nrows = r1 - r0;
+            if (MODE < 2) {
+                if (tid < nrows) {
+                    const int32_t r = r0 + tid;
+                    const int32_t s0 = rp[r] - j0, s1 = rp[r + 1] - j0;
+                    T sum = 0;
+                    for (int32_t j = s0; j < s1; j++)
+                        sum += prod[j];
+                    epi(r, sum);
+                }
+            }
+            else {
+                // lpr lanes per row (a power of two inside one wavefront),
+                // strided partial sums combined with shuffles
+                int lpr = 1;
+                while (lpr < 16 && 2 * lpr * nrows <= BS)
+                    lpr *= 2;
+                const int row = tid / lpr, sub = tid % lpr;
+                T sum = 0;
+                int32_t r = r0 + row;
+                if (row < nrows) {
+                    const int32_t s0 = rp[r] - j0, s1 = rp[r + 1] - j0;
+                    for (int32_t j = s0 + sub; j < s1; j += lpr)
+                        sum += prod[j];
+                }
+                for (int o = lpr / 2; o > 0; o >>= 1)
+                    sum += __shfl_xor(sum, o, 64);
+                if (row < nrows && sub == 0)
+                    epi(r, sum);
+            }
+            lds_barrier();
         }
         else {
+            // one row longer than the stage: the work-group strides over it
             T sum = 0;
             for (int32_t j = j0 + tid; j < j1; j += BS)
                 sum += va[j] * x[ci[j]];
@@ -210,10 +268,12 @@ __device__ inline void stream_spmv(const int32_t *__restrict__ rp,
 // Matrix views for the SpMV-shaped kernels.  CsrView streams PHG's CSR over
 // row blocks (vectors in the original numbering); SellView walks the SELL-64
 // copy (vectors in the solver's permuted numbering).
-struct CsrView {
+template <int CAP, int MODE>
+struct CsrBlockView {
     const int32_t *rp, *ci, *rblk;
     int64_t nrblk;
 };
+using CsrView = CsrBlockView<NNZ_CAP, 0>;
 
 struct SellView {
     const int64_t *sptr;
@@ -221,11 +281,11 @@ struct SellView {
     int64_t nslice, n;
 };
 
-template <typename T, typename Epi>
-__device__ inline void spmv_apply(const CsrView &m, const T *va, const T *x,
-                                  Epi epi)
+template <typename T, int CAP, int MODE, typename Epi>
+__device__ inline void spmv_apply(const CsrBlockView<CAP, MODE> &m,
+                                  const T *va, const T *x, Epi epi)
 {
-    stream_spmv<T>(m.rp, m.ci, va, x, m.rblk, m.nrblk, epi);
+    stream_spmv<T, CAP, MODE>(m.rp, m.ci, va, x, m.rblk, m.nrblk, epi);
 }
 
 // Wavefront-level CSR stream: each wavefront owns a private LDS slice and
@@ -234,8 +294,9 @@ __device__ inline void spmv_apply(const CsrView &m, const T *va, const T *x,
 // together, the products go to the wavefront's LDS slice, and lane l sums row
 // r0 + l in storage order.  No work-group barrier is involved, so the four
 // wavefronts of a work-group progress independently.
-constexpr int WNNZ = 512;       // entries per wavefront row block (8 per lane)
-constexpr int WROWS = 64;
+constexpr int WNNZ = phg::SPMV_WAVE_NNZ;   // entries per wavefront block
+constexpr int WROWS = phg::SPMV_WAVE_ROWS;
+static_assert(WNNZ % 64 == 0 && WROWS <= 64, "one lane per row");
 
 struct CsrWaveView {
     const int32_t *rp, *ci, *wblk;
@@ -409,8 +470,8 @@ __global__ void __launch_bounds__(BS) k_spmv64(const int32_t *rp,
                                                double *y, double alpha,
                                                double beta)
 {
-    stream_spmv<double>(rp, ci, va, x, rblk, nrblk,
-                        [&](int32_t row, double s) {
+    stream_spmv<double, NNZ_CAP, 0>(rp, ci, va, x, rblk, nrblk,
+                                    [&](int32_t row, double s) {
                             // matvec.c:608-633 beta first, then alpha * A x
                             double yo = beta == 0.0 ? 0.0 : beta * y[row];
                             y[row] = yo + alpha * s;
@@ -690,6 +751,15 @@ __global__ void __launch_bounds__(BS) k_ir_correct(int64_t n, double *x,
         x[i] += s * (double)d[i];
 }
 
+// plain copy, the reference point for bandwidth and HBM-counter scaling
+__global__ void __launch_bounds__(BS) k_copy(int64_t n, const double *a,
+                                             double *b)
+{
+    for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * BS)
+        b[i] = a[i];
+}
+
 __global__ void k_to_f32(int64_t n, const double *a, float *b)
 {
     for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
@@ -701,7 +771,33 @@ __global__ void k_to_f32(int64_t n, const double *a, float *b)
 // partial-sum segments (each MAX_GRID * 4 doubles)
 enum { SEG_NORMS = 0, SEG_RES, SEG_S, SEG_U, SEG_IR, SEG_N };
 // storage formats of the solver SpMV
-enum { FMT_CSR_BLOCK = 0, FMT_CSR_WAVE = 1, FMT_SELL = 2, FMT_N };
+enum {
+    FMT_B2K = 0,      // CsrBlockView<2048, 0>
+    FMT_B4K,          // CsrBlockView<4096, 0>
+    FMT_B2K_PAR,      // CsrBlockView<2048, 2>
+    FMT_B4K_PAR,      // CsrBlockView<4096, 2>
+    FMT_WAVE,         // CsrWaveView
+    FMT_SELL,         // SellView
+    FMT_N
+};
+using CsrB2K = CsrBlockView<2048, 0>;
+using CsrB4K = CsrBlockView<4096, 0>;
+using CsrB2KPar = CsrBlockView<2048, 2>;
+using CsrB4KPar = CsrBlockView<4096, 2>;
+
+// calls f with a value of the view type of format fmt
+template <typename Fn>
+int with_format(int fmt, Fn &&f)
+{
+    switch (fmt) {
+    case FMT_B2K: return f(CsrB2K{});
+    case FMT_B4K: return f(CsrB4K{});
+    case FMT_B2K_PAR: return f(CsrB2KPar{});
+    case FMT_B4K_PAR: return f(CsrB4KPar{});
+    case FMT_WAVE: return f(CsrWaveView{});
+    default: return f(SellView{});
+    }
+}
 
 struct Workspace {
     int64_t n = 0;
@@ -728,22 +824,31 @@ int grid_for(int64_t n)
 }
 
 // views and launch grids of the three formats
-CsrView view_of(const phg_hip_mat *A, CsrView *)
+template <int CAP, int MODE>
+CsrBlockView<CAP, MODE> view_of(const phg_hip_mat *A, CsrBlockView<CAP, MODE> *)
 {
-    return CsrView{A->d_rowptr, A->d_cols, A->d_rblk, A->nrblk};
+    if (CAP == phg::SPMV_LARGE_NNZ)
+        return CsrBlockView<CAP, MODE>{A->q_rowptr, A->q_cols, A->q_rblk1k,
+                                       A->q_nrblk1k};
+    return CsrBlockView<CAP, MODE>{A->q_rowptr, A->q_cols, A->q_rblk,
+                                   A->q_nrblk};
 }
 CsrWaveView view_of(const phg_hip_mat *A, CsrWaveView *)
 {
-    return CsrWaveView{A->d_rowptr, A->d_cols, A->d_wblk, A->nwblk};
+    return CsrWaveView{A->q_rowptr, A->q_cols, A->q_wblk, A->q_nwblk};
 }
 SellView view_of(const phg_hip_mat *A, SellView *)
 {
     return SellView{A->d_sptr, A->d_swidth, A->d_scols, A->nslice, A->n};
 }
-int64_t units_of(const phg_hip_mat *A, CsrView *) { return A->nrblk; }
+template <int CAP, int MODE>
+int64_t units_of(const phg_hip_mat *A, CsrBlockView<CAP, MODE> *)
+{
+    return CAP == phg::SPMV_LARGE_NNZ ? A->q_nrblk1k : A->q_nrblk;
+}
 int64_t units_of(const phg_hip_mat *A, CsrWaveView *)
 {
-    return (A->nwblk + BS / 64 - 1) / (BS / 64) * (BS / 64);
+    return (A->q_nwblk + BS / 64 - 1) / (BS / 64) * (BS / 64);
 }
 int64_t units_of(const phg_hip_mat *A, SellView *)
 {
@@ -761,21 +866,19 @@ int grid_spmv(const phg_hip_mat *A)
     return (int)std::max<int64_t>(1, std::min<int64_t>(u, cap_grid()));
 }
 
-template <typename View> struct FmtData;
-template <> struct FmtData<CsrView> {
-    static const double *v64(const phg_hip_mat *A) { return A->d_vals; }
-    static const float *v32(const phg_hip_mat *A) { return A->d_vals32; }
-    static const double *d64(const phg_hip_mat *A) { return A->d_diag; }
-    static const float *d32(const phg_hip_mat *A) { return A->d_diag32; }
-    static constexpr bool permuted = false;
+template <typename View> struct FmtData {
+    static const double *v64(const phg_hip_mat *A) { return A->q_vals; }
+    static const float *v32(const phg_hip_mat *A) { return A->q_vals32; }
+    static const double *d64(const phg_hip_mat *A) { return A->q_diag; }
+    static const float *d32(const phg_hip_mat *A) { return A->q_diag32; }
+    static const int32_t *perm(const phg_hip_mat *A) { return A->q_perm; }
 };
-template <> struct FmtData<CsrWaveView> : FmtData<CsrView> {};
 template <> struct FmtData<SellView> {
     static const double *v64(const phg_hip_mat *A) { return A->d_svals; }
     static const float *v32(const phg_hip_mat *A) { return A->d_svals32; }
     static const double *d64(const phg_hip_mat *A) { return A->d_sdiag; }
     static const float *d32(const phg_hip_mat *A) { return A->d_sdiag32; }
-    static constexpr bool permuted = true;
+    static const int32_t *perm(const phg_hip_mat *A) { return A->d_perm; }
 };
 
 Workspace *get_ws(const phg_hip_mat *A, int need32)
@@ -963,13 +1066,12 @@ int pcg64_impl(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x,
     int g = grid_for(n), gs = grid_spmv<View>(A);
     const double *bb = b->d;
     double *xx = x->d;
-    if (F::permuted) {
-        hipLaunchKernelGGL(k_gather, dim3(g), dim3(BS), 0, s, n,
-                           (const int32_t *)A->d_perm, (const double *)b->d,
-                           ws->bp);
-        hipLaunchKernelGGL(k_gather, dim3(g), dim3(BS), 0, s, n,
-                           (const int32_t *)A->d_perm, (const double *)x->d,
-                           ws->xp);
+    const int32_t *perm = F::perm(A);
+    if (perm) {
+        hipLaunchKernelGGL(k_gather, dim3(g), dim3(BS), 0, s, n, perm,
+                           (const double *)b->d, ws->bp);
+        hipLaunchKernelGGL(k_gather, dim3(g), dim3(BS), 0, s, n, perm,
+                           (const double *)x->d, ws->xp);
         bb = ws->bp;
         xx = ws->xp;
     }
@@ -986,10 +1088,9 @@ int pcg64_impl(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x,
     if ((rc = run_pcg_loop<double, View>(A, F::v64(A), F::d64(A), xx, ws->r,
                                          ws->p, ws->Ap, ws, maxit, gs, 8, &fin)))
         return rc;
-    if (F::permuted)
-        hipLaunchKernelGGL(k_scatter, dim3(g), dim3(BS), 0, s, n,
-                           (const int32_t *)A->d_perm, (const double *)ws->xp,
-                           x->d);
+    if (perm)
+        hipLaunchKernelGGL(k_scatter, dim3(g), dim3(BS), 0, s, n, perm,
+                           (const double *)ws->xp, x->d);
     PHG_HIP_CHECK(hipGetLastError());
     PHG_HIP_CHECK(hipStreamSynchronize(s));
     if (residual)
@@ -1018,13 +1119,12 @@ int mixed_impl(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x,
     int total_inner = 0, outer = 0, rc;
     const double *bb = b->d;
     double *xx = x->d;
-    if (F::permuted) {
-        hipLaunchKernelGGL(k_gather, dim3(g), dim3(BS), 0, s, n,
-                           (const int32_t *)A->d_perm, (const double *)b->d,
-                           ws->bp);
-        hipLaunchKernelGGL(k_gather, dim3(g), dim3(BS), 0, s, n,
-                           (const int32_t *)A->d_perm, (const double *)x->d,
-                           ws->xp);
+    const int32_t *perm = F::perm(A);
+    if (perm) {
+        hipLaunchKernelGGL(k_gather, dim3(g), dim3(BS), 0, s, n, perm,
+                           (const double *)b->d, ws->bp);
+        hipLaunchKernelGGL(k_gather, dim3(g), dim3(BS), 0, s, n, perm,
+                           (const double *)x->d, ws->xp);
         bb = ws->bp;
         xx = ws->xp;
     }
@@ -1075,10 +1175,9 @@ int mixed_impl(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x,
         PHG_HIP_CHECK(hipGetLastError());
         outer++;
     }
-    if (F::permuted)
-        hipLaunchKernelGGL(k_scatter, dim3(g), dim3(BS), 0, s, n,
-                           (const int32_t *)A->d_perm, (const double *)ws->xp,
-                           x->d);
+    if (perm)
+        hipLaunchKernelGGL(k_scatter, dim3(g), dim3(BS), 0, s, n, perm,
+                           (const double *)ws->xp, x->d);
     PHG_HIP_CHECK(hipGetLastError());
     PHG_HIP_CHECK(hipStreamSynchronize(s));
     if (inner_its)
@@ -1128,50 +1227,49 @@ int time_spmv(phg_hip_mat *A, const T *va, int reps, double *us)
 
 namespace phg {
 
-int build_row_blocks(phg_hip_mat *A)
+int split_row_blocks(const int32_t *d_rowptr, int64_t n, int64_t nnz,
+                     int64_t cap, int64_t rcap, int32_t **d, int64_t *nb)
 {
     // greedy split into runs of consecutive rows holding at most cap entries
     // and rcap rows; a longer row forms a block of its own
-    std::vector<int32_t> rp((size_t)A->n + 1);
+    std::vector<int32_t> rp((size_t)n + 1), blk;
     int rc;
-    if ((rc = download(rp.data(), A->d_rowptr, rp.size() * sizeof(int32_t))))
+    if ((rc = download(rp.data(), d_rowptr, rp.size() * sizeof(int32_t))))
         return rc;
-    auto split = [&](int64_t cap, int64_t rcap, int32_t **d, int64_t *nb) {
-        std::vector<int32_t> blk;
-        blk.reserve((size_t)(A->nnz / cap + A->n / rcap + 2));
-        blk.push_back(0);
-        int64_t cur_nnz = 0, cur_rows = 0;
-        for (int64_t r = 0; r < A->n; r++) {
-            int64_t len = rp[(size_t)r + 1] - rp[(size_t)r];
-            if (len > cap) {
-                if (cur_rows > 0)
-                    blk.push_back((int32_t)r);
-                blk.push_back((int32_t)(r + 1));
-                cur_nnz = cur_rows = 0;
-                continue;
-            }
-            if (cur_rows > 0 && (cur_nnz + len > cap || cur_rows == rcap)) {
+    blk.reserve((size_t)(nnz / cap + n / rcap + 2));
+    blk.push_back(0);
+    int64_t cur_nnz = 0, cur_rows = 0;
+    for (int64_t r = 0; r < n; r++) {
+        int64_t len = rp[(size_t)r + 1] - rp[(size_t)r];
+        if (len > cap) {
+            if (cur_rows > 0)
                 blk.push_back((int32_t)r);
-                cur_nnz = cur_rows = 0;
-            }
-            cur_nnz += len;
-            cur_rows++;
+            blk.push_back((int32_t)(r + 1));
+            cur_nnz = cur_rows = 0;
+            continue;
         }
-        if (cur_rows > 0)
-            blk.push_back((int32_t)A->n);
-        dfree(*d);
-        *d = nullptr;
-        *nb = (int64_t)blk.size() - 1;
-        int e;
-        if ((e = dmalloc((void **)d, blk.size() * sizeof(int32_t))) ||
-            (e = upload(*d, blk.data(), blk.size() * sizeof(int32_t))))
-            return e;
-        return PHG_HIP_OK;
-    };
-    if ((rc = split(NNZ_CAP, ROW_CAP, &A->d_rblk, &A->nrblk)) ||
-        (rc = split(WNNZ, WROWS, &A->d_wblk, &A->nwblk)))
+        if (cur_rows > 0 && (cur_nnz + len > cap || cur_rows == rcap)) {
+            blk.push_back((int32_t)r);
+            cur_nnz = cur_rows = 0;
+        }
+        cur_nnz += len;
+        cur_rows++;
+    }
+    if (cur_rows > 0)
+        blk.push_back((int32_t)n);
+    dfree(*d);
+    *d = nullptr;
+    *nb = (int64_t)blk.size() - 1;
+    if ((rc = dmalloc((void **)d, blk.size() * sizeof(int32_t))) ||
+        (rc = upload(*d, blk.data(), blk.size() * sizeof(int32_t))))
         return rc;
     return PHG_HIP_OK;
+}
+
+int build_row_blocks(phg_hip_mat *A)
+{
+    return split_row_blocks(A->d_rowptr, A->n, A->nnz, NNZ_CAP, ROW_CAP,
+                            &A->d_rblk, &A->nrblk);
 }
 
 void free_sell(phg_hip_mat *A)
@@ -1194,8 +1292,14 @@ int build_sell(phg_hip_mat *A)
     const int64_t n = A->n;
     std::vector<int32_t> rp((size_t)n + 1);
     int rc;
-    if ((rc = download(rp.data(), A->d_rowptr, rp.size() * sizeof(int32_t))))
+    if ((rc = download(rp.data(), A->q_rowptr, rp.size() * sizeof(int32_t))))
         return rc;
+    std::vector<int32_t> qperm;
+    if (A->q_perm) {
+        qperm.resize((size_t)n);
+        if ((rc = download(qperm.data(), A->q_perm, qperm.size() * sizeof(int32_t))))
+            return rc;
+    }
     // stable sort by decreasing length inside each window: rows of equal
     // length keep their order, so neighbouring rows stay neighbours
     std::vector<int32_t> perm((size_t)n), iperm((size_t)n);
@@ -1211,8 +1315,14 @@ int build_sell(phg_hip_mat *A)
         for (int64_t i = w0; i < w1; i++)
             perm[(size_t)i] = order[(size_t)(i - w0)];
     }
+    // perm/iperm relate SELL rows to solver-CSR rows here; the stored
+    // permutation maps SELL rows to PHG rows
     for (int64_t i = 0; i < n; i++)
         iperm[(size_t)perm[(size_t)i]] = (int32_t)i;
+    std::vector<int32_t> to_phg(perm);
+    if (A->q_perm)
+        for (int64_t i = 0; i < n; i++)
+            to_phg[(size_t)i] = qperm[(size_t)perm[(size_t)i]];
     const int64_t nslice = (n + 63) / 64;
     std::vector<int64_t> sptr((size_t)nslice + 1);
     std::vector<int32_t> width((size_t)nslice);
@@ -1245,14 +1355,17 @@ int build_sell(phg_hip_mat *A)
     }
     hipStream_t s = ctx()->stream;
     hipLaunchKernelGGL(k_sell_fill, dim3((unsigned)((nslice * 64 + 255) / 256)),
-                       dim3(256), 0, s, n, nslice, (const int32_t *)A->d_rowptr,
-                       (const int32_t *)A->d_cols, (const double *)A->d_vals,
-                       (const double *)A->d_diag, (const int32_t *)A->d_perm,
+                       dim3(256), 0, s, n, nslice, (const int32_t *)A->q_rowptr,
+                       (const int32_t *)A->q_cols, (const double *)A->q_vals,
+                       (const double *)A->q_diag, (const int32_t *)A->d_perm,
                        (const int32_t *)A->d_iperm, (const int64_t *)A->d_sptr,
                        (const int32_t *)A->d_swidth, A->d_scols, A->d_svals,
                        A->d_sdiag);
     PHG_HIP_CHECK(hipGetLastError());
     PHG_HIP_CHECK(hipStreamSynchronize(s));
+    // the solvers gather with the SELL -> PHG map
+    if ((rc = upload(A->d_perm, to_phg.data(), to_phg.size() * sizeof(int32_t))))
+        return rc;
     return PHG_HIP_OK;
 }
 
@@ -1260,14 +1373,14 @@ int ensure_fp32_copy(phg_hip_mat *A)
 {
     hipStream_t s = ctx()->stream;
     int rc;
-    if (A->d_vals32 == nullptr) {
-        if ((rc = dmalloc((void **)&A->d_vals32, (size_t)(A->nnz + 1) * sizeof(float))) ||
-            (rc = dmalloc((void **)&A->d_diag32, (size_t)A->n * sizeof(float))))
+    if (A->q_vals32 == nullptr) {
+        if ((rc = dmalloc((void **)&A->q_vals32, (size_t)(A->nnz + 1) * sizeof(float))) ||
+            (rc = dmalloc((void **)&A->q_diag32, (size_t)A->n * sizeof(float))))
             return rc;
         hipLaunchKernelGGL(k_to_f32, dim3(grid_for(A->nnz)), dim3(BS), 0, s,
-                           A->nnz, (const double *)A->d_vals, A->d_vals32);
+                           A->nnz, (const double *)A->q_vals, A->q_vals32);
         hipLaunchKernelGGL(k_to_f32, dim3(grid_for(A->n)), dim3(BS), 0, s, A->n,
-                           (const double *)A->d_diag, A->d_diag32);
+                           (const double *)A->q_diag, A->q_diag32);
     }
     if (A->d_svals != nullptr && A->d_svals32 == nullptr) {
         if ((rc = dmalloc((void **)&A->d_svals32, (size_t)(A->sell_nnz + 1) * sizeof(float))) ||
@@ -1395,14 +1508,10 @@ extern "C" int phg_hip_pcg(const phg_hip_mat *A_, const phg_hip_vec *b,
     phg_hip_mat *A = const_cast<phg_hip_mat *>(A_);
     if ((rc = ensure_format(A, A->fmt64, 0)))
         return rc;
-    switch (A->fmt64) {
-    case FMT_CSR_BLOCK:
-        return pcg64_impl<CsrView>(A, b, x, maxit, rtol, btol, atol, residual);
-    case FMT_SELL:
-        return pcg64_impl<SellView>(A, b, x, maxit, rtol, btol, atol, residual);
-    default:
-        return pcg64_impl<CsrWaveView>(A, b, x, maxit, rtol, btol, atol, residual);
-    }
+    return with_format(A->fmt64, [&](auto tag) {
+        using View = decltype(tag);
+        return pcg64_impl<View>(A, b, x, maxit, rtol, btol, atol, residual);
+    });
 }
 
 extern "C" int phg_hip_pcg_mixed(const phg_hip_mat *A_, const phg_hip_vec *b,
@@ -1416,17 +1525,11 @@ extern "C" int phg_hip_pcg_mixed(const phg_hip_mat *A_, const phg_hip_vec *b,
     phg_hip_mat *A = const_cast<phg_hip_mat *>(A_);
     if ((rc = ensure_format(A, A->fmt32, 1)))
         return rc;
-    switch (A->fmt32) {
-    case FMT_CSR_BLOCK:
-        return mixed_impl<CsrView>(A, b, x, maxit, rtol, btol, atol, inner_rtol,
-                                   inner_its, residual);
-    case FMT_SELL:
-        return mixed_impl<SellView>(A, b, x, maxit, rtol, btol, atol, inner_rtol,
-                                    inner_its, residual);
-    default:
-        return mixed_impl<CsrWaveView>(A, b, x, maxit, rtol, btol, atol,
-                                       inner_rtol, inner_its, residual);
-    }
+    return with_format(A->fmt32, [&](auto tag) {
+        using View = decltype(tag);
+        return mixed_impl<View>(A, b, x, maxit, rtol, btol, atol, inner_rtol,
+                                inner_its, residual);
+    });
 }
 
 extern "C" int phg_hip_mat_set_format(phg_hip_mat *A, int32_t fmt64,
@@ -1468,23 +1571,17 @@ extern "C" int phg_hip_mat_tune(phg_hip_mat *A, int32_t reps)
         return rc;
     for (int f = 0; f < FMT_N; f++) {
         double t64 = 0, t32 = 0;
-        switch (f) {
-        case FMT_CSR_BLOCK:
-            if ((rc = time_spmv<double, CsrView>(A, A->d_vals, reps, &t64)) ||
-                (rc = time_spmv<float, CsrView>(A, A->d_vals32, reps, &t32)))
-                return rc;
-            break;
-        case FMT_CSR_WAVE:
-            if ((rc = time_spmv<double, CsrWaveView>(A, A->d_vals, reps, &t64)) ||
-                (rc = time_spmv<float, CsrWaveView>(A, A->d_vals32, reps, &t32)))
-                return rc;
-            break;
-        default:
-            if ((rc = time_spmv<double, SellView>(A, A->d_svals, reps, &t64)) ||
-                (rc = time_spmv<float, SellView>(A, A->d_svals32, reps, &t32)))
-                return rc;
-            break;
-        }
+        rc = with_format(f, [&](auto tag) {
+            using View = decltype(tag);
+            using F = FmtData<View>;
+            int e;
+            if ((e = time_spmv<double, View>(A, F::v64(A), reps, &t64)) ||
+                (e = time_spmv<float, View>(A, F::v32(A), reps, &t32)))
+                return e;
+            return (int)PHG_HIP_OK;
+        });
+        if (rc)
+            return rc;
         A->tune_us[f][0] = t64;
         A->tune_us[f][1] = t32;
     }
@@ -1508,6 +1605,44 @@ extern "C" int phg_hip_mat_tune_time(const phg_hip_mat *A, int32_t fmt,
     if (A == nullptr || fmt < 0 || fmt >= FMT_N || us == nullptr)
         return phg::set_error(PHG_HIP_E_ARG, "phg_hip_mat_tune_time: bad args");
     *us = A->tune_us[fmt][fp32 ? 1 : 0];
+    return PHG_HIP_OK;
+}
+
+extern "C" int phg_hip_stream_copy(int64_t n, int32_t reps, double *us)
+{
+    int rc;
+    if ((rc = phg::require_ctx()))
+        return rc;
+    if (n <= 0 || reps <= 0 || us == nullptr)
+        return phg::set_error(PHG_HIP_E_ARG, "phg_hip_stream_copy: bad args");
+    hipStream_t s = phg::ctx()->stream;
+    double *a = nullptr, *b = nullptr;
+    if ((rc = phg::dmalloc((void **)&a, (size_t)n * sizeof(double))) ||
+        (rc = phg::dmalloc((void **)&b, (size_t)n * sizeof(double)))) {
+        phg::dfree(a);
+        return rc;
+    }
+    PHG_HIP_CHECK(hipMemsetAsync(a, 0, (size_t)n * sizeof(double), s));
+    int cus = phg::ctx()->num_cus > 0 ? phg::ctx()->num_cus : 256;
+    int g = (int)std::min<int64_t>((n + BS - 1) / BS, (int64_t)cus * 16);
+    hipEvent_t e0, e1;
+    PHG_HIP_CHECK(hipEventCreate(&e0));
+    PHG_HIP_CHECK(hipEventCreate(&e1));
+    for (int i = 0; i < reps + 1; i++) {
+        if (i == 1)
+            PHG_HIP_CHECK(hipEventRecord(e0, s));
+        hipLaunchKernelGGL(k_copy, dim3(g), dim3(BS), 0, s, n,
+                           (const double *)a, b);
+    }
+    PHG_HIP_CHECK(hipEventRecord(e1, s));
+    PHG_HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    PHG_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    phg::dfree(a);
+    phg::dfree(b);
+    *us = 1e3 * ms / reps;
     return PHG_HIP_OK;
 }
 

@@ -57,6 +57,9 @@ struct phg_hip_vec {
 
 struct phg_hip_mat {
     int64_t n = 0, nnz = 0;
+    int dim = 1;
+    // PHG-order CSR (matvec.c packed storage): phg_hip_mat_download and
+    // phg_hip_matvec work on it
     int32_t *d_rowptr = nullptr;     // [n+1] (nnz < 2^31 is checked)
     int32_t *d_cols = nullptr;       // [nnz]
     double *d_vals = nullptr;        // [nnz]
@@ -64,23 +67,34 @@ struct phg_hip_mat {
     uint8_t *d_bdry_row = nullptr;   // [n]
     double *d_diag = nullptr;        // [n], set after elimination
     int eliminated = 0;
-    // SpMV row blocks: consecutive rows whose entries fit the LDS stage of
-    // the streaming SpMV (or one long row); boundaries [nrblk + 1]
+    // row blocks of the PHG-order CSR for the streaming SpMV
     int32_t *d_rblk = nullptr;
     int64_t nrblk = 0;
-    // the same for the wavefront-level stream (smaller blocks)
-    int32_t *d_wblk = nullptr;
-    int64_t nwblk = 0;
-    // FP32 copies of the CSR values and diagonal (mixed precision)
-    float *d_vals32 = nullptr;
-    float *d_diag32 = nullptr;
+    // nodal coordinates of the scalar DOFs [ndof][3], kept from assembly to
+    // derive the solver numbering
+    double *d_xyz = nullptr;
+    int64_t ndof = 0;
+
+    // Solver copy: the eliminated matrix under a symmetric renumbering that
+    // orders DOFs along a Morton curve of their nodes (q_perm: solver index ->
+    // PHG index); the q_* arrays alias the PHG-order ones when q_perm is null
+    int reorder = 1;                 // 1: derive the spatial numbering
+    int32_t *q_perm = nullptr;
+    int q_owns = 0;
+    int32_t *q_rowptr = nullptr, *q_cols = nullptr;
+    double *q_vals = nullptr, *q_diag = nullptr;
+    float *q_vals32 = nullptr, *q_diag32 = nullptr;
+    // row blocks of SPMV_BLOCK_NNZ / SPMV_LARGE_NNZ / SPMV_WAVE_NNZ entries
+    int32_t *q_rblk = nullptr, *q_rblk1k = nullptr, *q_wblk = nullptr;
+    int64_t q_nrblk = 0, q_nrblk1k = 0, q_nwblk = 0;
+
     // SpMV storage format used by the solvers, per precision, and the times
     // the autotuner measured for each format [fmt][0: FP64, 1: FP32] (us)
-    int fmt64 = 1, fmt32 = 1;
-    double tune_us[3][2] = {{0, 0}, {0, 0}, {0, 0}};
-    // SELL-64 copy used by the solvers: rows sorted by length inside windows
-    // of SELL_WINDOW rows and renumbered symmetrically (perm: new -> old),
-    // slices of 64 rows stored column-major, padded with zero entries
+    int fmt64 = 0, fmt32 = 0;
+    double tune_us[6][2] = {};
+    // SELL-64 copy built from the solver CSR: rows sorted by length inside
+    // windows of 4096 rows (d_perm: SELL index -> PHG index, composed with
+    // q_perm), slices of 64 rows stored column-major, padded with zeros
     int64_t nslice = 0;
     int32_t *d_perm = nullptr, *d_iperm = nullptr;
     int64_t *d_sptr = nullptr;       // [nslice + 1] first entry of each slice
@@ -94,10 +108,19 @@ struct phg_hip_mat {
 };
 
 namespace phg {
+// row-block sizes of the streaming SpMV kernels (entries / rows per block)
+constexpr int SPMV_BLOCK_NNZ = 2048, SPMV_BLOCK_ROWS = 256;
+constexpr int SPMV_WAVE_NNZ = 512, SPMV_WAVE_ROWS = 64;
+constexpr int SPMV_LARGE_NNZ = 4096;
 // Krylov building blocks (krylov.hip) used by the C ABI layer.
 int spmv64(const phg_hip_mat *A, const double *x, double *y, double alpha,
            double beta, hipStream_t s);
 int build_row_blocks(phg_hip_mat *A);
+// solver copy (reorder.hip)
+int build_solver_copy(phg_hip_mat *A);
+void free_solver_copy(phg_hip_mat *A);
+int split_row_blocks(const int32_t *d_rowptr, int64_t n, int64_t nnz,
+                     int64_t cap, int64_t rcap, int32_t **d_blk, int64_t *nblk);
 int build_sell(phg_hip_mat *A);
 void free_sell(phg_hip_mat *A);
 int ensure_fp32_copy(phg_hip_mat *A);

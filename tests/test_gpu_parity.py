@@ -138,7 +138,7 @@ def test_pcg_matches_oracle(hip, kind, p, n):
     assert rel < 1e-9, rel
 
 
-@pytest.mark.parametrize("fmt", [0, 1, 2])
+@pytest.mark.parametrize("fmt", [0, 1, 2, 3, 4, 5])
 def test_pcg_every_spmv_layout(hip, fmt):
     """FP64 PCG and the mixed solver agree with the oracle on each layout."""
     pa = hip
@@ -166,7 +166,7 @@ def test_tuner_picks_measured_layout(hip):
     A = pa.assemble(sp, pb, kappa=c["kappa"], f=c["f"], g=c["g"])
     A.eliminate()
     t = A.tune(reps=5)
-    names = ("csr_block", "csr_wave", "sell64")
+    names = tuple(pa.FMT_NAMES.values())
     assert all(t[k][0] > 0 and t[k][1] > 0 for k in names)
     f64, f32 = t["chosen"]
     assert t[f64][0] == min(t[k][0] for k in names)
