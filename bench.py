@@ -84,7 +84,7 @@ def main():
     ap.add_argument("--order", type=int, default=2)
     ap.add_argument("--sigma", type=float, default=0.5)
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--rtol", type=float, default=1e-10,
+    ap.add_argument("--rtol", type=float, default=1e-8,
                     help="relative residual stopping rule of both solvers")
     ap.add_argument("--inner-rtol", type=float, default=1e-4)
     ap.add_argument("--cpu-n", type=int, default=40,
@@ -187,6 +187,12 @@ def main():
                       "avg_us": round(t64 * 1e6, 2), "bytes_per_launch": b64,
                       "launches": l64}
 
+    # achievable HBM rate: a plain copy kernel moving the FP64 SpMV's bytes
+    copy_n = spmv_bytes(n, nnz, 8, 8) // 16
+    copy_us = pa.stream_copy(copy_n, 20)
+    copy_gbs = 16 * copy_n / copy_us / 1e3
+    log(f"copy kernel: {16 * copy_n / 1e6:.1f} MB in {copy_us:.1f} us = {copy_gbs:.0f} GB/s")
+
     cpu = None
     if args.cpu_n > 0:
         c = cpu_baseline(args.cpu_n, args.order, args.sigma, args.seed, args.rtol)
@@ -229,6 +235,8 @@ def main():
         "spmv_layout_us": {k: v for k, v in tune.items() if k != "chosen"},
         "roofline": roof,
         "spmv_fp64": spmv64,
+        "copy_kernel": {"bytes": 16 * copy_n, "avg_us": round(copy_us, 2),
+                        "GBps": round(copy_gbs, 1)},
         "cpu_baseline": cpu,
     }
     if err_mp > 1e-8:
