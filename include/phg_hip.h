@@ -149,6 +149,20 @@ int phg_hip_pcg_mixed(const phg_hip_mat *A, const phg_hip_vec *b,
                       double atol, double inner_rtol, int32_t *inner_its,
                       double *residual);
 
+/* Mixed precision with reliable updates (no reference counterpart): one
+ * FP32 Jacobi-PCG recurrence (FP32 SpMV on the FP32 copy of A, FP32
+ * vectors, FP64 dot products) whose FP32 increment is folded into x (FP64)
+ * and whose residual is replaced by the FP64 residual b - A x whenever the
+ * FP32 residual has fallen below delta times its largest value since the
+ * previous replacement, or below the tolerance.  The search direction is
+ * kept across replacements.  Stops on an FP64 residual with the tolerance
+ * rule of phg_hip_pcg; maxit bounds the FP32 iterations.  *updates receives
+ * the number of FP64 residuals.  Returns the FP32 iteration count. */
+int phg_hip_pcg_reliable(const phg_hip_mat *A, const phg_hip_vec *b,
+                         phg_hip_vec *x, int32_t maxit, double rtol,
+                         double btol, double atol, double delta,
+                         int32_t *updates, double *residual);
+
 /*--------------------------------------------------- SpMV format autotuner */
 
 /* No reference counterpart (PHG has one CSR kernel, matvec.c:887).  The
@@ -162,7 +176,16 @@ int phg_hip_pcg_mixed(const phg_hip_mat *A, const phg_hip_vec *b,
  *   WAVE       the same per wavefront (512-entry blocks, wavefront-private
  *              LDS, no work-group barriers);
  *   SELL       SELL-64: slices of 64 rows sorted by length inside windows of
- *              4096 rows, stored column-major.
+ *              4096 rows, stored column-major;
+ *   DICT, DICT_NP
+ *              dictionary-compressed CSR: blocks of <= 4096 entries and
+ *              <= 256 rows, each with a dictionary of <= 256 column chunks
+ *              of width 256; an entry stores its column in 16 bits
+ *              (slot, offset), 6 bytes per FP32 entry instead of 8.  DICT
+ *              loads the next block while reducing the current one, DICT_NP
+ *              does not (fewer registers).  Not available for matrices with
+ *              a row that does not fit a block (phg_hip_mat_tune_time then
+ *              reports -1).
  * phg_hip_mat_tune times each layout on the matrix (HIP events, `reps`
  * launches) in FP64 and FP32 and keeps the fastest for phg_hip_pcg and
  * phg_hip_pcg_mixed respectively. */
@@ -172,11 +195,19 @@ int phg_hip_pcg_mixed(const phg_hip_mat *A, const phg_hip_vec *b,
 #define PHG_HIP_FMT_B4K_PAR    3
 #define PHG_HIP_FMT_WAVE       4
 #define PHG_HIP_FMT_SELL       5
-#define PHG_HIP_FMT_COUNT      6
+#define PHG_HIP_FMT_DICT       6
+#define PHG_HIP_FMT_DICT_NP    7
+#define PHG_HIP_FMT_COUNT      8
 int phg_hip_mat_tune(phg_hip_mat *A, int32_t reps);
 int phg_hip_mat_tune_time(const phg_hip_mat *A, int32_t fmt, int32_t fp32,
                           double *us);
 int phg_hip_mat_set_format(phg_hip_mat *A, int32_t fmt64, int32_t fmt32);
+/* Bytes one SpMV moves in a layout (matrix streams + one read of x + one
+ * write of y; x gathers counted once): the algorithmic bytes for the
+ * bandwidth figures.  The layout must have been built (by the tuner or by
+ * a solve that uses it). */
+int phg_hip_mat_format_bytes(const phg_hip_mat *A, int32_t fmt, int32_t fp32,
+                             int64_t *bytes);
 int phg_hip_mat_get_format(const phg_hip_mat *A, int32_t *fmt64,
                            int32_t *fmt32);
 

@@ -23,7 +23,7 @@ _LIB = os.path.join(_HERE, "lib")
 
 __all__ = [
     "Grid", "Space", "Mat", "Vec", "Problem", "init", "finalize",
-    "assemble", "element_matrices", "pcg", "pcg_mixed", "PhgHipError",
+    "assemble", "element_matrices", "pcg", "pcg_mixed", "pcg_reliable", "PhgHipError",
     "hip_lib", "grid_lib", "HIP_LIB_PATH", "GRID_LIB_PATH",
 ]
 
@@ -120,17 +120,21 @@ _sig(hip_lib, "phg_hip_pcg", C.c_int,
      [_vp, _vp, _vp, _i32, _dbl, _dbl, _dbl, _P(_dbl)])
 _sig(hip_lib, "phg_hip_pcg_mixed", C.c_int,
      [_vp, _vp, _vp, _i32, _dbl, _dbl, _dbl, _dbl, _P(_i32), _P(_dbl)])
+_sig(hip_lib, "phg_hip_pcg_reliable", C.c_int,
+     [_vp, _vp, _vp, _i32, _dbl, _dbl, _dbl, _dbl, _P(_i32), _P(_dbl)])
 _sig(hip_lib, "phg_hip_mat_tune", C.c_int, [_vp, _i32])
 _sig(hip_lib, "phg_hip_mat_tune_time", C.c_int, [_vp, _i32, _i32, _P(_dbl)])
 _sig(hip_lib, "phg_hip_mat_set_format", C.c_int, [_vp, _i32, _i32])
 _sig(hip_lib, "phg_hip_mat_get_format", C.c_int, [_vp, _P(_i32), _P(_i32)])
+_sig(hip_lib, "phg_hip_mat_format_bytes", C.c_int, [_vp, _i32, _i32, _P(_i64)])
 _sig(hip_lib, "phg_hip_profile", C.c_int, [_i32])
 _sig(hip_lib, "phg_hip_stream_copy", C.c_int, [_i64, _i32, _P(_dbl)])
 _sig(hip_lib, "phg_hip_profile_read", C.c_int, [_i32, _P(_i64), _P(_dbl)])
 
 # SpMV layouts of the solvers (include/phg_hip.h)
 FMT_NAMES = {0: "csr_b2k", 1: "csr_b4k", 2: "csr_b2k_par", 3: "csr_b4k_par",
-             4: "csr_wave", 5: "sell64"}
+             4: "csr_wave", 5: "sell64", 6: "csr_dict",
+             7: "csr_dict_np"}
 
 # kernel classes of phg_hip_profile_read (include/phg_hip.h)
 K_PCG_P64, K_SPMV64, K_UPDATE64, K_PCG_P32, K_SPMV32, K_UPDATE32 = range(6)
@@ -346,6 +350,13 @@ class Mat:
         _check(hip_lib.phg_hip_mat_rhs(self._h, v._h), "mat_rhs")
         return v
 
+    def format_bytes(self, fmt: int, fp32: bool) -> int:
+        """Bytes one SpMV moves in layout `fmt` (matrix + x + y)."""
+        b = _i64()
+        _check(hip_lib.phg_hip_mat_format_bytes(self._h, fmt, int(fp32), C.byref(b)),
+               "mat_format_bytes")
+        return b.value
+
     def tune(self, reps: int = 10) -> dict:
         """Times the SpMV layouts and keeps the fastest per precision.
         Returns {layout: (fp64_us, fp32_us)} and the choice."""
@@ -444,3 +455,15 @@ def pcg_mixed(A: Mat, b: Vec, x: Vec, maxit=100, rtol=1e-10, btol=1e-10, atol=0.
                                              inner_rtol, C.byref(inner), C.byref(res)),
                    "pcg_mixed")
     return outer, inner.value, res.value
+
+
+def pcg_reliable(A: Mat, b: Vec, x: Vec, maxit=100000, rtol=1e-10, btol=1e-10,
+                 atol=0.0, delta=0.1):
+    """FP32 PCG with reliable FP64 residual updates.
+    Returns (fp32_iterations, fp64_residual_updates, residual)."""
+    res = _dbl()
+    upd = _i32()
+    nits = _check(hip_lib.phg_hip_pcg_reliable(A._h, b._h, x._h, maxit, rtol, btol,
+                                               atol, delta, C.byref(upd),
+                                               C.byref(res)), "pcg_reliable")
+    return nits, upd.value, res.value

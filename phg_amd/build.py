@@ -38,6 +38,7 @@ HIP_UNITS = {
     "assemble.hip": ["-ffp-contract=off"],
     "krylov.hip": [],
     "reorder.hip": [],
+    "dictcsr.hip": [],
 }
 
 

@@ -119,7 +119,13 @@ struct State {
     double rz, rz_prev, res, tol, rb, r0;
     double rtol, btol, atol;
     int32_t nits, done, maxit, xnz;
+    // reliable-update solver: largest FP32 residual since the last FP64
+    // residual, update threshold, slot kind, number of FP64 residuals
+    double rmax, delta;
+    int32_t mode, updates;
 };
+// slot kinds of the reliable-update solver
+constexpr int RU_START = 0, RU_ITER = 1, RU_UPDATE = 2;
 constexpr int RUNNING = 0, CONVERGED = 1, BREAKDOWN = 2, ZERO_DIAG = 3;
 
 // IsZero (utils.h:451)
@@ -409,6 +415,160 @@ __device__ inline void spmv_apply(const SellView &m, const T *__restrict__ va,
         if (i < m.n)
             epi((int32_t)i, acc);
     }
+}
+
+// Dictionary-compressed CSR stream (layout built in dictcsr.hip).  A
+// work-group takes a contiguous range of blocks.  Per block each work-item
+// loads 8 consecutive 16-bit indices and 8 values with 16-byte loads (two
+// groups of 8 per work-item for a 4096-entry block), decodes the columns
+// through the block's dictionary in LDS, gathers x, stores the products in
+// LDS, and one work-item per row sums its products in storage order.  The
+// loads of the next block are issued before the products of the current
+// one are formed, so the matrix stream stays in flight across the LDS
+// barriers and the row sums.
+struct DictView {
+    const int32_t *rp, *brow, *bent, *bdict, *dict;
+    const uint16_t *idx;
+    int64_t nblk;
+};
+
+template <typename T> struct Vals8 { T v[8]; };
+
+template <typename T>
+__device__ inline Vals8<T> load8(const T *p)
+{
+    Vals8<T> r;
+    if (sizeof(T) == 4) {
+        const float4 a = ((const float4 *)p)[0], b = ((const float4 *)p)[1];
+        r.v[0] = a.x; r.v[1] = a.y; r.v[2] = a.z; r.v[3] = a.w;
+        r.v[4] = b.x; r.v[5] = b.y; r.v[6] = b.z; r.v[7] = b.w;
+    }
+    else {
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const double2 a = ((const double2 *)p)[k];
+            r.v[2 * k] = a.x;
+            r.v[2 * k + 1] = a.y;
+        }
+    }
+    return r;
+}
+
+template <typename T> struct DictStage {
+    static constexpr int NCH = phg::DICT_BLOCK_NNZ / (BS * 8);
+    uint4 ib[NCH];
+    Vals8<T> vb[NCH];
+    int32_t d, f0, f1, q0, q1, s0, s1;
+};
+
+template <typename T>
+__device__ inline void dict_fetch(const DictView &m, const T *__restrict__ va,
+                                  int64_t b, DictStage<T> &st)
+{
+    const int tid = threadIdx.x;
+    st.f0 = m.bent[b];
+    st.f1 = m.bent[b + 1];
+    st.q0 = m.brow[b];
+    st.q1 = m.brow[b + 1];
+#pragma unroll
+    for (int k = 0; k < DictStage<T>::NCH; k++) {
+        const int32_t e = st.f0 + (tid + BS * k) * 8;
+        if (e < st.f1) {
+            st.ib[k] = *(const uint4 *)(m.idx + e);
+            st.vb[k] = load8(va + e);
+        }
+    }
+    const int32_t d0 = m.bdict[b], nd = m.bdict[b + 1] - d0;
+    st.d = tid < nd ? m.dict[d0 + tid] : 0;
+    const int32_t r = st.q0 + tid;
+    if (r < st.q1) {
+        st.s0 = m.rp[r];
+        st.s1 = m.rp[r + 1];
+    }
+}
+
+template <typename T>
+__device__ inline void store8(T *p, const T (&v)[8])
+{
+    if (sizeof(T) == 4) {
+        ((float4 *)p)[0] = make_float4(v[0], v[1], v[2], v[3]);
+        ((float4 *)p)[1] = make_float4(v[4], v[5], v[6], v[7]);
+    }
+    else {
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            ((double2 *)p)[k] = make_double2(v[2 * k], v[2 * k + 1]);
+    }
+}
+
+// PF = 1: the next block's stream is loaded into registers while the current
+// block is reduced; PF = 0: no register stage (fewer VGPRs, more waves)
+template <int PF, typename T, typename Epi>
+__device__ inline void dict_spmv(const DictView &m, const T *__restrict__ va,
+                                 const T *__restrict__ x, Epi epi)
+{
+    constexpr int NCH = DictStage<T>::NCH;
+    static_assert(phg::DICT_BLOCK_ROWS <= BS, "one work-item per row");
+    __shared__ __align__(16) T prod[phg::DICT_BLOCK_NNZ];
+    __shared__ int32_t dsh[256];
+    int64_t b0, b1;
+    row_range(m.nblk, b0, b1);
+    if (b0 >= b1)
+        return;
+    const int tid = threadIdx.x;
+    DictStage<T> cur, nxt;
+    dict_fetch(m, va, b0, cur);
+    for (int64_t b = b0; b < b1; b++) {
+        if (!PF && b > b0)
+            dict_fetch(m, va, b, cur);
+        dsh[tid] = cur.d;
+        lds_barrier();
+        T xg[NCH][8];
+#pragma unroll
+        for (int k = 0; k < NCH; k++) {
+            const int32_t e = cur.f0 + (tid + BS * k) * 8;
+            if (e < cur.f1) {
+                const uint32_t w[4] = {cur.ib[k].x, cur.ib[k].y, cur.ib[k].z,
+                                       cur.ib[k].w};
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    const uint32_t u = (w[j >> 1] >> (16 * (j & 1))) & 0xffffu;
+                    xg[k][j] = x[dsh[u >> 8] + (int32_t)(u & 255u)];
+                }
+            }
+        }
+        if (PF && b + 1 < b1)
+            dict_fetch(m, va, b + 1, nxt);
+#pragma unroll
+        for (int k = 0; k < NCH; k++) {
+            const int32_t e = cur.f0 + (tid + BS * k) * 8;
+            if (e < cur.f1) {
+                T pr[8];
+#pragma unroll
+                for (int j = 0; j < 8; j++)
+                    pr[j] = cur.vb[k].v[j] * xg[k][j];
+                store8(prod + (e - cur.f0), pr);
+            }
+        }
+        lds_barrier();
+        if (tid < cur.q1 - cur.q0) {
+            T sum = 0;
+            for (int32_t j = cur.s0 - cur.f0; j < cur.s1 - cur.f0; j++)
+                sum += prod[j];
+            epi(cur.q0 + tid, sum);
+        }
+        if (PF)
+            cur = nxt;
+    }
+}
+
+template <int PF> struct DictViewT : DictView {};
+template <typename T, int PF, typename Epi>
+__device__ inline void spmv_apply(const DictViewT<PF> &m,
+                                  const T *__restrict__ va,
+                                  const T *__restrict__ x, Epi epi)
+{
+    dict_spmv<PF, T>(m, va, x, epi);
 }
 
 // SELL construction: entries of permuted row i (original row perm[i]) with
@@ -751,6 +911,242 @@ __global__ void __launch_bounds__(BS) k_ir_correct(int64_t n, double *x,
         x[i] += s * (double)d[i];
 }
 
+//------------------------------------------ reliable-update mixed precision
+// FP32 PCG whose residual is replaced by the FP64 residual b - A x (x kept
+// in FP64) whenever the FP32 residual has dropped by the factor delta below
+// its largest value since the previous replacement, or has reached the
+// tolerance.  The search direction is kept across replacements, so the
+// Krylov recurrence is not restarted; convergence is only declared on an
+// FP64 residual.  One slot of launches per step:
+//   k_ru_p       decides the slot kind; ITER: p = z + beta p;
+//                UPDATE: x += x32, x32 = 0
+//   k_ru_spmv    ITER only: Ap = A32 p, partials of (p, Ap)
+//   k_ru_update  ITER only: x32 += alpha p, r32 -= alpha Ap, partials
+//   k_ru_resid   UPDATE only: r32 = b - A x in FP64, partials
+// k_ru_p of the next slot reads the partials of the last kernel that ran
+// (grid g after ITER, grid gs after UPDATE or the prologue).
+
+// prologue: r32 = b - A x (or b when x is zero), x32 = 0, partials of
+// (r,r), (r,z), #zero diagonals computed from the FP64 residual
+template <typename View>
+__global__ void __launch_bounds__(BS) k_ru_init(
+    int64_t n, View mv, const double *va, const double *diag, const double *b,
+    double *x, float *r32, float *x32, const double *part_in, int np_in,
+    double *part_out, State *st0)
+{
+    double nb[2];
+    sum_partials<2>(part_in, np_in, nb);
+    const double rb = sqrt(nb[0]);
+    const bool bzero = is_zero(rb);
+    const bool xnz = !is_zero(nb[1] / (double)n);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st0->rb = rb;
+        st0->xnz = xnz;
+    }
+    double v[3] = {0.0, 0.0, 0.0};
+    auto acc = [&](int64_t row, double ri) {
+        r32[row] = (float)ri;
+        x32[row] = 0.0f;
+        double d = diag[row];
+        v[0] += ri * ri;
+        if (d == 0.0)
+            v[2] += 1.0;
+        else
+            v[1] += ri * (ri / d);
+    };
+    if (xnz && !bzero) {
+        spmv_apply<double>(mv, va, x,
+                           [&](int32_t row, double s) { acc(row, -s + b[row]); });
+    }
+    else {
+        for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
+             i += (int64_t)gridDim.x * BS) {
+            if (bzero)
+                x[i] = 0.0;
+            acc(i, bzero ? 0.0 : b[i]);
+        }
+    }
+    write_partials<3>(v, part_out);
+}
+
+__global__ void __launch_bounds__(BS) k_ru_p(int64_t n, const float *r32,
+                                             const float *diag32, float *p32,
+                                             double *x, float *x32,
+                                             const double *part_in, int np_iter,
+                                             int np_full, const State *sin,
+                                             State *sout)
+{
+    const State s = *sin;
+    const bool w0 = blockIdx.x == 0 && threadIdx.x == 0;
+    if (s.done) {
+        if (w0)
+            *sout = s;
+        return;
+    }
+    double t[3];
+    sum_partials<3>(part_in, s.mode == RU_ITER ? np_iter : np_full, t);
+    const double res = sqrt(t[0]), rz = t[1];
+    State o = s;
+    o.res = res;
+    int kind = RU_ITER;           // what this slot does
+    double beta = 0.0;
+    bool first = false;
+    if (s.mode == RU_START) {
+        // r0 = b - A x in FP64: tolerance as get_tol (solver-phg.c:104)
+        if (is_zero(s.rb)) {
+            o.res = 0.0;
+            o.done = CONVERGED;
+        }
+        else {
+            double tol = res * s.rtol;
+            if (tol < s.rb * s.btol)
+                tol = s.rb * s.btol;
+            if (tol < s.atol)
+                tol = s.atol;
+            o.tol = tol;
+            o.r0 = res;
+            o.rmax = res;
+            if (!(res > tol && s.maxit > 0))
+                o.done = CONVERGED;
+            else if (t[2] != 0.0)
+                o.done = ZERO_DIAG;
+            else {
+                o.nits = 1;
+                o.rz = rz;
+                first = true;
+            }
+        }
+    }
+    else if (s.mode == RU_UPDATE) {
+        // the residual was just replaced by the FP64 one
+        o.rmax = res;
+        if (!(res > s.tol))
+            o.done = CONVERGED;
+        else if (s.nits >= s.maxit)
+            o.done = CONVERGED;
+        else if (s.rz == 0.0)
+            o.done = BREAKDOWN;
+        else {
+            o.nits = s.nits + 1;
+            o.rz = rz;
+            beta = rz / s.rz;
+        }
+    }
+    else {
+        if (!(res > s.tol) || res < s.delta * s.rmax || s.nits >= s.maxit) {
+            kind = RU_UPDATE;       // fold x32 into x, then FP64 residual
+            o.updates = s.updates + 1;
+        }
+        else if (s.rz == 0.0)
+            o.done = BREAKDOWN;
+        else {
+            o.nits = s.nits + 1;
+            o.rz = rz;
+            o.rmax = res > s.rmax ? res : s.rmax;
+            beta = rz / s.rz;
+        }
+    }
+    o.mode = kind;
+    if (w0)
+        *sout = o;
+    if (o.done)
+        return;
+    if (kind == RU_UPDATE) {
+        for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
+             i += (int64_t)gridDim.x * BS) {
+            x[i] += (double)x32[i];
+            x32[i] = 0.0f;
+        }
+        return;
+    }
+    const float tb = (float)beta;
+    for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * BS) {
+        float z = r32[i] / diag32[i];
+        p32[i] = first ? z : tb * p32[i] + z;
+    }
+}
+
+template <typename View>
+__global__ void __launch_bounds__(BS) k_ru_spmv(View mv, const float *va,
+                                                const float *p, float *Ap,
+                                                double *part_out,
+                                                const State *st)
+{
+    if (st->done || st->mode != RU_ITER)
+        return;
+    double v[1] = {0.0};
+    spmv_apply<float>(mv, va, p, [&](int32_t row, float s) {
+        Ap[row] = s;
+        v[0] += (double)p[row] * (double)s;
+    });
+    write_partials<1>(v, part_out);
+}
+
+__global__ void __launch_bounds__(BS) k_ru_update(int64_t n, float *x32,
+                                                  const float *p, float *r,
+                                                  const float *Ap,
+                                                  const float *diag,
+                                                  const double *part_in,
+                                                  int np_in, double *part_out,
+                                                  State *st)
+{
+    if (st->done || st->mode != RU_ITER)
+        return;
+    double pap[1];
+    sum_partials<1>(part_in, np_in, pap);
+    if (pap[0] == 0.0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0)
+            st->done = BREAKDOWN;
+        return;
+    }
+    const float alpha = (float)(st->rz / pap[0]);
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * BS) {
+        x32[i] += alpha * p[i];
+        float ri = r[i] - alpha * Ap[i];
+        r[i] = ri;
+        float z = ri / diag[i];
+        v[0] += (double)ri * (double)ri;
+        v[1] += (double)ri * (double)z;
+    }
+    write_partials<3>(v, part_out);
+}
+
+template <typename View>
+__global__ void __launch_bounds__(BS) k_ru_resid(View mv, const double *va,
+                                                 const double *diag,
+                                                 const double *b,
+                                                 const double *x, float *r32,
+                                                 double *part_out,
+                                                 const State *st)
+{
+    if (st->done || st->mode != RU_UPDATE)
+        return;
+    double v[3] = {0.0, 0.0, 0.0};
+    spmv_apply<double>(mv, va, x, [&](int32_t row, double s) {
+        double ri = b[row] - s;
+        r32[row] = (float)ri;
+        double d = diag[row];
+        v[0] += ri * ri;
+        if (d == 0.0)
+            v[2] += 1.0;
+        else
+            v[1] += ri * (ri / d);
+    });
+    write_partials<3>(v, part_out);
+}
+
+// x += x32 (final fold of the FP32 increment)
+__global__ void __launch_bounds__(BS) k_ru_fold(int64_t n, double *x,
+                                                const float *x32)
+{
+    for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * BS)
+        x[i] += (double)x32[i];
+}
+
 // plain copy, the reference point for bandwidth and HBM-counter scaling
 __global__ void __launch_bounds__(BS) k_copy(int64_t n, const double *a,
                                              double *b)
@@ -778,6 +1174,8 @@ enum {
     FMT_B4K_PAR,      // CsrBlockView<4096, 2>
     FMT_WAVE,         // CsrWaveView
     FMT_SELL,         // SellView
+    FMT_DICT,         // DictViewT<1>
+    FMT_DICT_NP,      // DictViewT<0>
     FMT_N
 };
 using CsrB2K = CsrBlockView<2048, 0>;
@@ -795,6 +1193,8 @@ int with_format(int fmt, Fn &&f)
     case FMT_B2K_PAR: return f(CsrB2KPar{});
     case FMT_B4K_PAR: return f(CsrB4KPar{});
     case FMT_WAVE: return f(CsrWaveView{});
+    case FMT_DICT: return f(DictViewT<1>{});
+    case FMT_DICT_NP: return f(DictViewT<0>{});
     default: return f(SellView{});
     }
 }
@@ -837,6 +1237,21 @@ CsrWaveView view_of(const phg_hip_mat *A, CsrWaveView *)
 {
     return CsrWaveView{A->q_rowptr, A->q_cols, A->q_wblk, A->q_nwblk};
 }
+template <int PF>
+DictViewT<PF> view_of(const phg_hip_mat *A, DictViewT<PF> *)
+{
+    DictViewT<PF> v;
+    v.rp = A->c_rowptr;
+    v.brow = A->c_blk_row;
+    v.bent = A->c_blk_ent;
+    v.bdict = A->c_blk_dict;
+    v.dict = A->c_dict;
+    v.idx = A->c_idx;
+    v.nblk = A->c_nblk;
+    return v;
+}
+template <int PF>
+int64_t units_of(const phg_hip_mat *A, DictViewT<PF> *) { return A->c_nblk; }
 SellView view_of(const phg_hip_mat *A, SellView *)
 {
     return SellView{A->d_sptr, A->d_swidth, A->d_scols, A->nslice, A->n};
@@ -869,6 +1284,13 @@ int grid_spmv(const phg_hip_mat *A)
 template <typename View> struct FmtData {
     static const double *v64(const phg_hip_mat *A) { return A->q_vals; }
     static const float *v32(const phg_hip_mat *A) { return A->q_vals32; }
+    static const double *d64(const phg_hip_mat *A) { return A->q_diag; }
+    static const float *d32(const phg_hip_mat *A) { return A->q_diag32; }
+    static const int32_t *perm(const phg_hip_mat *A) { return A->q_perm; }
+};
+template <int PF> struct FmtData<DictViewT<PF>> {
+    static const double *v64(const phg_hip_mat *A) { return A->c_val64; }
+    static const float *v32(const phg_hip_mat *A) { return A->c_val32; }
     static const double *d64(const phg_hip_mat *A) { return A->q_diag; }
     static const float *d32(const phg_hip_mat *A) { return A->q_diag32; }
     static const int32_t *perm(const phg_hip_mat *A) { return A->q_perm; }
@@ -1187,6 +1609,118 @@ int mixed_impl(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x,
     return outer;
 }
 
+// FP32 PCG with reliable FP64 residual updates, in the numbering of the
+// format; returns the number of FP32 iterations
+template <typename View>
+int reliable_impl(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x,
+                  int32_t maxit, double rtol, double btol, double atol,
+                  double delta, int32_t *updates, double *residual)
+{
+    using F = FmtData<View>;
+    hipStream_t s = phg::ctx()->stream;
+    Workspace *ws = get_ws(A, 1);
+    if (ws == nullptr)
+        return phg::set_error(PHG_HIP_E_NOMEM, "workspace allocation failed");
+    const int64_t n = A->n;
+    int g = grid_for(n), gs = grid_spmv<View>(A);
+    const View mv = view_of(A, (View *)nullptr);
+    State h;
+    memset(&h, 0, sizeof(h));
+    h.rtol = rtol;
+    h.btol = btol;
+    h.atol = atol;
+    h.maxit = maxit;
+    h.delta = delta;
+    h.mode = RU_START;
+    PHG_HIP_CHECK(hipMemcpyAsync(ws->st, &h, sizeof(State),
+                                 hipMemcpyHostToDevice, s));
+    const double *bb = b->d;
+    double *xx = x->d;
+    const int32_t *perm = F::perm(A);
+    if (perm) {
+        hipLaunchKernelGGL(k_gather, dim3(g), dim3(BS), 0, s, n, perm,
+                           (const double *)b->d, ws->bp);
+        hipLaunchKernelGGL(k_gather, dim3(g), dim3(BS), 0, s, n, perm,
+                           (const double *)x->d, ws->xp);
+        bb = ws->bp;
+        xx = ws->xp;
+    }
+    double *pres = ws->seg(SEG_RES), *ps = ws->seg(SEG_S);
+    hipLaunchKernelGGL(k_pcg_init_norms, dim3(g), dim3(BS), 0, s, n, bb,
+                       (const double *)xx, ws->seg(SEG_NORMS));
+    hipLaunchKernelGGL((k_ru_init<View>), dim3(gs), dim3(BS), 0, s, n, mv,
+                       F::v64(A), F::d64(A), bb, xx, ws->r32, ws->x32,
+                       (const double *)ws->seg(SEG_NORMS), g, pres, ws->st);
+    PHG_HIP_CHECK(hipGetLastError());
+    State *hm = (State *)phg::ctx()->host_mailbox;
+    const bool prof = g_prof.on;
+    int64_t k = 0;
+    int chunk = prof ? 1 : 16, rc;
+    for (;;) {
+        for (int i = 0; i < chunk; i++, k++) {
+            State *sin = ws->st + (k & 1), *sout = ws->st + ((k + 1) & 1);
+            if (prof) prof_mark(KP_P32, s);
+            hipLaunchKernelGGL(k_ru_p, dim3(g), dim3(BS), 0, s, n,
+                               (const float *)ws->r32, F::d32(A), ws->p32, xx,
+                               ws->x32, (const double *)pres, g, gs,
+                               (const State *)sin, sout);
+            if (prof) { prof_mark(KP_P32, s); prof_mark(KP_SPMV32, s); }
+            hipLaunchKernelGGL((k_ru_spmv<View>), dim3(gs), dim3(BS), 0, s, mv,
+                               F::v32(A), (const float *)ws->p32, ws->Ap32, ps,
+                               (const State *)sout);
+            if (prof) { prof_mark(KP_SPMV32, s); prof_mark(KP_UPD32, s); }
+            hipLaunchKernelGGL(k_ru_update, dim3(g), dim3(BS), 0, s, n,
+                               ws->x32, (const float *)ws->p32, ws->r32,
+                               (const float *)ws->Ap32, F::d32(A),
+                               (const double *)ps, gs, pres, sout);
+            if (prof) { prof_mark(KP_UPD32, s); prof_mark(KP_SPMV64, s); }
+            hipLaunchKernelGGL((k_ru_resid<View>), dim3(gs), dim3(BS), 0, s,
+                               mv, F::v64(A), F::d64(A), bb,
+                               (const double *)xx, ws->r32, pres,
+                               (const State *)sout);
+            if (prof) prof_mark(KP_SPMV64, s);
+        }
+        PHG_HIP_CHECK(hipGetLastError());
+        if ((rc = read_state(ws->st + (k & 1), hm, s)))
+            return rc;
+        if (prof) {
+            // keep the timing of the kernels that did the work of the slot
+            const int drop[3] = {hm->mode == RU_ITER ? KP_SPMV64 : KP_SPMV32,
+                                 hm->mode == RU_ITER ? -1 : KP_UPD32, -1};
+            for (int d : drop)
+                if (d >= 0 && !hm->done)
+                    g_prof.used[d] -= 2;
+            if (hm->done)
+                for (int d : {KP_P32, KP_SPMV32, KP_UPD32, KP_SPMV64})
+                    g_prof.used[d] -= 2;
+        }
+        if (hm->done != RUNNING)
+            break;
+        if (k > 2 * (int64_t)maxit + 4096)
+            return phg::set_error(PHG_HIP_E_HIP, "PCG state did not advance");
+        if (!prof)
+            chunk = std::min(chunk * 2, 64);
+    }
+    if (prof)
+        prof_collect();
+    State fin = *hm;
+    hipLaunchKernelGGL(k_ru_fold, dim3(g), dim3(BS), 0, s, n, xx,
+                       (const float *)ws->x32);
+    if (perm)
+        hipLaunchKernelGGL(k_scatter, dim3(g), dim3(BS), 0, s, n, perm,
+                           (const double *)ws->xp, x->d);
+    PHG_HIP_CHECK(hipGetLastError());
+    PHG_HIP_CHECK(hipStreamSynchronize(s));
+    if (updates)
+        *updates = fin.updates;
+    if (residual)
+        *residual = fin.res;
+    if (fin.done == ZERO_DIAG)
+        return phg::set_error(PHG_HIP_E_ZERO_DIAG,
+                              "zero diagonal in Jacobi preconditioner");
+    return fin.nits;
+}
+
 // time `reps` launches of the PCG SpMV kernel of one format and precision
 template <typename T, typename View>
 int time_spmv(phg_hip_mat *A, const T *va, int reps, double *us)
@@ -1417,6 +1951,13 @@ int ensure_format(phg_hip_mat *A, int fmt, int need32)
     int rc;
     if (fmt == FMT_SELL && A->d_svals == nullptr && (rc = phg::build_sell(A)))
         return rc;
+    if (fmt == FMT_DICT || fmt == FMT_DICT_NP) {
+        if ((rc = phg::build_dict(A)))
+            return rc;
+        if (A->dict_state != 1)
+            return phg::set_error(PHG_HIP_E_ARG,
+                                  "matrix has no dictionary-compressed layout");
+    }
     if (need32 && (rc = phg::ensure_fp32_copy(A)))
         return rc;
     return PHG_HIP_OK;
@@ -1532,6 +2073,27 @@ extern "C" int phg_hip_pcg_mixed(const phg_hip_mat *A_, const phg_hip_vec *b,
     });
 }
 
+extern "C" int phg_hip_pcg_reliable(const phg_hip_mat *A_,
+                                    const phg_hip_vec *b, phg_hip_vec *x,
+                                    int32_t maxit, double rtol, double btol,
+                                    double atol, double delta,
+                                    int32_t *updates, double *residual)
+{
+    int rc;
+    if ((rc = check_solve_args(A_, b, x)))
+        return rc;
+    if (!(delta > 0.0 && delta < 1.0))
+        return phg::set_error(PHG_HIP_E_ARG, "delta must lie in (0, 1)");
+    phg_hip_mat *A = const_cast<phg_hip_mat *>(A_);
+    if ((rc = ensure_format(A, A->fmt32, 1)))
+        return rc;
+    return with_format(A->fmt32, [&](auto tag) {
+        using View = decltype(tag);
+        return reliable_impl<View>(A, b, x, maxit, rtol, btol, atol, delta,
+                                   updates, residual);
+    });
+}
+
 extern "C" int phg_hip_mat_set_format(phg_hip_mat *A, int32_t fmt64,
                                       int32_t fmt32)
 {
@@ -1567,10 +2129,14 @@ extern "C" int phg_hip_mat_tune(phg_hip_mat *A, int32_t reps)
         return phg::set_error(PHG_HIP_E_ARG, "phg_hip_mat_tune: matrix not assembled");
     if (reps <= 0)
         reps = 10;
-    if ((rc = ensure_format(A, FMT_SELL, 1)))
+    if ((rc = ensure_format(A, FMT_SELL, 1)) || (rc = phg::build_dict(A)))
         return rc;
     for (int f = 0; f < FMT_N; f++) {
         double t64 = 0, t32 = 0;
+        if ((f == FMT_DICT || f == FMT_DICT_NP) && A->dict_state != 1) {
+            A->tune_us[f][0] = A->tune_us[f][1] = -1.0;   // not representable
+            continue;
+        }
         rc = with_format(f, [&](auto tag) {
             using View = decltype(tag);
             using F = FmtData<View>;
@@ -1587,6 +2153,8 @@ extern "C" int phg_hip_mat_tune(phg_hip_mat *A, int32_t reps)
     }
     int b64 = 0, b32 = 0;
     for (int f = 1; f < FMT_N; f++) {
+        if (A->tune_us[f][0] < 0.0)
+            continue;
         if (A->tune_us[f][0] < A->tune_us[b64][0])
             b64 = f;
         if (A->tune_us[f][1] < A->tune_us[b32][1])
@@ -1594,8 +2162,39 @@ extern "C" int phg_hip_mat_tune(phg_hip_mat *A, int32_t reps)
     }
     A->fmt64 = b64;
     A->fmt32 = b32;
+    // copies no solver uses are released
     if (b64 != FMT_SELL && b32 != FMT_SELL)
-        phg::free_sell(A);    // the permuted copy is not needed
+        phg::free_sell(A);
+    if (b64 != FMT_DICT && b32 != FMT_DICT && b64 != FMT_DICT_NP &&
+        b32 != FMT_DICT_NP)
+        phg::free_dict(A);
+    return PHG_HIP_OK;
+}
+
+extern "C" int phg_hip_mat_format_bytes(const phg_hip_mat *A, int32_t fmt,
+                                        int32_t fp32, int64_t *bytes)
+{
+    if (A == nullptr || fmt < 0 || fmt >= FMT_N || bytes == nullptr ||
+        !A->eliminated)
+        return phg::set_error(PHG_HIP_E_ARG, "phg_hip_mat_format_bytes: bad args");
+    const int64_t sv = fp32 ? 4 : 8, n = A->n;
+    const int64_t vec = 2 * n * sv;            // one read of x, one write of y
+    switch (fmt) {
+    case FMT_SELL:
+        if (A->d_svals == nullptr)
+            return phg::set_error(PHG_HIP_E_STATE, "SELL layout not built");
+        *bytes = A->sell_nnz * (sv + 4) + A->nslice * 12 + vec;
+        break;
+    case FMT_DICT:
+    case FMT_DICT_NP:
+        if (A->dict_state != 1)
+            return phg::set_error(PHG_HIP_E_STATE, "dictionary layout not built");
+        *bytes = A->c_nent * (sv + 2) + (n + 1) * 4 + (A->c_nblk + 1) * 12 +
+                 A->c_ndict * 4 + vec;
+        break;
+    default:
+        *bytes = A->nnz * (sv + 4) + (n + 1) * 4 + vec;
+    }
     return PHG_HIP_OK;
 }
 

@@ -91,7 +91,7 @@ struct phg_hip_mat {
     // SpMV storage format used by the solvers, per precision, and the times
     // the autotuner measured for each format [fmt][0: FP64, 1: FP32] (us)
     int fmt64 = 0, fmt32 = 0;
-    double tune_us[6][2] = {};
+    double tune_us[8][2] = {};
     // SELL-64 copy built from the solver CSR: rows sorted by length inside
     // windows of 4096 rows (d_perm: SELL index -> PHG index, composed with
     // q_perm), slices of 64 rows stored column-major, padded with zeros
@@ -105,6 +105,20 @@ struct phg_hip_mat {
     double *d_sdiag = nullptr;       // permuted diagonal
     float *d_sdiag32 = nullptr;
     int64_t sell_nnz = 0;            // stored entries including padding
+    // Dictionary-compressed CSR of the solver matrix (dictcsr.hip): blocks
+    // of rows, per block a dictionary of <= 256 column-chunk bases, 16-bit
+    // entries slot << 8 | column & 255; entries of a block start at a
+    // multiple of 8 (padding: index 0, value 0)
+    int dict_state = 0;              // 0 not built, 1 built, -1 not representable
+    int64_t c_nblk = 0, c_nent = 0, c_ndict = 0;
+    int32_t *c_rowptr = nullptr;     // [n+1] entry positions (padded)
+    int32_t *c_blk_row = nullptr;    // [nblk+1] first row of each block
+    int32_t *c_blk_ent = nullptr;    // [nblk+1] first entry of each block
+    int32_t *c_blk_dict = nullptr;   // [nblk+1] first dictionary slot
+    int32_t *c_dict = nullptr;       // chunk bases (multiples of 256)
+    uint16_t *c_idx = nullptr;
+    float *c_val32 = nullptr;
+    double *c_val64 = nullptr;
 };
 
 namespace phg {
@@ -112,6 +126,7 @@ namespace phg {
 constexpr int SPMV_BLOCK_NNZ = 2048, SPMV_BLOCK_ROWS = 256;
 constexpr int SPMV_WAVE_NNZ = 512, SPMV_WAVE_ROWS = 64;
 constexpr int SPMV_LARGE_NNZ = 4096;
+constexpr int DICT_BLOCK_NNZ = 4096, DICT_BLOCK_ROWS = 256;
 // Krylov building blocks (krylov.hip) used by the C ABI layer.
 int spmv64(const phg_hip_mat *A, const double *x, double *y, double alpha,
            double beta, hipStream_t s);
@@ -124,5 +139,8 @@ int split_row_blocks(const int32_t *d_rowptr, int64_t n, int64_t nnz,
 int build_sell(phg_hip_mat *A);
 void free_sell(phg_hip_mat *A);
 int ensure_fp32_copy(phg_hip_mat *A);
+// dictionary-compressed CSR (dictcsr.hip)
+int build_dict(phg_hip_mat *A);
+void free_dict(phg_hip_mat *A);
 int setup_diagonal(phg_hip_mat *A);
 }  // namespace phg

@@ -15,6 +15,17 @@ import sys
 import numpy as np
 
 
+def _layout_bytes(pa, A):
+    """Bytes per SpMV of the layouts still held by the matrix."""
+    out = {}
+    for f, name in pa.FMT_NAMES.items():
+        try:
+            out[name] = [A.format_bytes(f, False), A.format_bytes(f, True)]
+        except pa.PhgHipError:
+            pass
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--n", type=int, default=64)
@@ -41,6 +52,7 @@ def main():
            "copy_doubles": copy_n, "copy_bytes": 16 * copy_n,
            "copy_us": us_copy, "copy_GBps": 16 * copy_n / us_copy / 1e3,
            "layouts_us": {k: v for k, v in t.items() if k != "chosen"},
+           "layout_bytes": _layout_bytes(pa, A),
            "chosen": t["chosen"]}
     print(json.dumps(out), flush=True)
     return 0

@@ -138,7 +138,7 @@ def test_pcg_matches_oracle(hip, kind, p, n):
     assert rel < 1e-9, rel
 
 
-@pytest.mark.parametrize("fmt", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("fmt", [0, 1, 2, 3, 4, 5, 6, 7])
 def test_pcg_every_spmv_layout(hip, fmt):
     """FP64 PCG and the mixed solver agree with the oracle on each layout."""
     pa = hip
@@ -156,6 +156,9 @@ def test_pcg_every_spmv_layout(hip, fmt):
     x2 = pa.Vec(A.n)
     pa.pcg_mixed(A, A.rhs(), x2, rtol=1e-11, btol=1e-11, inner_rtol=1e-4)
     assert np.linalg.norm(x2.numpy() - xo) / np.linalg.norm(xo) < 1e-8
+    x3 = pa.Vec(A.n)
+    pa.pcg_reliable(A, A.rhs(), x3, rtol=1e-11, btol=1e-11)
+    assert np.linalg.norm(x3.numpy() - xo) / np.linalg.norm(xo) < 1e-8
 
 
 def test_tuner_picks_measured_layout(hip):
@@ -251,4 +254,52 @@ def test_mixed_precision_refinement(hip):
                                      inner_rtol=1e-4)
     rel = np.linalg.norm(x.numpy() - xo) / np.linalg.norm(xo)
     assert outer >= 2 and inner > 0
-    assert rel < 1e-8, rel   # north-star bar: 1e-8 relative to the FP64 oracle
+    assert rel < 1e-8, rel   # 1e-8 relative to the FP64 oracle
+
+
+@pytest.mark.parametrize("delta", [0.1, 0.01])
+def test_reliable_update_solver(hip, delta):
+    """FP32 PCG with FP64 residual replacement: the stopping test runs on the
+    FP64 residual, so |b - A x| <= tol holds for the returned x, and the
+    recurrence is not restarted, so the iteration count stays close to FP64
+    PCG's."""
+    pa = hip
+    c = poisson_case(8, 2, "kappa", sigma=0.5)
+    pb = pa.Problem.poisson(use_kappa=True)
+    sp, A, Ao = _assemble_both(pa, c, pb)
+    A.eliminate()
+    orc.eliminate(Ao, c["bdry"])
+    b = Ao.rhs
+    xo, _, _ = orc.pcg(Ao, b, rtol=1e-14, btol=1e-14)
+    _, nits64, _ = orc.pcg(Ao, b, rtol=1e-10, btol=1e-10)
+    x = pa.Vec(A.n)
+    nits, upd, res = pa.pcg_reliable(A, A.rhs(), x, rtol=1e-10, btol=1e-10,
+                                     delta=delta)
+    xv = x.numpy()
+    r = b - orc.matvec(Ao, xv)
+    tol = 1e-10 * np.linalg.norm(b)        # r0 = b for x0 = 0
+    assert np.linalg.norm(r) <= tol * (1 + 1e-6), (np.linalg.norm(r), tol)
+    assert abs(res - np.linalg.norm(r)) <= 1e-3 * tol
+    assert upd >= 2
+    assert nits <= 1.3 * nits64 + 5, (nits, nits64)
+    assert np.linalg.norm(xv - xo) / np.linalg.norm(xo) < 1e-8
+
+
+def test_reliable_nonzero_guess_and_zero_rhs(hip):
+    pa = hip
+    c = poisson_case(4, 2, "kappa")
+    pb = pa.Problem.poisson(use_kappa=True)
+    sp, A, Ao = _assemble_both(pa, c, pb)
+    A.eliminate()
+    orc.eliminate(Ao, c["bdry"])
+    xo, _, _ = orc.pcg(Ao, Ao.rhs, rtol=1e-14, btol=1e-14)
+    x0 = np.random.default_rng(3).standard_normal(A.n)
+    x = pa.Vec(A.n, x0)
+    nits, upd, res = pa.pcg_reliable(A, A.rhs(), x, rtol=1e-11, btol=0.0)
+    assert nits > 0
+    assert np.linalg.norm(x.numpy() - xo) / np.linalg.norm(xo) < 1e-8
+    z = pa.Vec(A.n)
+    x2 = pa.Vec(A.n, x0)
+    nits, upd, res = pa.pcg_reliable(A, z, x2)
+    assert nits == 0 and res == 0.0
+    assert np.all(x2.numpy() == 0.0)
