@@ -182,6 +182,28 @@ __device__ inline T block_sum(T v)
     return s;
 }
 
+// Sum of p[s0 .. s1) in index order (the order of matvec.c:920-928); the
+// loads are issued in groups of eight so the LDS latency is paid once per
+// group rather than once per entry.
+template <typename T>
+__device__ inline T ordered_sum(const T *p, int32_t s0, int32_t s1)
+{
+    T sum = 0;
+    int32_t j = s0;
+    for (; j + 8 <= s1; j += 8) {
+        T v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            v[k] = p[j + k];
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            sum += v[k];
+    }
+    for (; j < s1; j++)
+        sum += p[j];
+    return sum;
+}
+
 // Row-block streaming SpMV, a family for the autotuner: CAP entries staged
 // per block; MODE 0 loads them in a loop unrolled by four, MODE 1 in one
 // batch of CAP / BS per work-item; in both one work-item sums a row in
@@ -196,14 +218,25 @@ __device__ inline void stream_spmv(const int32_t *__restrict__ rp,
                                    int64_t nrblk, Epi epi)
 {
     constexpr int PER = CAP / BS;
+    constexpr int MB = 64;     // blocks whose bounds are staged in LDS at once
     static_assert(CAP % BS == 0, "whole entries per work-item");
     __shared__ T prod[CAP];
+    __shared__ int32_t mrow[MB + 1], ment[MB + 1];
     int64_t b0, b1;
     row_range(nrblk, b0, b1);
     const int tid = threadIdx.x;
-    for (int64_t b = b0; b < b1; b++) {
-        const int32_t r0 = rblk[b], r1 = rblk[b + 1];
-        const int32_t j0 = rp[r0], j1 = rp[r1];
+    for (int64_t c0 = b0; c0 < b1; c0 += MB) {
+      const int nb = (int)(b1 - c0 < MB ? b1 - c0 : MB);
+      lds_barrier();
+      if (tid <= nb) {
+          const int32_t r = rblk[c0 + tid];
+          mrow[tid] = r;
+          ment[tid] = rp[r];
+      }
+      lds_barrier();
+      for (int i = 0; i < nb; i++) {
+        const int32_t r0 = mrow[i], r1 = mrow[i + 1];
+        const int32_t j0 = ment[i], j1 = ment[i + 1];
         if (j1 - j0 <= CAP) {
             if (MODE == 0) {
 #pragma unroll 4
@@ -232,10 +265,8 @@ __device__ inline void stream_spmv(const int32_t *__restrict__ rp,
                 if (tid < nrows) {
                     const int32_t r = r0 + tid;
                     const int32_t s0 = rp[r] - j0, s1 = rp[r + 1] - j0;
-                    T sum = 0;
-                    for (int32_t j = s0; j < s1; j++)
-                        sum += prod[j];
-                    epi(r, sum);
+                    T sum = ordered_sum(prod, s0, s1);
+                    epi(r, sum, x[r]);
                 }
             }
             else {
@@ -255,7 +286,7 @@ __device__ inline void stream_spmv(const int32_t *__restrict__ rp,
                 for (int o = lpr / 2; o > 0; o >>= 1)
                     sum += __shfl_xor(sum, o, 64);
                 if (row < nrows && sub == 0)
-                    epi(r, sum);
+                    epi(r, sum, x[r]);
             }
             lds_barrier();
         }
@@ -266,11 +297,14 @@ __device__ inline void stream_spmv(const int32_t *__restrict__ rp,
                 sum += va[j] * x[ci[j]];
             sum = block_sum<T>(sum);
             if (tid == 0)
-                epi(r0, sum);
+                epi(r0, sum, x[r0]);
         }
+      }
     }
 }
 
+// Every SpMV hands (row, sum, x[row]) to its epilogue; x[row] is what the
+// PCG kernels need for (p, Ap), and a layout may load it early.
 // Matrix views for the SpMV-shaped kernels.  CsrView streams PHG's CSR over
 // row blocks (vectors in the original numbering); SellView walks the SELL-64
 // copy (vectors in the solver's permuted numbering).
@@ -350,10 +384,8 @@ __device__ inline void spmv_apply(const CsrWaveView &m,
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             if (lane < nrows) {
-                T sum = 0;
-                for (int32_t j = rs - j0; j < re - j0; j++)
-                    sum += prod[j];
-                epi(r0 + lane, sum);
+                T sum = ordered_sum(prod, rs - j0, re - j0);
+                epi(r0 + lane, sum, x[r0 + lane]);
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -368,7 +400,7 @@ __device__ inline void spmv_apply(const CsrWaveView &m,
             for (int o = 32; o > 0; o >>= 1)
                 sum += __shfl_xor(sum, o, 64);
             if (lane == 0)
-                epi(r0, sum);
+                epi(r0, sum, x[r0]);
         }
     }
 }
@@ -413,7 +445,7 @@ __device__ inline void spmv_apply(const SellView &m, const T *__restrict__ va,
         }
         const int64_t i = sl * 64 + lane;
         if (i < m.n)
-            epi((int32_t)i, acc);
+            epi((int32_t)i, acc, x[i]);
     }
 }
 
@@ -459,17 +491,27 @@ template <typename T> struct DictStage {
     uint4 ib[NCH];
     Vals8<T> vb[NCH];
     int32_t d, f0, f1, q0, q1, s0, s1;
+    T xr;          // x of the work-item's row (for the epilogue)
+};
+
+// block metadata staged in LDS: entry start, first row, first dictionary
+// slot of blocks c0 .. c0 + DICT_META (one more for the ends)
+constexpr int DICT_META = 64;
+struct DictMeta {
+    int32_t ent[DICT_META + 1], row[DICT_META + 1], dic[DICT_META + 1];
 };
 
 template <typename T>
-__device__ inline void dict_fetch(const DictView &m, const T *__restrict__ va,
-                                  int64_t b, DictStage<T> &st)
+__device__ inline void dict_fetch(const DictView &m, const DictMeta &md,
+                                  const T *__restrict__ va,
+                                  const T *__restrict__ x, int i,
+                                  DictStage<T> &st)
 {
     const int tid = threadIdx.x;
-    st.f0 = m.bent[b];
-    st.f1 = m.bent[b + 1];
-    st.q0 = m.brow[b];
-    st.q1 = m.brow[b + 1];
+    st.f0 = md.ent[i];
+    st.f1 = md.ent[i + 1];
+    st.q0 = md.row[i];
+    st.q1 = md.row[i + 1];
 #pragma unroll
     for (int k = 0; k < DictStage<T>::NCH; k++) {
         const int32_t e = st.f0 + (tid + BS * k) * 8;
@@ -478,12 +520,13 @@ __device__ inline void dict_fetch(const DictView &m, const T *__restrict__ va,
             st.vb[k] = load8(va + e);
         }
     }
-    const int32_t d0 = m.bdict[b], nd = m.bdict[b + 1] - d0;
+    const int32_t d0 = md.dic[i], nd = md.dic[i + 1] - d0;
     st.d = tid < nd ? m.dict[d0 + tid] : 0;
     const int32_t r = st.q0 + tid;
     if (r < st.q1) {
         st.s0 = m.rp[r];
         st.s1 = m.rp[r + 1];
+        st.xr = x[r];
     }
 }
 
@@ -502,63 +545,74 @@ __device__ inline void store8(T *p, const T (&v)[8])
 }
 
 // PF = 1: the next block's stream is loaded into registers while the current
-// block is reduced; PF = 0: no register stage (fewer VGPRs, more waves)
+// block is reduced; PF = 0: no register stage (fewer VGPRs, more waves).
+// The metadata of up to DICT_META blocks is loaded into LDS in one step, so
+// the stream loads of a block wait on no global load of their own.
 template <int PF, typename T, typename Epi>
 __device__ inline void dict_spmv(const DictView &m, const T *__restrict__ va,
                                  const T *__restrict__ x, Epi epi)
 {
     constexpr int NCH = DictStage<T>::NCH;
     static_assert(phg::DICT_BLOCK_ROWS <= BS, "one work-item per row");
+    static_assert(DICT_META < BS, "one work-item per metadata entry");
     __shared__ __align__(16) T prod[phg::DICT_BLOCK_NNZ];
     __shared__ int32_t dsh[256];
+    __shared__ DictMeta md;
     int64_t b0, b1;
     row_range(m.nblk, b0, b1);
-    if (b0 >= b1)
-        return;
     const int tid = threadIdx.x;
-    DictStage<T> cur, nxt;
-    dict_fetch(m, va, b0, cur);
-    for (int64_t b = b0; b < b1; b++) {
-        if (!PF && b > b0)
-            dict_fetch(m, va, b, cur);
-        dsh[tid] = cur.d;
+    for (int64_t c0 = b0; c0 < b1; c0 += DICT_META) {
+        const int nb = (int)(b1 - c0 < DICT_META ? b1 - c0 : DICT_META);
+        lds_barrier();      // the previous chunk is done with md
+        if (tid <= nb) {
+            md.ent[tid] = m.bent[c0 + tid];
+            md.row[tid] = m.brow[c0 + tid];
+            md.dic[tid] = m.bdict[c0 + tid];
+        }
         lds_barrier();
-        T xg[NCH][8];
+        DictStage<T> cur, nxt;
+        dict_fetch(m, md, va, x, 0, cur);
+        for (int i = 0; i < nb; i++) {
+            if (!PF && i > 0)
+                dict_fetch(m, md, va, x, i, cur);
+            dsh[tid] = cur.d;
+            lds_barrier();
+            T xg[NCH][8];
 #pragma unroll
-        for (int k = 0; k < NCH; k++) {
-            const int32_t e = cur.f0 + (tid + BS * k) * 8;
-            if (e < cur.f1) {
-                const uint32_t w[4] = {cur.ib[k].x, cur.ib[k].y, cur.ib[k].z,
-                                       cur.ib[k].w};
+            for (int k = 0; k < NCH; k++) {
+                const int32_t e = cur.f0 + (tid + BS * k) * 8;
+                if (e < cur.f1) {
+                    const uint32_t w[4] = {cur.ib[k].x, cur.ib[k].y,
+                                           cur.ib[k].z, cur.ib[k].w};
 #pragma unroll
-                for (int j = 0; j < 8; j++) {
-                    const uint32_t u = (w[j >> 1] >> (16 * (j & 1))) & 0xffffu;
-                    xg[k][j] = x[dsh[u >> 8] + (int32_t)(u & 255u)];
+                    for (int j = 0; j < 8; j++) {
+                        const uint32_t u =
+                            (w[j >> 1] >> (16 * (j & 1))) & 0xffffu;
+                        xg[k][j] = x[dsh[u >> 8] + (int32_t)(u & 255u)];
+                    }
                 }
             }
-        }
-        if (PF && b + 1 < b1)
-            dict_fetch(m, va, b + 1, nxt);
+            if (PF && i + 1 < nb)
+                dict_fetch(m, md, va, x, i + 1, nxt);
 #pragma unroll
-        for (int k = 0; k < NCH; k++) {
-            const int32_t e = cur.f0 + (tid + BS * k) * 8;
-            if (e < cur.f1) {
-                T pr[8];
+            for (int k = 0; k < NCH; k++) {
+                const int32_t e = cur.f0 + (tid + BS * k) * 8;
+                if (e < cur.f1) {
+                    T pr[8];
 #pragma unroll
-                for (int j = 0; j < 8; j++)
-                    pr[j] = cur.vb[k].v[j] * xg[k][j];
-                store8(prod + (e - cur.f0), pr);
+                    for (int j = 0; j < 8; j++)
+                        pr[j] = cur.vb[k].v[j] * xg[k][j];
+                    store8(prod + (e - cur.f0), pr);
+                }
             }
+            lds_barrier();
+            if (tid < cur.q1 - cur.q0) {
+                T sum = ordered_sum(prod, cur.s0 - cur.f0, cur.s1 - cur.f0);
+                epi(cur.q0 + tid, sum, cur.xr);
+            }
+            if (PF)
+                cur = nxt;
         }
-        lds_barrier();
-        if (tid < cur.q1 - cur.q0) {
-            T sum = 0;
-            for (int32_t j = cur.s0 - cur.f0; j < cur.s1 - cur.f0; j++)
-                sum += prod[j];
-            epi(cur.q0 + tid, sum);
-        }
-        if (PF)
-            cur = nxt;
     }
 }
 
@@ -631,7 +685,7 @@ __global__ void __launch_bounds__(BS) k_spmv64(const int32_t *rp,
                                                double beta)
 {
     stream_spmv<double, NNZ_CAP, 0>(rp, ci, va, x, rblk, nrblk,
-                                    [&](int32_t row, double s) {
+                                    [&](int32_t row, double s, double xr) {
                             // matvec.c:608-633 beta first, then alpha * A x
                             double yo = beta == 0.0 ? 0.0 : beta * y[row];
                             y[row] = yo + alpha * s;
@@ -714,7 +768,7 @@ __global__ void __launch_bounds__(BS) k_pcg_init_res(
     };
     if (xnz && !bzero) {
         spmv_apply<double>(mv, va, x,
-                           [&](int32_t row, double s) { acc(row, -s + b[row]); });
+                           [&](int32_t row, double s, double xr) { acc(row, -s + b[row]); });
     }
     else {
         for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
@@ -814,9 +868,9 @@ __global__ void __launch_bounds__(BS) k_pcg_spmv(
     if (st->done)
         return;
     double v[1] = {0.0};
-    spmv_apply<T>(mv, va, p, [&](int32_t row, T s) {
+    spmv_apply<T>(mv, va, p, [&](int32_t row, T s, T xr) {
         Ap[row] = s;
-        v[0] += (double)p[row] * (double)s;
+        v[0] += (double)xr * (double)s;     // xr = p[row]
     });
     write_partials<1>(v, part_out);
 }
@@ -860,7 +914,7 @@ __global__ void __launch_bounds__(BS) k_ir_residual(
     double *part_out)
 {
     double v[2] = {0.0, 0.0};
-    spmv_apply<double>(mv, va, x, [&](int32_t row, double s) {
+    spmv_apply<double>(mv, va, x, [&](int32_t row, double s, double xr) {
         double bi = b[row], ri = bi - s;
         r[row] = ri;
         v[0] += ri * ri;
@@ -956,7 +1010,7 @@ __global__ void __launch_bounds__(BS) k_ru_init(
     };
     if (xnz && !bzero) {
         spmv_apply<double>(mv, va, x,
-                           [&](int32_t row, double s) { acc(row, -s + b[row]); });
+                           [&](int32_t row, double s, double xr) { acc(row, -s + b[row]); });
     }
     else {
         for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
@@ -1076,9 +1130,9 @@ __global__ void __launch_bounds__(BS) k_ru_spmv(View mv, const float *va,
     if (st->done || st->mode != RU_ITER)
         return;
     double v[1] = {0.0};
-    spmv_apply<float>(mv, va, p, [&](int32_t row, float s) {
+    spmv_apply<float>(mv, va, p, [&](int32_t row, float s, float xr) {
         Ap[row] = s;
-        v[0] += (double)p[row] * (double)s;
+        v[0] += (double)xr * (double)s;     // xr = p[row]
     });
     write_partials<1>(v, part_out);
 }
@@ -1125,7 +1179,7 @@ __global__ void __launch_bounds__(BS) k_ru_resid(View mv, const double *va,
     if (st->done || st->mode != RU_UPDATE)
         return;
     double v[3] = {0.0, 0.0, 0.0};
-    spmv_apply<double>(mv, va, x, [&](int32_t row, double s) {
+    spmv_apply<double>(mv, va, x, [&](int32_t row, double s, double xr) {
         double ri = b[row] - s;
         r32[row] = (float)ri;
         double d = diag[row];
@@ -1272,13 +1326,35 @@ int64_t units_of(const phg_hip_mat *A, SellView *)
 
 // For the wavefront stream the work-group range is taken over wavefront
 // blocks (waves interleave inside it), so the grid is sized by groups of 4.
-template <typename View>
-int grid_spmv(const phg_hip_mat *A)
+// The grid of an SpMV kernel is at most the number of work-groups the chip
+// holds at once for that kernel (its occupancy times the CU count), so every
+// work-group starts in the first wave of dispatch and walks its contiguous
+// range of blocks; a larger grid would run a second, partly empty round.
+int resident_groups(const void *kern)
+{
+    static std::vector<std::pair<const void *, int>> cache;
+    for (auto &c : cache)
+        if (c.first == kern)
+            return c.second;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, BS, 0) !=
+            hipSuccess ||
+        per_cu <= 0)
+        per_cu = 8;
+    int cus = phg::ctx()->num_cus > 0 ? phg::ctx()->num_cus : 256;
+    int g = std::min(per_cu * cus, MAX_GRID);
+    cache.emplace_back(kern, g);
+    return g;
+}
+
+template <typename View, typename K>
+int grid_spmv(const phg_hip_mat *A, K kern)
 {
     int64_t u = units_of(A, (View *)nullptr);
     if (std::is_same<View, CsrWaveView>::value)
         u /= BS / 64;
-    return (int)std::max<int64_t>(1, std::min<int64_t>(u, cap_grid()));
+    const int cap = resident_groups((const void *)kern);
+    return (int)std::max<int64_t>(1, std::min<int64_t>(u, cap));
 }
 
 template <typename View> struct FmtData {
@@ -1383,7 +1459,7 @@ void launch_iteration(const phg_hip_mat *A, const T *va, const T *diag,
                       int64_t k, int np_first, hipStream_t s)
 {
     int64_t n = A->n;
-    int g = grid_for(n), gs = grid_spmv<View>(A);
+    int g = grid_for(n), gs = grid_spmv<View>(A, k_pcg_spmv<T, View>);
     const int np_in = k == 0 ? np_first : g;
     State *sin = st + (k & 1), *sout = st + ((k + 1) & 1);
     double *pres = ws->seg(SEG_RES), *ps = ws->seg(SEG_S);
@@ -1485,7 +1561,7 @@ int pcg64_impl(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x,
     PHG_HIP_CHECK(hipMemcpyAsync(ws->st, &h, sizeof(State),
                                  hipMemcpyHostToDevice, s));
     const int64_t n = A->n;
-    int g = grid_for(n), gs = grid_spmv<View>(A);
+    int g = grid_for(n), gs = grid_spmv<View>(A, k_pcg_spmv<double, View>);
     const double *bb = b->d;
     double *xx = x->d;
     const int32_t *perm = F::perm(A);
@@ -1535,7 +1611,7 @@ int mixed_impl(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x,
     if (ws == nullptr)
         return phg::set_error(PHG_HIP_E_NOMEM, "workspace allocation failed");
     const int64_t n = A->n;
-    int g = grid_for(n), gs = grid_spmv<View>(A);
+    int g = grid_for(n), gs = grid_spmv<View>(A, k_pcg_spmv<float, View>);
     double *hm = (double *)phg::ctx()->host_mailbox;
     double tol = 0.0, res = 0.0;
     int total_inner = 0, outer = 0, rc;
@@ -1622,7 +1698,7 @@ int reliable_impl(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x,
     if (ws == nullptr)
         return phg::set_error(PHG_HIP_E_NOMEM, "workspace allocation failed");
     const int64_t n = A->n;
-    int g = grid_for(n), gs = grid_spmv<View>(A);
+    int g = grid_for(n), gs = grid_spmv<View>(A, k_ru_spmv<View>);
     const View mv = view_of(A, (View *)nullptr);
     State h;
     memset(&h, 0, sizeof(h));
@@ -1736,7 +1812,7 @@ int time_spmv(phg_hip_mat *A, const T *va, int reps, double *us)
     T *p = (T *)(sizeof(T) == 8 ? (void *)ws->p : (void *)ws->p32);
     T *Ap = (T *)(sizeof(T) == 8 ? (void *)ws->Ap : (void *)ws->Ap32);
     PHG_HIP_CHECK(hipMemsetAsync(p, 0, (size_t)A->n * sizeof(T), s));
-    int gs = grid_spmv<View>(A);
+    int gs = grid_spmv<View>(A, k_pcg_spmv<T, View>);
     hipEvent_t e0, e1;
     PHG_HIP_CHECK(hipEventCreate(&e0));
     PHG_HIP_CHECK(hipEventCreate(&e1));
@@ -1935,7 +2011,7 @@ int spmv64(const phg_hip_mat *A, const double *x, double *y, double alpha,
 {
     if (A->nrblk == 0)
         return PHG_HIP_OK;
-    hipLaunchKernelGGL(k_spmv64, dim3(grid_spmv<CsrView>(A)), dim3(BS), 0, s,
+    hipLaunchKernelGGL(k_spmv64, dim3(grid_spmv<CsrView>(A, k_spmv64)), dim3(BS), 0, s,
                        (const int32_t *)A->d_rowptr, (const int32_t *)A->d_cols,
                        (const double *)A->d_vals, (const int32_t *)A->d_rblk,
                        A->nrblk, x, y, alpha, beta);
