@@ -163,6 +163,29 @@ int phg_hip_pcg_reliable(const phg_hip_mat *A, const phg_hip_vec *b,
                          double btol, double atol, double delta,
                          int32_t *updates, double *residual);
 
+/*---------------------------------------------------------------- adjoint */
+
+/* Adjoint gradient (no reference counterpart; PHG has no adjoint).  For a
+ * scalar problem assembled with a P0 coefficient kappa (use_kappa = 1) and
+ * eliminated (A), the forward solution u and a target ud (both FP64, PHG
+ * numbering, Dirichlet values included), with
+ *     J = 1/2 \int (u_h - ud_h)^2 dx = 1/2 (u - ud)^T M (u - ud)
+ * (M: the Pk mass matrix with the rule of quad.c:5582 phgQuadBasDotBas):
+ *   - the adjoint right-hand side M (u - ud), zero on Dirichlet rows, is
+ *     accumulated in element order,
+ *   - the transposed system A^T lambda = M (u - ud) is solved with the FP64
+ *     PCG of phg_hip_pcg (A is symmetric after elimination) from 0 to the
+ *     relative tolerance rtol,
+ *   - grad[e] = dJ/dlog(kappa_e) = -kappa_e \int_e grad u_h . grad lambda_h
+ *     with the stiffness rule of quad.c:5260.
+ * *J receives the objective, lambda (optional) the adjoint solution,
+ * *nits the adjoint PCG iterations; grad is a host array [nelem]. */
+int phg_hip_adjoint_logkappa(phg_hip_space *sp, const phg_hip_problem *pb,
+                             const double *kappa, const phg_hip_mat *A,
+                             const phg_hip_vec *u, const phg_hip_vec *ud,
+                             int32_t maxit, double rtol, double *J,
+                             double *grad, phg_hip_vec *lambda, int32_t *nits);
+
 /*--------------------------------------------------- SpMV format autotuner */
 
 /* No reference counterpart (PHG has one CSR kernel, matvec.c:887).  The

@@ -74,6 +74,10 @@ _sig("orc_pcg", C.c_int, [_P(_Csr), _P(_dbl), _P(_dbl), C.c_int, _dbl, _dbl, _db
                           _P(_dbl)])
 _sig("orc_matvec", None, [_P(_Csr), _P(_dbl), _P(_dbl)])
 _sig("orc_csr_free", None, [_P(_Csr)])
+_sig("orc_mass_apply", _dbl, [C.c_int, _i64, _P(_dbl), _P(_i32), _P(_i32), _i64,
+                              _P(_dbl), _P(_dbl)])
+_sig("orc_adjoint_grad", None, [C.c_int, _i64, _P(_dbl), _P(_i32), _P(_i32), _P(_dbl),
+                                _P(_dbl), _P(_dbl), _P(_dbl)])
 
 
 def _ptr(a, t):
@@ -221,3 +225,29 @@ def pcg_arrays(rowptr, cols, vals, b, maxit=10000, rtol=1e-10, btol=1e-10, atol=
     nits = lib.orc_pcg(C.byref(A._s), _ptr(b, _dbl), _ptr(x, _dbl), maxit, rtol, btol,
                        atol, C.byref(res))
     return x, nits, res.value
+
+
+def mass_apply(p, verts, elem_verts, dof_map, e):
+    """(1/2 e^T M e, M e) for the Pk mass matrix, element order."""
+    verts = np.ascontiguousarray(verts, dtype=np.float64)
+    ev = np.ascontiguousarray(elem_verts, dtype=np.int32)
+    m = np.ascontiguousarray(dof_map, dtype=np.int32)
+    e = np.ascontiguousarray(e, dtype=np.float64)
+    Me = np.empty_like(e)
+    J = lib.orc_mass_apply(p, ev.shape[0], _ptr(verts, _dbl), _ptr(ev, _i32),
+                           _ptr(m, _i32), len(e), _ptr(e, _dbl), _ptr(Me, _dbl))
+    return J, Me
+
+
+def adjoint_grad(p, verts, elem_verts, dof_map, kappa, u, lam):
+    """dJ/dlog(kappa_e) = -kappa_e lambda_e^T K1_e u_e per element."""
+    verts = np.ascontiguousarray(verts, dtype=np.float64)
+    ev = np.ascontiguousarray(elem_verts, dtype=np.int32)
+    m = np.ascontiguousarray(dof_map, dtype=np.int32)
+    kap = np.ascontiguousarray(kappa, dtype=np.float64)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    lam = np.ascontiguousarray(lam, dtype=np.float64)
+    g = np.empty(ev.shape[0])
+    lib.orc_adjoint_grad(p, ev.shape[0], _ptr(verts, _dbl), _ptr(ev, _i32), _ptr(m, _i32),
+                         _ptr(kap, _dbl), _ptr(u, _dbl), _ptr(lam, _dbl), _ptr(g, _dbl))
+    return g

@@ -732,3 +732,75 @@ void orc_csr_free(orc_csr *A)
     free(A->rhs);
     free(A);
 }
+
+/*------------------------------------------------------------ adjoint
+ * No reference counterpart (PHG has no adjoint).  Restated from the
+ * definitions: J = 1/2 (u - ud)^T M (u - ud) with M the Pk mass matrix
+ * (quad.c:5582 phgQuadBasDotBas, order 2p), the adjoint right-hand side
+ * M (u - ud) accumulated element by element in index order, and
+ * dJ/dlog(kappa_e) = -kappa_e lambda_e^T K1_e u_e with K1_e the element
+ * stiffness for kappa = 1 (quad.c:5260). */
+double orc_mass_apply(int p, int64_t nelem, const double *verts,
+                      const int32_t *elem_verts, const int32_t *map,
+                      int64_t ndof, const double *e, double *Me)
+{
+    int N = p == 1 ? 4 : (p == 2 ? 10 : 20);
+    orc_problem mp = {p, 1, 1, 1.0, -1, 0.0, 0.0};
+    double *K = xmalloc((size_t)N * N * sizeof(double));
+    double *B = xmalloc((size_t)N * sizeof(double));
+    double *fz = calloc((size_t)N, sizeof(double));
+    double J = 0.0;
+    for (int64_t i = 0; i < ndof; i++)
+        Me[i] = 0.0;
+    for (int64_t el = 0; el < nelem; el++) {
+        double x[4][3];
+        const int32_t *m = map + (int64_t)N * el;
+        for (int a = 0; a < 4; a++)
+            for (int c = 0; c < 3; c++)
+                x[a][c] = verts[3 * (int64_t)elem_verts[4 * el + a] + c];
+        /* kappa = 0 removes the stiffness term: K = M_e */
+        orc_element(&mp, x, 0.0, fz, K, B);
+        for (int i = 0; i < N; i++) {
+            double s = 0.0;
+            for (int j = 0; j < N; j++)
+                s += K[i * N + j] * e[m[j]];
+            Me[m[i]] += s;
+            J += 0.5 * e[m[i]] * s;
+        }
+    }
+    free(K);
+    free(B);
+    free(fz);
+    return J;
+}
+
+void orc_adjoint_grad(int p, int64_t nelem, const double *verts,
+                      const int32_t *elem_verts, const int32_t *map,
+                      const double *kappa, const double *u,
+                      const double *lam, double *grad)
+{
+    int N = p == 1 ? 4 : (p == 2 ? 10 : 20);
+    orc_problem sp = {p, 1, 1, 0.0, -1, 0.0, 0.0};
+    double *K = xmalloc((size_t)N * N * sizeof(double));
+    double *B = xmalloc((size_t)N * sizeof(double));
+    double *fz = calloc((size_t)N, sizeof(double));
+    for (int64_t el = 0; el < nelem; el++) {
+        double x[4][3];
+        const int32_t *m = map + (int64_t)N * el;
+        for (int a = 0; a < 4; a++)
+            for (int c = 0; c < 3; c++)
+                x[a][c] = verts[3 * (int64_t)elem_verts[4 * el + a] + c];
+        orc_element(&sp, x, 1.0, fz, K, B);
+        double s = 0.0;
+        for (int i = 0; i < N; i++) {
+            double t = 0.0;
+            for (int j = 0; j < N; j++)
+                t += K[i * N + j] * u[m[j]];
+            s += lam[m[i]] * t;
+        }
+        grad[el] = -kappa[el] * s;
+    }
+    free(K);
+    free(B);
+    free(fz);
+}

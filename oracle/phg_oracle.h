@@ -89,6 +89,19 @@ void orc_matvec(const orc_csr *A, const double *x, double *y);
 
 void orc_csr_free(orc_csr *A);
 
+/* Adjoint of J = 1/2 (u - ud)^T M (u - ud) for scalar Pk problems (no
+ * reference counterpart).  orc_mass_apply: Me = M e accumulated in element
+ * order, returns 1/2 e^T M e.  orc_adjoint_grad: grad[e] =
+ * dJ/dlog(kappa_e) = -kappa_e lambda_e^T K1_e u_e (K1_e: stiffness with
+ * kappa = 1). */
+double orc_mass_apply(int p, int64_t nelem, const double *verts,
+                      const int32_t *elem_verts, const int32_t *map,
+                      int64_t ndof, const double *e, double *Me);
+void orc_adjoint_grad(int p, int64_t nelem, const double *verts,
+                      const int32_t *elem_verts, const int32_t *map,
+                      const double *kappa, const double *u,
+                      const double *lam, double *grad);
+
 #ifdef __cplusplus
 }
 #endif

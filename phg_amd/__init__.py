@@ -23,7 +23,7 @@ _LIB = os.path.join(_HERE, "lib")
 
 __all__ = [
     "Grid", "Space", "Mat", "Vec", "Problem", "init", "finalize",
-    "assemble", "element_matrices", "pcg", "pcg_mixed", "pcg_reliable", "PhgHipError",
+    "assemble", "element_matrices", "pcg", "pcg_mixed", "pcg_reliable", "adjoint_logkappa", "PhgHipError",
     "hip_lib", "grid_lib", "HIP_LIB_PATH", "GRID_LIB_PATH",
 ]
 
@@ -102,6 +102,9 @@ _sig(hip_lib, "phg_hip_element_matrices", C.c_int,
 _sig(hip_lib, "phg_hip_assemble", C.c_int,
      [_vp, _P(Problem), _P(_dbl), _P(_dbl), _P(_dbl), _P(_vp)])
 _sig(hip_lib, "phg_hip_mat_eliminate", C.c_int, [_vp])
+_sig(hip_lib, "phg_hip_adjoint_logkappa", C.c_int,
+     [_vp, _P(Problem), _P(_dbl), _vp, _vp, _vp, _i32, _dbl, _P(_dbl), _P(_dbl), _vp,
+      _P(_i32)])
 _sig(hip_lib, "phg_hip_mat_info", C.c_int, [_vp, _P(_i64), _P(_i64)])
 _sig(hip_lib, "phg_hip_mat_download", C.c_int,
      [_vp, _P(_i64), _P(_i32), _P(_dbl), _P(_dbl)])
@@ -416,6 +419,21 @@ def assemble(sp: Space, pb: Problem, kappa=None, f=None, g=None) -> Mat:
     _check(hip_lib.phg_hip_assemble(sp._h, C.byref(pb), _ptr(kap, _dbl), _ptr(f, _dbl),
                                     _ptr(g, _dbl), C.byref(h)), "assemble")
     return Mat(h)
+
+
+def adjoint_logkappa(sp: Space, pb: Problem, kappa, A: Mat, u: Vec, ud: Vec,
+                     maxit=100000, rtol=1e-12, lam: Vec | None = None):
+    """Adjoint gradient of J = 1/2 int (u_h - ud_h)^2 with respect to log kappa_e.
+    Returns (J, grad[nelem], adjoint iterations); fills `lam` when given."""
+    kap = _opt(kappa)
+    J = _dbl()
+    nits = _i32()
+    grad = np.empty(sp.nelem)
+    _check(hip_lib.phg_hip_adjoint_logkappa(
+        sp._h, C.byref(pb), _ptr(kap, _dbl), A._h, u._h, ud._h, maxit, rtol,
+        C.byref(J), _ptr(grad, _dbl), lam._h if lam is not None else None,
+        C.byref(nits)), "adjoint_logkappa")
+    return J.value, grad, nits.value
 
 
 def stream_copy(n: int, reps: int = 20) -> float:

@@ -552,6 +552,154 @@ __global__ void __launch_bounds__(128) k_element(ElemArgs a)
     }
 }
 
+//---------------------------------------------------------------- adjoint
+// No reference counterpart (PHG has no adjoint).  For J = 1/2 (u - ud)^T M
+// (u - ud), with M the Pk mass matrix integrated by the rule of
+// phgQuadBasDotBas (quad.c:5582, order 2p):
+//   k_adj_mass<P>   per element: Me_e = M_e (u - ud)_e into [nelem][N] and
+//                   the element's share 1/2 e_e . Me_e;
+//   k_adj_rows      per row: the adjoint right-hand side, the element
+//                   contributions summed in element order (zero on Dirichlet
+//                   rows, whose adjoint value is fixed at 0);
+//   k_adj_grad<P>   per element: dJ/dlog(kappa_e) = -kappa_e vol
+//                   sum_q w_q grad u(x_q) . grad lambda(x_q) with the
+//                   stiffness rule of phgQuadGradBasAGradBas (quad.c:5260),
+//                   i.e. -kappa_e lambda_e^T K1_e u_e without forming K1_e.
+struct AdjArgs {
+    int64_t nelem;
+    const double *verts;
+    const int32_t *elem_verts;
+    const int32_t *map;
+    const double *kappa;
+    const double *u, *ud, *lam;
+    double *Me;        // [nelem][N]
+    double *Jel;       // [nelem]
+    double *grad;      // [nelem]
+};
+
+template <int P>
+__global__ void __launch_bounds__(128) k_adj_mass(AdjArgs a)
+{
+    constexpr int N = P == 1 ? 4 : (P == 2 ? 10 : 20);
+    __shared__ Tables T;
+    build_tables<P>(T, 1, -1, 1);
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.nelem)
+        return;
+    double x[4][3];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        int64_t iv = a.elem_verts[4 * e + v];
+        x[v][0] = a.verts[3 * iv + 0];
+        x[v][1] = a.verts[3 * iv + 1];
+        x[v][2] = a.verts[3 * iv + 2];
+    }
+    const double vol = volume(x);
+    const int32_t *m = a.map + e * (int64_t)N;
+    double el[N];
+#pragma unroll
+    for (int k = 0; k < N; k++)
+        el[k] = a.u[m[k]] - a.ud[m[k]];
+    // values of e_h at the mass points, then Me_i = vol sum_q w_q phi_i e_h
+    double eq[NQ];
+    for (int q = 0; q < T.nq_m; q++) {
+        double t = 0.0;
+#pragma unroll
+        for (int k = 0; k < N; k++)
+            t += T.bm[q][k] * el[k];
+        eq[q] = t;
+    }
+    double J = 0.0;
+    for (int i = 0; i < N; i++) {
+        double d = 0.0;
+        for (int q = 0; q < T.nq_m; q++)
+            d += T.bm[q][i] * T.wm[q] * eq[q];
+        d *= vol;
+        a.Me[e * N + i] = d;
+        J += el[i] * d;
+    }
+    a.Jel[e] = 0.5 * J;
+}
+
+__global__ void k_adj_rows(int64_t n, int N, const int64_t *off,
+                           const int32_t *inc, const uint8_t *bdry,
+                           const double *Me, double *rhs)
+{
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n)
+        return;
+    double s = 0.0;
+    if (!bdry[r])
+        for (int64_t t = off[r]; t < off[r + 1]; t++)
+            s += Me[inc[t]];          // code e * N + a indexes [nelem][N]
+    rhs[r] = s;
+}
+
+template <int P>
+__global__ void __launch_bounds__(128) k_adj_grad(AdjArgs a)
+{
+    constexpr int N = P == 1 ? 4 : (P == 2 ? 10 : 20);
+    constexpr int NQS = P == 1 ? 1 : (P == 2 ? 4 : 14);
+    __shared__ Tables T;
+    build_tables<P>(T, 1, -1, 0);
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.nelem)
+        return;
+    double x[4][3];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        int64_t iv = a.elem_verts[4 * e + v];
+        x[v][0] = a.verts[3 * iv + 0];
+        x[v][1] = a.verts[3 * iv + 1];
+        x[v][2] = a.verts[3 * iv + 2];
+    }
+    double J[16];
+    jacobian(x, J);
+    const double vol = volume(x);
+    const int32_t *m = a.map + e * (int64_t)N;
+    double gu[NQS][3], gl[NQS][3];
+    for (int q = 0; q < NQS; q++)
+        for (int c = 0; c < 3; c++)
+            gu[q][c] = gl[q][c] = 0.0;
+    for (int k = 0; k < N; k++) {
+        const double uk = a.u[m[k]], lk = a.lam[m[k]];
+        for (int q = 0; q < NQS; q++) {
+            double g[3];
+            xgrad<P>(k, T.gs[q][k], J, g);
+            for (int c = 0; c < 3; c++) {
+                gu[q][c] += uk * g[c];
+                gl[q][c] += lk * g[c];
+            }
+        }
+    }
+    double s = 0.0;
+    for (int q = 0; q < NQS; q++)
+        s += (gu[q][0] * gl[q][0] + gu[q][1] * gl[q][1] + gu[q][2] * gl[q][2]) *
+             T.ws[q];
+    a.grad[e] = -a.kappa[e] * (s * vol);
+}
+
+// fixed-order sum of n values: per-block partials, then one block
+__global__ void __launch_bounds__(256) k_sum_blocks(int64_t n, const double *v,
+                                                    double *part)
+{
+    __shared__ double sh[256];
+    int64_t per = (n + gridDim.x - 1) / gridDim.x;
+    int64_t i0 = blockIdx.x * per, i1 = i0 + per < n ? i0 + per : n;
+    double t = 0.0;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += 256)
+        t += v[i];
+    sh[threadIdx.x] = t;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        part[blockIdx.x] = sh[0];
+}
+
 //------------------------------------------------------ nodal coordinates
 // barycentric node of local basis b (dof.c:4925), combined in the element's
 // local vertex order as geom.c:1259 phgGeomLambda2XYZ does
@@ -925,6 +1073,51 @@ int check_problem(const phg_hip_space *sp, const phg_hip_problem *pb)
 }
 
 // element matrices into device buffers K (nelem*M*M) and B (nelem*M)
+// Incidence lists: for every row (scalar DOF * dim + component) the codes
+// e * M + a of the (element, local index) pairs that touch it, in element
+// order -- the order in which PHG's element loop adds to the row.  cnt[r]
+// holds the list lengths on return.
+int build_incidence(const phg_hip_space *sp, int dim, int32_t **cnt_out,
+                    int64_t **off_out, int32_t **inc_out)
+{
+    hipStream_t s = phg::ctx()->stream;
+    const int N = sp->nbas, M = N * dim;
+    const int64_t n = sp->ndof * dim, ne = sp->nelem;
+    int32_t *cnt = nullptr, *inc = nullptr;
+    int64_t *off = nullptr, ninc = 0;
+    int rc;
+    if ((rc = phg::dmalloc((void **)&cnt, (size_t)(n + 1) * sizeof(int32_t))) ||
+        (rc = phg::dmalloc((void **)&off, (size_t)(n + 1) * sizeof(int64_t))) ||
+        (rc = phg::dmalloc((void **)&inc, (size_t)(ne * M + 1) * sizeof(int32_t))))
+        goto fail;
+    if (hipMemsetAsync(cnt, 0, (size_t)(n + 1) * sizeof(int32_t), s) != hipSuccess) {
+        rc = phg::set_error(PHG_HIP_E_HIP, "memset failed");
+        goto fail;
+    }
+    hipLaunchKernelGGL(k_inc_count, dim3(nblk(ne * M, 256)), dim3(256), 0, s,
+                       ne, N, dim, (const int32_t *)sp->d_map, cnt);
+    if ((rc = scan_i32(n, cnt, off, &ninc, s)))
+        goto fail;
+    if (hipMemsetAsync(cnt, 0, (size_t)(n + 1) * sizeof(int32_t), s) != hipSuccess) {
+        rc = phg::set_error(PHG_HIP_E_HIP, "memset failed");
+        goto fail;
+    }
+    hipLaunchKernelGGL(k_inc_fill, dim3(nblk(ne * M, 256)), dim3(256), 0, s,
+                       ne, N, dim, (const int32_t *)sp->d_map,
+                       (const int64_t *)off, cnt, inc);
+    hipLaunchKernelGGL(k_inc_sort, dim3(nblk(n, 256)), dim3(256), 0, s, n,
+                       (const int64_t *)off, inc);
+    *cnt_out = cnt;
+    *off_out = off;
+    *inc_out = inc;
+    return PHG_HIP_OK;
+fail:
+    phg::dfree(cnt);
+    phg::dfree(off);
+    phg::dfree(inc);
+    return rc;
+}
+
 int element_pass(phg_hip_space *sp, const phg_hip_problem *pb,
                  const double *kappa_h, const double *f_h, double **dK,
                  double **dB, double **dkap_out)
@@ -1112,21 +1305,9 @@ extern "C" int phg_hip_assemble(phg_hip_space *sp, const phg_hip_problem *pb,
         goto done;
 
     // incidence
-    if ((rc = phg::dmalloc((void **)&cnt, (size_t)(n + 1) * sizeof(int32_t))) ||
-        (rc = phg::dmalloc((void **)&inc_off, (size_t)(n + 1) * sizeof(int64_t))) ||
-        (rc = phg::dmalloc((void **)&inc, (size_t)(ne * M + 1) * sizeof(int32_t))))
+    if ((rc = build_incidence(sp, dim, &cnt, &inc_off, &inc)))
         goto done;
-    PHG_HIP_CHECK(hipMemsetAsync(cnt, 0, (size_t)(n + 1) * sizeof(int32_t), s));
-    hipLaunchKernelGGL(k_inc_count, dim3(nblk(ne * M, 256)), dim3(256), 0, s,
-                       ne, N, dim, (const int32_t *)sp->d_map, cnt);
-    if ((rc = scan_i32(n, cnt, inc_off, &ninc, s)))
-        goto done;
-    PHG_HIP_CHECK(hipMemsetAsync(cnt, 0, (size_t)(n + 1) * sizeof(int32_t), s));
-    hipLaunchKernelGGL(k_inc_fill, dim3(nblk(ne * M, 256)), dim3(256), 0, s,
-                       ne, N, dim, (const int32_t *)sp->d_map,
-                       (const int64_t *)inc_off, cnt, inc);
-    hipLaunchKernelGGL(k_inc_sort, dim3(nblk(n, 256)), dim3(256), 0, s, n,
-                       (const int64_t *)inc_off, inc);
+    (void)ninc;
     // scratch segments: count * M entries per row (upper bound)
     if ((rc = phg::dmalloc((void **)&count, (size_t)(n + 1) * sizeof(int32_t))) ||
         (rc = phg::dmalloc((void **)&scr_off, (size_t)(n + 1) * sizeof(int64_t))))
@@ -1338,4 +1519,122 @@ extern "C" int phg_hip_mat_destroy(phg_hip_mat *A)
     phg::free_solver_copy(A);
     delete A;
     return PHG_HIP_OK;
+}
+
+//----------------------------------------------------------------- adjoint
+extern "C" int phg_hip_adjoint_logkappa(phg_hip_space *sp,
+                                        const phg_hip_problem *pb,
+                                        const double *kappa,
+                                        const phg_hip_mat *A,
+                                        const phg_hip_vec *u,
+                                        const phg_hip_vec *ud, int32_t maxit,
+                                        double rtol, double *J, double *grad,
+                                        phg_hip_vec *lambda, int32_t *nits)
+{
+    int rc;
+    if ((rc = phg::require_ctx()) || (rc = check_problem(sp, pb)))
+        return rc;
+    if (pb->dim != 1 || !pb->use_kappa || kappa == nullptr || A == nullptr ||
+        u == nullptr || ud == nullptr || grad == nullptr)
+        return phg::set_error(PHG_HIP_E_ARG,
+                              "phg_hip_adjoint_logkappa: scalar problem with "
+                              "kappa, matrix, u, ud and grad required");
+    const int64_t n = sp->ndof, ne = sp->nelem;
+    const int N = sp->nbas;
+    if (A->n != n || u->n != n || ud->n != n || (lambda && lambda->n != n))
+        return phg::set_error(PHG_HIP_E_ARG, "phg_hip_adjoint_logkappa: size mismatch");
+    hipStream_t s = phg::ctx()->stream;
+    double *dk = nullptr, *Me = nullptr, *Jel = nullptr, *part = nullptr,
+           *dgrad = nullptr;
+    int32_t *cnt = nullptr, *inc = nullptr;
+    int64_t *off = nullptr;
+    phg_hip_vec *rhs = nullptr, *lam = nullptr;
+    const int nparts = 256;
+    std::vector<double> hp(nparts);
+    AdjArgs a;
+    if ((rc = phg::dmalloc((void **)&dk, (size_t)ne * sizeof(double))) ||
+        (rc = phg::upload(dk, kappa, (size_t)ne * sizeof(double))) ||
+        (rc = phg::dmalloc((void **)&Me, (size_t)ne * N * sizeof(double))) ||
+        (rc = phg::dmalloc((void **)&Jel, (size_t)ne * sizeof(double))) ||
+        (rc = phg::dmalloc((void **)&dgrad, (size_t)ne * sizeof(double))) ||
+        (rc = phg::dmalloc((void **)&part, (size_t)nparts * sizeof(double))) ||
+        (rc = build_incidence(sp, 1, &cnt, &off, &inc)) ||
+        (rc = phg_hip_vec_create(n, &rhs)))
+        goto done;
+    if (lambda == nullptr && (rc = phg_hip_vec_create(n, &lam)))
+        goto done;
+    a.nelem = ne;
+    a.verts = sp->d_verts;
+    a.elem_verts = sp->d_elem_verts;
+    a.map = sp->d_map;
+    a.kappa = dk;
+    a.u = u->d;
+    a.ud = ud->d;
+    a.lam = lambda ? lambda->d : lam->d;
+    a.Me = Me;
+    a.Jel = Jel;
+    a.grad = dgrad;
+    {
+        dim3 g(nblk(ne, 128)), b(128);
+        if (sp->order == 1)
+            hipLaunchKernelGGL(k_adj_mass<1>, g, b, 0, s, a);
+        else if (sp->order == 2)
+            hipLaunchKernelGGL(k_adj_mass<2>, g, b, 0, s, a);
+        else
+            hipLaunchKernelGGL(k_adj_mass<3>, g, b, 0, s, a);
+    }
+    hipLaunchKernelGGL(k_sum_blocks, dim3(nparts), dim3(256), 0, s, ne,
+                       (const double *)Jel, part);
+    hipLaunchKernelGGL(k_adj_rows, dim3(nblk(n, 256)), dim3(256), 0, s, n, N,
+                       (const int64_t *)off, (const int32_t *)inc,
+                       (const uint8_t *)A->d_bdry_row, (const double *)Me,
+                       rhs->d);
+    PHG_HIP_CHECK(hipGetLastError());
+    if ((rc = phg::download(hp.data(), part, nparts * sizeof(double))))
+        goto done;
+    if (J) {
+        double t = 0.0;
+        for (int i = 0; i < nparts; i++)
+            t += hp[(size_t)i];
+        *J = t;
+    }
+    {
+        // the transposed system: A is symmetric (Dirichlet rows e_i, interior
+        // block with the boundary columns removed), so A^T = A
+        phg_hip_vec *l = lambda ? lambda : lam;
+        PHG_HIP_CHECK(hipMemsetAsync(l->d, 0, (size_t)n * sizeof(double), s));
+        double res = 0.0;
+        int it = phg_hip_pcg(A, rhs, l, maxit, rtol, rtol, 0.0, &res);
+        if (it < 0) {
+            rc = it;
+            goto done;
+        }
+        if (nits)
+            *nits = it;
+    }
+    {
+        dim3 g(nblk(ne, 128)), b(128);
+        if (sp->order == 1)
+            hipLaunchKernelGGL(k_adj_grad<1>, g, b, 0, s, a);
+        else if (sp->order == 2)
+            hipLaunchKernelGGL(k_adj_grad<2>, g, b, 0, s, a);
+        else
+            hipLaunchKernelGGL(k_adj_grad<3>, g, b, 0, s, a);
+    }
+    PHG_HIP_CHECK(hipGetLastError());
+    rc = phg::download(grad, dgrad, (size_t)ne * sizeof(double));
+done:
+    phg::dfree(dk);
+    phg::dfree(Me);
+    phg::dfree(Jel);
+    phg::dfree(dgrad);
+    phg::dfree(part);
+    phg::dfree(cnt);
+    phg::dfree(off);
+    phg::dfree(inc);
+    if (rhs)
+        phg_hip_vec_destroy(rhs);
+    if (lam)
+        phg_hip_vec_destroy(lam);
+    return rc;
 }

@@ -361,3 +361,44 @@ def test_pcg_against_scipy_cg():
     assert abs(len(its) - nits) <= 2
     # the reported residual is |b - A x|
     assert abs(res - np.linalg.norm(A.rhs - S @ x)) < 1e-8 * np.linalg.norm(A.rhs)
+
+
+# ---------------------------------------------------------------- adjoint
+def test_oracle_mass_apply_closed_forms():
+    """e = 1: 1/2 e^T M e = 1/2 |domain| and (M 1)_i = int phi_i (the f = 1
+    load); e = x (in every Pk space): 1/2 int x^2 = 1/6."""
+    from tests._cases import poisson_case
+    for p in (1, 2, 3):
+        c = poisson_case(2, p, "kappa")
+        one = np.ones(c["ndof"])
+        J, Me = orc.mass_apply(p, c["verts"], c["elem_verts"], c["map"], one)
+        assert abs(J - 0.5) < 1e-14
+        pb = orc.problem(p)
+        A = orc.assemble(pb, c["verts"], c["elem_verts"], c["map"], c["ndof"],
+                         np.zeros(c["ndof"], np.uint8), np.zeros(c["ndof"]), one)
+        assert np.allclose(Me, A.rhs, rtol=1e-13, atol=1e-16)
+        J, _ = orc.mass_apply(p, c["verts"], c["elem_verts"], c["map"],
+                              c["coords"][:, 0].copy())
+        assert abs(J - 1.0 / 6.0) < 1e-14
+
+
+@pytest.mark.parametrize("p,n", [(1, 3), (2, 2)])
+def test_oracle_adjoint_matches_central_differences(p, n):
+    """dJ/dlog(kappa_e) of the CPU adjoint against (J(+h) - J(-h)) / 2h
+    with tightly converged solves, on every element of a tiny mesh."""
+    from tests._adjoint import c3_case, oracle_adjoint, oracle_objective, oracle_solve
+    c = c3_case(n, p)
+    kap = c["kappa"]
+    J, g, u, ud, lam = oracle_adjoint(c, kap)
+    assert J > 0
+    h = 1e-6
+    fd = np.empty_like(g)
+    for e in range(len(g)):
+        kp, km = kap.copy(), kap.copy()
+        kp[e] *= np.exp(h)
+        km[e] *= np.exp(-h)
+        _, up = oracle_solve(c, kp)
+        _, um = oracle_solve(c, km)
+        fd[e] = (oracle_objective(c, up, ud) - oracle_objective(c, um, ud)) / (2 * h)
+    err = np.abs(g - fd).max() / np.abs(g).max()
+    assert err < 1e-6, err
