@@ -163,6 +163,34 @@ int phg_hip_pcg_reliable(const phg_hip_mat *A, const phg_hip_vec *b,
                          double btol, double atol, double delta,
                          int32_t *updates, double *residual);
 
+/*------------------------------------------------ many right-hand sides */
+
+/* Block of right-hand sides (no reference counterpart: solver-phg.c:495
+ * solver_pcg asserts nvec == 1, so PHG solves them one after another).
+ * Solves A x_c = b_c for c < nrhs <= 16 with nrhs Jacobi-PCG recurrences
+ * that share one SpMM per iteration over row-major multivectors
+ * ([n][16], solver numbering); 3-component problems use a 3x3 block copy
+ * of the matrix.  Each column has its own tolerance
+ * max(rtol*|r0|, btol*|b|, atol), alpha, beta and iteration count, as
+ * phg_hip_pcg would give it.  mixed = 0: FP64 throughout; mixed = 1: FP32
+ * SpMM and vectors with reliable FP64 residual updates (threshold delta),
+ * convergence tested on FP64 residuals.  x holds the initial guesses on
+ * input.  nits[c], residual[c] (optional) receive each column's iteration
+ * count and final residual.  Returns the largest iteration count. */
+int phg_hip_pcg_block(const phg_hip_mat *A, int32_t nrhs,
+                      const phg_hip_vec *const *b, phg_hip_vec *const *x,
+                      int32_t maxit, double rtol, double btol, double atol,
+                      int32_t mixed, double delta, int32_t *nits,
+                      double *residual);
+/* SpMM autotuner: times the wavefront shapes (1, 2 or 4 lanes per row and
+ * right-hand side; shape index 0..2) in FP64 and FP32 and keeps the
+ * fastest for phg_hip_pcg_block.  phg_hip_spmm_tune_time reports the
+ * measured microseconds; phg_hip_spmm_bytes the bytes one SpMM moves. */
+int phg_hip_spmm_tune(phg_hip_mat *A, int32_t reps);
+int phg_hip_spmm_tune_time(const phg_hip_mat *A, int32_t shape, int32_t fp32,
+                           double *us);
+int phg_hip_spmm_bytes(const phg_hip_mat *A, int32_t fp32, int64_t *bytes);
+
 /*---------------------------------------------------------------- adjoint */
 
 /* Adjoint gradient (no reference counterpart; PHG has no adjoint).  For a

@@ -23,7 +23,7 @@ _LIB = os.path.join(_HERE, "lib")
 
 __all__ = [
     "Grid", "Space", "Mat", "Vec", "Problem", "init", "finalize",
-    "assemble", "element_matrices", "pcg", "pcg_mixed", "pcg_reliable", "adjoint_logkappa", "PhgHipError",
+    "assemble", "element_matrices", "pcg", "pcg_mixed", "pcg_reliable", "pcg_block", "spmm_tune", "adjoint_logkappa", "PhgHipError",
     "hip_lib", "grid_lib", "HIP_LIB_PATH", "GRID_LIB_PATH",
 ]
 
@@ -125,6 +125,11 @@ _sig(hip_lib, "phg_hip_pcg_mixed", C.c_int,
      [_vp, _vp, _vp, _i32, _dbl, _dbl, _dbl, _dbl, _P(_i32), _P(_dbl)])
 _sig(hip_lib, "phg_hip_pcg_reliable", C.c_int,
      [_vp, _vp, _vp, _i32, _dbl, _dbl, _dbl, _dbl, _P(_i32), _P(_dbl)])
+_sig(hip_lib, "phg_hip_pcg_block", C.c_int,
+     [_vp, _i32, _P(_vp), _P(_vp), _i32, _dbl, _dbl, _dbl, _i32, _dbl, _P(_i32), _P(_dbl)])
+_sig(hip_lib, "phg_hip_spmm_tune", C.c_int, [_vp, _i32])
+_sig(hip_lib, "phg_hip_spmm_tune_time", C.c_int, [_vp, _i32, _i32, _P(_dbl)])
+_sig(hip_lib, "phg_hip_spmm_bytes", C.c_int, [_vp, _i32, _P(_i64)])
 _sig(hip_lib, "phg_hip_mat_tune", C.c_int, [_vp, _i32])
 _sig(hip_lib, "phg_hip_mat_tune_time", C.c_int, [_vp, _i32, _i32, _P(_dbl)])
 _sig(hip_lib, "phg_hip_mat_set_format", C.c_int, [_vp, _i32, _i32])
@@ -434,6 +439,48 @@ def adjoint_logkappa(sp: Space, pb: Problem, kappa, A: Mat, u: Vec, ud: Vec,
         C.byref(J), _ptr(grad, _dbl), lam._h if lam is not None else None,
         C.byref(nits)), "adjoint_logkappa")
     return J.value, grad, nits.value
+
+
+BLOCK_MAX = 16
+SPMM_SHAPES = (1, 2, 4)
+
+
+def pcg_block(A: "Mat", bs, xs, maxit=100000, rtol=1e-10, btol=1e-10, atol=0.0,
+              mixed=False, delta=0.1):
+    """Jacobi-PCG for up to 16 right-hand sides sharing one SpMM per
+    iteration.  Returns (nits[k], residual[k])."""
+    k = len(bs)
+    if k != len(xs) or not 1 <= k <= BLOCK_MAX:
+        raise ValueError("1 <= len(bs) == len(xs) <= 16 required")
+    barr = (_vp * k)(*[b._h for b in bs])
+    xarr = (_vp * k)(*[x._h for x in xs])
+    nits = np.zeros(k, dtype=np.int32)
+    res = np.zeros(k)
+    _check(hip_lib.phg_hip_pcg_block(A._h, k, barr, xarr, maxit, rtol, btol, atol,
+                                     int(mixed), delta, _ptr(nits, _i32), _ptr(res, _dbl)),
+           "pcg_block")
+    return nits, res
+
+
+def spmm_tune(A: "Mat", reps: int = 10) -> dict:
+    """Times the SpMM wavefront shapes; returns {shape: (fp64_us, fp32_us)}."""
+    _check(hip_lib.phg_hip_spmm_tune(A._h, reps), "spmm_tune")
+    out = {}
+    for i, sh in enumerate(SPMM_SHAPES):
+        t = []
+        for fp32 in (0, 1):
+            us = _dbl()
+            _check(hip_lib.phg_hip_spmm_tune_time(A._h, i, fp32, C.byref(us)),
+                   "spmm_tune_time")
+            t.append(us.value)
+        out[sh] = tuple(t)
+    return out
+
+
+def spmm_bytes(A: "Mat", fp32: bool) -> int:
+    b = _i64()
+    _check(hip_lib.phg_hip_spmm_bytes(A._h, int(fp32), C.byref(b)), "spmm_bytes")
+    return b.value
 
 
 def stream_copy(n: int, reps: int = 20) -> float:

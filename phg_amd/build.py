@@ -39,6 +39,7 @@ HIP_UNITS = {
     "krylov.hip": [],
     "reorder.hip": [],
     "dictcsr.hip": [],
+    "block.hip": [],
 }
 
 

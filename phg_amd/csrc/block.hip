@@ -1,0 +1,981 @@
+// Many right-hand sides: Jacobi-PCG over a block of up to KB = 16 vectors.
+//
+// PHG's PCG solves one vector at a time (solver-phg.c:495 asserts
+// nvec == 1); here the KB recurrences advance together so that every
+// iteration reads the matrix once for all of them: the SpMV becomes an SpMM
+// over multivectors stored row-major, [n][KB], so the KB values of one row
+// are one contiguous 64 / 128-byte segment.  Each column keeps its own
+// alpha, beta and stopping test (solver-phg.c:535-580 per column); a column
+// that has converged stops changing.
+//
+// Matrix: the solver copy (Morton numbering).  For vector problems (3
+// components per node) the scalar CSR is regrouped into 3x3 blocks (BSR3),
+// one column index per 9 values; scalar problems use the CSR itself.
+//
+// Mixed precision: FP32 SpMM and vectors with reliable updates -- when a
+// column's FP32 residual has dropped by delta below its largest value since
+// the previous replacement, or reaches the tolerance, the FP32 increments of
+// all columns are folded into x (FP64) and every residual is replaced by
+// the FP64 residual b - A x from an FP64 SpMM; convergence is only declared
+// on FP64 residuals.
+//
+// One iteration slot launches
+//   k_blk_ctrl   (1 work-group) column totals -> status, beta, slot kind
+//   k_blk_p      p = z + beta p  (or, in an update slot, x += x32)
+//   k_blk_spmm   Ap = A p, partial (p, Ap) per column
+//   k_blk_alpha  (1 work-group) alpha per column
+//   k_blk_update x32 += alpha p, r -= alpha Ap, partial (r,r), (r,z)
+//   k_blk_resid  update slots only: r = b - A x in FP64, partials
+#include "phg_hip_internal.h"
+
+#include <math.h>
+#include <string.h>
+#include <algorithm>
+#include <vector>
+
+namespace {
+
+constexpr int KB = 16;            // vectors per block
+constexpr int TB = 256;           // work-group size
+constexpr int NXCD = 8;
+constexpr int MAXG = 1024;        // partial-sum slots per value and column
+constexpr int BSR_MAXB = 128;     // blocks per block row (builder limit)
+
+__device__ inline int64_t xcd_block()
+{
+    int64_t nb = gridDim.x, b = blockIdx.x;
+    int64_t x = b % NXCD, k = b / NXCD;
+    int64_t q = nb / NXCD, rem = nb % NXCD;
+    return x * q + (x < rem ? x : rem) + k;
+}
+
+// contiguous range of [0, n) for this work-group
+__device__ inline void group_range(int64_t n, int64_t &a, int64_t &b)
+{
+    int64_t g = xcd_block(), ng = gridDim.x;
+    int64_t per = (n + ng - 1) / ng;
+    a = g * per;
+    b = a + per < n ? a + per : n;
+}
+
+//------------------------------------------------------------- BSR3 build
+// block row R = scalar rows 3R..3R+2; blocks in the order their columns
+// first appear scanning the three rows in storage order
+__global__ void k_bsr_count(int64_t nb, const int32_t *rp, const int32_t *ci,
+                            int32_t *cnt, int32_t *overflow)
+{
+    int64_t R = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (R >= nb)
+        return;
+    int32_t seen[BSR_MAXB];
+    int k = 0;
+    for (int p = 0; p < 3; p++)
+        for (int32_t j = rp[3 * R + p]; j < rp[3 * R + p + 1]; j++) {
+            int32_t C = ci[j] / 3;
+            int i = 0;
+            while (i < k && seen[i] != C)
+                i++;
+            if (i == k) {
+                if (k == BSR_MAXB) {
+                    *overflow = 1;
+                    cnt[R] = 0;
+                    return;
+                }
+                seen[k++] = C;
+            }
+        }
+    cnt[R] = k;
+}
+
+template <typename T>
+__global__ void k_bsr_fill(int64_t nb, const int32_t *rp, const int32_t *ci,
+                           const double *va, const int64_t *bptr,
+                           int32_t *bcol, T *bval)
+{
+    int64_t R = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (R >= nb)
+        return;
+    const int64_t b0 = bptr[R];
+    int k = 0;
+    for (int64_t i = b0; i < bptr[R + 1]; i++)
+        for (int t = 0; t < 9; t++)
+            bval[i * 9 + t] = (T)0;
+    for (int p = 0; p < 3; p++)
+        for (int32_t j = rp[3 * R + p]; j < rp[3 * R + p + 1]; j++) {
+            int32_t C = ci[j] / 3, q = ci[j] % 3;
+            int i = 0;
+            while (i < k && bcol[b0 + i] != C)
+                i++;
+            if (i == k)
+                bcol[b0 + k++] = C;
+            bval[(b0 + i) * 9 + p * 3 + q] = (T)va[j];
+        }
+}
+
+__global__ void k_i64_to_i32(int64_t n, const int64_t *a, int32_t *b)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= n)
+        b[i] = (int32_t)a[i];
+}
+
+//------------------------------------------------------------------ SpMM
+// Block CSR with B x B blocks (B = 1: the CSR itself).  Lane mapping inside
+// a wavefront: S slots per row, KB columns: lane = (row * S + s) * KB + c;
+// 64 / (S * KB) block rows per wavefront.  Slot s takes the row's blocks
+// s, s + S, ...; the S partial sums are combined with shuffles.
+struct BsrView {
+    const int32_t *bptr, *bcol;
+    int64_t nb;          // block rows
+};
+
+template <typename T, int B, int S, typename Epi>
+__device__ inline void bsr_spmm(const BsrView &m, const T *__restrict__ va,
+                                const T *__restrict__ X, Epi epi)
+{
+    constexpr int LPR = S * KB;          // lanes per block row
+    constexpr int RPW = 64 / LPR;        // block rows per wavefront
+    constexpr int RPG = RPW * (TB / 64); // block rows per work-group step
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int c = lane % KB, s = (lane / KB) % S, rw = lane / LPR;
+    int64_t g0, g1;
+    group_range((m.nb + RPG - 1) / RPG, g0, g1);
+    for (int64_t gi = g0; gi < g1; gi++) {
+        const int64_t R = gi * RPG + wv * RPW + rw;
+        T acc[B];
+#pragma unroll
+        for (int p = 0; p < B; p++)
+            acc[p] = 0;
+        if (R < m.nb) {
+            const int32_t j1 = m.bptr[R + 1];
+#pragma unroll 2
+            for (int32_t j = m.bptr[R] + s; j < j1; j += S) {
+                const int64_t C = m.bcol[j];
+                T xq[B];
+#pragma unroll
+                for (int q = 0; q < B; q++)
+                    xq[q] = X[(C * B + q) * KB + c];
+                const T *a = va + (int64_t)j * (B * B);
+#pragma unroll
+                for (int p = 0; p < B; p++)
+#pragma unroll
+                    for (int q = 0; q < B; q++)
+                        acc[p] += a[p * B + q] * xq[q];
+            }
+        }
+#pragma unroll
+        for (int o = KB; o < LPR; o <<= 1)
+#pragma unroll
+            for (int p = 0; p < B; p++)
+                acc[p] += __shfl_xor(acc[p], o, 64);
+        if (R < m.nb && s == 0)
+#pragma unroll
+            for (int p = 0; p < B; p++)
+                epi(R * B + p, c, acc[p]);
+    }
+}
+
+// per-column sums of a work-group: v[c] held by the lanes of column c
+// (lane % KB == c); written to part[(k * KB + c) * G + block]
+template <int NV>
+__device__ inline void write_col_partials(double (&v)[NV], double *part)
+{
+    __shared__ double sh[NV][TB / 64][KB];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        double t = v[k];
+#pragma unroll
+        for (int o = KB; o < 64; o <<= 1)
+            t += __shfl_xor(t, o, 64);
+        if (lane < KB)
+            sh[k][wv][lane] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < NV * KB) {
+        const int k = threadIdx.x / KB, c = threadIdx.x % KB;
+        double t = 0.0;
+#pragma unroll
+        for (int w = 0; w < TB / 64; w++)
+            t += sh[k][w][c];
+        part[((int64_t)k * KB + c) * gridDim.x + blockIdx.x] = t;
+    }
+}
+
+//--------------------------------------------------------------- state
+constexpr int ST_ITER = 1, ST_UPDATE = 2, ST_START = 0;
+struct BState {
+    double rz[KB], res[KB], tol[KB], rmax[KB], alpha[KB], beta[KB], r0[KB];
+    double rtol, btol, atol, delta;
+    int32_t active[KB], nits[KB], first[KB];
+    int32_t zero_x[KB];   // b == 0: x = 0 (solver-phg.c:511-518)
+    int32_t kind, maxit, mixed, updates, all_done, slots;
+};
+
+// column totals of NV values from `np` partials each
+template <int NV>
+__device__ inline void col_totals(const double *part, int np, double (&t)[NV][KB])
+{
+    __shared__ double sh[NV * KB][TB / 16];
+    // 16 threads per (value, column) pair would need NV*KB*16 threads; use
+    // a strided loop: thread t handles pair t % (NV*KB), chunk t / (NV*KB)
+    const int pairs = NV * KB;
+    const int per = TB / pairs;           // threads per pair (>= 1)
+    const int pr = threadIdx.x % pairs, ch = threadIdx.x / pairs;
+    double a = 0.0;
+    if (ch < per)
+        for (int i = ch; i < np; i += per)
+            a += part[(int64_t)pr * np + i];
+    if (ch < per)
+        sh[pr][ch] = a;
+    __syncthreads();
+    if (threadIdx.x < pairs) {
+        double s = 0.0;
+        for (int k = 0; k < per; k++)
+            s += sh[threadIdx.x][k];
+        sh[threadIdx.x][0] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; k++)
+#pragma unroll
+        for (int c = 0; c < KB; c++)
+            t[k][c] = sh[k * KB + c][0];
+    __syncthreads();
+}
+
+// Reads the column totals left by the previous slot -- (r,r), (r,z),
+// #zero diagonals, (b,b) -- from the iteration kernel (after an iteration
+// slot) or from the FP64 residual kernel (after the prologue or an update
+// slot), advances every column's recurrence and picks the next slot kind.
+__global__ void __launch_bounds__(TB) k_blk_ctrl(const double *part_it, int np_it,
+                                                const double *part_up, int np_up,
+                                                BState *st)
+{
+    const int last = st->kind;
+    const bool from_update = last != ST_ITER;
+    double t[4][KB];
+    if (from_update)
+        col_totals<4>(part_up, np_up, t);
+    else
+        col_totals<4>(part_it, np_it, t);
+    if (threadIdx.x != 0)
+        return;
+    BState &S = *st;
+    if (S.all_done)
+        return;
+    bool want_update = false;
+    int alive = 0;
+    double nbeta[KB], nrz[KB];
+    int nfirst[KB];
+    for (int c = 0; c < KB; c++) {
+        if (!S.active[c])
+            continue;
+        const double res = sqrt(t[0][c]), rz = t[1][c];
+        if (last == ST_ITER)
+            S.nits[c]++;              // the slot just run iterated column c
+        if (last == ST_START) {
+            // get_tol (solver-phg.c:104) per column on the initial residual
+            const double rb = sqrt(t[3][c]);
+            double tol = res * S.rtol;
+            if (tol < rb * S.btol)
+                tol = rb * S.btol;
+            if (tol < S.atol)
+                tol = S.atol;
+            S.tol[c] = tol;
+            S.r0[c] = res;
+            S.res[c] = res;
+            S.rmax[c] = res;
+            if (rb < 10.0 * 2.2250738585072014e-308) {
+                S.zero_x[c] = 1;
+                S.res[c] = 0.0;
+            }
+            if (S.zero_x[c] || !(res > tol) || S.maxit <= 0 || t[2][c] != 0.0) {
+                S.active[c] = 0;
+                continue;
+            }
+            nbeta[c] = 0.0;
+            nrz[c] = rz;
+            nfirst[c] = 1;
+            alive++;
+            continue;
+        }
+        S.res[c] = res;
+        const bool fp64_res = !S.mixed || from_update;
+        if (fp64_res && !(res > S.tol[c])) {
+            S.active[c] = 0;          // converged on an FP64 residual
+            continue;
+        }
+        if (S.nits[c] >= S.maxit) {
+            if (S.mixed && !from_update)
+                want_update = true;   // finish on an FP64 residual
+            else {
+                S.active[c] = 0;
+                continue;
+            }
+        }
+        if (S.mixed && !from_update &&
+            (!(res > S.tol[c]) || res < S.delta * S.rmax[c]))
+            want_update = true;
+        if (from_update || res > S.rmax[c])
+            S.rmax[c] = res;
+        if (S.rz[c] == 0.0) {       // solver-phg.c:547 breakdown
+            S.active[c] = 0;
+            continue;
+        }
+        nbeta[c] = rz / S.rz[c];
+        nrz[c] = rz;
+        nfirst[c] = 0;
+        alive++;
+    }
+    if (alive == 0) {
+        S.all_done = 1;
+        S.kind = ST_ITER;
+        return;
+    }
+    S.kind = want_update ? ST_UPDATE : ST_ITER;
+    if (S.kind == ST_UPDATE) {
+        // the FP32 totals are discarded: after the FP64 residual the next
+        // slot takes beta from it against the rz that built the current p
+        S.updates++;
+        return;
+    }
+    for (int c = 0; c < KB; c++)
+        if (S.active[c]) {
+            S.beta[c] = nbeta[c];
+            S.rz[c] = nrz[c];
+            S.first[c] = nfirst[c];
+        }
+    S.slots++;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(TB) k_blk_p(int64_t n, const T *r,
+                                             const T *diag, T *p, double *x,
+                                             T *x32, const BState *st)
+{
+    const BState &S = *st;
+    if (S.all_done)
+        return;
+    const int c = threadIdx.x % KB;
+    const int64_t total = n * KB;
+    if (S.kind == ST_UPDATE) {
+        for (int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x; i < total;
+             i += (int64_t)gridDim.x * TB) {
+            x[i] += (double)x32[i];
+            x32[i] = (T)0;
+        }
+        return;
+    }
+    const bool act = S.active[c] != 0, first = S.first[c] != 0;
+    const T beta = (T)S.beta[c];
+    if (!act)
+        return;
+    for (int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * TB) {
+        const T z = r[i] / diag[i / KB];
+        p[i] = first ? z : beta * p[i] + z;
+    }
+}
+
+template <typename T, int B, int S>
+__global__ void __launch_bounds__(TB) k_blk_spmm(BsrView m, const T *va,
+                                                const T *p, T *Ap,
+                                                double *part, const BState *st)
+{
+    if (st->all_done || st->kind != ST_ITER)
+        return;
+    double v[1] = {0.0};
+    bsr_spmm<T, B, S>(m, va, p, [&](int64_t row, int c, T s) {
+        Ap[row * KB + c] = s;
+        v[0] += (double)p[row * KB + c] * (double)s;
+    });
+    write_col_partials<1>(v, part);
+}
+
+__global__ void __launch_bounds__(TB) k_blk_alpha(const double *part, int np,
+                                                 BState *st)
+{
+    if (st->all_done || st->kind != ST_ITER)
+        return;
+    double t[1][KB];
+    col_totals<1>(part, np, t);
+    if (threadIdx.x != 0)
+        return;
+    for (int c = 0; c < KB; c++) {
+        if (!st->active[c])
+            continue;
+        if (t[0][c] == 0.0) {           // solver-phg.c:561
+            st->active[c] = 0;
+            st->alpha[c] = 0.0;
+            continue;
+        }
+        st->alpha[c] = st->rz[c] / t[0][c];
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(TB) k_blk_update(int64_t n, T *x, const T *p,
+                                                  T *r, const T *Ap,
+                                                  const T *diag, double *part,
+                                                  const BState *st)
+{
+    if (st->all_done || st->kind != ST_ITER)
+        return;
+    const int c = threadIdx.x % KB;
+    const bool act = st->active[c] != 0;
+    const T alpha = (T)st->alpha[c];
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    const int64_t total = n * KB;
+    for (int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * TB) {
+        T ri = r[i];
+        if (act) {
+            x[i] += alpha * p[i];
+            ri = ri - alpha * Ap[i];
+            r[i] = ri;
+        }
+        const T z = ri / diag[i / KB];
+        v[0] += (double)ri * (double)ri;
+        v[1] += (double)ri * (double)z;
+    }
+    write_col_partials<4>(v, part);
+}
+
+// FP64 residual r = b - A x for every column; r_out (T) receives it;
+// partials (r,r), (r,z), #zero diagonals, (b,b)
+template <typename T, int B, int S>
+__global__ void __launch_bounds__(TB) k_blk_resid(BsrView m, const double *va,
+                                                 const double *diag,
+                                                 const double *b,
+                                                 const double *x, T *r_out,
+                                                 T *x32_zero, double *part,
+                                                 const BState *st, int force)
+{
+    if (!force && (st->all_done || st->kind != ST_UPDATE))
+        return;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    bsr_spmm<double, B, S>(m, va, x, [&](int64_t row, int c, double s) {
+        const int64_t i = row * KB + c;
+        const double bi = b[i], ri = bi - s, d = diag[row];
+        r_out[i] = (T)ri;
+        if (x32_zero)
+            x32_zero[i] = (T)0;
+        v[0] += ri * ri;
+        if (d == 0.0)
+            v[2] += 1.0;
+        else
+            v[1] += ri * (ri / d);
+        v[3] += bi * bi;
+    });
+    write_col_partials<4>(v, part);
+}
+
+// pack / unpack between KB separate vectors (PHG numbering) and the
+// row-major multivector in the solver numbering
+struct PtrTab {
+    const double *src[KB];
+    double *dst[KB];
+};
+
+__global__ void k_blk_pack(int64_t n, int nrhs, const int32_t *perm,
+                           PtrTab t, double *out)
+{
+    const int64_t total = n * KB;
+    for (int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * TB) {
+        const int64_t row = i / KB;
+        const int c = (int)(i % KB);
+        const int64_t src = perm ? perm[row] : row;
+        out[i] = c < nrhs ? t.src[c][src] : 0.0;
+    }
+}
+
+// zero_mask bit c: column c is written as zero
+__global__ void k_blk_unpack(int64_t n, int nrhs, const int32_t *perm,
+                             const double *in, PtrTab t, int zero_mask)
+{
+    const int64_t total = n * KB;
+    for (int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * TB) {
+        const int64_t row = i / KB;
+        const int c = (int)(i % KB);
+        if (c < nrhs)
+            t.dst[c][perm ? perm[row] : row] = (zero_mask >> c) & 1 ? 0.0 : in[i];
+    }
+}
+
+__global__ void k_blk_fold(int64_t total, double *x, const float *x32)
+{
+    for (int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * TB)
+        x[i] += (double)x32[i];
+}
+
+inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+int grid_vec(int64_t total)
+{
+    int cus = phg::ctx()->num_cus > 0 ? phg::ctx()->num_cus : 256;
+    return (int)std::max<int64_t>(1, std::min<int64_t>((total + TB - 1) / TB,
+                                                       std::min(cus * 4, MAXG)));
+}
+
+template <typename K>
+int grid_res(const void *kern, int64_t units)
+{
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, TB, 0) !=
+            hipSuccess ||
+        per_cu <= 0)
+        per_cu = 4;
+    int cus = phg::ctx()->num_cus > 0 ? phg::ctx()->num_cus : 256;
+    return (int)std::max<int64_t>(
+        1, std::min<int64_t>(units, std::min(per_cu * cus, MAXG)));
+}
+
+}  // namespace
+
+namespace phg {
+
+void free_bsr(phg_hip_mat *A)
+{
+    if (A->b_owns) {
+        dfree(A->b_ptr);
+        dfree(A->b_col);
+    }
+    dfree(A->b_val64);
+    dfree(A->b_val32);
+    A->b_ptr = A->b_col = nullptr;
+    A->b_val64 = nullptr;
+    A->b_val32 = nullptr;
+    A->b_owns = 0;
+    A->b_nb = A->b_nblk = 0;
+    A->b_dim = 0;
+}
+
+// block copy of the solver matrix: 3x3 blocks for 3-component problems,
+// the CSR itself (1x1 blocks) otherwise
+int build_bsr(phg_hip_mat *A)
+{
+    if (A->b_dim != 0)
+        return PHG_HIP_OK;
+    int rc;
+    hipStream_t s = ctx()->stream;
+    if ((rc = ensure_fp32_copy(A)))
+        return rc;
+    if (A->dim != 3 || A->n % 3 != 0) {
+        A->b_ptr = A->q_rowptr;
+        A->b_col = A->q_cols;
+        A->b_nb = A->n;
+        A->b_nblk = A->nnz;
+        A->b_owns = 0;
+        A->b_dim = 1;
+        return PHG_HIP_OK;
+    }
+    const int64_t nb = A->n / 3;
+    int32_t *cnt = nullptr, *ovf = nullptr;
+    int64_t *ptr64 = nullptr;
+    void *tmp = nullptr;
+    size_t tb = 0;
+    int32_t h_ovf = 0;
+    int64_t nbk = 0;
+    auto cleanup = [&]() {
+        dfree(cnt); dfree(ovf); dfree(ptr64); dfree(tmp);
+    };
+    if ((rc = dmalloc((void **)&cnt, (size_t)(nb + 1) * 4)) ||
+        (rc = dmalloc((void **)&ovf, 4)) ||
+        (rc = dmalloc((void **)&ptr64, (size_t)(nb + 1) * 8))) {
+        cleanup();
+        return rc;
+    }
+    PHG_HIP_CHECK(hipMemsetAsync(ovf, 0, 4, s));
+    PHG_HIP_CHECK(hipMemsetAsync(cnt + nb, 0, 4, s));
+    hipLaunchKernelGGL(k_bsr_count, dim3(nblk(nb, 128)), dim3(128), 0, s, nb,
+                       (const int32_t *)A->q_rowptr, (const int32_t *)A->q_cols,
+                       cnt, ovf);
+    if ((rc = download(&h_ovf, ovf, 4))) {
+        cleanup();
+        return rc;
+    }
+    if (h_ovf) {
+        cleanup();
+        return set_error(PHG_HIP_E_ARG, "block row with more than %d blocks",
+                         BSR_MAXB);
+    }
+    // exclusive scan of the counts on the host (nb + 1 values)
+    {
+        std::vector<int32_t> hc((size_t)nb);
+        std::vector<int64_t> hp((size_t)nb + 1);
+        if ((rc = download(hc.data(), cnt, (size_t)nb * 4))) {
+            cleanup();
+            return rc;
+        }
+        hp[0] = 0;
+        for (int64_t i = 0; i < nb; i++)
+            hp[(size_t)i + 1] = hp[(size_t)i] + hc[(size_t)i];
+        nbk = hp[(size_t)nb];
+        if (nbk >= INT32_MAX / 9) {
+            cleanup();
+            return set_error(PHG_HIP_E_ARG, "too many blocks");
+        }
+        if ((rc = upload(ptr64, hp.data(), hp.size() * 8))) {
+            cleanup();
+            return rc;
+        }
+    }
+    if ((rc = dmalloc((void **)&A->b_ptr, (size_t)(nb + 1) * 4)) ||
+        (rc = dmalloc((void **)&A->b_col, (size_t)(nbk + 1) * 4)) ||
+        (rc = dmalloc((void **)&A->b_val64, (size_t)(nbk + 1) * 9 * 8)) ||
+        (rc = dmalloc((void **)&A->b_val32, (size_t)(nbk + 1) * 9 * 4))) {
+        A->b_owns = 1;
+        free_bsr(A);
+        cleanup();
+        return rc;
+    }
+    A->b_owns = 1;
+    hipLaunchKernelGGL(k_bsr_fill<double>, dim3(nblk(nb, 128)), dim3(128), 0, s,
+                       nb, (const int32_t *)A->q_rowptr,
+                       (const int32_t *)A->q_cols, (const double *)A->q_vals,
+                       (const int64_t *)ptr64, A->b_col, A->b_val64);
+    hipLaunchKernelGGL(k_bsr_fill<float>, dim3(nblk(nb, 128)), dim3(128), 0, s,
+                       nb, (const int32_t *)A->q_rowptr,
+                       (const int32_t *)A->q_cols, (const double *)A->q_vals,
+                       (const int64_t *)ptr64, A->b_col, A->b_val32);
+    hipLaunchKernelGGL(k_i64_to_i32, dim3(nblk(nb + 1, 256)), dim3(256), 0, s,
+                       nb, (const int64_t *)ptr64, A->b_ptr);
+    PHG_HIP_CHECK(hipGetLastError());
+    PHG_HIP_CHECK(hipStreamSynchronize(s));
+    cleanup();
+    A->b_nb = nb;
+    A->b_nblk = nbk;
+    A->b_dim = 3;
+    return PHG_HIP_OK;
+}
+
+}  // namespace phg
+
+namespace {
+
+struct BlkWork {
+    int64_t n = 0;
+    double *b = nullptr, *x = nullptr, *r64 = nullptr;     // [n][KB]
+    void *r = nullptr, *p = nullptr, *Ap = nullptr, *x32 = nullptr;
+    double *part_it = nullptr, *part_up = nullptr, *part_s = nullptr;
+    BState *st = nullptr;
+    void release()
+    {
+        phg::dfree(b); phg::dfree(x); phg::dfree(r64); phg::dfree(r);
+        phg::dfree(p); phg::dfree(Ap); phg::dfree(x32); phg::dfree(part_it);
+        phg::dfree(part_up); phg::dfree(part_s); phg::dfree(st);
+        *this = BlkWork();
+    }
+};
+
+BlkWork g_bw;
+
+int get_work(int64_t n)
+{
+    if (g_bw.n == n)
+        return PHG_HIP_OK;
+    g_bw.release();
+    const size_t nk = (size_t)n * KB;
+    int rc;
+    if ((rc = phg::dmalloc((void **)&g_bw.b, nk * 8)) ||
+        (rc = phg::dmalloc((void **)&g_bw.x, nk * 8)) ||
+        (rc = phg::dmalloc(&g_bw.r, nk * 8)) ||
+        (rc = phg::dmalloc(&g_bw.p, nk * 8)) ||
+        (rc = phg::dmalloc(&g_bw.Ap, nk * 8)) ||
+        (rc = phg::dmalloc(&g_bw.x32, nk * 8)) ||
+        (rc = phg::dmalloc((void **)&g_bw.part_it, (size_t)4 * KB * MAXG * 8)) ||
+        (rc = phg::dmalloc((void **)&g_bw.part_up, (size_t)4 * KB * MAXG * 8)) ||
+        (rc = phg::dmalloc((void **)&g_bw.part_s, (size_t)KB * MAXG * 8)) ||
+        (rc = phg::dmalloc((void **)&g_bw.st, sizeof(BState)))) {
+        g_bw.release();
+        return rc;
+    }
+    g_bw.n = n;
+    return PHG_HIP_OK;
+}
+
+// block values of precision T: the CSR copies for 1x1 blocks
+template <typename T, int B>
+const void *mat_vals(const phg_hip_mat *A)
+{
+    if (B == 1)
+        return sizeof(T) == 4 ? (const void *)A->q_vals32 : (const void *)A->q_vals;
+    return sizeof(T) == 4 ? (const void *)A->b_val32 : (const void *)A->b_val64;
+}
+
+// one solve with element type T for the iteration and SpMM shape (B, S)
+template <typename T, int B, int S>
+int block_solve(phg_hip_mat *A, int mixed, int32_t maxit, double rtol,
+                double btol, double atol, double delta, BState *hfin)
+{
+    hipStream_t s = phg::ctx()->stream;
+    const int64_t n = A->n, total = n * KB;
+    const BsrView m{A->b_ptr, A->b_col, A->b_nb};
+    const T *va = (const T *)mat_vals<T, B>(A);
+    const double *va64 = B == 1 ? A->q_vals : A->b_val64;
+    const T *diag = (const T *)(sizeof(T) == 4 ? (const void *)A->q_diag32
+                                               : (const void *)A->q_diag);
+    constexpr int RPG = (64 / (S * KB)) * (TB / 64);
+    const int64_t units = (A->b_nb + RPG - 1) / RPG;
+    const int gs = grid_res<T>((const void *)k_blk_spmm<T, B, S>, units);
+    const int gr = grid_res<T>((const void *)k_blk_resid<T, B, S>, units);
+    const int gv = grid_vec(total);
+    T *r = (T *)g_bw.r, *p = (T *)g_bw.p, *Ap = (T *)g_bw.Ap;
+    // FP64 mode iterates on x itself; mixed mode on the FP32 increment
+    T *xi = mixed ? (T *)g_bw.x32 : (T *)(void *)g_bw.x;
+    BState h;
+    memset(&h, 0, sizeof(h));
+    h.rtol = rtol;
+    h.btol = btol;
+    h.atol = atol;
+    h.delta = delta;
+    h.maxit = maxit;
+    h.mixed = mixed;
+    h.kind = ST_START;
+    for (int c = 0; c < KB; c++)
+        h.active[c] = 1;
+    PHG_HIP_CHECK(hipMemcpyAsync(g_bw.st, &h, sizeof(h), hipMemcpyHostToDevice, s));
+    // prologue: r = b - A x in FP64 (x32 = 0 in mixed mode)
+    hipLaunchKernelGGL((k_blk_resid<T, B, S>), dim3(gr), dim3(TB), 0, s, m,
+                       va64, (const double *)A->q_diag, (const double *)g_bw.b,
+                       (const double *)g_bw.x, r, mixed ? (T *)g_bw.x32 : (T *)nullptr,
+                       g_bw.part_up, (const BState *)g_bw.st, 1);
+    PHG_HIP_CHECK(hipGetLastError());
+    BState *hm = (BState *)phg::ctx()->host_mailbox;
+    static_assert(sizeof(BState) <= 4096, "mailbox size");
+    int chunk = 8;
+    int64_t slot = 0;
+    for (;;) {
+        for (int i = 0; i < chunk; i++, slot++) {
+            hipLaunchKernelGGL(k_blk_ctrl, dim3(1), dim3(TB), 0, s,
+                               (const double *)g_bw.part_it, gv,
+                               (const double *)g_bw.part_up, gr, g_bw.st);
+            hipLaunchKernelGGL(k_blk_p<T>, dim3(gv), dim3(TB), 0, s, n,
+                               (const T *)r, diag, p, g_bw.x, (T *)g_bw.x32,
+                               (const BState *)g_bw.st);
+            hipLaunchKernelGGL((k_blk_spmm<T, B, S>), dim3(gs), dim3(TB), 0, s,
+                               m, va, (const T *)p, Ap, g_bw.part_s,
+                               (const BState *)g_bw.st);
+            hipLaunchKernelGGL(k_blk_alpha, dim3(1), dim3(TB), 0, s,
+                               (const double *)g_bw.part_s, gs, g_bw.st);
+            hipLaunchKernelGGL(k_blk_update<T>, dim3(gv), dim3(TB), 0, s, n, xi,
+                               (const T *)p, r, (const T *)Ap, diag,
+                               g_bw.part_it, (const BState *)g_bw.st);
+            if (mixed)
+                hipLaunchKernelGGL((k_blk_resid<T, B, S>), dim3(gr), dim3(TB), 0,
+                                   s, m, va64, (const double *)A->q_diag,
+                                   (const double *)g_bw.b,
+                                   (const double *)g_bw.x, r, (T *)nullptr,
+                                   g_bw.part_up, (const BState *)g_bw.st, 0);
+        }
+        PHG_HIP_CHECK(hipGetLastError());
+        PHG_HIP_CHECK(hipMemcpyAsync(hm, g_bw.st, sizeof(BState),
+                                     hipMemcpyDeviceToHost, s));
+        PHG_HIP_CHECK(hipStreamSynchronize(s));
+        if (hm->all_done)
+            break;
+        if (slot > 2 * (int64_t)maxit + 4096)
+            return phg::set_error(PHG_HIP_E_HIP, "block PCG did not finish");
+        chunk = std::min(chunk * 2, 64);
+    }
+    *hfin = *hm;
+    if (mixed)
+        hipLaunchKernelGGL(k_blk_fold, dim3(gv), dim3(TB), 0, s, total, g_bw.x,
+                           (const float *)g_bw.x32);
+    PHG_HIP_CHECK(hipGetLastError());
+    return PHG_HIP_OK;
+}
+
+template <typename T, int B>
+int block_solve_s(phg_hip_mat *A, int S, int mixed, int32_t maxit, double rtol,
+                  double btol, double atol, double delta, BState *hfin)
+{
+    switch (S) {
+    case 1: return block_solve<T, B, 1>(A, mixed, maxit, rtol, btol, atol, delta, hfin);
+    case 2: return block_solve<T, B, 2>(A, mixed, maxit, rtol, btol, atol, delta, hfin);
+    default: return block_solve<T, B, 4>(A, mixed, maxit, rtol, btol, atol, delta, hfin);
+    }
+}
+
+// time `reps` SpMM launches of one shape (all columns active)
+template <typename T, int B, int S>
+int time_spmm(phg_hip_mat *A, int reps, double *us)
+{
+    hipStream_t s = phg::ctx()->stream;
+    const int64_t n = A->n;
+    const BsrView m{A->b_ptr, A->b_col, A->b_nb};
+    const T *va = (const T *)mat_vals<T, B>(A);
+    constexpr int RPG = (64 / (S * KB)) * (TB / 64);
+    const int64_t units = (A->b_nb + RPG - 1) / RPG;
+    const int gs = grid_res<T>((const void *)k_blk_spmm<T, B, S>, units);
+    BState h;
+    memset(&h, 0, sizeof(h));
+    h.kind = ST_ITER;
+    PHG_HIP_CHECK(hipMemcpyAsync(g_bw.st, &h, sizeof(h), hipMemcpyHostToDevice, s));
+    PHG_HIP_CHECK(hipMemsetAsync(g_bw.p, 0, (size_t)n * KB * sizeof(T), s));
+    hipEvent_t e0, e1;
+    PHG_HIP_CHECK(hipEventCreate(&e0));
+    PHG_HIP_CHECK(hipEventCreate(&e1));
+    for (int i = 0; i < reps + 2; i++) {
+        if (i == 2)
+            PHG_HIP_CHECK(hipEventRecord(e0, s));
+        hipLaunchKernelGGL((k_blk_spmm<T, B, S>), dim3(gs), dim3(TB), 0, s, m,
+                           va, (const T *)g_bw.p, (T *)g_bw.Ap, g_bw.part_s,
+                           (const BState *)g_bw.st);
+    }
+    PHG_HIP_CHECK(hipEventRecord(e1, s));
+    PHG_HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    PHG_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *us = ms * 1e3 / reps;
+    return PHG_HIP_OK;
+}
+
+template <typename T, int B>
+int time_spmm_s(phg_hip_mat *A, int S, int reps, double *us)
+{
+    switch (S) {
+    case 1: return time_spmm<T, B, 1>(A, reps, us);
+    case 2: return time_spmm<T, B, 2>(A, reps, us);
+    default: return time_spmm<T, B, 4>(A, reps, us);
+    }
+}
+
+const int kShapes[3] = {1, 2, 4};
+
+}  // namespace
+
+//================================================================= C ABI
+
+extern "C" int phg_hip_pcg_block(const phg_hip_mat *A_, int32_t nrhs,
+                                 const phg_hip_vec *const *b,
+                                 phg_hip_vec *const *x, int32_t maxit,
+                                 double rtol, double btol, double atol,
+                                 int32_t mixed, double delta, int32_t *nits,
+                                 double *residual)
+{
+    int rc;
+    if ((rc = phg::require_ctx()))
+        return rc;
+    phg_hip_mat *A = const_cast<phg_hip_mat *>(A_);
+    if (A == nullptr || b == nullptr || x == nullptr || nrhs < 1 || nrhs > KB)
+        return phg::set_error(PHG_HIP_E_ARG,
+                              "phg_hip_pcg_block: 1 <= nrhs <= %d required", KB);
+    if (!A->eliminated || A->d_diag == nullptr)
+        return phg::set_error(PHG_HIP_E_ARG,
+                              "matrix not assembled: call phg_hip_mat_eliminate");
+    for (int c = 0; c < nrhs; c++)
+        if (b[c] == nullptr || x[c] == nullptr || b[c]->n != A->n ||
+            x[c]->n != A->n)
+            return phg::set_error(PHG_HIP_E_ARG, "phg_hip_pcg_block: vector %d", c);
+    if (mixed && !(delta > 0.0 && delta < 1.0))
+        return phg::set_error(PHG_HIP_E_ARG, "delta must lie in (0, 1)");
+    if ((rc = phg::build_bsr(A)) || (rc = get_work(A->n)))
+        return rc;
+    hipStream_t s = phg::ctx()->stream;
+    const int64_t n = A->n, total = n * KB;
+    const int gv = grid_vec(total);
+    PtrTab t;
+    memset(&t, 0, sizeof(t));
+    for (int c = 0; c < nrhs; c++) {
+        t.src[c] = b[c]->d;
+        t.dst[c] = x[c]->d;
+    }
+    hipLaunchKernelGGL(k_blk_pack, dim3(gv), dim3(TB), 0, s, n, nrhs,
+                       (const int32_t *)A->q_perm, t, g_bw.b);
+    for (int c = 0; c < nrhs; c++)
+        t.src[c] = x[c]->d;
+    hipLaunchKernelGGL(k_blk_pack, dim3(gv), dim3(TB), 0, s, n, nrhs,
+                       (const int32_t *)A->q_perm, t, g_bw.x);
+    PHG_HIP_CHECK(hipGetLastError());
+    BState fin;
+    const int S = A->spmm_shape[mixed ? 1 : 0] > 0 ? A->spmm_shape[mixed ? 1 : 0] : 1;
+    if (A->b_dim == 3)
+        rc = mixed ? block_solve_s<float, 3>(A, S, 1, maxit, rtol, btol, atol, delta, &fin)
+                   : block_solve_s<double, 3>(A, S, 0, maxit, rtol, btol, atol, delta, &fin);
+    else
+        rc = mixed ? block_solve_s<float, 1>(A, S, 1, maxit, rtol, btol, atol, delta, &fin)
+                   : block_solve_s<double, 1>(A, S, 0, maxit, rtol, btol, atol, delta, &fin);
+    if (rc)
+        return rc;
+    int zmask = 0;
+    for (int c = 0; c < nrhs; c++)
+        if (fin.zero_x[c])
+            zmask |= 1 << c;
+    hipLaunchKernelGGL(k_blk_unpack, dim3(gv), dim3(TB), 0, s, n, nrhs,
+                       (const int32_t *)A->q_perm, (const double *)g_bw.x, t,
+                       zmask);
+    PHG_HIP_CHECK(hipGetLastError());
+    PHG_HIP_CHECK(hipStreamSynchronize(s));
+    int32_t mx = 0;
+    for (int c = 0; c < nrhs; c++) {
+        const int32_t it = fin.nits[c];
+        if (nits)
+            nits[c] = it;
+        if (residual)
+            residual[c] = fin.res[c];
+        mx = std::max(mx, it);
+    }
+    return mx;
+}
+
+extern "C" int phg_hip_spmm_tune(phg_hip_mat *A, int32_t reps)
+{
+    int rc;
+    if ((rc = phg::require_ctx()))
+        return rc;
+    if (A == nullptr || !A->eliminated)
+        return phg::set_error(PHG_HIP_E_ARG, "phg_hip_spmm_tune: matrix not assembled");
+    if ((rc = phg::build_bsr(A)) || (rc = get_work(A->n)))
+        return rc;
+    if (reps <= 0)
+        reps = 10;
+    for (int prec = 0; prec < 2; prec++) {
+        int best = 0;
+        for (int k = 0; k < 3; k++) {
+            double us = 0.0;
+            const int S = kShapes[k];
+            if (A->b_dim == 3)
+                rc = prec ? time_spmm_s<float, 3>(A, S, reps, &us)
+                          : time_spmm_s<double, 3>(A, S, reps, &us);
+            else
+                rc = prec ? time_spmm_s<float, 1>(A, S, reps, &us)
+                          : time_spmm_s<double, 1>(A, S, reps, &us);
+            if (rc)
+                return rc;
+            A->spmm_us[k][prec] = us;
+            if (us < A->spmm_us[best][prec])
+                best = k;
+        }
+        A->spmm_shape[prec] = kShapes[best];
+    }
+    return PHG_HIP_OK;
+}
+
+extern "C" int phg_hip_spmm_tune_time(const phg_hip_mat *A, int32_t shape,
+                                      int32_t fp32, double *us)
+{
+    if (A == nullptr || us == nullptr || shape < 0 || shape > 2)
+        return phg::set_error(PHG_HIP_E_ARG, "phg_hip_spmm_tune_time: bad args");
+    *us = A->spmm_us[shape][fp32 ? 1 : 0];
+    return PHG_HIP_OK;
+}
+
+extern "C" int phg_hip_spmm_bytes(const phg_hip_mat *A, int32_t fp32,
+                                  int64_t *bytes)
+{
+    if (A == nullptr || bytes == nullptr || A->b_dim == 0)
+        return phg::set_error(PHG_HIP_E_STATE, "block layout not built");
+    const int64_t sv = fp32 ? 4 : 8, B = A->b_dim;
+    // blocks (values + column) + block row pointers + one read of X and
+    // one write of Y, KB columns each
+    *bytes = A->b_nblk * (B * B * sv + 4) + (A->b_nb + 1) * 4 +
+             2 * A->n * KB * sv;
+    return PHG_HIP_OK;
+}

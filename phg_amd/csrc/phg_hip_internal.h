@@ -119,6 +119,17 @@ struct phg_hip_mat {
     uint16_t *c_idx = nullptr;
     float *c_val32 = nullptr;
     double *c_val64 = nullptr;
+    // block copy for the many-right-hand-side solver (block.hip): 3x3
+    // blocks for 3-component problems (b_dim 3), else the solver CSR itself
+    // (b_dim 1, arrays aliased); b_dim 0: not built
+    int b_dim = 0, b_owns = 0;
+    int64_t b_nb = 0, b_nblk = 0;
+    int32_t *b_ptr = nullptr, *b_col = nullptr;
+    double *b_val64 = nullptr;
+    float *b_val32 = nullptr;
+    // SpMM wavefront shape (slots per row) per precision and tuner times
+    int spmm_shape[2] = {0, 0};
+    double spmm_us[3][2] = {};
 };
 
 namespace phg {
@@ -139,6 +150,9 @@ int split_row_blocks(const int32_t *d_rowptr, int64_t n, int64_t nnz,
 int build_sell(phg_hip_mat *A);
 void free_sell(phg_hip_mat *A);
 int ensure_fp32_copy(phg_hip_mat *A);
+// block copy for many right-hand sides (block.hip)
+int build_bsr(phg_hip_mat *A);
+void free_bsr(phg_hip_mat *A);
 // dictionary-compressed CSR (dictcsr.hip)
 int build_dict(phg_hip_mat *A);
 void free_dict(phg_hip_mat *A);

@@ -107,6 +107,7 @@ namespace phg {
 void free_solver_copy(phg_hip_mat *A)
 {
     free_dict(A);
+    free_bsr(A);
     if (A->q_owns) {
         dfree(A->q_rowptr);
         dfree(A->q_cols);
