@@ -182,14 +182,26 @@ int phg_hip_pcg_block(const phg_hip_mat *A, int32_t nrhs,
                       int32_t maxit, double rtol, double btol, double atol,
                       int32_t mixed, double delta, int32_t *nits,
                       double *residual);
-/* SpMM autotuner: times the wavefront shapes (1, 2 or 4 lanes per row and
- * right-hand side; shape index 0..2) in FP64 and FP32 and keeps the
- * fastest for phg_hip_pcg_block.  phg_hip_spmm_tune_time reports the
+/* SpMM autotuner: times the wavefront shapes -- index 0..5 = (slots per
+ * block row, columns per lane) (1,1) (2,1) (4,1) (4,4) (8,4) (16,4); a lane
+ * with 4 columns loads them with one 16-byte access -- in FP64 and FP32 and
+ * keeps the fastest for phg_hip_pcg_block.  phg_hip_spmm_tune_time reports the
  * measured microseconds; phg_hip_spmm_bytes the bytes one SpMM moves. */
 int phg_hip_spmm_tune(phg_hip_mat *A, int32_t reps);
 int phg_hip_spmm_tune_time(const phg_hip_mat *A, int32_t shape, int32_t fp32,
                            double *us);
 int phg_hip_spmm_bytes(const phg_hip_mat *A, int32_t fp32, int64_t *bytes);
+int phg_hip_spmm_set_shape(phg_hip_mat *A, int32_t shape64, int32_t shape32);
+
+/*------------------------------------------------------------ DOF norms */
+
+/* dof-utils.c:2872 phgDofNormL2_ of a scalar Pk DOF v (values in PHG
+ * numbering) and the L2 norm of its gradient (dof-utils.c:875
+ * phgDofGradient_ then phgDofNormL2_), by element quadrature of orders 2p
+ * and 2p-2, summed in a fixed order.  Drivers report the error of a
+ * solution as the norms of u_h - I_h u (examples/poisson.c:437-460). */
+int phg_hip_dof_norms(phg_hip_space *sp, const phg_hip_vec *v, double *l2,
+                      double *grad_l2);
 
 /*---------------------------------------------------------------- adjoint */
 

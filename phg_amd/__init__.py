@@ -23,7 +23,7 @@ _LIB = os.path.join(_HERE, "lib")
 
 __all__ = [
     "Grid", "Space", "Mat", "Vec", "Problem", "init", "finalize",
-    "assemble", "element_matrices", "pcg", "pcg_mixed", "pcg_reliable", "pcg_block", "spmm_tune", "adjoint_logkappa", "PhgHipError",
+    "assemble", "element_matrices", "pcg", "pcg_mixed", "pcg_reliable", "pcg_block", "dof_norms", "spmm_tune", "adjoint_logkappa", "PhgHipError",
     "hip_lib", "grid_lib", "HIP_LIB_PATH", "GRID_LIB_PATH",
 ]
 
@@ -102,6 +102,7 @@ _sig(hip_lib, "phg_hip_element_matrices", C.c_int,
 _sig(hip_lib, "phg_hip_assemble", C.c_int,
      [_vp, _P(Problem), _P(_dbl), _P(_dbl), _P(_dbl), _P(_vp)])
 _sig(hip_lib, "phg_hip_mat_eliminate", C.c_int, [_vp])
+_sig(hip_lib, "phg_hip_dof_norms", C.c_int, [_vp, _vp, _P(_dbl), _P(_dbl)])
 _sig(hip_lib, "phg_hip_adjoint_logkappa", C.c_int,
      [_vp, _P(Problem), _P(_dbl), _vp, _vp, _vp, _i32, _dbl, _P(_dbl), _P(_dbl), _vp,
       _P(_i32)])
@@ -130,6 +131,7 @@ _sig(hip_lib, "phg_hip_pcg_block", C.c_int,
 _sig(hip_lib, "phg_hip_spmm_tune", C.c_int, [_vp, _i32])
 _sig(hip_lib, "phg_hip_spmm_tune_time", C.c_int, [_vp, _i32, _i32, _P(_dbl)])
 _sig(hip_lib, "phg_hip_spmm_bytes", C.c_int, [_vp, _i32, _P(_i64)])
+_sig(hip_lib, "phg_hip_spmm_set_shape", C.c_int, [_vp, _i32, _i32])
 _sig(hip_lib, "phg_hip_mat_tune", C.c_int, [_vp, _i32])
 _sig(hip_lib, "phg_hip_mat_tune_time", C.c_int, [_vp, _i32, _i32, _P(_dbl)])
 _sig(hip_lib, "phg_hip_mat_set_format", C.c_int, [_vp, _i32, _i32])
@@ -426,6 +428,13 @@ def assemble(sp: Space, pb: Problem, kappa=None, f=None, g=None) -> Mat:
     return Mat(h)
 
 
+def dof_norms(sp: Space, v: Vec):
+    """(||v_h||_L2, ||grad v_h||_L2) of a scalar DOF (phgDofNormL2)."""
+    a, b = _dbl(), _dbl()
+    _check(hip_lib.phg_hip_dof_norms(sp._h, v._h, C.byref(a), C.byref(b)), "dof_norms")
+    return a.value, b.value
+
+
 def adjoint_logkappa(sp: Space, pb: Problem, kappa, A: Mat, u: Vec, ud: Vec,
                      maxit=100000, rtol=1e-12, lam: Vec | None = None):
     """Adjoint gradient of J = 1/2 int (u_h - ud_h)^2 with respect to log kappa_e.
@@ -442,7 +451,7 @@ def adjoint_logkappa(sp: Space, pb: Problem, kappa, A: Mat, u: Vec, ud: Vec,
 
 
 BLOCK_MAX = 16
-SPMM_SHAPES = (1, 2, 4)
+SPMM_SHAPES = ((1, 1), (2, 1), (4, 1), (4, 4), (8, 4), (16, 4))
 
 
 def pcg_block(A: "Mat", bs, xs, maxit=100000, rtol=1e-10, btol=1e-10, atol=0.0,
@@ -475,6 +484,11 @@ def spmm_tune(A: "Mat", reps: int = 10) -> dict:
             t.append(us.value)
         out[sh] = tuple(t)
     return out
+
+
+def spmm_set_shape(A: "Mat", shape64: int, shape32: int) -> None:
+    """Selects SpMM shapes by index into SPMM_SHAPES."""
+    _check(hip_lib.phg_hip_spmm_set_shape(A._h, shape64, shape32), "spmm_set_shape")
 
 
 def spmm_bytes(A: "Mat", fp32: bool) -> int:

@@ -679,6 +679,65 @@ __global__ void __launch_bounds__(128) k_adj_grad(AdjArgs a)
     a.grad[e] = -a.kappa[e] * (s * vol);
 }
 
+// dof-utils.c:2872 phgDofNormL2_ and the L2 norm of dof-utils.c:875
+// phgDofGradient_ for a scalar Pk DOF e: per element
+//   l2[e] = vol sum_q w_q e_h(x_q)^2        (mass rule, order 2p)
+//   h1[e] = vol sum_q w_q |grad e_h(x_q)|^2 (stiffness rule, order 2p-2)
+template <int P>
+__global__ void __launch_bounds__(128) k_dof_norms(int64_t nelem,
+                                                   const double *verts,
+                                                   const int32_t *elem_verts,
+                                                   const int32_t *map,
+                                                   const double *v,
+                                                   double *l2, double *h1)
+{
+    constexpr int N = P == 1 ? 4 : (P == 2 ? 10 : 20);
+    constexpr int NQS = P == 1 ? 1 : (P == 2 ? 4 : 14);
+    __shared__ Tables T;
+    build_tables<P>(T, 1, -1, 1);
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nelem)
+        return;
+    double x[4][3];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        int64_t iv = elem_verts[4 * e + k];
+        x[k][0] = verts[3 * iv + 0];
+        x[k][1] = verts[3 * iv + 1];
+        x[k][2] = verts[3 * iv + 2];
+    }
+    double J[16];
+    jacobian(x, J);
+    const double vol = volume(x);
+    const int32_t *m = map + e * (int64_t)N;
+    double el[N];
+#pragma unroll
+    for (int k = 0; k < N; k++)
+        el[k] = v[m[k]];
+    double a = 0.0;
+    for (int q = 0; q < T.nq_m; q++) {
+        double t = 0.0;
+#pragma unroll
+        for (int k = 0; k < N; k++)
+            t += T.bm[q][k] * el[k];
+        a += t * t * T.wm[q];
+    }
+    l2[e] = a * vol;
+    double b = 0.0;
+    for (int q = 0; q < NQS; q++) {
+        double g[3] = {0.0, 0.0, 0.0};
+        for (int k = 0; k < N; k++) {
+            double gk[3];
+            xgrad<P>(k, T.gs[q][k], J, gk);
+            g[0] += el[k] * gk[0];
+            g[1] += el[k] * gk[1];
+            g[2] += el[k] * gk[2];
+        }
+        b += (g[0] * g[0] + g[1] * g[1] + g[2] * g[2]) * T.ws[q];
+    }
+    h1[e] = b * vol;
+}
+
 // fixed-order sum of n values: per-block partials, then one block
 __global__ void __launch_bounds__(256) k_sum_blocks(int64_t n, const double *v,
                                                     double *part)
@@ -1637,4 +1696,78 @@ done:
     if (lam)
         phg_hip_vec_destroy(lam);
     return rc;
+}
+
+//------------------------------------------------------------- DOF norms
+namespace {
+int sum_device(const double *v, int64_t n, double *out)
+{
+    hipStream_t s = phg::ctx()->stream;
+    const int nparts = 256;
+    double *part = nullptr;
+    int rc;
+    if ((rc = phg::dmalloc((void **)&part, nparts * sizeof(double))))
+        return rc;
+    hipLaunchKernelGGL(k_sum_blocks, dim3(nparts), dim3(256), 0, s, n, v, part);
+    std::vector<double> hp(nparts);
+    rc = phg::download(hp.data(), part, nparts * sizeof(double));
+    phg::dfree(part);
+    if (rc)
+        return rc;
+    double t = 0.0;
+    for (int i = 0; i < nparts; i++)
+        t += hp[(size_t)i];
+    *out = t;
+    return PHG_HIP_OK;
+}
+}  // namespace
+
+extern "C" int phg_hip_dof_norms(phg_hip_space *sp, const phg_hip_vec *v,
+                                 double *l2, double *h1)
+{
+    int rc;
+    if ((rc = phg::require_ctx()))
+        return rc;
+    if (sp == nullptr || v == nullptr || v->n != sp->ndof)
+        return phg::set_error(PHG_HIP_E_ARG,
+                              "phg_hip_dof_norms: a scalar DOF of the space required");
+    hipStream_t s = phg::ctx()->stream;
+    const int64_t ne = sp->nelem;
+    double *dl = nullptr, *dh = nullptr;
+    if ((rc = phg::dmalloc((void **)&dl, (size_t)ne * sizeof(double))) ||
+        (rc = phg::dmalloc((void **)&dh, (size_t)ne * sizeof(double)))) {
+        phg::dfree(dl);
+        return rc;
+    }
+    dim3 g(nblk(ne, 128)), b(128);
+    if (sp->order == 1)
+        hipLaunchKernelGGL(k_dof_norms<1>, g, b, 0, s, ne,
+                           (const double *)sp->d_verts,
+                           (const int32_t *)sp->d_elem_verts,
+                           (const int32_t *)sp->d_map, (const double *)v->d, dl, dh);
+    else if (sp->order == 2)
+        hipLaunchKernelGGL(k_dof_norms<2>, g, b, 0, s, ne,
+                           (const double *)sp->d_verts,
+                           (const int32_t *)sp->d_elem_verts,
+                           (const int32_t *)sp->d_map, (const double *)v->d, dl, dh);
+    else
+        hipLaunchKernelGGL(k_dof_norms<3>, g, b, 0, s, ne,
+                           (const double *)sp->d_verts,
+                           (const int32_t *)sp->d_elem_verts,
+                           (const int32_t *)sp->d_map, (const double *)v->d, dl, dh);
+    double a = 0.0, c = 0.0;
+    if ((rc = hipGetLastError() == hipSuccess ? PHG_HIP_OK
+                                              : phg::set_error(PHG_HIP_E_HIP, "norm kernel")) ||
+        (rc = sum_device(dl, ne, &a)) || (rc = sum_device(dh, ne, &c))) {
+        phg::dfree(dl);
+        phg::dfree(dh);
+        return rc;
+    }
+    phg::dfree(dl);
+    phg::dfree(dh);
+    if (l2)
+        *l2 = sqrt(a);
+    if (h1)
+        *h1 = sqrt(c);
+    return PHG_HIP_OK;
 }

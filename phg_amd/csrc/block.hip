@@ -31,6 +31,7 @@
 #include <math.h>
 #include <string.h>
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 namespace {
@@ -120,77 +121,118 @@ __global__ void k_i64_to_i32(int64_t n, const int64_t *a, int32_t *b)
 }
 
 //------------------------------------------------------------------ SpMM
-// Block CSR with B x B blocks (B = 1: the CSR itself).  Lane mapping inside
-// a wavefront: S slots per row, KB columns: lane = (row * S + s) * KB + c;
-// 64 / (S * KB) block rows per wavefront.  Slot s takes the row's blocks
-// s, s + S, ...; the S partial sums are combined with shuffles.
+// Block CSR with B x B blocks (B = 1: the CSR itself).  A lane works on CPL
+// consecutive columns of the multivector (CPL = 1, or 4 with 16-byte
+// loads); NCG = KB / CPL column groups.  Lane mapping inside a wavefront:
+// lane = (row * S + s) * NCG + cg, 64 / (S * NCG) block rows per wavefront.
+// Slot s takes the row's blocks s, s + S, ...; the S partial sums are
+// combined with shuffles.
 struct BsrView {
     const int32_t *bptr, *bcol;
     int64_t nb;          // block rows
 };
 
-template <typename T, int B, int S, typename Epi>
+template <typename T, int CPL> struct Cols { T v[CPL]; };
+
+template <typename T, int CPL>
+__device__ inline Cols<T, CPL> load_cols(const T *p)
+{
+    Cols<T, CPL> r;
+    if (CPL == 1)
+        r.v[0] = p[0];
+    else if (sizeof(T) == 4) {
+        const float4 a = *(const float4 *)p;
+        r.v[0] = a.x; r.v[1] = a.y; r.v[2] = a.z; r.v[3] = a.w;
+    }
+    else {
+        const double2 a = ((const double2 *)p)[0], b = ((const double2 *)p)[1];
+        r.v[0] = a.x; r.v[1] = a.y; r.v[2] = b.x; r.v[3] = b.y;
+    }
+    return r;
+}
+
+// block rows one work-group step covers for a shape
+template <int S, int CPL>
+constexpr int rows_per_group() { return (64 / (S * (KB / CPL))) * (TB / 64); }
+
+template <typename T, int B, int S, int CPL, typename Epi>
 __device__ inline void bsr_spmm(const BsrView &m, const T *__restrict__ va,
                                 const T *__restrict__ X, Epi epi)
 {
-    constexpr int LPR = S * KB;          // lanes per block row
+    constexpr int NCG = KB / CPL;
+    constexpr int LPR = S * NCG;         // lanes per block row
     constexpr int RPW = 64 / LPR;        // block rows per wavefront
     constexpr int RPG = RPW * (TB / 64); // block rows per work-group step
+    static_assert(LPR <= 64 && 64 % LPR == 0, "wavefront shape");
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int c = lane % KB, s = (lane / KB) % S, rw = lane / LPR;
+    const int cg = lane % NCG, s = (lane / NCG) % S, rw = lane / LPR;
     int64_t g0, g1;
     group_range((m.nb + RPG - 1) / RPG, g0, g1);
     for (int64_t gi = g0; gi < g1; gi++) {
         const int64_t R = gi * RPG + wv * RPW + rw;
-        T acc[B];
+        T acc[B][CPL];
 #pragma unroll
         for (int p = 0; p < B; p++)
-            acc[p] = 0;
+#pragma unroll
+            for (int i = 0; i < CPL; i++)
+                acc[p][i] = 0;
         if (R < m.nb) {
             const int32_t j1 = m.bptr[R + 1];
 #pragma unroll 2
             for (int32_t j = m.bptr[R] + s; j < j1; j += S) {
                 const int64_t C = m.bcol[j];
-                T xq[B];
+                Cols<T, CPL> xq[B];
 #pragma unroll
                 for (int q = 0; q < B; q++)
-                    xq[q] = X[(C * B + q) * KB + c];
+                    xq[q] = load_cols<T, CPL>(X + (C * B + q) * KB + cg * CPL);
                 const T *a = va + (int64_t)j * (B * B);
 #pragma unroll
                 for (int p = 0; p < B; p++)
 #pragma unroll
-                    for (int q = 0; q < B; q++)
-                        acc[p] += a[p * B + q] * xq[q];
+                    for (int q = 0; q < B; q++) {
+                        const T apq = a[p * B + q];
+#pragma unroll
+                        for (int i = 0; i < CPL; i++)
+                            acc[p][i] += apq * xq[q].v[i];
+                    }
             }
         }
 #pragma unroll
-        for (int o = KB; o < LPR; o <<= 1)
+        for (int o = NCG; o < LPR; o <<= 1)
 #pragma unroll
             for (int p = 0; p < B; p++)
-                acc[p] += __shfl_xor(acc[p], o, 64);
+#pragma unroll
+                for (int i = 0; i < CPL; i++)
+                    acc[p][i] += __shfl_xor(acc[p][i], o, 64);
         if (R < m.nb && s == 0)
 #pragma unroll
             for (int p = 0; p < B; p++)
-                epi(R * B + p, c, acc[p]);
+#pragma unroll
+                for (int i = 0; i < CPL; i++)
+                    epi(R * B + p, cg * CPL + i, i, acc[p][i]);
     }
 }
 
-// per-column sums of a work-group: v[c] held by the lanes of column c
-// (lane % KB == c); written to part[(k * KB + c) * G + block]
-template <int NV>
-__device__ inline void write_col_partials(double (&v)[NV], double *part)
+// per-column sums of a work-group: v[k][i] held by the lanes of column
+// group cg (columns cg * CPL + i, cg = lane % (KB / CPL)); written to
+// part[(k * KB + c) * G + block]
+template <int NV, int CPL>
+__device__ inline void write_col_partials(double (&v)[NV][CPL], double *part)
 {
+    constexpr int NCG = KB / CPL;
     __shared__ double sh[NV][TB / 64][KB];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-    for (int k = 0; k < NV; k++) {
-        double t = v[k];
+    for (int k = 0; k < NV; k++)
 #pragma unroll
-        for (int o = KB; o < 64; o <<= 1)
-            t += __shfl_xor(t, o, 64);
-        if (lane < KB)
-            sh[k][wv][lane] = t;
-    }
+        for (int i = 0; i < CPL; i++) {
+            double t = v[k][i];
+#pragma unroll
+            for (int o = NCG; o < 64; o <<= 1)
+                t += __shfl_xor(t, o, 64);
+            if (lane < NCG)
+                sh[k][wv][lane * CPL + i] = t;
+        }
     __syncthreads();
     if (threadIdx.x < NV * KB) {
         const int k = threadIdx.x / KB, c = threadIdx.x % KB;
@@ -378,19 +420,19 @@ __global__ void __launch_bounds__(TB) k_blk_p(int64_t n, const T *r,
     }
 }
 
-template <typename T, int B, int S>
+template <typename T, int B, int S, int CPL>
 __global__ void __launch_bounds__(TB) k_blk_spmm(BsrView m, const T *va,
                                                 const T *p, T *Ap,
                                                 double *part, const BState *st)
 {
     if (st->all_done || st->kind != ST_ITER)
         return;
-    double v[1] = {0.0};
-    bsr_spmm<T, B, S>(m, va, p, [&](int64_t row, int c, T s) {
+    double v[1][CPL] = {};
+    bsr_spmm<T, B, S, CPL>(m, va, p, [&](int64_t row, int c, int i, T s) {
         Ap[row * KB + c] = s;
-        v[0] += (double)p[row * KB + c] * (double)s;
+        v[0][i] += (double)p[row * KB + c] * (double)s;
     });
-    write_col_partials<1>(v, part);
+    write_col_partials<1, CPL>(v, part);
 }
 
 __global__ void __launch_bounds__(TB) k_blk_alpha(const double *part, int np,
@@ -425,7 +467,7 @@ __global__ void __launch_bounds__(TB) k_blk_update(int64_t n, T *x, const T *p,
     const int c = threadIdx.x % KB;
     const bool act = st->active[c] != 0;
     const T alpha = (T)st->alpha[c];
-    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    double v[4][1] = {};
     const int64_t total = n * KB;
     for (int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x; i < total;
          i += (int64_t)gridDim.x * TB) {
@@ -436,15 +478,15 @@ __global__ void __launch_bounds__(TB) k_blk_update(int64_t n, T *x, const T *p,
             r[i] = ri;
         }
         const T z = ri / diag[i / KB];
-        v[0] += (double)ri * (double)ri;
-        v[1] += (double)ri * (double)z;
+        v[0][0] += (double)ri * (double)ri;
+        v[1][0] += (double)ri * (double)z;
     }
-    write_col_partials<4>(v, part);
+    write_col_partials<4, 1>(v, part);
 }
 
 // FP64 residual r = b - A x for every column; r_out (T) receives it;
 // partials (r,r), (r,z), #zero diagonals, (b,b)
-template <typename T, int B, int S>
+template <typename T, int B, int S, int CPL>
 __global__ void __launch_bounds__(TB) k_blk_resid(BsrView m, const double *va,
                                                  const double *diag,
                                                  const double *b,
@@ -454,21 +496,21 @@ __global__ void __launch_bounds__(TB) k_blk_resid(BsrView m, const double *va,
 {
     if (!force && (st->all_done || st->kind != ST_UPDATE))
         return;
-    double v[4] = {0.0, 0.0, 0.0, 0.0};
-    bsr_spmm<double, B, S>(m, va, x, [&](int64_t row, int c, double s) {
+    double v[4][CPL] = {};
+    bsr_spmm<double, B, S, CPL>(m, va, x, [&](int64_t row, int c, int k, double s) {
         const int64_t i = row * KB + c;
         const double bi = b[i], ri = bi - s, d = diag[row];
         r_out[i] = (T)ri;
         if (x32_zero)
             x32_zero[i] = (T)0;
-        v[0] += ri * ri;
+        v[0][k] += ri * ri;
         if (d == 0.0)
-            v[2] += 1.0;
+            v[2][k] += 1.0;
         else
-            v[1] += ri * (ri / d);
-        v[3] += bi * bi;
+            v[1][k] += ri * (ri / d);
+        v[3][k] += bi * bi;
     });
-    write_col_partials<4>(v, part);
+    write_col_partials<4, CPL>(v, part);
 }
 
 // pack / unpack between KB separate vectors (PHG numbering) and the
@@ -708,7 +750,7 @@ const void *mat_vals(const phg_hip_mat *A)
 }
 
 // one solve with element type T for the iteration and SpMM shape (B, S)
-template <typename T, int B, int S>
+template <typename T, int B, int S, int CPL>
 int block_solve(phg_hip_mat *A, int mixed, int32_t maxit, double rtol,
                 double btol, double atol, double delta, BState *hfin)
 {
@@ -719,10 +761,11 @@ int block_solve(phg_hip_mat *A, int mixed, int32_t maxit, double rtol,
     const double *va64 = B == 1 ? A->q_vals : A->b_val64;
     const T *diag = (const T *)(sizeof(T) == 4 ? (const void *)A->q_diag32
                                                : (const void *)A->q_diag);
-    constexpr int RPG = (64 / (S * KB)) * (TB / 64);
+    constexpr int RPG = rows_per_group<S, CPL>();
+    static_assert(RPG > 0, "shape");
     const int64_t units = (A->b_nb + RPG - 1) / RPG;
-    const int gs = grid_res<T>((const void *)k_blk_spmm<T, B, S>, units);
-    const int gr = grid_res<T>((const void *)k_blk_resid<T, B, S>, units);
+    const int gs = grid_res<T>((const void *)k_blk_spmm<T, B, S, CPL>, units);
+    const int gr = grid_res<T>((const void *)k_blk_resid<T, B, S, CPL>, units);
     const int gv = grid_vec(total);
     T *r = (T *)g_bw.r, *p = (T *)g_bw.p, *Ap = (T *)g_bw.Ap;
     // FP64 mode iterates on x itself; mixed mode on the FP32 increment
@@ -740,11 +783,19 @@ int block_solve(phg_hip_mat *A, int mixed, int32_t maxit, double rtol,
         h.active[c] = 1;
     PHG_HIP_CHECK(hipMemcpyAsync(g_bw.st, &h, sizeof(h), hipMemcpyHostToDevice, s));
     // prologue: r = b - A x in FP64 (x32 = 0 in mixed mode)
-    hipLaunchKernelGGL((k_blk_resid<T, B, S>), dim3(gr), dim3(TB), 0, s, m,
+    hipLaunchKernelGGL((k_blk_resid<T, B, S, CPL>), dim3(gr), dim3(TB), 0, s, m,
                        va64, (const double *)A->q_diag, (const double *)g_bw.b,
                        (const double *)g_bw.x, r, mixed ? (T *)g_bw.x32 : (T *)nullptr,
                        g_bw.part_up, (const BState *)g_bw.st, 1);
-    PHG_HIP_CHECK(hipGetLastError());
+    {
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess)
+            return phg::set_error(PHG_HIP_E_HIP,
+                                  "block PCG prologue launch (grid %d / %d / %d, "
+                                  "%lld block rows): %s",
+                                  gr, gs, gv, (long long)A->b_nb,
+                                  hipGetErrorString(e));
+    }
     BState *hm = (BState *)phg::ctx()->host_mailbox;
     static_assert(sizeof(BState) <= 4096, "mailbox size");
     int chunk = 8;
@@ -757,7 +808,7 @@ int block_solve(phg_hip_mat *A, int mixed, int32_t maxit, double rtol,
             hipLaunchKernelGGL(k_blk_p<T>, dim3(gv), dim3(TB), 0, s, n,
                                (const T *)r, diag, p, g_bw.x, (T *)g_bw.x32,
                                (const BState *)g_bw.st);
-            hipLaunchKernelGGL((k_blk_spmm<T, B, S>), dim3(gs), dim3(TB), 0, s,
+            hipLaunchKernelGGL((k_blk_spmm<T, B, S, CPL>), dim3(gs), dim3(TB), 0, s,
                                m, va, (const T *)p, Ap, g_bw.part_s,
                                (const BState *)g_bw.st);
             hipLaunchKernelGGL(k_blk_alpha, dim3(1), dim3(TB), 0, s,
@@ -766,7 +817,7 @@ int block_solve(phg_hip_mat *A, int mixed, int32_t maxit, double rtol,
                                (const T *)p, r, (const T *)Ap, diag,
                                g_bw.part_it, (const BState *)g_bw.st);
             if (mixed)
-                hipLaunchKernelGGL((k_blk_resid<T, B, S>), dim3(gr), dim3(TB), 0,
+                hipLaunchKernelGGL((k_blk_resid<T, B, S, CPL>), dim3(gr), dim3(TB), 0,
                                    s, m, va64, (const double *)A->q_diag,
                                    (const double *)g_bw.b,
                                    (const double *)g_bw.x, r, (T *)nullptr,
@@ -790,28 +841,48 @@ int block_solve(phg_hip_mat *A, int mixed, int32_t maxit, double rtol,
     return PHG_HIP_OK;
 }
 
-template <typename T, int B>
-int block_solve_s(phg_hip_mat *A, int S, int mixed, int32_t maxit, double rtol,
-                  double btol, double atol, double delta, BState *hfin)
+// SpMM shapes the tuner chooses from: (slots per row, columns per lane)
+constexpr int NSHAPE = 6;
+const int kShapeS[NSHAPE] = {1, 2, 4, 4, 8, 16};
+const int kShapeC[NSHAPE] = {1, 1, 1, 4, 4, 4};
+
+template <int V> using IC = std::integral_constant<int, V>;
+
+template <typename Fn>
+int with_shape(int k, Fn &&f)
 {
-    switch (S) {
-    case 1: return block_solve<T, B, 1>(A, mixed, maxit, rtol, btol, atol, delta, hfin);
-    case 2: return block_solve<T, B, 2>(A, mixed, maxit, rtol, btol, atol, delta, hfin);
-    default: return block_solve<T, B, 4>(A, mixed, maxit, rtol, btol, atol, delta, hfin);
+    switch (k) {
+    case 0: return f(IC<1>{}, IC<1>{});
+    case 1: return f(IC<2>{}, IC<1>{});
+    case 2: return f(IC<4>{}, IC<1>{});
+    case 3: return f(IC<4>{}, IC<4>{});
+    case 4: return f(IC<8>{}, IC<4>{});
+    default: return f(IC<16>{}, IC<4>{});
     }
 }
 
+template <typename T, int B>
+int block_solve_s(phg_hip_mat *A, int shape, int mixed, int32_t maxit, double rtol,
+                  double btol, double atol, double delta, BState *hfin)
+{
+    return with_shape(shape, [&](auto S, auto C) {
+        return block_solve<T, B, decltype(S)::value, decltype(C)::value>(
+            A, mixed, maxit, rtol, btol, atol, delta, hfin);
+    });
+}
+
 // time `reps` SpMM launches of one shape (all columns active)
-template <typename T, int B, int S>
+template <typename T, int B, int S, int CPL>
 int time_spmm(phg_hip_mat *A, int reps, double *us)
 {
     hipStream_t s = phg::ctx()->stream;
     const int64_t n = A->n;
     const BsrView m{A->b_ptr, A->b_col, A->b_nb};
     const T *va = (const T *)mat_vals<T, B>(A);
-    constexpr int RPG = (64 / (S * KB)) * (TB / 64);
+    constexpr int RPG = rows_per_group<S, CPL>();
+    static_assert(RPG > 0, "shape");
     const int64_t units = (A->b_nb + RPG - 1) / RPG;
-    const int gs = grid_res<T>((const void *)k_blk_spmm<T, B, S>, units);
+    const int gs = grid_res<T>((const void *)k_blk_spmm<T, B, S, CPL>, units);
     BState h;
     memset(&h, 0, sizeof(h));
     h.kind = ST_ITER;
@@ -823,7 +894,7 @@ int time_spmm(phg_hip_mat *A, int reps, double *us)
     for (int i = 0; i < reps + 2; i++) {
         if (i == 2)
             PHG_HIP_CHECK(hipEventRecord(e0, s));
-        hipLaunchKernelGGL((k_blk_spmm<T, B, S>), dim3(gs), dim3(TB), 0, s, m,
+        hipLaunchKernelGGL((k_blk_spmm<T, B, S, CPL>), dim3(gs), dim3(TB), 0, s, m,
                            va, (const T *)g_bw.p, (T *)g_bw.Ap, g_bw.part_s,
                            (const BState *)g_bw.st);
     }
@@ -838,16 +909,12 @@ int time_spmm(phg_hip_mat *A, int reps, double *us)
 }
 
 template <typename T, int B>
-int time_spmm_s(phg_hip_mat *A, int S, int reps, double *us)
+int time_spmm_s(phg_hip_mat *A, int shape, int reps, double *us)
 {
-    switch (S) {
-    case 1: return time_spmm<T, B, 1>(A, reps, us);
-    case 2: return time_spmm<T, B, 2>(A, reps, us);
-    default: return time_spmm<T, B, 4>(A, reps, us);
-    }
+    return with_shape(shape, [&](auto S, auto C) {
+        return time_spmm<T, B, decltype(S)::value, decltype(C)::value>(A, reps, us);
+    });
 }
-
-const int kShapes[3] = {1, 2, 4};
 
 }  // namespace
 
@@ -895,7 +962,7 @@ extern "C" int phg_hip_pcg_block(const phg_hip_mat *A_, int32_t nrhs,
                        (const int32_t *)A->q_perm, t, g_bw.x);
     PHG_HIP_CHECK(hipGetLastError());
     BState fin;
-    const int S = A->spmm_shape[mixed ? 1 : 0] > 0 ? A->spmm_shape[mixed ? 1 : 0] : 1;
+    const int S = A->spmm_shape[mixed ? 1 : 0];
     if (A->b_dim == 3)
         rc = mixed ? block_solve_s<float, 3>(A, S, 1, maxit, rtol, btol, atol, delta, &fin)
                    : block_solve_s<double, 3>(A, S, 0, maxit, rtol, btol, atol, delta, &fin);
@@ -938,9 +1005,9 @@ extern "C" int phg_hip_spmm_tune(phg_hip_mat *A, int32_t reps)
         reps = 10;
     for (int prec = 0; prec < 2; prec++) {
         int best = 0;
-        for (int k = 0; k < 3; k++) {
+        for (int k = 0; k < NSHAPE; k++) {
             double us = 0.0;
-            const int S = kShapes[k];
+            const int S = k;
             if (A->b_dim == 3)
                 rc = prec ? time_spmm_s<float, 3>(A, S, reps, &us)
                           : time_spmm_s<double, 3>(A, S, reps, &us);
@@ -953,7 +1020,7 @@ extern "C" int phg_hip_spmm_tune(phg_hip_mat *A, int32_t reps)
             if (us < A->spmm_us[best][prec])
                 best = k;
         }
-        A->spmm_shape[prec] = kShapes[best];
+        A->spmm_shape[prec] = best;
     }
     return PHG_HIP_OK;
 }
@@ -961,7 +1028,7 @@ extern "C" int phg_hip_spmm_tune(phg_hip_mat *A, int32_t reps)
 extern "C" int phg_hip_spmm_tune_time(const phg_hip_mat *A, int32_t shape,
                                       int32_t fp32, double *us)
 {
-    if (A == nullptr || us == nullptr || shape < 0 || shape > 2)
+    if (A == nullptr || us == nullptr || shape < 0 || shape >= NSHAPE)
         return phg::set_error(PHG_HIP_E_ARG, "phg_hip_spmm_tune_time: bad args");
     *us = A->spmm_us[shape][fp32 ? 1 : 0];
     return PHG_HIP_OK;
@@ -977,5 +1044,16 @@ extern "C" int phg_hip_spmm_bytes(const phg_hip_mat *A, int32_t fp32,
     // one write of Y, KB columns each
     *bytes = A->b_nblk * (B * B * sv + 4) + (A->b_nb + 1) * 4 +
              2 * A->n * KB * sv;
+    return PHG_HIP_OK;
+}
+
+extern "C" int phg_hip_spmm_set_shape(phg_hip_mat *A, int32_t shape64,
+                                      int32_t shape32)
+{
+    if (A == nullptr || shape64 < 0 || shape64 >= NSHAPE || shape32 < 0 ||
+        shape32 >= NSHAPE)
+        return phg::set_error(PHG_HIP_E_ARG, "phg_hip_spmm_set_shape: bad args");
+    A->spmm_shape[0] = shape64;
+    A->spmm_shape[1] = shape32;
     return PHG_HIP_OK;
 }

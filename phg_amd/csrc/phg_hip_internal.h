@@ -127,9 +127,9 @@ struct phg_hip_mat {
     int32_t *b_ptr = nullptr, *b_col = nullptr;
     double *b_val64 = nullptr;
     float *b_val32 = nullptr;
-    // SpMM wavefront shape (slots per row) per precision and tuner times
+    // SpMM wavefront shape index per precision (block.hip) and tuner times
     int spmm_shape[2] = {0, 0};
-    double spmm_us[3][2] = {};
+    double spmm_us[8][2] = {};
 };
 
 namespace phg {

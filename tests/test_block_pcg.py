@@ -40,13 +40,15 @@ def _oracle_solution(c, pb_o, f, tol=1e-14):
     return Ao, x
 
 
+@pytest.mark.parametrize("shape", [0, 2, 3, 5])
 @pytest.mark.parametrize("mixed", [False, True])
-def test_block_elasticity_16_rhs(hip, mixed):
+def test_block_elasticity_16_rhs(hip, mixed, shape):
     pa = hip
     c = elasticity_case(3, 2)
     c["kappa"] = None
     pb = pa.Problem.elasticity()
     sp, A, bs, fs = _loads(pa, c, pb, 16, seed=4)
+    pa.spmm_set_shape(A, shape, shape)
     xs = [pa.Vec(A.n) for _ in bs]
     nits, res = pa.pcg_block(A, bs, xs, rtol=1e-10, btol=1e-10, mixed=mixed)
     pb_o = orc.problem(2, dim=3)
@@ -63,8 +65,9 @@ def test_block_elasticity_16_rhs(hip, mixed):
         assert np.linalg.norm(xb - xo) / np.linalg.norm(xo) < 1e-8
 
 
+@pytest.mark.parametrize("shape", [1, 4])
 @pytest.mark.parametrize("mixed", [False, True])
-def test_block_scalar_5_rhs_residuals(hip, mixed):
+def test_block_scalar_5_rhs_residuals(hip, mixed, shape):
     """Scalar P2 kappa-Poisson, 5 columns (1x1 blocks): the returned
     residual is the FP64 residual of the returned x, below each column's
     tolerance."""
@@ -72,6 +75,7 @@ def test_block_scalar_5_rhs_residuals(hip, mixed):
     c = poisson_case(5, 2, "kappa")
     pb = pa.Problem.poisson(use_kappa=True)
     sp, A, bs, fs = _loads(pa, c, pb, 5, seed=8)
+    pa.spmm_set_shape(A, shape, shape)
     xs = [pa.Vec(A.n) for _ in bs]
     nits, res = pa.pcg_block(A, bs, xs, rtol=1e-11, btol=1e-11, mixed=mixed)
     pb_o = orc.problem(2, use_kappa=True)

@@ -303,3 +303,22 @@ def test_reliable_nonzero_guess_and_zero_rhs(hip):
     nits, upd, res = pa.pcg_reliable(A, z, x2)
     assert nits == 0 and res == 0.0
     assert np.all(x2.numpy() == 0.0)
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_dof_norms(hip, p):
+    """phgDofNormL2 of a DOF and of its gradient: closed forms for v = x
+    (1/3 and 1 on the unit cube) and the oracle's quadratic forms
+    v^T M v, v^T K1 v for a random DOF."""
+    pa = hip
+    c = poisson_case(3, p, "kappa")
+    sp = _space(pa, c)
+    l2, h1 = pa.dof_norms(sp, pa.Vec(c["ndof"], c["coords"][:, 0].copy()))
+    assert abs(l2 * l2 - 1.0 / 3.0) < 1e-13 and abs(h1 * h1 - 1.0) < 1e-13
+    v = np.random.default_rng(p).standard_normal(c["ndof"])
+    l2, h1 = pa.dof_norms(sp, pa.Vec(c["ndof"], v))
+    half_vMv, _ = orc.mass_apply(p, c["verts"], c["elem_verts"], c["map"], v)
+    vKv = -orc.adjoint_grad(p, c["verts"], c["elem_verts"], c["map"],
+                            np.ones(c["grid"].nelem), v, v).sum()
+    assert abs(l2 * l2 - 2 * half_vMv) <= 1e-12 * 2 * half_vMv
+    assert abs(h1 * h1 - vKv) <= 1e-12 * vKv

@@ -402,3 +402,17 @@ def test_oracle_adjoint_matches_central_differences(p, n):
         fd[e] = (oracle_objective(c, up, ud) - oracle_objective(c, um, ud)) / (2 * h)
     err = np.abs(g - fd).max() / np.abs(g).max()
     assert err < 1e-6, err
+
+
+def test_oracle_stiffness_energy_closed_form():
+    """-sum_e adjoint_grad(kappa=1, u=lam=v) = v^T K1 v = int |grad v|^2:
+    v = x gives 1, v = x + 2y - z gives 6 on the unit cube."""
+    from tests._cases import poisson_case
+    for p in (1, 2, 3):
+        c = poisson_case(2, p, "kappa")
+        xyz = c["coords"]
+        for v, want in ((xyz[:, 0].copy(), 1.0),
+                        (xyz[:, 0] + 2 * xyz[:, 1] - xyz[:, 2], 6.0)):
+            g = orc.adjoint_grad(p, c["verts"], c["elem_verts"], c["map"],
+                                 np.ones(c["grid"].nelem), v, v)
+            assert abs(-g.sum() - want) < 1e-13
