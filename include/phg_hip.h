@@ -163,6 +163,23 @@ int phg_hip_pcg_reliable(const phg_hip_mat *A, const phg_hip_vec *b,
                          double btol, double atol, double delta,
                          int32_t *updates, double *residual);
 
+/*------------------------------------------------------- many matrices */
+
+/* Batched solve of several independent systems A[i] x[i] = b[i] (no
+ * reference counterpart), each with the mixed-precision PCG of
+ * phg_hip_pcg_reliable (its own layout, tolerance and iteration count).
+ * The systems run concurrently on a pool of streams; with use_graph = 1 a
+ * fixed chunk of iterations of every system is captured once into a HIP
+ * graph (fork / join over the streams) and the graph is replayed until
+ * every system has converged, so each replay is one launch for the whole
+ * batch.  nmat <= 16; a matrix may appear once.  nits[i], residual[i]
+ * (optional) per system.  Returns the largest iteration count. */
+int phg_hip_pcg_multi(int32_t nmat, phg_hip_mat *const *A,
+                      const phg_hip_vec *const *b, phg_hip_vec *const *x,
+                      int32_t maxit, double rtol, double btol, double atol,
+                      double delta, int32_t use_graph, int32_t *nits,
+                      double *residual);
+
 /*------------------------------------------------ many right-hand sides */
 
 /* Block of right-hand sides (no reference counterpart: solver-phg.c:495

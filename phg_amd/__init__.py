@@ -23,7 +23,7 @@ _LIB = os.path.join(_HERE, "lib")
 
 __all__ = [
     "Grid", "Space", "Mat", "Vec", "Problem", "init", "finalize",
-    "assemble", "element_matrices", "pcg", "pcg_mixed", "pcg_reliable", "pcg_block", "dof_norms", "spmm_tune", "adjoint_logkappa", "PhgHipError",
+    "assemble", "element_matrices", "pcg", "pcg_mixed", "pcg_reliable", "pcg_block", "pcg_multi", "dof_norms", "spmm_tune", "adjoint_logkappa", "PhgHipError",
     "hip_lib", "grid_lib", "HIP_LIB_PATH", "GRID_LIB_PATH",
 ]
 
@@ -81,6 +81,8 @@ def _sig(lib, name, res, args):
 _sig(grid_lib, "phg_grid_new_kuhn", _P(_GridStruct), [_i32, _P(C.c_int)])
 _sig(grid_lib, "phg_grid_read_albert", _P(_GridStruct), [_cp])
 _sig(grid_lib, "phg_grid_write_albert", C.c_int, [_P(_GridStruct), _cp])
+_sig(grid_lib, "phg_grid_refine_toward", _P(_GridStruct),
+     [_P(_GridStruct), _P(_dbl), C.c_int, _dbl, _dbl, _P(C.c_int)])
 _sig(grid_lib, "phg_grid_free", None, [_P(_GridStruct)])
 _sig(grid_lib, "phg_dof_nbas", C.c_int, [C.c_int])
 _sig(grid_lib, "phg_dof_count", _i64, [_P(_GridStruct), C.c_int])
@@ -128,6 +130,9 @@ _sig(hip_lib, "phg_hip_pcg_reliable", C.c_int,
      [_vp, _vp, _vp, _i32, _dbl, _dbl, _dbl, _dbl, _P(_i32), _P(_dbl)])
 _sig(hip_lib, "phg_hip_pcg_block", C.c_int,
      [_vp, _i32, _P(_vp), _P(_vp), _i32, _dbl, _dbl, _dbl, _i32, _dbl, _P(_i32), _P(_dbl)])
+_sig(hip_lib, "phg_hip_pcg_multi", C.c_int,
+     [_i32, _P(_vp), _P(_vp), _P(_vp), _i32, _dbl, _dbl, _dbl, _dbl, _i32, _P(_i32),
+      _P(_dbl)])
 _sig(hip_lib, "phg_hip_spmm_tune", C.c_int, [_vp, _i32])
 _sig(hip_lib, "phg_hip_spmm_tune_time", C.c_int, [_vp, _i32, _i32, _P(_dbl)])
 _sig(hip_lib, "phg_hip_spmm_bytes", C.c_int, [_vp, _i32, _P(_i64)])
@@ -195,6 +200,15 @@ class Grid:
     @classmethod
     def read_albert(cls, path: str):
         return cls(grid_lib.phg_grid_read_albert(path.encode()))
+
+    def refine_toward(self, point, levels: int, radius: float = 0.45, shrink: float = 1.0,
+                      bcode=(1, 1, 1, 1, 1, 1)) -> "Grid":
+        """Locally refined copy: `levels` bisection sweeps (conforming closure)
+        of the elements within radius * shrink^l of `point` (Kuhn meshes)."""
+        pt = np.ascontiguousarray(point, dtype=np.float64)
+        bc = (C.c_int * 6)(*bcode)
+        return Grid(grid_lib.phg_grid_refine_toward(self._p, _ptr(pt, _dbl), int(levels),
+                                                    float(radius), float(shrink), bc))
 
     def write_albert(self, path: str) -> None:
         if grid_lib.phg_grid_write_albert(self._p, path.encode()) != 0:
@@ -468,6 +482,24 @@ def pcg_block(A: "Mat", bs, xs, maxit=100000, rtol=1e-10, btol=1e-10, atol=0.0,
     _check(hip_lib.phg_hip_pcg_block(A._h, k, barr, xarr, maxit, rtol, btol, atol,
                                      int(mixed), delta, _ptr(nits, _i32), _ptr(res, _dbl)),
            "pcg_block")
+    return nits, res
+
+
+def pcg_multi(As, bs, xs, maxit=100000, rtol=1e-10, btol=1e-10, atol=0.0, delta=0.1,
+              use_graph=True):
+    """Mixed-precision PCG on several independent systems at once (streams,
+    optionally one HIP graph replay per chunk).  Returns (nits[k], residual[k])."""
+    k = len(As)
+    if not (k == len(bs) == len(xs)) or not 1 <= k <= 16:
+        raise ValueError("1 <= len(As) == len(bs) == len(xs) <= 16 required")
+    aarr = (_vp * k)(*[A._h for A in As])
+    barr = (_vp * k)(*[b._h for b in bs])
+    xarr = (_vp * k)(*[x._h for x in xs])
+    nits = np.zeros(k, dtype=np.int32)
+    res = np.zeros(k)
+    _check(hip_lib.phg_hip_pcg_multi(k, aarr, barr, xarr, maxit, rtol, btol, atol, delta,
+                                     int(use_graph), _ptr(nits, _i32), _ptr(res, _dbl)),
+           "pcg_multi")
     return nits, res
 
 

@@ -1576,6 +1576,7 @@ extern "C" int phg_hip_mat_destroy(phg_hip_mat *A)
     phg::dfree(A->d_xyz);
     phg::free_sell(A);
     phg::free_solver_copy(A);
+    phg::free_workspace(A);
     delete A;
     return PHG_HIP_OK;
 }

@@ -31,6 +31,8 @@
 #include <stdio.h>
 #include <string.h>
 #include <algorithm>
+#include <functional>
+#include <memory>
 #include <type_traits>
 #include <vector>
 
@@ -39,6 +41,7 @@ namespace {
 constexpr int BS = 256;      // work-group size (4 wavefronts of 64)
 constexpr int NXCD = 8;
 constexpr int MAX_GRID = 2048;   // partial-sum slots per reduced value
+constexpr int MULTI_MAX = 16;    // matrices per batched solve
 
 // dispatch order places work-group b on XCD b % 8; return the logical index
 // that gives XCD x the x-th contiguous share of [0, gridDim.x)
@@ -1379,34 +1382,49 @@ template <> struct FmtData<SellView> {
     static const int32_t *perm(const phg_hip_mat *A) { return A->d_perm; }
 };
 
-Workspace *get_ws(const phg_hip_mat *A, int need32)
+void release_ws(Workspace *w)
 {
-    static Workspace ws;     // one solve at a time per process (one stream)
-    if (ws.n != A->n) {
-        phg::dfree(ws.r); phg::dfree(ws.p); phg::dfree(ws.Ap);
-        phg::dfree(ws.xp); phg::dfree(ws.bp);
-        phg::dfree(ws.partials); phg::dfree(ws.st); phg::dfree(ws.scal);
-        phg::dfree(ws.r32); phg::dfree(ws.p32); phg::dfree(ws.Ap32);
-        phg::dfree(ws.x32);
-        ws = Workspace();
+    phg::dfree(w->r); phg::dfree(w->p); phg::dfree(w->Ap);
+    phg::dfree(w->xp); phg::dfree(w->bp);
+    phg::dfree(w->partials); phg::dfree(w->st); phg::dfree(w->scal);
+    phg::dfree(w->r32); phg::dfree(w->p32); phg::dfree(w->Ap32);
+    phg::dfree(w->x32);
+    *w = Workspace();
+}
+
+// Solver vectors of a matrix: each matrix owns its workspace, so solves of
+// different matrices can be in flight at the same time (on different
+// streams); solves of one matrix are ordered.
+Workspace *get_ws(const phg_hip_mat *A_, int need32)
+{
+    phg_hip_mat *A = const_cast<phg_hip_mat *>(A_);
+    Workspace *w = (Workspace *)A->ws;
+    if (w == nullptr) {
+        w = new Workspace();
+        A->ws = w;
+    }
+    if (w->n != A->n) {
+        release_ws(w);
         size_t n8 = (size_t)A->n * sizeof(double);
-        if (phg::dmalloc((void **)&ws.r, n8) || phg::dmalloc((void **)&ws.p, n8) ||
-            phg::dmalloc((void **)&ws.Ap, n8) || phg::dmalloc((void **)&ws.xp, n8) ||
-            phg::dmalloc((void **)&ws.bp, n8) ||
-            phg::dmalloc((void **)&ws.partials,
+        if (phg::dmalloc((void **)&w->r, n8) || phg::dmalloc((void **)&w->p, n8) ||
+            phg::dmalloc((void **)&w->Ap, n8) || phg::dmalloc((void **)&w->xp, n8) ||
+            phg::dmalloc((void **)&w->bp, n8) ||
+            phg::dmalloc((void **)&w->partials,
                          (size_t)SEG_N * 4 * MAX_GRID * sizeof(double)) ||
-            phg::dmalloc((void **)&ws.scal, 16 * sizeof(double)) ||
-            phg::dmalloc((void **)&ws.st, 3 * sizeof(State)))
+            phg::dmalloc((void **)&w->scal, 16 * sizeof(double)) ||
+            phg::dmalloc((void **)&w->st, 3 * sizeof(State))) {
+            release_ws(w);
             return nullptr;
-        ws.n = A->n;
+        }
+        w->n = A->n;
     }
-    if (need32 && ws.r32 == nullptr) {
+    if (need32 && w->r32 == nullptr) {
         size_t n4 = (size_t)A->n * sizeof(float);
-        if (phg::dmalloc((void **)&ws.r32, n4) || phg::dmalloc((void **)&ws.p32, n4) ||
-            phg::dmalloc((void **)&ws.Ap32, n4) || phg::dmalloc((void **)&ws.x32, n4))
+        if (phg::dmalloc((void **)&w->r32, n4) || phg::dmalloc((void **)&w->p32, n4) ||
+            phg::dmalloc((void **)&w->Ap32, n4) || phg::dmalloc((void **)&w->x32, n4))
             return nullptr;
     }
-    return &ws;
+    return w;
 }
 
 // Optional per-kernel timing with HIP events recorded on the library stream
@@ -1685,21 +1703,34 @@ int mixed_impl(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x,
     return outer;
 }
 
-// FP32 PCG with reliable FP64 residual updates, in the numbering of the
-// format; returns the number of FP32 iterations
+// FP32 PCG with reliable FP64 residual updates, in pieces: the prologue
+// (gather into the solver numbering, r = b - A x), one slot of launches
+// (k_ru_p, k_ru_spmv, k_ru_update, k_ru_resid) and the end (fold x32 into x,
+// scatter back).  Slots write the ping-pong state st[(k + 1) & 1].
 template <typename View>
-int reliable_impl(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x,
-                  int32_t maxit, double rtol, double btol, double atol,
-                  double delta, int32_t *updates, double *residual)
+struct RuPlan {
+    View mv;
+    int g = 0, gs = 0;
+    const double *bb = nullptr;
+    double *xx = nullptr;
+    const int32_t *perm = nullptr;
+    Workspace *ws = nullptr;
+};
+
+template <typename View>
+int ru_prologue(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x,
+                int32_t maxit, double rtol, double btol, double atol,
+                double delta, hipStream_t s, RuPlan<View> *pl)
 {
     using F = FmtData<View>;
-    hipStream_t s = phg::ctx()->stream;
     Workspace *ws = get_ws(A, 1);
     if (ws == nullptr)
         return phg::set_error(PHG_HIP_E_NOMEM, "workspace allocation failed");
     const int64_t n = A->n;
-    int g = grid_for(n), gs = grid_spmv<View>(A, k_ru_spmv<View>);
-    const View mv = view_of(A, (View *)nullptr);
+    pl->ws = ws;
+    pl->g = grid_for(n);
+    pl->gs = grid_spmv<View>(A, k_ru_spmv<View>);
+    pl->mv = view_of(A, (View *)nullptr);
     State h;
     memset(&h, 0, sizeof(h));
     h.rtol = rtol;
@@ -1710,52 +1741,93 @@ int reliable_impl(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x,
     h.mode = RU_START;
     PHG_HIP_CHECK(hipMemcpyAsync(ws->st, &h, sizeof(State),
                                  hipMemcpyHostToDevice, s));
-    const double *bb = b->d;
-    double *xx = x->d;
-    const int32_t *perm = F::perm(A);
-    if (perm) {
-        hipLaunchKernelGGL(k_gather, dim3(g), dim3(BS), 0, s, n, perm,
+    PHG_HIP_CHECK(hipStreamSynchronize(s));    // h lives on this stack
+    pl->bb = b->d;
+    pl->xx = x->d;
+    pl->perm = F::perm(A);
+    const int g = pl->g;
+    if (pl->perm) {
+        hipLaunchKernelGGL(k_gather, dim3(g), dim3(BS), 0, s, n, pl->perm,
                            (const double *)b->d, ws->bp);
-        hipLaunchKernelGGL(k_gather, dim3(g), dim3(BS), 0, s, n, perm,
+        hipLaunchKernelGGL(k_gather, dim3(g), dim3(BS), 0, s, n, pl->perm,
                            (const double *)x->d, ws->xp);
-        bb = ws->bp;
-        xx = ws->xp;
+        pl->bb = ws->bp;
+        pl->xx = ws->xp;
     }
-    double *pres = ws->seg(SEG_RES), *ps = ws->seg(SEG_S);
-    hipLaunchKernelGGL(k_pcg_init_norms, dim3(g), dim3(BS), 0, s, n, bb,
-                       (const double *)xx, ws->seg(SEG_NORMS));
-    hipLaunchKernelGGL((k_ru_init<View>), dim3(gs), dim3(BS), 0, s, n, mv,
-                       F::v64(A), F::d64(A), bb, xx, ws->r32, ws->x32,
-                       (const double *)ws->seg(SEG_NORMS), g, pres, ws->st);
+    hipLaunchKernelGGL(k_pcg_init_norms, dim3(g), dim3(BS), 0, s, n, pl->bb,
+                       (const double *)pl->xx, ws->seg(SEG_NORMS));
+    hipLaunchKernelGGL((k_ru_init<View>), dim3(pl->gs), dim3(BS), 0, s, n,
+                       pl->mv, F::v64(A), F::d64(A), pl->bb, pl->xx, ws->r32,
+                       ws->x32, (const double *)ws->seg(SEG_NORMS), g,
+                       ws->seg(SEG_RES), ws->st);
     PHG_HIP_CHECK(hipGetLastError());
+    return PHG_HIP_OK;
+}
+
+template <typename View>
+void ru_slot(const phg_hip_mat *A, const RuPlan<View> &pl, int64_t k,
+             hipStream_t s, bool prof)
+{
+    using F = FmtData<View>;
+    Workspace *ws = pl.ws;
+    const int64_t n = A->n;
+    const int g = pl.g, gs = pl.gs;
+    double *pres = ws->seg(SEG_RES), *ps = ws->seg(SEG_S);
+    State *sin = ws->st + (k & 1), *sout = ws->st + ((k + 1) & 1);
+    if (prof) prof_mark(KP_P32, s);
+    hipLaunchKernelGGL(k_ru_p, dim3(g), dim3(BS), 0, s, n,
+                       (const float *)ws->r32, F::d32(A), ws->p32, pl.xx,
+                       ws->x32, (const double *)pres, g, gs,
+                       (const State *)sin, sout);
+    if (prof) { prof_mark(KP_P32, s); prof_mark(KP_SPMV32, s); }
+    hipLaunchKernelGGL((k_ru_spmv<View>), dim3(gs), dim3(BS), 0, s, pl.mv,
+                       F::v32(A), (const float *)ws->p32, ws->Ap32, ps,
+                       (const State *)sout);
+    if (prof) { prof_mark(KP_SPMV32, s); prof_mark(KP_UPD32, s); }
+    hipLaunchKernelGGL(k_ru_update, dim3(g), dim3(BS), 0, s, n,
+                       ws->x32, (const float *)ws->p32, ws->r32,
+                       (const float *)ws->Ap32, F::d32(A),
+                       (const double *)ps, gs, pres, sout);
+    if (prof) { prof_mark(KP_UPD32, s); prof_mark(KP_SPMV64, s); }
+    hipLaunchKernelGGL((k_ru_resid<View>), dim3(gs), dim3(BS), 0, s,
+                       pl.mv, F::v64(A), F::d64(A), pl.bb,
+                       (const double *)pl.xx, ws->r32, pres,
+                       (const State *)sout);
+    if (prof) prof_mark(KP_SPMV64, s);
+}
+
+template <typename View>
+void ru_finish(const phg_hip_mat *A, const RuPlan<View> &pl, phg_hip_vec *x,
+               hipStream_t s)
+{
+    const int64_t n = A->n;
+    hipLaunchKernelGGL(k_ru_fold, dim3(pl.g), dim3(BS), 0, s, n, pl.xx,
+                       (const float *)pl.ws->x32);
+    if (pl.perm)
+        hipLaunchKernelGGL(k_scatter, dim3(pl.g), dim3(BS), 0, s, n, pl.perm,
+                           (const double *)pl.ws->xp, x->d);
+}
+
+// FP32 PCG with reliable FP64 residual updates, in the numbering of the
+// format; returns the number of FP32 iterations
+template <typename View>
+int reliable_impl(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x,
+                  int32_t maxit, double rtol, double btol, double atol,
+                  double delta, int32_t *updates, double *residual)
+{
+    hipStream_t s = phg::ctx()->stream;
+    RuPlan<View> pl;
+    int rc;
+    if ((rc = ru_prologue<View>(A, b, x, maxit, rtol, btol, atol, delta, s, &pl)))
+        return rc;
+    Workspace *ws = pl.ws;
     State *hm = (State *)phg::ctx()->host_mailbox;
     const bool prof = g_prof.on;
     int64_t k = 0;
-    int chunk = prof ? 1 : 16, rc;
+    int chunk = prof ? 1 : 16;
     for (;;) {
-        for (int i = 0; i < chunk; i++, k++) {
-            State *sin = ws->st + (k & 1), *sout = ws->st + ((k + 1) & 1);
-            if (prof) prof_mark(KP_P32, s);
-            hipLaunchKernelGGL(k_ru_p, dim3(g), dim3(BS), 0, s, n,
-                               (const float *)ws->r32, F::d32(A), ws->p32, xx,
-                               ws->x32, (const double *)pres, g, gs,
-                               (const State *)sin, sout);
-            if (prof) { prof_mark(KP_P32, s); prof_mark(KP_SPMV32, s); }
-            hipLaunchKernelGGL((k_ru_spmv<View>), dim3(gs), dim3(BS), 0, s, mv,
-                               F::v32(A), (const float *)ws->p32, ws->Ap32, ps,
-                               (const State *)sout);
-            if (prof) { prof_mark(KP_SPMV32, s); prof_mark(KP_UPD32, s); }
-            hipLaunchKernelGGL(k_ru_update, dim3(g), dim3(BS), 0, s, n,
-                               ws->x32, (const float *)ws->p32, ws->r32,
-                               (const float *)ws->Ap32, F::d32(A),
-                               (const double *)ps, gs, pres, sout);
-            if (prof) { prof_mark(KP_UPD32, s); prof_mark(KP_SPMV64, s); }
-            hipLaunchKernelGGL((k_ru_resid<View>), dim3(gs), dim3(BS), 0, s,
-                               mv, F::v64(A), F::d64(A), bb,
-                               (const double *)xx, ws->r32, pres,
-                               (const State *)sout);
-            if (prof) prof_mark(KP_SPMV64, s);
-        }
+        for (int i = 0; i < chunk; i++, k++)
+            ru_slot<View>(A, pl, k, s, prof);
         PHG_HIP_CHECK(hipGetLastError());
         if ((rc = read_state(ws->st + (k & 1), hm, s)))
             return rc;
@@ -1780,11 +1852,7 @@ int reliable_impl(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x,
     if (prof)
         prof_collect();
     State fin = *hm;
-    hipLaunchKernelGGL(k_ru_fold, dim3(g), dim3(BS), 0, s, n, xx,
-                       (const float *)ws->x32);
-    if (perm)
-        hipLaunchKernelGGL(k_scatter, dim3(g), dim3(BS), 0, s, n, perm,
-                           (const double *)ws->xp, x->d);
+    ru_finish<View>(A, pl, x, s);
     PHG_HIP_CHECK(hipGetLastError());
     PHG_HIP_CHECK(hipStreamSynchronize(s));
     if (updates)
@@ -1795,6 +1863,30 @@ int reliable_impl(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x,
         return phg::set_error(PHG_HIP_E_ZERO_DIAG,
                               "zero diagonal in Jacobi preconditioner");
     return fin.nits;
+}
+
+// One matrix of a batch: its type-erased slot launcher and state.
+struct MultiItem {
+    std::function<void(int64_t, hipStream_t)> slot;
+    std::function<void(hipStream_t)> finish;
+    State *st = nullptr;
+};
+
+template <typename View>
+int multi_prepare(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x,
+                  int32_t maxit, double rtol, double btol, double atol,
+                  double delta, hipStream_t s, MultiItem *it)
+{
+    auto pl = std::make_shared<RuPlan<View>>();
+    int rc;
+    if ((rc = ru_prologue<View>(A, b, x, maxit, rtol, btol, atol, delta, s, pl.get())))
+        return rc;
+    it->slot = [A, pl](int64_t k, hipStream_t st) {
+        ru_slot<View>(A, *pl, k, st, false);
+    };
+    it->finish = [A, pl, x](hipStream_t st) { ru_finish<View>(A, *pl, x, st); };
+    it->st = pl->ws->st;
+    return PHG_HIP_OK;
 }
 
 // time `reps` launches of the PCG SpMV kernel of one format and precision
@@ -2006,6 +2098,16 @@ int ensure_fp32_copy(phg_hip_mat *A)
     return PHG_HIP_OK;
 }
 
+void free_workspace(phg_hip_mat *A)
+{
+    Workspace *w = (Workspace *)A->ws;
+    if (w) {
+        release_ws(w);
+        delete w;
+        A->ws = nullptr;
+    }
+}
+
 int spmv64(const phg_hip_mat *A, const double *x, double *y, double alpha,
            double beta, hipStream_t s)
 {
@@ -2168,6 +2270,143 @@ extern "C" int phg_hip_pcg_reliable(const phg_hip_mat *A_,
         return reliable_impl<View>(A, b, x, maxit, rtol, btol, atol, delta,
                                    updates, residual);
     });
+}
+
+extern "C" int phg_hip_pcg_multi(int32_t nmat, phg_hip_mat *const *Av,
+                                 const phg_hip_vec *const *b,
+                                 phg_hip_vec *const *x, int32_t maxit,
+                                 double rtol, double btol, double atol,
+                                 double delta, int32_t use_graph,
+                                 int32_t *nits, double *residual)
+{
+    int rc;
+    if ((rc = phg::require_ctx()))
+        return rc;
+    if (nmat < 1 || nmat > MULTI_MAX || Av == nullptr || b == nullptr ||
+        x == nullptr)
+        return phg::set_error(PHG_HIP_E_ARG, "phg_hip_pcg_multi: 1 <= nmat <= %d",
+                              MULTI_MAX);
+    if (!(delta > 0.0 && delta < 1.0))
+        return phg::set_error(PHG_HIP_E_ARG, "delta must lie in (0, 1)");
+    hipStream_t s = phg::ctx()->stream;
+    std::vector<MultiItem> items((size_t)nmat);
+    for (int i = 0; i < nmat; i++) {
+        if ((rc = check_solve_args(Av[i], b[i], x[i])))
+            return rc;
+        phg_hip_mat *A = Av[i];
+        for (int j = 0; j < i; j++)
+            if (Av[j] == A)
+                return phg::set_error(PHG_HIP_E_ARG,
+                                      "phg_hip_pcg_multi: a matrix appears twice");
+        if ((rc = ensure_format(A, A->fmt32, 1)))
+            return rc;
+        rc = with_format(A->fmt32, [&](auto tag) {
+            using View = decltype(tag);
+            return multi_prepare<View>(A, b[i], x[i], maxit, rtol, btol, atol,
+                                       delta, s, &items[(size_t)i]);
+        });
+        if (rc)
+            return rc;
+    }
+    PHG_HIP_CHECK(hipStreamSynchronize(s));
+    // one stream per matrix (up to the pool size), forked from and joined to
+    // the library stream
+    hipStream_t *pool = nullptr;
+    int npool = 0;
+    if ((rc = phg::aux_streams(&pool, &npool)))
+        return rc;
+    const int CH = 16;          // slots per replay (even: the state parity holds)
+    State *hm = (State *)phg::ctx()->host_mailbox;
+    static_assert(sizeof(State) * MULTI_MAX <= 4096, "mailbox size");
+    hipEvent_t fork, join[MULTI_MAX];
+    PHG_HIP_CHECK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    for (int i = 0; i < nmat; i++)
+        PHG_HIP_CHECK(hipEventCreateWithFlags(&join[i], hipEventDisableTiming));
+    auto enqueue = [&](hipStream_t main) -> hipError_t {
+        hipError_t e;
+        if ((e = hipEventRecord(fork, main)) != hipSuccess)
+            return e;
+        for (int i = 0; i < nmat; i++) {
+            hipStream_t si = pool[i % npool];
+            if ((e = hipStreamWaitEvent(si, fork, 0)) != hipSuccess)
+                return e;
+            for (int k = 0; k < CH; k++)
+                items[(size_t)i].slot(k, si);
+            if ((e = hipMemcpyAsync(hm + i, items[(size_t)i].st + (CH & 1),
+                                    sizeof(State), hipMemcpyDeviceToHost, si)) !=
+                hipSuccess)
+                return e;
+            if ((e = hipEventRecord(join[i], si)) != hipSuccess ||
+                (e = hipStreamWaitEvent(main, join[i], 0)) != hipSuccess)
+                return e;
+        }
+        return hipGetLastError();
+    };
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    auto cleanup = [&]() {
+        if (exec)
+            (void)hipGraphExecDestroy(exec);
+        if (graph)
+            (void)hipGraphDestroy(graph);
+        (void)hipEventDestroy(fork);
+        for (int i = 0; i < nmat; i++)
+            (void)hipEventDestroy(join[i]);
+    };
+    hipError_t e = hipSuccess;
+    if (use_graph) {
+        if ((e = hipStreamBeginCapture(s, hipStreamCaptureModeGlobal)) == hipSuccess) {
+            hipError_t e1 = enqueue(s);
+            e = hipStreamEndCapture(s, &graph);
+            if (e1 != hipSuccess)
+                e = e1;
+        }
+        if (e == hipSuccess)
+            e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            cleanup();
+            return phg::hip_fail(e, "graph capture of the batched solve", __FILE__,
+                                 __LINE__);
+        }
+    }
+    int64_t replays = 0;
+    for (;;) {
+        e = use_graph ? hipGraphLaunch(exec, s) : enqueue(s);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(s);
+        if (e != hipSuccess) {
+            cleanup();
+            return phg::hip_fail(e, "batched solve", __FILE__, __LINE__);
+        }
+        replays++;
+        bool all = true;
+        for (int i = 0; i < nmat; i++)
+            all = all && hm[i].done != RUNNING;
+        if (all)
+            break;
+        if (replays * CH > 2 * (int64_t)maxit + 4096) {
+            cleanup();
+            return phg::set_error(PHG_HIP_E_HIP, "batched solve did not finish");
+        }
+    }
+    std::vector<State> fin(hm, hm + nmat);
+    cleanup();
+    for (int i = 0; i < nmat; i++)
+        items[(size_t)i].finish(s);
+    PHG_HIP_CHECK(hipGetLastError());
+    PHG_HIP_CHECK(hipStreamSynchronize(s));
+    int32_t mx = 0;
+    for (int i = 0; i < nmat; i++) {
+        if (fin[(size_t)i].done == ZERO_DIAG)
+            return phg::set_error(PHG_HIP_E_ZERO_DIAG,
+                                  "zero diagonal in Jacobi preconditioner");
+        if (nits)
+            nits[i] = fin[(size_t)i].nits;
+        if (residual)
+            residual[i] = fin[(size_t)i].res;
+        mx = std::max(mx, fin[(size_t)i].nits);
+    }
+    return mx;
 }
 
 extern "C" int phg_hip_mat_set_format(phg_hip_mat *A, int32_t fmt64,

@@ -27,9 +27,14 @@ struct Context {
     int num_cus = 0;
     // pinned host mailbox for solver state read-back
     void *host_mailbox = nullptr;
+    // extra streams (default priority) for concurrent solves, made on demand
+    static constexpr int NAUX = 8;
+    hipStream_t aux[NAUX] = {};
+    int naux = 0;
 };
 Context *ctx();                // nullptr if not initialised
 int require_ctx();             // PHG_HIP_OK or PHG_HIP_E_STATE
+int aux_streams(hipStream_t **pool, int *n);   // the extra stream pool
 
 // device memory helpers (stream-ordered frees are not used: every buffer is
 // owned by a handle and freed on destroy)
@@ -127,6 +132,8 @@ struct phg_hip_mat {
     int32_t *b_ptr = nullptr, *b_col = nullptr;
     double *b_val64 = nullptr;
     float *b_val32 = nullptr;
+    // solver workspace of this matrix (krylov.hip)
+    void *ws = nullptr;
     // SpMM wavefront shape index per precision (block.hip) and tuner times
     int spmm_shape[2] = {0, 0};
     double spmm_us[8][2] = {};
@@ -141,6 +148,7 @@ constexpr int DICT_BLOCK_NNZ = 4096, DICT_BLOCK_ROWS = 256;
 // Krylov building blocks (krylov.hip) used by the C ABI layer.
 int spmv64(const phg_hip_mat *A, const double *x, double *y, double alpha,
            double beta, hipStream_t s);
+void free_workspace(phg_hip_mat *A);
 int build_row_blocks(phg_hip_mat *A);
 // solver copy (reorder.hip)
 int build_solver_copy(phg_hip_mat *A);

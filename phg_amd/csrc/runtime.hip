@@ -106,12 +106,33 @@ extern "C" int phg_hip_init(int32_t device)
     return PHG_HIP_OK;
 }
 
+namespace phg {
+int aux_streams(hipStream_t **pool, int *n)
+{
+    int rc;
+    if ((rc = require_ctx()))
+        return rc;
+    while (g_ctx.naux < Context::NAUX) {
+        hipStream_t st;
+        PHG_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        g_ctx.aux[g_ctx.naux++] = st;
+    }
+    *pool = g_ctx.aux;
+    *n = g_ctx.naux;
+    return PHG_HIP_OK;
+}
+}  // namespace phg
+
 extern "C" int phg_hip_finalize(void)
 {
     using namespace phg;
     if (!g_ready)
         return PHG_HIP_OK;
     (void)hipStreamSynchronize(g_ctx.stream);
+    for (int i = 0; i < g_ctx.naux; i++) {
+        (void)hipStreamSynchronize(g_ctx.aux[i]);
+        (void)hipStreamDestroy(g_ctx.aux[i]);
+    }
     (void)hipHostFree(g_ctx.host_mailbox);
     (void)hipStreamDestroy(g_ctx.stream);
     g_ctx = Context();
