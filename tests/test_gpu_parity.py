@@ -322,3 +322,36 @@ def test_dof_norms(hip, p):
                             np.ones(c["grid"].nelem), v, v).sum()
     assert abs(l2 * l2 - 2 * half_vMv) <= 1e-12 * 2 * half_vMv
     assert abs(h1 * h1 - vKv) <= 1e-12 * vKv
+
+
+@pytest.mark.parametrize("p", [2, 3])
+def test_cube4_fixture_assembly_and_solve(hip, p):
+    """The reference's test/cube4.dat mesh (a refined, unstructured
+    tetrahedralisation): CSR bit for bit against the oracle before and after
+    elimination, and the PCG solution."""
+    import os
+    pa = hip
+    g = pa.Grid.read_albert(os.path.join(os.path.dirname(__file__), "golden", "cube4.dat"))
+    nd = g.ndof(p)
+    bd = g.dof_bdry(p)
+    xyz = g.dof_coords(p)
+    kappa = np.exp(np.random.default_rng(12).normal(0.0, 0.5, size=g.nelem))
+    f = 1.0 + xyz[:, 0]
+    gv = xyz[:, 1] * xyz[:, 2]
+    sp = pa.Space(g, p, bdry=bd)
+    pb = pa.Problem.poisson(use_kappa=True)
+    A = pa.assemble(sp, pb, kappa=kappa, f=f, g=gv)
+    Ao = orc.assemble(orc.problem(p, use_kappa=True), g.verts(), g.elem_verts(),
+                      g.dof_map(p), nd, bd, gv, f, kappa)
+    rp, ci, va, rhs = A.download()
+    assert np.array_equal(ci, Ao.cols) and _bits_equal(va, Ao.vals)
+    assert _bits_equal(rhs, Ao.rhs)
+    A.eliminate()
+    orc.eliminate(Ao, bd)
+    rp, ci, va, rhs = A.download()
+    assert np.array_equal(ci, Ao.cols) and _bits_equal(va, Ao.vals)
+    assert _bits_equal(rhs, Ao.rhs)
+    xo, _, _ = orc.pcg(Ao, Ao.rhs, rtol=1e-13, btol=1e-13)
+    x = pa.Vec(A.n)
+    pa.pcg(A, A.rhs(), x, rtol=1e-12, btol=1e-12)
+    assert np.linalg.norm(x.numpy() - xo) / np.linalg.norm(xo) < 1e-9
