@@ -43,6 +43,22 @@ def spmv_bytes(n, nnz, s_val, s_vec):
     return nnz * (s_val + 4) + (n + 1) * 4 + n * s_vec + n * s_vec
 
 
+def measured_traffic(kernel_key, nnz):
+    """HBM bytes per launch of a kernel from the committed PMC summary
+    (profiles/hbm_traffic.json: 2 x FETCH_SIZE + WRITE_SIZE, scaled by a copy
+    kernel of known size), when it was measured on this workload."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = t.get("kernels", {}).get(kernel_key)
+    if not k or k.get("nnz") != nnz:
+        return None
+    return k["bytes"]
+
+
 def build_problem(n, p, sigma, seed):
     import phg_amd as pa
     t0 = time.perf_counter()
@@ -200,9 +216,11 @@ def main():
             t32 = ms32 / l32 * 1e-3
             b32 = A.format_bytes(f32, True)
             ach = b32 / t32 / 1e9
+            tkey = (f"k_ru_spmv<{tune['chosen'][1]}>" if args.mixed == "reliable"
+                    else f"k_pcg_spmv<float,{tune['chosen'][1]}>")
             roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                    "traffic": None,
+                    "traffic": measured_traffic(tkey, nnz),
                     "kernel": (f"k_ru_spmv<{tune['chosen'][1]}> (FP32 SpMV + (p,Ap))"
                                if args.mixed == "reliable" else
                                f"k_pcg_spmv<float,{tune['chosen'][1]}> (FP32 SpMV + (p,Ap))"),
