@@ -101,6 +101,19 @@ int phg_hip_assemble(phg_hip_space *sp, const phg_hip_problem *pb,
  * solver.c:734 -> :508 update_rhs). */
 int phg_hip_mat_eliminate(phg_hip_mat *A);
 
+/* A system built elsewhere (e.g. PHG's own packed matrix, matvec.c:279
+ * phgMatPack: packed_ind / packed_cols / packed_data) uploaded as CSR:
+ * rowptr[n+1] (0-based), cols[nnz], vals[nnz], rhs[n], bdry[n] (1 for
+ * Dirichlet rows).  eliminated = 0: the matrix before phgMatAssemble's
+ * boundary treatment (call phg_hip_mat_eliminate next); eliminated = 1: the
+ * matrix after it (boundary columns already moved, Dirichlet rows e_i),
+ * ready for the solvers (diagonal as matvec.c:500 phgMatSetupDiagonal).
+ * No node coordinates are given, so the solvers keep the given numbering. */
+int phg_hip_mat_from_csr(int64_t n, const int64_t *rowptr, const int32_t *cols,
+                         const double *vals, const double *rhs,
+                         const uint8_t *bdry, int32_t eliminated,
+                         phg_hip_mat **A);
+
 int phg_hip_mat_info(const phg_hip_mat *A, int64_t *nrows, int64_t *nnz);
 /* matvec.c:6134 phgMatDumpCSR equivalent: copies the CSR (0-based, row
  * pointers widened to int64) and the right-hand side to the host. */

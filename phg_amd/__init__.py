@@ -23,7 +23,7 @@ _LIB = os.path.join(_HERE, "lib")
 
 __all__ = [
     "Grid", "Space", "Mat", "Vec", "Problem", "init", "finalize",
-    "assemble", "element_matrices", "pcg", "pcg_mixed", "pcg_reliable", "pcg_block", "pcg_multi", "dof_norms", "spmm_tune", "adjoint_logkappa", "PhgHipError",
+    "assemble", "element_matrices", "pcg", "pcg_mixed", "pcg_reliable", "pcg_block", "pcg_multi", "mat_from_csr", "dof_norms", "spmm_tune", "adjoint_logkappa", "PhgHipError",
     "hip_lib", "grid_lib", "HIP_LIB_PATH", "GRID_LIB_PATH",
 ]
 
@@ -104,6 +104,8 @@ _sig(hip_lib, "phg_hip_element_matrices", C.c_int,
 _sig(hip_lib, "phg_hip_assemble", C.c_int,
      [_vp, _P(Problem), _P(_dbl), _P(_dbl), _P(_dbl), _P(_vp)])
 _sig(hip_lib, "phg_hip_mat_eliminate", C.c_int, [_vp])
+_sig(hip_lib, "phg_hip_mat_from_csr", C.c_int,
+     [_i64, _P(_i64), _P(_i32), _P(_dbl), _P(_dbl), _P(C.c_uint8), _i32, _P(_vp)])
 _sig(hip_lib, "phg_hip_dof_norms", C.c_int, [_vp, _vp, _P(_dbl), _P(_dbl)])
 _sig(hip_lib, "phg_hip_adjoint_logkappa", C.c_int,
      [_vp, _P(Problem), _P(_dbl), _vp, _vp, _vp, _i32, _dbl, _P(_dbl), _P(_dbl), _vp,
@@ -527,6 +529,20 @@ def spmm_bytes(A: "Mat", fp32: bool) -> int:
     b = _i64()
     _check(hip_lib.phg_hip_spmm_bytes(A._h, int(fp32), C.byref(b)), "spmm_bytes")
     return b.value
+
+
+def mat_from_csr(rowptr, cols, vals, rhs, bdry, eliminated: bool) -> "Mat":
+    """Uploads a host CSR system (e.g. PHG's packed matrix)."""
+    rp = np.ascontiguousarray(rowptr, dtype=np.int64)
+    ci = np.ascontiguousarray(cols, dtype=np.int32)
+    va = np.ascontiguousarray(vals, dtype=np.float64)
+    rh = np.ascontiguousarray(rhs, dtype=np.float64)
+    bd = np.ascontiguousarray(bdry, dtype=np.uint8)
+    h = _vp()
+    _check(hip_lib.phg_hip_mat_from_csr(len(rp) - 1, _ptr(rp, _i64), _ptr(ci, _i32),
+                                        _ptr(va, _dbl), _ptr(rh, _dbl), _ptr(bd, C.c_uint8),
+                                        int(eliminated), C.byref(h)), "mat_from_csr")
+    return Mat(h)
 
 
 def stream_copy(n: int, reps: int = 20) -> float:

@@ -326,6 +326,23 @@ __device__ inline void xgrad(int k, const double *G, const double *J,
     }
 }
 
+// diagonal of each row (matvec.c:500 phgMatSetupDiagonal: linear search;
+// 0 when the row has no diagonal entry)
+__global__ void k_diag(int64_t n, const int32_t *rowptr, const int32_t *cols,
+                       const double *vals, double *diag)
+{
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n)
+        return;
+    double d = 0.0;
+    for (int32_t j = rowptr[r]; j < rowptr[r + 1]; j++)
+        if (cols[j] == r) {
+            d = vals[j];
+            break;
+        }
+    diag[r] = d;
+}
+
 //--------------------------------------------------------- element kernel
 struct ElemArgs {
     int64_t nelem;
@@ -1771,4 +1788,68 @@ extern "C" int phg_hip_dof_norms(phg_hip_space *sp, const phg_hip_vec *v,
     if (h1)
         *h1 = sqrt(c);
     return PHG_HIP_OK;
+}
+
+//------------------------------------------------------- CSR from the host
+extern "C" int phg_hip_mat_from_csr(int64_t n, const int64_t *rowptr,
+                                    const int32_t *cols, const double *vals,
+                                    const double *rhs, const uint8_t *bdry,
+                                    int32_t eliminated, phg_hip_mat **Aout)
+{
+    int rc;
+    if ((rc = phg::require_ctx()))
+        return rc;
+    if (n <= 0 || rowptr == nullptr || cols == nullptr || vals == nullptr ||
+        rhs == nullptr || bdry == nullptr || Aout == nullptr)
+        return phg::set_error(PHG_HIP_E_ARG, "phg_hip_mat_from_csr: bad args");
+    const int64_t nnz = rowptr[n];
+    if (rowptr[0] != 0 || nnz < 0 || nnz >= (int64_t)INT32_MAX)
+        return phg::set_error(PHG_HIP_E_ARG, "phg_hip_mat_from_csr: bad row pointers");
+    std::vector<int32_t> rp32((size_t)n + 1);
+    for (int64_t i = 0; i <= n; i++) {
+        if (i > 0 && rowptr[i] < rowptr[i - 1])
+            return phg::set_error(PHG_HIP_E_ARG, "row pointers decrease at %lld",
+                                  (long long)i);
+        rp32[(size_t)i] = (int32_t)rowptr[i];
+    }
+    for (int64_t j = 0; j < nnz; j++)
+        if (cols[j] < 0 || cols[j] >= n)
+            return phg::set_error(PHG_HIP_E_ARG, "column %d out of range at %lld",
+                                  cols[j], (long long)j);
+    hipStream_t s = phg::ctx()->stream;
+    phg_hip_mat *A = new phg_hip_mat();
+    A->n = n;
+    A->nnz = nnz;
+    A->dim = 1;
+    A->ndof = n;
+    if ((rc = phg::dmalloc((void **)&A->d_rowptr, (size_t)(n + 1) * 4)) ||
+        (rc = phg::dmalloc((void **)&A->d_cols, (size_t)(nnz + 1) * 4)) ||
+        (rc = phg::dmalloc((void **)&A->d_vals, (size_t)(nnz + 1) * 8)) ||
+        (rc = phg::dmalloc((void **)&A->d_rhs, (size_t)n * 8)) ||
+        (rc = phg::dmalloc((void **)&A->d_bdry_row, (size_t)n)) ||
+        (rc = phg::upload(A->d_rowptr, rp32.data(), (size_t)(n + 1) * 4)) ||
+        (rc = phg::upload(A->d_cols, cols, (size_t)nnz * 4)) ||
+        (rc = phg::upload(A->d_vals, vals, (size_t)nnz * 8)) ||
+        (rc = phg::upload(A->d_rhs, rhs, (size_t)n * 8)) ||
+        (rc = phg::upload(A->d_bdry_row, bdry, (size_t)n)))
+        goto fail;
+    if (eliminated) {
+        if ((rc = phg::dmalloc((void **)&A->d_diag, (size_t)n * 8)))
+            goto fail;
+        hipLaunchKernelGGL(k_diag, dim3(nblk(n, 256)), dim3(256), 0, s, n,
+                           (const int32_t *)A->d_rowptr, (const int32_t *)A->d_cols,
+                           (const double *)A->d_vals, A->d_diag);
+        if (hipGetLastError() != hipSuccess) {
+            rc = phg::set_error(PHG_HIP_E_HIP, "diagonal kernel");
+            goto fail;
+        }
+        A->eliminated = 1;
+        if ((rc = phg::build_row_blocks(A)) || (rc = phg::build_solver_copy(A)))
+            goto fail;
+    }
+    *Aout = A;
+    return PHG_HIP_OK;
+fail:
+    phg_hip_mat_destroy(A);
+    return rc;
 }

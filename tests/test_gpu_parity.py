@@ -355,3 +355,28 @@ def test_cube4_fixture_assembly_and_solve(hip, p):
     x = pa.Vec(A.n)
     pa.pcg(A, A.rhs(), x, rtol=1e-12, btol=1e-12)
     assert np.linalg.norm(x.numpy() - xo) / np.linalg.norm(xo) < 1e-9
+
+
+@pytest.mark.parametrize("eliminated", [False, True])
+def test_mat_from_csr_phg_packed_matrix(hip, eliminated):
+    """A system handed over as a host CSR (the form of PHG's packed matrix):
+    before boundary treatment, eliminated on the GPU bit for bit as the
+    oracle does; or already eliminated; the PCG solutions agree."""
+    pa = hip
+    c = poisson_case(5, 2, "sin")
+    Ao = orc.assemble(orc.problem(2), c["verts"], c["elem_verts"], c["map"], c["ndof"],
+                      c["bdry"], c["g"], c["f"])
+    if eliminated:
+        orc.eliminate(Ao, c["bdry"])
+    A = pa.mat_from_csr(Ao.rowptr, Ao.cols, Ao.vals, Ao.rhs, c["bdry"], eliminated)
+    if not eliminated:
+        A.eliminate()
+        orc.eliminate(Ao, c["bdry"])
+        rp, ci, va, rhs = A.download()
+        assert np.array_equal(ci, Ao.cols) and _bits_equal(va, Ao.vals)
+        assert _bits_equal(rhs, Ao.rhs)
+    xo, nits_o, _ = orc.pcg(Ao, Ao.rhs, rtol=1e-12, btol=1e-12)
+    x = pa.Vec(A.n)
+    nits, _ = pa.pcg(A, A.rhs(), x, rtol=1e-12, btol=1e-12)
+    assert abs(nits - nits_o) <= 2
+    assert np.linalg.norm(x.numpy() - xo) / np.linalg.norm(xo) < 1e-10
