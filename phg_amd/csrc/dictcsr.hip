@@ -12,7 +12,10 @@
 // 16-byte loads; the padding entries have value 0 and index 0 and are
 // counted in the last row of the block (they add +0 to its sum).
 //
-// Entries keep the order of the solver CSR inside every row.  The builder
+// Inside a block, groups of up to 512 entries (one wavefront's share) are
+// stored transposed (8 entries per work-item, see the builder); row
+// pointers index the untransposed order, which is the solver CSR's
+// storage order within every row.  The builder
 // runs on the host over the solver CSR once per matrix; a matrix with a row
 // longer than a block or needing more than 256 chunks in one row is marked
 // as not representable (the autotuner then skips the layout).
@@ -58,8 +61,8 @@ int build_dict(phg_hip_mat *A)
     std::vector<int32_t> mark((size_t)nchunk, -1), slot((size_t)nchunk, 0);
     std::vector<int32_t> brow, bent, bdict, dict, crp((size_t)n + 1);
     std::vector<int32_t> fresh;
-    std::vector<uint16_t> idx;
-    std::vector<double> val;
+    std::vector<uint16_t> idx, tidx;
+    std::vector<double> val, tval;
     idx.reserve((size_t)nnz + (size_t)nnz / 64 + 64);
     val.reserve(idx.capacity());
     int64_t r = 0;
@@ -117,6 +120,23 @@ int build_dict(phg_hip_mat *A)
         while (idx.size() % 8) {
             idx.push_back(0);
             val.push_back(0.0);
+        }
+        // store each group of up to 512 entries transposed: memory position
+        // 8 l + j holds entry j L + l of the group (L = group size / 8), so
+        // the work-item of lane l loads its 8 entries with 16-byte accesses
+        // while the j-th gather of a wavefront reads 64 consecutive entries
+        {
+            const int64_t m = (int64_t)idx.size() - e0;
+            for (int64_t g0 = 0; g0 < m; g0 += DICT_GROUP) {
+                const int64_t gm = std::min<int64_t>(DICT_GROUP, m - g0), L = gm / 8;
+                tidx.assign(idx.begin() + e0 + g0, idx.begin() + e0 + g0 + gm);
+                tval.assign(val.begin() + e0 + g0, val.begin() + e0 + g0 + gm);
+                for (int64_t l = 0; l < L; l++)
+                    for (int j = 0; j < 8; j++) {
+                        idx[(size_t)(e0 + g0 + 8 * l + j)] = tidx[(size_t)(j * L + l)];
+                        val[(size_t)(e0 + g0 + 8 * l + j)] = tval[(size_t)(j * L + l)];
+                    }
+            }
         }
         stamp++;
         if (idx.size() > (size_t)INT32_MAX - DICT_BLOCK_NNZ) {

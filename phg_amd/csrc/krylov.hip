@@ -455,9 +455,11 @@ __device__ inline void spmv_apply(const SellView &m, const T *__restrict__ va,
 // Dictionary-compressed CSR stream (layout built in dictcsr.hip).  A
 // work-group takes a contiguous range of blocks.  Per block each work-item
 // loads 8 consecutive 16-bit indices and 8 values with 16-byte loads (two
-// groups of 8 per work-item for a 4096-entry block), decodes the columns
-// through the block's dictionary in LDS, gathers x, stores the products in
-// LDS, and one work-item per row sums its products in storage order.  The
+// groups of 8 per work-item for a 4096-entry block); the layout stores each
+// wavefront's 512 entries transposed, so the j-th gather of a wavefront
+// reads 64 consecutive entries of the block (few cache lines).  Columns are
+// decoded through the block's dictionary in LDS, the products go to LDS in
+// storage order, and one work-item per row sums its products in order.  The
 // loads of the next block are issued before the products of the current
 // one are formed, so the matrix stream stays in flight across the LDS
 // barriers and the row sums.
@@ -601,11 +603,14 @@ __device__ inline void dict_spmv(const DictView &m, const T *__restrict__ va,
             for (int k = 0; k < NCH; k++) {
                 const int32_t e = cur.f0 + (tid + BS * k) * 8;
                 if (e < cur.f1) {
-                    T pr[8];
+                    // group q of this wavefront's 512 entries, L entries per
+                    // gather: entry j of lane l is j * L + l of the group
+                    const int32_t q0 = ((e - cur.f0) / phg::DICT_GROUP) * phg::DICT_GROUP;
+                    const int32_t L = min(phg::DICT_GROUP, cur.f1 - cur.f0 - q0) / 8;
+                    const int l = (tid & 63);
 #pragma unroll
                     for (int j = 0; j < 8; j++)
-                        pr[j] = cur.vb[k].v[j] * xg[k][j];
-                    store8(prod + (e - cur.f0), pr);
+                        prod[q0 + j * L + l] = cur.vb[k].v[j] * xg[k][j];
                 }
             }
             lds_barrier();
@@ -693,6 +698,47 @@ __global__ void __launch_bounds__(BS) k_spmv64(const int32_t *rp,
                             double yo = beta == 0.0 ? 0.0 : beta * y[row];
                             y[row] = yo + alpha * s;
                         });
+}
+
+// 4 consecutive elements with 16-byte accesses (float4, or two double2)
+template <typename T> struct Q4 { T v[4]; };
+template <typename T>
+__device__ inline Q4<T> ld4(const T *p, int64_t q)
+{
+    Q4<T> r;
+    if (sizeof(T) == 4) {
+        const float4 a = ((const float4 *)p)[q];
+        r.v[0] = a.x; r.v[1] = a.y; r.v[2] = a.z; r.v[3] = a.w;
+    }
+    else {
+        const double2 a = ((const double2 *)p)[2 * q], b = ((const double2 *)p)[2 * q + 1];
+        r.v[0] = a.x; r.v[1] = a.y; r.v[2] = b.x; r.v[3] = b.y;
+    }
+    return r;
+}
+template <typename T>
+__device__ inline void st4(T *p, int64_t q, const Q4<T> &r)
+{
+    if (sizeof(T) == 4)
+        ((float4 *)p)[q] = make_float4(r.v[0], r.v[1], r.v[2], r.v[3]);
+    else {
+        ((double2 *)p)[2 * q] = make_double2(r.v[0], r.v[1]);
+        ((double2 *)p)[2 * q + 1] = make_double2(r.v[2], r.v[3]);
+    }
+}
+
+// calls f(i) for every element index of [0, n) this work-item owns, the
+// groups of 4 through fq(q) (elements 4q .. 4q+3), the n % 4 tail elements
+// through f(i) on the first work-group
+template <typename FQ, typename F>
+__device__ inline void for_each4(int64_t n, FQ fq, F f)
+{
+    const int64_t n4 = n >> 2;
+    for (int64_t q = (int64_t)blockIdx.x * BS + threadIdx.x; q < n4;
+         q += (int64_t)gridDim.x * BS)
+        fq(q);
+    if (blockIdx.x == 0 && (int64_t)threadIdx.x < n - 4 * n4)
+        f(4 * n4 + threadIdx.x);
 }
 
 //------------------------------------------------------------------ BLAS-1
@@ -857,11 +903,21 @@ __global__ void __launch_bounds__(BS) k_pcg_p(int64_t n, const T *r,
         return;
     const bool first = s.nits == 0;
     const T tb = (T)beta;
-    for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * BS) {
+    for_each4(n, [&](int64_t q) {
+        const Q4<T> rv = ld4(r, q), dv = ld4(diag, q);
+        Q4<T> pv;
+        if (!first)
+            pv = ld4((const T *)p, q);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const T z = rv.v[k] / dv.v[k];
+            pv.v[k] = first ? z : tb * pv.v[k] + z;
+        }
+        st4(p, q, pv);
+    }, [&](int64_t i) {
         T z = r[i] / diag[i];
         p[i] = first ? z : tb * p[i] + z;
-    }
+    });
 }
 
 template <typename T, typename View>
@@ -897,15 +953,28 @@ __global__ void __launch_bounds__(BS) k_pcg_update(int64_t n, T *x,
     }
     const T alpha = (T)(st->rz / pap[0]);
     double v[3] = {0.0, 0.0, 0.0};
-    for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * BS) {
+    for_each4(n, [&](int64_t q) {
+        Q4<T> xv = ld4((const T *)x, q), pv = ld4(p, q), rv = ld4((const T *)r, q),
+              av = ld4(Ap, q), dv = ld4(diag, q);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            xv.v[k] += alpha * pv.v[k];
+            const T ri = rv.v[k] - alpha * av.v[k];
+            rv.v[k] = ri;
+            const T z = ri / dv.v[k];
+            v[0] += (double)ri * (double)ri;
+            v[1] += (double)ri * (double)z;
+        }
+        st4(x, q, xv);
+        st4(r, q, rv);
+    }, [&](int64_t i) {
         x[i] += alpha * p[i];
         T ri = r[i] - alpha * Ap[i];
         r[i] = ri;
         T z = ri / diag[i];
         v[0] += (double)ri * (double)ri;
         v[1] += (double)ri * (double)z;
-    }
+    });
     write_partials<3>(v, part_out);
 }
 
@@ -1109,19 +1178,36 @@ __global__ void __launch_bounds__(BS) k_ru_p(int64_t n, const float *r32,
     if (o.done)
         return;
     if (kind == RU_UPDATE) {
-        for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
-             i += (int64_t)gridDim.x * BS) {
+        for_each4(n, [&](int64_t q) {
+            Q4<double> xv = ld4((const double *)x, q);
+            Q4<float> dv = ld4((const float *)x32, q);
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                xv.v[k] += (double)dv.v[k];
+            st4(x, q, xv);
+            st4(x32, q, Q4<float>{{0.0f, 0.0f, 0.0f, 0.0f}});
+        }, [&](int64_t i) {
             x[i] += (double)x32[i];
             x32[i] = 0.0f;
-        }
+        });
         return;
     }
     const float tb = (float)beta;
-    for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * BS) {
+    for_each4(n, [&](int64_t q) {
+        const Q4<float> rv = ld4(r32, q), dv = ld4(diag32, q);
+        Q4<float> pv;
+        if (!first)
+            pv = ld4((const float *)p32, q);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const float z = rv.v[k] / dv.v[k];
+            pv.v[k] = first ? z : tb * pv.v[k] + z;
+        }
+        st4(p32, q, pv);
+    }, [&](int64_t i) {
         float z = r32[i] / diag32[i];
         p32[i] = first ? z : tb * p32[i] + z;
-    }
+    });
 }
 
 template <typename View>
@@ -1159,15 +1245,29 @@ __global__ void __launch_bounds__(BS) k_ru_update(int64_t n, float *x32,
     }
     const float alpha = (float)(st->rz / pap[0]);
     double v[3] = {0.0, 0.0, 0.0};
-    for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * BS) {
+    auto one = [&](int64_t i) {
         x32[i] += alpha * p[i];
         float ri = r[i] - alpha * Ap[i];
         r[i] = ri;
         float z = ri / diag[i];
         v[0] += (double)ri * (double)ri;
         v[1] += (double)ri * (double)z;
-    }
+    };
+    for_each4(n, [&](int64_t q) {
+        Q4<float> xv = ld4(x32, q), pv = ld4(p, q), rv = ld4((const float *)r, q),
+                  av = ld4(Ap, q), dv = ld4(diag, q);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            xv.v[k] += alpha * pv.v[k];
+            const float ri = rv.v[k] - alpha * av.v[k];
+            rv.v[k] = ri;
+            const float z = ri / dv.v[k];
+            v[0] += (double)ri * (double)ri;
+            v[1] += (double)ri * (double)z;
+        }
+        st4(x32, q, xv);
+        st4(r, q, rv);
+    }, one);
     write_partials<3>(v, part_out);
 }
 

@@ -145,6 +145,7 @@ constexpr int SPMV_BLOCK_NNZ = 2048, SPMV_BLOCK_ROWS = 256;
 constexpr int SPMV_WAVE_NNZ = 512, SPMV_WAVE_ROWS = 64;
 constexpr int SPMV_LARGE_NNZ = 4096;
 constexpr int DICT_BLOCK_NNZ = 4096, DICT_BLOCK_ROWS = 256;
+constexpr int DICT_GROUP = 512;   // entries of one wavefront (64 x 8)
 // Krylov building blocks (krylov.hip) used by the C ABI layer.
 int spmv64(const phg_hip_mat *A, const double *x, double *y, double alpha,
            double beta, hipStream_t s);
