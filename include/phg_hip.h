@@ -278,7 +278,12 @@ int phg_hip_adjoint_logkappa(phg_hip_space *sp, const phg_hip_problem *pb,
  *              loads the next block while reducing the current one, DICT_NP
  *              does not (fewer registers).  Not available for matrices with
  *              a row that does not fit a block (phg_hip_mat_tune_time then
- *              reports -1).
+ *              reports -1);
+ *   XS, XS_PF  staged lines: the same blocks with a dictionary of <= 128
+ *              16-column lines of x; the SpMV reads those lines into LDS
+ *              with coalesced loads and an entry's 16-bit code (slot,
+ *              offset) is its LDS index (XS_PF loads the next block while
+ *              reducing the current one).
  * phg_hip_mat_tune times each layout on the matrix (HIP events, `reps`
  * launches) in FP64 and FP32 and keeps the fastest for phg_hip_pcg and
  * phg_hip_pcg_mixed respectively. */
@@ -290,7 +295,9 @@ int phg_hip_adjoint_logkappa(phg_hip_space *sp, const phg_hip_problem *pb,
 #define PHG_HIP_FMT_SELL       5
 #define PHG_HIP_FMT_DICT       6
 #define PHG_HIP_FMT_DICT_NP    7
-#define PHG_HIP_FMT_COUNT      8
+#define PHG_HIP_FMT_XS         8
+#define PHG_HIP_FMT_XS_PF      9
+#define PHG_HIP_FMT_COUNT      10
 int phg_hip_mat_tune(phg_hip_mat *A, int32_t reps);
 int phg_hip_mat_tune_time(const phg_hip_mat *A, int32_t fmt, int32_t fp32,
                           double *us);

@@ -151,7 +151,7 @@ _sig(hip_lib, "phg_hip_profile_read", C.c_int, [_i32, _P(_i64), _P(_dbl)])
 # SpMV layouts of the solvers (include/phg_hip.h)
 FMT_NAMES = {0: "csr_b2k", 1: "csr_b4k", 2: "csr_b2k_par", 3: "csr_b4k_par",
              4: "csr_wave", 5: "sell64", 6: "csr_dict",
-             7: "csr_dict_np"}
+             7: "csr_dict_np", 8: "csr_xstage", 9: "csr_xstage_pf"}
 
 # kernel classes of phg_hip_profile_read (include/phg_hip.h)
 K_PCG_P64, K_SPMV64, K_UPDATE64, K_PCG_P32, K_SPMV32, K_UPDATE32 = range(6)

@@ -843,6 +843,8 @@ int block_solve(phg_hip_mat *A, int mixed, int32_t maxit, double rtol,
 
 // SpMM shapes the tuner chooses from: (slots per row, columns per lane)
 constexpr int NSHAPE = 6;
+static_assert(NSHAPE <= (int)(sizeof(phg_hip_mat::spmm_us) / sizeof(phg_hip_mat::spmm_us[0])),
+              "SpMM timing table too small");
 const int kShapeS[NSHAPE] = {1, 2, 4, 4, 8, 16};
 const int kShapeC[NSHAPE] = {1, 1, 1, 4, 4, 4};
 

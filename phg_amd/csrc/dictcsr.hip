@@ -2,9 +2,12 @@
 //
 // Rows (in the solver numbering) are grouped into blocks of at most
 // DICT_BLOCK_NNZ entries and DICT_BLOCK_ROWS rows.  For each block the
-// distinct 256-column chunks (column >> 8) its entries touch are listed in a
-// dictionary of at most 256 chunk bases, and every entry stores its column as
-// 16 bits: slot << 8 | (column & 255).  In the spatial (Morton) numbering a
+// distinct column groups (column >> shift) its entries touch are listed in a
+// dictionary of at most max_slots group bases, and every entry stores its
+// column as 16 bits: slot << shift | (column & (2^shift - 1)).  Two
+// instances are built: 256-column chunks (shift 8, 256 slots), decoded
+// through the dictionary; and 16-column lines (shift 4, 128 slots), whose x
+// values the SpMV stages in LDS so that the 16-bit code is the LDS index.  In the spatial (Morton) numbering a
 // block's columns fall into a few dozen chunks, so the column stream shrinks
 // from 4 to 2 bytes per entry -- the SpMV moves 6 instead of 8 bytes per
 // FP32 entry and 10 instead of 12 per FP64 entry.  Each block's entries
@@ -26,29 +29,37 @@
 
 namespace phg {
 
-void free_dict(phg_hip_mat *A)
+void free_dict_layout(DictLayout *L)
 {
-    dfree(A->c_rowptr);
-    dfree(A->c_blk_row);
-    dfree(A->c_blk_ent);
-    dfree(A->c_blk_dict);
-    dfree(A->c_dict);
-    dfree(A->c_idx);
-    dfree(A->c_val32);
-    dfree(A->c_val64);
-    A->c_rowptr = A->c_blk_row = A->c_blk_ent = A->c_blk_dict = A->c_dict =
-        nullptr;
-    A->c_idx = nullptr;
-    A->c_val32 = nullptr;
-    A->c_val64 = nullptr;
-    A->c_nblk = A->c_nent = 0;
-    A->dict_state = 0;
+    dfree(L->rowptr);
+    dfree(L->blk_row);
+    dfree(L->blk_ent);
+    dfree(L->blk_dict);
+    dfree(L->dict);
+    dfree(L->idx);
+    dfree(L->val32);
+    dfree(L->val64);
+    *L = DictLayout();
 }
 
-int build_dict(phg_hip_mat *A)
+void free_dict(phg_hip_mat *A)
 {
-    if (A->dict_state != 0)
+    free_dict_layout(&A->cd);
+    free_dict_layout(&A->xs);
+}
+
+int build_dict(phg_hip_mat *A) { return build_dict_layout(A, 8, 256, &A->cd); }
+
+int build_xstage(phg_hip_mat *A)
+{
+    return build_dict_layout(A, XS_SHIFT, XS_SLOTS, &A->xs);
+}
+
+int build_dict_layout(phg_hip_mat *A, int shift, int max_slots, DictLayout *Lo)
+{
+    if (Lo->state != 0)
         return PHG_HIP_OK;
+    const int32_t lmask = (1 << shift) - 1;
     const int64_t n = A->n, nnz = A->nnz;
     int rc;
     std::vector<int32_t> rp((size_t)n + 1), ci((size_t)nnz);
@@ -57,7 +68,7 @@ int build_dict(phg_hip_mat *A)
         (rc = download(ci.data(), A->q_cols, ci.size() * sizeof(int32_t))) ||
         (rc = download(va.data(), A->q_vals, va.size() * sizeof(double))))
         return rc;
-    const int64_t nchunk = (n + 255) / 256;
+    const int64_t nchunk = (n + lmask) >> shift;
     std::vector<int32_t> mark((size_t)nchunk, -1), slot((size_t)nchunk, 0);
     std::vector<int32_t> brow, bent, bdict, dict, crp((size_t)n + 1);
     std::vector<int32_t> fresh;
@@ -78,7 +89,7 @@ int build_dict(phg_hip_mat *A)
             const int64_t j0 = rp[(size_t)r], j1 = rp[(size_t)r + 1];
             const int64_t len = j1 - j0;
             if (len > DICT_BLOCK_NNZ) {
-                A->dict_state = -1;
+                Lo->state = -1;
                 return PHG_HIP_OK;
             }
             if ((int64_t)idx.size() - e0 + len > DICT_BLOCK_NNZ)
@@ -86,17 +97,17 @@ int build_dict(phg_hip_mat *A)
             // chunks of this row not yet in the block's dictionary
             fresh.clear();
             for (int64_t j = j0; j < j1; j++) {
-                int32_t c = ci[(size_t)j] >> 8;
+                int32_t c = ci[(size_t)j] >> shift;
                 if (mark[(size_t)c] != stamp && mark[(size_t)c] != -2 - stamp) {
                     mark[(size_t)c] = -2 - stamp;    // tentatively new
                     fresh.push_back(c);
                 }
             }
-            if (nslot + (int)fresh.size() > 256) {
+            if (nslot + (int)fresh.size() > max_slots) {
                 for (int32_t c : fresh)
                     mark[(size_t)c] = -1;
                 if (nrows == 0) {
-                    A->dict_state = -1;
+                    Lo->state = -1;
                     return PHG_HIP_OK;
                 }
                 break;
@@ -104,12 +115,12 @@ int build_dict(phg_hip_mat *A)
             for (int32_t c : fresh) {
                 mark[(size_t)c] = stamp;
                 slot[(size_t)c] = nslot++;
-                dict.push_back(c * 256);
+                dict.push_back(c << shift);
             }
             crp[(size_t)r] = (int32_t)idx.size();
             for (int64_t j = j0; j < j1; j++) {
                 int32_t c = ci[(size_t)j];
-                idx.push_back((uint16_t)(slot[(size_t)(c >> 8)] << 8 | (c & 255)));
+                idx.push_back((uint16_t)(slot[(size_t)(c >> shift)] << shift | (c & lmask)));
                 val.push_back(va[(size_t)j]);
             }
             nrows++;
@@ -140,7 +151,7 @@ int build_dict(phg_hip_mat *A)
         }
         stamp++;
         if (idx.size() > (size_t)INT32_MAX - DICT_BLOCK_NNZ) {
-            A->dict_state = -1;
+            Lo->state = -1;
             return PHG_HIP_OK;
         }
     }
@@ -152,29 +163,33 @@ int build_dict(phg_hip_mat *A)
     for (size_t i = 0; i < val.size(); i++)
         val32[i] = (float)val[i];
     const int64_t nb = (int64_t)brow.size() - 1, ne = (int64_t)idx.size();
-    if ((rc = dmalloc((void **)&A->c_rowptr, crp.size() * 4)) ||
-        (rc = dmalloc((void **)&A->c_blk_row, brow.size() * 4)) ||
-        (rc = dmalloc((void **)&A->c_blk_ent, bent.size() * 4)) ||
-        (rc = dmalloc((void **)&A->c_blk_dict, bdict.size() * 4)) ||
-        (rc = dmalloc((void **)&A->c_dict, dict.size() * 4)) ||
-        (rc = dmalloc((void **)&A->c_idx, (size_t)ne * 2)) ||
-        (rc = dmalloc((void **)&A->c_val32, (size_t)ne * 4)) ||
-        (rc = dmalloc((void **)&A->c_val64, (size_t)ne * 8)) ||
-        (rc = upload(A->c_rowptr, crp.data(), crp.size() * 4)) ||
-        (rc = upload(A->c_blk_row, brow.data(), brow.size() * 4)) ||
-        (rc = upload(A->c_blk_ent, bent.data(), bent.size() * 4)) ||
-        (rc = upload(A->c_blk_dict, bdict.data(), bdict.size() * 4)) ||
-        (rc = upload(A->c_dict, dict.data(), dict.size() * 4)) ||
-        (rc = upload(A->c_idx, idx.data(), (size_t)ne * 2)) ||
-        (rc = upload(A->c_val32, val32.data(), (size_t)ne * 4)) ||
-        (rc = upload(A->c_val64, val.data(), (size_t)ne * 8))) {
-        free_dict(A);
+    DictLayout L;
+    L.shift = shift;
+    L.max_slots = max_slots;
+    if ((rc = dmalloc((void **)&L.rowptr, crp.size() * 4)) ||
+        (rc = dmalloc((void **)&L.blk_row, brow.size() * 4)) ||
+        (rc = dmalloc((void **)&L.blk_ent, bent.size() * 4)) ||
+        (rc = dmalloc((void **)&L.blk_dict, bdict.size() * 4)) ||
+        (rc = dmalloc((void **)&L.dict, dict.size() * 4)) ||
+        (rc = dmalloc((void **)&L.idx, (size_t)ne * 2)) ||
+        (rc = dmalloc((void **)&L.val32, (size_t)ne * 4)) ||
+        (rc = dmalloc((void **)&L.val64, (size_t)ne * 8)) ||
+        (rc = upload(L.rowptr, crp.data(), crp.size() * 4)) ||
+        (rc = upload(L.blk_row, brow.data(), brow.size() * 4)) ||
+        (rc = upload(L.blk_ent, bent.data(), bent.size() * 4)) ||
+        (rc = upload(L.blk_dict, bdict.data(), bdict.size() * 4)) ||
+        (rc = upload(L.dict, dict.data(), dict.size() * 4)) ||
+        (rc = upload(L.idx, idx.data(), (size_t)ne * 2)) ||
+        (rc = upload(L.val32, val32.data(), (size_t)ne * 4)) ||
+        (rc = upload(L.val64, val.data(), (size_t)ne * 8))) {
+        free_dict_layout(&L);
         return rc;
     }
-    A->c_nblk = nb;
-    A->c_nent = ne;
-    A->c_ndict = (int64_t)dict.size();
-    A->dict_state = 1;
+    L.nblk = nb;
+    L.nent = ne;
+    L.ndict = (int64_t)dict.size();
+    L.state = 1;
+    *Lo = L;
     return PHG_HIP_OK;
 }
 

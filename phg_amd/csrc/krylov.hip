@@ -624,6 +624,95 @@ __device__ inline void dict_spmv(const DictView &m, const T *__restrict__ va,
     }
 }
 
+// Staged-line CSR stream (the shift-4 dictionary layout): per block the
+// distinct 16-column lines of x its entries use (at most XS_SLOTS) are read
+// into LDS with coalesced loads; an entry's 16-bit code slot << 4 | offset
+// is then its LDS index, so the per-entry gathers of x go to LDS instead of
+// the vector L1.  Stream loads, product stage and ordered row sums as in
+// dict_spmv.  PF = 1 loads the next block's stream into registers while the
+// current block is reduced.
+struct XsView {
+    const int32_t *rp, *brow, *bent, *bdict, *dict;
+    const uint16_t *idx;
+    int64_t nblk, n;
+};
+template <int PF> struct XsViewT : XsView {};
+
+template <int PF, typename T, typename Epi>
+__device__ inline void xs_spmv(const XsView &m, const T *__restrict__ va,
+                               const T *__restrict__ x, Epi epi)
+{
+    constexpr int NCH = DictStage<T>::NCH;
+    constexpr int LW = 1 << phg::XS_SHIFT;
+    static_assert(phg::XS_SLOTS <= BS, "one work-item per line");
+    __shared__ __align__(16) T prod[phg::DICT_BLOCK_NNZ];
+    __shared__ __align__(16) T xsh[phg::XS_SLOTS * LW];
+    __shared__ int32_t lines[phg::XS_SLOTS];
+    __shared__ DictMeta md;
+    const DictView &dv = reinterpret_cast<const DictView &>(m);
+    int64_t b0, b1;
+    row_range(m.nblk, b0, b1);
+    const int tid = threadIdx.x;
+    for (int64_t c0 = b0; c0 < b1; c0 += DICT_META) {
+        const int nb = (int)(b1 - c0 < DICT_META ? b1 - c0 : DICT_META);
+        lds_barrier();
+        if (tid <= nb) {
+            md.ent[tid] = m.bent[c0 + tid];
+            md.row[tid] = m.brow[c0 + tid];
+            md.dic[tid] = m.bdict[c0 + tid];
+        }
+        lds_barrier();
+        DictStage<T> cur, nxt;
+        dict_fetch(dv, md, va, x, 0, cur);
+        for (int i = 0; i < nb; i++) {
+            if (!PF && i > 0)
+                dict_fetch(dv, md, va, x, i, cur);
+            const int nd = md.dic[i + 1] - md.dic[i];
+            if (tid < phg::XS_SLOTS)
+                lines[tid] = cur.d;
+            lds_barrier();
+            for (int e = tid; e < nd * LW; e += BS) {
+                const int64_t col = (int64_t)lines[e >> phg::XS_SHIFT] + (e & (LW - 1));
+                xsh[e] = col < m.n ? x[col] : (T)0;
+            }
+            if (PF && i + 1 < nb)
+                dict_fetch(dv, md, va, x, i + 1, nxt);
+            lds_barrier();
+#pragma unroll
+            for (int k = 0; k < NCH; k++) {
+                const int32_t e = cur.f0 + (tid + BS * k) * 8;
+                if (e < cur.f1) {
+                    const int32_t q0 = ((e - cur.f0) / phg::DICT_GROUP) * phg::DICT_GROUP;
+                    const int32_t L = min(phg::DICT_GROUP, cur.f1 - cur.f0 - q0) / 8;
+                    const int l = tid & 63;
+                    const uint32_t w[4] = {cur.ib[k].x, cur.ib[k].y, cur.ib[k].z,
+                                           cur.ib[k].w};
+#pragma unroll
+                    for (int j = 0; j < 8; j++) {
+                        const uint32_t u = (w[j >> 1] >> (16 * (j & 1))) & 0xffffu;
+                        prod[q0 + j * L + l] = cur.vb[k].v[j] * xsh[u];
+                    }
+                }
+            }
+            lds_barrier();
+            if (tid < cur.q1 - cur.q0) {
+                T sum = ordered_sum(prod, cur.s0 - cur.f0, cur.s1 - cur.f0);
+                epi(cur.q0 + tid, sum, cur.xr);
+            }
+            if (PF)
+                cur = nxt;
+        }
+    }
+}
+
+template <typename T, int PF, typename Epi>
+__device__ inline void spmv_apply(const XsViewT<PF> &m,
+                                  const T *__restrict__ va,
+                                  const T *__restrict__ x, Epi epi)
+{
+    xs_spmv<PF, T>(m, va, x, epi);
+}
+
 template <int PF> struct DictViewT : DictView {};
 template <typename T, int PF, typename Epi>
 __device__ inline void spmv_apply(const DictViewT<PF> &m,
@@ -1333,8 +1422,11 @@ enum {
     FMT_SELL,         // SellView
     FMT_DICT,         // DictViewT<1>
     FMT_DICT_NP,      // DictViewT<0>
+    FMT_XS,           // XsViewT<0>
+    FMT_XS_PF,        // XsViewT<1>
     FMT_N
 };
+static_assert(FMT_N == PHG_HIP_FMT_COUNT, "format list and C ABI disagree");
 using CsrB2K = CsrBlockView<2048, 0>;
 using CsrB4K = CsrBlockView<4096, 0>;
 using CsrB2KPar = CsrBlockView<2048, 2>;
@@ -1352,6 +1444,8 @@ int with_format(int fmt, Fn &&f)
     case FMT_WAVE: return f(CsrWaveView{});
     case FMT_DICT: return f(DictViewT<1>{});
     case FMT_DICT_NP: return f(DictViewT<0>{});
+    case FMT_XS: return f(XsViewT<0>{});
+    case FMT_XS_PF: return f(XsViewT<1>{});
     default: return f(SellView{});
     }
 }
@@ -1398,17 +1492,33 @@ template <int PF>
 DictViewT<PF> view_of(const phg_hip_mat *A, DictViewT<PF> *)
 {
     DictViewT<PF> v;
-    v.rp = A->c_rowptr;
-    v.brow = A->c_blk_row;
-    v.bent = A->c_blk_ent;
-    v.bdict = A->c_blk_dict;
-    v.dict = A->c_dict;
-    v.idx = A->c_idx;
-    v.nblk = A->c_nblk;
+    v.rp = A->cd.rowptr;
+    v.brow = A->cd.blk_row;
+    v.bent = A->cd.blk_ent;
+    v.bdict = A->cd.blk_dict;
+    v.dict = A->cd.dict;
+    v.idx = A->cd.idx;
+    v.nblk = A->cd.nblk;
     return v;
 }
 template <int PF>
-int64_t units_of(const phg_hip_mat *A, DictViewT<PF> *) { return A->c_nblk; }
+int64_t units_of(const phg_hip_mat *A, DictViewT<PF> *) { return A->cd.nblk; }
+template <int PF>
+XsViewT<PF> view_of(const phg_hip_mat *A, XsViewT<PF> *)
+{
+    XsViewT<PF> v;
+    v.rp = A->xs.rowptr;
+    v.brow = A->xs.blk_row;
+    v.bent = A->xs.blk_ent;
+    v.bdict = A->xs.blk_dict;
+    v.dict = A->xs.dict;
+    v.idx = A->xs.idx;
+    v.nblk = A->xs.nblk;
+    v.n = A->n;
+    return v;
+}
+template <int PF>
+int64_t units_of(const phg_hip_mat *A, XsViewT<PF> *) { return A->xs.nblk; }
 SellView view_of(const phg_hip_mat *A, SellView *)
 {
     return SellView{A->d_sptr, A->d_swidth, A->d_scols, A->nslice, A->n};
@@ -1442,8 +1552,10 @@ int resident_groups(const void *kern)
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, BS, 0) !=
             hipSuccess ||
-        per_cu <= 0)
-        per_cu = 8;
+        per_cu <= 0) {
+        (void)hipGetLastError();   // the query's failure is not the caller's
+        per_cu = 1;
+    }
     int cus = phg::ctx()->num_cus > 0 ? phg::ctx()->num_cus : 256;
     int g = std::min(per_cu * cus, MAX_GRID);
     cache.emplace_back(kern, g);
@@ -1467,9 +1579,16 @@ template <typename View> struct FmtData {
     static const float *d32(const phg_hip_mat *A) { return A->q_diag32; }
     static const int32_t *perm(const phg_hip_mat *A) { return A->q_perm; }
 };
+template <int PF> struct FmtData<XsViewT<PF>> {
+    static const double *v64(const phg_hip_mat *A) { return A->xs.val64; }
+    static const float *v32(const phg_hip_mat *A) { return A->xs.val32; }
+    static const double *d64(const phg_hip_mat *A) { return A->q_diag; }
+    static const float *d32(const phg_hip_mat *A) { return A->q_diag32; }
+    static const int32_t *perm(const phg_hip_mat *A) { return A->q_perm; }
+};
 template <int PF> struct FmtData<DictViewT<PF>> {
-    static const double *v64(const phg_hip_mat *A) { return A->c_val64; }
-    static const float *v32(const phg_hip_mat *A) { return A->c_val32; }
+    static const double *v64(const phg_hip_mat *A) { return A->cd.val64; }
+    static const float *v32(const phg_hip_mat *A) { return A->cd.val32; }
     static const double *d64(const phg_hip_mat *A) { return A->q_diag; }
     static const float *d32(const phg_hip_mat *A) { return A->q_diag32; }
     static const int32_t *perm(const phg_hip_mat *A) { return A->q_perm; }
@@ -2015,12 +2134,20 @@ int time_spmv(phg_hip_mat *A, const T *va, int reps, double *us)
                            view_of(A, (View *)nullptr), va, (const T *)p, Ap,
                            ws->seg(SEG_S), (const State *)(ws->st + 2));
     }
-    PHG_HIP_CHECK(hipEventRecord(e1, s));
-    PHG_HIP_CHECK(hipEventSynchronize(e1));
+    hipError_t err = hipGetLastError();
     float ms = 0.f;
-    PHG_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    if (err == hipSuccess)
+        err = hipEventRecord(e1, s);
+    if (err == hipSuccess)
+        err = hipEventSynchronize(e1);
+    if (err == hipSuccess)
+        err = hipEventElapsedTime(&ms, e0, e1);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    if (err != hipSuccess)
+        return phg::set_error(PHG_HIP_E_HIP, "SpMV timing (grid %d, %lld units): %s",
+                              gs, (long long)units_of(A, (View *)nullptr),
+                              hipGetErrorString(err));
     *us = 1e3 * ms / reps;
     return PHG_HIP_OK;
 }
@@ -2232,9 +2359,16 @@ int ensure_format(phg_hip_mat *A, int fmt, int need32)
     if (fmt == FMT_DICT || fmt == FMT_DICT_NP) {
         if ((rc = phg::build_dict(A)))
             return rc;
-        if (A->dict_state != 1)
+        if (A->cd.state != 1)
             return phg::set_error(PHG_HIP_E_ARG,
                                   "matrix has no dictionary-compressed layout");
+    }
+    if (fmt == FMT_XS || fmt == FMT_XS_PF) {
+        if ((rc = phg::build_xstage(A)))
+            return rc;
+        if (A->xs.state != 1)
+            return phg::set_error(PHG_HIP_E_ARG,
+                                  "matrix has no staged-line layout");
     }
     if (need32 && (rc = phg::ensure_fp32_copy(A)))
         return rc;
@@ -2544,11 +2678,13 @@ extern "C" int phg_hip_mat_tune(phg_hip_mat *A, int32_t reps)
         return phg::set_error(PHG_HIP_E_ARG, "phg_hip_mat_tune: matrix not assembled");
     if (reps <= 0)
         reps = 10;
-    if ((rc = ensure_format(A, FMT_SELL, 1)) || (rc = phg::build_dict(A)))
+    if ((rc = ensure_format(A, FMT_SELL, 1)) || (rc = phg::build_dict(A)) ||
+        (rc = phg::build_xstage(A)))
         return rc;
     for (int f = 0; f < FMT_N; f++) {
         double t64 = 0, t32 = 0;
-        if ((f == FMT_DICT || f == FMT_DICT_NP) && A->dict_state != 1) {
+        if (((f == FMT_DICT || f == FMT_DICT_NP) && A->cd.state != 1) ||
+            ((f == FMT_XS || f == FMT_XS_PF) && A->xs.state != 1)) {
             A->tune_us[f][0] = A->tune_us[f][1] = -1.0;   // not representable
             continue;
         }
@@ -2580,9 +2716,11 @@ extern "C" int phg_hip_mat_tune(phg_hip_mat *A, int32_t reps)
     // copies no solver uses are released
     if (b64 != FMT_SELL && b32 != FMT_SELL)
         phg::free_sell(A);
-    if (b64 != FMT_DICT && b32 != FMT_DICT && b64 != FMT_DICT_NP &&
-        b32 != FMT_DICT_NP)
-        phg::free_dict(A);
+    auto uses = [&](int a, int b) { return b64 == a || b32 == a || b64 == b || b32 == b; };
+    if (!uses(FMT_DICT, FMT_DICT_NP))
+        phg::free_dict_layout(&A->cd);
+    if (!uses(FMT_XS, FMT_XS_PF))
+        phg::free_dict_layout(&A->xs);
     return PHG_HIP_OK;
 }
 
@@ -2602,11 +2740,15 @@ extern "C" int phg_hip_mat_format_bytes(const phg_hip_mat *A, int32_t fmt,
         break;
     case FMT_DICT:
     case FMT_DICT_NP:
-        if (A->dict_state != 1)
+    case FMT_XS:
+    case FMT_XS_PF: {
+        const DictLayout &L = (fmt == FMT_XS || fmt == FMT_XS_PF) ? A->xs : A->cd;
+        if (L.state != 1)
             return phg::set_error(PHG_HIP_E_STATE, "dictionary layout not built");
-        *bytes = A->c_nent * (sv + 2) + (n + 1) * 4 + (A->c_nblk + 1) * 12 +
-                 A->c_ndict * 4 + vec;
+        *bytes = L.nent * (sv + 2) + (n + 1) * 4 + (L.nblk + 1) * 12 +
+                 L.ndict * 4 + vec;
         break;
+    }
     default:
         *bytes = A->nnz * (sv + 4) + (n + 1) * 4 + vec;
     }
@@ -2648,14 +2790,20 @@ extern "C" int phg_hip_stream_copy(int64_t n, int32_t reps, double *us)
         hipLaunchKernelGGL(k_copy, dim3(g), dim3(BS), 0, s, n,
                            (const double *)a, b);
     }
-    PHG_HIP_CHECK(hipEventRecord(e1, s));
-    PHG_HIP_CHECK(hipEventSynchronize(e1));
+    hipError_t err = hipGetLastError();
     float ms = 0.f;
-    PHG_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    if (err == hipSuccess)
+        err = hipEventRecord(e1, s);
+    if (err == hipSuccess)
+        err = hipEventSynchronize(e1);
+    if (err == hipSuccess)
+        err = hipEventElapsedTime(&ms, e0, e1);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     phg::dfree(a);
     phg::dfree(b);
+    if (err != hipSuccess)
+        return phg::set_error(PHG_HIP_E_HIP, "copy timing: %s", hipGetErrorString(err));
     *us = 1e3 * ms / reps;
     return PHG_HIP_OK;
 }

@@ -20,6 +20,10 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line);
             return ::phg::hip_fail(e_, #call, __FILE__, __LINE__);            \
     } while (0)
 
+// SpMV storage formats (PHG_HIP_FMT_*) the per-matrix timing table holds
+constexpr int MAX_FORMATS = 16;
+static_assert(PHG_HIP_FMT_COUNT <= MAX_FORMATS, "timing table too small");
+
 // library context ----------------------------------------------------------
 struct Context {
     int device = -1;
@@ -60,6 +64,25 @@ struct phg_hip_vec {
     double *d = nullptr;
 };
 
+// Blocks of rows; per block a dictionary of the distinct column groups
+// (column >> shift) its entries touch; every entry stores its column as 16
+// bits: slot << shift | (column & (2^shift - 1)).  Entries of a block start
+// at a multiple of 8 (padding: index 0, value 0) and each group of 512
+// entries is stored transposed (8 per work-item).
+struct DictLayout {
+    int state = 0;                   // 0 not built, 1 built, -1 not representable
+    int shift = 0, max_slots = 0;
+    int64_t nblk = 0, nent = 0, ndict = 0;
+    int32_t *rowptr = nullptr;       // [n+1] entry positions (padded)
+    int32_t *blk_row = nullptr;      // [nblk+1] first row of each block
+    int32_t *blk_ent = nullptr;      // [nblk+1] first entry of each block
+    int32_t *blk_dict = nullptr;     // [nblk+1] first dictionary slot
+    int32_t *dict = nullptr;         // group bases (multiples of 2^shift)
+    uint16_t *idx = nullptr;
+    float *val32 = nullptr;
+    double *val64 = nullptr;
+};
+
 struct phg_hip_mat {
     int64_t n = 0, nnz = 0;
     int dim = 1;
@@ -96,7 +119,7 @@ struct phg_hip_mat {
     // SpMV storage format used by the solvers, per precision, and the times
     // the autotuner measured for each format [fmt][0: FP64, 1: FP32] (us)
     int fmt64 = 0, fmt32 = 0;
-    double tune_us[8][2] = {};
+    double tune_us[phg::MAX_FORMATS][2] = {};
     // SELL-64 copy built from the solver CSR: rows sorted by length inside
     // windows of 4096 rows (d_perm: SELL index -> PHG index, composed with
     // q_perm), slices of 64 rows stored column-major, padded with zeros
@@ -110,20 +133,10 @@ struct phg_hip_mat {
     double *d_sdiag = nullptr;       // permuted diagonal
     float *d_sdiag32 = nullptr;
     int64_t sell_nnz = 0;            // stored entries including padding
-    // Dictionary-compressed CSR of the solver matrix (dictcsr.hip): blocks
-    // of rows, per block a dictionary of <= 256 column-chunk bases, 16-bit
-    // entries slot << 8 | column & 255; entries of a block start at a
-    // multiple of 8 (padding: index 0, value 0)
-    int dict_state = 0;              // 0 not built, 1 built, -1 not representable
-    int64_t c_nblk = 0, c_nent = 0, c_ndict = 0;
-    int32_t *c_rowptr = nullptr;     // [n+1] entry positions (padded)
-    int32_t *c_blk_row = nullptr;    // [nblk+1] first row of each block
-    int32_t *c_blk_ent = nullptr;    // [nblk+1] first entry of each block
-    int32_t *c_blk_dict = nullptr;   // [nblk+1] first dictionary slot
-    int32_t *c_dict = nullptr;       // chunk bases (multiples of 256)
-    uint16_t *c_idx = nullptr;
-    float *c_val32 = nullptr;
-    double *c_val64 = nullptr;
+    // Dictionary-compressed CSR copies of the solver matrix (dictcsr.hip):
+    // cd with 256-column chunks (columns decoded through the dictionary),
+    // xs with 16-column lines (the lines are staged in LDS by the SpMV)
+    DictLayout cd, xs;
     // block copy for the many-right-hand-side solver (block.hip): 3x3
     // blocks for 3-component problems (b_dim 3), else the solver CSR itself
     // (b_dim 1, arrays aliased); b_dim 0: not built
@@ -162,8 +175,13 @@ int ensure_fp32_copy(phg_hip_mat *A);
 // block copy for many right-hand sides (block.hip)
 int build_bsr(phg_hip_mat *A);
 void free_bsr(phg_hip_mat *A);
-// dictionary-compressed CSR (dictcsr.hip)
-int build_dict(phg_hip_mat *A);
-void free_dict(phg_hip_mat *A);
+// dictionary-compressed CSR (dictcsr.hip): shift 8 / 256 slots (chunk
+// dictionary) or shift 4 / XS_SLOTS slots (staged lines)
+constexpr int XS_SHIFT = 4, XS_SLOTS = 128;
+int build_dict_layout(phg_hip_mat *A, int shift, int max_slots, DictLayout *L);
+int build_dict(phg_hip_mat *A);     // A->cd
+int build_xstage(phg_hip_mat *A);   // A->xs
+void free_dict_layout(DictLayout *L);
+void free_dict(phg_hip_mat *A);     // both
 int setup_diagonal(phg_hip_mat *A);
 }  // namespace phg

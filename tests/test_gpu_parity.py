@@ -138,7 +138,7 @@ def test_pcg_matches_oracle(hip, kind, p, n):
     assert rel < 1e-9, rel
 
 
-@pytest.mark.parametrize("fmt", [0, 1, 2, 3, 4, 5, 6, 7])
+@pytest.mark.parametrize("fmt", list(range(10)))
 def test_pcg_every_spmv_layout(hip, fmt):
     """FP64 PCG and the mixed solver agree with the oracle on each layout."""
     pa = hip
@@ -170,7 +170,7 @@ def test_tuner_picks_measured_layout(hip):
     A.eliminate()
     t = A.tune(reps=5)
     names = tuple(pa.FMT_NAMES.values())
-    assert all(t[k][0] > 0 and t[k][1] > 0 for k in names)
+    assert all(t[k][0] > 0 and t[k][1] > 0 for k in names), t
     f64, f32 = t["chosen"]
     assert t[f64][0] == min(t[k][0] for k in names)
     assert t[f32][1] == min(t[k][1] for k in names)
