@@ -1,27 +1,29 @@
 // Dictionary-compressed CSR of the solver matrix.
 //
 // Rows (in the solver numbering) are grouped into blocks of at most
-// DICT_BLOCK_NNZ entries and DICT_BLOCK_ROWS rows.  For each block the
-// distinct column groups (column >> shift) its entries touch are listed in a
-// dictionary of at most max_slots group bases, and every entry stores its
-// column as 16 bits: slot << shift | (column & (2^shift - 1)).  Two
-// instances are built: 256-column chunks (shift 8, 256 slots), decoded
-// through the dictionary; and 16-column lines (shift 4, 128 slots), whose x
-// values the SpMV stages in LDS so that the 16-bit code is the LDS index.  In the spatial (Morton) numbering a
-// block's columns fall into a few dozen chunks, so the column stream shrinks
-// from 4 to 2 bytes per entry -- the SpMV moves 6 instead of 8 bytes per
-// FP32 entry and 10 instead of 12 per FP64 entry.  Each block's entries
-// start at a multiple of 8 so a work-item loads 8 indices and 8 values with
-// 16-byte loads; the padding entries have value 0 and index 0 and are
-// counted in the last row of the block (they add +0 to its sum).
+// block_nnz entries (DICT_BLOCK_NNZ or XS_BLOCK_NNZ) and DICT_BLOCK_ROWS
+// rows.  For each block the distinct column groups (column >> shift) its
+// entries touch are listed in a dictionary of at most max_slots group bases,
+// and every entry stores its column as 16 bits:
+// slot << shift | (column & (2^shift - 1)).  Two instances are built:
+// 256-column chunks (shift 8, 256 slots), decoded through the dictionary;
+// and 16-column lines (shift 4, 128 slots), whose x values the SpMV stages
+// in LDS so that the 16-bit code is the LDS index.  In the spatial (Morton)
+// numbering a block's columns fall into a few dozen chunks (about 80 lines),
+// so the column stream shrinks from 4 to 2 bytes per entry -- the SpMV
+// moves 6 instead of 8 bytes per FP32 entry and 10 instead of 12 per FP64
+// entry.  Each block's entries start at a multiple of 8 so a work-item loads
+// 8 indices and 8 values with 16-byte loads; the padding entries have value
+// 0 and index 0 and are counted in the last row of the block (they add +0
+// to its sum).
 //
 // Inside a block, groups of up to 512 entries (one wavefront's share) are
 // stored transposed (8 entries per work-item, see the builder); row
-// pointers index the untransposed order, which is the solver CSR's
-// storage order within every row.  The builder
-// runs on the host over the solver CSR once per matrix; a matrix with a row
-// longer than a block or needing more than 256 chunks in one row is marked
-// as not representable (the autotuner then skips the layout).
+// pointers index the untransposed order, which is the solver CSR's storage
+// order within every row.  The builder runs on the host over the solver CSR
+// once per matrix; a matrix with a row longer than a block or needing more
+// than max_slots groups in one row is marked as not representable (the
+// autotuner then skips the layout).
 #include "phg_hip_internal.h"
 
 #include <algorithm>
@@ -48,14 +50,18 @@ void free_dict(phg_hip_mat *A)
     free_dict_layout(&A->xs);
 }
 
-int build_dict(phg_hip_mat *A) { return build_dict_layout(A, 8, 256, &A->cd); }
+int build_dict(phg_hip_mat *A)
+{
+    return build_dict_layout(A, 8, 256, DICT_BLOCK_NNZ, &A->cd);
+}
 
 int build_xstage(phg_hip_mat *A)
 {
-    return build_dict_layout(A, XS_SHIFT, XS_SLOTS, &A->xs);
+    return build_dict_layout(A, XS_SHIFT, XS_SLOTS, XS_BLOCK_NNZ, &A->xs);
 }
 
-int build_dict_layout(phg_hip_mat *A, int shift, int max_slots, DictLayout *Lo)
+int build_dict_layout(phg_hip_mat *A, int shift, int max_slots, int block_nnz,
+                      DictLayout *Lo)
 {
     if (Lo->state != 0)
         return PHG_HIP_OK;
@@ -88,11 +94,11 @@ int build_dict_layout(phg_hip_mat *A, int shift, int max_slots, DictLayout *Lo)
         while (r < n && nrows < DICT_BLOCK_ROWS) {
             const int64_t j0 = rp[(size_t)r], j1 = rp[(size_t)r + 1];
             const int64_t len = j1 - j0;
-            if (len > DICT_BLOCK_NNZ) {
+            if (len > block_nnz) {
                 Lo->state = -1;
                 return PHG_HIP_OK;
             }
-            if ((int64_t)idx.size() - e0 + len > DICT_BLOCK_NNZ)
+            if ((int64_t)idx.size() - e0 + len > block_nnz)
                 break;
             // chunks of this row not yet in the block's dictionary
             fresh.clear();
@@ -150,7 +156,7 @@ int build_dict_layout(phg_hip_mat *A, int shift, int max_slots, DictLayout *Lo)
             }
         }
         stamp++;
-        if (idx.size() > (size_t)INT32_MAX - DICT_BLOCK_NNZ) {
+        if (idx.size() > (size_t)INT32_MAX - block_nnz) {
             Lo->state = -1;
             return PHG_HIP_OK;
         }

@@ -491,41 +491,42 @@ __device__ inline Vals8<T> load8(const T *p)
     return r;
 }
 
-template <typename T> struct DictStage {
-    static constexpr int NCH = phg::DICT_BLOCK_NNZ / (BS * 8);
+template <typename T, int CAP = phg::DICT_BLOCK_NNZ> struct DictStage {
+    static constexpr int NCH = CAP / (BS * 8);
+    static_assert(NCH >= 1 && NCH * BS * 8 == CAP, "block capacity");
     uint4 ib[NCH];
     Vals8<T> vb[NCH];
     int32_t d, f0, f1, q0, q1, s0, s1;
     T xr;          // x of the work-item's row (for the epilogue)
 };
 
-// block metadata staged in LDS: entry start, first row, first dictionary
-// slot of blocks c0 .. c0 + DICT_META (one more for the ends)
+// blocks whose metadata a work-group stages in LDS at a time
 constexpr int DICT_META = 64;
-struct DictMeta {
-    int32_t ent[DICT_META + 1], row[DICT_META + 1], dic[DICT_META + 1];
-};
 
-template <typename T>
-__device__ inline void dict_fetch(const DictView &m, const DictMeta &md,
-                                  const T *__restrict__ va,
-                                  const T *__restrict__ x, int i,
-                                  DictStage<T> &st)
+// loads block (entries [f0, f1), rows [q0, q1), dictionary slots [d0, d1))
+// into registers: 8 indices and 8 values per work-item and chunk, the
+// work-item's dictionary slot, and the row pointers and x of its row
+template <typename T, int CAP>
+__device__ inline void stage_fetch(const DictView &m, const T *__restrict__ va,
+                                   const T *__restrict__ x, int32_t f0,
+                                   int32_t f1, int32_t q0, int32_t q1,
+                                   int32_t d0, int32_t d1,
+                                   DictStage<T, CAP> &st)
 {
     const int tid = threadIdx.x;
-    st.f0 = md.ent[i];
-    st.f1 = md.ent[i + 1];
-    st.q0 = md.row[i];
-    st.q1 = md.row[i + 1];
+    st.f0 = f0;
+    st.f1 = f1;
+    st.q0 = q0;
+    st.q1 = q1;
 #pragma unroll
-    for (int k = 0; k < DictStage<T>::NCH; k++) {
+    for (int k = 0; k < DictStage<T, CAP>::NCH; k++) {
         const int32_t e = st.f0 + (tid + BS * k) * 8;
         if (e < st.f1) {
             st.ib[k] = *(const uint4 *)(m.idx + e);
             st.vb[k] = load8(va + e);
         }
     }
-    const int32_t d0 = md.dic[i], nd = md.dic[i + 1] - d0;
+    const int32_t nd = d1 - d0;
     st.d = tid < nd ? m.dict[d0 + tid] : 0;
     const int32_t r = st.q0 + tid;
     if (r < st.q1) {
@@ -533,6 +534,69 @@ __device__ inline void dict_fetch(const DictView &m, const DictMeta &md,
         st.s1 = m.rp[r + 1];
         st.xr = x[r];
     }
+}
+
+// Blocks of one XCD swept together: the blocks are split into NXCD equal
+// regions, region k processed by the work-groups the dispatcher places on
+// XCD k (blockIdx % NXCD == k, as xcd_block assumes), work-group j of the
+// XCD taking blocks lo + j, lo + j + w, ... (w work-groups on the XCD).
+// At any time an XCD streams one window of consecutive blocks (DRAM page
+// locality, as a copy kernel), and the x entries a region gathers stay in
+// that XCD's L2.
+__device__ inline void xcd_sweep(int64_t n, int64_t &first, int64_t &end,
+                                 int64_t &step)
+{
+    const int64_t nb = gridDim.x, b = blockIdx.x;
+    if (nb < NXCD) {
+        first = b;
+        end = n;
+        step = nb;
+        return;
+    }
+    const int64_t x = b % NXCD, j = b / NXCD;
+    step = nb / NXCD + (x < nb % NXCD ? 1 : 0);
+    first = n * x / NXCD + j;
+    end = n * (x + 1) / NXCD;
+}
+
+// block metadata of up to DICT_META blocks of a sweep, staged in LDS
+struct SweepMeta {
+    int32_t f0[DICT_META], f1[DICT_META], q0[DICT_META], q1[DICT_META];
+    int32_t d0[DICT_META], d1[DICT_META];
+};
+
+// stages the metadata of blocks c0, c0 + step, ... (at most DICT_META, all
+// before end) and returns their number; brackets the writes with LDS
+// barriers so the previous chunk is done with md
+__device__ inline int sweep_meta(const DictView &m, int64_t c0, int64_t end,
+                                 int64_t step, SweepMeta &md)
+{
+    static_assert(DICT_META <= BS, "one work-item per metadata entry");
+    const int64_t left = (end - c0 + step - 1) / step;
+    const int nb = (int)(left < DICT_META ? left : DICT_META);
+    const int tid = threadIdx.x;
+    lds_barrier();
+    if (tid < nb) {
+        const int64_t blk = c0 + (int64_t)tid * step;
+        md.f0[tid] = m.bent[blk];
+        md.f1[tid] = m.bent[blk + 1];
+        md.q0[tid] = m.brow[blk];
+        md.q1[tid] = m.brow[blk + 1];
+        md.d0[tid] = m.bdict[blk];
+        md.d1[tid] = m.bdict[blk + 1];
+    }
+    lds_barrier();
+    return nb;
+}
+
+template <typename T, int CAP>
+__device__ inline void sweep_fetch(const DictView &m, const SweepMeta &md,
+                                   const T *__restrict__ va,
+                                   const T *__restrict__ x, int i,
+                                   DictStage<T, CAP> &st)
+{
+    stage_fetch(m, va, x, md.f0[i], md.f1[i], md.q0[i], md.q1[i], md.d0[i],
+                md.d1[i], st);
 }
 
 template <typename T>
@@ -559,27 +623,19 @@ __device__ inline void dict_spmv(const DictView &m, const T *__restrict__ va,
 {
     constexpr int NCH = DictStage<T>::NCH;
     static_assert(phg::DICT_BLOCK_ROWS <= BS, "one work-item per row");
-    static_assert(DICT_META < BS, "one work-item per metadata entry");
     __shared__ __align__(16) T prod[phg::DICT_BLOCK_NNZ];
     __shared__ int32_t dsh[256];
-    __shared__ DictMeta md;
-    int64_t b0, b1;
-    row_range(m.nblk, b0, b1);
+    __shared__ SweepMeta md;
+    int64_t first, end, step;
+    xcd_sweep(m.nblk, first, end, step);
     const int tid = threadIdx.x;
-    for (int64_t c0 = b0; c0 < b1; c0 += DICT_META) {
-        const int nb = (int)(b1 - c0 < DICT_META ? b1 - c0 : DICT_META);
-        lds_barrier();      // the previous chunk is done with md
-        if (tid <= nb) {
-            md.ent[tid] = m.bent[c0 + tid];
-            md.row[tid] = m.brow[c0 + tid];
-            md.dic[tid] = m.bdict[c0 + tid];
-        }
-        lds_barrier();
+    for (int64_t c0 = first; c0 < end; c0 += step * DICT_META) {
+        const int nb = sweep_meta(m, c0, end, step, md);
         DictStage<T> cur, nxt;
-        dict_fetch(m, md, va, x, 0, cur);
+        sweep_fetch(m, md, va, x, 0, cur);
         for (int i = 0; i < nb; i++) {
             if (!PF && i > 0)
-                dict_fetch(m, md, va, x, i, cur);
+                sweep_fetch(m, md, va, x, i, cur);
             dsh[tid] = cur.d;
             lds_barrier();
             T xg[NCH][8];
@@ -598,7 +654,7 @@ __device__ inline void dict_spmv(const DictView &m, const T *__restrict__ va,
                 }
             }
             if (PF && i + 1 < nb)
-                dict_fetch(m, md, va, x, i + 1, nxt);
+                sweep_fetch(m, md, va, x, i + 1, nxt);
 #pragma unroll
             for (int k = 0; k < NCH; k++) {
                 const int32_t e = cur.f0 + (tid + BS * k) * 8;
@@ -642,32 +698,26 @@ template <int PF, typename T, typename Epi>
 __device__ inline void xs_spmv(const XsView &m, const T *__restrict__ va,
                                const T *__restrict__ x, Epi epi)
 {
-    constexpr int NCH = DictStage<T>::NCH;
+    using Stage = DictStage<T, phg::XS_BLOCK_NNZ>;
+    constexpr int NCH = Stage::NCH;
     constexpr int LW = 1 << phg::XS_SHIFT;
     static_assert(phg::XS_SLOTS <= BS, "one work-item per line");
-    __shared__ __align__(16) T prod[phg::DICT_BLOCK_NNZ];
+    __shared__ __align__(16) T prod[phg::XS_BLOCK_NNZ];
     __shared__ __align__(16) T xsh[phg::XS_SLOTS * LW];
     __shared__ int32_t lines[phg::XS_SLOTS];
-    __shared__ DictMeta md;
+    __shared__ SweepMeta md;
     const DictView &dv = reinterpret_cast<const DictView &>(m);
-    int64_t b0, b1;
-    row_range(m.nblk, b0, b1);
+    int64_t first, end, step;
+    xcd_sweep(m.nblk, first, end, step);
     const int tid = threadIdx.x;
-    for (int64_t c0 = b0; c0 < b1; c0 += DICT_META) {
-        const int nb = (int)(b1 - c0 < DICT_META ? b1 - c0 : DICT_META);
-        lds_barrier();
-        if (tid <= nb) {
-            md.ent[tid] = m.bent[c0 + tid];
-            md.row[tid] = m.brow[c0 + tid];
-            md.dic[tid] = m.bdict[c0 + tid];
-        }
-        lds_barrier();
-        DictStage<T> cur, nxt;
-        dict_fetch(dv, md, va, x, 0, cur);
+    for (int64_t c0 = first; c0 < end; c0 += step * DICT_META) {
+        const int nb = sweep_meta(dv, c0, end, step, md);
+        Stage cur, nxt;
+        sweep_fetch(dv, md, va, x, 0, cur);
         for (int i = 0; i < nb; i++) {
             if (!PF && i > 0)
-                dict_fetch(dv, md, va, x, i, cur);
-            const int nd = md.dic[i + 1] - md.dic[i];
+                sweep_fetch(dv, md, va, x, i, cur);
+            const int nd = md.d1[i] - md.d0[i];
             if (tid < phg::XS_SLOTS)
                 lines[tid] = cur.d;
             lds_barrier();
@@ -676,7 +726,7 @@ __device__ inline void xs_spmv(const XsView &m, const T *__restrict__ va,
                 xsh[e] = col < m.n ? x[col] : (T)0;
             }
             if (PF && i + 1 < nb)
-                dict_fetch(dv, md, va, x, i + 1, nxt);
+                sweep_fetch(dv, md, va, x, i + 1, nxt);
             lds_barrier();
 #pragma unroll
             for (int k = 0; k < NCH; k++) {

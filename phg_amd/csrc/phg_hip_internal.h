@@ -175,10 +175,12 @@ int ensure_fp32_copy(phg_hip_mat *A);
 // block copy for many right-hand sides (block.hip)
 int build_bsr(phg_hip_mat *A);
 void free_bsr(phg_hip_mat *A);
-// dictionary-compressed CSR (dictcsr.hip): shift 8 / 256 slots (chunk
-// dictionary) or shift 4 / XS_SLOTS slots (staged lines)
-constexpr int XS_SHIFT = 4, XS_SLOTS = 128;
-int build_dict_layout(phg_hip_mat *A, int shift, int max_slots, DictLayout *L);
+// dictionary-compressed CSR (dictcsr.hip): shift 8 / 256 slots / blocks of
+// DICT_BLOCK_NNZ entries (chunk dictionary) or shift 4 / XS_SLOTS slots /
+// blocks of XS_BLOCK_NNZ entries (staged lines)
+constexpr int XS_SHIFT = 4, XS_SLOTS = 128, XS_BLOCK_NNZ = 4096;
+int build_dict_layout(phg_hip_mat *A, int shift, int max_slots, int block_nnz,
+                      DictLayout *L);
 int build_dict(phg_hip_mat *A);     // A->cd
 int build_xstage(phg_hip_mat *A);   // A->xs
 void free_dict_layout(DictLayout *L);
