@@ -193,6 +193,34 @@ int phg_hip_pcg_multi(int32_t nmat, phg_hip_mat *const *A,
                       double delta, int32_t use_graph, int32_t *nits,
                       double *residual);
 
+/*------------------------------------------------- matrix-free operator */
+
+/* Matrix-free FP32 operator for the inner iterations of
+ * phg_hip_pcg_reliable / phg_hip_pcg_multi (no reference counterpart).
+ * For the eliminated matrix A of a scalar stiffness problem (pb->dim 1, no
+ * mass term; P1-P3 on the space sp, kappa as given to phg_hip_assemble) it
+ * stores per element the ten coefficients kappa_e vol_e grad lambda_a .
+ * grad lambda_b, and applies K_e = sum_m c_m T^m (the stiffness of
+ * quad.c:5260 phgQuadGradBasAGradBas, factored through fixed reference
+ * tensors T^m) to x on the matrix cores (bf16 MFMA, operands split into
+ * three bf16 parts so products keep FP32 precision), then sums each row
+ * over its elements.  Dirichlet rows and columns are treated as
+ * phg_hip_mat_eliminate leaves them.  The FP64 residuals of the solvers keep
+ * using the assembled matrix.  After the call the solvers use the operator;
+ * phg_hip_mat_use_element_operator(A, 0 / 1) switches it off / on. */
+int phg_hip_mat_element_operator(phg_hip_mat *A, const phg_hip_space *sp,
+                                 const phg_hip_problem *pb, const double *kappa);
+int phg_hip_mat_use_element_operator(phg_hip_mat *A, int32_t on);
+/* y = A x through the operator (FP32 arithmetic, FP64 vectors in PHG's
+ * numbering); for checks against phg_hip_matvec. */
+int phg_hip_element_apply(const phg_hip_mat *A, const phg_hip_vec *x,
+                          phg_hip_vec *y);
+/* Algorithmic bytes of one application: element kernel (map, coefficients,
+ * element results, x) and row gather (incidence codes, element results,
+ * row offsets, flags, p, Ap). */
+int phg_hip_element_bytes(const phg_hip_mat *A, int64_t *elem_bytes,
+                          int64_t *gather_bytes);
+
 /*------------------------------------------------ many right-hand sides */
 
 /* Block of right-hand sides (no reference counterpart: solver-phg.c:495

@@ -135,6 +135,10 @@ _sig(hip_lib, "phg_hip_pcg_block", C.c_int,
 _sig(hip_lib, "phg_hip_pcg_multi", C.c_int,
      [_i32, _P(_vp), _P(_vp), _P(_vp), _i32, _dbl, _dbl, _dbl, _dbl, _i32, _P(_i32),
       _P(_dbl)])
+_sig(hip_lib, "phg_hip_mat_element_operator", C.c_int, [_vp, _vp, _P(Problem), _P(_dbl)])
+_sig(hip_lib, "phg_hip_mat_use_element_operator", C.c_int, [_vp, _i32])
+_sig(hip_lib, "phg_hip_element_apply", C.c_int, [_vp, _vp, _vp])
+_sig(hip_lib, "phg_hip_element_bytes", C.c_int, [_vp, _P(_i64), _P(_i64)])
 _sig(hip_lib, "phg_hip_spmm_tune", C.c_int, [_vp, _i32])
 _sig(hip_lib, "phg_hip_spmm_tune_time", C.c_int, [_vp, _i32, _i32, _P(_dbl)])
 _sig(hip_lib, "phg_hip_spmm_bytes", C.c_int, [_vp, _i32, _P(_i64)])
@@ -410,6 +414,27 @@ class Mat:
 
     def matvec(self, x: Vec, y: Vec, alpha=1.0, beta=0.0) -> None:
         _check(hip_lib.phg_hip_matvec(alpha, self._h, x._h, beta, y._h), "matvec")
+
+    def element_operator(self, sp: "Space", pb: "Problem", kappa=None) -> None:
+        """Builds the matrix-free FP32 operator (MFMA element products) that
+        the reliable-update and batched solvers then use for their FP32
+        iterations (include/phg_hip.h phg_hip_mat_element_operator)."""
+        kap = _opt(kappa)
+        _check(hip_lib.phg_hip_mat_element_operator(self._h, sp._h, C.byref(pb),
+                                                    _ptr(kap, _dbl)), "mat_element_operator")
+
+    def use_element_operator(self, on: bool) -> None:
+        _check(hip_lib.phg_hip_mat_use_element_operator(self._h, 1 if on else 0),
+               "mat_use_element_operator")
+
+    def element_apply(self, x: Vec, y: Vec) -> None:
+        _check(hip_lib.phg_hip_element_apply(self._h, x._h, y._h), "element_apply")
+
+    def element_bytes(self):
+        a, b = _i64(), _i64()
+        _check(hip_lib.phg_hip_element_bytes(self._h, C.byref(a), C.byref(b)),
+               "element_bytes")
+        return a.value, b.value
 
     def __del__(self):
         if getattr(self, "_h", None):

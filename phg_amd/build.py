@@ -40,6 +40,7 @@ HIP_UNITS = {
     "reorder.hip": [],
     "dictcsr.hip": [],
     "block.hip": [],
+    "eop.hip": [],
 }
 
 

@@ -132,6 +132,20 @@ __device__ __constant__ int c_ev[6][2] = {{0, 1}, {0, 2}, {0, 3},
                                           {1, 2}, {1, 3}, {2, 3}};
 __device__ __constant__ int c_fv[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3},
                                           {0, 1, 2}};
+// the same tables for code that also runs on the host
+__host__ __device__ inline void edge_verts(int k, int &a, int &b)
+{
+    constexpr int t[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+    a = t[k][0];
+    b = t[k][1];
+}
+__host__ __device__ inline void face_verts(int k, int &a, int &b, int &c)
+{
+    constexpr int t[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
+    a = t[k][0];
+    b = t[k][1];
+    c = t[k][2];
+}
 
 // lagrange.c:80 / :160 / :392 (full-block evaluation)
 __device__ inline void bas_values(int p, const double *lam, double *v)
@@ -184,7 +198,7 @@ __device__ inline void bas_values(int p, const double *lam, double *v)
 }
 
 // lagrange.c:111 / :248 / :538: gradients with respect to lambda, [nb][4]
-__device__ inline void bas_grad_lambda(int p, const double *lam, double *g)
+__host__ __device__ inline void bas_grad_lambda(int p, const double *lam, double *g)
 {
     int nb = p == 1 ? 4 : (p == 2 ? 10 : 20);
     for (int i = 0; i < 4 * nb; i++)
@@ -198,7 +212,8 @@ __device__ inline void bas_grad_lambda(int p, const double *lam, double *g)
         for (int k = 0; k < 4; k++)
             g[4 * k + k] = 4. * lam[k] - 1.;
         for (int k = 0; k < 6; k++) {
-            int a = c_ev[k][0], b = c_ev[k][1];
+            int a, b;
+            edge_verts(k, a, b);
             g[4 * (4 + k) + a] = 4. * lam[b];
             g[4 * (4 + k) + b] = 4. * lam[a];
         }
@@ -207,7 +222,8 @@ __device__ inline void bas_grad_lambda(int p, const double *lam, double *g)
     for (int k = 0; k < 4; k++)
         g[4 * k + k] = (13.5 * lam[k] - 9.) * lam[k] + 1.;
     for (int k = 0; k < 6; k++) {
-        int a = c_ev[k][0], b = c_ev[k][1];
+        int a, b;
+        edge_verts(k, a, b);
         double *g0 = g + 4 * (4 + 2 * k), *g1 = g0 + 4;
         g0[a] = (27. * lam[a] - 4.5) * lam[b];
         g0[b] = 4.5 * lam[a] * (3. * lam[a] - 1.);
@@ -215,7 +231,8 @@ __device__ inline void bas_grad_lambda(int p, const double *lam, double *g)
         g1[b] = (27. * lam[b] - 4.5) * lam[a];
     }
     for (int k = 0; k < 4; k++) {
-        int a = c_fv[k][0], b = c_fv[k][1], c = c_fv[k][2];
+        int a, b, c;
+        face_verts(k, a, b, c);
         double *gg = g + 4 * (16 + k);
         gg[a] = 27. * lam[b] * lam[c];
         gg[b] = 27. * lam[a] * lam[c];
@@ -1343,7 +1360,87 @@ fail:
     return rc;
 }
 
+// Element coefficients of the stiffness factorisation used by the
+// matrix-free operator: K_e = sum_m c_m(e) T^m with, for the pairs
+// m = (a, b), a <= b, of barycentric coordinates,
+//   c_m(e) = kappa_e vol_e (grad lambda_a . grad lambda_b)
+// (grad lambda_a = row a of the Jacobian, geom.c:103).  Element i of the
+// output is mesh element eperm[i].
+__global__ void k_eop_coef(int64_t ne, const double *verts, const int32_t *ev,
+                           const double *kappa, const int32_t *eperm,
+                           float *coef)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ne)
+        return;
+    const int64_t e = eperm[i];
+    double x[4][3];
+    for (int v = 0; v < 4; v++) {
+        const int64_t g = ev[4 * e + v];
+        for (int c = 0; c < 3; c++)
+            x[v][c] = verts[3 * g + c];
+    }
+    double J[16];
+    jacobian(x, J);
+    const double sc = (kappa ? kappa[e] : 1.0) * volume(x);
+    int m = 0;
+    for (int a = 0; a < 4; a++)
+        for (int b = a; b < 4; b++, m++)
+            coef[10 * i + m] = (float)(sc * (J[4 * a] * J[4 * b] + J[4 * a + 1] * J[4 * b + 1] +
+                                             J[4 * a + 2] * J[4 * b + 2]));
+}
+
 }  // namespace
+
+namespace phg {
+
+// T^m_ij = sum_q w_q (d phi_i / d lambda_a)(d phi_j / d lambda_b) (+ the
+// (b, a) term for a < b) on the rule of order 2(p-1) (exact for the
+// integrand; quad.c rules, lagrange.c gradients), so that
+// sum_m c_m(e) T^m_ij = kappa_e vol_e sum_q w_q grad phi_i . grad phi_j,
+// the element stiffness of quad.c:5260 phgQuadGradBasAGradBas.
+// T is [10][nb][nb], pairs m in the order (0,0) (0,1) (0,2) (0,3) (1,1) ...
+void eop_reference_tensors(int p, double *T)
+{
+    const int nb = p == 1 ? 4 : (p == 2 ? 10 : 20);
+    Rule q;
+    make_rule(2 * (p - 1), q);
+    std::vector<double> S((size_t)16 * nb * nb, 0.0);   // S[a][b][i][j]
+    double g[4 * NB];
+    for (int k = 0; k < q.np; k++) {
+        bas_grad_lambda(p, q.pts[k], g);
+        for (int a = 0; a < 4; a++)
+            for (int b = 0; b < 4; b++)
+                for (int i = 0; i < nb; i++)
+                    for (int j = 0; j < nb; j++)
+                        S[(((size_t)(4 * a + b) * nb) + i) * nb + j] +=
+                            q.w[k] * g[4 * i + a] * g[4 * j + b];
+    }
+    int m = 0;
+    for (int a = 0; a < 4; a++)
+        for (int b = a; b < 4; b++, m++)
+            for (int i = 0; i < nb; i++)
+                for (int j = 0; j < nb; j++) {
+                    double t = S[(((size_t)(4 * a + b) * nb) + i) * nb + j];
+                    if (a != b)
+                        t += S[(((size_t)(4 * b + a) * nb) + i) * nb + j];
+                    T[((size_t)m * nb + i) * nb + j] = t;
+                }
+}
+
+int eop_coefficients(const phg_hip_space *sp, const double *d_kappa,
+                     const int32_t *d_eperm, float *d_coef, hipStream_t s)
+{
+    if (sp->nelem == 0)
+        return PHG_HIP_OK;
+    hipLaunchKernelGGL(k_eop_coef, dim3(nblk(sp->nelem, 256)), dim3(256), 0, s,
+                       sp->nelem, (const double *)sp->d_verts,
+                       (const int32_t *)sp->d_elem_verts, d_kappa, d_eperm, d_coef);
+    PHG_HIP_CHECK(hipGetLastError());
+    return PHG_HIP_OK;
+}
+
+}  // namespace phg
 
 //============================================================== C ABI ====
 

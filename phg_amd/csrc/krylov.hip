@@ -772,6 +772,72 @@ __device__ inline void spmv_apply(const DictViewT<PF> &m,
     dict_spmv<PF, T>(m, va, x, epi);
 }
 
+// Per-row sums of the matrix-free operator (eop.hip): a Dirichlet row of
+// the solver numbering is diag * p, any other row the sum of the element
+// results y[code] of its incidence list, in element order.  Blocks of at
+// most EOP_GATHER_NNZ incidences and EOP_GATHER_ROWS rows, swept per XCD;
+// the incidences of a block are gathered into LDS with coalesced code
+// loads, then one work-item sums its row.  A row with more incidences than
+// the LDS stage is a block of its own and is summed from global memory.
+struct EopGather {
+    const int32_t *off, *inc, *blk;
+    const uint8_t *bdry;
+    const float *y;
+    int64_t nblk;
+};
+
+template <typename Epi>
+__device__ inline void eop_gather(const EopGather &m, const float *__restrict__ diag,
+                                  const float *__restrict__ p, Epi epi)
+{
+    static_assert(phg::EOP_GATHER_ROWS <= BS, "one work-item per row");
+    __shared__ float prod[phg::EOP_GATHER_NNZ];
+    int64_t first, end, step;
+    xcd_sweep(m.nblk, first, end, step);
+    const int tid = threadIdx.x;
+    for (int64_t b = first; b < end; b += step) {
+        const int32_t r0 = m.blk[b], r1 = m.blk[b + 1];
+        const int32_t f0 = m.off[r0], f1 = m.off[r1];
+        const bool fits = f1 - f0 <= phg::EOP_GATHER_NNZ;
+        lds_barrier();
+        if (fits)
+            for (int32_t e = f0 + tid; e < f1; e += BS)
+                prod[e - f0] = m.y[m.inc[e]];
+        lds_barrier();
+        if (tid < r1 - r0) {
+            const int32_t r = r0 + tid;
+            float sum;
+            if (m.bdry[r])
+                sum = diag[r] * p[r];
+            else if (fits)
+                sum = ordered_sum(prod, m.off[r] - f0, m.off[r + 1] - f0);
+            else {
+                sum = 0.0f;
+                for (int32_t e = m.off[r]; e < m.off[r + 1]; e++)
+                    sum += m.y[m.inc[e]];
+            }
+            epi(r, sum, p[r]);
+        }
+    }
+}
+
+// Ap = A p through the element results, partials of (p, Ap); with a state,
+// only in reliable-update ITER slots
+__global__ void __launch_bounds__(BS) k_eop_gather(EopGather m, const float *diag,
+                                                   const float *p, float *Ap,
+                                                   double *part_out,
+                                                   const State *st)
+{
+    if (st != nullptr && (st->done || st->mode != RU_ITER))
+        return;
+    double v[1] = {0.0};
+    eop_gather(m, diag, p, [&](int32_t row, float s, float xr) {
+        Ap[row] = s;
+        v[0] += (double)xr * (double)s;
+    });
+    write_partials<1>(v, part_out);
+}
+
 // SELL construction: entries of permuted row i (original row perm[i]) with
 // column indices renumbered through iperm; padding repeats the row's own
 // index with value 0
@@ -820,6 +886,23 @@ __global__ void k_scatter(int64_t n, const int32_t *perm, const double *src,
     for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * BS)
         dst[perm[i]] = src[i];
+}
+
+// FP64 vector in PHG numbering <-> FP32 vector in the solver numbering
+__global__ void k_gather_f32(int64_t n, const int32_t *perm, const double *src,
+                             float *dst)
+{
+    for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * BS)
+        dst[i] = (float)src[perm ? perm[i] : i];
+}
+
+__global__ void k_scatter_f32(int64_t n, const int32_t *perm, const float *src,
+                              double *dst)
+{
+    for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * BS)
+        dst[perm ? perm[i] : i] = (double)src[i];
 }
 
 //------------------------------------------------------------- plain SpMV
@@ -1976,10 +2059,30 @@ int mixed_impl(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x,
 // (gather into the solver numbering, r = b - A x), one slot of launches
 // (k_ru_p, k_ru_spmv, k_ru_update, k_ru_resid) and the end (fold x32 into x,
 // scatter back).  Slots write the ping-pong state st[(k + 1) & 1].
+EopGather eop_view(const EopData *eo)
+{
+    return EopGather{eo->inc_off, eo->inc, eo->gblk, eo->bdry, eo->y, eo->ngblk};
+}
+
+// gather grid, fixed when first asked (outside any graph capture: the
+// solver prologues ask before capturing)
+int eop_grid(EopData *eo)
+{
+    if (eo->grid_g == 0)
+        eo->grid_g = (int)std::max<int64_t>(
+            1, std::min<int64_t>(eo->ngblk, resident_groups((const void *)k_eop_gather)));
+    return eo->grid_g;
+}
+
 template <typename View>
 struct RuPlan {
     View mv;
     int g = 0, gs = 0;
+    // FP32 SpMV through the matrix-free operator when eo is set; gs32 is the
+    // number of partials the FP32 SpMV writes
+    const EopData *eo = nullptr;
+    EopGather eg{};
+    int gs32 = 0;
     const double *bb = nullptr;
     double *xx = nullptr;
     const int32_t *perm = nullptr;
@@ -2000,6 +2103,12 @@ int ru_prologue(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x,
     pl->g = grid_for(n);
     pl->gs = grid_spmv<View>(A, k_ru_spmv<View>);
     pl->mv = view_of(A, (View *)nullptr);
+    pl->gs32 = pl->gs;
+    if (A->eop != nullptr && A->eop->on) {
+        pl->eo = A->eop;
+        pl->eg = eop_view(A->eop);
+        pl->gs32 = eop_grid(A->eop);
+    }
     State h;
     memset(&h, 0, sizeof(h));
     h.rtol = rtol;
@@ -2049,14 +2158,21 @@ void ru_slot(const phg_hip_mat *A, const RuPlan<View> &pl, int64_t k,
                        ws->x32, (const double *)pres, g, gs,
                        (const State *)sin, sout);
     if (prof) { prof_mark(KP_P32, s); prof_mark(KP_SPMV32, s); }
-    hipLaunchKernelGGL((k_ru_spmv<View>), dim3(gs), dim3(BS), 0, s, pl.mv,
-                       F::v32(A), (const float *)ws->p32, ws->Ap32, ps,
-                       (const State *)sout);
+    if (pl.eo != nullptr) {
+        phg::eop_elements(pl.eo, ws->p32, &sout->done, &sout->mode, RU_ITER, s);
+        hipLaunchKernelGGL(k_eop_gather, dim3(pl.gs32), dim3(BS), 0, s, pl.eg,
+                           F::d32(A), (const float *)ws->p32, ws->Ap32, ps,
+                           (const State *)sout);
+    }
+    else
+        hipLaunchKernelGGL((k_ru_spmv<View>), dim3(gs), dim3(BS), 0, s, pl.mv,
+                           F::v32(A), (const float *)ws->p32, ws->Ap32, ps,
+                           (const State *)sout);
     if (prof) { prof_mark(KP_SPMV32, s); prof_mark(KP_UPD32, s); }
     hipLaunchKernelGGL(k_ru_update, dim3(g), dim3(BS), 0, s, n,
                        ws->x32, (const float *)ws->p32, ws->r32,
                        (const float *)ws->Ap32, F::d32(A),
-                       (const double *)ps, gs, pres, sout);
+                       (const double *)ps, pl.gs32, pres, sout);
     if (prof) { prof_mark(KP_UPD32, s); prof_mark(KP_SPMV64, s); }
     hipLaunchKernelGGL((k_ru_resid<View>), dim3(gs), dim3(BS), 0, s,
                        pl.mv, F::v64(A), F::d64(A), pl.bb,
@@ -2716,6 +2832,55 @@ extern "C" int phg_hip_mat_get_format(const phg_hip_mat *A, int32_t *fmt64,
         *fmt64 = A->fmt64;
     if (fmt32)
         *fmt32 = A->fmt32;
+    return PHG_HIP_OK;
+}
+
+extern "C" int phg_hip_element_apply(const phg_hip_mat *A_, const phg_hip_vec *x,
+                                     phg_hip_vec *y)
+{
+    int rc;
+    if ((rc = phg::require_ctx()))
+        return rc;
+    phg_hip_mat *A = const_cast<phg_hip_mat *>(A_);
+    if (A == nullptr || A->eop == nullptr || x == nullptr || y == nullptr ||
+        x->n != A->n || y->n != A->n || x->d == y->d)
+        return phg::set_error(PHG_HIP_E_ARG,
+                              "phg_hip_element_apply: operator not built, size "
+                              "mismatch or x aliases y");
+    hipStream_t s = phg::ctx()->stream;
+    Workspace *ws = get_ws(A, 1);
+    if (ws == nullptr)
+        return phg::set_error(PHG_HIP_E_NOMEM, "workspace allocation failed");
+    const int64_t n = A->n;
+    const int g = grid_for(n);
+    hipLaunchKernelGGL(k_gather_f32, dim3(g), dim3(BS), 0, s, n,
+                       (const int32_t *)A->q_perm, (const double *)x->d, ws->p32);
+    phg::eop_elements(A->eop, ws->p32, nullptr, nullptr, 0, s);
+    hipLaunchKernelGGL(k_eop_gather, dim3(eop_grid(A->eop)), dim3(BS), 0, s,
+                       eop_view(A->eop), (const float *)A->q_diag32,
+                       (const float *)ws->p32, ws->Ap32, ws->seg(SEG_S),
+                       (const State *)nullptr);
+    hipLaunchKernelGGL(k_scatter_f32, dim3(g), dim3(BS), 0, s, n,
+                       (const int32_t *)A->q_perm, (const float *)ws->Ap32, y->d);
+    PHG_HIP_CHECK(hipGetLastError());
+    PHG_HIP_CHECK(hipStreamSynchronize(s));
+    return PHG_HIP_OK;
+}
+
+extern "C" int phg_hip_element_bytes(const phg_hip_mat *A, int64_t *elem_bytes,
+                                     int64_t *gather_bytes)
+{
+    if (A == nullptr || A->eop == nullptr || elem_bytes == nullptr ||
+        gather_bytes == nullptr)
+        return phg::set_error(PHG_HIP_E_ARG, "phg_hip_element_bytes: operator not built");
+    const EopData *eo = A->eop;
+    // element kernel: map + coefficients read, element results written, x
+    // read once
+    *elem_bytes = eo->nelem * ((int64_t)eo->nb * 4 + 40 + (int64_t)eo->nb * 4) + eo->n * 4;
+    // gather: codes and element results read once, row offsets, flags, p
+    // read, Ap written, row blocks
+    *gather_bytes = eo->ninc * 4 + eo->nelem * (int64_t)eo->nb * 4 +
+                    (eo->n + 1) * 4 + eo->n * (1 + 4 + 4) + (eo->ngblk + 1) * 4;
     return PHG_HIP_OK;
 }
 

@@ -83,6 +83,30 @@ struct DictLayout {
     double *val64 = nullptr;
 };
 
+// Matrix-free FP32 operator of a scalar stiffness matrix (eop.hip), in the
+// solver numbering: y = sum_e P_e^T K_e P_e x with
+// K_e = sum_m coef[e][m] T^m (10 pairs m of barycentric coordinates, T^m the
+// reference tensors), the element products on bf16 MFMA with FP32 operands
+// split into three bf16 parts, then an ordered per-row sum over the row's
+// incident (element, local index) pairs.  Elements are in Morton order of
+// their centroids; Dirichlet DOFs have map -1 (their columns are the ones
+// elimination removes) and their rows are diag * x.
+struct EopData {
+    int p = 0, nb = 0, ks = 0;        // order, basis size, K steps of 32
+    int on = 0;                       // used by the FP32 solver slots
+    int64_t nelem = 0, n = 0, ninc = 0;
+    int32_t *map = nullptr;           // [nelem][nb] solver index or -1
+    float *coef = nullptr;            // [nelem][10]
+    uint16_t *tiles = nullptr;        // [nb][ks][64 lanes][8] bf16 A fragments
+    int32_t *inc_off = nullptr;       // [n+1]
+    int32_t *inc = nullptr;           // [ninc] codes e * nb + a, by element
+    int32_t *gblk = nullptr;          // row blocks of the gather
+    int64_t ngblk = 0;
+    uint8_t *bdry = nullptr;          // [n] Dirichlet rows
+    float *y = nullptr;               // [nelem][nb] element results
+    int grid_e = 0, grid_g = 0;       // launch grids (element, gather)
+};
+
 struct phg_hip_mat {
     int64_t n = 0, nnz = 0;
     int dim = 1;
@@ -150,6 +174,8 @@ struct phg_hip_mat {
     // SpMM wavefront shape index per precision (block.hip) and tuner times
     int spmm_shape[2] = {0, 0};
     double spmm_us[8][2] = {};
+    // matrix-free FP32 operator (eop.hip), null unless built
+    EopData *eop = nullptr;
 };
 
 namespace phg {
@@ -186,4 +212,15 @@ int build_xstage(phg_hip_mat *A);   // A->xs
 void free_dict_layout(DictLayout *L);
 void free_dict(phg_hip_mat *A);     // both
 int setup_diagonal(phg_hip_mat *A);
+// matrix-free operator (eop.hip; reference data from assemble.hip)
+constexpr int EOP_GATHER_NNZ = 2048, EOP_GATHER_ROWS = 256;
+void eop_reference_tensors(int p, double *T);
+int eop_coefficients(const phg_hip_space *sp, const double *d_kappa,
+                     const int32_t *d_eperm, float *d_coef, hipStream_t s);
+// launches the element products y_e = K_e x_e of every element into eo->y
+// (the caller checks the launch); the kernel returns at once when done is
+// set and (*done != 0 or *mode != want)
+void eop_elements(const EopData *eo, const float *x, const int32_t *done,
+                  const int32_t *mode, int32_t want, hipStream_t s);
+void free_eop(phg_hip_mat *A);
 }  // namespace phg

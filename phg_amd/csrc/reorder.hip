@@ -108,6 +108,7 @@ void free_solver_copy(phg_hip_mat *A)
 {
     free_dict(A);
     free_bsr(A);
+    free_eop(A);       // built in this numbering
     if (A->q_owns) {
         dfree(A->q_rowptr);
         dfree(A->q_cols);

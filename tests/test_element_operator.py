@@ -1,0 +1,137 @@
+"""Matrix-free FP32 operator (phg_hip_mat_element_operator): MFMA element
+products with three-part bf16 operands against the assembled FP64 matrix,
+and the reliable-update / batched solvers running their FP32 iterations on
+it.  The FP64 side (assembled CSR, oracle) is the reference."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from phg_amd.multimesh import assemble_batch, batch_meshes
+from tests._cases import elasticity_case, poisson_case
+
+pytestmark = pytest.mark.gpu
+
+
+def _system(pa, kind, p, n):
+    c = poisson_case(n, p, kind)
+    pb = pa.Problem.poisson(use_kappa=(kind == "kappa"))
+    sp = pa.Space(c["grid"], p, bdry=c["bdry"])
+    A = pa.assemble(sp, pb, kappa=c["kappa"], f=c["f"], g=c["g"])
+    A.eliminate()
+    return c, pb, sp, A
+
+
+# (kind, order, mesh n): kappa and plain Laplacian, P1-P3
+APPLY_CASES = [("kappa", 1, 6), ("sin", 1, 5), ("kappa", 2, 4), ("sin", 2, 3),
+               ("kappa", 3, 3), ("sin", 3, 2)]
+
+
+@pytest.mark.parametrize("kind,p,n", APPLY_CASES)
+def test_apply_matches_assembled_matrix(hip, kind, p, n):
+    """FP32 accuracy against the FP64 SpMV of the eliminated matrix.  The
+    bound 2e-6 is what FP32 products and sums reach here; keeping only two
+    bf16 parts (16 bits) of the operands gives errors near 1e-5, a wrong
+    fragment layout or coefficient order errors of order 1."""
+    pa = hip
+    c, pb, sp, A = _system(pa, kind, p, n)
+    A.element_operator(sp, pb, kappa=c["kappa"])
+    rng = np.random.default_rng(10 + p)
+    for trial in range(2):
+        xv = rng.standard_normal(A.n)
+        x, y64, y32 = pa.Vec(A.n, xv), pa.Vec(A.n), pa.Vec(A.n)
+        A.matvec(x, y64)
+        A.element_apply(x, y32)
+        a, b = y64.numpy(), y32.numpy()
+        rel = np.linalg.norm(b - a) / np.linalg.norm(a)
+        assert rel < 2e-6, (trial, rel)
+        assert np.max(np.abs(b - a)) < 2e-5 * np.max(np.abs(a))
+
+
+def test_apply_dirichlet_rows_are_diagonal(hip):
+    """Rows of Dirichlet DOFs are diag * x; a vector supported on Dirichlet
+    DOFs only touches no interior row (elimination removed those columns)."""
+    pa = hip
+    c, pb, sp, A = _system(pa, "kappa", 2, 3)
+    A.element_operator(sp, pb, kappa=c["kappa"])
+    bd = c["bdry"].astype(bool)
+    xv = np.where(bd, 1.0 + np.arange(A.n) % 7, 0.0)
+    x, y64, y32 = pa.Vec(A.n, xv), pa.Vec(A.n), pa.Vec(A.n)
+    A.matvec(x, y64)
+    A.element_apply(x, y32)
+    b = y32.numpy()
+    assert np.all(b[~bd] == 0.0)
+    np.testing.assert_allclose(b[bd], y64.numpy()[bd], rtol=1e-7)
+
+
+@pytest.mark.parametrize("p,n", [(2, 4), (3, 3)])
+def test_reliable_solver_on_element_operator(hip, p, n):
+    pa = hip
+    c, pb, sp, A = _system(pa, "kappa", p, n)
+    Ao = orc.assemble(orc.problem(p, use_kappa=True), c["verts"], c["elem_verts"], c["map"],
+                      c["ndof"], c["bdry"], c["g"], c["f"], c["kappa"])
+    orc.eliminate(Ao, c["bdry"])
+    xo, nits_o, _ = orc.pcg(Ao, Ao.rhs, rtol=1e-13, btol=1e-13, maxit=100000)
+    x_csr = pa.Vec(A.n)
+    n_csr, _, _ = pa.pcg_reliable(A, A.rhs(), x_csr, rtol=1e-10, btol=1e-10)
+    A.element_operator(sp, pb, kappa=c["kappa"])
+    xs = []
+    for _ in range(2):
+        x = pa.Vec(A.n)
+        nits, upd, res = pa.pcg_reliable(A, A.rhs(), x, rtol=1e-10, btol=1e-10)
+        xs.append(x.numpy())
+        # FP64 residual tolerance reached; FP32 iteration count as on the CSR
+        assert upd >= 1
+        assert abs(nits - n_csr) <= max(3, n_csr // 10)
+        assert np.linalg.norm(xs[-1] - xo) / np.linalg.norm(xo) < 1e-8
+    assert np.array_equal(xs[0], xs[1])           # deterministic
+    A.use_element_operator(False)
+    x2 = pa.Vec(A.n)
+    n2, _, _ = pa.pcg_reliable(A, A.rhs(), x2, rtol=1e-10, btol=1e-10)
+    assert n2 == n_csr and np.array_equal(x2.numpy(), x_csr.numpy())
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_multi_on_element_operator_matches_single(hip, graph):
+    pa = hip
+    meshes = batch_meshes(nmesh=3, n=3, seed=3, lmin=1, lmax=3)
+    p = 3
+    sys_ = assemble_batch(meshes, p=p, seed=4)
+    rng = np.random.default_rng(4)
+    pb = pa.Problem.poisson(use_kappa=True)
+    As, bs = [], []
+    for (g, _, _), (sp, A, b) in zip(meshes, sys_):
+        # the batch's kappa is internal to assemble_batch: rebuild with a
+        # known one
+        kappa = np.exp(rng.normal(0.0, 0.5, size=g.nelem))
+        nd = g.ndof(p)
+        A2 = pa.assemble(sp, pb, kappa=kappa, f=np.ones(nd), g=np.zeros(nd))
+        A2.eliminate()
+        A2.element_operator(sp, pb, kappa=kappa)
+        As.append(A2)
+        bs.append(A2.rhs())
+    xs = [pa.Vec(A.n) for A in As]
+    nits, res = pa.pcg_multi(As, bs, xs, rtol=1e-10, btol=1e-10, use_graph=graph)
+    for i, (A, b, x) in enumerate(zip(As, bs, xs)):
+        x1 = pa.Vec(A.n)
+        n1, _, r1 = pa.pcg_reliable(A, b, x1, rtol=1e-10, btol=1e-10)
+        assert nits[i] == n1 and res[i] == r1
+        assert np.array_equal(x.numpy(), x1.numpy())
+
+
+def test_element_operator_rejects_other_problems(hip):
+    pa = hip
+    c = elasticity_case(2)
+    sp = pa.Space(c["grid"], c["p"], bdry=c["bdry"])
+    pb = pa.Problem.elasticity()
+    A = pa.assemble(sp, pb, f=c["f"], g=c["g"])
+    A.eliminate()
+    with pytest.raises(pa.PhgHipError):
+        A.element_operator(sp, pb)
+    c, pb, sp, A = _system(pa, "sin", 2, 2)
+    mass = pa.Problem.poisson(use_kappa=False, mass_coef=1.0)
+    with pytest.raises(pa.PhgHipError):
+        A.element_operator(sp, mass)
+    with pytest.raises(pa.PhgHipError):
+        A.use_element_operator(True)    # not built on this matrix
