@@ -199,8 +199,8 @@ int phg_hip_pcg_multi(int32_t nmat, phg_hip_mat *const *A,
  * phg_hip_pcg_reliable / phg_hip_pcg_multi (no reference counterpart).
  * For the eliminated matrix A of a scalar stiffness problem (pb->dim 1, no
  * mass term; P1-P3 on the space sp, kappa as given to phg_hip_assemble) it
- * stores per element the ten coefficients kappa_e vol_e grad lambda_a .
- * grad lambda_b, and applies K_e = sum_m c_m T^m (the stiffness of
+ * stores per element the six coefficients kappa_e vol_e grad lambda_a .
+ * grad lambda_b (a <= b in 1..3), and applies K_e = sum_m c_m T^m (the stiffness of
  * quad.c:5260 phgQuadGradBasAGradBas, factored through fixed reference
  * tensors T^m) to x on the matrix cores (bf16 MFMA, operands split into
  * three bf16 parts so products keep FP32 precision), then sums each row

@@ -1362,10 +1362,13 @@ fail:
 
 // Element coefficients of the stiffness factorisation used by the
 // matrix-free operator: K_e = sum_m c_m(e) T^m with, for the pairs
-// m = (a, b), a <= b, of barycentric coordinates,
+// m = (a, b), 1 <= a <= b <= 3, of independent barycentric coordinates,
 //   c_m(e) = kappa_e vol_e (grad lambda_a . grad lambda_b)
-// (grad lambda_a = row a of the Jacobian, geom.c:103).  Element i of the
-// output is mesh element eperm[i].
+// (grad lambda_a = row a of the Jacobian, geom.c:103; grad lambda_0 =
+// -(grad lambda_1 + grad lambda_2 + grad lambda_3) is eliminated, which
+// keeps the 3 x 3 metric nonsingular and the sum free of the cancellation
+// a 4 x 4 Gram matrix of rank 3 would add).  Element i of the output is
+// mesh element eperm[i].
 __global__ void k_eop_coef(int64_t ne, const double *verts, const int32_t *ev,
                            const double *kappa, const int32_t *eperm,
                            float *coef)
@@ -1384,46 +1387,49 @@ __global__ void k_eop_coef(int64_t ne, const double *verts, const int32_t *ev,
     jacobian(x, J);
     const double sc = (kappa ? kappa[e] : 1.0) * volume(x);
     int m = 0;
-    for (int a = 0; a < 4; a++)
+    for (int a = 1; a < 4; a++)
         for (int b = a; b < 4; b++, m++)
-            coef[10 * i + m] = (float)(sc * (J[4 * a] * J[4 * b] + J[4 * a + 1] * J[4 * b + 1] +
-                                             J[4 * a + 2] * J[4 * b + 2]));
+            coef[6 * i + m] = (float)(sc * (J[4 * a] * J[4 * b] + J[4 * a + 1] * J[4 * b + 1] +
+                                            J[4 * a + 2] * J[4 * b + 2]));
 }
 
 }  // namespace
 
 namespace phg {
 
-// T^m_ij = sum_q w_q (d phi_i / d lambda_a)(d phi_j / d lambda_b) (+ the
-// (b, a) term for a < b) on the rule of order 2(p-1) (exact for the
-// integrand; quad.c rules, lagrange.c gradients), so that
-// sum_m c_m(e) T^m_ij = kappa_e vol_e sum_q w_q grad phi_i . grad phi_j,
+// T^m_ij = sum_q w_q D_a phi_i D_b phi_j (+ the (b, a) term for a < b),
+// D_a = d / d lambda_a - d / d lambda_0 (a = 1..3), on the rule of order
+// 2(p-1) (exact for the integrand; quad.c rules, lagrange.c gradients), so
+// that sum_m c_m(e) T^m_ij = kappa_e vol_e sum_q w_q grad phi_i . grad phi_j,
 // the element stiffness of quad.c:5260 phgQuadGradBasAGradBas.
-// T is [10][nb][nb], pairs m in the order (0,0) (0,1) (0,2) (0,3) (1,1) ...
+// T is [6][nb][nb], pairs m in the order (1,1) (1,2) (1,3) (2,2) (2,3) (3,3).
 void eop_reference_tensors(int p, double *T)
 {
     const int nb = p == 1 ? 4 : (p == 2 ? 10 : 20);
     Rule q;
     make_rule(2 * (p - 1), q);
-    std::vector<double> S((size_t)16 * nb * nb, 0.0);   // S[a][b][i][j]
-    double g[4 * NB];
+    std::vector<double> S((size_t)9 * nb * nb, 0.0);   // S[a-1][b-1][i][j]
+    double g[4 * NB], d[3 * NB];
     for (int k = 0; k < q.np; k++) {
         bas_grad_lambda(p, q.pts[k], g);
-        for (int a = 0; a < 4; a++)
-            for (int b = 0; b < 4; b++)
+        for (int i = 0; i < nb; i++)
+            for (int a = 1; a < 4; a++)
+                d[3 * i + a - 1] = g[4 * i + a] - g[4 * i];
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++)
                 for (int i = 0; i < nb; i++)
                     for (int j = 0; j < nb; j++)
-                        S[(((size_t)(4 * a + b) * nb) + i) * nb + j] +=
-                            q.w[k] * g[4 * i + a] * g[4 * j + b];
+                        S[(((size_t)(3 * a + b) * nb) + i) * nb + j] +=
+                            q.w[k] * d[3 * i + a] * d[3 * j + b];
     }
     int m = 0;
-    for (int a = 0; a < 4; a++)
-        for (int b = a; b < 4; b++, m++)
+    for (int a = 0; a < 3; a++)
+        for (int b = a; b < 3; b++, m++)
             for (int i = 0; i < nb; i++)
                 for (int j = 0; j < nb; j++) {
-                    double t = S[(((size_t)(4 * a + b) * nb) + i) * nb + j];
+                    double t = S[(((size_t)(3 * a + b) * nb) + i) * nb + j];
                     if (a != b)
-                        t += S[(((size_t)(4 * b + a) * nb) + i) * nb + j];
+                        t += S[(((size_t)(3 * b + a) * nb) + i) * nb + j];
                     T[((size_t)m * nb + i) * nb + j] = t;
                 }
 }

@@ -3,27 +3,28 @@
 //
 // For an affine tetrahedron the stiffness of quad.c:5260
 // (phgQuadGradBasAGradBas with a P0 coefficient) factors as
-//     K_e = sum_{m=0..9} c_m(e) T^m,
+//     K_e = sum_{m=0..5} c_m(e) T^m,
 //     c_m(e) = kappa_e vol_e (grad lambda_a . grad lambda_b),  m = (a <= b),
-// with reference tensors T^m fixed per order (assemble.hip computes both).
-// Applying the operator to x is then, per element,
+// over the independent barycentric coordinates a, b = 1..3, with reference
+// tensors T^m fixed per order (assemble.hip computes both).  Applying the
+// operator to x is then, per element,
 //     y_e[i] = sum_m c_m(e) (T^m x_e)[i],
-// and T^m x_e for 16 elements at a time is a matrix product: rows (i, m)
-// of the stacked T against the 16 gathered element vectors.  It runs as
+// and T^m x_e for 16 elements at a time is a matrix product: rows (i, m) of
+// the stacked T against the 16 gathered element vectors.  It runs as
 // v_mfma_f32_16x16x32_bf16 with FP32 precision kept by splitting both
 // operands into three bf16 parts (t = t0 + t1 + t2, x = x0 + x1 + x2, exact
 // for FP32 x) and accumulating the six products whose order is at most
 // 2^-16 relative (t0x0, t0x1, t1x0, t0x2, t1x1, t2x0) in FP32; the dropped
-// terms are below 2^-24.  One output tile per local index i holds the 16
-// slots m (10 used) of 16 elements; the A fragments (T parts) of all tiles
-// stay in LDS for the kernel, the B fragments (x parts) are written to LDS
-// per 16 elements.
+// terms are below 2^-24.  One 16-row output tile holds the slots m (8 rows,
+// 6 used) of two local indices i = 2t, 2t + 1 for 16 elements; the A
+// fragments (T parts) of all tiles stay in LDS for the kernel, the B
+// fragments (x parts) are written to LDS per 16 elements.
 //
 // The per-row sums y_i = sum over the row's incident (element, local index)
 // pairs, in element order, run in krylov.hip (k_eop_gather) with the
 // solver's epilogue.  Against the assembled FP32 CSR the operator moves
-// map + coefficients + element results (P3: 80 + 40 + 2 x 80 B per element
-// plus 4 B per incidence) instead of about 2 KB of matrix entries per
+// map + coefficients + element results (P3: 80 + 24 + 2 x 80 B per element
+// plus 4 B per incidence) instead of about 1.4 KB of matrix entries per
 // element.
 #include "phg_hip_internal.h"
 
@@ -38,18 +39,21 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int EB = 1024;          // work-items per work-group (16 wavefronts)
+constexpr int EB = 512;           // work-items per work-group (8 wavefronts)
 constexpr int EW = EB / 64;
+constexpr int NC = 6;             // coefficients per element
 
 template <int P> struct EopShape {
     static constexpr int NB = P == 1 ? 4 : (P == 2 ? 10 : 20);
+    static constexpr int NT = NB / 2;                 // output tiles
     static constexpr int KS = (6 * NB + 31) / 32;     // K steps of 32
     static constexpr int KP = KS * 32 + 8;            // B row stride (bank skew)
 };
 
-// A fragment of (tile i, K step s) for lane l: row m = l % 16, K indices
-// 32 s + 8 (l / 16) + u, u = 0..7; K index k = p NB + j is product p of the
-// split, column j of T^m (row i).
+// A fragment of (tile t, K step s) for lane l: row R = l % 16 is slot
+// m = R % 8 of local index i = 2 t + R / 8; K indices 32 s + 8 (l / 16) + u,
+// u = 0..7; K index k = q NB + j is product q of the split, column j of
+// T^m (row i).
 constexpr int kTPart[6] = {0, 0, 1, 0, 1, 2};
 constexpr int kXPart[6] = {0, 1, 0, 2, 1, 0};
 
@@ -61,14 +65,14 @@ k_eop_elem(int64_t nelem, const int32_t *__restrict__ map,
            const int32_t *done, const int32_t *mode, int32_t want)
 {
     using S = EopShape<P>;
-    constexpr int NB = S::NB, KS = S::KS, KP = S::KP;
+    constexpr int NB = S::NB, NT = S::NT, KS = S::KS, KP = S::KP;
     if (done != nullptr && (*done != 0 || *mode != want))
         return;
-    __shared__ uint4 at[NB * KS * 64];
+    __shared__ uint4 at[NT * KS * 64];
     __shared__ __align__(16) __bf16 bt[EW][16][KP];
     const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
     const int n = l & 15, g = l >> 4;
-    for (int t = tid; t < NB * KS * 64; t += EB)
+    for (int t = tid; t < NT * KS * 64; t += EB)
         at[t] = tiles[t];
     // K padding of the B rows stays zero
     constexpr int PADW = KP - 6 * NB;
@@ -107,38 +111,36 @@ k_eop_elem(int64_t nelem, const int32_t *__restrict__ map,
         }
         __syncthreads();
         if (tile < ntile) {
+            // D rows 4 g + r of this lane are slots m = 4 (g % 2) + r
             float c[4];
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int m = 4 * g + r;
-                c[r] = (ok && m < 10) ? coef[e * 10 + m] : 0.0f;
+                const int m = 4 * (g & 1) + r;
+                c[r] = (ok && m < NC) ? coef[e * NC + m] : 0.0f;
             }
-            float yv[(NB + 3) / 4];
+            float yv[(NT + 1) / 2];
 #pragma unroll
-            for (int i = 0; i < NB; i++) {
+            for (int t = 0; t < NT; t++) {
                 f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
                 for (int s = 0; s < KS; s++) {
-                    const uint4 av = at[(i * KS + s) * 64 + l];
+                    const uint4 av = at[(t * KS + s) * 64 + l];
                     const bf16x8 a = __builtin_bit_cast(bf16x8, av);
                     const bf16x8 b = *(const bf16x8 *)&bt[w][n][32 * s + 8 * g];
                     acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
                 }
-                // rows m = 4 g + r of column n: contract with c_m, then add
-                // the four row groups of the element
+                // contract with c_m, then add the two row groups of local
+                // index 2 t + g / 2 (lanes g and g ^ 1)
                 float v = c[0] * acc[0] + c[1] * acc[1] + c[2] * acc[2] + c[3] * acc[3];
                 v += __shfl_xor(v, 16, 64);
-                v += __shfl_xor(v, 32, 64);
-                if ((i & 3) == g)
-                    yv[i >> 2] = v;
+                if ((t & 1) == (g & 1))
+                    yv[t >> 1] = v;
             }
             if (ok) {
 #pragma unroll
-                for (int qd = 0; qd < (NB + 3) / 4; qd++) {
-                    const int i = 4 * qd + g;
-                    if (i < NB)
-                        y[e * NB + i] = yv[qd];
-                }
+                for (int t = 0; t < NT; t++)
+                    if ((t & 1) == (g & 1))
+                        y[e * NB + 2 * t + (g >> 1)] = yv[t >> 1];
             }
         }
     }
@@ -160,13 +162,13 @@ double bf16_value(uint16_t h)
     return (double)f;
 }
 
-// A fragments [nb][ks][64][8] of the stacked split T (see above)
+// A fragments [nb / 2][ks][64][8] of the stacked split T (see above)
 void build_tiles(int p, int nb, int ks, std::vector<uint16_t> &out)
 {
-    std::vector<double> T((size_t)10 * nb * nb);
+    std::vector<double> T((size_t)NC * nb * nb);
     phg::eop_reference_tensors(p, T.data());
     // three-part split of every T entry
-    std::vector<uint16_t> part((size_t)3 * 10 * nb * nb);
+    std::vector<uint16_t> part((size_t)3 * T.size());
     for (size_t k = 0; k < T.size(); k++) {
         double r = T[k];
         for (int q = 0; q < 3; q++) {
@@ -175,18 +177,20 @@ void build_tiles(int p, int nb, int ks, std::vector<uint16_t> &out)
             r -= bf16_value(h);
         }
     }
-    out.assign((size_t)nb * ks * 64 * 8, 0);
-    for (int i = 0; i < nb; i++)
+    const int nt = nb / 2;
+    out.assign((size_t)nt * ks * 64 * 8, 0);
+    for (int t = 0; t < nt; t++)
         for (int s = 0; s < ks; s++)
             for (int l = 0; l < 64; l++)
                 for (int u = 0; u < 8; u++) {
-                    const int m = l % 16, k = 32 * s + 8 * (l / 16) + u;
+                    const int R = l % 16, m = R % 8, i = 2 * t + R / 8;
+                    const int k = 32 * s + 8 * (l / 16) + u;
                     const int q = k / nb, j = k % nb;
                     uint16_t v = 0;
-                    if (m < 10 && q < 6)
+                    if (m < NC && q < 6)
                         v = part[(size_t)kTPart[q] * T.size() +
                                  ((size_t)m * nb + i) * nb + j];
-                    out[(((size_t)i * ks + s) * 64 + l) * 8 + u] = v;
+                    out[(((size_t)t * ks + s) * 64 + l) * 8 + u] = v;
                 }
 }
 
@@ -381,7 +385,7 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
         return code;
     };
     if ((rc = dmalloc((void **)&eo->map, emap.size() * 4)) ||
-        (rc = dmalloc((void **)&eo->coef, (size_t)ne * 10 * 4)) ||
+        (rc = dmalloc((void **)&eo->coef, (size_t)ne * NC * 4)) ||
         (rc = dmalloc((void **)&eo->tiles, tiles.size() * 2)) ||
         (rc = dmalloc((void **)&eo->inc_off, off.size() * 4)) ||
         (rc = dmalloc((void **)&eo->inc, inc.size() * 4)) ||

@@ -2876,7 +2876,7 @@ extern "C" int phg_hip_element_bytes(const phg_hip_mat *A, int64_t *elem_bytes,
     const EopData *eo = A->eop;
     // element kernel: map + coefficients read, element results written, x
     // read once
-    *elem_bytes = eo->nelem * ((int64_t)eo->nb * 4 + 40 + (int64_t)eo->nb * 4) + eo->n * 4;
+    *elem_bytes = eo->nelem * ((int64_t)eo->nb * 4 + 24 + (int64_t)eo->nb * 4) + eo->n * 4;
     // gather: codes and element results read once, row offsets, flags, p
     // read, Ap written, row blocks
     *gather_bytes = eo->ninc * 4 + eo->nelem * (int64_t)eo->nb * 4 +

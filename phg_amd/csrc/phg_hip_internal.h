@@ -85,8 +85,9 @@ struct DictLayout {
 
 // Matrix-free FP32 operator of a scalar stiffness matrix (eop.hip), in the
 // solver numbering: y = sum_e P_e^T K_e P_e x with
-// K_e = sum_m coef[e][m] T^m (10 pairs m of barycentric coordinates, T^m the
-// reference tensors), the element products on bf16 MFMA with FP32 operands
+// K_e = sum_m coef[e][m] T^m (6 pairs m of the independent barycentric
+// coordinates, T^m the reference tensors), the element products on bf16
+// MFMA with FP32 operands
 // split into three bf16 parts, then an ordered per-row sum over the row's
 // incident (element, local index) pairs.  Elements are in Morton order of
 // their centroids; Dirichlet DOFs have map -1 (their columns are the ones
@@ -96,8 +97,8 @@ struct EopData {
     int on = 0;                       // used by the FP32 solver slots
     int64_t nelem = 0, n = 0, ninc = 0;
     int32_t *map = nullptr;           // [nelem][nb] solver index or -1
-    float *coef = nullptr;            // [nelem][10]
-    uint16_t *tiles = nullptr;        // [nb][ks][64 lanes][8] bf16 A fragments
+    float *coef = nullptr;            // [nelem][6]
+    uint16_t *tiles = nullptr;        // [nb / 2][ks][64 lanes][8] bf16 A fragments
     int32_t *inc_off = nullptr;       // [n+1]
     int32_t *inc = nullptr;           // [ninc] codes e * nb + a, by element
     int32_t *gblk = nullptr;          // row blocks of the gather

@@ -37,20 +37,35 @@ def batch_meshes(nmesh=8, n=16, seed=5, lmin=2, lmax=8, radius=0.4):
     return out
 
 
-def assemble_batch(meshes, p=3, sigma=0.5, seed=6):
-    """Assembles and eliminates each problem; returns [(space, A, b)]."""
+def assemble_batch(meshes, p=3, sigma=0.5, seed=6, element_operator=False,
+                   kappas=None):
+    """Assembles and eliminates each problem; returns [(space, A, b)].  With
+    element_operator, each matrix also gets the matrix-free FP32 operator
+    (MFMA element products) for the FP32 iterations of the solvers; the
+    coefficients drawn are appended to the list `kappas` when given."""
     import phg_amd as pa
     rng = np.random.default_rng(seed)
     pb = pa.Problem.poisson(use_kappa=True)
     out = []
     for g, _, _ in meshes:
         kappa = np.exp(rng.normal(0.0, sigma, size=g.nelem))
+        if kappas is not None:
+            kappas.append(kappa)
         nd = g.ndof(p)
         sp = pa.Space(g, p)
         A = pa.assemble(sp, pb, kappa=kappa, f=np.ones(nd), g=np.zeros(nd))
         A.eliminate()
+        if element_operator:
+            A.element_operator(sp, pb, kappa=kappa)
         out.append((sp, A, A.rhs()))
     return out
+
+
+def A_fmt32(A):
+    """Format code of the FP32 SpMV layout the tuner chose."""
+    import phg_amd as pa
+    name = A.format()[1]
+    return next(k for k, v in pa.FMT_NAMES.items() if v == name)
 
 
 def bench(nmesh=8, n=16, p=3, rtol=1e-8, reps=2):
@@ -60,7 +75,8 @@ def bench(nmesh=8, n=16, p=3, rtol=1e-8, reps=2):
     t_mesh = time.perf_counter() - t0
     pa.synchronize()
     t0 = time.perf_counter()
-    sys_ = assemble_batch(meshes, p)
+    kappas = []
+    sys_ = assemble_batch(meshes, p, kappas=kappas)
     pa.synchronize()
     t_asm = time.perf_counter() - t0
     As = [A for _, A, _ in sys_]
@@ -97,6 +113,20 @@ def bench(nmesh=8, n=16, p=3, rtol=1e-8, reps=2):
     t_str, its_str = timed(run_multi(False))
     t_gr, its_gr = timed(run_multi(True))
     same = all(np.array_equal(a, x.numpy()) for a, x in zip(x_seq, xs))
+    x_csr = [x.numpy() for x in xs]
+    # the same batch with the FP32 iterations on the matrix-free operator
+    pa.synchronize()
+    t0 = time.perf_counter()
+    pb = pa.Problem.poisson(use_kappa=True)
+    for (sp, A, _), kap in zip(sys_, kappas):
+        A.element_operator(sp, pb, kappa=kap)
+    pa.synchronize()
+    t_eop_build = time.perf_counter() - t0
+    t_eseq, its_eseq = timed(run_seq)
+    t_egr, its_egr = timed(run_multi(True))
+    dev = max(float(np.linalg.norm(x.numpy() - a) / np.linalg.norm(a))
+              for a, x in zip(x_csr, xs))
+    elem_b = [A.element_bytes() for A in As]
     tot = sum(ndofs)
     return {"config": "c5",
             "workload": (f"{nmesh} P{p} kappa-Poisson problems on {n}^3x6 Kuhn meshes refined "
@@ -110,7 +140,16 @@ def bench(nmesh=8, n=16, p=3, rtol=1e-8, reps=2):
             "dofs_per_s_graph": round(tot / t_gr, 1),
             "speedup_graph_vs_sequential": round(t_seq / t_gr, 3),
             "its": [int(i) for i in its_seq], "its_graph": [int(i) for i in its_gr],
-            "graph_solution_bitwise_equal_to_sequential": bool(same)}
+            "graph_solution_bitwise_equal_to_sequential": bool(same),
+            "element_operator": {
+                "build_s": round(t_eop_build, 3),
+                "sequential_s": round(t_eseq, 4), "graph_s": round(t_egr, 4),
+                "dofs_per_s_graph": round(tot / t_egr, 1),
+                "speedup_vs_csr_graph": round(t_gr / t_egr, 3),
+                "its": [int(i) for i in its_eseq], "its_graph": [int(i) for i in its_egr],
+                "max_rel_diff_vs_csr_solution": dev,
+                "bytes_per_apply": [a + b for a, b in elem_b],
+                "csr_fp32_bytes_per_spmv": [A.format_bytes(A_fmt32(A), True) for A in As]}}
 
 
 def main():

@@ -96,21 +96,9 @@ def test_reliable_solver_on_element_operator(hip, p, n):
 def test_multi_on_element_operator_matches_single(hip, graph):
     pa = hip
     meshes = batch_meshes(nmesh=3, n=3, seed=3, lmin=1, lmax=3)
-    p = 3
-    sys_ = assemble_batch(meshes, p=p, seed=4)
-    rng = np.random.default_rng(4)
-    pb = pa.Problem.poisson(use_kappa=True)
-    As, bs = [], []
-    for (g, _, _), (sp, A, b) in zip(meshes, sys_):
-        # the batch's kappa is internal to assemble_batch: rebuild with a
-        # known one
-        kappa = np.exp(rng.normal(0.0, 0.5, size=g.nelem))
-        nd = g.ndof(p)
-        A2 = pa.assemble(sp, pb, kappa=kappa, f=np.ones(nd), g=np.zeros(nd))
-        A2.eliminate()
-        A2.element_operator(sp, pb, kappa=kappa)
-        As.append(A2)
-        bs.append(A2.rhs())
+    sys_ = assemble_batch(meshes, p=3, seed=4, element_operator=True)
+    As = [A for _, A, _ in sys_]
+    bs = [b for _, _, b in sys_]
     xs = [pa.Vec(A.n) for A in As]
     nits, res = pa.pcg_multi(As, bs, xs, rtol=1e-10, btol=1e-10, use_graph=graph)
     for i, (A, b, x) in enumerate(zip(As, bs, xs)):
