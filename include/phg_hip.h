@@ -220,6 +220,10 @@ int phg_hip_element_apply(const phg_hip_mat *A, const phg_hip_vec *x,
  * row offsets, flags, p, Ap). */
 int phg_hip_element_bytes(const phg_hip_mat *A, int64_t *elem_bytes,
                           int64_t *gather_bytes);
+/* Average time (us) of `reps` launches of the element kernel and of the
+ * row gather, each measured with HIP events on the library stream. */
+int phg_hip_element_time(const phg_hip_mat *A, int32_t reps, double *elem_us,
+                         double *gather_us);
 
 /*------------------------------------------------ many right-hand sides */
 

@@ -139,6 +139,7 @@ _sig(hip_lib, "phg_hip_mat_element_operator", C.c_int, [_vp, _vp, _P(Problem), _
 _sig(hip_lib, "phg_hip_mat_use_element_operator", C.c_int, [_vp, _i32])
 _sig(hip_lib, "phg_hip_element_apply", C.c_int, [_vp, _vp, _vp])
 _sig(hip_lib, "phg_hip_element_bytes", C.c_int, [_vp, _P(_i64), _P(_i64)])
+_sig(hip_lib, "phg_hip_element_time", C.c_int, [_vp, _i32, _P(_dbl), _P(_dbl)])
 _sig(hip_lib, "phg_hip_spmm_tune", C.c_int, [_vp, _i32])
 _sig(hip_lib, "phg_hip_spmm_tune_time", C.c_int, [_vp, _i32, _i32, _P(_dbl)])
 _sig(hip_lib, "phg_hip_spmm_bytes", C.c_int, [_vp, _i32, _P(_i64)])
@@ -434,6 +435,13 @@ class Mat:
         a, b = _i64(), _i64()
         _check(hip_lib.phg_hip_element_bytes(self._h, C.byref(a), C.byref(b)),
                "element_bytes")
+        return a.value, b.value
+
+    def element_time(self, reps: int = 20):
+        """(element kernel us, row gather us) per launch."""
+        a, b = _dbl(), _dbl()
+        _check(hip_lib.phg_hip_element_time(self._h, reps, C.byref(a), C.byref(b)),
+               "element_time")
         return a.value, b.value
 
     def __del__(self):

@@ -57,6 +57,10 @@ template <int P> struct EopShape {
 constexpr int kTPart[6] = {0, 0, 1, 0, 1, 2};
 constexpr int kXPart[6] = {0, 1, 0, 2, 1, 0};
 
+// element tiles of 16 a wavefront takes per round: their gathers are
+// issued together and every A fragment read from LDS feeds ET products
+template <int P> constexpr int kEt = P == 3 ? 2 : 4;
+
 template <int P>
 __global__ void __launch_bounds__(EB)
 k_eop_elem(int64_t nelem, const int32_t *__restrict__ map,
@@ -65,7 +69,8 @@ k_eop_elem(int64_t nelem, const int32_t *__restrict__ map,
            const int32_t *done, const int32_t *mode, int32_t want)
 {
     using S = EopShape<P>;
-    constexpr int NB = S::NB, NT = S::NT, KS = S::KS, KP = S::KP;
+    constexpr int NB = S::NB, NT = S::NT, KS = S::KS, KP = S::KP, ET = kEt<P>;
+    constexpr int NJ = (NB + 3) / 4;      // local indices j per lane
     if (done != nullptr && (*done != 0 || *mode != want))
         return;
     __shared__ uint4 at[NT * KS * 64];
@@ -81,25 +86,43 @@ k_eop_elem(int64_t nelem, const int32_t *__restrict__ map,
     const int64_t ntile = (nelem + 15) / 16;
     // every wavefront of the work-group runs the same number of rounds, so
     // the work-group barriers below are reached by all of them
-    for (int64_t base = (int64_t)blockIdx.x * EW; base < ntile;
-         base += (int64_t)gridDim.x * EW) {
-        const int64_t tile = base + w;
-        const int64_t e = tile * 16 + n;
-        const bool ok = tile < ntile && e < nelem;
-        __syncthreads();    // B rows of the previous round are consumed
-        if (tile < ntile) {
+    for (int64_t base = (int64_t)blockIdx.x * EW * ET; base < ntile;
+         base += (int64_t)gridDim.x * EW * ET) {
+        const int64_t tile0 = base + (int64_t)w * ET;
+        // gathers of the ET tiles: x of local indices j = g, g + 4, ... and
+        // the coefficients of the D rows 4 g + r (slots m = 4 (g % 2) + r)
+        float xv[ET][NJ], c[ET][4];
 #pragma unroll
-            for (int j0 = 0; j0 < NB; j0 += 4) {
-                const int j = j0 + g;
+        for (int et = 0; et < ET; et++) {
+            const int64_t e = (tile0 + et) * 16 + n;
+            const bool ok = tile0 + et < ntile && e < nelem;
+#pragma unroll
+            for (int jj = 0; jj < NJ; jj++) {
+                const int j = 4 * jj + g;
+                int32_t col = -1;
+                if (ok && j < NB)
+                    col = map[e * NB + j];
+                xv[et][jj] = col >= 0 ? x[col] : 0.0f;
+            }
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int m = 4 * (g & 1) + r;
+                c[et][r] = (ok && m < NC) ? coef[e * NC + m] : 0.0f;
+            }
+        }
+        // split into bf16 parts through the wavefront's LDS rows, one tile
+        // at a time, into B fragments
+        bf16x8 b[ET][KS];
+#pragma unroll
+        for (int et = 0; et < ET; et++) {
+            __syncthreads();    // the previous fragments are read
+#pragma unroll
+            for (int jj = 0; jj < NJ; jj++) {
+                const int j = 4 * jj + g;
                 if (j < NB) {
-                    float xv = 0.0f;
-                    if (ok) {
-                        const int32_t c = map[e * NB + j];
-                        if (c >= 0)
-                            xv = x[c];
-                    }
-                    const __bf16 x0 = (__bf16)xv;
-                    const float r1 = xv - (float)x0;
+                    const float v = xv[et][jj];
+                    const __bf16 x0 = (__bf16)v;
+                    const float r1 = v - (float)x0;
                     const __bf16 x1 = (__bf16)r1;
                     const __bf16 x2 = (__bf16)(r1 - (float)x1);
                     const __bf16 xp[3] = {x0, x1, x2};
@@ -108,39 +131,45 @@ k_eop_elem(int64_t nelem, const int32_t *__restrict__ map,
                         bt[w][n][q * NB + j] = xp[kXPart[q]];
                 }
             }
+            __syncthreads();
+#pragma unroll
+            for (int s2 = 0; s2 < KS; s2++)
+                b[et][s2] = *(const bf16x8 *)&bt[w][n][32 * s2 + 8 * g];
         }
-        __syncthreads();
-        if (tile < ntile) {
-            // D rows 4 g + r of this lane are slots m = 4 (g % 2) + r
-            float c[4];
+        float yv[ET][(NT + 1) / 2];
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int m = 4 * (g & 1) + r;
-                c[r] = (ok && m < NC) ? coef[e * NC + m] : 0.0f;
+        for (int t = 0; t < NT; t++) {
+            f32x4 acc[ET];
+#pragma unroll
+            for (int et = 0; et < ET; et++)
+                acc[et] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int s2 = 0; s2 < KS; s2++) {
+                const bf16x8 a = __builtin_bit_cast(bf16x8, at[(t * KS + s2) * 64 + l]);
+#pragma unroll
+                for (int et = 0; et < ET; et++)
+                    acc[et] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[et][s2], acc[et],
+                                                                      0, 0, 0);
             }
-            float yv[(NT + 1) / 2];
+            // contract with c_m, then add the two row groups of local index
+            // 2 t + g / 2 (lanes g and g ^ 1)
 #pragma unroll
-            for (int t = 0; t < NT; t++) {
-                f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-                for (int s = 0; s < KS; s++) {
-                    const uint4 av = at[(t * KS + s) * 64 + l];
-                    const bf16x8 a = __builtin_bit_cast(bf16x8, av);
-                    const bf16x8 b = *(const bf16x8 *)&bt[w][n][32 * s + 8 * g];
-                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
-                }
-                // contract with c_m, then add the two row groups of local
-                // index 2 t + g / 2 (lanes g and g ^ 1)
-                float v = c[0] * acc[0] + c[1] * acc[1] + c[2] * acc[2] + c[3] * acc[3];
+            for (int et = 0; et < ET; et++) {
+                float v = c[et][0] * acc[et][0] + c[et][1] * acc[et][1] +
+                          c[et][2] * acc[et][2] + c[et][3] * acc[et][3];
                 v += __shfl_xor(v, 16, 64);
                 if ((t & 1) == (g & 1))
-                    yv[t >> 1] = v;
+                    yv[et][t >> 1] = v;
             }
-            if (ok) {
+        }
+#pragma unroll
+        for (int et = 0; et < ET; et++) {
+            const int64_t e = (tile0 + et) * 16 + n;
+            if (tile0 + et < ntile && e < nelem) {
 #pragma unroll
                 for (int t = 0; t < NT; t++)
                     if ((t & 1) == (g & 1))
-                        y[e * NB + 2 * t + (g >> 1)] = yv[t >> 1];
+                        y[e * NB + 2 * t + (g >> 1)] = yv[et][t >> 1];
             }
         }
     }
@@ -414,7 +443,8 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
     dperm = nullptr;
     if (e != hipSuccess)
         return fail(hip_fail(e, "element operator set-up", __FILE__, __LINE__));
-    const int64_t need = ((ne + 15) / 16 + EW - 1) / EW;
+    const int et = p == 1 ? kEt<1> : (p == 2 ? kEt<2> : kEt<3>);
+    const int64_t need = ((ne + 15) / 16 + (int64_t)EW * et - 1) / ((int64_t)EW * et);
     eo->grid_e = (int)std::max<int64_t>(1, std::min<int64_t>(need, grid_elem(p)));
     eo->on = 1;
     return PHG_HIP_OK;

@@ -2867,6 +2867,61 @@ extern "C" int phg_hip_element_apply(const phg_hip_mat *A_, const phg_hip_vec *x
     return PHG_HIP_OK;
 }
 
+extern "C" int phg_hip_element_time(const phg_hip_mat *A_, int32_t reps,
+                                    double *elem_us, double *gather_us)
+{
+    int rc;
+    if ((rc = phg::require_ctx()))
+        return rc;
+    phg_hip_mat *A = const_cast<phg_hip_mat *>(A_);
+    if (A == nullptr || A->eop == nullptr || elem_us == nullptr || gather_us == nullptr)
+        return phg::set_error(PHG_HIP_E_ARG, "phg_hip_element_time: operator not built");
+    if (reps <= 0)
+        reps = 10;
+    hipStream_t s = phg::ctx()->stream;
+    Workspace *ws = get_ws(A, 1);
+    if (ws == nullptr)
+        return phg::set_error(PHG_HIP_E_NOMEM, "workspace allocation failed");
+    PHG_HIP_CHECK(hipMemsetAsync(ws->p32, 0, (size_t)A->n * sizeof(float), s));
+    const int gg = eop_grid(A->eop);
+    hipEvent_t ev[3];
+    for (int i = 0; i < 3; i++)
+        PHG_HIP_CHECK(hipEventCreate(&ev[i]));
+    double t[2] = {0.0, 0.0};
+    hipError_t err = hipSuccess;
+    for (int k = 0; k < 2 && err == hipSuccess; k++) {
+        for (int i = 0; i < reps + 2; i++) {
+            if (i == 2)
+                err = hipEventRecord(ev[0], s);
+            if (k == 0)
+                phg::eop_elements(A->eop, ws->p32, nullptr, nullptr, 0, s);
+            else
+                hipLaunchKernelGGL(k_eop_gather, dim3(gg), dim3(BS), 0, s,
+                                   eop_view(A->eop), (const float *)A->q_diag32,
+                                   (const float *)ws->p32, ws->Ap32, ws->seg(SEG_S),
+                                   (const State *)nullptr);
+        }
+        if (err == hipSuccess)
+            err = hipGetLastError();
+        if (err == hipSuccess)
+            err = hipEventRecord(ev[1], s);
+        if (err == hipSuccess)
+            err = hipEventSynchronize(ev[1]);
+        float ms = 0.f;
+        if (err == hipSuccess)
+            err = hipEventElapsedTime(&ms, ev[0], ev[1]);
+        t[k] = 1e3 * ms / reps;
+    }
+    for (int i = 0; i < 3; i++)
+        (void)hipEventDestroy(ev[i]);
+    if (err != hipSuccess)
+        return phg::set_error(PHG_HIP_E_HIP, "element operator timing: %s",
+                              hipGetErrorString(err));
+    *elem_us = t[0];
+    *gather_us = t[1];
+    return PHG_HIP_OK;
+}
+
 extern "C" int phg_hip_element_bytes(const phg_hip_mat *A, int64_t *elem_bytes,
                                      int64_t *gather_bytes)
 {

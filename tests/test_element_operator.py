@@ -47,6 +47,9 @@ def test_apply_matches_assembled_matrix(hip, kind, p, n):
         rel = np.linalg.norm(b - a) / np.linalg.norm(a)
         assert rel < 2e-6, (trial, rel)
         assert np.max(np.abs(b - a)) < 2e-5 * np.max(np.abs(a))
+    te, tg = A.element_time(reps=3)
+    eb, gb = A.element_bytes()
+    assert te > 0 and tg > 0 and eb > 0 and gb > 0
 
 
 def test_apply_dirichlet_rows_are_diagonal(hip):
