@@ -1269,10 +1269,14 @@ __global__ void __launch_bounds__(BS) k_ir_correct(int64_t n, double *x,
 //   k_ru_p       decides the slot kind; ITER: p = z + beta p;
 //                UPDATE: x += x32, x32 = 0
 //   k_ru_spmv    ITER only: Ap = A32 p, partials of (p, Ap)
-//   k_ru_update  ITER only: x32 += alpha p, r32 -= alpha Ap, partials
-//   k_ru_resid   UPDATE only: r32 = b - A x in FP64, partials
-// k_ru_p of the next slot reads the partials of the last kernel that ran
-// (grid g after ITER, grid gs after UPDATE or the prologue).
+//   k_ru_update  ITER: x32 += alpha p, r32 -= alpha Ap, partials;
+//                UPDATE: r32 = b - A x in FP64 with the solver CSR, partials
+// so an ITER slot runs no kernel that only exits.  A layout with its own
+// row order (SELL) adds k_ru_resid, which computes the UPDATE residual
+// through the layout instead.  k_ru_p of the next slot reads the partials
+// of the kernel that wrote them: after ITER k_ru_update (grid g), after
+// UPDATE k_ru_update or k_ru_resid (grid g or gs), after the prologue
+// k_ru_init (grid gs).
 
 // prologue: r32 = b - A x (or b when x is zero), x32 = 0, partials of
 // (r,r), (r,z), #zero diagonals computed from the FP64 residual
@@ -1321,8 +1325,8 @@ __global__ void __launch_bounds__(BS) k_ru_p(int64_t n, const float *r32,
                                              const float *diag32, float *p32,
                                              double *x, float *x32,
                                              const double *part_in, int np_iter,
-                                             int np_full, const State *sin,
-                                             State *sout)
+                                             int np_upd, int np_start,
+                                             const State *sin, State *sout)
 {
     const State s = *sin;
     const bool w0 = blockIdx.x == 0 && threadIdx.x == 0;
@@ -1332,7 +1336,9 @@ __global__ void __launch_bounds__(BS) k_ru_p(int64_t n, const float *r32,
         return;
     }
     double t[3];
-    sum_partials<3>(part_in, s.mode == RU_ITER ? np_iter : np_full, t);
+    sum_partials<3>(part_in,
+                    s.mode == RU_ITER ? np_iter : (s.mode == RU_UPDATE ? np_upd : np_start),
+                    t);
     const double res = sqrt(t[0]), rz = t[1];
     State o = s;
     o.res = res;
@@ -1448,15 +1454,61 @@ __global__ void __launch_bounds__(BS) k_ru_spmv(View mv, const float *va,
     write_partials<1>(v, part_out);
 }
 
+// FP64 residual of a replacement slot: r32 = b - A x with the solver CSR
+// (row blocks of the streaming SpMV, rows summed in storage order),
+// partials of (r,r), (r,z) and the number of zero diagonals as k_ru_init
+// writes them
+__device__ inline void ru_residual64(const int32_t *__restrict__ rp,
+                                     const int32_t *__restrict__ ci,
+                                     const double *__restrict__ va,
+                                     const int32_t *__restrict__ rblk,
+                                     int64_t nrblk,
+                                     const double *__restrict__ diag,
+                                     const double *__restrict__ b,
+                                     const double *__restrict__ x, float *r32,
+                                     double *part_out)
+{
+    double v[3] = {0.0, 0.0, 0.0};
+    stream_spmv<double, phg::SPMV_BLOCK_NNZ, 0>(
+        rp, ci, va, x, rblk, nrblk, [&](int32_t row, double s, double) {
+            const double ri = b[row] - s;
+            r32[row] = (float)ri;
+            const double d = diag[row];
+            v[0] += ri * ri;
+            if (d == 0.0)
+                v[2] += 1.0;
+            else
+                v[1] += ri * (ri / d);
+        });
+    write_partials<3>(v, part_out);
+}
+
+// ITER: x32 += alpha p, r32 -= alpha Ap, partials; UPDATE: the FP64
+// residual (x was folded by k_ru_p)
 __global__ void __launch_bounds__(BS) k_ru_update(int64_t n, float *x32,
                                                   const float *p, float *r,
                                                   const float *Ap,
                                                   const float *diag,
                                                   const double *part_in,
                                                   int np_in, double *part_out,
-                                                  State *st)
+                                                  State *st, const int32_t *rp,
+                                                  const int32_t *ci,
+                                                  const double *va,
+                                                  const int32_t *rblk,
+                                                  int64_t nrblk,
+                                                  const double *diag64,
+                                                  const double *b,
+                                                  const double *x,
+                                                  int resid_here)
 {
-    if (st->done || st->mode != RU_ITER)
+    if (st->done)
+        return;
+    if (st->mode == RU_UPDATE) {
+        if (resid_here)
+            ru_residual64(rp, ci, va, rblk, nrblk, diag64, b, x, r, part_out);
+        return;
+    }
+    if (st->mode != RU_ITER)
         return;
     double pap[1];
     sum_partials<1>(part_in, np_in, pap);
@@ -1493,6 +1545,8 @@ __global__ void __launch_bounds__(BS) k_ru_update(int64_t n, float *x32,
     write_partials<3>(v, part_out);
 }
 
+// UPDATE slots of layouts with their own row order (SELL): r32 = b - A x
+// in FP64 through the layout, partials
 template <typename View>
 __global__ void __launch_bounds__(BS) k_ru_resid(View mv, const double *va,
                                                  const double *diag,
@@ -2083,6 +2137,9 @@ struct RuPlan {
     const EopData *eo = nullptr;
     EopGather eg{};
     int gs32 = 0;
+    // the FP64 residual of UPDATE slots runs in k_ru_update (layouts in
+    // the solver numbering) or in k_ru_resid (layouts with their own order)
+    bool resid_in_update = true;
     const double *bb = nullptr;
     double *xx = nullptr;
     const int32_t *perm = nullptr;
@@ -2104,7 +2161,9 @@ int ru_prologue(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x,
     pl->gs = grid_spmv<View>(A, k_ru_spmv<View>);
     pl->mv = view_of(A, (View *)nullptr);
     pl->gs32 = pl->gs;
-    if (A->eop != nullptr && A->eop->on) {
+    pl->resid_in_update = F::perm(A) == A->q_perm;
+    // the operator works in the solver numbering, as every layout but SELL
+    if (A->eop != nullptr && A->eop->on && pl->resid_in_update) {
         pl->eo = A->eop;
         pl->eg = eop_view(A->eop);
         pl->gs32 = eop_grid(A->eop);
@@ -2155,8 +2214,8 @@ void ru_slot(const phg_hip_mat *A, const RuPlan<View> &pl, int64_t k,
     if (prof) prof_mark(KP_P32, s);
     hipLaunchKernelGGL(k_ru_p, dim3(g), dim3(BS), 0, s, n,
                        (const float *)ws->r32, F::d32(A), ws->p32, pl.xx,
-                       ws->x32, (const double *)pres, g, gs,
-                       (const State *)sin, sout);
+                       ws->x32, (const double *)pres, g,
+                       pl.resid_in_update ? g : gs, gs, (const State *)sin, sout);
     if (prof) { prof_mark(KP_P32, s); prof_mark(KP_SPMV32, s); }
     if (pl.eo != nullptr) {
         phg::eop_elements(pl.eo, ws->p32, &sout->done, &sout->mode, RU_ITER, s);
@@ -2172,13 +2231,17 @@ void ru_slot(const phg_hip_mat *A, const RuPlan<View> &pl, int64_t k,
     hipLaunchKernelGGL(k_ru_update, dim3(g), dim3(BS), 0, s, n,
                        ws->x32, (const float *)ws->p32, ws->r32,
                        (const float *)ws->Ap32, F::d32(A),
-                       (const double *)ps, pl.gs32, pres, sout);
-    if (prof) { prof_mark(KP_UPD32, s); prof_mark(KP_SPMV64, s); }
-    hipLaunchKernelGGL((k_ru_resid<View>), dim3(gs), dim3(BS), 0, s,
-                       pl.mv, F::v64(A), F::d64(A), pl.bb,
-                       (const double *)pl.xx, ws->r32, pres,
-                       (const State *)sout);
-    if (prof) prof_mark(KP_SPMV64, s);
+                       (const double *)ps, pl.gs32, pres, sout,
+                       (const int32_t *)A->q_rowptr, (const int32_t *)A->q_cols,
+                       (const double *)A->q_vals, (const int32_t *)A->q_rblk,
+                       A->q_nrblk, (const double *)A->q_diag,
+                       pl.bb, (const double *)pl.xx, pl.resid_in_update ? 1 : 0);
+    if (prof) prof_mark(KP_UPD32, s);
+    if (!pl.resid_in_update)
+        hipLaunchKernelGGL((k_ru_resid<View>), dim3(gs), dim3(BS), 0, s,
+                           pl.mv, F::v64(A), F::d64(A), pl.bb,
+                           (const double *)pl.xx, ws->r32, pres,
+                           (const State *)sout);
 }
 
 template <typename View>
@@ -2218,13 +2281,13 @@ int reliable_impl(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x,
             return rc;
         if (prof) {
             // keep the timing of the kernels that did the work of the slot
-            const int drop[3] = {hm->mode == RU_ITER ? KP_SPMV64 : KP_SPMV32,
+            const int drop[3] = {hm->mode == RU_ITER ? -1 : KP_SPMV32,
                                  hm->mode == RU_ITER ? -1 : KP_UPD32, -1};
             for (int d : drop)
                 if (d >= 0 && !hm->done)
                     g_prof.used[d] -= 2;
             if (hm->done)
-                for (int d : {KP_P32, KP_SPMV32, KP_UPD32, KP_SPMV64})
+                for (int d : {KP_P32, KP_SPMV32, KP_UPD32})
                     g_prof.used[d] -= 2;
         }
         if (hm->done != RUNNING)
