@@ -120,6 +120,13 @@ int phg_hip_mat_info(const phg_hip_mat *A, int64_t *nrows, int64_t *nnz);
 int phg_hip_mat_download(const phg_hip_mat *A, int64_t *rowptr, int32_t *cols,
                          double *vals, double *rhs);
 int phg_hip_mat_destroy(phg_hip_mat *A);
+/* matvec.c:6134 phgMatDumpCSR: writes fn as the size, the n+1 row pointers
+ * (0-based), the column indices and the values ("%0.17lg"), one number per
+ * line, rows in storage order; eliminates A first when it was not (as
+ * PHG assembles first).  fn == NULL does nothing. */
+int phg_hip_mat_dump_csr(phg_hip_mat *A, const char *fn);
+/* matvec.c:6245 phgVecDump: the size, then the values ("%0.17lg"). */
+int phg_hip_vec_dump(const phg_hip_vec *v, const char *fn);
 
 /*---------------------------------------------------------------- vectors */
 

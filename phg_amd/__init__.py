@@ -114,6 +114,8 @@ _sig(hip_lib, "phg_hip_mat_info", C.c_int, [_vp, _P(_i64), _P(_i64)])
 _sig(hip_lib, "phg_hip_mat_download", C.c_int,
      [_vp, _P(_i64), _P(_i32), _P(_dbl), _P(_dbl)])
 _sig(hip_lib, "phg_hip_mat_destroy", C.c_int, [_vp])
+_sig(hip_lib, "phg_hip_mat_dump_csr", C.c_int, [_vp, _cp])
+_sig(hip_lib, "phg_hip_vec_dump", C.c_int, [_vp, _cp])
 _sig(hip_lib, "phg_hip_vec_create", C.c_int, [_i64, _P(_vp)])
 _sig(hip_lib, "phg_hip_vec_destroy", C.c_int, [_vp])
 _sig(hip_lib, "phg_hip_vec_upload", C.c_int, [_vp, _P(_dbl)])
@@ -321,6 +323,10 @@ class Vec:
         _check(hip_lib.phg_hip_vec_download(self._h, _ptr(a, _dbl)), "vec_download")
         return a
 
+    def dump(self, fn: str) -> None:
+        """phgVecDump text format (matvec.c:6245)."""
+        _check(hip_lib.phg_hip_vec_dump(self._h, fn.encode()), "vec_dump")
+
     def dot(self, other: "Vec") -> float:
         r = _dbl()
         _check(hip_lib.phg_hip_vec_dot(self._h, other._h, C.byref(r)), "vec_dot")
@@ -415,6 +421,10 @@ class Mat:
 
     def matvec(self, x: Vec, y: Vec, alpha=1.0, beta=0.0) -> None:
         _check(hip_lib.phg_hip_matvec(alpha, self._h, x._h, beta, y._h), "matvec")
+
+    def dump_csr(self, fn: str) -> None:
+        """phgMatDumpCSR text format (matvec.c:6134)."""
+        _check(hip_lib.phg_hip_mat_dump_csr(self._h, fn.encode()), "mat_dump_csr")
 
     def element_operator(self, sp: "Space", pb: "Problem", kappa=None) -> None:
         """Builds the matrix-free FP32 operator (MFMA element products) that

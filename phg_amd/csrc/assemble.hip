@@ -1769,6 +1769,67 @@ extern "C" int phg_hip_mat_download(const phg_hip_mat *A, int64_t *rowptr,
     return PHG_HIP_OK;
 }
 
+// matvec.c:6134 phgMatDumpCSR: the size, the n + 1 row pointers (0-based),
+// the column indices and the values ("%0.17lg"), one number per line, of
+// the rows in storage order; like PHG it assembles (eliminates) first
+extern "C" int phg_hip_mat_dump_csr(phg_hip_mat *A, const char *fn)
+{
+    int rc;
+    if ((rc = phg::require_ctx()))
+        return rc;
+    if (A == nullptr)
+        return phg::set_error(PHG_HIP_E_ARG, "null matrix");
+    if (fn == nullptr)
+        return PHG_HIP_OK;      // matvec.c:6185: nothing to do
+    if (!A->eliminated && (rc = phg_hip_mat_eliminate(A)))
+        return rc;
+    std::vector<int32_t> rp((size_t)A->n + 1), ci((size_t)A->nnz);
+    std::vector<double> va((size_t)A->nnz);
+    if ((rc = phg::download(rp.data(), A->d_rowptr, rp.size() * sizeof(int32_t))) ||
+        (rc = phg::download(ci.data(), A->d_cols, ci.size() * sizeof(int32_t))) ||
+        (rc = phg::download(va.data(), A->d_vals, va.size() * sizeof(double))))
+        return rc;
+    FILE *fp = fopen(fn, "w+t");
+    if (fp == nullptr)
+        return phg::set_error(PHG_HIP_E_ARG, "cannot open file \"%s\"", fn);
+    fprintf(fp, "%lld\n", (long long)A->n);
+    for (int64_t i = 0; i <= A->n; i++)
+        fprintf(fp, "%d\n", rp[(size_t)i]);
+    for (int64_t k = 0; k < A->nnz; k++)
+        fprintf(fp, "%d\n", ci[(size_t)k]);
+    for (int64_t k = 0; k < A->nnz; k++)
+        fprintf(fp, "%0.17lg\n", va[(size_t)k]);
+    const bool bad = ferror(fp) != 0;
+    if (fclose(fp) != 0 || bad)
+        return phg::set_error(PHG_HIP_E_ARG, "write error on \"%s\"", fn);
+    return PHG_HIP_OK;
+}
+
+// matvec.c:6245 phgVecDump: the size, then one value ("%0.17lg") a line
+extern "C" int phg_hip_vec_dump(const phg_hip_vec *v, const char *fn)
+{
+    int rc;
+    if ((rc = phg::require_ctx()))
+        return rc;
+    if (v == nullptr)
+        return phg::set_error(PHG_HIP_E_ARG, "null vector");
+    if (fn == nullptr)
+        return PHG_HIP_OK;
+    std::vector<double> h((size_t)v->n);
+    if ((rc = phg::download(h.data(), v->d, h.size() * sizeof(double))))
+        return rc;
+    FILE *fp = fopen(fn, "w+t");
+    if (fp == nullptr)
+        return phg::set_error(PHG_HIP_E_ARG, "cannot open file \"%s\"", fn);
+    fprintf(fp, "%lld\n", (long long)v->n);
+    for (int64_t i = 0; i < v->n; i++)
+        fprintf(fp, "%0.17lg\n", h[(size_t)i]);
+    const bool bad = ferror(fp) != 0;
+    if (fclose(fp) != 0 || bad)
+        return phg::set_error(PHG_HIP_E_ARG, "write error on \"%s\"", fn);
+    return PHG_HIP_OK;
+}
+
 extern "C" int phg_hip_mat_destroy(phg_hip_mat *A)
 {
     if (A == nullptr)
