@@ -158,6 +158,17 @@ int phg_hip_pcg(const phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x,
                 int32_t maxit, double rtol, double btol, double atol,
                 double *residual);
 
+/* solver-phg.c:2457 BiCGstab_Solve / :607 solver_BiCGstab with the Jacobi
+ * PC (pcg_pc_type default), FP64, for matrices that need not be
+ * symmetric: same arguments and stopping rule as phg_hip_pcg.  The
+ * |s| <= 1e-60 exit returns the residual of the updated x, as the
+ * reference does; b = 0 gives x = 0, 0 iterations and residual 1 (the
+ * reference leaves its caller's initial value).  Returns the iteration
+ * count or an error code. */
+int phg_hip_bicgstab(const phg_hip_mat *A, const phg_hip_vec *b,
+                     phg_hip_vec *x, int32_t maxit, double rtol, double btol,
+                     double atol, double *residual);
+
 /* Mixed precision (no reference counterpart): FP64 iterative refinement
  * r = b - A x in FP64, correction from Jacobi-PCG in FP32 arithmetic on an
  * FP32 copy of A with FP64 dot products, x += d in FP64.  Stops on the

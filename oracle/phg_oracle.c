@@ -722,6 +722,138 @@ done:
     return nits;
 }
 
+/* matvec.c:1237 phgVecAXPBY: y = b y (skipped for b = 1), then y += a x
+ * (skipped for a = 0); the special cases a, b = +-1 of the reference give
+ * the same values */
+static void axpby(int64_t n, double a, const double *x, double b, double *y)
+{
+    if (a == 0.0 && b == 0.0) {
+        memset(y, 0, (size_t)n * sizeof(double));
+        return;
+    }
+    if (b != 1.0)
+        for (int64_t i = 0; i < n; i++)
+            y[i] = b * y[i];
+    if (a == 0.0)
+        return;
+    for (int64_t i = 0; i < n; i++)
+        y[i] += a * x[i];
+}
+
+/* solver-phg.c:207 jacobi_pc; returns -1 on a zero diagonal */
+static int jacobi(int64_t n, const double *diag, const double *r, double *z)
+{
+    for (int64_t i = 0; i < n; i++) {
+        if (diag[i] == 0.0)
+            return -1;
+        z[i] = r[i] / diag[i];
+    }
+    return 0;
+}
+
+int orc_bicgstab(const orc_csr *A, const double *b, double *x, int maxit,
+                 double rtol, double btol, double atol, double *residual)
+{
+    /* solver-phg.c:607-782 solver_BiCGstab with the Jacobi PC */
+    int64_t n = A->n;
+    size_t sz = (size_t)n * sizeof(double);
+    double *diag = xmalloc(sz), *r1 = xmalloc(sz), *rg = xmalloc(sz);
+    double *rh = xmalloc(sz), *pg = xmalloc(sz), *ph = xmalloc(sz);
+    double *sg = xmalloc(sz), *sh = xmalloc(sz), *tg = xmalloc(sz);
+    double *vg = xmalloc(sz);
+    double rb, r0, res = 0.0, tol, x1 = 0.0;
+    double rho0 = 0.0, rho1, alpha = 0.0, beta, omega = 0.0;
+    int nits = 0;
+
+    for (int64_t i = 0; i < n; i++) {
+        diag[i] = 0.0;
+        for (int64_t j = A->rowptr[i]; j < A->rowptr[i + 1]; j++)
+            if (A->cols[j] == i) {
+                diag[i] = A->vals[j];
+                break;
+            }
+    }
+    rb = norm2(n, b);
+    if (fabs(rb) < 10.0 * DBL_MIN) {
+        /* solver-phg.c:631: x = 0, the residual is left as the caller set it */
+        memset(x, 0, sz);
+        res = 1.0;
+        goto done;
+    }
+    for (int64_t i = 0; i < n; i++)
+        x1 += fabs(x[i]);
+    if (!(fabs(x1 / (double)n) < 10.0 * DBL_MIN)) {
+        orc_matvec(A, x, r1);
+        memcpy(rg, r1, sz);
+        axpby(n, 1.0, b, -1.0, rg);
+    }
+    else
+        memcpy(rg, b, sz);
+    r0 = res = norm2(n, rg);
+    tol = r0 * rtol;
+    if (tol < rb * btol)
+        tol = rb * btol;
+    if (tol < atol)
+        tol = atol;
+    memcpy(rh, rg, sz);
+    memset(sh, 0, sz);
+    memset(ph, 0, sz);
+    (void)r0;
+
+    while (res > tol && nits < maxit) {
+        nits++;
+        rho1 = dot(n, rg, rh);
+        if (rho1 == 0.0)
+            break;                          /* "method failed" */
+        if (nits == 1)
+            memcpy(pg, rg, sz);
+        else {
+            beta = (rho1 * alpha) / (rho0 * omega);
+            memcpy(sg, pg, sz);
+            axpby(n, -omega, vg, 1.0, sg);
+            memcpy(pg, rg, sz);
+            axpby(n, beta, sg, 1.0, pg);
+        }
+        rho0 = rho1;
+        if (jacobi(n, diag, pg, ph)) {
+            nits = -1;
+            goto done;
+        }
+        orc_matvec(A, ph, vg);
+        alpha = rho1 / dot(n, rh, vg);
+        memcpy(sg, rg, sz);
+        axpby(n, -alpha, vg, 1.0, sg);
+        omega = norm2(n, sg);
+        if (omega <= 1e-60) {
+            axpby(n, alpha, ph, 1.0, x);
+            orc_matvec(A, x, r1);
+            memcpy(rg, b, sz);
+            axpby(n, -1.0, r1, 1.0, rg);
+            res = norm2(n, rg);
+            break;
+        }
+        if (jacobi(n, diag, sg, sh)) {
+            nits = -1;
+            goto done;
+        }
+        orc_matvec(A, sh, tg);
+        omega = dot(n, tg, sg) / dot(n, tg, tg);
+        axpby(n, omega, sh, alpha, ph);
+        axpby(n, 1.0, ph, 1.0, x);
+        memcpy(rg, sg, sz);
+        axpby(n, -omega, tg, 1.0, rg);
+        res = norm2(n, rg);
+        if (res <= tol)
+            break;
+    }
+done:
+    if (residual)
+        *residual = res;
+    free(diag); free(r1); free(rg); free(rh); free(pg); free(ph);
+    free(sg); free(sh); free(tg); free(vg);
+    return nits;
+}
+
 void orc_csr_free(orc_csr *A)
 {
     if (!A)

@@ -84,6 +84,12 @@ void orc_eliminate(orc_csr *A, const uint8_t *bdry_rows);
 int orc_pcg(const orc_csr *A, const double *b, double *x, int maxit,
             double rtol, double btol, double atol, double *residual);
 
+/* solver-phg.c:607 solver_BiCGstab with jacobi_pc, same stopping rule;
+ * returns nits (<0: zero diagonal); b = 0 gives x = 0, residual 1 (the
+ * reference leaves the caller's initial value). */
+int orc_bicgstab(const orc_csr *A, const double *b, double *x, int maxit,
+                 double rtol, double btol, double atol, double *residual);
+
 /* y = A x (matvec.c:887-930, row sums in storage order). */
 void orc_matvec(const orc_csr *A, const double *x, double *y);
 

@@ -23,7 +23,7 @@ _LIB = os.path.join(_HERE, "lib")
 
 __all__ = [
     "Grid", "Space", "Mat", "Vec", "Problem", "init", "finalize",
-    "assemble", "element_matrices", "pcg", "pcg_mixed", "pcg_reliable", "pcg_block", "pcg_multi", "mat_from_csr", "dof_norms", "spmm_tune", "adjoint_logkappa", "PhgHipError",
+    "assemble", "element_matrices", "pcg", "bicgstab", "pcg_mixed", "pcg_reliable", "pcg_block", "pcg_multi", "mat_from_csr", "dof_norms", "spmm_tune", "adjoint_logkappa", "PhgHipError",
     "hip_lib", "grid_lib", "HIP_LIB_PATH", "GRID_LIB_PATH",
 ]
 
@@ -127,6 +127,8 @@ _sig(hip_lib, "phg_hip_vec_norm2", C.c_int, [_vp, _P(_dbl)])
 _sig(hip_lib, "phg_hip_vec_axpby", C.c_int, [_dbl, _vp, _dbl, _vp])
 _sig(hip_lib, "phg_hip_vec_copy", C.c_int, [_vp, _vp])
 _sig(hip_lib, "phg_hip_pcg", C.c_int,
+     [_vp, _vp, _vp, _i32, _dbl, _dbl, _dbl, _P(_dbl)])
+_sig(hip_lib, "phg_hip_bicgstab", C.c_int,
      [_vp, _vp, _vp, _i32, _dbl, _dbl, _dbl, _P(_dbl)])
 _sig(hip_lib, "phg_hip_pcg_mixed", C.c_int,
      [_vp, _vp, _vp, _i32, _dbl, _dbl, _dbl, _dbl, _P(_i32), _P(_dbl)])
@@ -612,6 +614,14 @@ def pcg(A: Mat, b: Vec, x: Vec, maxit=10000, rtol=1e-10, btol=1e-10, atol=0.0):
     res = _dbl()
     nits = _check(hip_lib.phg_hip_pcg(A._h, b._h, x._h, maxit, rtol, btol, atol,
                                       C.byref(res)), "pcg")
+    return nits, res.value
+
+
+def bicgstab(A: Mat, b: Vec, x: Vec, maxit=10000, rtol=1e-10, btol=1e-10, atol=0.0):
+    """FP64 Jacobi-BiCGstab (solver-phg.c:607). Returns (nits, residual)."""
+    res = _dbl()
+    nits = _check(hip_lib.phg_hip_bicgstab(A._h, b._h, x._h, maxit, rtol, btol, atol,
+                                           C.byref(res)), "bicgstab")
     return nits, res.value
 
 

@@ -1259,6 +1259,294 @@ __global__ void __launch_bounds__(BS) k_ir_correct(int64_t n, double *x,
         x[i] += s * (double)d[i];
 }
 
+//------------------------------------------------------------------ BiCGstab
+// solver-phg.c:607 solver_BiCGstab with jacobi_pc, FP64, one grid pass per
+// step of the reference's loop:
+//   k_bi_p      loop control (residual test, rho1 == 0), beta,
+//               p = r + beta (p - omega v), ph = p / diag
+//   k_bi_spmv1  v = A ph, partials of (rh, v)
+//   k_bi_s      alpha = rho1 / (rh, v), s = r - alpha v, sh = s / diag,
+//               partials of (s, s)
+//   k_bi_spmv2  |s| <= 1e-60 exit (flag), else t = A sh, partials of
+//               (t, s), (t, t)
+//   k_bi_update omega = (t, s) / (t, t), ph = alpha ph + omega sh,
+//               x += ph, r = s - omega t, partials of (r, r), (r, rh);
+//               on the exit flag x += alpha ph (the caller then forms the
+//               true residual, solver-phg.c:718-727)
+// Every kernel reads the whole state at its start and work-group 0 writes
+// the next copy (two copies alternate per launch), so no work-group reads
+// a value another one writes in the same launch.
+struct BiState {
+    double rb, tol, res, rho0, rho1, alpha, omega, beta;
+    double rtol, btol, atol;
+    int32_t nits, done, maxit, small;
+};
+constexpr int BI_SMALL_S = 4;    // done: the |s| <= 1e-60 exit was taken
+
+__global__ void __launch_bounds__(BS) k_bi_p(int64_t n, const double *r,
+                                             const double *diag, double *p,
+                                             double *ph, const double *v,
+                                             const double *part_in, int np_init,
+                                             int np_iter, const BiState *sin,
+                                             BiState *sout)
+{
+    const BiState s = *sin;
+    const bool w0 = blockIdx.x == 0 && threadIdx.x == 0;
+    if (s.done) {
+        if (w0)
+            *sout = s;
+        return;
+    }
+    double t[3];                 // (r,r), (r,rh), #zero diagonals
+    sum_partials<3>(part_in, s.nits == 0 ? np_init : np_iter, t);
+    const double res = sqrt(t[0]);
+    BiState o = s;
+    o.res = res;
+    bool stop = false;
+    if (s.nits == 0) {
+        if (is_zero(s.rb)) {
+            o.res = 1.0;         // solver-phg.c:631: the caller's value
+            o.done = CONVERGED;
+            stop = true;
+        }
+        else {
+            double tol = res * s.rtol;       // get_tol (solver-phg.c:104)
+            if (tol < s.rb * s.btol)
+                tol = s.rb * s.btol;
+            if (tol < s.atol)
+                tol = s.atol;
+            o.tol = tol;
+            if (!(res > tol && s.maxit > 0)) {
+                o.done = CONVERGED;
+                stop = true;
+            }
+            else if (t[2] != 0.0) {          // solver-phg.c:233
+                o.done = ZERO_DIAG;
+                stop = true;
+            }
+        }
+    }
+    else if (!(res > s.tol && s.nits < s.maxit)) {
+        o.done = CONVERGED;
+        stop = true;
+    }
+    double beta = 0.0;
+    if (!stop) {
+        o.nits = s.nits + 1;
+        const double rho1 = t[1];
+        if (rho1 == 0.0) {                   // solver-phg.c:685
+            o.done = BREAKDOWN;
+            stop = true;
+        }
+        else {
+            if (o.nits > 1)
+                beta = (rho1 * s.alpha) / (s.rho0 * s.omega);
+            o.beta = beta;
+            o.rho1 = rho1;
+            o.rho0 = rho1;
+        }
+    }
+    if (w0)
+        *sout = o;
+    if (stop)
+        return;
+    const bool first = o.nits == 1;
+    const double mo = -s.omega;
+    for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * BS) {
+        double pi;
+        if (first)
+            pi = r[i];
+        else {
+            const double si = p[i] + mo * v[i];
+            pi = r[i] + beta * si;
+        }
+        p[i] = pi;
+        ph[i] = pi / diag[i];
+    }
+}
+
+template <typename View>
+__global__ void __launch_bounds__(BS) k_bi_spmv1(View mv, const double *va,
+                                                 const double *ph, double *v,
+                                                 const double *rh, double *part_out,
+                                                 const BiState *sin, BiState *sout)
+{
+    const BiState s = *sin;
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        *sout = s;
+    if (s.done)
+        return;
+    double a[1] = {0.0};
+    spmv_apply<double>(mv, va, ph, [&](int32_t row, double sum, double) {
+        v[row] = sum;
+        a[0] += rh[row] * sum;
+    });
+    write_partials<1>(a, part_out);
+}
+
+__global__ void __launch_bounds__(BS) k_bi_s(int64_t n, const double *r,
+                                             const double *v, const double *diag,
+                                             double *sg, double *sh,
+                                             const double *part_in, int np_in,
+                                             double *part_out, const BiState *sin,
+                                             BiState *sout)
+{
+    const BiState s = *sin;
+    const bool w0 = blockIdx.x == 0 && threadIdx.x == 0;
+    if (s.done) {
+        if (w0)
+            *sout = s;
+        return;
+    }
+    double rv[1];
+    sum_partials<1>(part_in, np_in, rv);
+    BiState o = s;
+    o.alpha = s.rho1 / rv[0];
+    if (w0)
+        *sout = o;
+    const double ma = -o.alpha;
+    double a[1] = {0.0};
+    for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * BS) {
+        const double si = r[i] + ma * v[i];
+        sg[i] = si;
+        sh[i] = si / diag[i];
+        a[0] += si * si;
+    }
+    write_partials<1>(a, part_out);
+}
+
+template <typename View>
+__global__ void __launch_bounds__(BS) k_bi_spmv2(View mv, const double *va,
+                                                 const double *sh, double *tg,
+                                                 const double *sg,
+                                                 const double *part_in, int np_in,
+                                                 double *part_out,
+                                                 const BiState *sin, BiState *sout)
+{
+    const BiState s = *sin;
+    const bool w0 = blockIdx.x == 0 && threadIdx.x == 0;
+    if (s.done) {
+        if (w0)
+            *sout = s;
+        return;
+    }
+    double ss[1];
+    sum_partials<1>(part_in, np_in, ss);
+    BiState o = s;
+    o.small = sqrt(ss[0]) <= 1e-60 ? 1 : 0;  // solver-phg.c:717
+    if (w0)
+        *sout = o;
+    if (o.small)
+        return;
+    double a[2] = {0.0, 0.0};
+    spmv_apply<double>(mv, va, sh, [&](int32_t row, double sum, double) {
+        tg[row] = sum;
+        a[0] += sum * sg[row];
+        a[1] += sum * sum;
+    });
+    write_partials<2>(a, part_out);
+}
+
+__global__ void __launch_bounds__(BS) k_bi_update(int64_t n, double *x, double *ph,
+                                                  const double *sh, double *r,
+                                                  const double *sg, const double *tg,
+                                                  const double *rh,
+                                                  const double *part_in, int np_in,
+                                                  double *part_out,
+                                                  const BiState *sin, BiState *sout)
+{
+    const BiState s = *sin;
+    const bool w0 = blockIdx.x == 0 && threadIdx.x == 0;
+    if (s.done) {
+        if (w0)
+            *sout = s;
+        return;
+    }
+    if (s.small) {
+        BiState o = s;
+        o.done = BI_SMALL_S;
+        if (w0)
+            *sout = o;
+        for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
+             i += (int64_t)gridDim.x * BS)
+            x[i] += s.alpha * ph[i];
+        return;
+    }
+    double t[2];
+    sum_partials<2>(part_in, np_in, t);
+    BiState o = s;
+    o.omega = t[0] / t[1];
+    if (w0)
+        *sout = o;
+    const double al = s.alpha, om = o.omega, mo = -o.omega;
+    double a[3] = {0.0, 0.0, 0.0};
+    for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * BS) {
+        const double pi = al * ph[i] + om * sh[i];
+        ph[i] = pi;
+        x[i] += pi;
+        const double ri = sg[i] + mo * tg[i];
+        r[i] = ri;
+        a[0] += ri * ri;
+        a[1] += ri * rh[i];
+    }
+    write_partials<3>(a, part_out);
+}
+
+// prologue: r = b - A x (or b when x is zero; x = 0 when b is zero),
+// rh = r, partials of (r,r), (r,rh) = (r,r) and the zero diagonals
+template <typename View>
+__global__ void __launch_bounds__(BS) k_bi_init(int64_t n, View mv, const double *va,
+                                                const double *diag, const double *b,
+                                                double *x, double *r, double *rh,
+                                                const double *part_in, int np_in,
+                                                double *part_out, BiState *st0)
+{
+    double nb[2];
+    sum_partials<2>(part_in, np_in, nb);
+    const double rb = sqrt(nb[0]);
+    const bool bzero = is_zero(rb);
+    const bool xnz = !is_zero(nb[1] / (double)n);
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        st0->rb = rb;
+    double a[3] = {0.0, 0.0, 0.0};
+    auto acc = [&](int64_t row, double ri) {
+        r[row] = ri;
+        rh[row] = ri;
+        a[0] += ri * ri;
+        a[1] += ri * ri;
+        if (diag[row] == 0.0)
+            a[2] += 1.0;
+    };
+    if (xnz && !bzero)
+        spmv_apply<double>(mv, va, x,
+                           [&](int32_t row, double sum, double) { acc(row, -sum + b[row]); });
+    else
+        for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
+             i += (int64_t)gridDim.x * BS) {
+            if (bzero)
+                x[i] = 0.0;
+            acc(i, bzero ? 0.0 : b[i]);
+        }
+    write_partials<3>(a, part_out);
+}
+
+// true residual of the |s| <= 1e-60 exit: partials of (b - A x)^2
+template <typename View>
+__global__ void __launch_bounds__(BS) k_bi_true_res(View mv, const double *va,
+                                                    const double *b, const double *x,
+                                                    double *part_out)
+{
+    double a[1] = {0.0};
+    spmv_apply<double>(mv, va, x, [&](int32_t row, double sum, double) {
+        const double ri = -sum + b[row];
+        a[0] += ri * ri;
+    });
+    write_partials<1>(a, part_out);
+}
+
 //------------------------------------------ reliable-update mixed precision
 // FP32 PCG whose residual is replaced by the FP64 residual b - A x (x kept
 // in FP64) whenever the FP32 residual has dropped by the factor delta below
@@ -1644,6 +1932,9 @@ struct Workspace {
     double *scal = nullptr;               // small device scalars
     float *r32 = nullptr, *p32 = nullptr, *Ap32 = nullptr, *x32 = nullptr;
     State *st = nullptr;                  // [2] ping-pong copies + 1 spare
+    // BiCGstab: rh, ph, v, s, sh, t (one allocation) and its states
+    double *bi = nullptr;
+    BiState *bst = nullptr;
     double *seg(int k) { return partials + (size_t)k * 4 * MAX_GRID; }
 };
 
@@ -1795,6 +2086,8 @@ void release_ws(Workspace *w)
     phg::dfree(w->partials); phg::dfree(w->st); phg::dfree(w->scal);
     phg::dfree(w->r32); phg::dfree(w->p32); phg::dfree(w->Ap32);
     phg::dfree(w->x32);
+    phg::dfree(w->bi);
+    phg::dfree(w->bst);
     *w = Workspace();
 }
 
@@ -1964,6 +2257,122 @@ int run_pcg_loop(const phg_hip_mat *A, const T *va, const T *diag, T *x, T *r,
 }
 
 int ensure_format(phg_hip_mat *A, int fmt, int need32);
+
+// solver-phg.c:607 solver_BiCGstab in the numbering of the format
+template <typename View>
+int bicgstab_impl(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x,
+                  int32_t maxit, double rtol, double btol, double atol,
+                  double *residual)
+{
+    using F = FmtData<View>;
+    hipStream_t s = phg::ctx()->stream;
+    Workspace *ws = get_ws(A, 0);
+    if (ws == nullptr)
+        return phg::set_error(PHG_HIP_E_NOMEM, "solver workspace allocation failed");
+    const int64_t n = A->n;
+    if (ws->bi == nullptr &&
+        (phg::dmalloc((void **)&ws->bi, (size_t)6 * n * sizeof(double)) ||
+         phg::dmalloc((void **)&ws->bst, 2 * sizeof(BiState))))
+        return phg::set_error(PHG_HIP_E_NOMEM, "BiCGstab workspace allocation failed");
+    double *rh = ws->bi, *ph = rh + n, *v = ph + n, *sg = v + n, *sh = sg + n,
+           *tg = sh + n;
+    BiState h;
+    memset(&h, 0, sizeof(h));
+    h.rtol = rtol;
+    h.btol = btol;
+    h.atol = atol;
+    h.maxit = maxit;
+    PHG_HIP_CHECK(hipMemcpyAsync(ws->bst, &h, sizeof(BiState), hipMemcpyHostToDevice, s));
+    PHG_HIP_CHECK(hipStreamSynchronize(s));     // h lives on this stack
+    const int g = grid_for(n), gs = grid_spmv<View>(A, k_bi_spmv1<View>);
+    const View mv = view_of(A, (View *)nullptr);
+    const double *va = F::v64(A), *d = F::d64(A);
+    const double *bb = b->d;
+    double *xx = x->d;
+    const int32_t *perm = F::perm(A);
+    if (perm) {
+        hipLaunchKernelGGL(k_gather, dim3(g), dim3(BS), 0, s, n, perm,
+                           (const double *)b->d, ws->bp);
+        hipLaunchKernelGGL(k_gather, dim3(g), dim3(BS), 0, s, n, perm,
+                           (const double *)x->d, ws->xp);
+        bb = ws->bp;
+        xx = ws->xp;
+    }
+    double *pn = ws->seg(SEG_NORMS), *pr = ws->seg(SEG_RES), *pv = ws->seg(SEG_S),
+           *pss = ws->seg(SEG_U), *pt = ws->seg(SEG_IR);
+    hipLaunchKernelGGL(k_pcg_init_norms, dim3(g), dim3(BS), 0, s, n, bb,
+                       (const double *)xx, pn);
+    hipLaunchKernelGGL((k_bi_init<View>), dim3(gs), dim3(BS), 0, s, n, mv, va, d, bb,
+                       xx, ws->r, rh, (const double *)pn, g, pr, ws->bst);
+    PHG_HIP_CHECK(hipGetLastError());
+    BiState *st = ws->bst;
+    BiState *hm = (BiState *)phg::ctx()->host_mailbox;
+    int64_t j = 0;                       // kernel launches so far
+    auto sio = [&](int64_t k) { return std::make_pair(st + (k & 1), st + ((k + 1) & 1)); };
+    int chunk = 8;
+    for (int64_t it = 0;; ) {
+        for (int c = 0; c < chunk; c++, it++) {
+            auto p0 = sio(j++);
+            hipLaunchKernelGGL(k_bi_p, dim3(g), dim3(BS), 0, s, n,
+                               (const double *)ws->r, d, ws->p, ph, (const double *)v,
+                               (const double *)pr, gs, g, (const BiState *)p0.first,
+                               p0.second);
+            auto p1 = sio(j++);
+            hipLaunchKernelGGL((k_bi_spmv1<View>), dim3(gs), dim3(BS), 0, s, mv, va,
+                               (const double *)ph, v, (const double *)rh, pv,
+                               (const BiState *)p1.first, p1.second);
+            auto p2 = sio(j++);
+            hipLaunchKernelGGL(k_bi_s, dim3(g), dim3(BS), 0, s, n, (const double *)ws->r,
+                               (const double *)v, d, sg, sh, (const double *)pv, gs, pss,
+                               (const BiState *)p2.first, p2.second);
+            auto p3 = sio(j++);
+            hipLaunchKernelGGL((k_bi_spmv2<View>), dim3(gs), dim3(BS), 0, s, mv, va,
+                               (const double *)sh, tg, (const double *)sg,
+                               (const double *)pss, g, pt, (const BiState *)p3.first,
+                               p3.second);
+            auto p4 = sio(j++);
+            hipLaunchKernelGGL(k_bi_update, dim3(g), dim3(BS), 0, s, n, xx, ph,
+                               (const double *)sh, ws->r, (const double *)sg,
+                               (const double *)tg, (const double *)rh,
+                               (const double *)pt, gs, pr, (const BiState *)p4.first,
+                               p4.second);
+        }
+        PHG_HIP_CHECK(hipGetLastError());
+        PHG_HIP_CHECK(hipMemcpyAsync(hm, st + (j & 1), sizeof(BiState),
+                                     hipMemcpyDeviceToHost, s));
+        PHG_HIP_CHECK(hipStreamSynchronize(s));
+        if (hm->done != RUNNING)
+            break;
+        if (it > (int64_t)maxit + 256)
+            return phg::set_error(PHG_HIP_E_HIP, "BiCGstab state did not advance");
+        chunk = std::min(chunk * 2, 64);
+    }
+    BiState fin = *hm;
+    if (fin.done == BI_SMALL_S) {
+        // solver-phg.c:720-726: residual of the updated x
+        hipLaunchKernelGGL((k_bi_true_res<View>), dim3(gs), dim3(BS), 0, s, mv, va, bb,
+                           (const double *)xx, pr);
+        PHG_HIP_CHECK(hipGetLastError());
+        std::vector<double> part((size_t)gs);
+        PHG_HIP_CHECK(hipMemcpyAsync(part.data(), pr, (size_t)gs * sizeof(double),
+                                     hipMemcpyDeviceToHost, s));
+        PHG_HIP_CHECK(hipStreamSynchronize(s));
+        double sum = 0.0;
+        for (double q : part)
+            sum += q;
+        fin.res = sqrt(sum);
+    }
+    if (perm)
+        hipLaunchKernelGGL(k_scatter, dim3(g), dim3(BS), 0, s, n, perm,
+                           (const double *)ws->xp, x->d);
+    PHG_HIP_CHECK(hipGetLastError());
+    PHG_HIP_CHECK(hipStreamSynchronize(s));
+    if (residual)
+        *residual = fin.res;
+    if (fin.done == ZERO_DIAG)
+        return phg::set_error(PHG_HIP_E_ZERO_DIAG, "zero diagonal in Jacobi preconditioner");
+    return fin.nits;
+}
 
 // FP64 PCG in the numbering of the format
 template <typename View>
@@ -2693,6 +3102,22 @@ extern "C" int phg_hip_pcg(const phg_hip_mat *A_, const phg_hip_vec *b,
     return with_format(A->fmt64, [&](auto tag) {
         using View = decltype(tag);
         return pcg64_impl<View>(A, b, x, maxit, rtol, btol, atol, residual);
+    });
+}
+
+extern "C" int phg_hip_bicgstab(const phg_hip_mat *A_, const phg_hip_vec *b,
+                                phg_hip_vec *x, int32_t maxit, double rtol,
+                                double btol, double atol, double *residual)
+{
+    int rc;
+    if ((rc = check_solve_args(A_, b, x)))
+        return rc;
+    phg_hip_mat *A = const_cast<phg_hip_mat *>(A_);
+    if ((rc = ensure_format(A, A->fmt64, 0)))
+        return rc;
+    return with_format(A->fmt64, [&](auto tag) {
+        using View = decltype(tag);
+        return bicgstab_impl<View>(A, b, x, maxit, rtol, btol, atol, residual);
     });
 }
 

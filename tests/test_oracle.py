@@ -416,3 +416,76 @@ def test_oracle_stiffness_energy_closed_form():
             g = orc.adjoint_grad(p, c["verts"], c["elem_verts"], c["map"],
                                  np.ones(c["grid"].nelem), v, v)
             assert abs(-g.sum() - want) < 1e-13
+
+
+def _solver_test0_system(N=12):
+    """test/solver_test0.c: row 0 all ones (rhs N), rows 1.. the identity
+    (rhs 1); the solution is x = 1."""
+    rows, cols, vals = [], [], []
+    rp = [0]
+    for i in range(N):
+        if i == 0:
+            cols += list(range(N))
+            vals += [1.0] * N
+        else:
+            cols.append(i)
+            vals.append(1.0)
+        rp.append(len(cols))
+    b = np.ones(N)
+    b[0] = float(N)
+    return np.array(rp), np.array(cols), np.array(vals), b
+
+
+def test_bicgstab_solver_test0_known_answer():
+    rp, ci, va, b = _solver_test0_system()
+    A = orc.csr_from_arrays(rp, ci, va)
+    x, nits, res = orc.bicgstab(A, b, rtol=1e-14, btol=1e-14)
+    assert nits > 0
+    np.testing.assert_allclose(x, 1.0, rtol=0, atol=1e-12)
+
+
+def test_bicgstab_diagonal_system_one_step():
+    """With the Jacobi PC a diagonal system is solved by the first step:
+    s = r - A D^-1 r = 0 takes the |s| <= 1e-60 exit with x = D^-1 b."""
+    n = 9
+    d = np.arange(1.0, n + 1.0)
+    A = orc.csr_from_arrays(np.arange(n + 1), np.arange(n), d)
+    b = np.linspace(-2.0, 3.0, n)
+    x, nits, res = orc.bicgstab(A, b)
+    assert nits == 1 and res == 0.0
+    assert np.array_equal(x, b / d)
+
+
+def test_bicgstab_spd_and_nonsymmetric_match_direct_solves():
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    c = poisson_case(3, 2, "kappa")
+    Ao = orc.assemble(orc.problem(2, use_kappa=True), c["verts"], c["elem_verts"], c["map"],
+                      c["ndof"], c["bdry"], c["g"], c["f"], c["kappa"])
+    orc.eliminate(Ao, c["bdry"])
+    xd = spla.spsolve(Ao.to_scipy().tocsc(), Ao.rhs)
+    x, nits, res = orc.bicgstab(Ao, Ao.rhs, rtol=1e-13, btol=1e-13)
+    assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-9
+    # nonsymmetric, diagonally dominant
+    rng = np.random.default_rng(3)
+    n = 300
+    M = sp.random(n, n, density=0.03, random_state=4, format="csr") * 2.0 - \
+        sp.random(n, n, density=0.03, random_state=5, format="csr")
+    M = (M + sp.diags(np.abs(M).sum(axis=1).A.ravel() + 1.0)).tocsr()
+    M.sort_indices()
+    xs = rng.standard_normal(n)
+    b = M @ xs
+    A = orc.csr_from_arrays(M.indptr, M.indices, M.data)
+    x, nits, res = orc.bicgstab(A, b, rtol=1e-13, btol=1e-13)
+    assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-10
+    # the reported residual is the recursive one; the true residual agrees
+    assert np.linalg.norm(b - M @ x) < 10 * max(res, 1e-13 * np.linalg.norm(b))
+
+
+def test_bicgstab_zero_rhs_and_maxit():
+    rp, ci, va, b = _solver_test0_system()
+    A = orc.csr_from_arrays(rp, ci, va)
+    x, nits, res = orc.bicgstab(A, np.zeros(len(b)), x0=np.ones(len(b)))
+    assert nits == 0 and res == 1.0 and not x.any()
+    x, nits, res = orc.bicgstab(A, b, maxit=1, rtol=1e-30, btol=1e-30)
+    assert nits == 1
