@@ -854,6 +854,211 @@ done:
     return nits;
 }
 
+/* matvec.c:552 phgMatVec with A == NULL (the identity): y := alpha x +
+ * beta y with the reference's case split (matvec.c:606-737) */
+static void vaxpby(int64_t n, double alpha, const double *x, double beta,
+                   double *y)
+{
+    if (alpha == 0.0) {
+        if (beta == 0.0)
+            memset(y, 0, (size_t)n * sizeof(double));
+        else if (beta == -1.0)
+            for (int64_t i = 0; i < n; i++)
+                y[i] = -y[i];
+        else if (beta != 1.0)
+            for (int64_t i = 0; i < n; i++)
+                y[i] *= beta;
+        return;
+    }
+    for (int64_t i = 0; i < n; i++) {
+        double ax = alpha == 1.0 ? x[i] : (alpha == -1.0 ? -x[i] : alpha * x[i]);
+        if (beta == 0.0)
+            y[i] = ax;
+        else if (beta == -1.0)
+            y[i] = ax - y[i];
+        else if (beta == 1.0)
+            y[i] += ax;
+        else
+            y[i] = ax + beta * y[i];
+    }
+}
+
+/* phgMatVec with a matrix: y := beta y, then y_i += row sum of
+ * a_ij (alpha x_j) in storage order (matvec.c:606-633, :842-930) */
+static void amatvec(const orc_csr *A, double alpha, const double *x,
+                    double beta, double *y)
+{
+    int64_t n = A->n;
+    double *data = xmalloc((size_t)n * sizeof(double));
+    vaxpby(n, 0.0, NULL, beta, y);
+    for (int64_t i = 0; i < n; i++)
+        data[i] = alpha == 1.0 ? x[i] : (alpha == -1.0 ? -x[i] : alpha * x[i]);
+    for (int64_t i = 0; i < n; i++) {
+        int64_t j0 = A->rowptr[i], j1 = A->rowptr[i + 1];
+        if (j1 == j0)
+            continue;
+        double s = A->vals[j0] * data[A->cols[j0]];
+        for (int64_t j = j0 + 1; j < j1; j++)
+            s += A->vals[j] * data[A->cols[j]];
+        y[i] += s;
+    }
+    free(data);
+}
+
+int orc_flgmres(const orc_csr *A, const double *b, double *x, int restart,
+                int augk, int maxit, double rtol, double btol, double atol,
+                double *residual)
+{
+    /* solver-phg.c:2063-2340 solver_FLGMRES_r (right-preconditioned,
+     * flexible, augmented by the last augk corrections) with jacobi_pc */
+    int64_t n = A->n;
+    size_t sz = (size_t)n * sizeof(double);
+    int nits = 0, i, j, k, itr_out, mk, zi, m;
+    double res, r0, r_norm, b_norm, tol, tmp, h1, h2, x1 = 0.0;
+    double *diag = xmalloc(sz), *z = xmalloc(sz), *rv = xmalloc(sz);
+    if (restart <= 0)
+        restart = maxit + 1;
+    if (augk <= 0)
+        augk = 2;
+    for (int64_t q = 0; q < n; q++) {
+        diag[q] = 0.0;
+        for (int64_t jj = A->rowptr[q]; jj < A->rowptr[q + 1]; jj++)
+            if (A->cols[jj] == q) {
+                diag[q] = A->vals[jj];
+                break;
+            }
+    }
+    b_norm = norm2(n, b);
+    if (fabs(b_norm) < 10.0 * DBL_MIN) {
+        memset(x, 0, sz);
+        if (residual)
+            *residual = 1.0;        /* GMRES_Solve's initial value */
+        free(diag); free(z); free(rv);
+        return 0;
+    }
+    mk = restart;
+    m = restart + augk;             /* largest subspace */
+    double **v = xmalloc((size_t)(m + 2) * sizeof(double *));
+    double **flv = xmalloc((size_t)(m + 2) * sizeof(double *));
+    double **zp = xmalloc((size_t)augk * sizeof(double *));
+    for (i = 0; i < m + 2; i++) {
+        v[i] = xmalloc(sz);
+        flv[i] = xmalloc(sz);
+    }
+    for (i = 0; i < augk; i++)
+        zp[i] = xmalloc(sz);
+    double *H = xmalloc((size_t)(m + 2) * (m + 2) * sizeof(double));
+#define HH(a, c) H[(size_t)(a) * (m + 2) + (c)]
+    double *y = xmalloc((size_t)(m + 2) * 4 * sizeof(double));
+    double *sv = y + m + 2, *c1 = y + 2 * (m + 2), *c2 = y + 3 * (m + 2);
+
+    memcpy(z, b, sz);
+    for (int64_t q = 0; q < n; q++)
+        x1 += fabs(x[q]);
+    if (x1 != 0.0)
+        amatvec(A, -1.0, x, 1.0, z);
+    r0 = r_norm = res = norm2(n, z);
+    tol = r0 * rtol;
+    if (tol < b_norm * btol)
+        tol = b_norm * btol;
+    if (tol < atol)
+        tol = atol;
+    (void)r0;
+    sv[0] = 1.0;
+    itr_out = 0;
+    while (nits < maxit) {
+        restart = itr_out <= augk ? mk + itr_out : mk + augk;
+        sv[1] = r_norm = norm2(n, z);
+        vaxpby(n, 1.0 / r_norm, z, 0.0, v[1]);
+        i = 0;
+        memset(H, 0, (size_t)(m + 2) * (m + 2) * sizeof(double));
+        while (i < restart && r_norm > tol && nits < maxit) {
+            i++, nits++;
+            double *v1 = v[i + 1];
+            if (jacobi(n, diag, i <= mk ? v[i] : zp[i - mk - 1], rv)) {
+                nits = -1;
+                goto done;
+            }
+            amatvec(A, 1.0, rv, 0.0, v1);
+            memcpy(flv[i], rv, sz);
+            for (k = 1; k <= i; k++) {          /* modified Gram-Schmidt */
+                HH(i, k) = dot(n, v1, v[k]);
+                vaxpby(n, -HH(i, k), v[k], 1.0, v1);
+            }
+            HH(i, i + 1) = norm2(n, v1);
+            if (fabs(HH(i, i + 1)) < 10.0 * DBL_MIN) {
+                i--;                            /* happy breakdown */
+                break;
+            }
+            vaxpby(n, 0.0, NULL, 1.0 / HH(i, i + 1), v1);
+            for (j = 1; j < i; j++) {           /* previous rotations */
+                h1 = c1[j] * HH(i, j) + c2[j] * HH(i, j + 1);
+                h2 = -c2[j] * HH(i, j) + c1[j] * HH(i, j + 1);
+                HH(i, j) = h1;
+                HH(i, j + 1) = h2;
+            }
+            tmp = sqrt(HH(i, i) * HH(i, i) + HH(i, i + 1) * HH(i, i + 1));
+            if (fabs(tmp) < 10.0 * DBL_MIN)
+                tmp = 1.e-16;
+            c1[i] = HH(i, i) / tmp;
+            c2[i] = HH(i, i + 1) / tmp;
+            sv[i + 1] = -c2[i] * sv[i];
+            sv[i] = c1[i] * sv[i];
+            HH(i, i) = c1[i] * HH(i, i) + c2[i] * HH(i, i + 1);
+            r_norm = fabs(sv[i + 1]);
+            if (r_norm <= tol)
+                break;
+        }
+        for (j = i; j > 0; j--) {               /* H y = s */
+            y[j] = sv[j] / HH(j, j);
+            for (k = j - 1; k > 0; k--)
+                sv[k] -= HH(j, k) * y[j];
+        }
+        if (i > 0) {
+            vaxpby(n, y[1], flv[1], 0.0, z);
+            for (j = 2; j <= i; j++)
+                vaxpby(n, y[j], flv[j], 1.0, z);
+        }
+        vaxpby(n, 1.0, z, 1.0, x);
+        if (i > 0) {
+            vaxpby(n, y[1], v[1], 0.0, z);
+            if (i <= mk) {
+                for (j = 2; j <= i; j++)
+                    vaxpby(n, y[j], v[j], 1.0, z);
+            }
+            else {
+                int ak = itr_out < augk ? itr_out : augk;
+                for (j = 2; j <= mk; j++)
+                    vaxpby(n, y[j], v[j], 1.0, z);
+                for (j = 1; j <= ak; j++)
+                    vaxpby(n, y[j + mk], zp[j - 1], 1.0, z);
+            }
+        }
+        zi = itr_out % augk;
+        memcpy(zp[zi], z, sz);
+        if (r_norm <= tol) {
+            res = r_norm;
+            break;
+        }
+        memcpy(z, b, sz);
+        amatvec(A, -1.0, x, 1.0, z);
+        itr_out++;
+    }
+done:
+#undef HH
+    if (residual)
+        *residual = res;
+    for (i = 0; i < m + 2; i++) {
+        free(v[i]);
+        free(flv[i]);
+    }
+    for (i = 0; i < augk; i++)
+        free(zp[i]);
+    free(v); free(flv); free(zp); free(H); free(y);
+    free(diag); free(z); free(rv);
+    return nits;
+}
+
 void orc_csr_free(orc_csr *A)
 {
     if (!A)

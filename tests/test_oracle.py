@@ -489,3 +489,74 @@ def test_bicgstab_zero_rhs_and_maxit():
     assert nits == 0 and res == 1.0 and not x.any()
     x, nits, res = orc.bicgstab(A, b, maxit=1, rtol=1e-30, btol=1e-30)
     assert nits == 1
+
+
+def _nonsym(n, seed):
+    import scipy.sparse as sp
+    M = sp.random(n, n, density=0.05, random_state=seed, format="csr") * 2.0 - \
+        sp.random(n, n, density=0.05, random_state=seed + 1, format="csr")
+    M = (M + sp.diags(np.abs(M).sum(axis=1).A.ravel() * 0.6 + 0.5)).tocsr()
+    M.sort_indices()
+    return M
+
+
+def test_gmres_solver_test0_known_answer():
+    rp, ci, va, b = _solver_test0_system()
+    A = orc.csr_from_arrays(rp, ci, va)
+    x, nits, res = orc.gmres(A, b, rtol=1e-14, btol=1e-14)
+    assert nits > 0 and res <= 1e-14 * np.linalg.norm(b)
+    np.testing.assert_allclose(x, 1.0, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("restart,augk", [(10, 2), (3, 2), (5, 1)])
+def test_gmres_matches_direct_solves(restart, augk):
+    import scipy.sparse.linalg as spla
+    c = poisson_case(3, 2, "kappa")
+    Ao = orc.assemble(orc.problem(2, use_kappa=True), c["verts"], c["elem_verts"], c["map"],
+                      c["ndof"], c["bdry"], c["g"], c["f"], c["kappa"])
+    orc.eliminate(Ao, c["bdry"])
+    xd = spla.spsolve(Ao.to_scipy().tocsc(), Ao.rhs)
+    x, nits, res = orc.gmres(Ao, Ao.rhs, restart=restart, augk=augk, rtol=1e-12, btol=1e-12)
+    assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-9
+    M = _nonsym(200, 7)
+    xs = np.random.default_rng(8).standard_normal(200)
+    b = M @ xs
+    A = orc.csr_from_arrays(M.indptr, M.indices, M.data)
+    x, nits, res = orc.gmres(A, b, restart=restart, augk=augk, rtol=1e-12, btol=1e-12)
+    assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-9
+    # converged: the reported residual is the Givens estimate, the true one agrees
+    assert res <= 1e-12 * np.linalg.norm(b) * (1 + 1e-9)
+    assert np.linalg.norm(b - M @ x) < 100 * res
+
+
+def test_gmres_first_cycle_is_minimal_residual():
+    """After one cycle of m steps, x = M^-1 V y minimises |b - A x| over
+    x in M^-1 K_m(A M^-1, b) (right preconditioning): compare with a dense
+    least-squares solve over an orthonormal Krylov basis."""
+    n, m = 60, 6
+    M = _nonsym(n, 11)
+    A = M.toarray()
+    d = np.diag(A)
+    b = np.random.default_rng(12).standard_normal(n)
+    Ao = orc.csr_from_arrays(M.indptr, M.indices, M.data)
+    x, nits, res = orc.gmres(Ao, b, restart=m, augk=2, maxit=m, rtol=1e-30, btol=1e-30)
+    assert nits == m
+    assert res == np.linalg.norm(b)          # not converged: |r0| is reported
+    AM = A / d[None, :]
+    K = np.empty((n, m))
+    v = b / np.linalg.norm(b)
+    for j in range(m):
+        K[:, j] = v
+        v = AM @ v
+    Q, _ = np.linalg.qr(K)
+    u, *_ = np.linalg.lstsq(AM @ Q, b, rcond=None)
+    best = np.linalg.norm(b - AM @ Q @ u)
+    got = np.linalg.norm(b - A @ x)
+    assert abs(got - best) <= 1e-9 * np.linalg.norm(b)
+
+
+def test_gmres_zero_rhs():
+    rp, ci, va, b = _solver_test0_system()
+    A = orc.csr_from_arrays(rp, ci, va)
+    x, nits, res = orc.gmres(A, np.zeros(len(b)), x0=np.ones(len(b)))
+    assert nits == 0 and res == 1.0 and not x.any()
