@@ -169,6 +169,20 @@ int phg_hip_bicgstab(const phg_hip_mat *A, const phg_hip_vec *b,
                      phg_hip_vec *x, int32_t maxit, double rtol, double btol,
                      double atol, double *residual);
 
+/* solver-phg.c:2484 GMRES_Solve with PHG's default -gmres_type flgmres_r
+ * (solver-phg.c:2063: right preconditioned, flexible, the search space
+ * augmented by the corrections of the last augk cycles) and the Jacobi
+ * PC.  restart and augk are -gmres_restart (PHG default 10) and
+ * -lgmres_augk (default 2); restart <= 0 means maxit + 1 and augk <= 0
+ * means 2 as in the reference, and restart + augk must stay below 40.
+ * Stopping rule as phg_hip_pcg on the Givens residual estimate.  As in the
+ * reference the residual reported is |r0| unless the tolerance was
+ * reached, and b = 0 gives x = 0 and residual 1.  Returns the number of
+ * Arnoldi steps or an error code. */
+int phg_hip_gmres(const phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x,
+                  int32_t restart, int32_t augk, int32_t maxit, double rtol,
+                  double btol, double atol, double *residual);
+
 /* Mixed precision (no reference counterpart): FP64 iterative refinement
  * r = b - A x in FP64, correction from Jacobi-PCG in FP32 arithmetic on an
  * FP32 copy of A with FP64 dot products, x += d in FP64.  Stops on the

@@ -23,7 +23,7 @@ _LIB = os.path.join(_HERE, "lib")
 
 __all__ = [
     "Grid", "Space", "Mat", "Vec", "Problem", "init", "finalize",
-    "assemble", "element_matrices", "pcg", "bicgstab", "pcg_mixed", "pcg_reliable", "pcg_block", "pcg_multi", "mat_from_csr", "dof_norms", "spmm_tune", "adjoint_logkappa", "PhgHipError",
+    "assemble", "element_matrices", "pcg", "bicgstab", "gmres", "pcg_mixed", "pcg_reliable", "pcg_block", "pcg_multi", "mat_from_csr", "dof_norms", "spmm_tune", "adjoint_logkappa", "PhgHipError",
     "hip_lib", "grid_lib", "HIP_LIB_PATH", "GRID_LIB_PATH",
 ]
 
@@ -130,6 +130,8 @@ _sig(hip_lib, "phg_hip_pcg", C.c_int,
      [_vp, _vp, _vp, _i32, _dbl, _dbl, _dbl, _P(_dbl)])
 _sig(hip_lib, "phg_hip_bicgstab", C.c_int,
      [_vp, _vp, _vp, _i32, _dbl, _dbl, _dbl, _P(_dbl)])
+_sig(hip_lib, "phg_hip_gmres", C.c_int,
+     [_vp, _vp, _vp, _i32, _i32, _i32, _dbl, _dbl, _dbl, _P(_dbl)])
 _sig(hip_lib, "phg_hip_pcg_mixed", C.c_int,
      [_vp, _vp, _vp, _i32, _dbl, _dbl, _dbl, _dbl, _P(_i32), _P(_dbl)])
 _sig(hip_lib, "phg_hip_pcg_reliable", C.c_int,
@@ -622,6 +624,16 @@ def bicgstab(A: Mat, b: Vec, x: Vec, maxit=10000, rtol=1e-10, btol=1e-10, atol=0
     res = _dbl()
     nits = _check(hip_lib.phg_hip_bicgstab(A._h, b._h, x._h, maxit, rtol, btol, atol,
                                            C.byref(res)), "bicgstab")
+    return nits, res.value
+
+
+def gmres(A: Mat, b: Vec, x: Vec, restart=10, augk=2, maxit=10000, rtol=1e-10, btol=1e-10,
+          atol=0.0):
+    """FP64 Jacobi-FLGMRES (solver-phg.c:2063, PHG's default GMRES).
+    Returns (nits, residual)."""
+    res = _dbl()
+    nits = _check(hip_lib.phg_hip_gmres(A._h, b._h, x._h, restart, augk, maxit, rtol, btol,
+                                        atol, C.byref(res)), "gmres")
     return nits, res.value
 
 

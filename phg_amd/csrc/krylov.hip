@@ -1547,6 +1547,135 @@ __global__ void __launch_bounds__(BS) k_bi_true_res(View mv, const double *va,
     write_partials<1>(a, part_out);
 }
 
+//------------------------------------------------------------------- GMRES
+// solver-phg.c:2063 solver_FLGMRES_r (PHG's default GMRES) with jacobi_pc:
+// the vector work of one Arnoldi step runs on the device --
+//   k_gm_pc        flv_i = src / diag (src = v_i or an augmentation vector)
+//   k_gm_spmv      v_{i+1} = A flv_i, partials of (v_{i+1}, v_1)
+//   k_gm_mgs       modified Gram-Schmidt step k: h_k from the partials,
+//                  v_{i+1} -= h_k v_k, partials of (v_{i+1}, v_{k+1}) or of
+//                  |v_{i+1}|^2 after the last k
+//   k_gm_scale     h_{i+1} = |v_{i+1}|, v_{i+1} /= h_{i+1} (not on a zero
+//                  h_{i+1}), and the next step's flv = v_{i+1} / diag
+// -- and the host reads the column h_1..h_{i+1} and applies the Givens
+// rotations, least-squares solve and restart logic of the reference in
+// FP64.  Work-group 0 writes each h_k to hcol.
+constexpr int GM_MAXV = 40;          // largest subspace (restart + augk)
+
+__global__ void __launch_bounds__(BS) k_zero(int64_t n, double *x)
+{
+    for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * BS)
+        x[i] = 0.0;
+}
+
+__global__ void __launch_bounds__(BS) k_gm_pc(int64_t n, const double *src,
+                                              const double *diag, double *dst)
+{
+    for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * BS)
+        dst[i] = src[i] / diag[i];
+}
+
+template <typename View>
+__global__ void __launch_bounds__(BS) k_gm_spmv(View mv, const double *va,
+                                                const double *f, double *v,
+                                                const double *v1, double *part_out)
+{
+    double a[1] = {0.0};
+    spmv_apply<double>(mv, va, f, [&](int32_t row, double s, double) {
+        v[row] = s;
+        a[0] += s * v1[row];
+    });
+    write_partials<1>(a, part_out);
+}
+
+__global__ void __launch_bounds__(BS) k_gm_mgs(int64_t n, double *w, const double *vk,
+                                               const double *vnext,
+                                               const double *part_in, int np_in,
+                                               double *part_out, double *hcol, int k)
+{
+    double h[1];
+    sum_partials<1>(part_in, np_in, h);
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        hcol[k] = h[0];
+    const double mh = -h[0];
+    double a[1] = {0.0};
+    for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * BS) {
+        const double wi = w[i] + mh * vk[i];
+        w[i] = wi;
+        a[0] += wi * (vnext ? vnext[i] : wi);
+    }
+    write_partials<1>(a, part_out);
+}
+
+__global__ void __launch_bounds__(BS) k_gm_scale(int64_t n, double *w,
+                                                 const double *part_in, int np_in,
+                                                 double *hcol, int k,
+                                                 const double *diag, double *fnext)
+{
+    double h2[1];
+    sum_partials<1>(part_in, np_in, h2);
+    const double h = sqrt(h2[0]);
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        hcol[k] = h;
+    if (is_zero(h))
+        return;                      // happy breakdown: the host stops
+    const double sc = 1.0 / h;
+    for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * BS) {
+        const double wi = sc * w[i];
+        w[i] = wi;
+        if (fnext)
+            fnext[i] = wi / diag[i];
+    }
+}
+
+// out = sum_j c_j src_j in the reference's order (c_1 src_1, then + c_j
+// src_j), then dst += out (add_to) or dst = out
+struct GmCombo {
+    const double *src[GM_MAXV];
+    double c[GM_MAXV];
+    int m;
+};
+__global__ void __launch_bounds__(BS) k_gm_combo(int64_t n, GmCombo cb, double *dst,
+                                                 int add_to)
+{
+    for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * BS) {
+        double z = cb.c[0] * cb.src[0][i];
+        for (int j = 1; j < cb.m; j++)
+            z += cb.c[j] * cb.src[j][i];
+        dst[i] = add_to ? dst[i] + z : z;
+    }
+}
+
+// z = b - A x (or b when x is zero), partials of |z|^2 and the number of
+// zero diagonals
+template <typename View>
+__global__ void __launch_bounds__(BS) k_gm_resid(int64_t n, View mv, const double *va,
+                                                 const double *diag, const double *b,
+                                                 const double *x, int xnz, double *z,
+                                                 double *part_out)
+{
+    double a[2] = {0.0, 0.0};
+    auto acc = [&](int64_t row, double zi) {
+        z[row] = zi;
+        a[0] += zi * zi;
+        if (diag[row] == 0.0)
+            a[1] += 1.0;
+    };
+    if (xnz)
+        spmv_apply<double>(mv, va, x,
+                           [&](int32_t row, double s, double) { acc(row, b[row] + (-s)); });
+    else
+        for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
+             i += (int64_t)gridDim.x * BS)
+            acc(i, b[i]);
+    write_partials<2>(a, part_out);
+}
+
 //------------------------------------------ reliable-update mixed precision
 // FP32 PCG whose residual is replaced by the FP64 residual b - A x (x kept
 // in FP64) whenever the FP32 residual has dropped by the factor delta below
@@ -1935,6 +2064,9 @@ struct Workspace {
     // BiCGstab: rh, ph, v, s, sh, t (one allocation) and its states
     double *bi = nullptr;
     BiState *bst = nullptr;
+    // GMRES basis, preconditioned and augmentation vectors (gm_cnt of n)
+    double *gm = nullptr;
+    int gm_cnt = 0;
     double *seg(int k) { return partials + (size_t)k * 4 * MAX_GRID; }
 };
 
@@ -2088,6 +2220,7 @@ void release_ws(Workspace *w)
     phg::dfree(w->x32);
     phg::dfree(w->bi);
     phg::dfree(w->bst);
+    phg::dfree(w->gm);
     *w = Workspace();
 }
 
@@ -2372,6 +2505,256 @@ int bicgstab_impl(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x,
     if (fin.done == ZERO_DIAG)
         return phg::set_error(PHG_HIP_E_ZERO_DIAG, "zero diagonal in Jacobi preconditioner");
     return fin.nits;
+}
+
+// value k of np work-group partials, summed on the host in block order
+int host_partial_sum(const double *d, int np, int k, hipStream_t s, double *out)
+{
+    std::vector<double> h((size_t)np);
+    PHG_HIP_CHECK(hipMemcpyAsync(h.data(), d + (size_t)k * np, (size_t)np * sizeof(double),
+                                 hipMemcpyDeviceToHost, s));
+    PHG_HIP_CHECK(hipStreamSynchronize(s));
+    double t = 0.0;
+    for (int i = 0; i < np; i++)
+        t += h[(size_t)i];
+    *out = t;
+    return PHG_HIP_OK;
+}
+
+// IsZero (utils.h:451) on the host
+inline bool is_zero_host(double a) { return fabs(a) < 10.0 * 2.2250738585072014e-308; }
+
+// solver-phg.c:2063 solver_FLGMRES_r in the numbering of the format
+template <typename View>
+int gmres_impl(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x, int restart,
+               int augk, int32_t maxit, double rtol, double btol, double atol,
+               double *residual)
+{
+    using F = FmtData<View>;
+    hipStream_t s = phg::ctx()->stream;
+    if (restart <= 0)
+        restart = maxit + 1;
+    if (augk <= 0)
+        augk = 2;
+    const int mk = restart, m = restart + augk;
+    if (m + 1 > GM_MAXV || augk > 8)
+        return phg::set_error(PHG_HIP_E_ARG,
+                              "phg_hip_gmres: restart + augk must be below %d", GM_MAXV);
+    Workspace *ws = get_ws(A, 0);
+    if (ws == nullptr)
+        return phg::set_error(PHG_HIP_E_NOMEM, "solver workspace allocation failed");
+    const int64_t n = A->n;
+    // v_1..v_{m+1}, flv_1..flv_m, zp_0..zp_{augk-1}, z
+    const int cnt = (m + 1) + m + augk + 1;
+    if (ws->gm_cnt < cnt) {
+        phg::dfree(ws->gm);
+        ws->gm = nullptr;
+        ws->gm_cnt = 0;
+        if (phg::dmalloc((void **)&ws->gm, (size_t)cnt * n * sizeof(double)))
+            return phg::set_error(PHG_HIP_E_NOMEM, "GMRES workspace allocation failed");
+        ws->gm_cnt = cnt;
+    }
+    auto V = [&](int j) { return ws->gm + (size_t)(j - 1) * n; };              // 1..m+1
+    auto FL = [&](int j) { return ws->gm + (size_t)(m + j) * n; };             // 1..m
+    auto ZP = [&](int j) { return ws->gm + (size_t)(2 * m + 1 + j) * n; };     // 0..augk-1
+    double *z = ws->gm + (size_t)(2 * m + 1 + augk) * n;
+    const int g = grid_for(n), gs = grid_spmv<View>(A, k_gm_spmv<View>);
+    const View mv = view_of(A, (View *)nullptr);
+    const double *va = F::v64(A), *d = F::d64(A);
+    const double *bb = b->d;
+    double *xx = x->d;
+    const int32_t *perm = F::perm(A);
+    if (perm) {
+        hipLaunchKernelGGL(k_gather, dim3(g), dim3(BS), 0, s, n, perm,
+                           (const double *)b->d, ws->bp);
+        hipLaunchKernelGGL(k_gather, dim3(g), dim3(BS), 0, s, n, perm,
+                           (const double *)x->d, ws->xp);
+        bb = ws->bp;
+        xx = ws->xp;
+    }
+    double *pn = ws->seg(SEG_NORMS), *pr = ws->seg(SEG_RES);
+    double *pp[2] = {ws->seg(SEG_S), ws->seg(SEG_U)};
+    double *hcol = ws->scal;         // 16 doubles: h_1..h_{i+1} of a step
+    if (m + 2 > 16) {
+        // longer columns live in the partials segment after the MGS ones
+        hcol = ws->seg(SEG_IR);
+    }
+    int rc;
+    double nb[2];
+    hipLaunchKernelGGL(k_pcg_init_norms, dim3(g), dim3(BS), 0, s, n, bb,
+                       (const double *)xx, pn);
+    PHG_HIP_CHECK(hipGetLastError());
+    if ((rc = host_partial_sum(pn, g, 0, s, &nb[0])) ||
+        (rc = host_partial_sum(pn, g, 1, s, &nb[1])))
+        return rc;
+    const double b_norm = sqrt(nb[0]);
+    std::vector<double> H((size_t)(m + 2) * (m + 2)), yv((size_t)(m + 2)),
+        sv((size_t)(m + 2)), c1((size_t)(m + 2)), c2((size_t)(m + 2));
+    auto Hm = [&](int a, int c) -> double & { return H[(size_t)a * (m + 2) + c]; };
+    int nits = 0;
+    double res = 1.0;                // GMRES_Solve's initial value
+    if (is_zero_host(b_norm)) {
+        hipLaunchKernelGGL(k_zero, dim3(g), dim3(BS), 0, s, n, xx);
+    }
+    else {
+        // z = b - A x when x != 0 (solver-phg.c:2158: phgVecNorm1(x) != 0)
+        hipLaunchKernelGGL((k_gm_resid<View>), dim3(gs), dim3(BS), 0, s, n, mv, va, d,
+                           bb, (const double *)xx, nb[1] != 0.0 ? 1 : 0, z, pr);
+        PHG_HIP_CHECK(hipGetLastError());
+        double zz, nzero;
+        if ((rc = host_partial_sum(pr, gs, 0, s, &zz)) ||
+            (rc = host_partial_sum(pr, gs, 1, s, &nzero)))
+            return rc;
+        double r_norm = sqrt(zz);
+        res = r_norm;
+        double tol = r_norm * rtol;
+        if (tol < b_norm * btol)
+            tol = b_norm * btol;
+        if (tol < atol)
+            tol = atol;
+        if (nzero != 0.0 && maxit > 0 && r_norm > tol)
+            return phg::set_error(PHG_HIP_E_ZERO_DIAG,
+                                  "zero diagonal in Jacobi preconditioner");
+        sv[0] = 1.0;
+        int itr_out = 0;
+        bool first_cycle = true;
+        while (nits < maxit) {
+            const int rs = itr_out <= augk ? mk + itr_out : mk + augk;
+            if (!first_cycle) {
+                if ((rc = host_partial_sum(pr, gs, 0, s, &zz)))
+                    return rc;
+            }
+            first_cycle = false;
+            sv[1] = r_norm = sqrt(zz);
+            // v_1 = z / |z| (vaxpby(1 / r_norm, z, 0, v_1)); flv_1 = v_1 / diag
+            {
+                GmCombo cb{};
+                cb.src[0] = z;
+                cb.c[0] = 1.0 / r_norm;
+                cb.m = 1;
+                hipLaunchKernelGGL(k_gm_combo, dim3(g), dim3(BS), 0, s, n, cb, V(1), 0);
+                hipLaunchKernelGGL(k_gm_pc, dim3(g), dim3(BS), 0, s, n,
+                                   (const double *)V(1), d, FL(1));
+            }
+            int i = 0;
+            std::fill(H.begin(), H.end(), 0.0);
+            while (i < rs && r_norm > tol && nits < maxit) {
+                i++, nits++;
+                if (i > mk)
+                    hipLaunchKernelGGL(k_gm_pc, dim3(g), dim3(BS), 0, s, n,
+                                       (const double *)ZP(i - mk - 1), d, FL(i));
+                double *w = V(i + 1);
+                hipLaunchKernelGGL((k_gm_spmv<View>), dim3(gs), dim3(BS), 0, s, mv, va,
+                                   (const double *)FL(i), w, (const double *)V(1), pp[0]);
+                for (int k = 1; k <= i; k++)
+                    hipLaunchKernelGGL(k_gm_mgs, dim3(g), dim3(BS), 0, s, n, w,
+                                       (const double *)V(k),
+                                       k < i ? (const double *)V(k + 1) : (const double *)nullptr,
+                                       (const double *)pp[(k - 1) & 1], k == 1 ? gs : g,
+                                       pp[k & 1], hcol, k);
+                // the next step's input is v_{i+1} unless it uses an
+                // augmentation vector
+                hipLaunchKernelGGL(k_gm_scale, dim3(g), dim3(BS), 0, s, n, w,
+                                   (const double *)pp[i & 1], g, hcol, i + 1, d,
+                                   i + 1 <= mk ? FL(i + 1) : (double *)nullptr);
+                PHG_HIP_CHECK(hipGetLastError());
+                double *hm = (double *)phg::ctx()->host_mailbox;
+                PHG_HIP_CHECK(hipMemcpyAsync(hm, hcol, (size_t)(i + 2) * sizeof(double),
+                                             hipMemcpyDeviceToHost, s));
+                PHG_HIP_CHECK(hipStreamSynchronize(s));
+                for (int k = 1; k <= i + 1; k++)
+                    Hm(i, k) = hm[k];
+                if (is_zero_host(Hm(i, i + 1))) {
+                    i--;                 // happy breakdown
+                    break;
+                }
+                for (int j = 1; j < i; j++) {
+                    const double h1 = c1[j] * Hm(i, j) + c2[j] * Hm(i, j + 1);
+                    const double h2 = -c2[j] * Hm(i, j) + c1[j] * Hm(i, j + 1);
+                    Hm(i, j) = h1;
+                    Hm(i, j + 1) = h2;
+                }
+                double tmp = sqrt(Hm(i, i) * Hm(i, i) + Hm(i, i + 1) * Hm(i, i + 1));
+                if (is_zero_host(tmp))
+                    tmp = 1.e-16;
+                c1[i] = Hm(i, i) / tmp;
+                c2[i] = Hm(i, i + 1) / tmp;
+                sv[i + 1] = -c2[i] * sv[i];
+                sv[i] = c1[i] * sv[i];
+                Hm(i, i) = c1[i] * Hm(i, i) + c2[i] * Hm(i, i + 1);
+                r_norm = fabs(sv[i + 1]);
+                if (r_norm <= tol)
+                    break;
+            }
+            for (int j = i; j > 0; j--) {
+                yv[j] = sv[j] / Hm(j, j);
+                for (int k = j - 1; k > 0; k--)
+                    sv[k] -= Hm(j, k) * yv[j];
+            }
+            // x += sum_j y_j flv_j (or += z when the cycle made no step)
+            GmCombo cb{};
+            if (i > 0) {
+                for (int j = 1; j <= i; j++) {
+                    cb.src[j - 1] = FL(j);
+                    cb.c[j - 1] = yv[j];
+                }
+                cb.m = i;
+            }
+            else {
+                cb.src[0] = z;
+                cb.c[0] = 1.0;
+                cb.m = 1;
+            }
+            hipLaunchKernelGGL(k_gm_combo, dim3(g), dim3(BS), 0, s, n, cb, xx, 1);
+            // the cycle's correction in the unpreconditioned space becomes
+            // augmentation vector itr_out % augk
+            const int zi = itr_out % augk;
+            GmCombo cz{};
+            if (i > 0) {
+                const int top = i <= mk ? i : mk;
+                for (int j = 1; j <= top; j++) {
+                    cz.src[j - 1] = V(j);
+                    cz.c[j - 1] = yv[j];
+                }
+                cz.m = top;
+                if (i > mk) {
+                    const int ak = itr_out < augk ? itr_out : augk;
+                    for (int j = 1; j <= ak; j++) {
+                        cz.src[cz.m] = ZP(j - 1);
+                        cz.c[cz.m] = yv[j + mk];
+                        cz.m++;
+                    }
+                }
+            }
+            else {
+                cz.src[0] = z;
+                cz.c[0] = 1.0;
+                cz.m = 1;
+            }
+            // rows are read before they are written, so zp_zi may be a
+            // source; the result goes to z first, then to zp_zi
+            hipLaunchKernelGGL(k_gm_combo, dim3(g), dim3(BS), 0, s, n, cz, z, 0);
+            hipLaunchKernelGGL(k_copy, dim3(g), dim3(BS), 0, s, n, (const double *)z,
+                               ZP(zi));
+            PHG_HIP_CHECK(hipGetLastError());
+            if (r_norm <= tol) {
+                res = r_norm;
+                break;
+            }
+            hipLaunchKernelGGL((k_gm_resid<View>), dim3(gs), dim3(BS), 0, s, n, mv, va, d,
+                               bb, (const double *)xx, 1, z, pr);
+            PHG_HIP_CHECK(hipGetLastError());
+            itr_out++;
+        }
+    }
+    if (perm)
+        hipLaunchKernelGGL(k_scatter, dim3(g), dim3(BS), 0, s, n, perm,
+                           (const double *)ws->xp, x->d);
+    PHG_HIP_CHECK(hipGetLastError());
+    PHG_HIP_CHECK(hipStreamSynchronize(s));
+    if (residual)
+        *residual = res;
+    return nits;
 }
 
 // FP64 PCG in the numbering of the format
@@ -3118,6 +3501,23 @@ extern "C" int phg_hip_bicgstab(const phg_hip_mat *A_, const phg_hip_vec *b,
     return with_format(A->fmt64, [&](auto tag) {
         using View = decltype(tag);
         return bicgstab_impl<View>(A, b, x, maxit, rtol, btol, atol, residual);
+    });
+}
+
+extern "C" int phg_hip_gmres(const phg_hip_mat *A_, const phg_hip_vec *b,
+                             phg_hip_vec *x, int32_t restart, int32_t augk,
+                             int32_t maxit, double rtol, double btol, double atol,
+                             double *residual)
+{
+    int rc;
+    if ((rc = check_solve_args(A_, b, x)))
+        return rc;
+    phg_hip_mat *A = const_cast<phg_hip_mat *>(A_);
+    if ((rc = ensure_format(A, A->fmt64, 0)))
+        return rc;
+    return with_format(A->fmt64, [&](auto tag) {
+        using View = decltype(tag);
+        return gmres_impl<View>(A, b, x, restart, augk, maxit, rtol, btol, atol, residual);
     });
 }
 
