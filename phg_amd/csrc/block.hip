@@ -50,14 +50,6 @@ __device__ inline int64_t xcd_block()
     return x * q + (x < rem ? x : rem) + k;
 }
 
-// contiguous range of [0, n) for this work-group
-__device__ inline void group_range(int64_t n, int64_t &a, int64_t &b)
-{
-    int64_t g = xcd_block(), ng = gridDim.x;
-    int64_t per = (n + ng - 1) / ng;
-    a = g * per;
-    b = a + per < n ? a + per : n;
-}
 
 //------------------------------------------------------------- BSR3 build
 // block row R = scalar rows 3R..3R+2; blocks in the order their columns
@@ -166,9 +158,9 @@ __device__ inline void bsr_spmm(const BsrView &m, const T *__restrict__ va,
     static_assert(LPR <= 64 && 64 % LPR == 0, "wavefront shape");
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int cg = lane % NCG, s = (lane / NCG) % S, rw = lane / LPR;
-    int64_t g0, g1;
-    group_range((m.nb + RPG - 1) / RPG, g0, g1);
-    for (int64_t gi = g0; gi < g1; gi++) {
+    int64_t g0, g1, gstep;
+    phg::xcd_sweep((m.nb + RPG - 1) / RPG, g0, g1, gstep);
+    for (int64_t gi = g0; gi < g1; gi += gstep) {
         const int64_t R = gi * RPG + wv * RPW + rw;
         T acc[B][CPL];
 #pragma unroll

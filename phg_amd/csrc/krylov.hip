@@ -536,29 +536,6 @@ __device__ inline void stage_fetch(const DictView &m, const T *__restrict__ va,
     }
 }
 
-// Blocks of one XCD swept together: the blocks are split into NXCD equal
-// regions, region k processed by the work-groups the dispatcher places on
-// XCD k (blockIdx % NXCD == k, as xcd_block assumes), work-group j of the
-// XCD taking blocks lo + j, lo + j + w, ... (w work-groups on the XCD).
-// At any time an XCD streams one window of consecutive blocks (DRAM page
-// locality, as a copy kernel), and the x entries a region gathers stay in
-// that XCD's L2.
-__device__ inline void xcd_sweep(int64_t n, int64_t &first, int64_t &end,
-                                 int64_t &step)
-{
-    const int64_t nb = gridDim.x, b = blockIdx.x;
-    if (nb < NXCD) {
-        first = b;
-        end = n;
-        step = nb;
-        return;
-    }
-    const int64_t x = b % NXCD, j = b / NXCD;
-    step = nb / NXCD + (x < nb % NXCD ? 1 : 0);
-    first = n * x / NXCD + j;
-    end = n * (x + 1) / NXCD;
-}
-
 // block metadata of up to DICT_META blocks of a sweep, staged in LDS
 struct SweepMeta {
     int32_t f0[DICT_META], f1[DICT_META], q0[DICT_META], q1[DICT_META];
@@ -627,7 +604,7 @@ __device__ inline void dict_spmv(const DictView &m, const T *__restrict__ va,
     __shared__ int32_t dsh[256];
     __shared__ SweepMeta md;
     int64_t first, end, step;
-    xcd_sweep(m.nblk, first, end, step);
+    phg::xcd_sweep(m.nblk, first, end, step);
     const int tid = threadIdx.x;
     for (int64_t c0 = first; c0 < end; c0 += step * DICT_META) {
         const int nb = sweep_meta(m, c0, end, step, md);
@@ -708,7 +685,7 @@ __device__ inline void xs_spmv(const XsView &m, const T *__restrict__ va,
     __shared__ SweepMeta md;
     const DictView &dv = reinterpret_cast<const DictView &>(m);
     int64_t first, end, step;
-    xcd_sweep(m.nblk, first, end, step);
+    phg::xcd_sweep(m.nblk, first, end, step);
     const int tid = threadIdx.x;
     for (int64_t c0 = first; c0 < end; c0 += step * DICT_META) {
         const int nb = sweep_meta(dv, c0, end, step, md);
@@ -793,7 +770,7 @@ __device__ inline void eop_gather(const EopGather &m, const float *__restrict__ 
     static_assert(phg::EOP_GATHER_ROWS <= BS, "one work-item per row");
     __shared__ float prod[phg::EOP_GATHER_NNZ];
     int64_t first, end, step;
-    xcd_sweep(m.nblk, first, end, step);
+    phg::xcd_sweep(m.nblk, first, end, step);
     const int tid = threadIdx.x;
     for (int64_t b = first; b < end; b += step) {
         const int32_t r0 = m.blk[b], r1 = m.blk[b + 1];

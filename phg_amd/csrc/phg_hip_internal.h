@@ -213,6 +213,28 @@ int build_xstage(phg_hip_mat *A);   // A->xs
 void free_dict_layout(DictLayout *L);
 void free_dict(phg_hip_mat *A);     // both
 int setup_diagonal(phg_hip_mat *A);
+// Rows (or blocks) of one XCD swept together: [0, n) is split into 8 equal
+// regions, region k processed by the work-groups the dispatcher places on
+// XCD k (blockIdx % 8 == k), work-group j of the XCD taking items
+// lo + j, lo + j + w, ... (w work-groups on the XCD).  At any time an XCD
+// streams one window of consecutive items (DRAM page locality, as a copy
+// kernel), and the x entries a region gathers stay in that XCD's L2.
+__device__ inline void xcd_sweep(int64_t n, int64_t &first, int64_t &end,
+                                 int64_t &step)
+{
+    constexpr int64_t kXcd = 8;
+    const int64_t nb = gridDim.x, b = blockIdx.x;
+    if (nb < kXcd) {
+        first = b;
+        end = n;
+        step = nb;
+        return;
+    }
+    const int64_t x = b % kXcd, j = b / kXcd;
+    step = nb / kXcd + (x < nb % kXcd ? 1 : 0);
+    first = n * x / kXcd + j;
+    end = n * (x + 1) / kXcd;
+}
 // matrix-free operator (eop.hip; reference data from assemble.hip)
 constexpr int EOP_GATHER_NNZ = 2048, EOP_GATHER_ROWS = 256;
 void eop_reference_tensors(int p, double *T);
