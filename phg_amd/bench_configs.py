@@ -177,6 +177,46 @@ def c4(seq=16):
             "spmm_bytes": [b64, b32]}
 
 
+def solvers(n=32, p=2, rtol=1e-8):
+    """PHG's Krylov solvers on one kappa-Poisson system (Jacobi PC, FP64,
+    x0 = 0, rtol = btol): PCG, BiCGstab and the default GMRES (flgmres_r,
+    restart 10, augk 2)."""
+    import phg_amd as pa
+    g = pa.Grid.kuhn(n)
+    nd = g.ndof(p)
+    rng = np.random.default_rng(2024)
+    kappa = np.exp(rng.normal(0.0, 0.5, size=g.nelem))
+    sp = pa.Space(g, p)
+    A = pa.assemble(sp, pa.Problem.poisson(use_kappa=True), kappa=kappa, f=np.ones(nd),
+                    g=np.zeros(nd))
+    A.eliminate()
+    A.tune(reps=10)
+    b = A.rhs()
+    out = {"config": "solvers",
+           "workload": f"P{p} kappa-Poisson {n}^3x6 ({A.n} DOFs), Jacobi PC, rtol {rtol}",
+           "fp64_layout": A.format()[0]}
+    xs = {}
+    for name, fn in (("pcg", lambda x: pa.pcg(A, b, x, rtol=rtol, btol=rtol)),
+                     ("bicgstab", lambda x: pa.bicgstab(A, b, x, rtol=rtol, btol=rtol)),
+                     ("gmres", lambda x: pa.gmres(A, b, x, maxit=20000, rtol=rtol,
+                                                  btol=rtol))):
+        x = pa.Vec(A.n)
+
+        def run(x=x, fn=fn):
+            x.axpby(0.0, x, 0.0)
+            return fn(x)
+        t, (nits, res) = _timed(run, reps=2)
+        xs[name] = x.numpy()
+        out[name] = {"s": round(t, 4), "its": int(nits), "residual": res,
+                     "us_per_iteration": round(t / max(nits, 1) * 1e6, 1),
+                     "dofs_per_s": round(A.n / t, 1)}
+    ref = xs["pcg"]
+    for name in ("bicgstab", "gmres"):
+        out[name]["rel_diff_vs_pcg"] = float(np.linalg.norm(xs[name] - ref) /
+                                             np.linalg.norm(ref))
+    return out
+
+
 def main(argv):
     import phg_amd as pa
     pa.init(0)
