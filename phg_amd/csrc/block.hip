@@ -169,24 +169,42 @@ __device__ inline void bsr_spmm(const BsrView &m, const T *__restrict__ va,
             for (int i = 0; i < CPL; i++)
                 acc[p][i] = 0;
         if (R < m.nb) {
-            const int32_t j1 = m.bptr[R + 1];
-#pragma unroll 2
-            for (int32_t j = m.bptr[R] + s; j < j1; j += S) {
-                const int64_t C = m.bcol[j];
-                Cols<T, CPL> xq[B];
+            // the row's block columns, loaded once (coalesced) by the
+            // row's lanes and handed out by shuffles, so the X gathers of
+            // consecutive blocks do not wait on a column load each
+            const int32_t j0 = m.bptr[R], nblk = m.bptr[R + 1] - j0;
+            const int base = lane - lane % LPR, lr = lane % LPR;
+            constexpr int NT = BSR_MAXB / LPR;
+            int32_t colt[NT];
 #pragma unroll
-                for (int q = 0; q < B; q++)
-                    xq[q] = load_cols<T, CPL>(X + (C * B + q) * KB + cg * CPL);
-                const T *a = va + (int64_t)j * (B * B);
+            for (int t = 0; t < NT; t++)
+                colt[t] = t * LPR + lr < nblk ? m.bcol[j0 + t * LPR + lr] : 0;
+            // slot s takes blocks s, s + S, ... (t * LPR + s + S v)
 #pragma unroll
-                for (int p = 0; p < B; p++)
+            for (int t = 0; t < NT; t++) {
+                if (t * LPR >= nblk)
+                    break;
 #pragma unroll
-                    for (int q = 0; q < B; q++) {
-                        const T apq = a[p * B + q];
+                for (int v = 0; v < LPR / S; v++) {
+                    const int jj = t * LPR + s + S * v;
+                    const int64_t C = __shfl(colt[t], base + s + S * v, 64);
+                    if (jj < nblk) {
+                        Cols<T, CPL> xq[B];
 #pragma unroll
-                        for (int i = 0; i < CPL; i++)
-                            acc[p][i] += apq * xq[q].v[i];
+                        for (int q = 0; q < B; q++)
+                            xq[q] = load_cols<T, CPL>(X + (C * B + q) * KB + cg * CPL);
+                        const T *a = va + (int64_t)(j0 + jj) * (B * B);
+#pragma unroll
+                        for (int p = 0; p < B; p++)
+#pragma unroll
+                            for (int q = 0; q < B; q++) {
+                                const T apq = a[p * B + q];
+#pragma unroll
+                                for (int i = 0; i < CPL; i++)
+                                    acc[p][i] += apq * xq[q].v[i];
+                            }
                     }
+                }
             }
         }
 #pragma unroll
