@@ -235,8 +235,11 @@ int phg_hip_pcg_multi(int32_t nmat, phg_hip_mat *const *A,
  * grad lambda_b (a <= b in 1..3), and applies K_e = sum_m c_m T^m (the stiffness of
  * quad.c:5260 phgQuadGradBasAGradBas, factored through fixed reference
  * tensors T^m) to x on the matrix cores (bf16 MFMA, operands split into
- * three bf16 parts so products keep FP32 precision), then sums each row
- * over its elements.  Dirichlet rows and columns are treated as
+ * three bf16 parts so products keep FP32 precision).  Elements are taken in
+ * blocks of consecutive elements (Morton order); each row's sum over its
+ * elements runs in a fixed order (in LDS within a block, then over blocks
+ * for rows shared between blocks), so results are deterministic.  Dirichlet
+ * rows and columns are treated as
  * phg_hip_mat_eliminate leaves them.  The FP64 residuals of the solvers keep
  * using the assembled matrix.  After the call the solvers use the operator;
  * phg_hip_mat_use_element_operator(A, 0 / 1) switches it off / on. */
@@ -247,15 +250,19 @@ int phg_hip_mat_use_element_operator(phg_hip_mat *A, int32_t on);
  * numbering); for checks against phg_hip_matvec. */
 int phg_hip_element_apply(const phg_hip_mat *A, const phg_hip_vec *x,
                           phg_hip_vec *y);
-/* Algorithmic bytes of one application: element kernel (map, coefficients,
- * element results, x) and row gather (incidence codes, element results,
- * row offsets, flags, p, Ap). */
+/* Algorithmic bytes of one application: block kernel (per element packed
+ * indices and coefficients; per touched row its index, slot start, x and
+ * partial slot; interior rows of y, partials of shared rows) and fix-up
+ * (shared rows from their partials, Dirichlet rows). */
 int phg_hip_element_bytes(const phg_hip_mat *A, int64_t *elem_bytes,
                           int64_t *gather_bytes);
-/* Average time (us) of `reps` launches of the element kernel and of the
- * row gather, each measured with HIP events on the library stream. */
+/* Average time (us) of `reps` launches of the block kernel and of the
+ * fix-up kernel, each measured with HIP events on the library stream. */
 int phg_hip_element_time(const phg_hip_mat *A, int32_t reps, double *elem_us,
                          double *gather_us);
+/* Block layout of the operator: out[0] blocks, [1] touched-row entries,
+ * [2] interior rows, [3] shared rows, [4] partial sums, [5] Dirichlet rows. */
+int phg_hip_element_layout(const phg_hip_mat *A, int64_t out[6]);
 
 /*------------------------------------------------ many right-hand sides */
 

@@ -146,6 +146,7 @@ _sig(hip_lib, "phg_hip_mat_use_element_operator", C.c_int, [_vp, _i32])
 _sig(hip_lib, "phg_hip_element_apply", C.c_int, [_vp, _vp, _vp])
 _sig(hip_lib, "phg_hip_element_bytes", C.c_int, [_vp, _P(_i64), _P(_i64)])
 _sig(hip_lib, "phg_hip_element_time", C.c_int, [_vp, _i32, _P(_dbl), _P(_dbl)])
+_sig(hip_lib, "phg_hip_element_layout", C.c_int, [_vp, _P(_i64)])
 _sig(hip_lib, "phg_hip_spmm_tune", C.c_int, [_vp, _i32])
 _sig(hip_lib, "phg_hip_spmm_tune_time", C.c_int, [_vp, _i32, _i32, _P(_dbl)])
 _sig(hip_lib, "phg_hip_spmm_bytes", C.c_int, [_vp, _i32, _P(_i64)])
@@ -450,6 +451,14 @@ class Mat:
         _check(hip_lib.phg_hip_element_bytes(self._h, C.byref(a), C.byref(b)),
                "element_bytes")
         return a.value, b.value
+
+    def element_layout(self):
+        """Block layout of the matrix-free operator: blocks, touched-row
+        entries, interior rows, shared rows, partial sums, Dirichlet rows."""
+        out = (_i64 * 6)()
+        _check(hip_lib.phg_hip_element_layout(self._h, out), "element_layout")
+        return {k: int(v) for k, v in zip(
+            ("blocks", "entries", "interior", "shared", "partials", "dirichlet"), out)}
 
     def element_time(self, reps: int = 20):
         """(element kernel us, row gather us) per launch."""

@@ -1,5 +1,5 @@
 // Matrix-free FP32 operator of a scalar stiffness matrix, element products on
-// the matrix cores.
+// the matrix cores, row sums in LDS.
 //
 // For an affine tetrahedron the stiffness of quad.c:5260
 // (phgQuadGradBasAGradBas with a P0 coefficient) factors as
@@ -20,12 +20,21 @@
 // fragments (T parts) of all tiles stay in LDS for the kernel, the B
 // fragments (x parts) are written to LDS per 16 elements.
 //
-// The per-row sums y_i = sum over the row's incident (element, local index)
-// pairs, in element order, run in krylov.hip (k_eop_gather) with the
-// solver's epilogue.  Against the assembled FP32 CSR the operator moves
-// map + coefficients + element results (P3: 80 + 24 + 2 x 80 B per element
-// plus 4 B per incidence) instead of about 1.4 KB of matrix entries per
-// element.
+// Element blocks.  The elements, in Morton order of their centroids, are cut
+// into blocks of at most EBLK elements touching at most KMAX rows.  A
+// work-group takes a block at a time:
+//   1. x of the block's touched rows into LDS (one gather per row);
+//   2. the element products, each element-local result stored into an LDS
+//      slot -- the slots of a row are consecutive and in element order;
+//   3. one work-item per touched row sums its slots in that order.  A row
+//      touched by this block only (interior) is written to y directly; a
+//      row shared with other blocks writes a partial, and k_eop_fix sums the
+//      partials of each shared row in block order and sets the Dirichlet
+//      rows (diag * x).
+// Every sum has a fixed order, so the operator is deterministic.  Per
+// element the kernel streams 4 B per local index (touched-row index and slot,
+// packed) and 24 B of coefficients, about 1/3 of the FP32 CSR entries the
+// element contributes (P2: 64 B against ~240 B).
 #include "phg_hip_internal.h"
 
 #include <string.h>
@@ -42,12 +51,25 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int EB = 512;           // work-items per work-group (8 wavefronts)
 constexpr int EW = EB / 64;
 constexpr int NC = 6;             // coefficients per element
+constexpr int FB = 1024;          // work-items of the fix-up kernel
+constexpr int FI = 1;             // its items per work-item and sweep
+constexpr uint32_t KD_NONE = 0xffffffffu;
 
 template <int P> struct EopShape {
     static constexpr int NB = P == 1 ? 4 : (P == 2 ? 10 : 20);
     static constexpr int NT = NB / 2;                 // output tiles
     static constexpr int KS = (6 * NB + 31) / 32;     // K steps of 32
     static constexpr int KP = KS * 32 + 8;            // B row stride (bank skew)
+    // per lane: x of NJ local indices j, results of NO local indices i;
+    // their 16-bit indices packed into a chunk of W dwords
+    static constexpr int NJ = (NB + 3) / 4;
+    static constexpr int NO = (NT + 1) / 2;
+    static constexpr int W = (NJ + NO + 1) / 2;
+    // elements and touched rows per block; slots per block (< 2^16)
+    static constexpr int EBLK = P == 1 ? 2048 : (P == 2 ? 1024 : 512);
+    static constexpr int KMAX = P == 1 ? 1024 : (P == 2 ? 2048 : 3072);
+    static constexpr int SMAX = EBLK * NB;
+    static_assert(SMAX < 65535 && KMAX < 65535, "16-bit slot and row indices");
 };
 
 // A fragment of (tile t, K step s) for lane l: row R = l % 16 is slot
@@ -59,120 +81,437 @@ constexpr int kXPart[6] = {0, 1, 0, 2, 1, 0};
 
 // element tiles of 16 a wavefront takes per round: their gathers are
 // issued together and every A fragment read from LDS feeds ET products
-template <int P> constexpr int kEt = P == 3 ? 2 : 4;
+template <int P> constexpr int kEt = P == 1 ? 2 : 1;
 
+// P3 runs the bf16 split products above; P1 / P2 run FP32 MFMA
+// (v_mfma_f32_16x16x4f32): K = NB local indices in steps of 4, the A
+// fragments (one T value per lane and step) in registers and the B
+// fragments straight from the lane's gathered x, no split and no LDS
+// staging.  Lane layout: A[l % 16][l / 16], B[l / 16][l % 16], D rows
+// 4 (l / 16) + r of column l % 16.
+template <int P> constexpr bool kSplit = P == 3;
+
+struct BlockView {
+    int64_t nblk;
+    const int32_t *eb_elem;   // [nblk + 1] element ranges
+    const int32_t *eb_row;    // [nblk + 1] touched-row ranges
+    const int32_t *eb_nint;   // [nblk] interior rows (first in the list)
+    const int32_t *rows;      // [nent] solver rows
+    const uint16_t *rs;       // [nent + nblk] slot starts, block b at eb_row[b] + b
+    const int32_t *ppos;      // [nent] partial position of a shared row
+    const uint32_t *idx;      // [nelem][4][W] index chunks of the four lane groups
+    const float *coef;        // [nelem][6]
+    const void *tiles;        // A fragments: bf16 (split path) or FP32
+    float *part;              // partials of shared rows
+};
+
+// work-group total of one double per work-item, to out[blockIdx.x]
+template <int NTH>
+__device__ inline void group_total(double v, double *out)
+{
+    __shared__ double sh[NTH / 64];
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o, 64);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        sh[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < NTH / 64; k++)
+            s += sh[k];
+        out[blockIdx.x] = s;
+    }
+}
+
+// the wavefront's own LDS rows written by some lanes and read by others:
+// LDS operations of one wavefront take effect in issue order, so only the
+// compiler has to be kept from moving them across this point
+__device__ inline void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// element of index idx of an array under 4 GiB: a 32-bit byte offset from
+// the base, so the load takes the base in scalar registers
+template <typename T>
+__device__ inline T ld32(const T *base, uint32_t idx)
+{
+    return *(const T *)((const char *)base + idx * (uint32_t)sizeof(T));
+}
+
+// Lane l = 16 g + n of a wavefront works on element n of a tile.  It
+// gathers x of the local indices j = NJ g + u (split path; the B fragment K
+// positions of j are filled through LDS) or j = 4 u + g (FP32 path: K step
+// u), u < NJ, and stores the results of the local
+// indices i = 2 t + g / 2 with t = g mod 2 (the D rows it holds after the
+// contraction), t = 2 o + g % 2 (o < NO).  Its chunk of the element's index
+// record holds, as 16-bit values, the block-local row of each j and the
+// block-local slot of each i (0xffff: Dirichlet / none).
+//
+// One round = ET element tiles.  The loads are branch-free (element clamped
+// into the block) and hold raw values; elements past the block are masked
+// where the results are stored, so the loads of a round stay in flight
+// while the previous round computes.
+template <int ET, int W> struct Round {
+    uint32_t ch[ET][W];
+    float cf[ET][4];
+};
+
+template <int NB, int W, int ET>
+__device__ inline void load_round(const BlockView &m, int32_t e0, int32_t ne, int w, int n,
+                                  int g, int rd, Round<ET, W> &r)
+{
+    // 32-bit offsets: nelem * nb < 2^30 is checked at set-up
+#pragma unroll
+    for (int et = 0; et < ET; et++) {
+        const int el = ((rd * EW + w) * ET + et) * 16 + n;
+        const uint32_t e = (uint32_t)(e0 + min(el, ne - 1));
+        const uint32_t base = (e * 4 + (uint32_t)g) * W;
+#pragma unroll
+        for (int u = 0; u < W; u++)
+            r.ch[et][u] = ld32(m.idx, base + u);
+        // coefficients m = 4 (g % 2) + q; for odd g the slots 6, 7 read the
+        // next element's (finite: the array has two entries of padding) and
+        // meet all-zero rows of the T tiles
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            r.cf[et][q] = ld32(m.coef, e * NC + 4 * (g & 1) + q);
+    }
+}
+
+// 16-bit entry h of a lane's index chunk
+template <int W>
+__device__ inline uint32_t chunk_half(const uint32_t (&ch)[W], int h)
+{
+    return (ch[h >> 1] >> (16 * (h & 1))) & 0xffffu;
+}
+
+// wavefronts per SIMD the block kernel is compiled for: three work-groups
+// per CU for P2, two for P1 (LDS and VGPRs both fit), one for P3
+template <int P> constexpr int kWaves = P == 3 ? 2 : (P == 2 ? 6 : 4);
+
+// y = A x over the element blocks: interior rows of y, partials of shared
+// rows, the work-group's part of (x, y) over interior rows in part_out
 template <int P>
-__global__ void __launch_bounds__(EB)
-k_eop_elem(int64_t nelem, const int32_t *__restrict__ map,
-           const float *__restrict__ coef, const uint4 *__restrict__ tiles,
-           const float *__restrict__ x, float *__restrict__ y,
-           const int32_t *done, const int32_t *mode, int32_t want)
+__global__ void __launch_bounds__(EB, kWaves<P>)
+k_eop_block(BlockView m, const float *__restrict__ x, float *__restrict__ y,
+            double *part_out, const int32_t *done, const int32_t *mode, int32_t want)
 {
     using S = EopShape<P>;
     constexpr int NB = S::NB, NT = S::NT, KS = S::KS, KP = S::KP, ET = kEt<P>;
-    constexpr int NJ = (NB + 3) / 4;      // local indices j per lane
+    constexpr int NJ = S::NJ, NO = S::NO, W = S::W;
+    constexpr int RPT = (S::KMAX + EB - 1) / EB;   // touched rows per work-item
     if (done != nullptr && (*done != 0 || *mode != want))
         return;
-    __shared__ uint4 at[NT * KS * 64];
-    __shared__ __align__(16) __bf16 bt[EW][16][KP];
+    constexpr bool SPLIT = kSplit<P>;
+    __shared__ uint4 at[SPLIT ? NT * KS * 64 : 1];
+    __shared__ __align__(16) __bf16 bt[SPLIT ? EW : 1][16][KP];
+    __shared__ float xs[S::KMAX + 1];
+    __shared__ float slot[S::SMAX + 1];
     const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
     const int n = l & 15, g = l >> 4;
-    for (int t = tid; t < NT * KS * 64; t += EB)
-        at[t] = tiles[t];
-    // K padding of the B rows stays zero
-    constexpr int PADW = KP - 6 * NB;
-    for (int t = l; t < 16 * PADW; t += 64)
-        bt[w][t / PADW][6 * NB + t % PADW] = (__bf16)0.0f;
-    const int64_t ntile = (nelem + 15) / 16;
-    // every wavefront of the work-group runs the same number of rounds, so
-    // the work-group barriers below are reached by all of them
-    for (int64_t base = (int64_t)blockIdx.x * EW * ET; base < ntile;
-         base += (int64_t)gridDim.x * EW * ET) {
-        const int64_t tile0 = base + (int64_t)w * ET;
-        // gathers of the ET tiles: x of local indices j = g, g + 4, ... and
-        // the coefficients of the D rows 4 g + r (slots m = 4 (g % 2) + r)
-        float xv[ET][NJ], c[ET][4];
+    float af[SPLIT ? 1 : NT][SPLIT ? 1 : NJ];
+    if constexpr (SPLIT) {
+        for (int t = tid; t < NT * KS * 64; t += EB)
+            at[t] = ((const uint4 *)m.tiles)[t];
+        // K padding of the B rows stays zero
+        constexpr int PADW = KP - 6 * NB;
+        for (int t = l; t < 16 * PADW; t += 64)
+            bt[w][t / PADW][6 * NB + t % PADW] = (__bf16)0.0f;
+    }
+    else {
 #pragma unroll
-        for (int et = 0; et < ET; et++) {
-            const int64_t e = (tile0 + et) * 16 + n;
-            const bool ok = tile0 + et < ntile && e < nelem;
+        for (int t = 0; t < NT; t++)
 #pragma unroll
-            for (int jj = 0; jj < NJ; jj++) {
-                const int j = 4 * jj + g;
-                int32_t col = -1;
-                if (ok && j < NB)
-                    col = map[e * NB + j];
-                xv[et][jj] = col >= 0 ? x[col] : 0.0f;
-            }
+            for (int u = 0; u < NJ; u++)
+                af[t][u] = ((const float *)m.tiles)[(t * NJ + u) * 64 + l];
+    }
+    if (tid == 0)
+        xs[S::KMAX] = 0.0f;     // x of Dirichlet and absent local indices
+    double dot = 0.0;
+    int64_t first, end, step;
+    phg::xcd_sweep(m.nblk, first, end, step);
+    // block header; touched-row loads of a block (index clamped, the row
+    // tables carry one entry of padding; raw values, masked where used)
+    auto header = [&](int64_t bb, int32_t &k0, int32_t &nk, int32_t &nint, int32_t &e0,
+                      int32_t &ne) {
+        k0 = m.eb_row[bb];
+        nk = m.eb_row[bb + 1] - k0;
+        nint = m.eb_nint[bb];
+        e0 = m.eb_elem[bb];
+        ne = m.eb_elem[bb + 1] - e0;
+    };
+    auto row_loads = [&](int64_t bb, int32_t k0, int32_t nk, int32_t (&rw)[RPT],
+                         uint32_t (&qr)[RPT]) {
+        const int kc = max(nk - 1, 0);
+        const uint16_t *rs = m.rs + k0 + bb;
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int m = 4 * (g & 1) + r;
-                c[et][r] = (ok && m < NC) ? coef[e * NC + m] : 0.0f;
-            }
+        for (int i = 0; i < RPT; i++) {
+            const int k = min(tid + EB * i, kc);
+            rw[i] = m.rows[k0 + k];
+            qr[i] = (uint32_t)rs[k] | (uint32_t)rs[k + 1] << 16;
         }
-        // split into bf16 parts through the wavefront's LDS rows, one tile
-        // at a time, into B fragments
-        bf16x8 b[ET][KS];
+    };
+    if (first < end) {
+        // software pipeline over the work-group's blocks: while block b
+        // computes, the touched rows, x values and first element round of
+        // the next block are already loading
+        int64_t b = first;
+        int32_t k0, nk, nint, e0, ne;
+        header(b, k0, nk, nint, e0, ne);
+        int32_t rw[RPT];
+        uint32_t qr[RPT];
+        float xr[RPT];
+        row_loads(b, k0, nk, rw, qr);
 #pragma unroll
-        for (int et = 0; et < ET; et++) {
-            __syncthreads();    // the previous fragments are read
-#pragma unroll
-            for (int jj = 0; jj < NJ; jj++) {
-                const int j = 4 * jj + g;
-                if (j < NB) {
-                    const float v = xv[et][jj];
-                    const __bf16 x0 = (__bf16)v;
-                    const float r1 = v - (float)x0;
-                    const __bf16 x1 = (__bf16)r1;
-                    const __bf16 x2 = (__bf16)(r1 - (float)x1);
-                    const __bf16 xp[3] = {x0, x1, x2};
-#pragma unroll
-                    for (int q = 0; q < 6; q++)
-                        bt[w][n][q * NB + j] = xp[kXPart[q]];
-                }
-            }
-            __syncthreads();
-#pragma unroll
-            for (int s2 = 0; s2 < KS; s2++)
-                b[et][s2] = *(const bf16x8 *)&bt[w][n][32 * s2 + 8 * g];
-        }
-        float yv[ET][(NT + 1) / 2];
-#pragma unroll
-        for (int t = 0; t < NT; t++) {
-            f32x4 acc[ET];
-#pragma unroll
-            for (int et = 0; et < ET; et++)
-                acc[et] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-            for (int s2 = 0; s2 < KS; s2++) {
-                const bf16x8 a = __builtin_bit_cast(bf16x8, at[(t * KS + s2) * 64 + l]);
+        for (int i = 0; i < RPT; i++)
+            xr[i] = x[rw[i]];
+        Round<ET, W> ra, rb;
+        load_round<NB, W>(m, e0, ne, w, n, g, 0, ra);
+        // one round: x from the block's rows, B fragments, products, slots
+        auto compute = [&](int rd, Round<ET, W> &cur) {
+            float yv[ET][NO];
+            if constexpr (SPLIT) {
+                // x of the lane's local indices j = NJ g + u (Dirichlet and
+                // none: row KMAX of xs, which holds 0)
+                float xv[ET][NJ];
 #pragma unroll
                 for (int et = 0; et < ET; et++)
-                    acc[et] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[et][s2], acc[et],
-                                                                      0, 0, 0);
+#pragma unroll
+                    for (int u = 0; u < NJ; u++)
+                        xv[et][u] = xs[min(chunk_half(cur.ch[et], u), (uint32_t)S::KMAX)];
+                // split into bf16 parts through the wavefront's LDS rows, one
+                // tile at a time, into B fragments
+                bf16x8 bf[ET][KS];
+#pragma unroll
+                for (int et = 0; et < ET; et++) {
+                    wave_lds_sync();    // the previous fragments are read
+#pragma unroll
+                    for (int u = 0; u < NJ; u++) {
+                        const int j = NJ * g + u;
+                        const float v = xv[et][u];
+                        const __bf16 x0 = (__bf16)v;
+                        const float r1 = v - (float)x0;
+                        const __bf16 x1 = (__bf16)r1;
+                        const __bf16 x2 = (__bf16)(r1 - (float)x1);
+                        const __bf16 xp[3] = {x0, x1, x2};
+                        // j past NB (its x is 0) goes to the zero K padding
+#pragma unroll
+                        for (int q = 0; q < 6; q++)
+                            bt[w][n][j < NB ? q * NB + j : 6 * NB] = xp[kXPart[q]];
+                    }
+                    wave_lds_sync();
+#pragma unroll
+                    for (int s2 = 0; s2 < KS; s2++)
+                        bf[et][s2] = *(const bf16x8 *)&bt[w][n][32 * s2 + 8 * g];
+                }
+#pragma unroll
+                for (int t = 0; t < NT; t++) {
+                    f32x4 acc[ET];
+#pragma unroll
+                    for (int et = 0; et < ET; et++)
+                        acc[et] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                    for (int s2 = 0; s2 < KS; s2++) {
+                        const bf16x8 a = __builtin_bit_cast(bf16x8, at[(t * KS + s2) * 64 + l]);
+#pragma unroll
+                        for (int et = 0; et < ET; et++)
+                            acc[et] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bf[et][s2],
+                                                                              acc[et], 0, 0, 0);
+                    }
+                    // contract with c_m, then add the two row groups of local
+                    // index 2 t + g / 2 (lanes g and g ^ 1)
+#pragma unroll
+                    for (int et = 0; et < ET; et++) {
+                        float v = cur.cf[et][0] * acc[et][0] + cur.cf[et][1] * acc[et][1] +
+                                  cur.cf[et][2] * acc[et][2] + cur.cf[et][3] * acc[et][3];
+                        v += __shfl_xor(v, 16, 64);
+                        if ((t & 1) == (g & 1))
+                            yv[et][t >> 1] = v;
+                    }
+                }
             }
-            // contract with c_m, then add the two row groups of local index
-            // 2 t + g / 2 (lanes g and g ^ 1)
+            else {
+                // x of the lane's local indices j = 4 u + g (Dirichlet and
+                // none: row KMAX of xs, which holds 0) are the B fragments
+                float xv[ET][NJ];
+#pragma unroll
+                for (int et = 0; et < ET; et++)
+#pragma unroll
+                    for (int u = 0; u < NJ; u++)
+                        xv[et][u] = xs[min(chunk_half(cur.ch[et], u), (uint32_t)S::KMAX)];
+#pragma unroll
+                for (int t = 0; t < NT; t++) {
+                    f32x4 acc[ET];
+#pragma unroll
+                    for (int et = 0; et < ET; et++)
+                        acc[et] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                    for (int u = 0; u < NJ; u++)
+#pragma unroll
+                        for (int et = 0; et < ET; et++)
+                            acc[et] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[t][u], xv[et][u],
+                                                                           acc[et], 0, 0, 0);
+#pragma unroll
+                    for (int et = 0; et < ET; et++) {
+                        float v = cur.cf[et][0] * acc[et][0] + cur.cf[et][1] * acc[et][1] +
+                                  cur.cf[et][2] * acc[et][2] + cur.cf[et][3] * acc[et][3];
+                        v += __shfl_xor(v, 16, 64);
+                        if ((t & 1) == (g & 1))
+                            yv[et][t >> 1] = v;
+                    }
+                }
+            }
+            // element results into their slots; slot SMAX takes the
+            // results nobody reads (none, Dirichlet, elements past the block)
 #pragma unroll
             for (int et = 0; et < ET; et++) {
-                float v = c[et][0] * acc[et][0] + c[et][1] * acc[et][1] +
-                          c[et][2] * acc[et][2] + c[et][3] * acc[et][3];
-                v += __shfl_xor(v, 16, 64);
-                if ((t & 1) == (g & 1))
-                    yv[et][t >> 1] = v;
+                const bool ok = ((rd * EW + w) * ET + et) * 16 + n < ne;
+#pragma unroll
+                for (int o = 0; o < NO; o++) {
+                    const uint32_t sl = ok ? chunk_half(cur.ch[et], NJ + o) : 0xffffu;
+                    slot[min(sl, (uint32_t)S::SMAX)] = yv[et][o];
+                }
             }
-        }
+        };
+        while (true) {
+            const int64_t bn = b + step;
+            const bool more = bn < end;
+            // the next block, or this one again past the last (its loads
+            // are then harmless and unused)
+            const int64_t bq = more ? bn : b;
+            int32_t k0n, nkn, nintn, e0n, nen;
+            header(bq, k0n, nkn, nintn, e0n, nen);
+            uint32_t qq[RPT];
+            __syncthreads();    // the previous block's xs and slot reads are done
 #pragma unroll
-        for (int et = 0; et < ET; et++) {
-            const int64_t e = (tile0 + et) * 16 + n;
-            if (tile0 + et < ntile && e < nelem) {
+            for (int i = 0; i < RPT; i++) {
+                const bool in = tid + EB * i < nk;
+                if (in)
+                    xs[tid + EB * i] = xr[i];
+                qq[i] = in ? qr[i] : 0u;
+            }
+            __syncthreads();
+            int32_t rwn[RPT];
+            uint32_t qrn[RPT];
+            row_loads(bq, k0n, nkn, rwn, qrn);
+            // rounds in pairs, the loads of round rd + 1 issued before round
+            // rd computes; the round data never moves between registers (a
+            // move would wait for the loads in flight).  Loads are
+            // unconditional (a round past the block reads its last
+            // element, an odd round count gets an empty last round) and the
+            // last load of a block is the next block's first round
+            const int ntile = (ne + 15) / 16;
+            const int nround = (ntile + EW * ET - 1) / (EW * ET);
+            for (int rd = 0; rd < nround; rd += 2) {
+                load_round<NB, W>(m, e0, ne, w, n, g, rd + 1, rb);
+                compute(rd, ra);
+                const bool last = rd + 2 >= nround;
+                load_round<NB, W>(m, last ? e0n : e0, last ? nen : ne, w, n, g,
+                                  last ? 0 : rd + 2, ra);
+                compute(rd + 1, rb);
+            }
+            // the next block's x, this block's output rows and partial slots
+            float xrn[RPT];
+            int32_t orow[RPT], opp[RPT];
+            const int kc = max(nk - 1, 0);
 #pragma unroll
-                for (int t = 0; t < NT; t++)
-                    if ((t & 1) == (g & 1))
-                        y[e * NB + 2 * t + (g >> 1)] = yv[et][t >> 1];
+            for (int i = 0; i < RPT; i++) {
+                xrn[i] = x[rwn[i]];
+                orow[i] = m.rows[k0 + min(tid + EB * i, kc)];
+                opp[i] = m.ppos[k0 + min(tid + EB * i, kc)];
+            }
+            __syncthreads();
+            // row sums, slots in element order
+#pragma unroll
+            for (int i = 0; i < RPT; i++) {
+                const int k = tid + EB * i;
+                // four interleaved partial sums, combined in a fixed order
+                const int q0 = qq[i] & 0xffffu, q1 = qq[i] >> 16;
+                float s4[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+                int q = q0;
+                for (; q + 4 <= q1; q += 4)
+#pragma unroll
+                    for (int u = 0; u < 4; u++)
+                        s4[u] += slot[q + u];
+                for (int u = 0; q < q1; q++, u++)
+                    s4[u] += slot[q];
+                const float sum = (s4[0] + s4[1]) + (s4[2] + s4[3]);
+                if (k < nint) {
+                    y[orow[i]] = sum;
+                    dot += (double)xs[k] * (double)sum;
+                }
+                else if (k < nk)
+                    m.part[opp[i]] = sum;
+            }
+            if (!more)
+                break;
+            b = bn;
+            k0 = k0n;
+            nk = nkn;
+            nint = nintn;
+            e0 = e0n;
+            ne = nen;
+#pragma unroll
+            for (int i = 0; i < RPT; i++) {
+                qr[i] = qrn[i];
+                xr[i] = xrn[i];
             }
         }
     }
+    group_total<EB>(dot, part_out);
+}
+
+// shared rows (sum of their partials in block order) and Dirichlet rows
+// (diag * x); the work-group's part of (x, y) over them in part_out
+__global__ void __launch_bounds__(FB)
+k_eop_fix(int64_t nfx, int64_t nitem, const int32_t *__restrict__ rows,
+          const int32_t *__restrict__ off, const float *__restrict__ part,
+          const float *__restrict__ diag, const float *__restrict__ x,
+          float *__restrict__ y, double *part_out, const int32_t *done,
+          const int32_t *mode, int32_t want)
+{
+    if (done != nullptr && (*done != 0 || *mode != want))
+        return;
+    // FI items per work-item and sweep, their loads issued together
+    // (indices clamped; a Dirichlet item has the empty partial range)
+    double dot = 0.0;
+    const int64_t last = nitem - 1;
+    for (int64_t t0 = (int64_t)blockIdx.x * FB * FI + threadIdx.x; t0 < nitem;
+         t0 += (int64_t)gridDim.x * FB * FI) {
+        int32_t r[FI], q0[FI], q1[FI];
+        float xr[FI], dg[FI];
+#pragma unroll
+        for (int k = 0; k < FI; k++) {
+            const int64_t t = min(t0 + (int64_t)k * FB, last);
+            r[k] = rows[t];
+            q0[k] = off[min(t, nfx)];
+            q1[k] = off[min(t + 1, nfx)];
+        }
+#pragma unroll
+        for (int k = 0; k < FI; k++) {
+            xr[k] = x[r[k]];
+            dg[k] = diag[r[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < FI; k++) {
+            const int64_t t = t0 + (int64_t)k * FB;
+            float sum = 0.0f;
+            for (int32_t q = q0[k]; q < q1[k]; q++)
+                sum += part[q];
+            if (t >= nfx)
+                sum = dg[k] * xr[k];
+            if (t < nitem) {
+                y[r[k]] = sum;
+                dot += (double)xr[k] * (double)sum;
+            }
+        }
+    }
+    group_total<FB>(dot, part_out);
 }
 
 // host bf16 (round to nearest even) of a float
@@ -223,6 +562,24 @@ void build_tiles(int p, int nb, int ks, std::vector<uint16_t> &out)
                 }
 }
 
+// FP32 A values [nb / 2][nj][64] of the FP32 path: lane l of tile t and
+// K step u holds T^m[i][j], row R = l % 16 -> m = R % 8, i = 2 t + R / 8,
+// j = 4 u + l / 16 (0 for m >= 6 or j >= nb)
+void build_tiles_f32(int p, int nb, std::vector<float> &out)
+{
+    std::vector<double> T((size_t)NC * nb * nb);
+    phg::eop_reference_tensors(p, T.data());
+    const int nt = nb / 2, nj = (nb + 3) / 4;
+    out.assign((size_t)nt * nj * 64, 0.0f);
+    for (int t = 0; t < nt; t++)
+        for (int u = 0; u < nj; u++)
+            for (int l = 0; l < 64; l++) {
+                const int R = l % 16, m = R % 8, i = 2 * t + R / 8, j = 4 * u + l / 16;
+                if (m < NC && j < nb)
+                    out[((size_t)t * nj + u) * 64 + l] = (float)T[((size_t)m * nb + i) * nb + j];
+            }
+}
+
 uint64_t spread3(uint64_t v)
 {
     v &= 0x1fffffull;
@@ -234,18 +591,28 @@ uint64_t spread3(uint64_t v)
     return v;
 }
 
-int grid_elem(int p)
+template <int P>
+int grid_block()
 {
-    const void *k = p == 1 ? (const void *)k_eop_elem<1>
-                  : (p == 2 ? (const void *)k_eop_elem<2> : (const void *)k_eop_elem<3>);
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, EB, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_eop_block<P>,
+                                                     EB, 0) != hipSuccess ||
         per_cu <= 0) {
         (void)hipGetLastError();
         per_cu = 1;
     }
     const int cus = phg::ctx()->num_cus > 0 ? phg::ctx()->num_cus : 256;
     return per_cu * cus;
+}
+
+template <int P>
+void launch_block(const EopData *eo, const float *x, float *y, double *part_out,
+                  const int32_t *done, const int32_t *mode, int32_t want, hipStream_t s)
+{
+    const BlockView m{eo->nblk, eo->eb_elem, eo->eb_row, eo->eb_nint, eo->rows, eo->rs,
+                      eo->ppos, eo->idx, eo->coef, (const void *)eo->tiles, eo->part};
+    hipLaunchKernelGGL(k_eop_block<P>, dim3(eo->grid_b), dim3(EB), 0, s, m, x, y,
+                       part_out, done, mode, want);
 }
 
 }  // namespace
@@ -257,37 +624,39 @@ void free_eop(phg_hip_mat *A)
     EopData *eo = A->eop;
     if (eo == nullptr)
         return;
-    dfree(eo->map);
+    dfree(eo->eb_elem);
+    dfree(eo->eb_row);
+    dfree(eo->eb_nint);
+    dfree(eo->rows);
+    dfree(eo->rs);
+    dfree(eo->ppos);
+    dfree(eo->idx);
     dfree(eo->coef);
     dfree(eo->tiles);
-    dfree(eo->inc_off);
-    dfree(eo->inc);
-    dfree(eo->gblk);
-    dfree(eo->bdry);
-    dfree(eo->y);
+    dfree(eo->part);
+    dfree(eo->fx_rows);
+    dfree(eo->fx_off);
     delete eo;
     A->eop = nullptr;
 }
 
-void eop_elements(const EopData *eo, const float *x, const int32_t *done,
-                  const int32_t *mode, int32_t want, hipStream_t s)
+void eop_apply(const EopData *eo, const float *x, const float *diag, float *y,
+               double *part_out, const int32_t *done, const int32_t *mode, int32_t want,
+               hipStream_t s, int which)
 {
-    if (eo->nelem == 0)
-        return;
-    const int grid = eo->grid_e;
-    const uint4 *t = (const uint4 *)eo->tiles;
-    if (eo->p == 1)
-        hipLaunchKernelGGL(k_eop_elem<1>, dim3(grid), dim3(EB), 0, s, eo->nelem,
-                           (const int32_t *)eo->map, (const float *)eo->coef, t, x,
-                           eo->y, done, mode, want);
-    else if (eo->p == 2)
-        hipLaunchKernelGGL(k_eop_elem<2>, dim3(grid), dim3(EB), 0, s, eo->nelem,
-                           (const int32_t *)eo->map, (const float *)eo->coef, t, x,
-                           eo->y, done, mode, want);
-    else
-        hipLaunchKernelGGL(k_eop_elem<3>, dim3(grid), dim3(EB), 0, s, eo->nelem,
-                           (const int32_t *)eo->map, (const float *)eo->coef, t, x,
-                           eo->y, done, mode, want);
+    if (which != 2) {
+        if (eo->p == 1)
+            launch_block<1>(eo, x, y, part_out, done, mode, want, s);
+        else if (eo->p == 2)
+            launch_block<2>(eo, x, y, part_out, done, mode, want, s);
+        else
+            launch_block<3>(eo, x, y, part_out, done, mode, want, s);
+    }
+    if (which != 1)
+        hipLaunchKernelGGL(k_eop_fix, dim3(eo->grid_f), dim3(FB), 0, s, eo->nfx,
+                           eo->nfx + eo->ndir, (const int32_t *)eo->fx_rows,
+                           (const int32_t *)eo->fx_off, (const float *)eo->part, diag, x,
+                           y, part_out + eo->grid_b, done, mode, want);
 }
 
 }  // namespace phg
@@ -317,7 +686,7 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
     hipStream_t s = ctx()->stream;
     const int p = sp->order, nb = sp->nbas;
     const int64_t ne = sp->nelem, n = A->n;
-    if ((int64_t)nb * ne > INT32_MAX)
+    if ((int64_t)nb * ne >= (int64_t)1 << 30)
         return set_error(PHG_HIP_E_ARG, "phg_hip_mat_element_operator: too many elements");
     std::vector<int32_t> map((size_t)ne * nb), ev((size_t)ne * 4), perm;
     std::vector<double> xyz((size_t)sp->nvert * 3);
@@ -365,9 +734,8 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
         std::stable_sort(eperm.begin(), eperm.end(),
                          [&](int32_t a, int32_t b) { return key[(size_t)a] < key[(size_t)b]; });
     }
-    // element DOF map in the solver numbering, Dirichlet DOFs -1; row
-    // incidence lists (non-Dirichlet rows) in element order
-    std::vector<int32_t> emap((size_t)ne * nb), off((size_t)n + 1, 0), inc;
+    // element DOF map in the solver numbering, Dirichlet DOFs -1
+    std::vector<int32_t> emap((size_t)ne * nb);
     std::vector<uint8_t> bq((size_t)n);
     for (int64_t i = 0; i < n; i++)
         bq[(size_t)i] = brow[(size_t)(A->q_perm ? perm[(size_t)i] : i)];
@@ -375,35 +743,180 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
         const int64_t e = eperm[(size_t)e2];
         for (int a = 0; a < nb; a++) {
             const int32_t r = iperm[(size_t)map[(size_t)(e * nb + a)]];
-            const bool b = bq[(size_t)r] != 0;
-            emap[(size_t)(e2 * nb + a)] = b ? -1 : r;
-            if (!b)
-                off[(size_t)r + 1]++;
+            emap[(size_t)(e2 * nb + a)] = bq[(size_t)r] ? -1 : r;
         }
     }
-    for (int64_t i = 0; i < n; i++)
-        off[(size_t)i + 1] += off[(size_t)i];
-    const int64_t ninc = off[(size_t)n];
-    inc.resize((size_t)std::max<int64_t>(ninc, 1));
+    // element blocks: consecutive elements, at most eblk of them touching
+    // at most kmax rows; the touched rows of a block in first-touch order
+    const int eblk = p == 1 ? EopShape<1>::EBLK : (p == 2 ? EopShape<2>::EBLK : EopShape<3>::EBLK);
+    const int kmax = p == 1 ? EopShape<1>::KMAX : (p == 2 ? EopShape<2>::KMAX : EopShape<3>::KMAX);
+    std::vector<int32_t> eb_elem{0}, eb_row{0}, rows;
+    std::vector<int32_t> stamp((size_t)n, -1), cnt((size_t)n, 0);
     {
-        std::vector<int32_t> pos(off.begin(), off.end() - 1);
-        for (int64_t e2 = 0; e2 < ne; e2++)
-            for (int a = 0; a < nb; a++) {
-                const int32_t r = emap[(size_t)(e2 * nb + a)];
-                if (r >= 0)
-                    inc[(size_t)pos[(size_t)r]++] = (int32_t)(e2 * nb + a);
+        int32_t b = 0, nk = 0, nel = 0;
+        for (int64_t e = 0; e < ne; e++) {
+            const int32_t *me = &emap[(size_t)(e * nb)];
+            int fresh = 0;
+            for (int a = 0; a < nb; a++)
+                if (me[a] >= 0 && stamp[(size_t)me[a]] != b)
+                    fresh++;
+            if (nel > 0 && (nel == eblk || nk + fresh > kmax)) {
+                eb_elem.push_back((int32_t)e);
+                eb_row.push_back((int32_t)rows.size());
+                b++;
+                nk = nel = 0;
             }
+            for (int a = 0; a < nb; a++) {
+                const int32_t r = me[a];
+                if (r >= 0 && stamp[(size_t)r] != b) {
+                    stamp[(size_t)r] = b;
+                    rows.push_back(r);
+                    cnt[(size_t)r]++;
+                    nk++;
+                }
+            }
+            nel++;
+        }
+        if (nel > 0) {
+            eb_elem.push_back((int32_t)ne);
+            eb_row.push_back((int32_t)rows.size());
+        }
     }
+    const int64_t nblk = (int64_t)eb_elem.size() - 1, nent = (int64_t)rows.size();
+    if (nblk <= 0)
+        return set_error(PHG_HIP_E_ARG, "phg_hip_mat_element_operator: no elements");
+    // per block: interior rows (touched by this block only) first, then
+    // shared rows; slots of a row consecutive, in element order; the
+    // partials of a shared row in block order
+    std::vector<int32_t> eb_nint((size_t)nblk), ppos((size_t)nent, -1), kidx((size_t)n, 0);
+    std::vector<uint16_t> rs((size_t)(nent + nblk));
+    std::vector<uint32_t> kd((size_t)ne * nb, KD_NONE);
+    std::vector<int32_t> fxi((size_t)n, -1), seen((size_t)n, 0), fx_rows, fx_off{0};
+    // shared rows in row order (the fix-up walks them so), each with the
+    // range of its partials
+    for (int64_t r = 0; r < n; r++)
+        if (cnt[(size_t)r] > 1) {
+            fxi[(size_t)r] = (int32_t)fx_rows.size();
+            fx_rows.push_back((int32_t)r);
+            fx_off.push_back(fx_off.back() + cnt[(size_t)r]);
+        }
+    int64_t nint = 0;
+    for (int64_t b = 0; b < nblk; b++) {
+        const int32_t k0 = eb_row[(size_t)b], nk = eb_row[(size_t)b + 1] - k0;
+        const int32_t e0 = eb_elem[(size_t)b], e1 = eb_elem[(size_t)b + 1];
+        // slots per touched row (first-touch order)
+        std::vector<int32_t> first(rows.begin() + k0, rows.begin() + k0 + nk), ns((size_t)nk, 0);
+        for (int32_t k = 0; k < nk; k++)
+            kidx[(size_t)first[(size_t)k]] = k;
+        for (int64_t e = e0; e < e1; e++)
+            for (int a = 0; a < nb; a++) {
+                const int32_t r = emap[(size_t)(e * nb + a)];
+                if (r >= 0)
+                    ns[(size_t)kidx[(size_t)r]]++;
+            }
+        // interior rows first, then shared; each group by decreasing slot
+        // count, so the row sums of a wavefront have similar lengths
+        std::vector<int32_t> ord((size_t)nk);
+        std::iota(ord.begin(), ord.end(), 0);
+        std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t c) {
+            const bool ia = cnt[(size_t)first[(size_t)a]] == 1, ic = cnt[(size_t)first[(size_t)c]] == 1;
+            if (ia != ic)
+                return ia;
+            return ns[(size_t)a] > ns[(size_t)c];
+        });
+        int32_t ni = 0;
+        for (int32_t k = 0; k < nk; k++) {
+            const int32_t r = first[(size_t)ord[(size_t)k]];
+            rows[(size_t)(k0 + k)] = r;
+            if (cnt[(size_t)r] == 1)
+                ni++;
+        }
+        eb_nint[(size_t)b] = ni;
+        nint += ni;
+        int32_t k;
+        for (k = 0; k < nk; k++)
+            kidx[(size_t)rows[(size_t)(k0 + k)]] = k;
+        std::vector<int32_t> sc((size_t)nk + 1, 0);
+        for (int64_t e = e0; e < e1; e++)
+            for (int a = 0; a < nb; a++) {
+                const int32_t r = emap[(size_t)(e * nb + a)];
+                if (r >= 0)
+                    sc[(size_t)kidx[(size_t)r] + 1]++;
+            }
+        for (k = 0; k < nk; k++)
+            sc[(size_t)k + 1] += sc[(size_t)k];
+        for (k = 0; k <= nk; k++)
+            rs[(size_t)(k0 + b + k)] = (uint16_t)sc[(size_t)k];
+        for (int64_t e = e0; e < e1; e++)
+            for (int a = 0; a < nb; a++) {
+                const int32_t r = emap[(size_t)(e * nb + a)];
+                if (r >= 0) {
+                    const int32_t kl = kidx[(size_t)r];
+                    kd[(size_t)(e * nb + a)] = (uint32_t)kl | (uint32_t)sc[(size_t)kl]++ << 16;
+                }
+            }
+        for (k = ni; k < nk; k++) {
+            const int32_t r = rows[(size_t)(k0 + k)];
+            ppos[(size_t)(k0 + k)] = fx_off[(size_t)fxi[(size_t)r]] + seen[(size_t)r]++;
+        }
+    }
+    // lane-group index chunks (see load_round): x rows of j = NJ g + u,
+    // result slots of i = 2 t + g / 2, t = 2 o + g % 2
+    const int nj = (nb + 3) / 4, no = (nb / 2 + 1) / 2, wc = (nj + no + 1) / 2;
+    std::vector<uint32_t> idx((size_t)ne * 4 * wc);
+    for (int64_t e = 0; e < ne; e++)
+        for (int g = 0; g < 4; g++) {
+            std::vector<uint32_t> h((size_t)2 * wc, 0xffffu);
+            for (int u = 0; u < nj; u++) {
+                const int j = p == 3 ? nj * g + u : 4 * u + g;
+                if (j < nb)
+                    h[(size_t)u] = kd[(size_t)(e * nb + j)] & 0xffffu;
+            }
+            for (int o = 0; o < no; o++) {
+                const int t = 2 * o + (g & 1);
+                if (t < nb / 2)
+                    h[(size_t)(nj + o)] = kd[(size_t)(e * nb + 2 * t + (g >> 1))] >> 16;
+            }
+            for (int u = 0; u < wc; u++)
+                idx[(size_t)((e * 4 + g) * wc + u)] = h[(size_t)(2 * u)] | h[(size_t)(2 * u + 1)] << 16;
+        }
+    const int64_t nfx = (int64_t)fx_rows.size(), npart = fx_off.back();
+    // one entry of padding: the block kernel clamps its row-table loads to
+    // the block's last entry, or to the first when a block touches no row
+    rows.push_back(0);
+    ppos.push_back(0);
+    rs.push_back(0);
+    for (int64_t i = 0; i < n; i++)
+        if (bq[(size_t)i])
+            fx_rows.push_back((int32_t)i);
+    const int64_t ndir = (int64_t)fx_rows.size() - nfx;
     int ks = p == 1 ? EopShape<1>::KS : (p == 2 ? EopShape<2>::KS : EopShape<3>::KS);
-    std::vector<uint16_t> tiles;
-    build_tiles(p, nb, ks, tiles);
+    // A fragments: bf16 parts (P3) or FP32 values (P1, P2), as bytes
+    std::vector<uint8_t> tiles;
+    if (p == 3) {
+        std::vector<uint16_t> tb;
+        build_tiles(p, nb, ks, tb);
+        tiles.resize(tb.size() * 2);
+        memcpy(tiles.data(), tb.data(), tiles.size());
+    }
+    else {
+        std::vector<float> tf;
+        build_tiles_f32(p, nb, tf);
+        tiles.resize(tf.size() * 4);
+        memcpy(tiles.data(), tf.data(), tiles.size());
+    }
     EopData *eo = new EopData();
     eo->p = p;
     eo->nb = nb;
     eo->ks = ks;
     eo->nelem = ne;
     eo->n = n;
-    eo->ninc = ninc;
+    eo->nblk = nblk;
+    eo->nent = nent;
+    eo->nint = nint;
+    eo->nfx = nfx;
+    eo->ndir = ndir;
+    eo->npart = npart;
     A->eop = eo;
     double *dk = nullptr;
     int32_t *dperm = nullptr;
@@ -413,29 +926,36 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
         free_eop(A);
         return code;
     };
-    if ((rc = dmalloc((void **)&eo->map, emap.size() * 4)) ||
-        (rc = dmalloc((void **)&eo->coef, (size_t)ne * NC * 4)) ||
-        (rc = dmalloc((void **)&eo->tiles, tiles.size() * 2)) ||
-        (rc = dmalloc((void **)&eo->inc_off, off.size() * 4)) ||
-        (rc = dmalloc((void **)&eo->inc, inc.size() * 4)) ||
-        (rc = dmalloc((void **)&eo->bdry, (size_t)n)) ||
-        (rc = dmalloc((void **)&eo->y, (size_t)ne * nb * 4)) ||
-        (rc = dmalloc((void **)&dperm, (size_t)ne * 4)) ||
-        (rc = upload(eo->map, emap.data(), emap.size() * 4)) ||
-        (rc = upload(eo->tiles, tiles.data(), tiles.size() * 2)) ||
-        (rc = upload(eo->inc_off, off.data(), off.size() * 4)) ||
-        (rc = upload(eo->inc, inc.data(), inc.size() * 4)) ||
-        (rc = upload(eo->bdry, bq.data(), (size_t)n)) ||
-        (rc = upload(dperm, eperm.data(), (size_t)ne * 4)))
+    auto put = [&](void **d, const void *h, size_t bytes) {
+        int r2 = dmalloc(d, std::max<size_t>(bytes, 4));
+        if (r2 == 0 && bytes > 0)
+            r2 = upload(*d, h, bytes);
+        return r2;
+    };
+    if ((rc = put((void **)&eo->eb_elem, eb_elem.data(), eb_elem.size() * 4)) ||
+        (rc = put((void **)&eo->eb_row, eb_row.data(), eb_row.size() * 4)) ||
+        (rc = put((void **)&eo->eb_nint, eb_nint.data(), eb_nint.size() * 4)) ||
+        (rc = put((void **)&eo->rows, rows.data(), rows.size() * 4)) ||
+        (rc = put((void **)&eo->rs, rs.data(), rs.size() * 2)) ||
+        (rc = put((void **)&eo->ppos, ppos.data(), ppos.size() * 4)) ||
+        (rc = put((void **)&eo->idx, idx.data(), idx.size() * 4)) ||
+        (rc = put((void **)&eo->tiles, tiles.data(), tiles.size())) ||
+        (rc = put((void **)&eo->fx_rows, fx_rows.data(), fx_rows.size() * 4)) ||
+        (rc = put((void **)&eo->fx_off, fx_off.data(), fx_off.size() * 4)) ||
+        (rc = dmalloc((void **)&eo->part, (size_t)std::max<int64_t>(npart, 1) * 4)) ||
+        (rc = dmalloc((void **)&eo->coef, ((size_t)ne * NC + 2) * 4)) ||
+        (rc = put((void **)&dperm, eperm.data(), (size_t)ne * 4)))
         return fail(rc);
     if (pb->use_kappa &&
         ((rc = dmalloc((void **)&dk, (size_t)ne * 8)) ||
          (rc = upload(dk, kappa, (size_t)ne * 8))))
         return fail(rc);
-    if ((rc = eop_coefficients(sp, pb->use_kappa ? dk : nullptr, dperm, eo->coef, s)) ||
-        (rc = split_row_blocks(eo->inc_off, n, ninc, EOP_GATHER_NNZ, EOP_GATHER_ROWS,
-                               &eo->gblk, &eo->ngblk)))
+    if ((rc = eop_coefficients(sp, pb->use_kappa ? dk : nullptr, dperm, eo->coef, s)))
         return fail(rc);
+    // two entries of padding, read (and multiplied by zero) for the last
+    // element
+    if (hipMemsetAsync(eo->coef + ne * NC, 0, 2 * sizeof(float), s) != hipSuccess)
+        return fail(hip_fail(hipGetLastError(), "element operator set-up", __FILE__, __LINE__));
     hipError_t e = hipStreamSynchronize(s);
     dfree(dk);
     dfree(dperm);
@@ -443,9 +963,13 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
     dperm = nullptr;
     if (e != hipSuccess)
         return fail(hip_fail(e, "element operator set-up", __FILE__, __LINE__));
-    const int et = p == 1 ? kEt<1> : (p == 2 ? kEt<2> : kEt<3>);
-    const int64_t need = ((ne + 15) / 16 + (int64_t)EW * et - 1) / ((int64_t)EW * et);
-    eo->grid_e = (int)std::max<int64_t>(1, std::min<int64_t>(need, grid_elem(p)));
+    // grids: every block group resident; partial sums of both kernels fit
+    // the solvers' partial-sum segment
+    const int gb = p == 1 ? grid_block<1>() : (p == 2 ? grid_block<2>() : grid_block<3>());
+    eo->grid_b = (int)std::max<int64_t>(1, std::min<int64_t>({nblk, (int64_t)gb,
+                                                              (int64_t)EOP_MAX_PART / 2}));
+    eo->grid_f = (int)std::max<int64_t>(
+        1, std::min<int64_t>((nfx + ndir + FB * FI - 1) / (FB * FI), (int64_t)EOP_MAX_PART / 2));
     eo->on = 1;
     return PHG_HIP_OK;
 }

@@ -749,72 +749,6 @@ __device__ inline void spmv_apply(const DictViewT<PF> &m,
     dict_spmv<PF, T>(m, va, x, epi);
 }
 
-// Per-row sums of the matrix-free operator (eop.hip): a Dirichlet row of
-// the solver numbering is diag * p, any other row the sum of the element
-// results y[code] of its incidence list, in element order.  Blocks of at
-// most EOP_GATHER_NNZ incidences and EOP_GATHER_ROWS rows, swept per XCD;
-// the incidences of a block are gathered into LDS with coalesced code
-// loads, then one work-item sums its row.  A row with more incidences than
-// the LDS stage is a block of its own and is summed from global memory.
-struct EopGather {
-    const int32_t *off, *inc, *blk;
-    const uint8_t *bdry;
-    const float *y;
-    int64_t nblk;
-};
-
-template <typename Epi>
-__device__ inline void eop_gather(const EopGather &m, const float *__restrict__ diag,
-                                  const float *__restrict__ p, Epi epi)
-{
-    static_assert(phg::EOP_GATHER_ROWS <= BS, "one work-item per row");
-    __shared__ float prod[phg::EOP_GATHER_NNZ];
-    int64_t first, end, step;
-    phg::xcd_sweep(m.nblk, first, end, step);
-    const int tid = threadIdx.x;
-    for (int64_t b = first; b < end; b += step) {
-        const int32_t r0 = m.blk[b], r1 = m.blk[b + 1];
-        const int32_t f0 = m.off[r0], f1 = m.off[r1];
-        const bool fits = f1 - f0 <= phg::EOP_GATHER_NNZ;
-        lds_barrier();
-        if (fits)
-            for (int32_t e = f0 + tid; e < f1; e += BS)
-                prod[e - f0] = m.y[m.inc[e]];
-        lds_barrier();
-        if (tid < r1 - r0) {
-            const int32_t r = r0 + tid;
-            float sum;
-            if (m.bdry[r])
-                sum = diag[r] * p[r];
-            else if (fits)
-                sum = ordered_sum(prod, m.off[r] - f0, m.off[r + 1] - f0);
-            else {
-                sum = 0.0f;
-                for (int32_t e = m.off[r]; e < m.off[r + 1]; e++)
-                    sum += m.y[m.inc[e]];
-            }
-            epi(r, sum, p[r]);
-        }
-    }
-}
-
-// Ap = A p through the element results, partials of (p, Ap); with a state,
-// only in reliable-update ITER slots
-__global__ void __launch_bounds__(BS) k_eop_gather(EopGather m, const float *diag,
-                                                   const float *p, float *Ap,
-                                                   double *part_out,
-                                                   const State *st)
-{
-    if (st != nullptr && (st->done || st->mode != RU_ITER))
-        return;
-    double v[1] = {0.0};
-    eop_gather(m, diag, p, [&](int32_t row, float s, float xr) {
-        Ap[row] = s;
-        v[0] += (double)xr * (double)s;
-    });
-    write_partials<1>(v, part_out);
-}
-
 // SELL construction: entries of permuted row i (original row perm[i]) with
 // column indices renumbered through iperm; padding repeats the row's own
 // index with value 0
@@ -2882,21 +2816,6 @@ int mixed_impl(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x,
 // (gather into the solver numbering, r = b - A x), one slot of launches
 // (k_ru_p, k_ru_spmv, k_ru_update, k_ru_resid) and the end (fold x32 into x,
 // scatter back).  Slots write the ping-pong state st[(k + 1) & 1].
-EopGather eop_view(const EopData *eo)
-{
-    return EopGather{eo->inc_off, eo->inc, eo->gblk, eo->bdry, eo->y, eo->ngblk};
-}
-
-// gather grid, fixed when first asked (outside any graph capture: the
-// solver prologues ask before capturing)
-int eop_grid(EopData *eo)
-{
-    if (eo->grid_g == 0)
-        eo->grid_g = (int)std::max<int64_t>(
-            1, std::min<int64_t>(eo->ngblk, resident_groups((const void *)k_eop_gather)));
-    return eo->grid_g;
-}
-
 template <typename View>
 struct RuPlan {
     View mv;
@@ -2904,7 +2823,6 @@ struct RuPlan {
     // FP32 SpMV through the matrix-free operator when eo is set; gs32 is the
     // number of partials the FP32 SpMV writes
     const EopData *eo = nullptr;
-    EopGather eg{};
     int gs32 = 0;
     // the FP64 residual of UPDATE slots runs in k_ru_update (layouts in
     // the solver numbering) or in k_ru_resid (layouts with their own order)
@@ -2934,8 +2852,7 @@ int ru_prologue(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x,
     // the operator works in the solver numbering, as every layout but SELL
     if (A->eop != nullptr && A->eop->on && pl->resid_in_update) {
         pl->eo = A->eop;
-        pl->eg = eop_view(A->eop);
-        pl->gs32 = eop_grid(A->eop);
+        pl->gs32 = A->eop->grid_b + A->eop->grid_f;
     }
     State h;
     memset(&h, 0, sizeof(h));
@@ -2986,12 +2903,9 @@ void ru_slot(const phg_hip_mat *A, const RuPlan<View> &pl, int64_t k,
                        ws->x32, (const double *)pres, g,
                        pl.resid_in_update ? g : gs, gs, (const State *)sin, sout);
     if (prof) { prof_mark(KP_P32, s); prof_mark(KP_SPMV32, s); }
-    if (pl.eo != nullptr) {
-        phg::eop_elements(pl.eo, ws->p32, &sout->done, &sout->mode, RU_ITER, s);
-        hipLaunchKernelGGL(k_eop_gather, dim3(pl.gs32), dim3(BS), 0, s, pl.eg,
-                           F::d32(A), (const float *)ws->p32, ws->Ap32, ps,
-                           (const State *)sout);
-    }
+    if (pl.eo != nullptr)
+        phg::eop_apply(pl.eo, ws->p32, F::d32(A), ws->Ap32, ps, &sout->done, &sout->mode,
+                       RU_ITER, s);
     else
         hipLaunchKernelGGL((k_ru_spmv<View>), dim3(gs), dim3(BS), 0, s, pl.mv,
                            F::v32(A), (const float *)ws->p32, ws->Ap32, ps,
@@ -3720,11 +3634,8 @@ extern "C" int phg_hip_element_apply(const phg_hip_mat *A_, const phg_hip_vec *x
     const int g = grid_for(n);
     hipLaunchKernelGGL(k_gather_f32, dim3(g), dim3(BS), 0, s, n,
                        (const int32_t *)A->q_perm, (const double *)x->d, ws->p32);
-    phg::eop_elements(A->eop, ws->p32, nullptr, nullptr, 0, s);
-    hipLaunchKernelGGL(k_eop_gather, dim3(eop_grid(A->eop)), dim3(BS), 0, s,
-                       eop_view(A->eop), (const float *)A->q_diag32,
-                       (const float *)ws->p32, ws->Ap32, ws->seg(SEG_S),
-                       (const State *)nullptr);
+    phg::eop_apply(A->eop, ws->p32, (const float *)A->q_diag32, ws->Ap32, ws->seg(SEG_S),
+                   nullptr, nullptr, 0, s);
     hipLaunchKernelGGL(k_scatter_f32, dim3(g), dim3(BS), 0, s, n,
                        (const int32_t *)A->q_perm, (const float *)ws->Ap32, y->d);
     PHG_HIP_CHECK(hipGetLastError());
@@ -3748,7 +3659,6 @@ extern "C" int phg_hip_element_time(const phg_hip_mat *A_, int32_t reps,
     if (ws == nullptr)
         return phg::set_error(PHG_HIP_E_NOMEM, "workspace allocation failed");
     PHG_HIP_CHECK(hipMemsetAsync(ws->p32, 0, (size_t)A->n * sizeof(float), s));
-    const int gg = eop_grid(A->eop);
     hipEvent_t ev[3];
     for (int i = 0; i < 3; i++)
         PHG_HIP_CHECK(hipEventCreate(&ev[i]));
@@ -3758,13 +3668,8 @@ extern "C" int phg_hip_element_time(const phg_hip_mat *A_, int32_t reps,
         for (int i = 0; i < reps + 2; i++) {
             if (i == 2)
                 err = hipEventRecord(ev[0], s);
-            if (k == 0)
-                phg::eop_elements(A->eop, ws->p32, nullptr, nullptr, 0, s);
-            else
-                hipLaunchKernelGGL(k_eop_gather, dim3(gg), dim3(BS), 0, s,
-                                   eop_view(A->eop), (const float *)A->q_diag32,
-                                   (const float *)ws->p32, ws->Ap32, ws->seg(SEG_S),
-                                   (const State *)nullptr);
+            phg::eop_apply(A->eop, ws->p32, (const float *)A->q_diag32, ws->Ap32,
+                           ws->seg(SEG_S), nullptr, nullptr, 0, s, k + 1);
         }
         if (err == hipSuccess)
             err = hipGetLastError();
@@ -3794,13 +3699,31 @@ extern "C" int phg_hip_element_bytes(const phg_hip_mat *A, int64_t *elem_bytes,
         gather_bytes == nullptr)
         return phg::set_error(PHG_HIP_E_ARG, "phg_hip_element_bytes: operator not built");
     const EopData *eo = A->eop;
-    // element kernel: map + coefficients read, element results written, x
-    // read once
-    *elem_bytes = eo->nelem * ((int64_t)eo->nb * 4 + 24 + (int64_t)eo->nb * 4) + eo->n * 4;
-    // gather: codes and element results read once, row offsets, flags, p
-    // read, Ap written, row blocks
-    *gather_bytes = eo->ninc * 4 + eo->nelem * (int64_t)eo->nb * 4 +
-                    (eo->n + 1) * 4 + eo->n * (1 + 4 + 4) + (eo->ngblk + 1) * 4;
+    const int64_t nsh = eo->nent - eo->nint;    // shared-row entries = partials
+    // block kernel: packed indices and coefficients of every element, per
+    // touched-row entry its row (and partial slot when shared), slot start
+    // and x; interior rows of y and the partials written; block tables
+    const int64_t nj = (eo->nb + 3) / 4, no = (eo->nb / 2 + 1) / 2, wc = (nj + no + 1) / 2;
+    *elem_bytes = eo->nelem * (16 * wc + 24) + eo->nent * (4 + 2 + 4) +
+                  eo->nblk * (2 + 3 * 4) + nsh * (4 + 4) + eo->nint * 4;
+    // fix-up: rows, partial ranges and partials of shared rows, x and y of
+    // shared and Dirichlet rows, the Dirichlet diagonal
+    *gather_bytes = (eo->nfx + eo->ndir) * (4 + 4 + 4) + (eo->nfx + 1) * 4 +
+                    eo->npart * 4 + eo->ndir * 4;
+    return PHG_HIP_OK;
+}
+
+extern "C" int phg_hip_element_layout(const phg_hip_mat *A, int64_t out[6])
+{
+    if (A == nullptr || A->eop == nullptr || out == nullptr)
+        return phg::set_error(PHG_HIP_E_ARG, "phg_hip_element_layout: operator not built");
+    const EopData *eo = A->eop;
+    out[0] = eo->nblk;
+    out[1] = eo->nent;
+    out[2] = eo->nint;
+    out[3] = eo->nfx;
+    out[4] = eo->npart;
+    out[5] = eo->ndir;
     return PHG_HIP_OK;
 }
 

@@ -95,17 +95,26 @@ struct DictLayout {
 struct EopData {
     int p = 0, nb = 0, ks = 0;        // order, basis size, K steps of 32
     int on = 0;                       // used by the FP32 solver slots
-    int64_t nelem = 0, n = 0, ninc = 0;
-    int32_t *map = nullptr;           // [nelem][nb] solver index or -1
-    float *coef = nullptr;            // [nelem][6]
-    uint16_t *tiles = nullptr;        // [nb / 2][ks][64 lanes][8] bf16 A fragments
-    int32_t *inc_off = nullptr;       // [n+1]
-    int32_t *inc = nullptr;           // [ninc] codes e * nb + a, by element
-    int32_t *gblk = nullptr;          // row blocks of the gather
-    int64_t ngblk = 0;
-    uint8_t *bdry = nullptr;          // [n] Dirichlet rows
-    float *y = nullptr;               // [nelem][nb] element results
-    int grid_e = 0, grid_g = 0;       // launch grids (element, gather)
+    int64_t nelem = 0, n = 0;
+    // element blocks (consecutive elements in Morton order)
+    int64_t nblk = 0, nent = 0, nint = 0;   // blocks, touched-row entries, interior rows
+    int32_t *eb_elem = nullptr;       // [nblk+1] element ranges
+    int32_t *eb_row = nullptr;        // [nblk+1] touched-row entry ranges
+    int32_t *eb_nint = nullptr;       // [nblk] interior rows of each block
+    int32_t *rows = nullptr;          // [nent] solver rows, interior first per block
+    uint16_t *rs = nullptr;           // [nent+nblk] slot starts per block
+    int32_t *ppos = nullptr;          // [nent] partial slot of a shared row
+    uint32_t *idx = nullptr;          // [nelem][4][w] lane-group chunks of 16-bit row
+                                      // indices and slots (0xffff: none)
+    float *coef = nullptr;            // [nelem][6] + 2 entries of padding
+    uint8_t *tiles = nullptr;         // A fragments (bf16 parts for P3, FP32 for P1 / P2)
+    float *part = nullptr;            // [npart] partial sums of shared rows
+    int64_t npart = 0;
+    // fix-up: nfx shared rows (partial ranges fx_off) then ndir Dirichlet rows
+    int32_t *fx_rows = nullptr;       // [nfx + ndir]
+    int32_t *fx_off = nullptr;        // [nfx + 1]
+    int64_t nfx = 0, ndir = 0;
+    int grid_b = 0, grid_f = 0;       // launch grids; grid_b + grid_f partial sums
 };
 
 struct phg_hip_mat {
@@ -236,14 +245,17 @@ __device__ inline void xcd_sweep(int64_t n, int64_t &first, int64_t &end,
     end = n * (x + 1) / kXcd;
 }
 // matrix-free operator (eop.hip; reference data from assemble.hip)
-constexpr int EOP_GATHER_NNZ = 2048, EOP_GATHER_ROWS = 256;
+constexpr int EOP_MAX_PART = 2048;   // partial sums one apply writes, at most
 void eop_reference_tensors(int p, double *T);
 int eop_coefficients(const phg_hip_space *sp, const double *d_kappa,
                      const int32_t *d_eperm, float *d_coef, hipStream_t s);
-// launches the element products y_e = K_e x_e of every element into eo->y
-// (the caller checks the launch); the kernel returns at once when done is
-// set and (*done != 0 or *mode != want)
-void eop_elements(const EopData *eo, const float *x, const int32_t *done,
-                  const int32_t *mode, int32_t want, hipStream_t s);
+// launches y = A x of the operator in the solver numbering (diag: FP32
+// diagonal, for the Dirichlet rows), and the eo->grid_b + eo->grid_f
+// partial sums of (x, y) into part_out; which = 1 launches the block kernel
+// only, 2 the fix-up only (timing).  The caller checks the launch; the
+// kernels return at once when done is set and (*done != 0 or *mode != want)
+void eop_apply(const EopData *eo, const float *x, const float *diag, float *y,
+               double *part_out, const int32_t *done, const int32_t *mode, int32_t want,
+               hipStream_t s, int which = 0);
 void free_eop(phg_hip_mat *A);
 }  // namespace phg

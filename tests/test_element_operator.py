@@ -52,6 +52,39 @@ def test_apply_matches_assembled_matrix(hip, kind, p, n):
     assert te > 0 and tg > 0 and eb > 0 and gb > 0
 
 
+# meshes of several element blocks (blocks of 2048 / 1024 / 512 elements
+# for P1 / P2 / P3), so rows shared between blocks go through the partials
+MULTI_BLOCK_CASES = [("kappa", 1, 16), ("kappa", 2, 9), ("sin", 3, 6)]
+
+
+@pytest.mark.parametrize("kind,p,n", MULTI_BLOCK_CASES)
+def test_apply_over_many_blocks(hip, kind, p, n):
+    pa = hip
+    c, pb, sp, A = _system(pa, kind, p, n)
+    A.element_operator(sp, pb, kappa=c["kappa"])
+    lay = A.element_layout()
+    print("layout", p, n, lay, flush=True)
+    assert lay["blocks"] > 2 and lay["shared"] > 0
+    # every row is set once: by its only block, from partials, or Dirichlet
+    assert lay["interior"] + lay["shared"] + lay["dirichlet"] == A.n
+    assert lay["dirichlet"] == int(c["bdry"].astype(bool).sum())
+    assert lay["entries"] == lay["interior"] + lay["partials"]
+    rng = np.random.default_rng(20 + p)
+    xv = rng.standard_normal(A.n)
+    x, y64 = pa.Vec(A.n, xv), pa.Vec(A.n)
+    A.matvec(x, y64)
+    ys = []
+    for _ in range(2):
+        y32 = pa.Vec(A.n)
+        A.element_apply(x, y32)
+        ys.append(y32.numpy())
+    a = y64.numpy()
+    rel = np.linalg.norm(ys[0] - a) / np.linalg.norm(a)
+    assert rel < 2e-6, rel
+    assert np.max(np.abs(ys[0] - a)) < 2e-5 * np.max(np.abs(a))
+    assert np.array_equal(ys[0], ys[1])            # fixed summation order
+
+
 def test_apply_dirichlet_rows_are_diagonal(hip):
     """Rows of Dirichlet DOFs are diag * x; a vector supported on Dirichlet
     DOFs only touches no interior row (elimination removed those columns)."""
