@@ -115,6 +115,9 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=40,
                     help="cubes per side of the CPU baseline sample (0: skip)")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--operator", choices=("element", "csr"), default="element",
+                    help="FP32 operator of the mixed solver: the matrix-free element "
+                         "operator or the FP32 copy of the assembled CSR")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -143,6 +146,22 @@ def main():
     log(f"SpMV layouts (us fp64/fp32): " + ", ".join(
         f"{k} {v[0]:.1f}/{v[1]:.1f}" for k, v in tune.items() if k != "chosen")
         + f"; chosen {tune['chosen']} ({time.perf_counter() - t0:.2f}s)")
+
+    eop = None
+    if args.operator == "element":
+        t0 = time.perf_counter()
+        A.element_operator(sp, pb, kappa=kappa)
+        pa.synchronize()
+        t_eop = time.perf_counter() - t0
+        eb, fb = A.element_bytes()
+        tb, tf = A.element_time(reps=20)
+        eop = {"setup_s": round(t_eop, 4), "layout": A.element_layout(),
+               "block_kernel": {"bytes": eb, "avg_us": round(tb, 2),
+                                "GBps": round(eb / tb / 1e3, 1)},
+               "fixup_kernel": {"bytes": fb, "avg_us": round(tf, 2),
+                                "GBps": round(fb / tf / 1e3, 1)}}
+        log(f"element operator: set-up {t_eop:.3f}s, {eop['layout']}, block kernel "
+            f"{tb:.1f} us ({eb / tb / 1e3:.0f} GB/s), fix-up {tf:.1f} us")
 
     b = A.rhs()
     # FP64 reference solution (untimed, converged far below the target)
@@ -195,6 +214,16 @@ def main():
     sec_o, out_o = timed(run_other, max(1, args.steps), 1)
     err_o = np.linalg.norm(x.numpy() - xr) / nxr
     log(f"mixed (other scheme): {sec_o * 1e3:.2f} ms/solve, {out_o}, rel.err {err_o:.2e}")
+    # the same mixed solver on the FP32 copy of the assembled CSR
+    csr_mixed = None
+    if eop is not None:
+        A.use_element_operator(False)
+        sec_c, out_c = timed(run_mixed, max(1, args.steps), 1)
+        err_c = np.linalg.norm(x.numpy() - xr) / nxr
+        A.use_element_operator(True)
+        csr_mixed = {"ms_per_solve": round(sec_c * 1e3, 3), "value": round(nd / sec_c, 1),
+                     "fp32_its": out_c[0], "rel_err": float(f"{err_c:.3e}")}
+        log(f"mixed on the FP32 CSR: {sec_c * 1e3:.2f} ms/solve, {out_c}, rel.err {err_c:.2e}")
     # plain FP64 GPU path with the same stopping rule
     sec_64, (nits64, res64) = timed(run_fp64, max(1, args.steps), 1)
     err_64 = np.linalg.norm(x.numpy() - xr) / nxr
@@ -212,7 +241,22 @@ def main():
         pa.profile(False)
         inv = {v: k for k, v in pa.FMT_NAMES.items()}
         f64, f32 = inv[tune["chosen"][0]], inv[tune["chosen"][1]]
-        if l32 > 0:
+        if l32 > 0 and eop is not None:
+            # the FP32 operator apply: element-block kernel + fix-up, (p,Ap)
+            t32 = ms32 / l32 * 1e-3
+            b32 = eop["block_kernel"]["bytes"] + eop["fixup_kernel"]["bytes"]
+            ach = b32 / t32 / 1e9
+            tb, tf = (measured_traffic("k_eop_block<2>", nnz),
+                      measured_traffic("k_eop_fix", nnz))
+            roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                    "traffic": tb + tf if tb and tf else None,
+                    "kernel": "k_eop_block<2> + k_eop_fix (FP32 element operator apply "
+                              "+ (p,Ap))",
+                    "bytes_per_launch": b32, "avg_us": round(t32 * 1e6, 2),
+                    "launches": l32,
+                    "csr_equivalent_GBps": round(spmv_bytes(n, nnz, 4, 4) / t32 / 1e9, 1)}
+        elif l32 > 0:
             t32 = ms32 / l32 * 1e-3
             b32 = A.format_bytes(f32, True)
             ach = b32 / t32 / 1e9
@@ -273,7 +317,12 @@ def main():
                          f"({g.nelem} tets, {nd} DOFs), kappa=exp(N(0,{args.sigma}^2)) "
                          f"seed {args.seed}; " + (
                              f"FP32 Jacobi-PCG with reliable FP64 residual updates "
-                             f"(delta {args.delta})" if args.mixed == "reliable" else
+                             f"(delta {args.delta})" + (
+                                 ", FP32 iterations on the matrix-free element operator "
+                                 "(FP32 MFMA), FP64 residuals on the assembled CSR"
+                                 if args.operator == "element" else
+                                 ", FP32 iterations on the FP32 CSR")
+                             if args.mixed == "reliable" else
                              f"FP32 Jacobi-PCG (inner rtol {args.inner_rtol}) under "
                              f"FP64 iterative refinement") +
                          f" to FP64 relative residual {args.rtol}"),
@@ -291,6 +340,9 @@ def main():
         "assemble_s": round(t_asm, 4),
         "spmv_layout_us": {k: v for k, v in tune.items() if k != "chosen"},
         "roofline": roof,
+        "operator": args.operator,
+        "element_operator": eop,
+        "mixed_csr": csr_mixed,
         "spmv_fp64": spmv64,
         "copy_kernel": {"bytes": 16 * copy_n, "avg_us": round(copy_us, 2),
                         "GBps": round(copy_gbs, 1)},
