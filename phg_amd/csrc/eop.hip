@@ -28,9 +28,10 @@
 //      slot -- the slots of a row are consecutive and in element order;
 //   3. one work-item per touched row sums its slots in that order.  A row
 //      touched by this block only (interior) is written to y directly; a
-//      row shared with other blocks writes a partial, and k_eop_fix sums the
-//      partials of each shared row in block order and sets the Dirichlet
-//      rows (diag * x).
+//      row shared with other blocks writes a partial, summed in block order
+//      by k_eop_fix or, in the solvers, by their update kernel.  Each block
+//      also sets a slice of the Dirichlet rows (diag * x), so (x, y) is
+//      complete when the block kernel ends.
 // Every sum has a fixed order, so the operator is deterministic.  Per
 // element the kernel streams 4 B per local index (touched-row index and slot,
 // packed) and 24 B of coefficients, about 1/3 of the FP32 CSR entries the
@@ -51,8 +52,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int EB = 512;           // work-items per work-group (8 wavefronts)
 constexpr int EW = EB / 64;
 constexpr int NC = 6;             // coefficients per element
-constexpr int FB = 1024;          // work-items of the fix-up kernel
-constexpr int FI = 1;             // its items per work-item and sweep
+constexpr int FB = 256;           // work-items of the fix-up kernel
 constexpr uint32_t KD_NONE = 0xffffffffu;
 
 template <int P> struct EopShape {
@@ -103,6 +103,9 @@ struct BlockView {
     const float *coef;        // [nelem][6]
     const void *tiles;        // A fragments: bf16 (split path) or FP32
     float *part;              // partials of shared rows
+    const int32_t *drows;     // [ndir] Dirichlet rows, spread evenly over the blocks
+    int64_t ndir;
+    const float *diag;        // FP32 diagonal (Dirichlet rows: y = diag x)
 };
 
 // work-group total of one double per work-item, to out[blockIdx.x]
@@ -194,8 +197,9 @@ __device__ inline uint32_t chunk_half(const uint32_t (&ch)[W], int h)
 // per CU for P2, two for P1 (LDS and VGPRs both fit), one for P3
 template <int P> constexpr int kWaves = P == 3 ? 2 : (P == 2 ? 6 : 4);
 
-// y = A x over the element blocks: interior rows of y, partials of shared
-// rows, the work-group's part of (x, y) over interior rows in part_out
+// y = A x over the element blocks: interior and Dirichlet rows of y,
+// partials of shared rows, the work-group's part of (x, y) over all rows
+// in part_out
 template <int P>
 __global__ void __launch_bounds__(EB, kWaves<P>)
 k_eop_block(BlockView m, const float *__restrict__ x, float *__restrict__ y,
@@ -441,12 +445,24 @@ k_eop_block(BlockView m, const float *__restrict__ x, float *__restrict__ y,
                 for (int u = 0; q < q1; q++, u++)
                     s4[u] += slot[q];
                 const float sum = (s4[0] + s4[1]) + (s4[2] + s4[3]);
-                if (k < nint) {
-                    y[orow[i]] = sum;
+                // (x, y) of a shared row is the sum over its blocks of
+                // x times the block's partial
+                if (k < nk)
                     dot += (double)xs[k] * (double)sum;
-                }
+                if (k < nint)
+                    y[orow[i]] = sum;
                 else if (k < nk)
                     m.part[opp[i]] = sum;
+            }
+            // this block's share of the Dirichlet rows: y = diag x
+            {
+                const int64_t d0 = m.ndir * b / m.nblk, d1 = m.ndir * (b + 1) / m.nblk;
+                for (int64_t t = d0 + tid; t < d1; t += EB) {
+                    const int32_t r = m.drows[t];
+                    const float xv = x[r], v = m.diag[r] * xv;
+                    y[r] = v;
+                    dot += (double)xv * (double)v;
+                }
             }
             if (!more)
                 break;
@@ -466,52 +482,20 @@ k_eop_block(BlockView m, const float *__restrict__ x, float *__restrict__ y,
     group_total<EB>(dot, part_out);
 }
 
-// shared rows (sum of their partials in block order) and Dirichlet rows
-// (diag * x); the work-group's part of (x, y) over them in part_out
+// shared rows: the sum of their partials in block order (the solvers do
+// this in their update kernel instead; see eop_shared_code)
 __global__ void __launch_bounds__(FB)
-k_eop_fix(int64_t nfx, int64_t nitem, const int32_t *__restrict__ rows,
-          const int32_t *__restrict__ off, const float *__restrict__ part,
-          const float *__restrict__ diag, const float *__restrict__ x,
-          float *__restrict__ y, double *part_out, const int32_t *done,
-          const int32_t *mode, int32_t want)
+k_eop_fix(int64_t nfx, const int32_t *__restrict__ rows, const int32_t *__restrict__ off,
+          const float *__restrict__ part, float *__restrict__ y)
 {
-    if (done != nullptr && (*done != 0 || *mode != want))
-        return;
-    // FI items per work-item and sweep, their loads issued together
-    // (indices clamped; a Dirichlet item has the empty partial range)
-    double dot = 0.0;
-    const int64_t last = nitem - 1;
-    for (int64_t t0 = (int64_t)blockIdx.x * FB * FI + threadIdx.x; t0 < nitem;
-         t0 += (int64_t)gridDim.x * FB * FI) {
-        int32_t r[FI], q0[FI], q1[FI];
-        float xr[FI], dg[FI];
-#pragma unroll
-        for (int k = 0; k < FI; k++) {
-            const int64_t t = min(t0 + (int64_t)k * FB, last);
-            r[k] = rows[t];
-            q0[k] = off[min(t, nfx)];
-            q1[k] = off[min(t + 1, nfx)];
-        }
-#pragma unroll
-        for (int k = 0; k < FI; k++) {
-            xr[k] = x[r[k]];
-            dg[k] = diag[r[k]];
-        }
-#pragma unroll
-        for (int k = 0; k < FI; k++) {
-            const int64_t t = t0 + (int64_t)k * FB;
-            float sum = 0.0f;
-            for (int32_t q = q0[k]; q < q1[k]; q++)
-                sum += part[q];
-            if (t >= nfx)
-                sum = dg[k] * xr[k];
-            if (t < nitem) {
-                y[r[k]] = sum;
-                dot += (double)xr[k] * (double)sum;
-            }
-        }
+    for (int64_t t = (int64_t)blockIdx.x * FB + threadIdx.x; t < nfx;
+         t += (int64_t)gridDim.x * FB) {
+        float sum = 0.0f;
+        const int32_t q1 = off[t + 1];
+        for (int32_t q = off[t]; q < q1; q++)
+            sum += part[q];
+        y[rows[t]] = sum;
     }
-    group_total<FB>(dot, part_out);
 }
 
 // host bf16 (round to nearest even) of a float
@@ -606,11 +590,13 @@ int grid_block()
 }
 
 template <int P>
-void launch_block(const EopData *eo, const float *x, float *y, double *part_out,
-                  const int32_t *done, const int32_t *mode, int32_t want, hipStream_t s)
+void launch_block(const EopData *eo, const float *x, const float *diag, float *y,
+                  double *part_out, const int32_t *done, const int32_t *mode, int32_t want,
+                  hipStream_t s)
 {
     const BlockView m{eo->nblk, eo->eb_elem, eo->eb_row, eo->eb_nint, eo->rows, eo->rs,
-                      eo->ppos, eo->idx, eo->coef, (const void *)eo->tiles, eo->part};
+                      eo->ppos, eo->idx, eo->coef, (const void *)eo->tiles, eo->part,
+                      eo->fx_rows + eo->nfx, eo->ndir, diag};
     hipLaunchKernelGGL(k_eop_block<P>, dim3(eo->grid_b), dim3(EB), 0, s, m, x, y,
                        part_out, done, mode, want);
 }
@@ -636,27 +622,27 @@ void free_eop(phg_hip_mat *A)
     dfree(eo->part);
     dfree(eo->fx_rows);
     dfree(eo->fx_off);
+    dfree(eo->fxc);
     delete eo;
     A->eop = nullptr;
 }
 
 void eop_apply(const EopData *eo, const float *x, const float *diag, float *y,
                double *part_out, const int32_t *done, const int32_t *mode, int32_t want,
-               hipStream_t s, int which)
+               hipStream_t s, int which, bool fixup)
 {
     if (which != 2) {
         if (eo->p == 1)
-            launch_block<1>(eo, x, y, part_out, done, mode, want, s);
+            launch_block<1>(eo, x, diag, y, part_out, done, mode, want, s);
         else if (eo->p == 2)
-            launch_block<2>(eo, x, y, part_out, done, mode, want, s);
+            launch_block<2>(eo, x, diag, y, part_out, done, mode, want, s);
         else
-            launch_block<3>(eo, x, y, part_out, done, mode, want, s);
+            launch_block<3>(eo, x, diag, y, part_out, done, mode, want, s);
     }
-    if (which != 1)
+    if (which != 1 && fixup && eo->nfx > 0)
         hipLaunchKernelGGL(k_eop_fix, dim3(eo->grid_f), dim3(FB), 0, s, eo->nfx,
-                           eo->nfx + eo->ndir, (const int32_t *)eo->fx_rows,
-                           (const int32_t *)eo->fx_off, (const float *)eo->part, diag, x,
-                           y, part_out + eo->grid_b, done, mode, want);
+                           (const int32_t *)eo->fx_rows, (const int32_t *)eo->fx_off,
+                           (const float *)eo->part, y);
 }
 
 }  // namespace phg
@@ -967,9 +953,23 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
     // the solvers' partial-sum segment
     const int gb = p == 1 ? grid_block<1>() : (p == 2 ? grid_block<2>() : grid_block<3>());
     eo->grid_b = (int)std::max<int64_t>(1, std::min<int64_t>({nblk, (int64_t)gb,
-                                                              (int64_t)EOP_MAX_PART / 2}));
-    eo->grid_f = (int)std::max<int64_t>(
-        1, std::min<int64_t>((nfx + ndir + FB * FI - 1) / (FB * FI), (int64_t)EOP_MAX_PART / 2));
+                                                              (int64_t)EOP_MAX_PART}));
+    eo->grid_f = (int)std::max<int64_t>(1, std::min<int64_t>((nfx + FB - 1) / FB, 4096));
+    // per-row codes of the shared rows for the solvers' update kernel:
+    // partial start | count << 24 (when every start and count fits)
+    bool fits = npart < ((int64_t)1 << 24) - 1;
+    for (int64_t t = 0; t < nfx && fits; t++)
+        fits = fx_off[(size_t)t + 1] - fx_off[(size_t)t] < 256;
+    if (fits) {
+        std::vector<uint32_t> code((size_t)n, EOP_NOT_SHARED);
+        for (int64_t t = 0; t < nfx; t++)
+            code[(size_t)fx_rows[(size_t)t]] =
+                (uint32_t)fx_off[(size_t)t] |
+                (uint32_t)(fx_off[(size_t)t + 1] - fx_off[(size_t)t]) << 24;
+        if ((rc = dmalloc((void **)&eo->fxc, (size_t)n * 4)) ||
+            (rc = upload(eo->fxc, code.data(), (size_t)n * 4)))
+            return fail(rc);
+    }
     eo->on = 1;
     return PHG_HIP_OK;
 }

@@ -1827,7 +1827,9 @@ __global__ void __launch_bounds__(BS) k_ru_update(int64_t n, float *x32,
                                                   const double *diag64,
                                                   const double *b,
                                                   const double *x,
-                                                  int resid_here)
+                                                  int resid_here,
+                                                  const uint32_t *fxc,
+                                                  const float *fpart)
 {
     if (st->done)
         return;
@@ -1847,9 +1849,22 @@ __global__ void __launch_bounds__(BS) k_ru_update(int64_t n, float *x32,
     }
     const float alpha = (float)(st->rz / pap[0]);
     double v[3] = {0.0, 0.0, 0.0};
+    // Ap of a row shared between element blocks of the matrix-free
+    // operator: the sum of its partials in block order (code: partial start
+    // | count << 24)
+    auto shared = [&](uint32_t c, float a) {
+        if (c == phg::EOP_NOT_SHARED)
+            return a;
+        const uint32_t q0 = c & 0xffffffu, q1 = q0 + (c >> 24);
+        float sum = 0.0f;
+        for (uint32_t q = q0; q < q1; q++)
+            sum += fpart[q];
+        return sum;
+    };
     auto one = [&](int64_t i) {
         x32[i] += alpha * p[i];
-        float ri = r[i] - alpha * Ap[i];
+        const float api = fxc ? shared(fxc[i], Ap[i]) : Ap[i];
+        float ri = r[i] - alpha * api;
         r[i] = ri;
         float z = ri / diag[i];
         v[0] += (double)ri * (double)ri;
@@ -1858,6 +1873,13 @@ __global__ void __launch_bounds__(BS) k_ru_update(int64_t n, float *x32,
     for_each4(n, [&](int64_t q) {
         Q4<float> xv = ld4(x32, q), pv = ld4(p, q), rv = ld4((const float *)r, q),
                   av = ld4(Ap, q), dv = ld4(diag, q);
+        if (fxc) {
+            const uint4 cv = ((const uint4 *)fxc)[q];
+            av.v[0] = shared(cv.x, av.v[0]);
+            av.v[1] = shared(cv.y, av.v[1]);
+            av.v[2] = shared(cv.z, av.v[2]);
+            av.v[3] = shared(cv.w, av.v[3]);
+        }
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             xv.v[k] += alpha * pv.v[k];
@@ -2852,7 +2874,7 @@ int ru_prologue(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x,
     // the operator works in the solver numbering, as every layout but SELL
     if (A->eop != nullptr && A->eop->on && pl->resid_in_update) {
         pl->eo = A->eop;
-        pl->gs32 = A->eop->grid_b + A->eop->grid_f;
+        pl->gs32 = A->eop->grid_b;
     }
     State h;
     memset(&h, 0, sizeof(h));
@@ -2903,9 +2925,11 @@ void ru_slot(const phg_hip_mat *A, const RuPlan<View> &pl, int64_t k,
                        ws->x32, (const double *)pres, g,
                        pl.resid_in_update ? g : gs, gs, (const State *)sin, sout);
     if (prof) { prof_mark(KP_P32, s); prof_mark(KP_SPMV32, s); }
+    // the operator's shared rows are summed in k_ru_update when their codes
+    // exist, else by the fix-up kernel
     if (pl.eo != nullptr)
         phg::eop_apply(pl.eo, ws->p32, F::d32(A), ws->Ap32, ps, &sout->done, &sout->mode,
-                       RU_ITER, s);
+                       RU_ITER, s, 0, pl.eo->fxc == nullptr);
     else
         hipLaunchKernelGGL((k_ru_spmv<View>), dim3(gs), dim3(BS), 0, s, pl.mv,
                            F::v32(A), (const float *)ws->p32, ws->Ap32, ps,
@@ -2918,7 +2942,9 @@ void ru_slot(const phg_hip_mat *A, const RuPlan<View> &pl, int64_t k,
                        (const int32_t *)A->q_rowptr, (const int32_t *)A->q_cols,
                        (const double *)A->q_vals, (const int32_t *)A->q_rblk,
                        A->q_nrblk, (const double *)A->q_diag,
-                       pl.bb, (const double *)pl.xx, pl.resid_in_update ? 1 : 0);
+                       pl.bb, (const double *)pl.xx, pl.resid_in_update ? 1 : 0,
+                       pl.eo ? (const uint32_t *)pl.eo->fxc : nullptr,
+                       pl.eo ? (const float *)pl.eo->part : nullptr);
     if (prof) prof_mark(KP_UPD32, s);
     if (!pl.resid_in_update)
         hipLaunchKernelGGL((k_ru_resid<View>), dim3(gs), dim3(BS), 0, s,
@@ -3702,14 +3728,13 @@ extern "C" int phg_hip_element_bytes(const phg_hip_mat *A, int64_t *elem_bytes,
     const int64_t nsh = eo->nent - eo->nint;    // shared-row entries = partials
     // block kernel: packed indices and coefficients of every element, per
     // touched-row entry its row (and partial slot when shared), slot start
-    // and x; interior rows of y and the partials written; block tables
+    // and x; interior rows of y and the partials written; block tables;
+    // Dirichlet rows (row, x, diagonal read, y written)
     const int64_t nj = (eo->nb + 3) / 4, no = (eo->nb / 2 + 1) / 2, wc = (nj + no + 1) / 2;
     *elem_bytes = eo->nelem * (16 * wc + 24) + eo->nent * (4 + 2 + 4) +
-                  eo->nblk * (2 + 3 * 4) + nsh * (4 + 4) + eo->nint * 4;
-    // fix-up: rows, partial ranges and partials of shared rows, x and y of
-    // shared and Dirichlet rows, the Dirichlet diagonal
-    *gather_bytes = (eo->nfx + eo->ndir) * (4 + 4 + 4) + (eo->nfx + 1) * 4 +
-                    eo->npart * 4 + eo->ndir * 4;
+                  eo->nblk * (2 + 3 * 4) + nsh * (4 + 4) + eo->nint * 4 + eo->ndir * 16;
+    // fix-up: rows, partial ranges and partials of the shared rows, y
+    *gather_bytes = eo->nfx * (4 + 4 + 4) + 4 + eo->npart * 4;
     return PHG_HIP_OK;
 }
 

@@ -114,7 +114,9 @@ struct EopData {
     int32_t *fx_rows = nullptr;       // [nfx + ndir]
     int32_t *fx_off = nullptr;        // [nfx + 1]
     int64_t nfx = 0, ndir = 0;
-    int grid_b = 0, grid_f = 0;       // launch grids; grid_b + grid_f partial sums
+    uint32_t *fxc = nullptr;          // [n] partial start | count << 24 of a shared
+                                      // row, EOP_NOT_SHARED otherwise (null: no codes)
+    int grid_b = 0, grid_f = 0;       // launch grids; grid_b partial sums of (x, y)
 };
 
 struct phg_hip_mat {
@@ -246,16 +248,20 @@ __device__ inline void xcd_sweep(int64_t n, int64_t &first, int64_t &end,
 }
 // matrix-free operator (eop.hip; reference data from assemble.hip)
 constexpr int EOP_MAX_PART = 2048;   // partial sums one apply writes, at most
+constexpr uint32_t EOP_NOT_SHARED = 0xffffffffu;
 void eop_reference_tensors(int p, double *T);
 int eop_coefficients(const phg_hip_space *sp, const double *d_kappa,
                      const int32_t *d_eperm, float *d_coef, hipStream_t s);
 // launches y = A x of the operator in the solver numbering (diag: FP32
-// diagonal, for the Dirichlet rows), and the eo->grid_b + eo->grid_f
-// partial sums of (x, y) into part_out; which = 1 launches the block kernel
-// only, 2 the fix-up only (timing).  The caller checks the launch; the
-// kernels return at once when done is set and (*done != 0 or *mode != want)
+// diagonal, for the Dirichlet rows) and writes the eo->grid_b partial sums
+// of (x, y) into part_out.  The rows shared between element blocks are set
+// by a second kernel when fixup is true; with fixup false they are left to
+// the caller, which sums their partials (eo->part) through eo->fxc.  which
+// = 1 launches the block kernel only, 2 the fix-up only (timing).  The
+// caller checks the launch; the kernels return at once when done is set and
+// (*done != 0 or *mode != want)
 void eop_apply(const EopData *eo, const float *x, const float *diag, float *y,
                double *part_out, const int32_t *done, const int32_t *mode, int32_t want,
-               hipStream_t s, int which = 0);
+               hipStream_t s, int which = 0, bool fixup = true);
 void free_eop(phg_hip_mat *A);
 }  // namespace phg
