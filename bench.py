@@ -242,17 +242,16 @@ def main():
         inv = {v: k for k, v in pa.FMT_NAMES.items()}
         f64, f32 = inv[tune["chosen"][0]], inv[tune["chosen"][1]]
         if l32 > 0 and eop is not None:
-            # the FP32 operator apply: element-block kernel + fix-up, (p,Ap)
+            # the FP32 operator apply in the solver: the element-block kernel
+            # (interior and Dirichlet rows, partials of shared rows, (p,Ap));
+            # k_ru_update sums the shared rows' partials as it reads Ap
             t32 = ms32 / l32 * 1e-3
-            b32 = eop["block_kernel"]["bytes"] + eop["fixup_kernel"]["bytes"]
+            b32 = eop["block_kernel"]["bytes"]
             ach = b32 / t32 / 1e9
-            tb, tf = (measured_traffic("k_eop_block<2>", nnz),
-                      measured_traffic("k_eop_fix", nnz))
             roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                    "traffic": tb + tf if tb and tf else None,
-                    "kernel": "k_eop_block<2> + k_eop_fix (FP32 element operator apply "
-                              "+ (p,Ap))",
+                    "traffic": measured_traffic("k_eop_block<2>", nnz),
+                    "kernel": "k_eop_block<2> (FP32 element operator apply + (p,Ap))",
                     "bytes_per_launch": b32, "avg_us": round(t32 * 1e6, 2),
                     "launches": l32,
                     "csr_equivalent_GBps": round(spmv_bytes(n, nnz, 4, 4) / t32 / 1e9, 1)}
