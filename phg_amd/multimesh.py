@@ -128,7 +128,8 @@ def bench(nmesh=8, n=16, p=3, rtol=1e-8, reps=2):
               for a, x in zip(x_csr, xs))
     elem_b = [A.element_bytes() for A in As]
     # kernel level on the largest system: FP32 CSR SpMV (tuned layout) against
-    # the two operator kernels
+    # the operator's block kernel (the solvers sum the shared rows in their
+    # update kernel; the stand-alone fix-up is listed beside it)
     big = int(np.argmax(ndofs))
     Ab = As[big]
     t_el, t_ga = Ab.element_time(reps=20)
@@ -136,10 +137,10 @@ def bench(nmesh=8, n=16, p=3, rtol=1e-8, reps=2):
     kernel_level = {"system": big, "dofs": ndofs[big], "nelem": meshes[big][0].nelem,
                     "csr_fp32_layout": Ab.format()[1], "csr_fp32_spmv_us": round(t_csr, 2),
                     "csr_fp32_GBps": round(Ab.format_bytes(A_fmt32(Ab), True) / t_csr / 1e3, 1),
-                    "element_kernel_us": round(t_el, 2), "gather_kernel_us": round(t_ga, 2),
-                    "element_kernel_GBps": round(elem_b[big][0] / t_el / 1e3, 1),
-                    "gather_kernel_GBps": round(elem_b[big][1] / t_ga / 1e3, 1),
-                    "apply_speedup_vs_csr": round(t_csr / (t_el + t_ga), 3)}
+                    "block_kernel_us": round(t_el, 2), "fixup_kernel_us": round(t_ga, 2),
+                    "block_kernel_GBps": round(elem_b[big][0] / t_el / 1e3, 1),
+                    "fixup_kernel_GBps": round(elem_b[big][1] / t_ga / 1e3, 1),
+                    "block_speedup_vs_csr": round(t_csr / t_el, 3)}
     tot = sum(ndofs)
     return {"config": "c5",
             "workload": (f"{nmesh} P{p} kappa-Poisson problems on {n}^3x6 Kuhn meshes refined "
@@ -161,7 +162,7 @@ def bench(nmesh=8, n=16, p=3, rtol=1e-8, reps=2):
                 "speedup_vs_csr_graph": round(t_gr / t_egr, 3),
                 "its": [int(i) for i in its_eseq], "its_graph": [int(i) for i in its_egr],
                 "max_rel_diff_vs_csr_solution": dev,
-                "bytes_per_apply": [a + b for a, b in elem_b],
+                "block_kernel_bytes": [a for a, _ in elem_b],
                 "csr_fp32_bytes_per_spmv": [A.format_bytes(A_fmt32(A), True) for A in As],
                 "kernel_level_largest": kernel_level}}
 
