@@ -53,6 +53,9 @@ typedef struct {
     double  mass_coef;    /* scalar: + mass_coef * phgQuadBasDotBas (simplest.c:70) */
     int32_t rhs_order;    /* quadrature order of the load, <0: 2*order */
     double  E, nu;        /* elasticity moduli (elastic.c:39-40) */
+    int32_t qcache;       /* scalar Laplacian (no kappa, no mass): the integrals
+                             of examples/poisson.c -qcache (quad-cache.c:1016
+                             phgQCIntegrate_, one rule of order 2p) */
 } phg_hip_problem;
 
 /*--------------------------------------------------------------- runtime */

@@ -46,7 +46,8 @@ _P = C.POINTER
 
 class Problem(C.Structure):
     _fields_ = [("p", C.c_int), ("dim", C.c_int), ("use_kappa", C.c_int),
-                ("mass_coef", _dbl), ("rhs_order", C.c_int), ("E", _dbl), ("nu", _dbl)]
+                ("mass_coef", _dbl), ("rhs_order", C.c_int), ("E", _dbl), ("nu", _dbl),
+                ("qcache", C.c_int)]
 
 
 class _Csr(C.Structure):
@@ -92,8 +93,10 @@ def nbas(p):
     return {1: 4, 2: 10, 3: 20}[p]
 
 
-def problem(p, dim=1, use_kappa=False, mass_coef=0.0, rhs_order=-1, E=2.5, nu=0.25):
-    return Problem(p, dim, int(use_kappa), float(mass_coef), int(rhs_order), E, nu)
+def problem(p, dim=1, use_kappa=False, mass_coef=0.0, rhs_order=-1, E=2.5, nu=0.25,
+            qcache=False):
+    return Problem(p, dim, int(use_kappa), float(mass_coef), int(rhs_order), E, nu,
+                   int(qcache))
 
 
 def quad3d(order):

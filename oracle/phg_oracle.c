@@ -318,8 +318,15 @@ void orc_element(const orc_problem *pb, const double x[4][3], double kappa,
 
     if (dim == 1) {
         /* stiffness: order (p-1)+(p-1) [+0 for a P0 coefficient]
-         * (quad.c:5271-5279) */
-        nq = orc_quad3d(2 * (p - 1), &qp, &qw);
+         * (quad.c:5271-5279).  With the quadrature cache
+         * (examples/poisson.c:126 phgQCSetQuad) one rule of order 2p serves
+         * stiffness and load; quad-cache.c:1016 phgQCIntegrate_ sums, per
+         * point, d = sum_k g_j[k] g_i[k] (k = 0..2, from 0) and then
+         * val += d w, times the volume -- the operation order below.  Its
+         * x-gradients (quad-cache.c:85 grd) take the 4-term product for
+         * every basis; for P1 and vertex bases the other lambda derivatives
+         * are exact zeros, so xgrad's shortcuts give the same values. */
+        nq = orc_quad3d(pb->qcache ? 2 * p : 2 * (p - 1), &qp, &qw);
         for (int q = 0; q < nq; q++) {
             orc_grad_lambda(p, qp + 4 * q, glam);
             for (int k = 0; k < N; k++)
@@ -370,8 +377,16 @@ void orc_element(const orc_problem *pb, const double x[4][3], double kappa,
             }
             for (int i = 0; i < N; i++) {
                 double d = 0.0;
-                for (int q = 0; q < nq; q++)
-                    d += bv[q][i] * qw[q] * fq[q];
+                if (pb->qcache) {
+                    /* quad-cache.c:1016 phgQCIntegrate_(Q_BAS i, Q_f):
+                     * both data non-constant, M = N = K = 1 */
+                    for (int q = 0; q < nq; q++)
+                        d += bv[q][i] * fq[q] * qw[q];
+                }
+                else {
+                    for (int q = 0; q < nq; q++)
+                        d += bv[q][i] * qw[q] * fq[q];
+                }
                 b[i] = d * vol;
             }
         }
@@ -1082,7 +1097,7 @@ double orc_mass_apply(int p, int64_t nelem, const double *verts,
                       int64_t ndof, const double *e, double *Me)
 {
     int N = p == 1 ? 4 : (p == 2 ? 10 : 20);
-    orc_problem mp = {p, 1, 1, 1.0, -1, 0.0, 0.0};
+    orc_problem mp = {p, 1, 1, 1.0, -1, 0.0, 0.0, 0};
     double *K = xmalloc((size_t)N * N * sizeof(double));
     double *B = xmalloc((size_t)N * sizeof(double));
     double *fz = calloc((size_t)N, sizeof(double));
@@ -1117,7 +1132,7 @@ void orc_adjoint_grad(int p, int64_t nelem, const double *verts,
                       const double *lam, double *grad)
 {
     int N = p == 1 ? 4 : (p == 2 ? 10 : 20);
-    orc_problem sp = {p, 1, 1, 0.0, -1, 0.0, 0.0};
+    orc_problem sp = {p, 1, 1, 0.0, -1, 0.0, 0.0, 0};
     double *K = xmalloc((size_t)N * N * sizeof(double));
     double *B = xmalloc((size_t)N * sizeof(double));
     double *fz = calloc((size_t)N, sizeof(double));

@@ -42,6 +42,8 @@ typedef struct {
     int rhs_order;         /* quadrature order of the load; <0: 2*p
                               (phgQuadDofTimesBas QUAD_DEFAULT) */
     double E, nu;          /* elasticity (elastic.c:39) */
+    int qcache;            /* scalar Laplacian through the quadrature cache
+                              (examples/poisson.c:111-166 -qcache) */
 } orc_problem;
 
 /* Element matrix K[N*dim][N*dim] and load b[N*dim] for one element, as the

@@ -59,15 +59,17 @@ class _GridStruct(C.Structure):
 class Problem(C.Structure):
     """Integrand selector (include/phg_hip.h phg_hip_problem)."""
     _fields_ = [("dim", _i32), ("use_kappa", _i32), ("mass_coef", _dbl),
-                ("rhs_order", _i32), ("E", _dbl), ("nu", _dbl)]
+                ("rhs_order", _i32), ("E", _dbl), ("nu", _dbl), ("qcache", _i32)]
 
     @classmethod
-    def poisson(cls, use_kappa=False, mass_coef=0.0, rhs_order=-1):
-        return cls(1, int(use_kappa), float(mass_coef), int(rhs_order), 0.0, 0.0)
+    def poisson(cls, use_kappa=False, mass_coef=0.0, rhs_order=-1, qcache=False):
+        """qcache: the integrals of examples/poisson.c -qcache (quad-cache.c)."""
+        return cls(1, int(use_kappa), float(mass_coef), int(rhs_order), 0.0, 0.0,
+                   int(qcache))
 
     @classmethod
     def elasticity(cls, E=2.5, nu=0.25, rhs_order=-1):
-        return cls(3, 0, 0.0, int(rhs_order), float(E), float(nu))
+        return cls(3, 0, 0.0, int(rhs_order), float(E), float(nu), 0)
 
 
 def _sig(lib, name, res, args):

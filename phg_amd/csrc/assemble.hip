@@ -388,13 +388,15 @@ struct Tables {
 };
 
 template <int P>
-__device__ void build_tables(Tables &T, int dim, int rhs_order, int mass)
+__device__ void build_tables(Tables &T, int dim, int rhs_order, int mass, bool qc)
 {
     constexpr int N = P == 1 ? 4 : (P == 2 ? 10 : 20);
     // all work-items build the same rules; thread 0 publishes them
     if (threadIdx.x == 0) {
         Rule q;
-        make_rule(2 * (P - 1), q);   // stiffness (quad.c:5271; elastic.c:141)
+        // stiffness (quad.c:5271; elastic.c:141); with the quadrature cache
+        // the element's one rule of order 2p (poisson.c:126)
+        make_rule(qc ? 2 * P : 2 * (P - 1), q);
         T.nq_s = q.np;
         for (int i = 0; i < q.np; i++) {
             T.ws[i] = q.w[i];
@@ -424,13 +426,15 @@ __device__ void build_tables(Tables &T, int dim, int rhs_order, int mass)
     __syncthreads();
 }
 
-template <int P, int DIM>
+// QC: examples/poisson.c -qcache, the integrals through quad-cache.c:1016
+// phgQCIntegrate_ with the rule of order 2p for stiffness and load
+template <int P, int DIM, bool QC = false>
 __global__ void __launch_bounds__(128) k_element(ElemArgs a)
 {
     constexpr int N = P == 1 ? 4 : (P == 2 ? 10 : 20);
     constexpr int M = N * DIM;
     __shared__ Tables T;
-    build_tables<P>(T, DIM, a.rhs_order, a.mass_coef != 0.0);
+    build_tables<P>(T, DIM, a.rhs_order, a.mass_coef != 0.0, QC);
 
     int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= a.nelem)
@@ -451,7 +455,8 @@ __global__ void __launch_bounds__(128) k_element(ElemArgs a)
     double *B = a.B + e * (int64_t)M;
     const int32_t *m = a.map + e * (int64_t)N;
 
-    constexpr int NQS = P == 1 ? 1 : (P == 2 ? 4 : 14);   // rule 2(P-1)
+    // points of the stiffness rule: order 2(P-1), or 2P with the cache
+    constexpr int NQS = QC ? (P == 1 ? 4 : (P == 2 ? 14 : 24)) : (P == 1 ? 1 : (P == 2 ? 4 : 14));
     if (DIM == 1) {
         const double kap = a.use_kappa ? a.kappa[e] : 0.0;
         const int nq = NQS;
@@ -509,8 +514,15 @@ __global__ void __launch_bounds__(128) k_element(ElemArgs a)
         }
         for (int i = 0; i < N; i++) {
             double d = 0.0;
-            for (int q = 0; q < T.nq_r; q++)
-                d += T.br[q][i] * T.wr[q] * fq[q];
+            if (QC) {
+                // phgQCIntegrate_(Q_BAS i, f): (phi_i f) w per point
+                for (int q = 0; q < T.nq_r; q++)
+                    d += T.br[q][i] * fq[q] * T.wr[q];
+            }
+            else {
+                for (int q = 0; q < T.nq_r; q++)
+                    d += T.br[q][i] * T.wr[q] * fq[q];
+            }
             B[i] = d * vol;
         }
         return;
@@ -619,7 +631,7 @@ __global__ void __launch_bounds__(128) k_adj_mass(AdjArgs a)
 {
     constexpr int N = P == 1 ? 4 : (P == 2 ? 10 : 20);
     __shared__ Tables T;
-    build_tables<P>(T, 1, -1, 1);
+    build_tables<P>(T, 1, -1, 1, false);
     int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= a.nelem)
         return;
@@ -678,7 +690,7 @@ __global__ void __launch_bounds__(128) k_adj_grad(AdjArgs a)
     constexpr int N = P == 1 ? 4 : (P == 2 ? 10 : 20);
     constexpr int NQS = P == 1 ? 1 : (P == 2 ? 4 : 14);
     __shared__ Tables T;
-    build_tables<P>(T, 1, -1, 0);
+    build_tables<P>(T, 1, -1, 0, false);
     int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= a.nelem)
         return;
@@ -731,7 +743,7 @@ __global__ void __launch_bounds__(128) k_dof_norms(int64_t nelem,
     constexpr int N = P == 1 ? 4 : (P == 2 ? 10 : 20);
     constexpr int NQS = P == 1 ? 1 : (P == 2 ? 4 : 14);
     __shared__ Tables T;
-    build_tables<P>(T, 1, -1, 1);
+    build_tables<P>(T, 1, -1, 1, false);
     int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nelem)
         return;
@@ -1211,7 +1223,15 @@ int launch_element(const phg_hip_space *sp, const phg_hip_problem *pb,
     dim3 grid(nblk(sp->nelem, 128)), block(128);
     if (sp->nelem == 0)
         return PHG_HIP_OK;
-    if (pb->dim == 1) {
+    if (pb->dim == 1 && pb->qcache) {
+        if (sp->order == 1)
+            hipLaunchKernelGGL((k_element<1, 1, true>), grid, block, 0, s, a);
+        else if (sp->order == 2)
+            hipLaunchKernelGGL((k_element<2, 1, true>), grid, block, 0, s, a);
+        else
+            hipLaunchKernelGGL((k_element<3, 1, true>), grid, block, 0, s, a);
+    }
+    else if (pb->dim == 1) {
         if (sp->order == 1)
             hipLaunchKernelGGL((k_element<1, 1>), grid, block, 0, s, a);
         else if (sp->order == 2)
@@ -1247,6 +1267,11 @@ int check_problem(const phg_hip_space *sp, const phg_hip_problem *pb)
                               pb->rhs_order);
     if (pb->mass_coef != 0.0 && 2 * sp->order > 6)
         return phg::set_error(PHG_HIP_E_ARG, "mass order too high");
+    if (pb->qcache && (pb->dim != 1 || pb->use_kappa || pb->mass_coef != 0.0 ||
+                       (pb->rhs_order >= 0 && pb->rhs_order != 2 * sp->order)))
+        return phg::set_error(PHG_HIP_E_ARG,
+                              "qcache: the Laplacian of examples/poisson.c only (no kappa, "
+                              "no mass term, load on the order-2p rule)");
     return PHG_HIP_OK;
 }
 

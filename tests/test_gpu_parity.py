@@ -19,7 +19,8 @@ def _space(pa, c, dim=1):
 
 def _oracle_problem(c, pb):
     return orc.problem(c["p"], dim=pb.dim, use_kappa=pb.use_kappa,
-                       mass_coef=pb.mass_coef, rhs_order=pb.rhs_order, E=pb.E, nu=pb.nu)
+                       mass_coef=pb.mass_coef, rhs_order=pb.rhs_order, E=pb.E, nu=pb.nu,
+                       qcache=bool(pb.qcache))
 
 
 def _bits_equal(a, b):
@@ -33,6 +34,8 @@ CASES = [
     ("kappa", 1, dict(use_kappa=True)), ("kappa", 2, dict(use_kappa=True)),
     ("kappa", 3, dict(use_kappa=True)),
     ("sin", 2, dict(mass_coef=1.0)), ("sin", 3, dict(mass_coef=2.5)),
+    ("sin", 1, dict(qcache=True)), ("sin", 2, dict(qcache=True)),
+    ("sin", 3, dict(qcache=True)),
 ]
 
 
@@ -50,6 +53,17 @@ def test_element_matrices_bitexact(hip, kind, p, opts):
                              c["f"][m[e]])
         assert _bits_equal(K[e], Ko), f"element {e} stiffness differs"
         assert _bits_equal(B[e], Bo), f"element {e} load differs"
+
+
+def test_qcache_rejects_other_integrands(hip):
+    pa = hip
+    c = poisson_case(2, 2, "kappa")
+    sp = _space(pa, c)
+    for pb in (pa.Problem.poisson(use_kappa=True, qcache=True),
+               pa.Problem.poisson(mass_coef=1.0, qcache=True),
+               pa.Problem.poisson(rhs_order=2, qcache=True)):
+        with pytest.raises(pa.PhgHipError):
+            pa.assemble(sp, pb, kappa=c["kappa"], f=c["f"], g=c["g"])
 
 
 def test_elasticity_element_bitexact(hip):
@@ -78,11 +92,12 @@ def _assemble_both(pa, c, pb):
     return sp, A, Ao
 
 
-@pytest.mark.parametrize("kind,p,n", ASM_CASES)
-def test_csr_bitexact(hip, kind, p, n):
+@pytest.mark.parametrize("kind,p,n,qc", [a + (False,) for a in ASM_CASES] +
+                         [("sin", 1, 4, True), ("sin", 2, 3, True), ("sin", 3, 2, True)])
+def test_csr_bitexact(hip, kind, p, n, qc):
     pa = hip
     c = poisson_case(n, p, kind)
-    pb = pa.Problem.poisson(use_kappa=(kind == "kappa"))
+    pb = pa.Problem.poisson(use_kappa=(kind == "kappa"), qcache=qc)
     sp, A, Ao = _assemble_both(pa, c, pb)
     rp, ci, va, rhs = A.download()
     assert np.array_equal(rp, Ao.rowptr)

@@ -181,6 +181,40 @@ def test_load_f_one_is_basis_integral(p):
         assert np.allclose(b[:4], -vol / 20) and np.allclose(b[4:], vol / 5)
 
 
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_qcache_element_exact_integrals(p):
+    """examples/poisson.c -qcache: the integrals of quad-cache.c:1016 on the
+    one rule of order 2p.  Stiffness and load are exact there, so they match
+    the brute-force integrals; the load also matches phgQuadDofTimesBas on
+    the same rule to rounding (the products are taken in another order)."""
+    rng = np.random.default_rng(31)
+    nb = orc.nbas(p)
+    pts6, w6 = orc.quad3d(6)
+    for _ in range(3):
+        x = _rand_tet(rng)
+        floc = rng.standard_normal(nb)
+        Kq, bq = orc.element(orc.problem(p, qcache=True), x, f_loc=floc)
+        Kb = _brute_stiffness(p, x)
+        assert np.allclose(Kq, Kb, rtol=1e-11, atol=1e-11 * np.abs(Kb).max())
+        assert np.abs(Kq.sum(axis=1)).max() < 1e-11 * np.abs(Kq).max()
+        vol = abs(np.linalg.det(x[1:] - x[0])) / 6
+        V = np.array([orc.bas(p, pt) for pt in pts6])
+        bb = vol * (V.T * w6) @ (V @ floc)       # int f_h phi_i, exact
+        assert np.allclose(bq, bb, rtol=1e-12, atol=1e-13 * np.abs(bb).max())
+        _, b = orc.element(orc.problem(p), x, f_loc=floc)
+        assert np.allclose(bq, b, rtol=0, atol=1e-14 * np.abs(b).max())
+
+
+def test_qcache_kuhn_p1_closed_form():
+    h = 0.25
+    x = h * np.array([[0, 0, 0], [1, 1, 1], [1, 1, 0], [1, 0, 0]], float)
+    K, b = orc.element(orc.problem(1, qcache=True), x, f_loc=np.ones(4))
+    path = [0, 3, 2, 1]
+    T = np.array([[1, -1, 0, 0], [-1, 2, -1, 0], [0, -1, 2, -1], [0, 0, -1, 1]]) * h / 6
+    assert np.allclose(K[np.ix_(path, path)], T, atol=1e-15)
+    assert np.allclose(b, h ** 3 / 24)
+
+
 def test_elasticity_element_vs_strain_energy():
     rng = np.random.default_rng(5)
     x = _rand_tet(rng)
