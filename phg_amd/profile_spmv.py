@@ -4,9 +4,11 @@ Assembles the P2 variable-coefficient Poisson system on an n^3 x 6 Kuhn mesh
 (default 64: 2,146,689 DOFs), runs the SpMV layout tuner (every layout, FP64
 and FP32, `--reps` launches each) and a plain copy of the same byte count as
 the FP64 CSR SpMV, so that counter totals per kernel can be compared with
-the algorithmic bytes.  Prints the per-launch algorithmic bytes.
+the algorithmic bytes.  Prints the per-launch algorithmic bytes.  With
+--element-operator it also builds the matrix-free operator and launches its
+block and fix-up kernels `--reps` times each.
 
-    python -m phg_amd.profile_spmv --n 64 --reps 20
+    python -m phg_amd.profile_spmv --n 64 --reps 20 [--element-operator]
 """
 import argparse
 import json
@@ -31,6 +33,7 @@ def main():
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--element-operator", action="store_true")
     a = ap.parse_args()
     import phg_amd as pa
     pa.init(0)
@@ -39,8 +42,8 @@ def main():
     rng = np.random.default_rng(a.seed)
     kappa = np.exp(rng.normal(0.0, 0.5, size=g.nelem))
     sp = pa.Space(g, 2)
-    A = pa.assemble(sp, pa.Problem.poisson(use_kappa=True), kappa=kappa,
-                    f=np.ones(nd), g=np.zeros(nd))
+    pb = pa.Problem.poisson(use_kappa=True)
+    A = pa.assemble(sp, pb, kappa=kappa, f=np.ones(nd), g=np.zeros(nd))
     A.eliminate()
     n, nnz = A.info()
     t = A.tune(reps=a.reps)
@@ -54,6 +57,13 @@ def main():
            "layouts_us": {k: v for k, v in t.items() if k != "chosen"},
            "layout_bytes": _layout_bytes(pa, A),
            "chosen": t["chosen"]}
+    if a.element_operator:
+        A.element_operator(sp, pb, kappa=kappa)
+        tb, tf = A.element_time(reps=a.reps)
+        eb, fb = A.element_bytes()
+        out["element_operator"] = {"layout": A.element_layout(), "nelem": g.nelem,
+                                   "block_bytes": eb, "block_us": tb,
+                                   "fixup_bytes": fb, "fixup_us": tf}
     print(json.dumps(out), flush=True)
     return 0
 
