@@ -193,6 +193,30 @@ __device__ inline uint32_t chunk_half(const uint32_t (&ch)[W], int h)
     return (ch[h >> 1] >> (16 * (h & 1))) & 0xffffu;
 }
 
+// x of a lane's local indices, minus x_0 of its element (local index 0,
+// held by lane n of the wavefront).  Every T^m annihilates constants, so
+// T^m x_e = T^m (x_e - x_0 1) exactly; in FP32 the products then carry
+// rounding errors relative to the variation of x over the element rather
+// than to x itself -- the local differences of a smooth x are exact
+// (Sterbenz) and the Krylov iterations converge as on the assembled FP32
+// matrix.  Dirichlet and absent local indices (x = 0, row KMAX of xs) and
+// local indices past NB (zero rows of T) take -x_0 alike.
+template <int KMAX, int NJ, int ET, int W>
+__device__ inline void load_relative(const float *xs, const uint32_t (&ch)[ET][W], int n,
+                                     float (&xv)[ET][NJ])
+{
+#pragma unroll
+    for (int et = 0; et < ET; et++) {
+#pragma unroll
+        for (int u = 0; u < NJ; u++)
+            xv[et][u] = xs[min(chunk_half(ch[et], u), (uint32_t)KMAX)];
+        const float x0 = __shfl(xv[et][0], n, 64);
+#pragma unroll
+        for (int u = 0; u < NJ; u++)
+            xv[et][u] -= x0;
+    }
+}
+
 // wavefronts per SIMD the block kernel is compiled for: three work-groups
 // per CU for P2, two for P1 (LDS and VGPRs both fit), one for P3
 template <int P> constexpr int kWaves = P == 3 ? 2 : (P == 2 ? 6 : 4);
@@ -281,13 +305,9 @@ k_eop_block(BlockView m, const float *__restrict__ x, float *__restrict__ y,
             float yv[ET][NO];
             if constexpr (SPLIT) {
                 // x of the lane's local indices j = NJ g + u (Dirichlet and
-                // none: row KMAX of xs, which holds 0)
+                // none: row KMAX of xs, which holds 0), relative to x_0
                 float xv[ET][NJ];
-#pragma unroll
-                for (int et = 0; et < ET; et++)
-#pragma unroll
-                    for (int u = 0; u < NJ; u++)
-                        xv[et][u] = xs[min(chunk_half(cur.ch[et], u), (uint32_t)S::KMAX)];
+                load_relative<S::KMAX, NJ, ET, W>(xs, cur.ch, n, xv);
                 // split into bf16 parts through the wavefront's LDS rows, one
                 // tile at a time, into B fragments
                 bf16x8 bf[ET][KS];
@@ -341,13 +361,10 @@ k_eop_block(BlockView m, const float *__restrict__ x, float *__restrict__ y,
             }
             else {
                 // x of the lane's local indices j = 4 u + g (Dirichlet and
-                // none: row KMAX of xs, which holds 0) are the B fragments
+                // none: row KMAX of xs, which holds 0), relative to x_0, are
+                // the B fragments
                 float xv[ET][NJ];
-#pragma unroll
-                for (int et = 0; et < ET; et++)
-#pragma unroll
-                    for (int u = 0; u < NJ; u++)
-                        xv[et][u] = xs[min(chunk_half(cur.ch[et], u), (uint32_t)S::KMAX)];
+                load_relative<S::KMAX, NJ, ET, W>(xs, cur.ch, n, xv);
 #pragma unroll
                 for (int t = 0; t < NT; t++) {
                     f32x4 acc[ET];
