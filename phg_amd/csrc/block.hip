@@ -105,6 +105,23 @@ __global__ void k_bsr_fill(int64_t nb, const int32_t *rp, const int32_t *ci,
         }
 }
 
+// S_R = sum of the blocks of block row R, summed in FP64, stored in FP32
+template <int B>
+__global__ void k_bsr_rsum(int64_t nb, const int32_t *bptr, const double *va, float *out)
+{
+    int64_t R = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (R >= nb)
+        return;
+    double s[B * B];
+    for (int t = 0; t < B * B; t++)
+        s[t] = 0.0;
+    for (int32_t j = bptr[R]; j < bptr[R + 1]; j++)
+        for (int t = 0; t < B * B; t++)
+            s[t] += va[(int64_t)j * (B * B) + t];
+    for (int t = 0; t < B * B; t++)
+        out[R * (B * B) + t] = (float)s[t];
+}
+
 __global__ void k_i64_to_i32(int64_t n, const int64_t *a, int32_t *b)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -147,9 +164,16 @@ __device__ inline Cols<T, CPL> load_cols(const T *p)
 template <int S, int CPL>
 constexpr int rows_per_group() { return (64 / (S * (KB / CPL))) * (TB / 64); }
 
-template <typename T, int B, int S, int CPL, typename Epi>
+// REL: the products act on X_C - X_R (R the block row, C the block column)
+// and the row-block sum S_R = sum_C A_RC adds S_R X_R at the end.  Exact
+// algebra; in FP32 the products then round relative to the variation of X
+// across the row's stencil rather than to X itself.  The near-kernel modes
+// of the operator (constants, rigid translations), which the Krylov method
+// resolves last, lose most of their FP32 error that way.
+template <typename T, int B, int S, int CPL, typename Epi, bool REL = false>
 __device__ inline void bsr_spmm(const BsrView &m, const T *__restrict__ va,
-                                const T *__restrict__ X, Epi epi)
+                                const T *__restrict__ X, Epi epi,
+                                const T *__restrict__ rsum = nullptr)
 {
     constexpr int NCG = KB / CPL;
     constexpr int LPR = S * NCG;         // lanes per block row
@@ -168,6 +192,11 @@ __device__ inline void bsr_spmm(const BsrView &m, const T *__restrict__ va,
 #pragma unroll
             for (int i = 0; i < CPL; i++)
                 acc[p][i] = 0;
+        Cols<T, CPL> xr[B];
+        if (REL && R < m.nb)
+#pragma unroll
+            for (int q = 0; q < B; q++)
+                xr[q] = load_cols<T, CPL>(X + (R * B + q) * KB + cg * CPL);
         if (R < m.nb) {
             // the row's block columns, loaded once (coalesced) by the
             // row's lanes and handed out by shuffles, so the X gathers of
@@ -191,8 +220,13 @@ __device__ inline void bsr_spmm(const BsrView &m, const T *__restrict__ va,
                     if (jj < nblk) {
                         Cols<T, CPL> xq[B];
 #pragma unroll
-                        for (int q = 0; q < B; q++)
+                        for (int q = 0; q < B; q++) {
                             xq[q] = load_cols<T, CPL>(X + (C * B + q) * KB + cg * CPL);
+                            if (REL)
+#pragma unroll
+                                for (int i = 0; i < CPL; i++)
+                                    xq[q].v[i] -= xr[q].v[i];
+                        }
                         const T *a = va + (int64_t)(j0 + jj) * (B * B);
 #pragma unroll
                         for (int p = 0; p < B; p++)
@@ -214,12 +248,23 @@ __device__ inline void bsr_spmm(const BsrView &m, const T *__restrict__ va,
 #pragma unroll
                 for (int i = 0; i < CPL; i++)
                     acc[p][i] += __shfl_xor(acc[p][i], o, 64);
-        if (R < m.nb && s == 0)
+        if (R < m.nb && s == 0) {
+            if (REL)
+#pragma unroll
+                for (int p = 0; p < B; p++)
+#pragma unroll
+                    for (int q = 0; q < B; q++) {
+                        const T spq = rsum[R * (B * B) + p * B + q];
+#pragma unroll
+                        for (int i = 0; i < CPL; i++)
+                            acc[p][i] += spq * xr[q].v[i];
+                    }
 #pragma unroll
             for (int p = 0; p < B; p++)
 #pragma unroll
                 for (int i = 0; i < CPL; i++)
                     epi(R * B + p, cg * CPL + i, i, acc[p][i]);
+        }
     }
 }
 
@@ -430,18 +475,22 @@ __global__ void __launch_bounds__(TB) k_blk_p(int64_t n, const T *r,
     }
 }
 
+// the FP32 SpMM of the mixed solver runs in the relative form (rsum: the
+// FP32 row-block sums)
 template <typename T, int B, int S, int CPL>
 __global__ void __launch_bounds__(TB) k_blk_spmm(BsrView m, const T *va,
                                                 const T *p, T *Ap,
-                                                double *part, const BState *st)
+                                                double *part, const BState *st,
+                                                const T *rsum)
 {
     if (st->all_done || st->kind != ST_ITER)
         return;
     double v[1][CPL] = {};
-    bsr_spmm<T, B, S, CPL>(m, va, p, [&](int64_t row, int c, int i, T s) {
+    auto epi = [&](int64_t row, int c, int i, T s) {
         Ap[row * KB + c] = s;
         v[0][i] += (double)p[row * KB + c] * (double)s;
-    });
+    };
+    bsr_spmm<T, B, S, CPL, decltype(epi), sizeof(T) == 4>(m, va, p, epi, rsum);
     write_col_partials<1, CPL>(v, part);
 }
 
@@ -598,6 +647,8 @@ void free_bsr(phg_hip_mat *A)
     }
     dfree(A->b_val64);
     dfree(A->b_val32);
+    dfree(A->b_rsum32);
+    A->b_rsum32 = nullptr;
     A->b_ptr = A->b_col = nullptr;
     A->b_val64 = nullptr;
     A->b_val32 = nullptr;
@@ -623,6 +674,15 @@ int build_bsr(phg_hip_mat *A)
         A->b_nblk = A->nnz;
         A->b_owns = 0;
         A->b_dim = 1;
+        if ((rc = dmalloc((void **)&A->b_rsum32, (size_t)A->n * 4))) {
+            free_bsr(A);
+            return rc;
+        }
+        hipLaunchKernelGGL(k_bsr_rsum<1>, dim3(nblk(A->n, 128)), dim3(128), 0, s, A->n,
+                           (const int32_t *)A->q_rowptr, (const double *)A->q_vals,
+                           A->b_rsum32);
+        PHG_HIP_CHECK(hipGetLastError());
+        PHG_HIP_CHECK(hipStreamSynchronize(s));
         return PHG_HIP_OK;
     }
     const int64_t nb = A->n / 3;
@@ -696,6 +756,13 @@ int build_bsr(phg_hip_mat *A)
                        (const int64_t *)ptr64, A->b_col, A->b_val32);
     hipLaunchKernelGGL(k_i64_to_i32, dim3(nblk(nb + 1, 256)), dim3(256), 0, s,
                        nb, (const int64_t *)ptr64, A->b_ptr);
+    if ((rc = dmalloc((void **)&A->b_rsum32, (size_t)nb * 9 * 4))) {
+        free_bsr(A);
+        cleanup();
+        return rc;
+    }
+    hipLaunchKernelGGL(k_bsr_rsum<3>, dim3(nblk(nb, 128)), dim3(128), 0, s, nb,
+                       (const int32_t *)A->b_ptr, (const double *)A->b_val64, A->b_rsum32);
     PHG_HIP_CHECK(hipGetLastError());
     PHG_HIP_CHECK(hipStreamSynchronize(s));
     cleanup();
@@ -748,6 +815,13 @@ int get_work(int64_t n)
     }
     g_bw.n = n;
     return PHG_HIP_OK;
+}
+
+// row-block sums for the relative FP32 SpMM (null for FP64)
+template <typename T>
+const T *rsum_of(const phg_hip_mat *A)
+{
+    return sizeof(T) == 4 ? (const T *)(const void *)A->b_rsum32 : nullptr;
 }
 
 // block values of precision T: the CSR copies for 1x1 blocks
@@ -820,7 +894,7 @@ int block_solve(phg_hip_mat *A, int mixed, int32_t maxit, double rtol,
                                (const BState *)g_bw.st);
             hipLaunchKernelGGL((k_blk_spmm<T, B, S, CPL>), dim3(gs), dim3(TB), 0, s,
                                m, va, (const T *)p, Ap, g_bw.part_s,
-                               (const BState *)g_bw.st);
+                               (const BState *)g_bw.st, rsum_of<T>(A));
             hipLaunchKernelGGL(k_blk_alpha, dim3(1), dim3(TB), 0, s,
                                (const double *)g_bw.part_s, gs, g_bw.st);
             hipLaunchKernelGGL(k_blk_update<T>, dim3(gv), dim3(TB), 0, s, n, xi,
@@ -908,7 +982,7 @@ int time_spmm(phg_hip_mat *A, int reps, double *us)
             PHG_HIP_CHECK(hipEventRecord(e0, s));
         hipLaunchKernelGGL((k_blk_spmm<T, B, S, CPL>), dim3(gs), dim3(TB), 0, s, m,
                            va, (const T *)g_bw.p, (T *)g_bw.Ap, g_bw.part_s,
-                           (const BState *)g_bw.st);
+                           (const BState *)g_bw.st, rsum_of<T>(A));
     }
     PHG_HIP_CHECK(hipEventRecord(e1, s));
     PHG_HIP_CHECK(hipEventSynchronize(e1));

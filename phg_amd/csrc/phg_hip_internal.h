@@ -181,6 +181,7 @@ struct phg_hip_mat {
     int32_t *b_ptr = nullptr, *b_col = nullptr;
     double *b_val64 = nullptr;
     float *b_val32 = nullptr;
+    float *b_rsum32 = nullptr;       // [b_nb][b_dim^2] row-block sums (FP64 sums in FP32)
     // solver workspace of this matrix (krylov.hip)
     void *ws = nullptr;
     // SpMM wavefront shape index per precision (block.hip) and tuner times
