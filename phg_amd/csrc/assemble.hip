@@ -1504,6 +1504,15 @@ extern "C" int phg_hip_space_create(int32_t order, int64_t nvert,
     sp->nvert = nvert;
     sp->nelem = nelem;
     sp->ndof = ndof;
+    // bounding box of the vertices: every Lagrange node is a convex
+    // combination of its element's vertices, so it bounds the nodes too
+    for (int c = 0; c < 3; c++)
+        sp->lo[c] = sp->hi[c] = verts[c];
+    for (int64_t v = 0; v < nvert; v++)
+        for (int c = 0; c < 3; c++) {
+            sp->lo[c] = std::min(sp->lo[c], verts[3 * v + c]);
+            sp->hi[c] = std::max(sp->hi[c], verts[3 * v + c]);
+        }
     if ((rc = phg::dmalloc((void **)&sp->d_verts, (size_t)nvert * 3 * sizeof(double))) ||
         (rc = phg::dmalloc((void **)&sp->d_elem_verts, (size_t)nelem * 4 * sizeof(int32_t))) ||
         (rc = phg::dmalloc((void **)&sp->d_map, (size_t)nelem * nb * sizeof(int32_t))) ||
@@ -1614,6 +1623,11 @@ extern "C" int phg_hip_assemble(phg_hip_space *sp, const phg_hip_problem *pb,
     A->n = n;
     A->dim = dim;
     A->ndof = sp->ndof;
+    for (int c = 0; c < 3; c++) {
+        A->xyz_lo[c] = sp->lo[c];
+        A->xyz_hi[c] = sp->hi[c];
+    }
+    A->xyz_box = 1;
     if ((rc = phg::dmalloc((void **)&A->d_rhs, (size_t)n * sizeof(double))) ||
         (rc = phg::dmalloc((void **)&A->d_xyz, (size_t)sp->ndof * 3 * sizeof(double))))
         goto done;

@@ -57,6 +57,7 @@ struct phg_hip_space {
     int32_t *d_elem_verts = nullptr; // [nelem][4]
     int32_t *d_map = nullptr;        // [nelem][nbas]
     uint8_t *d_bdry = nullptr;       // [ndof]
+    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};   // bounding box of the vertices
 };
 
 struct phg_hip_vec {
@@ -137,6 +138,8 @@ struct phg_hip_mat {
     // nodal coordinates of the scalar DOFs [ndof][3], kept from assembly to
     // derive the solver numbering
     double *d_xyz = nullptr;
+    double xyz_lo[3] = {0, 0, 0}, xyz_hi[3] = {0, 0, 0};   // bounding box of the nodes
+    int xyz_box = 0;                                       // xyz_lo/hi set
     int64_t ndof = 0;
 
     // Solver copy: the eliminated matrix under a symmetric renumbering that

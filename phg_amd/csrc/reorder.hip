@@ -145,16 +145,27 @@ int build_solver_copy(phg_hip_mat *A)
     }
     else {
         const int64_t nd = A->ndof;
-        // bounding box of the nodes (host pass over the coordinates)
-        std::vector<double> xyz((size_t)nd * 3);
-        if ((rc = download(xyz.data(), A->d_xyz, xyz.size() * sizeof(double))))
-            return rc;
-        double lo[3] = {xyz[0], xyz[1], xyz[2]}, hi[3] = {xyz[0], xyz[1], xyz[2]};
-        for (int64_t i = 0; i < nd; i++)
+        // bounding box of the nodes: the mesh's, kept from the space, or a
+        // host pass over the coordinates
+        double lo[3], hi[3];
+        if (A->xyz_box) {
             for (int c = 0; c < 3; c++) {
-                lo[c] = std::min(lo[c], xyz[(size_t)(3 * i + c)]);
-                hi[c] = std::max(hi[c], xyz[(size_t)(3 * i + c)]);
+                lo[c] = A->xyz_lo[c];
+                hi[c] = A->xyz_hi[c];
             }
+        }
+        else {
+            std::vector<double> xyz((size_t)nd * 3);
+            if ((rc = download(xyz.data(), A->d_xyz, xyz.size() * sizeof(double))))
+                return rc;
+            for (int c = 0; c < 3; c++)
+                lo[c] = hi[c] = xyz[(size_t)c];
+            for (int64_t i = 0; i < nd; i++)
+                for (int c = 0; c < 3; c++) {
+                    lo[c] = std::min(lo[c], xyz[(size_t)(3 * i + c)]);
+                    hi[c] = std::max(hi[c], xyz[(size_t)(3 * i + c)]);
+                }
+        }
         // one scale for all axes keeps the curve isotropic
         double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
         double sc = ext > 0 ? 1.0 / ext : 1.0;
