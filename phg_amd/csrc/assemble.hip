@@ -1059,73 +1059,223 @@ __global__ void __launch_bounds__(64) k_row_walk_wave(WalkArgs w)
     }
 }
 
-__global__ void k_compact(int64_t n, const int64_t *scr_off,
-                          const int32_t *scols, const double *svals,
-                          const int64_t *rowptr, int32_t *cols, double *vals)
+// The same insertion with a hash table: a group of G lanes per row (64/G rows
+// per wavefront) steps through the row's incident elements in order; lane b
+// takes column b of the element row.  The columns of one element row are
+// distinct, so within a step each lane owns a different entry: a column
+// found in the row's table adds its value to that entry, a new one opens the
+// next entry, numbered by lane order.  Entries thus open in PHG's append
+// order and every entry sums its contributions in element order starting
+// from the first, as the row list does.  Rows with more than E distinct
+// columns, and Dirichlet rows, take the serial path.
+template <int G, int E>
+__global__ void __launch_bounds__(64) k_row_walk_hash(WalkArgs w)
 {
-    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    constexpr int TS = 2 * E, NG = 64 / G;
+    __shared__ int32_t key[NG][TS];
+    __shared__ int32_t ent[NG][TS];
+    __shared__ int32_t ecol[NG][E];
+    __shared__ double eval[NG][E];
+    const int lane = threadIdx.x, grp = lane / G, b = lane % G;
+    const uint64_t gm = (G == 64 ? ~0ull : (1ull << (G % 64)) - 1ull) << (grp * G % 64);
+    const uint64_t below = (1ull << lane) - 1ull;
+    const int M = w.N * w.dim;
+    for (int64_t r0 = (int64_t)blockIdx.x * NG; r0 < w.n; r0 += (int64_t)gridDim.x * NG) {
+        const int64_t r = r0 + grp;
+        const bool live = r < w.n;
+        const int64_t t0 = live ? w.inc_off[r] : 0;
+        const int ninc = live ? (int)(w.inc_off[r + 1] - t0) : 0;
+        const bool serial = live && w.bdry[r] != 0;
+        const int steps = serial ? 0 : ninc;
+        int smax = steps;
+        for (int o = 32; o >= 1; o >>= 1)
+            smax = max(smax, __shfl_xor(smax, o, 64));
+        for (int k = b; k < TS; k += G)
+            key[grp][k] = -1;
+        __syncthreads();
+        int32_t nc = 0;
+        bool over = false;
+        double rhs = 0.0;
+        for (int t = 0; t < smax; t++) {
+            int32_t c = -1;
+            double v = 0.0;
+            if (t < steps) {
+                const int32_t code = w.inc[t0 + t];
+                if (b < M) {
+                    const int64_t e = code / M;
+                    v = w.K[(int64_t)code * M + b];   // [e][a][b], code = e M + a
+                    if (v != 0.0)                     // matvec.c:5460
+                        c = (int32_t)row_of(w.map, e, b, w.N, w.dim);
+                }
+                if (b == 0) {
+                    const double bval = w.B[code];
+                    if (bval != 0.0)                  // matvec.c:5708
+                        rhs += bval;
+                }
+            }
+            int slot = -1;
+            bool fresh = false;
+            if (c >= 0) {
+                slot = (int)(((uint32_t)c * 2654435761u) >> 8) & (TS - 1);
+                for (int probe = 0; probe < TS; probe++) {
+                    const int32_t old = atomicCAS(&key[grp][slot], -1, c);
+                    if (old == -1) {
+                        fresh = true;
+                        break;
+                    }
+                    if (old == c)
+                        break;
+                    slot = (slot + 1) & (TS - 1);
+                }
+            }
+            const uint64_t fm = __ballot(fresh) & gm;
+            if (fresh) {
+                const int32_t p = nc + __popcll(fm & below);
+                if (p < E) {
+                    ent[grp][slot] = p;
+                    ecol[grp][p] = c;
+                    eval[grp][p] = v;
+                }
+            }
+            nc += __popcll(fm);
+            over = over || nc > E;
+            __syncthreads();
+            if (c >= 0 && !fresh && !over)
+                eval[grp][ent[grp][slot]] += v;
+            __syncthreads();
+        }
+        if (live && !serial && !over) {
+            int32_t *ocols = w.scols + w.scr_off[r];
+            double *ovals = w.svals + w.scr_off[r];
+            for (int k = b; k < nc; k += G) {
+                ocols[k] = ecol[grp][k];
+                ovals[k] = eval[grp][k];
+            }
+            if (b == 0) {
+                w.count[r] = nc;
+                w.rhs[r] = rhs;
+            }
+        }
+        else if (live && b == 0)
+            walk_row_serial(w, r);
+        __syncthreads();
+    }
+}
+
+// The row copies below give each row a group of RG consecutive lanes, so a
+// row's entries move in coalesced runs; RPB rows per 256-thread block.
+constexpr int RG = 16, RPB = 256 / RG;
+
+__device__ inline uint64_t group_mask(int lane)
+{
+    return ((1ull << RG) - 1ull) << (lane & ~(RG - 1));
+}
+
+__global__ void __launch_bounds__(256) k_compact(int64_t n, const int64_t *scr_off,
+                                                 const int32_t *scols, const double *svals,
+                                                 const int64_t *rowptr, int32_t *cols,
+                                                 double *vals)
+{
+    const int64_t r = (int64_t)blockIdx.x * RPB + threadIdx.x / RG;
+    const int sl = threadIdx.x % RG;
     if (r >= n)
         return;
-    int64_t s = scr_off[r], d = rowptr[r], len = rowptr[r + 1] - rowptr[r];
-    for (int64_t i = 0; i < len; i++) {
+    const int64_t s = scr_off[r], d = rowptr[r], len = rowptr[r + 1] - d;
+    for (int64_t i = sl; i < len; i += RG) {
         cols[d + i] = scols[s + i];
         vals[d + i] = svals[s + i];
     }
 }
 
 //-------------------------------------------------------------- elimination
-__global__ void k_elim_count(int64_t n, const int32_t *rowptr,
-                             const int32_t *cols, const uint8_t *bdry,
-                             int32_t *count)
+// Entry j of row r stays when its column is r or not a Dirichlet row.
+__global__ void __launch_bounds__(256) k_elim_count(int64_t n, const int32_t *rowptr,
+                                                    const int32_t *cols, const uint8_t *bdry,
+                                                    int32_t *count)
 {
-    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t r = (int64_t)blockIdx.x * RPB + threadIdx.x / RG;
+    const int sl = threadIdx.x % RG, lane = threadIdx.x % 64;
     if (r >= n)
         return;
+    const uint64_t gm = group_mask(lane);
     int32_t k = 0;
-    for (int32_t j = rowptr[r]; j < rowptr[r + 1]; j++) {
-        int32_t c = cols[j];
-        if (c == r || !bdry[c])
-            k++;
+    for (int32_t j0 = rowptr[r]; j0 < rowptr[r + 1]; j0 += RG) {
+        const int32_t j = j0 + sl;
+        bool keep = false;
+        if (j < rowptr[r + 1]) {
+            const int32_t c = cols[j];
+            keep = c == r || !bdry[c];
+        }
+        k += __popcll(__ballot(keep) & gm);
     }
-    count[r] = k;
+    if (sl == 0)
+        count[r] = k;
 }
 
-__global__ void k_elim_fill(int64_t n, const int32_t *rowptr,
-                            const int32_t *cols, const double *vals,
-                            const double *rhs, const uint8_t *bdry,
-                            const int64_t *nrowptr, int32_t *ncols,
-                            double *nvals, double *nrhs, double *diag)
+// Kept entries are written in their order; the rhs update runs over the
+// row's Dirichlet columns in column-list order, as the serial loop of
+// solver.c:707-725 does, so its rounding is the reference's.
+__global__ void __launch_bounds__(256) k_elim_fill(int64_t n, const int32_t *rowptr,
+                                                   const int32_t *cols, const double *vals,
+                                                   const double *rhs, const uint8_t *bdry,
+                                                   const int64_t *nrowptr, int32_t *ncols,
+                                                   double *nvals, double *nrhs, double *diag)
 {
-    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t r = (int64_t)blockIdx.x * RPB + threadIdx.x / RG;
+    const int sl = threadIdx.x % RG, lane = threadIdx.x % 64;
     if (r >= n)
         return;
+    const uint64_t gm = group_mask(lane), below = (1ull << lane) - 1ull;
+    const int32_t end = rowptr[r + 1];
     int64_t w = nrowptr[r];
     double b = rhs[r], dg = 0.0;
     bool have_diag = false;
-    for (int32_t j = rowptr[r]; j < rowptr[r + 1]; j++) {
-        int32_t c = cols[j];
-        double v = vals[j];
-        if (c == r || !bdry[c]) {
-            ncols[w] = c;
-            nvals[w++] = v;
-            if (c == r && !have_diag) {
-                dg = v;
-                have_diag = true;
-            }
+    for (int32_t j0 = rowptr[r]; j0 < end; j0 += RG) {
+        const int32_t j = j0 + sl;
+        int32_t c = -1;
+        double v = 0.0;
+        bool keep = false, elim = false, isdiag = false;
+        if (j < end) {
+            c = cols[j];
+            v = vals[j];
+            keep = c == r || !bdry[c];
+            elim = bdry[c] && c != r;
+            isdiag = c == r;
         }
-        if (bdry[c] && c != r) {
-            // solver.c:722, d_k is the diagonal of boundary row k
+        const uint64_t km = __ballot(keep) & gm;
+        if (keep) {
+            const int64_t o = w + __popcll(km & below);
+            ncols[o] = c;
+            nvals[o] = v;
+        }
+        w += __popcll(km);
+        const uint64_t dm = __ballot(isdiag) & gm;
+        if (dm != 0 && !have_diag) {
+            dg = __shfl(v, __ffsll((unsigned long long)dm) - 1, 64);
+            have_diag = true;
+        }
+        // solver.c:722, d_k is the diagonal of boundary row k
+        double t = 0.0;
+        if (elim) {
             double dk = 0.0;
             for (int32_t jj = rowptr[c]; jj < rowptr[c + 1]; jj++)
                 if (cols[jj] == c) {
                     dk = vals[jj];
                     break;
                 }
-            b -= 1.0 * rhs[c] * v / dk;
+            t = 1.0 * rhs[c] * v / dk;
+        }
+        uint64_t em = __ballot(elim) & gm;
+        while (em != 0) {
+            const int src = __ffsll((unsigned long long)em) - 1;
+            b -= __shfl(t, src, 64);
+            em &= em - 1;
         }
     }
-    nrhs[r] = b;
-    diag[r] = dg;
+    if (sl == 0) {
+        nrhs[r] = b;
+        diag[r] = dg;
+    }
 }
 
 //------------------------------------------------------------------ scan
@@ -1655,9 +1805,18 @@ extern "C" int phg_hip_assemble(phg_hip_space *sp, const phg_hip_problem *pb,
         w.svals = svals;
         w.count = count;
         w.rhs = A->d_rhs;
-        // one wavefront per row
-        hipLaunchKernelGGL(k_row_walk_wave, dim3((unsigned)std::min<int64_t>(n, 1 << 16)),
-                           dim3(64), 0, s, w);
+        // 64/G rows per wavefront, G the smallest of 16, 32, 64 holding
+        // an element row
+        const unsigned grid = (unsigned)std::min<int64_t>((n + 3) / 4, 1 << 16);
+        if (M <= 16)
+            hipLaunchKernelGGL((k_row_walk_hash<16, 128>), dim3(grid), dim3(64), 0, s, w);
+        else if (M <= 32)
+            hipLaunchKernelGGL((k_row_walk_hash<32, 256>), dim3(grid), dim3(64), 0, s, w);
+        else if (M <= 64)
+            hipLaunchKernelGGL((k_row_walk_hash<64, 512>), dim3(grid), dim3(64), 0, s, w);
+        else
+            hipLaunchKernelGGL(k_row_walk_wave, dim3((unsigned)std::min<int64_t>(n, 1 << 16)),
+                               dim3(64), 0, s, w);
         PHG_HIP_CHECK(hipGetLastError());
     }
     if ((rc = phg::dmalloc((void **)&rowptr64, (size_t)(n + 1) * sizeof(int64_t))) ||
@@ -1673,7 +1832,7 @@ extern "C" int phg_hip_assemble(phg_hip_space *sp, const phg_hip_problem *pb,
         (rc = phg::dmalloc((void **)&A->d_cols, (size_t)(nnz + 1) * sizeof(int32_t))) ||
         (rc = phg::dmalloc((void **)&A->d_vals, (size_t)(nnz + 1) * sizeof(double))))
         goto done;
-    hipLaunchKernelGGL(k_compact, dim3(nblk(n, 256)), dim3(256), 0, s, n,
+    hipLaunchKernelGGL(k_compact, dim3(nblk(n, RPB)), dim3(256), 0, s, n,
                        (const int64_t *)scr_off, (const int32_t *)scols,
                        (const double *)svals, (const int64_t *)rowptr64,
                        A->d_cols, A->d_vals);
@@ -1721,7 +1880,7 @@ extern "C" int phg_hip_mat_eliminate(phg_hip_mat *A)
     if ((rc = phg::dmalloc((void **)&count, (size_t)(n + 1) * sizeof(int32_t))) ||
         (rc = phg::dmalloc((void **)&nrow, (size_t)(n + 1) * sizeof(int64_t))))
         goto fail;
-    hipLaunchKernelGGL(k_elim_count, dim3(nblk(n, 256)), dim3(256), 0, s, n,
+    hipLaunchKernelGGL(k_elim_count, dim3(nblk(n, RPB)), dim3(256), 0, s, n,
                        (const int32_t *)A->d_rowptr, (const int32_t *)A->d_cols,
                        (const uint8_t *)A->d_bdry_row, count);
     if ((rc = scan_i32(n, count, nrow, &nnz, s)))
@@ -1732,7 +1891,7 @@ extern "C" int phg_hip_mat_eliminate(phg_hip_mat *A)
         (rc = phg::dmalloc((void **)&diag, (size_t)n * sizeof(double))) ||
         (rc = phg::dmalloc((void **)&nrow32, (size_t)(n + 1) * sizeof(int32_t))))
         goto fail;
-    hipLaunchKernelGGL(k_elim_fill, dim3(nblk(n, 256)), dim3(256), 0, s, n,
+    hipLaunchKernelGGL(k_elim_fill, dim3(nblk(n, RPB)), dim3(256), 0, s, n,
                        (const int32_t *)A->d_rowptr, (const int32_t *)A->d_cols,
                        (const double *)A->d_vals, (const double *)A->d_rhs,
                        (const uint8_t *)A->d_bdry_row, (const int64_t *)nrow,

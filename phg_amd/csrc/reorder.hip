@@ -73,22 +73,27 @@ __global__ void k_perm_lengths(int64_t n, const int32_t *rp,
     len[i] = rp[r + 1] - rp[r];
 }
 
-__global__ void k_perm_fill(int64_t n, const int32_t *rp, const int32_t *ci,
-                            const double *va, const double *diag,
-                            const int32_t *perm, const int32_t *iperm,
-                            const int64_t *qrp, int32_t *qci, double *qva,
-                            double *qdiag)
+// one group of PG lanes per row: the row's entries move in coalesced runs
+constexpr int PG = 16, PRB = TB / PG;
+
+__global__ void __launch_bounds__(TB) k_perm_fill(int64_t n, const int32_t *rp, const int32_t *ci,
+                                                  const double *va, const double *diag,
+                                                  const int32_t *perm, const int32_t *iperm,
+                                                  const int64_t *qrp, int32_t *qci, double *qva,
+                                                  double *qdiag)
 {
-    int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x;
+    const int64_t i = (int64_t)blockIdx.x * PRB + threadIdx.x / PG;
+    const int sl = threadIdx.x % PG;
     if (i >= n)
         return;
-    int32_t r = perm[i];
-    int64_t d = qrp[i];
-    for (int32_t j = rp[r]; j < rp[r + 1]; j++, d++) {
-        qci[d] = iperm[ci[j]];
-        qva[d] = va[j];
+    const int32_t r = perm[i], b = rp[r], len = rp[r + 1] - b;
+    const int64_t d = qrp[i];
+    for (int32_t j = sl; j < len; j += PG) {
+        qci[d + j] = iperm[ci[b + j]];
+        qva[d + j] = va[b + j];
     }
-    qdiag[i] = diag[r];
+    if (sl == 0)
+        qdiag[i] = diag[r];
 }
 
 __global__ void k_narrow(int64_t n, const int64_t *a, int32_t *b)
@@ -230,7 +235,7 @@ int build_solver_copy(phg_hip_mat *A)
             cleanup();
             return rc;
         }
-        hipLaunchKernelGGL(k_perm_fill, dim3(nb(n)), dim3(TB), 0, s, n,
+        hipLaunchKernelGGL(k_perm_fill, dim3((unsigned)((n + PRB - 1) / PRB)), dim3(TB), 0, s, n,
                            (const int32_t *)A->d_rowptr,
                            (const int32_t *)A->d_cols, (const double *)A->d_vals,
                            (const double *)A->d_diag, (const int32_t *)A->q_perm,

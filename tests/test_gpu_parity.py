@@ -114,9 +114,12 @@ def test_csr_bitexact(hip, kind, p, n, qc):
     assert _bits_equal(rhs, Ao.rhs)
 
 
-def test_elasticity_csr_bitexact(hip):
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_elasticity_csr_bitexact(hip, p):
+    # element rows of 12, 30 and 60 entries: four, two and one rows per
+    # wavefront in the insertion kernel
     pa = hip
-    c = elasticity_case(2)
+    c = elasticity_case(2, p)
     pb = pa.Problem.elasticity()
     bd3 = np.repeat(c["bdry"], 3)
     sp, A, Ao = _assemble_both(pa, c, pb)
