@@ -13,14 +13,14 @@
 //                      written to HBM.
 //   k_inc_*            element-row incidence (row -> (element, local index)
 //                      sorted by element index), built with a counting sort.
-//   k_row_walk_wave    one wavefront per row replays PHG's row-list insertion
-//                      (matvec.c:3045 add_columns_to_row: linear search,
-//                      add or append, zeros skipped) over the row's incident
-//                      elements in element order, plus the RHS accumulation of
-//                      matvec.c:5661: first occurrences found in parallel in
-//                      LDS, each entry summed in contribution order.
-//                      Dirichlet rows and rows with more than WALK_MAX
-//                      contributions take the serial replay on one lane.
+//   k_row_walk_hash    a group of 16/32/64 lanes per row replays PHG's
+//                      row-list insertion (matvec.c:3045 add_columns_to_row:
+//                      linear search, add or append, zeros skipped) over the
+//                      row's incident elements in element order, plus the RHS
+//                      accumulation of matvec.c:5661, with the row's columns
+//                      in an LDS hash table.  Dirichlet rows, and rows with
+//                      more distinct columns than the table holds, are
+//                      replayed serially.
 //   k_elim_*           serial path of phgMatAssemble (matvec.c:4430
 //                      save_bdry_columns(TRUE)) and solver.c:508 update_rhs.
 #include "phg_hip_internal.h"
@@ -975,90 +975,6 @@ __device__ inline void walk_row_serial(const WalkArgs &w, int64_t r)
     w.rhs[r] = rhs;
 }
 
-// The same insertion, one wavefront per row.  The row's contributions --
-// (incident element, local row a) in element order, then columns b of the
-// element row in order -- are listed in LDS with their column (-1 for an
-// exact zero, which PHG never inserts).  A contribution that is the first
-// with its column opens an entry; entries are numbered in contribution
-// order (the order columns are appended to PHG's row list), and each entry's
-// value is the sum of its contributions in contribution order, starting
-// from the first -- the additions PHG's row list performs.  Rows with more
-// than WALK_MAX contributions, and Dirichlet rows, take the serial path.
-constexpr int WALK_MAX = 1024;
-__global__ void __launch_bounds__(64) k_row_walk_wave(WalkArgs w)
-{
-    __shared__ int32_t col[WALK_MAX];
-    __shared__ double val[WALK_MAX];
-    __shared__ int32_t pos[WALK_MAX];
-    const int lane = threadIdx.x;
-    const int M = w.N * w.dim;
-    for (int64_t r = blockIdx.x; r < w.n; r += gridDim.x) {
-        const int64_t t0 = w.inc_off[r], ninc = w.inc_off[r + 1] - t0;
-        const int64_t K = ninc * M;
-        if (w.bdry[r] != 0 || K > WALK_MAX) {
-            if (lane == 0)
-                walk_row_serial(w, r);
-            continue;
-        }
-        for (int k = lane; k < K; k += 64) {
-            const int32_t code = w.inc[t0 + k / M];
-            const int64_t e = code / M;
-            const int a = code % M, b = k % M;
-            const double v = w.K[(e * M + a) * (int64_t)M + b];
-            col[k] = v == 0.0 ? -1 : (int32_t)row_of(w.map, e, b, w.N, w.dim);
-            val[k] = v;
-        }
-        __syncthreads();
-        // first occurrence of each column; entry numbers by a wavefront scan
-        int32_t base = 0;
-        for (int k0 = 0; k0 < K; k0 += 64) {
-            const int k = k0 + lane;
-            bool first = false;
-            if (k < K && col[k] >= 0) {
-                first = true;
-                const int32_t c = col[k];
-                for (int j = 0; j < k; j++)
-                    if (col[j] == c) {
-                        first = false;
-                        break;
-                    }
-            }
-            const uint64_t m = __ballot(first);
-            if (first)
-                pos[k] = base + __popcll(m & ((1ull << lane) - 1ull));
-            else if (k < K)
-                pos[k] = -1;
-            base += __popcll(m);
-        }
-        __syncthreads();
-        int32_t *ocols = w.scols + w.scr_off[r];
-        double *ovals = w.svals + w.scr_off[r];
-        for (int k = lane; k < K; k += 64) {
-            const int32_t p = pos[k];
-            if (p < 0)
-                continue;
-            const int32_t c = col[k];
-            double sum = val[k];
-            for (int j = k + 1; j < K; j++)
-                if (col[j] == c)
-                    sum += val[j];
-            ocols[p] = c;
-            ovals[p] = sum;
-        }
-        if (lane == 0) {
-            double rhs = 0.0;
-            for (int64_t t = 0; t < ninc; t++) {
-                const double bval = w.B[w.inc[t0 + t]];   // [nelem][M] at e M + a
-                if (bval != 0.0)                          // matvec.c:5708
-                    rhs += bval;
-            }
-            w.count[r] = base;
-            w.rhs[r] = rhs;
-        }
-        __syncthreads();
-    }
-}
-
 // The same insertion with a hash table: a group of G lanes per row (64/G rows
 // per wavefront) steps through the row's incident elements in order; lane b
 // takes column b of the element row.  The columns of one element row are
@@ -1812,11 +1728,8 @@ extern "C" int phg_hip_assemble(phg_hip_space *sp, const phg_hip_problem *pb,
             hipLaunchKernelGGL((k_row_walk_hash<16, 128>), dim3(grid), dim3(64), 0, s, w);
         else if (M <= 32)
             hipLaunchKernelGGL((k_row_walk_hash<32, 256>), dim3(grid), dim3(64), 0, s, w);
-        else if (M <= 64)
+        else   // M <= 60 for P1-P3, scalar or vector (check_problem)
             hipLaunchKernelGGL((k_row_walk_hash<64, 512>), dim3(grid), dim3(64), 0, s, w);
-        else
-            hipLaunchKernelGGL(k_row_walk_wave, dim3((unsigned)std::min<int64_t>(n, 1 << 16)),
-                               dim3(64), 0, s, w);
         PHG_HIP_CHECK(hipGetLastError());
     }
     if ((rc = phg::dmalloc((void **)&rowptr64, (size_t)(n + 1) * sizeof(int64_t))) ||
