@@ -38,10 +38,15 @@
 // element contributes (P2: 64 B against ~240 B).
 #include "phg_hip_internal.h"
 
+#include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
+#include <memory>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -54,6 +59,67 @@ constexpr int EW = EB / 64;
 constexpr int NC = 6;             // coefficients per element
 constexpr int FB = 256;           // work-items of the fix-up kernel
 constexpr uint32_t KD_NONE = 0xffffffffu;
+
+// Host set-up work split over up to 16 threads: f(lo, hi, t) for contiguous
+// chunks [lo, hi) of [0, n), at least `grain` items each, t the chunk
+// number.  A thread that cannot be started runs its chunk on the caller.
+inline int host_threads(int64_t n, int64_t grain = 1024)
+{
+    const int64_t hw = std::max(1u, std::thread::hardware_concurrency());
+    return (int)std::max<int64_t>(1, std::min<int64_t>({16, hw, n / grain}));
+}
+
+template <typename F>
+void host_parallel(int64_t n, F &&f, int64_t grain = 1024)
+{
+    const int T = host_threads(n, grain);
+    std::vector<std::thread> th;
+    for (int t = 1; t < T; t++) {
+        const int64_t lo = n * t / T, hi = n * (t + 1) / T;
+        try {
+            th.emplace_back([&f, lo, hi, t] { f(lo, hi, t); });
+        }
+        catch (...) {
+            f(lo, hi, t);
+        }
+    }
+    f(0, n / T, 0);
+    for (auto &x : th)
+        x.join();
+}
+
+// sort of v (std::sort per chunk on the host threads, then pairwise merges)
+template <typename T>
+void host_sort(std::vector<T> &v)
+{
+    const int64_t n = (int64_t)v.size();
+    const int nt = host_threads(n);
+    std::vector<int64_t> cut((size_t)nt + 1);
+    for (int t = 0; t <= nt; t++)
+        cut[(size_t)t] = n * t / nt;
+    host_parallel(n, [&](int64_t lo, int64_t hi, int) { std::sort(v.begin() + lo, v.begin() + hi); });
+    std::vector<T> w((size_t)n);
+    for (int width = 1; width < nt; width *= 2) {
+        std::vector<std::thread> th;
+        for (int t = 0; t < nt; t += 2 * width) {
+            const int64_t a = cut[(size_t)t], m = cut[(size_t)std::min(t + width, nt)],
+                          b = cut[(size_t)std::min(t + 2 * width, nt)];
+            auto merge = [&v, &w, a, m, b] {
+                std::merge(v.begin() + a, v.begin() + m, v.begin() + m, v.begin() + b,
+                           w.begin() + a);
+            };
+            try {
+                th.emplace_back(merge);
+            }
+            catch (...) {
+                merge();
+            }
+        }
+        for (auto &x : th)
+            x.join();
+        v.swap(w);
+    }
+}
 
 template <int P> struct EopShape {
     static constexpr int NB = P == 1 ? 4 : (P == 2 ? 10 : 20);
@@ -683,9 +749,21 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
         return set_error(PHG_HIP_E_ARG,
                          "phg_hip_mat_element_operator: needs the eliminated matrix "
                          "of a scalar stiffness problem (no mass term) on this space");
+    // PHG_HIP_SETUP_TRACE=1 prints the time of each set-up phase to stderr
+    static const bool trace = getenv("PHG_HIP_SETUP_TRACE") != nullptr;
+    auto t_last = std::chrono::steady_clock::now();
+    auto mark = [&](const char *what) {
+        if (!trace)
+            return;
+        const auto t = std::chrono::steady_clock::now();
+        fprintf(stderr, "element operator set-up: %-22s %8.2f ms\n", what,
+                std::chrono::duration<double, std::milli>(t - t_last).count());
+        t_last = t;
+    };
     if ((rc = ensure_fp32_copy(A)))
         return rc;
     free_eop(A);
+    mark("fp32 copy");
     hipStream_t s = ctx()->stream;
     const int p = sp->order, nb = sp->nbas;
     const int64_t ne = sp->nelem, n = A->n;
@@ -710,6 +788,7 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
     }
     else
         std::iota(iperm.begin(), iperm.end(), 0);
+    mark("downloads");
     // elements in Morton order of their centroids
     std::vector<int32_t> eperm((size_t)ne);
     {
@@ -721,34 +800,42 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
             }
         double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
         const double sc = ext > 0 ? 2097151.0 / ext : 0.0;
-        std::vector<uint64_t> key((size_t)ne);
-        for (int64_t e = 0; e < ne; e++) {
-            uint64_t q[3];
-            for (int c = 0; c < 3; c++) {
-                double m = 0.0;
-                for (int v = 0; v < 4; v++)
-                    m += xyz[(size_t)(3 * ev[(size_t)(4 * e + v)] + c)];
-                double f = (0.25 * m - lo[c]) * sc;
-                q[c] = (uint64_t)std::min(std::max(f, 0.0), 2097151.0);
+        // (key, element) pairs: sorting them is a stable sort by key
+        std::vector<std::pair<uint64_t, int32_t>> key((size_t)ne);
+        host_parallel(ne, [&](int64_t e0, int64_t e1, int) {
+            for (int64_t e = e0; e < e1; e++) {
+                uint64_t q[3];
+                for (int c = 0; c < 3; c++) {
+                    double m = 0.0;
+                    for (int v = 0; v < 4; v++)
+                        m += xyz[(size_t)(3 * ev[(size_t)(4 * e + v)] + c)];
+                    double f = (0.25 * m - lo[c]) * sc;
+                    q[c] = (uint64_t)std::min(std::max(f, 0.0), 2097151.0);
+                }
+                key[(size_t)e] = {spread3(q[0]) | spread3(q[1]) << 1 | spread3(q[2]) << 2,
+                                  (int32_t)e};
             }
-            key[(size_t)e] = spread3(q[0]) | spread3(q[1]) << 1 | spread3(q[2]) << 2;
-        }
-        std::iota(eperm.begin(), eperm.end(), 0);
-        std::stable_sort(eperm.begin(), eperm.end(),
-                         [&](int32_t a, int32_t b) { return key[(size_t)a] < key[(size_t)b]; });
+        });
+        host_sort(key);
+        for (int64_t e = 0; e < ne; e++)
+            eperm[(size_t)e] = key[(size_t)e].second;
     }
+    mark("element order");
     // element DOF map in the solver numbering, Dirichlet DOFs -1
     std::vector<int32_t> emap((size_t)ne * nb);
     std::vector<uint8_t> bq((size_t)n);
     for (int64_t i = 0; i < n; i++)
         bq[(size_t)i] = brow[(size_t)(A->q_perm ? perm[(size_t)i] : i)];
-    for (int64_t e2 = 0; e2 < ne; e2++) {
-        const int64_t e = eperm[(size_t)e2];
-        for (int a = 0; a < nb; a++) {
-            const int32_t r = iperm[(size_t)map[(size_t)(e * nb + a)]];
-            emap[(size_t)(e2 * nb + a)] = bq[(size_t)r] ? -1 : r;
+    host_parallel(ne, [&](int64_t lo, int64_t hi, int) {
+        for (int64_t e2 = lo; e2 < hi; e2++) {
+            const int64_t e = eperm[(size_t)e2];
+            for (int a = 0; a < nb; a++) {
+                const int32_t r = iperm[(size_t)map[(size_t)(e * nb + a)]];
+                emap[(size_t)(e2 * nb + a)] = bq[(size_t)r] ? -1 : r;
+            }
         }
-    }
+    });
+    mark("element map");
     // element blocks: consecutive elements, at most eblk of them touching
     // at most kmax rows; the touched rows of a block in first-touch order
     const int eblk = p == 1 ? EopShape<1>::EBLK : (p == 2 ? EopShape<2>::EBLK : EopShape<3>::EBLK);
@@ -791,7 +878,7 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
     // per block: interior rows (touched by this block only) first, then
     // shared rows; slots of a row consecutive, in element order; the
     // partials of a shared row in block order
-    std::vector<int32_t> eb_nint((size_t)nblk), ppos((size_t)nent, -1), kidx((size_t)n, 0);
+    std::vector<int32_t> eb_nint((size_t)nblk), ppos((size_t)nent, -1);
     std::vector<uint16_t> rs((size_t)(nent + nblk));
     std::vector<uint32_t> kd((size_t)ne * nb, KD_NONE);
     std::vector<int32_t> fxi((size_t)n, -1), seen((size_t)n, 0), fx_rows, fx_off{0};
@@ -803,86 +890,105 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
             fx_rows.push_back((int32_t)r);
             fx_off.push_back(fx_off.back() + cnt[(size_t)r]);
         }
+    mark("blocks");
+    // blocks are independent here (disjoint ranges of rows, rs and kd); each
+    // host thread keeps its own row -> local index map
+    host_parallel(nblk, [&](int64_t b0, int64_t b1, int) {
+        std::unique_ptr<int32_t[]> kid(new int32_t[(size_t)n]);
+        std::vector<int32_t> first, ns, ord, sc;
+        for (int64_t b = b0; b < b1; b++) {
+            const int32_t k0 = eb_row[(size_t)b], nk = eb_row[(size_t)b + 1] - k0;
+            const int32_t e0 = eb_elem[(size_t)b], e1 = eb_elem[(size_t)b + 1];
+            // slots per touched row (first-touch order)
+            first.assign(rows.begin() + k0, rows.begin() + k0 + nk);
+            ns.assign((size_t)nk, 0);
+            for (int32_t k = 0; k < nk; k++)
+                kid[(size_t)first[(size_t)k]] = k;
+            for (int64_t e = e0; e < e1; e++)
+                for (int a = 0; a < nb; a++) {
+                    const int32_t r = emap[(size_t)(e * nb + a)];
+                    if (r >= 0)
+                        ns[(size_t)kid[(size_t)r]]++;
+                }
+            // interior rows first, then shared; each group by decreasing slot
+            // count, so the row sums of a wavefront have similar lengths
+            ord.resize((size_t)nk);
+            std::iota(ord.begin(), ord.end(), 0);
+            std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t c) {
+                const bool ia = cnt[(size_t)first[(size_t)a]] == 1,
+                           ic = cnt[(size_t)first[(size_t)c]] == 1;
+                if (ia != ic)
+                    return ia;
+                return ns[(size_t)a] > ns[(size_t)c];
+            });
+            int32_t ni = 0;
+            for (int32_t k = 0; k < nk; k++) {
+                const int32_t r = first[(size_t)ord[(size_t)k]];
+                rows[(size_t)(k0 + k)] = r;
+                if (cnt[(size_t)r] == 1)
+                    ni++;
+            }
+            eb_nint[(size_t)b] = ni;
+            int32_t k;
+            for (k = 0; k < nk; k++)
+                kid[(size_t)rows[(size_t)(k0 + k)]] = k;
+            sc.assign((size_t)nk + 1, 0);
+            for (int64_t e = e0; e < e1; e++)
+                for (int a = 0; a < nb; a++) {
+                    const int32_t r = emap[(size_t)(e * nb + a)];
+                    if (r >= 0)
+                        sc[(size_t)kid[(size_t)r] + 1]++;
+                }
+            for (k = 0; k < nk; k++)
+                sc[(size_t)k + 1] += sc[(size_t)k];
+            for (k = 0; k <= nk; k++)
+                rs[(size_t)(k0 + b + k)] = (uint16_t)sc[(size_t)k];
+            for (int64_t e = e0; e < e1; e++)
+                for (int a = 0; a < nb; a++) {
+                    const int32_t r = emap[(size_t)(e * nb + a)];
+                    if (r >= 0) {
+                        const int32_t kl = kid[(size_t)r];
+                        kd[(size_t)(e * nb + a)] = (uint32_t)kl | (uint32_t)sc[(size_t)kl]++ << 16;
+                    }
+                }
+        }
+    }, 16);
+    // partials of the shared rows in block order
     int64_t nint = 0;
     for (int64_t b = 0; b < nblk; b++) {
         const int32_t k0 = eb_row[(size_t)b], nk = eb_row[(size_t)b + 1] - k0;
-        const int32_t e0 = eb_elem[(size_t)b], e1 = eb_elem[(size_t)b + 1];
-        // slots per touched row (first-touch order)
-        std::vector<int32_t> first(rows.begin() + k0, rows.begin() + k0 + nk), ns((size_t)nk, 0);
-        for (int32_t k = 0; k < nk; k++)
-            kidx[(size_t)first[(size_t)k]] = k;
-        for (int64_t e = e0; e < e1; e++)
-            for (int a = 0; a < nb; a++) {
-                const int32_t r = emap[(size_t)(e * nb + a)];
-                if (r >= 0)
-                    ns[(size_t)kidx[(size_t)r]]++;
-            }
-        // interior rows first, then shared; each group by decreasing slot
-        // count, so the row sums of a wavefront have similar lengths
-        std::vector<int32_t> ord((size_t)nk);
-        std::iota(ord.begin(), ord.end(), 0);
-        std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t c) {
-            const bool ia = cnt[(size_t)first[(size_t)a]] == 1, ic = cnt[(size_t)first[(size_t)c]] == 1;
-            if (ia != ic)
-                return ia;
-            return ns[(size_t)a] > ns[(size_t)c];
-        });
-        int32_t ni = 0;
-        for (int32_t k = 0; k < nk; k++) {
-            const int32_t r = first[(size_t)ord[(size_t)k]];
-            rows[(size_t)(k0 + k)] = r;
-            if (cnt[(size_t)r] == 1)
-                ni++;
-        }
-        eb_nint[(size_t)b] = ni;
-        nint += ni;
-        int32_t k;
-        for (k = 0; k < nk; k++)
-            kidx[(size_t)rows[(size_t)(k0 + k)]] = k;
-        std::vector<int32_t> sc((size_t)nk + 1, 0);
-        for (int64_t e = e0; e < e1; e++)
-            for (int a = 0; a < nb; a++) {
-                const int32_t r = emap[(size_t)(e * nb + a)];
-                if (r >= 0)
-                    sc[(size_t)kidx[(size_t)r] + 1]++;
-            }
-        for (k = 0; k < nk; k++)
-            sc[(size_t)k + 1] += sc[(size_t)k];
-        for (k = 0; k <= nk; k++)
-            rs[(size_t)(k0 + b + k)] = (uint16_t)sc[(size_t)k];
-        for (int64_t e = e0; e < e1; e++)
-            for (int a = 0; a < nb; a++) {
-                const int32_t r = emap[(size_t)(e * nb + a)];
-                if (r >= 0) {
-                    const int32_t kl = kidx[(size_t)r];
-                    kd[(size_t)(e * nb + a)] = (uint32_t)kl | (uint32_t)sc[(size_t)kl]++ << 16;
-                }
-            }
-        for (k = ni; k < nk; k++) {
+        nint += eb_nint[(size_t)b];
+        for (int32_t k = eb_nint[(size_t)b]; k < nk; k++) {
             const int32_t r = rows[(size_t)(k0 + k)];
             ppos[(size_t)(k0 + k)] = fx_off[(size_t)fxi[(size_t)r]] + seen[(size_t)r]++;
         }
     }
+    mark("block tables");
     // lane-group index chunks (see load_round): x rows of j = NJ g + u,
     // result slots of i = 2 t + g / 2, t = 2 o + g % 2
     const int nj = (nb + 3) / 4, no = (nb / 2 + 1) / 2, wc = (nj + no + 1) / 2;
     std::vector<uint32_t> idx((size_t)ne * 4 * wc);
-    for (int64_t e = 0; e < ne; e++)
-        for (int g = 0; g < 4; g++) {
-            std::vector<uint32_t> h((size_t)2 * wc, 0xffffu);
-            for (int u = 0; u < nj; u++) {
-                const int j = p == 3 ? nj * g + u : 4 * u + g;
-                if (j < nb)
-                    h[(size_t)u] = kd[(size_t)(e * nb + j)] & 0xffffu;
+    host_parallel(ne, [&](int64_t lo, int64_t hi, int) {
+        for (int64_t e = lo; e < hi; e++)
+            for (int g = 0; g < 4; g++) {
+                uint32_t h[16];   // 2 wc <= 10 halves (P3: nj = no = 5, wc = 5)
+                for (int u = 0; u < 2 * wc; u++)
+                    h[u] = 0xffffu;
+                for (int u = 0; u < nj; u++) {
+                    const int j = p == 3 ? nj * g + u : 4 * u + g;
+                    if (j < nb)
+                        h[u] = kd[(size_t)(e * nb + j)] & 0xffffu;
+                }
+                for (int o = 0; o < no; o++) {
+                    const int t = 2 * o + (g & 1);
+                    if (t < nb / 2)
+                        h[nj + o] = kd[(size_t)(e * nb + 2 * t + (g >> 1))] >> 16;
+                }
+                for (int u = 0; u < wc; u++)
+                    idx[(size_t)((e * 4 + g) * wc + u)] = h[2 * u] | h[2 * u + 1] << 16;
             }
-            for (int o = 0; o < no; o++) {
-                const int t = 2 * o + (g & 1);
-                if (t < nb / 2)
-                    h[(size_t)(nj + o)] = kd[(size_t)(e * nb + 2 * t + (g >> 1))] >> 16;
-            }
-            for (int u = 0; u < wc; u++)
-                idx[(size_t)((e * 4 + g) * wc + u)] = h[(size_t)(2 * u)] | h[(size_t)(2 * u + 1)] << 16;
-        }
+    });
+    mark("index chunks");
     const int64_t nfx = (int64_t)fx_rows.size(), npart = fx_off.back();
     // one entry of padding: the block kernel clamps its row-table loads to
     // the block's last entry, or to the first when a block touches no row
@@ -957,6 +1063,7 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
         return fail(rc);
     // two entries of padding, read (and multiplied by zero) for the last
     // element
+    mark("uploads");
     if (hipMemsetAsync(eo->coef + ne * NC, 0, 2 * sizeof(float), s) != hipSuccess)
         return fail(hip_fail(hipGetLastError(), "element operator set-up", __FILE__, __LINE__));
     hipError_t e = hipStreamSynchronize(s);
@@ -987,6 +1094,7 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
             (rc = upload(eo->fxc, code.data(), (size_t)n * 4)))
             return fail(rc);
     }
+    mark("coefficients, codes");
     eo->on = 1;
     return PHG_HIP_OK;
 }
