@@ -1031,7 +1031,7 @@ __global__ void __launch_bounds__(64) k_row_walk_hash(WalkArgs w)
             }
             int slot = -1;
             bool fresh = false;
-            if (c >= 0) {
+            if (c >= 0 && !over) {
                 slot = (int)(((uint32_t)c * 2654435761u) >> 8) & (TS - 1);
                 for (int probe = 0; probe < TS; probe++) {
                     const int32_t old = atomicCAS(&key[grp][slot], -1, c);

@@ -398,3 +398,61 @@ def test_mat_from_csr_phg_packed_matrix(hip, eliminated):
     nits, _ = pa.pcg(A, A.rhs(), x, rtol=1e-12, btol=1e-12)
     assert abs(nits - nits_o) <= 2
     assert np.linalg.norm(x.numpy() - xo) / np.linalg.norm(xo) < 1e-10
+
+
+def _fan_mesh(path, k):
+    """k tetrahedra above and k below a ring of k vertices around the origin:
+    the centre vertex touches 2k elements, so its row holds every node of
+    the mesh (P1: k + 3 columns) and, for k large enough, more distinct
+    columns than the insertion kernel's LDS table."""
+    th = 2 * np.pi * np.arange(k) / k
+    v = [(0.0, 0.0, 0.0)] + [(np.cos(t), np.sin(t), 0.0) for t in th] + \
+        [(0.0, 0.0, 1.0), (0.0, 0.0, -1.0)]
+    el = []
+    for i in range(1, k + 1):
+        j = i % k + 1
+        el.append((0, i, j, k + 1))
+        el.append((0, i, j, k + 2))
+    with open(path, "w") as fh:
+        fh.write(f"DIM: 3\nDIM_OF_WORLD: 3\nnumber of vertices: {len(v)}\n"
+                 f"number of elements: {len(el)}\n\nvertex coordinates:\n")
+        fh.writelines("%.17g %.17g %.17g\n" % (x, y, z) for x, y, z in v)
+        fh.write("\nelement vertices:\n")
+        fh.writelines(" ".join(map(str, e)) + "\n" for e in el)
+        # face j lies opposite vertex j: only the face away from the centre
+        # is on the boundary
+        fh.write("\nelement boundaries:\n" + "1 0 0 0\n" * len(el))
+        fh.write("\nelement type:\n" + "0\n" * len(el))
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_high_valence_rows_bitexact(hip, tmp_path, p):
+    """Rows with more distinct columns than the insertion kernel's hash table
+    holds take its serial replay; the CSR stays bit for bit the oracle's."""
+    pa = hip
+    path = str(tmp_path / "fan.dat")
+    _fan_mesh(path, 144)
+    g = pa.Grid.read_albert(path)
+    nd = g.ndof(p)
+    bd = g.dof_bdry(p)
+    xyz = g.dof_coords(p)
+    kappa = np.exp(np.random.default_rng(5).normal(0.0, 0.5, size=g.nelem))
+    f = 1.0 + xyz[:, 0]
+    gv = xyz[:, 1] * xyz[:, 2]
+    sp = pa.Space(g, p, bdry=bd)
+    pb = pa.Problem.poisson(use_kappa=True)
+    A = pa.assemble(sp, pb, kappa=kappa, f=f, g=gv)
+    Ao = orc.assemble(orc.problem(p, use_kappa=True), g.verts(), g.elem_verts(),
+                      g.dof_map(p), nd, bd, gv, f, kappa)
+    rp, ci, va, rhs = A.download()
+    # the centre vertex's row exceeds the table (P1, P2: 16 lanes and 128
+    # entries per row; P3: 32 lanes, 256 entries)
+    assert np.diff(Ao.rowptr).max() > {1: 128, 2: 128, 3: 256}[p]
+    assert np.array_equal(rp, Ao.rowptr)
+    assert np.array_equal(ci, Ao.cols) and _bits_equal(va, Ao.vals)
+    assert _bits_equal(rhs, Ao.rhs)
+    A.eliminate()
+    orc.eliminate(Ao, bd)
+    rp, ci, va, rhs = A.download()
+    assert np.array_equal(ci, Ao.cols) and _bits_equal(va, Ao.vals)
+    assert _bits_equal(rhs, Ao.rhs)
