@@ -1078,6 +1078,13 @@ __global__ void __launch_bounds__(64) k_row_walk_hash(WalkArgs w)
     }
 }
 
+__global__ void k_row_flags(int64_t n, int dim, const uint8_t *node, uint8_t *row)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        row[i] = node[i / dim];
+}
+
 // The row copies below give each row a group of RG consecutive lanes, so a
 // row's entries move in coalesced runs; RPB rows per 256-thread block.
 constexpr int RG = 16, RPB = 256 / RG;
@@ -1647,24 +1654,16 @@ extern "C" int phg_hip_assemble(phg_hip_space *sp, const phg_hip_problem *pb,
     uint8_t *bdry_row = nullptr;
     int64_t ninc = 0, nscr = 0, nnz = 0;
     phg_hip_mat *A = nullptr;
-    std::vector<uint8_t> brow;
 
     if ((rc = element_pass(sp, pb, kappa, f, &dK, &dB, nullptr)))
         return rc;
 
-    // per-row Dirichlet flags and values
-    {
-        std::vector<uint8_t> b1((size_t)sp->ndof);
-        if ((rc = phg::download(b1.data(), sp->d_bdry, (size_t)sp->ndof)))
-            goto done;
-        brow.resize((size_t)n);
-        for (int64_t i = 0; i < sp->ndof; i++)
-            for (int c = 0; c < dim; c++)
-                brow[(size_t)(i * dim + c)] = b1[(size_t)i];
-    }
-    if ((rc = phg::dmalloc((void **)&bdry_row, (size_t)n)) ||
-        (rc = phg::upload(bdry_row, brow.data(), (size_t)n)) ||
-        (rc = phg::dmalloc((void **)&dg, (size_t)n * sizeof(double))) ||
+    // per-row Dirichlet flags (each node's flag on its dim rows) and values
+    if ((rc = phg::dmalloc((void **)&bdry_row, (size_t)n)))
+        goto done;
+    hipLaunchKernelGGL(k_row_flags, dim3(nblk(n, 256)), dim3(256), 0, s, n, dim,
+                       (const uint8_t *)sp->d_bdry, bdry_row);
+    if ((rc = phg::dmalloc((void **)&dg, (size_t)n * sizeof(double))) ||
         (rc = phg::upload(dg, g, (size_t)n * sizeof(double))))
         goto done;
 

@@ -3097,12 +3097,20 @@ namespace phg {
 int split_row_blocks(const int32_t *d_rowptr, int64_t n, int64_t nnz,
                      int64_t cap, int64_t rcap, int32_t **d, int64_t *nb)
 {
-    // greedy split into runs of consecutive rows holding at most cap entries
-    // and rcap rows; a longer row forms a block of its own
-    std::vector<int32_t> rp((size_t)n + 1), blk;
+    std::vector<int32_t> rp((size_t)n + 1);
     int rc;
     if ((rc = download(rp.data(), d_rowptr, rp.size() * sizeof(int32_t))))
         return rc;
+    return split_row_blocks_host(rp.data(), n, nnz, cap, rcap, d, nb);
+}
+
+int split_row_blocks_host(const int32_t *rp, int64_t n, int64_t nnz, int64_t cap,
+                          int64_t rcap, int32_t **d, int64_t *nb)
+{
+    // greedy split into runs of consecutive rows holding at most cap entries
+    // and rcap rows; a longer row forms a block of its own
+    std::vector<int32_t> blk;
+    int rc;
     blk.reserve((size_t)(nnz / cap + n / rcap + 2));
     blk.push_back(0);
     int64_t cur_nnz = 0, cur_rows = 0;

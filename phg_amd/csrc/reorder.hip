@@ -247,12 +247,15 @@ int build_solver_copy(phg_hip_mat *A)
         PHG_HIP_CHECK(hipStreamSynchronize(s));
         cleanup();
     }
-    if ((rc = split_row_blocks(A->q_rowptr, n, A->nnz, SPMV_BLOCK_NNZ,
-                               SPMV_BLOCK_ROWS, &A->q_rblk, &A->q_nrblk)) ||
-        (rc = split_row_blocks(A->q_rowptr, n, A->nnz, SPMV_LARGE_NNZ,
-                               SPMV_BLOCK_ROWS, &A->q_rblk1k, &A->q_nrblk1k)) ||
-        (rc = split_row_blocks(A->q_rowptr, n, A->nnz, SPMV_WAVE_NNZ,
-                               SPMV_WAVE_ROWS, &A->q_wblk, &A->q_nwblk)))
+    // the three row-block splits of the solver copy from one download
+    std::vector<int32_t> qrp((size_t)n + 1);
+    if ((rc = download(qrp.data(), A->q_rowptr, qrp.size() * sizeof(int32_t))) ||
+        (rc = split_row_blocks_host(qrp.data(), n, A->nnz, SPMV_BLOCK_NNZ,
+                                    SPMV_BLOCK_ROWS, &A->q_rblk, &A->q_nrblk)) ||
+        (rc = split_row_blocks_host(qrp.data(), n, A->nnz, SPMV_LARGE_NNZ,
+                                    SPMV_BLOCK_ROWS, &A->q_rblk1k, &A->q_nrblk1k)) ||
+        (rc = split_row_blocks_host(qrp.data(), n, A->nnz, SPMV_WAVE_NNZ,
+                                    SPMV_WAVE_ROWS, &A->q_wblk, &A->q_nwblk)))
         return rc;
     return PHG_HIP_OK;
 }
