@@ -286,6 +286,28 @@ def main():
     copy_gbs = 16 * copy_n / copy_us / 1e3
     log(f"copy kernel: {16 * copy_n / 1e6:.1f} MB in {copy_us:.1f} us = {copy_gbs:.0f} GB/s")
 
+    # assembly + elimination again, warm (the first pass above loads the
+    # kernels' code objects), for the assemble+solve rates
+    pa.synchronize()
+    t0 = time.perf_counter()
+    A2 = pa.assemble(sp, pb, kappa=kappa, f=f, g=gv)
+    A2.eliminate()
+    pa.synchronize()
+    t_asm_warm = time.perf_counter() - t0
+    del A2
+    t_setup = eop["setup_s"] if eop else 0.0
+    # one solve per assembly: the operator's set-up is paid once per matrix,
+    # so a single solve is faster on the FP32 CSR
+    asm_solve = {"assemble_s": round(t_asm_warm, 4), "element_operator_setup_s": t_setup,
+                 "mixed_dofs_per_s": round(nd / (t_asm_warm + t_setup + sec_mp), 1),
+                 "mixed_fp32_csr_dofs_per_s": (round(nd / (t_asm_warm + sec_c), 1)
+                                               if csr_mixed else None),
+                 "fp64_dofs_per_s": round(nd / (t_asm_warm + sec_64), 1)}
+    log(f"assemble+solve: assembly {t_asm_warm * 1e3:.1f} ms (warm), operator set-up "
+        f"{t_setup * 1e3:.1f} ms; mixed {asm_solve['mixed_dofs_per_s'] / 1e6:.1f} M DOFs/s "
+        f"(FP32 CSR: {(asm_solve['mixed_fp32_csr_dofs_per_s'] or 0) / 1e6:.1f} M), "
+        f"FP64 {asm_solve['fp64_dofs_per_s'] / 1e6:.1f} M DOFs/s")
+
     cpu = None
     if args.cpu_n > 0:
         c = cpu_baseline(args.cpu_n, args.order, args.sigma, args.seed, args.rtol)
@@ -337,6 +359,7 @@ def main():
                                "value": round(nd / sec_o, 1),
                                "rel_err": float(f"{err_o:.3e}")},
         "assemble_s": round(t_asm, 4),
+        "assemble_plus_solve": asm_solve,
         "spmv_layout_us": {k: v for k, v in tune.items() if k != "chosen"},
         "roofline": roof,
         "operator": args.operator,
