@@ -47,6 +47,7 @@
 #include <memory>
 #include <numeric>
 #include <thread>
+#include <utility>
 #include <vector>
 
 namespace {
@@ -88,9 +89,31 @@ void host_parallel(int64_t n, F &&f, int64_t grain = 1024)
         x.join();
 }
 
-// sort of v (std::sort per chunk on the host threads, then pairwise merges)
+// Allocator whose value-initialisation is a no-op: the large set-up arrays
+// are first touched (so paged in) by the host threads that fill them.
 template <typename T>
-void host_sort(std::vector<T> &v)
+struct NoInit : std::allocator<T> {
+    template <typename U>
+    struct rebind {
+        using other = NoInit<U>;
+    };
+    NoInit() = default;
+    template <typename U>
+    NoInit(const NoInit<U> &) {}
+    template <typename U>
+    void construct(U *) noexcept {}
+    template <typename U, typename... Args>
+    void construct(U *q, Args &&...args)
+    {
+        ::new ((void *)q) U(std::forward<Args>(args)...);
+    }
+};
+template <typename T>
+using HostVec = std::vector<T, NoInit<T>>;
+
+// sort of v (std::sort per chunk on the host threads, then pairwise merges)
+template <typename V>
+void host_sort(V &v)
 {
     const int64_t n = (int64_t)v.size();
     const int nt = host_threads(n);
@@ -98,7 +121,7 @@ void host_sort(std::vector<T> &v)
     for (int t = 0; t <= nt; t++)
         cut[(size_t)t] = n * t / nt;
     host_parallel(n, [&](int64_t lo, int64_t hi, int) { std::sort(v.begin() + lo, v.begin() + hi); });
-    std::vector<T> w((size_t)n);
+    V w((size_t)n);
     for (int width = 1; width < nt; width *= 2) {
         std::vector<std::thread> th;
         for (int t = 0; t < nt; t += 2 * width) {
@@ -801,7 +824,7 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
         double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
         const double sc = ext > 0 ? 2097151.0 / ext : 0.0;
         // (key, element) pairs: sorting them is a stable sort by key
-        std::vector<std::pair<uint64_t, int32_t>> key((size_t)ne);
+        HostVec<std::pair<uint64_t, int32_t>> key((size_t)ne);
         host_parallel(ne, [&](int64_t e0, int64_t e1, int) {
             for (int64_t e = e0; e < e1; e++) {
                 uint64_t q[3];
@@ -822,7 +845,7 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
     }
     mark("element order");
     // element DOF map in the solver numbering, Dirichlet DOFs -1
-    std::vector<int32_t> emap((size_t)ne * nb);
+    HostVec<int32_t> emap((size_t)ne * nb);
     std::vector<uint8_t> bq((size_t)n);
     for (int64_t i = 0; i < n; i++)
         bq[(size_t)i] = brow[(size_t)(A->q_perm ? perm[(size_t)i] : i)];
@@ -880,7 +903,7 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
     // partials of a shared row in block order
     std::vector<int32_t> eb_nint((size_t)nblk), ppos((size_t)nent, -1);
     std::vector<uint16_t> rs((size_t)(nent + nblk));
-    std::vector<uint32_t> kd((size_t)ne * nb, KD_NONE);
+    HostVec<uint32_t> kd((size_t)ne * nb);   // every entry set per block
     std::vector<int32_t> fxi((size_t)n, -1), seen((size_t)n, 0), fx_rows, fx_off{0};
     // shared rows in row order (the fix-up walks them so), each with the
     // range of its partials
@@ -950,6 +973,8 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
                         const int32_t kl = kid[(size_t)r];
                         kd[(size_t)(e * nb + a)] = (uint32_t)kl | (uint32_t)sc[(size_t)kl]++ << 16;
                     }
+                    else
+                        kd[(size_t)(e * nb + a)] = KD_NONE;
                 }
         }
     }, 16);
@@ -967,7 +992,7 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
     // lane-group index chunks (see load_round): x rows of j = NJ g + u,
     // result slots of i = 2 t + g / 2, t = 2 o + g % 2
     const int nj = (nb + 3) / 4, no = (nb / 2 + 1) / 2, wc = (nj + no + 1) / 2;
-    std::vector<uint32_t> idx((size_t)ne * 4 * wc);
+    HostVec<uint32_t> idx((size_t)ne * 4 * wc);
     host_parallel(ne, [&](int64_t lo, int64_t hi, int) {
         for (int64_t e = lo; e < hi; e++)
             for (int g = 0; g < 4; g++) {
