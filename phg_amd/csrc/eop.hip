@@ -860,41 +860,64 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
     });
     mark("element map");
     // element blocks: consecutive elements, at most eblk of them touching
-    // at most kmax rows; the touched rows of a block in first-touch order
+    // at most kmax rows; the touched rows of a block in first-touch order.
+    // The greedy split runs on its own in each of nch fixed element ranges
+    // (one per 65,536 elements, at most 16), so the layout does not depend
+    // on the number of host threads.
     const int eblk = p == 1 ? EopShape<1>::EBLK : (p == 2 ? EopShape<2>::EBLK : EopShape<3>::EBLK);
     const int kmax = p == 1 ? EopShape<1>::KMAX : (p == 2 ? EopShape<2>::KMAX : EopShape<3>::KMAX);
-    std::vector<int32_t> eb_elem{0}, eb_row{0}, rows;
-    std::vector<int32_t> stamp((size_t)n, -1), cnt((size_t)n, 0);
-    {
-        int32_t b = 0, nk = 0, nel = 0;
-        for (int64_t e = 0; e < ne; e++) {
-            const int32_t *me = &emap[(size_t)(e * nb)];
-            int fresh = 0;
-            for (int a = 0; a < nb; a++)
-                if (me[a] >= 0 && stamp[(size_t)me[a]] != b)
-                    fresh++;
-            if (nel > 0 && (nel == eblk || nk + fresh > kmax)) {
-                eb_elem.push_back((int32_t)e);
-                eb_row.push_back((int32_t)rows.size());
-                b++;
-                nk = nel = 0;
-            }
-            for (int a = 0; a < nb; a++) {
-                const int32_t r = me[a];
-                if (r >= 0 && stamp[(size_t)r] != b) {
-                    stamp[(size_t)r] = b;
-                    rows.push_back(r);
-                    cnt[(size_t)r]++;
-                    nk++;
+    const int64_t nch = std::max<int64_t>(1, std::min<int64_t>(16, ne / 65536));
+    struct Split {
+        std::vector<int32_t> start, nrow, rows;   // per block: first element, rows
+    };
+    std::vector<Split> split((size_t)nch);
+    host_parallel(nch, [&](int64_t c0, int64_t c1, int) {
+        std::unique_ptr<int32_t[]> stamp(new int32_t[(size_t)n]);
+        std::fill(stamp.get(), stamp.get() + n, -1);
+        int32_t b = -1;   // block tag in stamp, unique within this thread
+        for (int64_t c = c0; c < c1; c++) {
+            Split &sl = split[(size_t)c];
+            int32_t nk = 0, nel = 0;
+            for (int64_t e = ne * c / nch; e < ne * (c + 1) / nch; e++) {
+                const int32_t *me = &emap[(size_t)(e * nb)];
+                int fresh = 0;
+                for (int a = 0; a < nb; a++)
+                    if (me[a] >= 0 && stamp[(size_t)me[a]] != b)
+                        fresh++;
+                if (nel == 0 || nel == eblk || nk + fresh > kmax) {
+                    if (nel > 0)
+                        sl.nrow.push_back(nk);
+                    sl.start.push_back((int32_t)e);
+                    b++;
+                    nk = nel = 0;
                 }
+                for (int a = 0; a < nb; a++) {
+                    const int32_t r = me[a];
+                    if (r >= 0 && stamp[(size_t)r] != b) {
+                        stamp[(size_t)r] = b;
+                        sl.rows.push_back(r);
+                        nk++;
+                    }
+                }
+                nel++;
             }
-            nel++;
+            if (nel > 0)
+                sl.nrow.push_back(nk);
         }
-        if (nel > 0) {
-            eb_elem.push_back((int32_t)ne);
-            eb_row.push_back((int32_t)rows.size());
+    }, 1);
+    std::vector<int32_t> eb_elem, eb_row{0}, rows;
+    for (const Split &sl : split) {
+        for (size_t t = 0; t < sl.start.size(); t++) {
+            eb_elem.push_back(sl.start[t]);
+            eb_row.push_back(eb_row.back() + sl.nrow[t]);
         }
+        rows.insert(rows.end(), sl.rows.begin(), sl.rows.end());
     }
+    eb_elem.push_back((int32_t)ne);
+    // number of blocks touching each row
+    std::vector<int32_t> cnt((size_t)n, 0);
+    for (const int32_t r : rows)
+        cnt[(size_t)r]++;
     const int64_t nblk = (int64_t)eb_elem.size() - 1, nent = (int64_t)rows.size();
     if (nblk <= 0)
         return set_error(PHG_HIP_E_ARG, "phg_hip_mat_element_operator: no elements");

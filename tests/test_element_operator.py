@@ -54,7 +54,8 @@ def test_apply_matches_assembled_matrix(hip, kind, p, n):
 
 # meshes of several element blocks (blocks of 2048 / 1024 / 512 elements
 # for P1 / P2 / P3), so rows shared between blocks go through the partials
-MULTI_BLOCK_CASES = [("kappa", 1, 16), ("kappa", 2, 9), ("sin", 3, 6)]
+# 32^3 x 6 = 196,608 elements: the block split runs on three element ranges
+MULTI_BLOCK_CASES = [("kappa", 1, 16), ("kappa", 2, 9), ("sin", 3, 6), ("kappa", 1, 32)]
 
 
 @pytest.mark.parametrize("kind,p,n", MULTI_BLOCK_CASES)
