@@ -792,13 +792,15 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
     const int64_t ne = sp->nelem, n = A->n;
     if ((int64_t)nb * ne >= (int64_t)1 << 30)
         return set_error(PHG_HIP_E_ARG, "phg_hip_mat_element_operator: too many elements");
-    std::vector<int32_t> map((size_t)ne * nb), ev((size_t)ne * 4), perm;
-    std::vector<double> xyz((size_t)sp->nvert * 3);
+    // the space keeps host copies of its element maps
+    const std::vector<int32_t> &map = sp->h_map, &ev = sp->h_elem_verts;
+    const std::vector<double> &xyz = sp->h_verts;
+    std::vector<int32_t> perm;
     std::vector<uint8_t> brow((size_t)n);
-    if ((rc = download(map.data(), sp->d_map, map.size() * 4)) ||
-        (rc = download(ev.data(), sp->d_elem_verts, ev.size() * 4)) ||
-        (rc = download(xyz.data(), sp->d_verts, xyz.size() * 8)) ||
-        (rc = download(brow.data(), A->d_bdry_row, (size_t)n)))
+    if (map.size() != (size_t)ne * nb || ev.size() != (size_t)ne * 4 ||
+        xyz.size() != (size_t)sp->nvert * 3)
+        return set_error(PHG_HIP_E_STATE, "phg_hip_mat_element_operator: space without host maps");
+    if ((rc = download(brow.data(), A->d_bdry_row, (size_t)n)))
         return rc;
     // solver numbering: solver index -> PHG index (identity without q_perm)
     std::vector<int32_t> iperm((size_t)n);

@@ -5,6 +5,8 @@
 #include <stdint.h>
 #include <stddef.h>
 
+#include <vector>
+
 #include "../../include/phg_hip.h"
 
 namespace phg {
@@ -58,6 +60,10 @@ struct phg_hip_space {
     int32_t *d_map = nullptr;        // [nelem][nbas]
     uint8_t *d_bdry = nullptr;       // [ndof]
     double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};   // bounding box of the vertices
+    // host copies of the element maps, kept for the element operator's
+    // layout builder
+    std::vector<int32_t> h_elem_verts, h_map;
+    std::vector<double> h_verts;
 };
 
 struct phg_hip_vec {
