@@ -213,6 +213,32 @@ def c4(seq=16):
             "spmm_bytes": [b64, b32]}
 
 
+def c4spmm(reps=20):
+    """The C4 system's SpMM alone (for kernel traces and counters): the
+    elasticity matrix of c4, the SpMM shape tuner, then `reps` launches of
+    the chosen shape in FP32 and in FP64 (16 right-hand sides)."""
+    import phg_amd as pa
+    n, p = 40, 2
+    g = pa.Grid.kuhn(n, (1, -1, -1, -1, -1, -1))
+    nd = g.ndof(p)
+    sp = pa.Space(g, p)
+    pb = pa.Problem.elasticity(E=2.5, nu=0.25)
+    fj = np.random.default_rng(40).uniform(-1.0, 1.0, size=3 * nd)
+    A = pa.assemble(sp, pb, f=fj, g=np.zeros(3 * nd))
+    A.eliminate()
+    shapes = pa.spmm_tune(A, reps=reps)
+    best32 = min(shapes, key=lambda s: shapes[s][1])
+    best64 = min(shapes, key=lambda s: shapes[s][0])
+    b32, b64 = pa.spmm_bytes(A, True), pa.spmm_bytes(A, False)
+    n_, nnz = A.info()
+    return {"config": "c4spmm", "workload": "vector P2 elasticity 40^3x6 SpMM, 16 columns",
+            "n": n_, "nnz": nnz, "shapes_us": {str(s): v for s, v in shapes.items()},
+            "fp32": {"shape": best32, "us": shapes[best32][1], "bytes": b32,
+                     "GBps": round(b32 / shapes[best32][1] / 1e3, 1)},
+            "fp64": {"shape": best64, "us": shapes[best64][0], "bytes": b64,
+                     "GBps": round(b64 / shapes[best64][0] / 1e3, 1)}}
+
+
 def solvers(n=32, p=2, rtol=1e-8):
     """PHG's Krylov solvers on one kappa-Poisson system (Jacobi PC, FP64,
     x0 = 0, rtol = btol): PCG, BiCGstab and the default GMRES (flgmres_r,
