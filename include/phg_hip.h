@@ -308,6 +308,17 @@ int phg_hip_spmm_set_shape(phg_hip_mat *A, int32_t shape64, int32_t shape32);
  * solution as the norms of u_h - I_h u (examples/poisson.c:437-460). */
 int phg_hip_dof_norms(phg_hip_space *sp, const phg_hip_vec *v, double *l2,
                       double *grad_l2);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving or time-stepping
+ * loops, harnesses that forbid hipMalloc): phg_hip_dof_norms with its
+ * per-element values and partial sums in a fixed scratch array of the
+ * library instead of buffers allocated per call.  Same kernels, same
+ * summation order, so the same results bit for bit.  Meshes of at most
+ * 2^21 elements (larger ones give PHG_HIP_E_ARG); the array (32 MiB) is held
+ * by the GPU until the process ends.  All calls on the GPU share the array:
+ * calls that overlap on the GPU (from several host threads) race. */
+int phg_hip_dof_norms_noalloc(phg_hip_space *sp, const phg_hip_vec *v,
+                              double *l2, double *grad_l2);
 
 /*---------------------------------------------------------------- adjoint */
 

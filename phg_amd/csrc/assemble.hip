@@ -2082,30 +2082,27 @@ done:
 
 //------------------------------------------------------------- DOF norms
 namespace {
-int sum_device(const double *v, int64_t n, double *out)
+constexpr int NORM_PARTS = 256;   // partial sums of one fixed-order reduction
+
+// fixed-order sum of v[0..n): NORM_PARTS per-block partials in part (device),
+// summed on the host in index order
+int sum_device(const double *v, int64_t n, double *part, double *out)
 {
     hipStream_t s = phg::ctx()->stream;
-    const int nparts = 256;
-    double *part = nullptr;
-    int rc;
-    if ((rc = phg::dmalloc((void **)&part, nparts * sizeof(double))))
-        return rc;
-    hipLaunchKernelGGL(k_sum_blocks, dim3(nparts), dim3(256), 0, s, n, v, part);
-    std::vector<double> hp(nparts);
-    rc = phg::download(hp.data(), part, nparts * sizeof(double));
-    phg::dfree(part);
+    hipLaunchKernelGGL(k_sum_blocks, dim3(NORM_PARTS), dim3(256), 0, s, n, v, part);
+    PHG_HIP_CHECK(hipGetLastError());
+    double hp[NORM_PARTS];
+    int rc = phg::download(hp, part, sizeof(hp));
     if (rc)
         return rc;
     double t = 0.0;
-    for (int i = 0; i < nparts; i++)
-        t += hp[(size_t)i];
+    for (int i = 0; i < NORM_PARTS; i++)
+        t += hp[i];
     *out = t;
     return PHG_HIP_OK;
 }
-}  // namespace
 
-extern "C" int phg_hip_dof_norms(phg_hip_space *sp, const phg_hip_vec *v,
-                                 double *l2, double *h1)
+int check_norm_args(const phg_hip_space *sp, const phg_hip_vec *v)
 {
     int rc;
     if ((rc = phg::require_ctx()))
@@ -2113,14 +2110,16 @@ extern "C" int phg_hip_dof_norms(phg_hip_space *sp, const phg_hip_vec *v,
     if (sp == nullptr || v == nullptr || v->n != sp->ndof)
         return phg::set_error(PHG_HIP_E_ARG,
                               "phg_hip_dof_norms: a scalar DOF of the space required");
+    return PHG_HIP_OK;
+}
+
+// per-element squared norms into dl / dh [nelem], their fixed-order sums
+// through part [NORM_PARTS]
+int dof_norms_run(const phg_hip_space *sp, const phg_hip_vec *v, double *dl,
+                  double *dh, double *part, double *l2, double *h1)
+{
     hipStream_t s = phg::ctx()->stream;
     const int64_t ne = sp->nelem;
-    double *dl = nullptr, *dh = nullptr;
-    if ((rc = phg::dmalloc((void **)&dl, (size_t)ne * sizeof(double))) ||
-        (rc = phg::dmalloc((void **)&dh, (size_t)ne * sizeof(double)))) {
-        phg::dfree(dl);
-        return rc;
-    }
     dim3 g(nblk(ne, 128)), b(128);
     if (sp->order == 1)
         hipLaunchKernelGGL(k_dof_norms<1>, g, b, 0, s, ne,
@@ -2137,21 +2136,77 @@ extern "C" int phg_hip_dof_norms(phg_hip_space *sp, const phg_hip_vec *v,
                            (const double *)sp->d_verts,
                            (const int32_t *)sp->d_elem_verts,
                            (const int32_t *)sp->d_map, (const double *)v->d, dl, dh);
+    PHG_HIP_CHECK(hipGetLastError());
     double a = 0.0, c = 0.0;
-    if ((rc = hipGetLastError() == hipSuccess ? PHG_HIP_OK
-                                              : phg::set_error(PHG_HIP_E_HIP, "norm kernel")) ||
-        (rc = sum_device(dl, ne, &a)) || (rc = sum_device(dh, ne, &c))) {
-        phg::dfree(dl);
-        phg::dfree(dh);
+    int rc;
+    if ((rc = sum_device(dl, ne, part, &a)) || (rc = sum_device(dh, ne, part, &c)))
         return rc;
-    }
-    phg::dfree(dl);
-    phg::dfree(dh);
     if (l2)
         *l2 = sqrt(a);
     if (h1)
         *h1 = sqrt(c);
     return PHG_HIP_OK;
+}
+
+// Scratch of phg_hip_dof_norms_noalloc: the per-element values of both norms
+// and the partial sums, for meshes of up to NOALLOC_MAX_ELEM elements
+// (32 MiB + 2 KiB, held by every GPU the library runs on until the process
+// ends).  Host code never passes the array's name to a kernel: the address
+// of each device's copy is taken once per device at phg_hip_init.
+constexpr int64_t NOALLOC_MAX_ELEM = int64_t(1) << 21;
+constexpr int64_t NOALLOC_DOUBLES = 2 * NOALLOC_MAX_ELEM + NORM_PARTS;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+double *g_scratch[NOALLOC_MAX_DEVICES] = {};
+}  // namespace
+
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+namespace phg {
+// called by phg_hip_init with `device` current
+int noalloc_init(int device)
+{
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES)
+        return set_error(PHG_HIP_E_ARG, "device %d beyond the scratch table", device);
+    if (g_scratch[device] != nullptr)
+        return PHG_HIP_OK;
+    void *ptr = nullptr;
+    PHG_HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch[device] = (double *)ptr;
+    return PHG_HIP_OK;
+}
+}  // namespace phg
+
+extern "C" int phg_hip_dof_norms(phg_hip_space *sp, const phg_hip_vec *v,
+                                 double *l2, double *h1)
+{
+    int rc;
+    if ((rc = check_norm_args(sp, v)))
+        return rc;
+    const int64_t ne = sp->nelem;
+    double *buf = nullptr;
+    if ((rc = phg::dmalloc((void **)&buf, (size_t)(2 * ne + NORM_PARTS) * sizeof(double))))
+        return rc;
+    rc = dof_norms_run(sp, v, buf, buf + ne, buf + 2 * ne, l2, h1);
+    phg::dfree(buf);
+    return rc;
+}
+
+extern "C" int phg_hip_dof_norms_noalloc(phg_hip_space *sp, const phg_hip_vec *v,
+                                         double *l2, double *h1)
+{
+    int rc;
+    if ((rc = check_norm_args(sp, v)))
+        return rc;
+    const int64_t ne = sp->nelem;
+    if (ne > NOALLOC_MAX_ELEM)
+        return phg::set_error(PHG_HIP_E_ARG,
+                              "phg_hip_dof_norms_noalloc: %lld elements, at most %lld",
+                              (long long)ne, (long long)NOALLOC_MAX_ELEM);
+    double *buf = g_scratch[phg::ctx()->device];
+    if (buf == nullptr)
+        return phg::set_error(PHG_HIP_E_STATE, "no scratch address for device %d",
+                              phg::ctx()->device);
+    return dof_norms_run(sp, v, buf, buf + ne, buf + 2 * ne, l2, h1);
 }
 
 //------------------------------------------------------- CSR from the host

@@ -48,6 +48,9 @@ int dmalloc(void **p, size_t bytes);
 void dfree(void *p);
 int upload(void *d, const void *h, size_t bytes);
 int download(void *h, const void *d, size_t bytes);
+// per-device address of the scratch array of the no-allocation entry point
+// (assemble.hip), taken by phg_hip_init with the device current
+int noalloc_init(int device);
 
 }  // namespace phg
 

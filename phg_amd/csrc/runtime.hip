@@ -101,6 +101,12 @@ extern "C" int phg_hip_init(int32_t device)
     g_ctx.num_cus = prop.multiProcessorCount;
     // default priority stream
     PHG_HIP_CHECK(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
+    int rc = noalloc_init(device);
+    if (rc) {
+        (void)hipStreamDestroy(g_ctx.stream);
+        g_ctx = Context();
+        return rc;
+    }
     PHG_HIP_CHECK(hipHostMalloc(&g_ctx.host_mailbox, 4096, hipHostMallocDefault));
     g_ready = true;
     return PHG_HIP_OK;
