@@ -151,6 +151,10 @@ int phg_hip_vec_dot(const phg_hip_vec *x, const phg_hip_vec *y, double *res);
 int phg_hip_vec_norm2(const phg_hip_vec *x, double *res);
 int phg_hip_vec_axpby(double a, const phg_hip_vec *x, double b, phg_hip_vec *y);
 int phg_hip_vec_copy(const phg_hip_vec *src, phg_hip_vec *dst);
+/* solver-phg.c:207 jacobi_pc: x = b ./ diag(A) for an eliminated A (the
+ * diagonal of matvec.c:500 phgMatSetupDiagonal); PHG_HIP_E_ZERO_DIAG if a
+ * diagonal entry is zero (solver-phg.c:233 stops there). */
+int phg_hip_pc_jacobi(const phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x);
 
 /*---------------------------------------------------------------- solvers */
 
@@ -158,6 +162,10 @@ int phg_hip_vec_copy(const phg_hip_vec *src, phg_hip_vec *dst);
  * solver-phg.c:207, FP64 throughout: x is the initial guess on input and
  * the solution on output, b the right-hand side.  Stops when
  * |r| <= max(rtol*|r0|, btol*|b|, atol) or after maxit iterations.
+ * pAp = 0 or rz = 0 ends the iterations (solver-phg.c:547-570, a warning
+ * there, not an error).  |b| = 0 gives x = 0 and 0 iterations and leaves
+ * *residual as the caller set it (solver-phg.c:497-501; PCG_Solve starts
+ * from 1.0).  A zero diagonal entry gives PHG_HIP_E_ZERO_DIAG.
  * Returns the iteration count (>= 0) or an error code. */
 int phg_hip_pcg(const phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x,
                 int32_t maxit, double rtol, double btol, double atol,

@@ -192,11 +192,16 @@ def eliminate(A: Csr, bdry_rows):
     return A
 
 
+def _arg(A: Csr):
+    """The C struct of an oracle CSR or of one built by csr_from_arrays."""
+    return A._p if A._p is not None else C.byref(A._s)
+
+
 def pcg(A: Csr, b, x0=None, maxit=10000, rtol=1e-10, btol=1e-10, atol=0.0):
     b = np.ascontiguousarray(b, dtype=np.float64)
     x = np.zeros(A.n) if x0 is None else np.array(x0, dtype=np.float64)
-    res = _dbl()
-    nits = lib.orc_pcg(A._p, _ptr(b, _dbl), _ptr(x, _dbl), maxit, rtol, btol, atol,
+    res = _dbl(1.0)     # PCG_Solve's initial value (solver-phg.c:2433)
+    nits = lib.orc_pcg(_arg(A), _ptr(b, _dbl), _ptr(x, _dbl), maxit, rtol, btol, atol,
                        C.byref(res))
     return x, nits, res.value
 
@@ -204,7 +209,7 @@ def pcg(A: Csr, b, x0=None, maxit=10000, rtol=1e-10, btol=1e-10, atol=0.0):
 def matvec(A: Csr, x):
     x = np.ascontiguousarray(x, dtype=np.float64)
     y = np.empty(A.n)
-    lib.orc_matvec(A._p, _ptr(x, _dbl), _ptr(y, _dbl))
+    lib.orc_matvec(_arg(A), _ptr(x, _dbl), _ptr(y, _dbl))
     return y
 
 
@@ -252,7 +257,7 @@ def pcg_arrays(rowptr, cols, vals, b, maxit=10000, rtol=1e-10, btol=1e-10, atol=
     A = csr_from_arrays(rowptr, cols, vals)
     b = np.ascontiguousarray(b, dtype=np.float64)
     x = np.zeros(A.n)
-    res = _dbl()
+    res = _dbl(1.0)     # PCG_Solve's initial value (solver-phg.c:2433)
     nits = lib.orc_pcg(C.byref(A._s), _ptr(b, _dbl), _ptr(x, _dbl), maxit, rtol, btol,
                        atol, C.byref(res))
     return x, nits, res.value

@@ -670,9 +670,14 @@ int orc_pcg(const orc_csr *A, const double *b, double *x, int maxit,
     rb = norm2(n, b);
     /* IsZero (utils.h:451): |a| < 10 * FLOAT_MIN */
     if (fabs(rb) < 10.0 * DBL_MIN) {
+        /* solver-phg.c:497-501: x = 0, *res left as the caller set it */
         memset(x, 0, (size_t)n * sizeof(double));
-        res = 0.0;
-        goto done;
+        free(diag);
+        free(r);
+        free(z);
+        free(p);
+        free(Ap);
+        return 0;
     }
     for (int64_t i = 0; i < n; i++)
         x1 += fabs(x[i]);

@@ -128,6 +128,7 @@ _sig(hip_lib, "phg_hip_vec_dot", C.c_int, [_vp, _vp, _P(_dbl)])
 _sig(hip_lib, "phg_hip_vec_norm2", C.c_int, [_vp, _P(_dbl)])
 _sig(hip_lib, "phg_hip_vec_axpby", C.c_int, [_dbl, _vp, _dbl, _vp])
 _sig(hip_lib, "phg_hip_vec_copy", C.c_int, [_vp, _vp])
+_sig(hip_lib, "phg_hip_pc_jacobi", C.c_int, [_vp, _vp, _vp])
 _sig(hip_lib, "phg_hip_pcg", C.c_int,
      [_vp, _vp, _vp, _i32, _dbl, _dbl, _dbl, _P(_dbl)])
 _sig(hip_lib, "phg_hip_bicgstab", C.c_int,
@@ -623,8 +624,10 @@ def profile_read(kind: int):
 
 
 def pcg(A: Mat, b: Vec, x: Vec, maxit=10000, rtol=1e-10, btol=1e-10, atol=0.0):
-    """FP64 Jacobi-PCG (solver-phg.c:485). Returns (nits, residual)."""
-    res = _dbl()
+    """FP64 Jacobi-PCG (solver-phg.c:485). Returns (nits, residual); the
+    residual starts at PCG_Solve's 1.0 (solver-phg.c:2433), which a zero
+    right-hand side leaves in place."""
+    res = _dbl(1.0)
     nits = _check(hip_lib.phg_hip_pcg(A._h, b._h, x._h, maxit, rtol, btol, atol,
                                       C.byref(res)), "pcg")
     return nits, res.value
@@ -670,3 +673,9 @@ def pcg_reliable(A: Mat, b: Vec, x: Vec, maxit=100000, rtol=1e-10, btol=1e-10,
                                                atol, delta, C.byref(upd),
                                                C.byref(res)), "pcg_reliable")
     return nits, upd.value, res.value
+
+
+def pc_jacobi(A: Mat, b: Vec, x: Vec) -> None:
+    """solver-phg.c:207 jacobi_pc: x = b ./ diag(A) (PhgHipError with code
+    PHG_HIP_E_ZERO_DIAG on a zero diagonal entry)."""
+    _check(hip_lib.phg_hip_pc_jacobi(A._h, b._h, x._h), "pc_jacobi")

@@ -75,6 +75,14 @@ def build(verbose: bool = False) -> None:
           os.path.join(HOST, "phg_grid.c"), os.path.join(HOST, "refine.c"),
           "-lm", "-o", tmp])
     os.replace(tmp, os.path.join(LIB, "libphg_grid.so"))
+
+    # PHG's OEM_SOLVER plugin and its serial PHG stand-ins, on the HIP library
+    tmp = os.path.join(LIB, "libphg_oem_hip.so.tmp")
+    _run(["gcc", "-O2", "-fPIC", "-shared", "-std=c11", "-Wall", "-Wextra",
+          "-Wno-unused-parameter", "-I" + os.path.join(ROOT, "include"),
+          os.path.join(HOST, "phg_oem_hip.c"), os.path.join(HOST, "phg_mini.c"),
+          "-L" + LIB, "-lphg_hip", "-Wl,-rpath,$ORIGIN", "-o", tmp])
+    os.replace(tmp, os.path.join(LIB, "libphg_oem_hip.so"))
     if verbose:
         print("built", os.listdir(LIB))
 

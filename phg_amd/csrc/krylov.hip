@@ -894,6 +894,24 @@ __global__ void __launch_bounds__(BS) k_final_sum(const double *partials,
         *out = o[0];
 }
 
+// solver-phg.c:207 jacobi_pc: x = b ./ diag, and the number of zero
+// diagonal entries in the partials
+__global__ void __launch_bounds__(BS) k_jacobi(int64_t n, const double *diag,
+                                               const double *b, double *x,
+                                               double *partials)
+{
+    double v[1] = {0.0};
+    for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * BS) {
+        const double d = diag[i];
+        if (d == 0.0)
+            v[0] += 1.0;
+        else
+            x[i] = b[i] / d;
+    }
+    write_partials<1>(v, partials);
+}
+
 __global__ void __launch_bounds__(BS) k_axpby(int64_t n, double a,
                                               const double *x, double b,
                                               double *y)
@@ -2740,7 +2758,8 @@ int pcg64_impl(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x,
                            (const double *)ws->xp, x->d);
     PHG_HIP_CHECK(hipGetLastError());
     PHG_HIP_CHECK(hipStreamSynchronize(s));
-    if (residual)
+    // |b| = 0: solver-phg.c:497-501 returns before setting *res
+    if (residual && !is_zero_host(fin.rb))
         *residual = fin.res;
     if (fin.done == ZERO_DIAG)
         return phg::set_error(PHG_HIP_E_ZERO_DIAG,
@@ -3360,6 +3379,38 @@ extern "C" int phg_hip_vec_dot(const phg_hip_vec *x, const phg_hip_vec *y,
     rc = phg::download(res, part + MAX_GRID, sizeof(double));
     phg::dfree(part);
     return rc;
+}
+
+extern "C" int phg_hip_pc_jacobi(const phg_hip_mat *A, const phg_hip_vec *b,
+                                 phg_hip_vec *x)
+{
+    int rc;
+    if ((rc = phg::require_ctx()))
+        return rc;
+    if (!A || !b || !x || b->n != A->n || x->n != A->n)
+        return phg::set_error(PHG_HIP_E_ARG, "phg_hip_pc_jacobi: bad args");
+    if (!A->eliminated || A->d_diag == nullptr)
+        return phg::set_error(PHG_HIP_E_ARG,
+                              "phg_hip_pc_jacobi: matrix not eliminated (no diagonal)");
+    hipStream_t s = phg::ctx()->stream;
+    double *part = nullptr;
+    if ((rc = phg::dmalloc((void **)&part, (MAX_GRID + 1) * sizeof(double))))
+        return rc;
+    int g = grid_for(A->n);
+    hipLaunchKernelGGL(k_jacobi, dim3(g), dim3(BS), 0, s, A->n, (const double *)A->d_diag,
+                       (const double *)b->d, x->d, part);
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(BS), 0, s,
+                       (const double *)part, g, part + MAX_GRID);
+    PHG_HIP_CHECK(hipGetLastError());
+    double nzero = 0.0;
+    rc = phg::download(&nzero, part + MAX_GRID, sizeof(double));
+    phg::dfree(part);
+    if (rc)
+        return rc;
+    if (nzero != 0.0)
+        return phg::set_error(PHG_HIP_E_ZERO_DIAG,
+                              "zero diagonal in Jacobi preconditioner (%.0f rows)", nzero);
+    return PHG_HIP_OK;
 }
 
 extern "C" int phg_hip_vec_norm2(const phg_hip_vec *x, double *res)

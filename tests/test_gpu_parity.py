@@ -236,7 +236,10 @@ def test_pcg_zero_rhs_returns_zero(hip):
     A.eliminate()
     x = pa.Vec(A.n, np.ones(A.n))
     nits, res = pa.pcg(A, pa.Vec(A.n), x)
+    # solver-phg.c:497-501: x = 0 and *res untouched, so PCG_Solve reports
+    # its initial 1.0
     assert nits == 0 and np.all(x.numpy() == 0.0)
+    assert res == 1.0
 
 
 def test_matvec_and_blas1(hip):

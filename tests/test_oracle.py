@@ -377,6 +377,21 @@ def test_pcg_diagonal_system_one_iteration():
     assert nits == 1 and np.allclose(x, 2.0, atol=0)
 
 
+def test_pcg_zero_rhs_and_breakdown():
+    """solver-phg.c:497-501: b = 0 gives x = 0, 0 iterations and leaves the
+    residual at PCG_Solve's 1.0; :547-570: rz = 0 or (p, Ap) = 0 ends the
+    loop with a warning, not an error.  On the indefinite diag(1, -1) with
+    b = (1, 1): z = p = (1, -1), A p = (1, 1), so (p, Ap) = 0 in iteration 1
+    and PCG stops there with x = 0 and the residual |b| = sqrt(2)."""
+    d = np.array([1.0, 2.0, 3.0])
+    x, nits, res = orc.pcg_arrays(np.arange(4), np.arange(3), d, np.zeros(3))
+    assert nits == 0 and res == 1.0 and not x.any()
+    # (p, Ap) = 0 in the first iteration: p = z = (1, 1), A p = (1, -1)
+    x, nits, res = orc.pcg_arrays(np.arange(3), np.arange(2), np.array([1.0, -1.0]),
+                                  np.array([1.0, 1.0]))
+    assert nits == 1 and not x.any() and res == np.sqrt(2.0)
+
+
 def test_pcg_against_scipy_cg():
     import scipy.sparse.linalg as spla
     c = poisson_case(4, 2, "kappa")
