@@ -43,10 +43,23 @@ def spmv_bytes(n, nnz, s_val, s_vec):
     return nnz * (s_val + 4) + (n + 1) * 4 + n * s_vec + n * s_vec
 
 
+def source_digest(files):
+    """sha256 (16 hex digits) of the kernel sources a measurement was made on."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in files:
+        with open(os.path.join(ROOT, f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
 def measured_traffic(kernel_key, nnz):
     """HBM bytes per launch of a kernel from the committed PMC summary
     (profiles/hbm_traffic.json: 2 x FETCH_SIZE + WRITE_SIZE, scaled by a copy
-    kernel of known size), when it was measured on this workload."""
+    kernel of known size), when it was measured on this workload and on the
+    kernel sources of this tree (their digest is stored with the numbers;
+    after a change of those sources the figure is stale and None is
+    returned)."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     try:
         with open(path) as f:
@@ -55,6 +68,9 @@ def measured_traffic(kernel_key, nnz):
         return None
     k = t.get("kernels", {}).get(kernel_key)
     if not k or k.get("nnz") != nnz:
+        return None
+    src = k.get("sources")
+    if not src or source_digest(src) != k.get("source_sha16"):
         return None
     return k["bytes"]
 
@@ -73,25 +89,40 @@ def build_problem(n, p, sigma, seed):
     return g, nd, bd, kappa, f, gv, t1 - t0
 
 
-def cpu_baseline(n_cpu, p, sigma, seed, rtol):
-    """Oracle (sequential C restatement of PHG's assembly + PCG) on a bounded
-    sample of the same workload family, one thread."""
-    import phg_amd as pa
+def cpu_baseline(g, p, kappa, its):
+    """Oracle (sequential C restatement of PHG's assembly + Jacobi-PCG,
+    solver-phg.c:485) on the benchmark's own system, one thread: the
+    assembly and elimination in full, then a bounded sample of `its` PCG
+    iterations from x0 = 0 (the solve to the stopping rule takes about a
+    minute on one core; PCG iterations cost the same throughout)."""
     import oracle as orc
-    g = pa.Grid.kuhn(n_cpu)
     nd = g.ndof(p)
     bd = g.dof_bdry(p)
-    rng = np.random.default_rng(seed)
-    kappa = np.exp(rng.normal(0.0, sigma, size=g.nelem))
     f, gv = np.ones(nd), np.zeros(nd)
     t0 = time.perf_counter()
     A = orc.assemble(orc.problem(p, use_kappa=True), g.verts(), g.elem_verts(),
                      g.dof_map(p), nd, bd, gv, f, kappa)
     orc.eliminate(A, bd)
     t1 = time.perf_counter()
-    x, nits, res = orc.pcg(A, A.rhs, rtol=rtol, btol=rtol)
+    x, nits, res = orc.pcg(A, A.rhs, maxit=its, rtol=1e-30, btol=1e-30)
     t2 = time.perf_counter()
-    return dict(ndof=nd, nelem=g.nelem, assemble_s=t1 - t0, solve_s=t2 - t1, nits=nits)
+    return dict(ndof=nd, nelem=g.nelem, assemble_s=t1 - t0, sample_s=t2 - t1,
+                sample_its=nits)
+
+
+def oracle_reference_error(x, n, order, sigma, seed):
+    """Relative 2-norm error of x against the oracle's FP64 solution of this
+    system (tests/golden/c2_64_oracle.npz: the oracle's PCG to 1e-12 at a
+    fixed sample of 4,096 DOFs, and the norm of the whole solution), when
+    the workload is that system; None otherwise."""
+    if (n, order, sigma, seed) != (64, 2, 0.5, 2024):
+        return None
+    ref = np.load(os.path.join(ROOT, "tests", "golden", "c2_64_oracle.npz"))
+    idx, xs = ref["index"], ref["x"]
+    return {"sampled_rel_err": float(np.linalg.norm(x[idx] - xs) / np.linalg.norm(xs)),
+            "norm_rel_diff": float(abs(np.linalg.norm(x) - float(ref["norm2"])) /
+                                   float(ref["norm2"])),
+            "samples": int(len(idx)), "oracle_its": int(ref["nits"])}
 
 
 def main():
@@ -112,8 +143,9 @@ def main():
     ap.add_argument("--delta", type=float, default=0.1,
                     help="reliable-update threshold")
     ap.add_argument("--inner-rtol", type=float, default=1e-4)
-    ap.add_argument("--cpu-n", type=int, default=40,
-                    help="cubes per side of the CPU baseline sample (0: skip)")
+    ap.add_argument("--cpu-its", type=int, default=150,
+                    help="PCG iterations of the CPU baseline sample on the benchmark's "
+                         "system (0: skip the CPU baseline)")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--operator", choices=("element", "csr"), default="element",
                     help="FP32 operator of the mixed solver: the matrix-free element "
@@ -203,7 +235,12 @@ def main():
 
     # mixed precision (the headline)
     sec_mp, out_mp = timed(run_mixed, args.steps, args.warmup)
-    err_mp = np.linalg.norm(x.numpy() - xr) / nxr
+    x_mp = x.numpy()
+    err_mp = np.linalg.norm(x_mp - xr) / nxr
+    err_oracle = oracle_reference_error(x_mp, args.n, args.order, args.sigma, args.seed)
+    if err_oracle is not None:
+        log(f"mixed vs the oracle's FP64 solution: sampled rel.err "
+            f"{err_oracle['sampled_rel_err']:.2e}, |x| rel.diff {err_oracle['norm_rel_diff']:.2e}")
     if args.mixed == "reliable":
         inner, outer, res_mp = out_mp
     else:
@@ -309,14 +346,20 @@ def main():
         f"FP64 {asm_solve['fp64_dofs_per_s'] / 1e6:.1f} M DOFs/s")
 
     cpu = None
-    if args.cpu_n > 0:
-        c = cpu_baseline(args.cpu_n, args.order, args.sigma, args.seed, args.rtol)
-        cpu = {"value": round(c["ndof"] / c["solve_s"], 1), "unit": "DOFs/s",
+    if args.cpu_its > 0:
+        c = cpu_baseline(g, args.order, kappa, args.cpu_its)
+        # the solve to the stopping rule takes the FP64 iteration count (the
+        # GPU FP64 PCG follows the oracle's recurrence, within 2 iterations)
+        t_solve = c["sample_s"] / c["sample_its"] * nits64
+        cpu = {"value": round(c["ndof"] / t_solve, 1), "unit": "DOFs/s",
                "cores": 1, "kind": "port",
                "sample": (f"oracle FP64 Jacobi-PCG (sequential C restatement of "
-                          f"solver-phg.c), P2 kappa-Poisson {args.cpu_n}^3x6 "
-                          f"({c['ndof']} DOFs), rtol {args.rtol}: {c['nits']} its in "
-                          f"{c['solve_s']:.2f}s; oracle assembly {c['assemble_s']:.2f}s")}
+                          f"solver-phg.c:485) on this system ({c['ndof']} DOFs): "
+                          f"{c['sample_its']} iterations in {c['sample_s']:.2f}s, "
+                          f"scaled to the {nits64} iterations of rtol {args.rtol} "
+                          f"({t_solve:.1f}s per solve); oracle assembly + elimination "
+                          f"{c['assemble_s']:.2f}s"),
+               "assemble_s": round(c["assemble_s"], 3)}
         log("cpu baseline:", cpu["sample"])
 
     value = nd / sec_mp
@@ -350,6 +393,7 @@ def main():
             "nnz": nnz, "parallelism": "single GPU",
         },
         "rel_err_vs_fp64": float(f"{err_mp:.3e}"),
+        "rel_err_vs_oracle": err_oracle,
         "fp64_pcg": {"value": round(nd / sec_64, 1), "ms_per_solve": round(sec_64 * 1e3, 3),
                      "its": nits64, "rel_err": float(f"{err_64:.3e}")},
         "mixed_vs_fp64": round(sec_64 / sec_mp, 3),
