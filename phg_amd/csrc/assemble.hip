@@ -27,6 +27,8 @@
 
 #include <stdio.h>
 #include <string.h>
+
+#include <atomic>
 #include <vector>
 
 #pragma clang fp contract(off)
@@ -1571,15 +1573,23 @@ extern "C" int phg_hip_space_create(int32_t order, int64_t nvert,
             return phg::set_error(PHG_HIP_E_ARG, "DOF index out of range");
     if (nelem * (int64_t)nb * 3 >= (int64_t)INT32_MAX)
         return phg::set_error(PHG_HIP_E_ARG, "mesh too large for int32 codes");
+    static std::atomic<uint64_t> last_uid{0};
     phg_hip_space *sp = new phg_hip_space();
+    sp->uid = ++last_uid;
     sp->order = order;
     sp->nbas = nb;
     sp->nvert = nvert;
     sp->nelem = nelem;
     sp->ndof = ndof;
-    sp->h_verts.assign(verts, verts + nvert * 3);
-    sp->h_elem_verts.assign(elem_verts, elem_verts + nelem * 4);
-    sp->h_map.assign(map, map + nelem * nb);
+    try {
+        sp->h_verts.assign(verts, verts + nvert * 3);
+        sp->h_elem_verts.assign(elem_verts, elem_verts + nelem * 4);
+        sp->h_map.assign(map, map + nelem * nb);
+    }
+    catch (const std::bad_alloc &) {
+        delete sp;
+        return phg::set_error(PHG_HIP_E_NOMEM, "phg_hip_space_create: out of host memory");
+    }
     // bounding box of the vertices: every Lagrange node is a convex
     // combination of its element's vertices, so it bounds the nodes too
     for (int c = 0; c < 3; c++)
@@ -1691,6 +1701,7 @@ extern "C" int phg_hip_assemble(phg_hip_space *sp, const phg_hip_problem *pb,
     A->n = n;
     A->dim = dim;
     A->ndof = sp->ndof;
+    A->space_uid = sp->uid;
     for (int c = 0; c < 3; c++) {
         A->xyz_lo[c] = sp->lo[c];
         A->xyz_hi[c] = sp->hi[c];

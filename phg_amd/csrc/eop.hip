@@ -43,6 +43,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <memory>
 #include <numeric>
@@ -63,7 +64,9 @@ constexpr uint32_t KD_NONE = 0xffffffffu;
 
 // Host set-up work split over up to 16 threads: f(lo, hi, t) for contiguous
 // chunks [lo, hi) of [0, n), at least `grain` items each, t the chunk
-// number.  A thread that cannot be started runs its chunk on the caller.
+// number.  A thread that cannot be started runs its chunk on the caller.  An
+// allocation that fails in any chunk is reported to the caller as
+// std::bad_alloc once every chunk has ended.
 inline int host_threads(int64_t n, int64_t grain = 1024)
 {
     const int64_t hw = std::max(1u, std::thread::hardware_concurrency());
@@ -75,18 +78,29 @@ void host_parallel(int64_t n, F &&f, int64_t grain = 1024)
 {
     const int T = host_threads(n, grain);
     std::vector<std::thread> th;
+    std::atomic<bool> failed{false};
+    auto run = [&f, &failed](int64_t lo, int64_t hi, int t) {
+        try {
+            f(lo, hi, t);
+        }
+        catch (...) {
+            failed = true;
+        }
+    };
     for (int t = 1; t < T; t++) {
         const int64_t lo = n * t / T, hi = n * (t + 1) / T;
         try {
-            th.emplace_back([&f, lo, hi, t] { f(lo, hi, t); });
+            th.emplace_back([&run, lo, hi, t] { run(lo, hi, t); });
         }
         catch (...) {
-            f(lo, hi, t);
+            run(lo, hi, t);
         }
     }
-    f(0, n / T, 0);
+    run(0, n / T, 0);
     for (auto &x : th)
         x.join();
+    if (failed)
+        throw std::bad_alloc();
 }
 
 // Allocator whose value-initialisation is a no-op: the large set-up arrays
@@ -589,11 +603,16 @@ k_eop_block(BlockView m, const float *__restrict__ x, float *__restrict__ y,
 }
 
 // shared rows: the sum of their partials in block order (the solvers do
-// this in their update kernel instead; see eop_shared_code)
+// this in their update kernel instead when the per-row codes exist).  In a
+// solver slot (done set) it returns at once, as the block kernel does, once
+// the solve has finished or the slot is not an iteration slot.
 __global__ void __launch_bounds__(FB)
 k_eop_fix(int64_t nfx, const int32_t *__restrict__ rows, const int32_t *__restrict__ off,
-          const float *__restrict__ part, float *__restrict__ y)
+          const float *__restrict__ part, float *__restrict__ y, const int32_t *done,
+          const int32_t *mode, int32_t want)
 {
+    if (done != nullptr && (*done != 0 || *mode != want))
+        return;
     for (int64_t t = (int64_t)blockIdx.x * FB + threadIdx.x; t < nfx;
          t += (int64_t)gridDim.x * FB) {
         float sum = 0.0f;
@@ -716,20 +735,9 @@ void free_eop(phg_hip_mat *A)
     EopData *eo = A->eop;
     if (eo == nullptr)
         return;
-    dfree(eo->eb_elem);
-    dfree(eo->eb_row);
-    dfree(eo->eb_nint);
-    dfree(eo->rows);
-    dfree(eo->rs);
-    dfree(eo->ppos);
-    dfree(eo->idx);
     dfree(eo->coef);
-    dfree(eo->tiles);
     dfree(eo->part);
-    dfree(eo->fx_rows);
-    dfree(eo->fx_off);
-    dfree(eo->fxc);
-    delete eo;
+    delete eo;          // drops this matrix's reference to the layout
     A->eop = nullptr;
 }
 
@@ -748,10 +756,26 @@ void eop_apply(const EopData *eo, const float *x, const float *diag, float *y,
     if (which != 1 && fixup && eo->nfx > 0)
         hipLaunchKernelGGL(k_eop_fix, dim3(eo->grid_f), dim3(FB), 0, s, eo->nfx,
                            (const int32_t *)eo->fx_rows, (const int32_t *)eo->fx_off,
-                           (const float *)eo->part, y);
+                           (const float *)eo->part, y, done, mode, want);
 }
 
 }  // namespace phg
+
+EopLayout::~EopLayout()
+{
+    phg::dfree(eb_elem);
+    phg::dfree(eb_row);
+    phg::dfree(eb_nint);
+    phg::dfree(rows);
+    phg::dfree(rs);
+    phg::dfree(ppos);
+    phg::dfree(idx);
+    phg::dfree(tiles);
+    phg::dfree(fx_rows);
+    phg::dfree(fx_off);
+    phg::dfree(fxc);
+    phg::dfree(eperm);
+}
 
 //============================================================== C ABI ====
 
@@ -792,6 +816,14 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
     const int64_t ne = sp->nelem, n = A->n;
     if ((int64_t)nb * ne >= (int64_t)1 << 30)
         return set_error(PHG_HIP_E_ARG, "phg_hip_mat_element_operator: too many elements");
+    // per-row codes of the shared rows for the solvers' update kernel, unless
+    // PHG_HIP_EOP_NO_ROW_CODES is set (then the fix-up kernel sums them)
+    const int codes = getenv("PHG_HIP_EOP_NO_ROW_CODES") == nullptr ? 1 : 0;
+    // A matrix assembled on this space has the space's Dirichlet rows and
+    // solver numbering (both functions of the mesh alone), so it shares the
+    // space's layout; only the coefficients depend on kappa.
+    const bool same_space = A->space_uid != 0 && A->space_uid == sp->uid;
+    auto build_layout = [&](EopLayout &L) -> int {
     // the space keeps host copies of its element maps
     const std::vector<int32_t> &map = sp->h_map, &ev = sp->h_elem_verts;
     const std::vector<double> &xyz = sp->h_verts;
@@ -1064,74 +1096,45 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
         tiles.resize(tf.size() * 4);
         memcpy(tiles.data(), tf.data(), tiles.size());
     }
-    EopData *eo = new EopData();
-    eo->p = p;
-    eo->nb = nb;
-    eo->ks = ks;
-    eo->nelem = ne;
-    eo->n = n;
-    eo->nblk = nblk;
-    eo->nent = nent;
-    eo->nint = nint;
-    eo->nfx = nfx;
-    eo->ndir = ndir;
-    eo->npart = npart;
-    A->eop = eo;
-    double *dk = nullptr;
-    int32_t *dperm = nullptr;
-    auto fail = [&](int code) {
-        dfree(dk);
-        dfree(dperm);
-        free_eop(A);
-        return code;
-    };
+    L.p = p;
+    L.nb = nb;
+    L.ks = ks;
+    L.nelem = ne;
+    L.n = n;
+    L.nblk = nblk;
+    L.nent = nent;
+    L.nint = nint;
+    L.nfx = nfx;
+    L.ndir = ndir;
+    L.npart = npart;
     auto put = [&](void **d, const void *h, size_t bytes) {
         int r2 = dmalloc(d, std::max<size_t>(bytes, 4));
         if (r2 == 0 && bytes > 0)
             r2 = upload(*d, h, bytes);
         return r2;
     };
-    if ((rc = put((void **)&eo->eb_elem, eb_elem.data(), eb_elem.size() * 4)) ||
-        (rc = put((void **)&eo->eb_row, eb_row.data(), eb_row.size() * 4)) ||
-        (rc = put((void **)&eo->eb_nint, eb_nint.data(), eb_nint.size() * 4)) ||
-        (rc = put((void **)&eo->rows, rows.data(), rows.size() * 4)) ||
-        (rc = put((void **)&eo->rs, rs.data(), rs.size() * 2)) ||
-        (rc = put((void **)&eo->ppos, ppos.data(), ppos.size() * 4)) ||
-        (rc = put((void **)&eo->idx, idx.data(), idx.size() * 4)) ||
-        (rc = put((void **)&eo->tiles, tiles.data(), tiles.size())) ||
-        (rc = put((void **)&eo->fx_rows, fx_rows.data(), fx_rows.size() * 4)) ||
-        (rc = put((void **)&eo->fx_off, fx_off.data(), fx_off.size() * 4)) ||
-        (rc = dmalloc((void **)&eo->part, (size_t)std::max<int64_t>(npart, 1) * 4)) ||
-        (rc = dmalloc((void **)&eo->coef, ((size_t)ne * NC + 2) * 4)) ||
-        (rc = put((void **)&dperm, eperm.data(), (size_t)ne * 4)))
-        return fail(rc);
-    if (pb->use_kappa &&
-        ((rc = dmalloc((void **)&dk, (size_t)ne * 8)) ||
-         (rc = upload(dk, kappa, (size_t)ne * 8))))
-        return fail(rc);
-    if ((rc = eop_coefficients(sp, pb->use_kappa ? dk : nullptr, dperm, eo->coef, s)))
-        return fail(rc);
-    // two entries of padding, read (and multiplied by zero) for the last
-    // element
+    if ((rc = put((void **)&L.eb_elem, eb_elem.data(), eb_elem.size() * 4)) ||
+        (rc = put((void **)&L.eb_row, eb_row.data(), eb_row.size() * 4)) ||
+        (rc = put((void **)&L.eb_nint, eb_nint.data(), eb_nint.size() * 4)) ||
+        (rc = put((void **)&L.rows, rows.data(), rows.size() * 4)) ||
+        (rc = put((void **)&L.rs, rs.data(), rs.size() * 2)) ||
+        (rc = put((void **)&L.ppos, ppos.data(), ppos.size() * 4)) ||
+        (rc = put((void **)&L.idx, idx.data(), idx.size() * 4)) ||
+        (rc = put((void **)&L.tiles, tiles.data(), tiles.size())) ||
+        (rc = put((void **)&L.fx_rows, fx_rows.data(), fx_rows.size() * 4)) ||
+        (rc = put((void **)&L.fx_off, fx_off.data(), fx_off.size() * 4)) ||
+        (rc = put((void **)&L.eperm, eperm.data(), (size_t)ne * 4)))
+        return rc;
     mark("uploads");
-    if (hipMemsetAsync(eo->coef + ne * NC, 0, 2 * sizeof(float), s) != hipSuccess)
-        return fail(hip_fail(hipGetLastError(), "element operator set-up", __FILE__, __LINE__));
-    hipError_t e = hipStreamSynchronize(s);
-    dfree(dk);
-    dfree(dperm);
-    dk = nullptr;
-    dperm = nullptr;
-    if (e != hipSuccess)
-        return fail(hip_fail(e, "element operator set-up", __FILE__, __LINE__));
     // grids: every block group resident; partial sums of both kernels fit
     // the solvers' partial-sum segment
     const int gb = p == 1 ? grid_block<1>() : (p == 2 ? grid_block<2>() : grid_block<3>());
-    eo->grid_b = (int)std::max<int64_t>(1, std::min<int64_t>({nblk, (int64_t)gb,
-                                                              (int64_t)EOP_MAX_PART}));
-    eo->grid_f = (int)std::max<int64_t>(1, std::min<int64_t>((nfx + FB - 1) / FB, 4096));
-    // per-row codes of the shared rows for the solvers' update kernel:
-    // partial start | count << 24 (when every start and count fits)
-    bool fits = npart < ((int64_t)1 << 24) - 1;
+    L.grid_b = (int)std::max<int64_t>(1, std::min<int64_t>({nblk, (int64_t)gb,
+                                                             (int64_t)EOP_MAX_PART}));
+    L.grid_f = (int)std::max<int64_t>(1, std::min<int64_t>((nfx + FB - 1) / FB, 4096));
+    // per-row codes: partial start | count << 24 (when every start and count
+    // fits)
+    bool fits = codes && npart < ((int64_t)1 << 24) - 1;
     for (int64_t t = 0; t < nfx && fits; t++)
         fits = fx_off[(size_t)t + 1] - fx_off[(size_t)t] < 256;
     if (fits) {
@@ -1140,11 +1143,87 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
             code[(size_t)fx_rows[(size_t)t]] =
                 (uint32_t)fx_off[(size_t)t] |
                 (uint32_t)(fx_off[(size_t)t + 1] - fx_off[(size_t)t]) << 24;
-        if ((rc = dmalloc((void **)&eo->fxc, (size_t)n * 4)) ||
-            (rc = upload(eo->fxc, code.data(), (size_t)n * 4)))
-            return fail(rc);
+        if ((rc = put((void **)&L.fxc, code.data(), (size_t)n * 4)))
+            return rc;
     }
-    mark("coefficients, codes");
+    mark("shared-row codes");
+    return PHG_HIP_OK;
+    };
+
+    std::shared_ptr<EopLayout> lay;
+    if (same_space && sp->eop_layout && sp->eop_layout->n == n && sp->eop_layout->p == p &&
+        sp->eop_layout_codes == codes) {
+        lay = sp->eop_layout;
+        mark("layout of the space");
+    }
+    else {
+        try {
+            lay = std::make_shared<EopLayout>();
+            rc = build_layout(*lay);
+        }
+        catch (const std::bad_alloc &) {
+            rc = set_error(PHG_HIP_E_NOMEM, "element operator set-up: out of host memory");
+        }
+        if (rc)
+            return rc;      // lay frees what was allocated
+        if (same_space) {
+            sp->eop_layout = lay;
+            sp->eop_layout_codes = codes;
+        }
+    }
+    // per matrix: the coefficients and the partial sums
+    EopData *eo = new EopData();
+    eo->p = lay->p;
+    eo->nb = lay->nb;
+    eo->ks = lay->ks;
+    eo->nelem = lay->nelem;
+    eo->n = lay->n;
+    eo->nblk = lay->nblk;
+    eo->nent = lay->nent;
+    eo->nint = lay->nint;
+    eo->eb_elem = lay->eb_elem;
+    eo->eb_row = lay->eb_row;
+    eo->eb_nint = lay->eb_nint;
+    eo->rows = lay->rows;
+    eo->rs = lay->rs;
+    eo->ppos = lay->ppos;
+    eo->idx = lay->idx;
+    eo->tiles = lay->tiles;
+    eo->npart = lay->npart;
+    eo->fx_rows = lay->fx_rows;
+    eo->fx_off = lay->fx_off;
+    eo->nfx = lay->nfx;
+    eo->ndir = lay->ndir;
+    eo->fxc = lay->fxc;
+    eo->grid_b = lay->grid_b;
+    eo->grid_f = lay->grid_f;
+    eo->lay = lay;
+    A->eop = eo;
+    double *dk = nullptr;
+    auto fail = [&](int code) {
+        dfree(dk);
+        free_eop(A);
+        return code;
+    };
+    if ((rc = dmalloc((void **)&eo->part, (size_t)std::max<int64_t>(lay->npart, 1) * 4)) ||
+        (rc = dmalloc((void **)&eo->coef, ((size_t)ne * NC + 2) * 4)))
+        return fail(rc);
+    if (pb->use_kappa &&
+        ((rc = dmalloc((void **)&dk, (size_t)ne * 8)) ||
+         (rc = upload(dk, kappa, (size_t)ne * 8))))
+        return fail(rc);
+    if ((rc = eop_coefficients(sp, pb->use_kappa ? dk : nullptr, lay->eperm, eo->coef, s)))
+        return fail(rc);
+    // two entries of padding, read (and multiplied by zero) for the last
+    // element
+    if (hipMemsetAsync(eo->coef + ne * NC, 0, 2 * sizeof(float), s) != hipSuccess)
+        return fail(hip_fail(hipGetLastError(), "element operator set-up", __FILE__, __LINE__));
+    hipError_t e = hipStreamSynchronize(s);
+    dfree(dk);
+    dk = nullptr;
+    if (e != hipSuccess)
+        return fail(hip_fail(e, "element operator set-up", __FILE__, __LINE__));
+    mark("coefficients");
     eo->on = 1;
     return PHG_HIP_OK;
 }

@@ -5,6 +5,7 @@
 #include <stdint.h>
 #include <stddef.h>
 
+#include <memory>
 #include <vector>
 
 #include "../../include/phg_hip.h"
@@ -54,10 +55,37 @@ int noalloc_init(int device);
 
 }  // namespace phg
 
+// Element-block layout of the matrix-free operator (eop.hip, see EopData
+// below): the element order, the blocks, their row tables, index chunks
+// and the shared rows.  It depends on the space (element maps, vertex
+// coordinates), its Dirichlet rows and the solver numbering, not on the
+// coefficients, so the operators of all matrices assembled on one space
+// share it.  The destructor frees the device arrays.
+struct EopLayout {
+    int p = 0, nb = 0, ks = 0;
+    int64_t nelem = 0, n = 0, nblk = 0, nent = 0, nint = 0, nfx = 0, ndir = 0, npart = 0;
+    int grid_b = 0, grid_f = 0;
+    int32_t *eb_elem = nullptr, *eb_row = nullptr, *eb_nint = nullptr, *rows = nullptr;
+    uint16_t *rs = nullptr;
+    int32_t *ppos = nullptr;
+    uint32_t *idx = nullptr;
+    uint8_t *tiles = nullptr;
+    int32_t *fx_rows = nullptr, *fx_off = nullptr;
+    uint32_t *fxc = nullptr;
+    int32_t *eperm = nullptr;        // [nelem] Morton position -> element
+    ~EopLayout();
+};
+
 // opaque handles -----------------------------------------------------------
 struct phg_hip_space {
     int32_t order, nbas;
     int64_t nvert, nelem, ndof;
+    uint64_t uid = 0;                // unique per space of the process
+    // element-operator layout of matrices assembled on this space (built by
+    // the first phg_hip_mat_element_operator on one; codes: with the per-row
+    // codes of the shared rows)
+    mutable std::shared_ptr<EopLayout> eop_layout;
+    mutable int eop_layout_codes = -1;
     double *d_verts = nullptr;       // [nvert][3]
     int32_t *d_elem_verts = nullptr; // [nelem][4]
     int32_t *d_map = nullptr;        // [nelem][nbas]
@@ -127,6 +155,8 @@ struct EopData {
     uint32_t *fxc = nullptr;          // [n] partial start | count << 24 of a shared
                                       // row, EOP_NOT_SHARED otherwise (null: no codes)
     int grid_b = 0, grid_f = 0;       // launch grids; grid_b partial sums of (x, y)
+    // the layout the pointers above (all but coef and part) belong to
+    std::shared_ptr<EopLayout> lay;
 };
 
 struct phg_hip_mat {
@@ -150,6 +180,7 @@ struct phg_hip_mat {
     double xyz_lo[3] = {0, 0, 0}, xyz_hi[3] = {0, 0, 0};   // bounding box of the nodes
     int xyz_box = 0;                                       // xyz_lo/hi set
     int64_t ndof = 0;
+    uint64_t space_uid = 0;          // uid of the space phg_hip_assemble used (0: none)
 
     // Solver copy: the eliminated matrix under a symmetric renumbering that
     // orders DOFs along a Morton curve of their nodes (q_perm: solver index ->

@@ -55,7 +55,10 @@ def test_apply_matches_assembled_matrix(hip, kind, p, n):
 # meshes of several element blocks (blocks of 2048 / 1024 / 512 elements
 # for P1 / P2 / P3), so rows shared between blocks go through the partials
 # 32^3 x 6 = 196,608 elements: the block split runs on three element ranges
-MULTI_BLOCK_CASES = [("kappa", 1, 16), ("kappa", 2, 9), ("sin", 3, 6), ("kappa", 1, 32)]
+# of 65,536 elements each, whole blocks; 33^3 x 6 = 215,622 elements: three
+# ranges whose ends fall inside a block, so each range ends with a short block
+MULTI_BLOCK_CASES = [("kappa", 1, 16), ("kappa", 2, 9), ("sin", 3, 6), ("kappa", 1, 32),
+                     ("kappa", 1, 33)]
 
 
 @pytest.mark.parametrize("kind,p,n", MULTI_BLOCK_CASES)
@@ -179,3 +182,68 @@ def test_element_operator_rejects_other_problems(hip):
         A.element_operator(sp, mass)
     with pytest.raises(pa.PhgHipError):
         A.use_element_operator(True)    # not built on this matrix
+
+
+def test_layout_shared_by_matrices_of_one_space(hip):
+    """The block layout depends on the mesh, its Dirichlet rows and the
+    solver numbering, not on kappa: a second matrix assembled on the same
+    space (another kappa) takes the space's layout and only computes its
+    coefficients.  Its operator equals, bit for bit, the one built from
+    scratch on a fresh space, and matches its own FP64 matrix."""
+    pa = hip
+    c = poisson_case(8, 2, "kappa")
+    pb = pa.Problem.poisson(use_kappa=True)
+    sp = pa.Space(c["grid"], 2, bdry=c["bdry"])
+    kappa2 = np.exp(np.random.default_rng(99).normal(0.0, 0.5, c["grid"].nelem))
+    A1 = pa.assemble(sp, pb, kappa=c["kappa"], f=c["f"], g=c["g"])
+    A1.eliminate()
+    A1.element_operator(sp, pb, kappa=c["kappa"])
+    A2 = pa.assemble(sp, pb, kappa=kappa2, f=c["f"], g=c["g"])
+    A2.eliminate()
+    A2.element_operator(sp, pb, kappa=kappa2)          # the space's layout
+    sp3 = pa.Space(c["grid"], 2, bdry=c["bdry"])
+    A3 = pa.assemble(sp3, pb, kappa=kappa2, f=c["f"], g=c["g"])
+    A3.eliminate()
+    A3.element_operator(sp3, pb, kappa=kappa2)         # built from scratch
+    assert A2.element_layout() == A1.element_layout() == A3.element_layout()
+    xv = np.random.default_rng(5).standard_normal(A2.n)
+    x, y64, y2, y3 = pa.Vec(A2.n, xv), pa.Vec(A2.n), pa.Vec(A2.n), pa.Vec(A2.n)
+    A2.matvec(x, y64)
+    A2.element_apply(x, y2)
+    A3.element_apply(x, y3)
+    assert np.array_equal(y2.numpy(), y3.numpy())
+    a = y64.numpy()
+    assert np.linalg.norm(y2.numpy() - a) / np.linalg.norm(a) < 2e-6
+    del A1                                              # the layout stays with A2
+    z = pa.Vec(A2.n)
+    A2.element_apply(x, z)
+    assert np.array_equal(z.numpy(), y2.numpy())
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_without_shared_row_codes(hip, monkeypatch, graph):
+    """With the per-row codes switched off (PHG_HIP_EOP_NO_ROW_CODES) the
+    shared rows are summed by the fix-up kernel in every iteration slot
+    instead of by the solver's update kernel, in the same block order, so
+    the reliable and the batched (graph-replayed) solves give the same
+    solutions bit for bit; the fix-up kernel returns at once in the slots
+    that are not iteration slots and after convergence."""
+    pa = hip
+    meshes = batch_meshes(nmesh=2, n=8, seed=3, lmin=1, lmax=2)   # several blocks each
+    runs = []
+    for off in (False, True):
+        if off:
+            monkeypatch.setenv("PHG_HIP_EOP_NO_ROW_CODES", "1")
+        sys_ = assemble_batch(meshes, p=2, seed=4, element_operator=True)
+        As = [A for _, A, _ in sys_]
+        bs = [b for _, _, b in sys_]
+        xs = [pa.Vec(A.n) for A in As]
+        nits, res = pa.pcg_multi(As, bs, xs, rtol=1e-10, btol=1e-10, use_graph=graph)
+        x1 = pa.Vec(As[0].n)
+        n1, _, r1 = pa.pcg_reliable(As[0], bs[0], x1, rtol=1e-10, btol=1e-10)
+        runs.append((list(nits), [x.numpy() for x in xs], n1, x1.numpy()))
+        monkeypatch.delenv("PHG_HIP_EOP_NO_ROW_CODES", raising=False)
+    (na, xa, n1a, x1a), (nb, xb, n1b, x1b) = runs
+    assert na == nb and n1a == n1b
+    assert all(np.array_equal(u, v) for u, v in zip(xa, xb))
+    assert np.array_equal(x1a, x1b)
