@@ -1225,18 +1225,35 @@ __global__ void k_scan_blocks(int64_t n, const int32_t *in, int64_t *out,
         bsum[blockIdx.x] = s[1023];
 }
 
-__global__ void k_scan_sums(int64_t nb, int64_t *bsum, int64_t *total)
+// exclusive scan of the nb block sums in place, total in *total: one
+// work-group of 1024, each work-item a consecutive chunk (integer sums, so
+// the result is the sequential scan's)
+__global__ void __launch_bounds__(1024) k_scan_sums(int64_t nb, int64_t *bsum,
+                                                    int64_t *total)
 {
-    // single work-item: the number of blocks is n/1024
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        int64_t acc = 0;
-        for (int64_t i = 0; i < nb; i++) {
-            int64_t v = bsum[i];
-            bsum[i] = acc;
-            acc += v;
-        }
-        *total = acc;
+    __shared__ int64_t sh[1024];
+    const int t = threadIdx.x;
+    const int64_t per = (nb + 1023) / 1024;
+    const int64_t i0 = min(nb, (int64_t)t * per), i1 = min(nb, i0 + per);
+    int64_t acc = 0;
+    for (int64_t i = i0; i < i1; i++)
+        acc += bsum[i];
+    sh[t] = acc;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const int64_t v = t >= o ? sh[t - o] : 0;
+        __syncthreads();
+        sh[t] += v;
+        __syncthreads();
     }
+    int64_t run = sh[t] - acc;
+    for (int64_t i = i0; i < i1; i++) {
+        const int64_t v = bsum[i];
+        bsum[i] = run;
+        run += v;
+    }
+    if (t == 1023)
+        *total = sh[1023];
 }
 
 __global__ void k_scan_add(int64_t n, int64_t *out, const int64_t *bsum,
@@ -1278,7 +1295,7 @@ int scan_i32(int64_t n, const int32_t *d_in, int64_t *d_out, int64_t *total_h,
     tot = bsum + nb;
     hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(1024), 0, s, n,
                        d_in, d_out, bsum);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1), 0, s, nb, bsum, tot);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, nb, bsum, tot);
     hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(1024), 0, s, n,
                        d_out, bsum, tot);
     PHG_HIP_CHECK(hipGetLastError());
@@ -1396,9 +1413,99 @@ fail:
     return rc;
 }
 
+}  // namespace
+
+// scratch buffers of the assembly workspace
+enum { WS_K, WS_B, WS_F, WS_KAPPA, WS_G, WS_COUNT, WS_SCOLS, WS_SVALS, WS_ROWPTR, WS_N };
+struct AsmWorkspace {
+    int dim = 0;                      // problem dimension of the incidence (0: none)
+    int32_t *cnt = nullptr, *inc = nullptr;
+    int64_t *inc_off = nullptr;
+    int64_t *scr_off = nullptr;       // scratch segment of each row (incidence * M)
+    int64_t nscr = 0;
+    void *buf[WS_N] = {};
+    size_t bytes[WS_N] = {};
+};
+
+void free_asm_workspace(AsmWorkspace *w)
+{
+    if (w == nullptr)
+        return;
+    phg::dfree(w->cnt);
+    phg::dfree(w->inc);
+    phg::dfree(w->inc_off);
+    phg::dfree(w->scr_off);
+    for (int i = 0; i < WS_N; i++)
+        phg::dfree(w->buf[i]);
+    delete w;
+}
+
+namespace {
+AsmWorkspace *workspace(phg_hip_space *sp)
+{
+    if (sp->asm_ws == nullptr)
+        sp->asm_ws = new AsmWorkspace();
+    return sp->asm_ws;
+}
+
+// a scratch buffer of at least `bytes`, kept by the space for later calls
+int ws_buffer(phg_hip_space *sp, int slot, size_t bytes, void **p)
+{
+    AsmWorkspace *w = workspace(sp);
+    if (w->bytes[slot] < bytes || w->buf[slot] == nullptr) {
+        phg::dfree(w->buf[slot]);
+        w->buf[slot] = nullptr;
+        w->bytes[slot] = 0;
+        int rc = phg::dmalloc(&w->buf[slot], bytes ? bytes : 1);
+        if (rc)
+            return rc;
+        w->bytes[slot] = bytes;
+    }
+    *p = w->buf[slot];
+    return PHG_HIP_OK;
+}
+
+// the row incidence of a problem dimension and its scratch segments, built
+// on the first call for that dimension
+int space_incidence(phg_hip_space *sp, int dim, AsmWorkspace **out)
+{
+    AsmWorkspace *w = workspace(sp);
+    *out = w;
+    if (w->dim == dim)
+        return PHG_HIP_OK;
+    hipStream_t s = phg::ctx()->stream;
+    phg::dfree(w->cnt);
+    phg::dfree(w->inc);
+    phg::dfree(w->inc_off);
+    phg::dfree(w->scr_off);
+    w->cnt = w->inc = nullptr;
+    w->inc_off = w->scr_off = nullptr;
+    w->dim = 0;
+    int rc;
+    if ((rc = build_incidence(sp, dim, &w->cnt, &w->inc_off, &w->inc)))
+        return rc;
+    const int64_t n = sp->ndof * dim;
+    const int M = sp->nbas * dim;
+    int32_t *count = nullptr;
+    if ((rc = phg::dmalloc((void **)&count, (size_t)(n + 1) * sizeof(int32_t))) ||
+        (rc = phg::dmalloc((void **)&w->scr_off, (size_t)(n + 1) * sizeof(int64_t)))) {
+        phg::dfree(count);
+        return rc;
+    }
+    // count[r] = cnt[r] * M (cnt holds the incidence sizes after the fill)
+    hipLaunchKernelGGL(k_scale_i32, dim3(nblk(n, 256)), dim3(256), 0, s, n,
+                       (const int32_t *)w->cnt, (int32_t)M, count);
+    rc = scan_i32(n, count, w->scr_off, &w->nscr, s);
+    phg::dfree(count);
+    if (rc)
+        return rc;
+    w->dim = dim;
+    return PHG_HIP_OK;
+}
+
 int element_pass(phg_hip_space *sp, const phg_hip_problem *pb,
                  const double *kappa_h, const double *f_h, double **dK,
-                 double **dB, double **dkap_out)
+                 double **dB)
 {
     hipStream_t s = phg::ctx()->stream;
     int N = sp->nbas, M = N * pb->dim, rc;
@@ -1408,56 +1515,39 @@ int element_pass(phg_hip_space *sp, const phg_hip_problem *pb,
         return phg::set_error(PHG_HIP_E_ARG, "use_kappa set but kappa is NULL");
     if (f_h == nullptr)
         return phg::set_error(PHG_HIP_E_ARG, "load vector f is NULL");
+    // element matrices and loads in the space's workspace
     size_t ksz = (size_t)sp->nelem * M * M * sizeof(double);
-    if ((rc = phg::dmalloc((void **)dK, ksz)) ||
-        (rc = phg::dmalloc((void **)dB, (size_t)sp->nelem * M * sizeof(double))) ||
-        (rc = phg::dmalloc((void **)&df, (size_t)sp->ndof * pb->dim * sizeof(double))))
-        goto fail;
-    if ((rc = phg::upload(df, f_h, (size_t)sp->ndof * pb->dim * sizeof(double))))
-        goto fail;
+    if ((rc = ws_buffer(sp, WS_K, ksz, (void **)dK)) ||
+        (rc = ws_buffer(sp, WS_B, (size_t)sp->nelem * M * sizeof(double), (void **)dB)) ||
+        (rc = ws_buffer(sp, WS_F, (size_t)sp->ndof * pb->dim * sizeof(double),
+                        (void **)&df)) ||
+        (rc = phg::upload(df, f_h, (size_t)sp->ndof * pb->dim * sizeof(double))))
+        return rc;
     if (pb->use_kappa) {
-        if ((rc = phg::dmalloc((void **)&dk, (size_t)sp->nelem * sizeof(double))) ||
+        if ((rc = ws_buffer(sp, WS_KAPPA, (size_t)sp->nelem * sizeof(double), (void **)&dk)) ||
             (rc = phg::upload(dk, kappa_h, (size_t)sp->nelem * sizeof(double))))
-            goto fail;
+            return rc;
     }
-    {
-        ElemArgs a;
-        a.nelem = sp->nelem;
-        a.verts = sp->d_verts;
-        a.elem_verts = sp->d_elem_verts;
-        a.map = sp->d_map;
-        a.kappa = dk;
-        a.f = df;
-        a.use_kappa = pb->use_kappa;
-        a.mass_coef = pb->mass_coef;
-        a.rhs_order = pb->rhs_order;
-        a.E = pb->E;
-        a.nu = pb->nu;
-        a.K = *dK;
-        a.B = *dB;
-        if ((rc = launch_element(sp, pb, a, s)))
-            goto fail;
-    }
-    {
-        hipError_t e = hipStreamSynchronize(s);
-        if (e != hipSuccess) {
-            rc = phg::hip_fail(e, "element kernel", __FILE__, __LINE__);
-            goto fail;
-        }
-    }
-    phg::dfree(df);
-    if (dkap_out)
-        *dkap_out = dk;
-    else
-        phg::dfree(dk);
+    ElemArgs a;
+    a.nelem = sp->nelem;
+    a.verts = sp->d_verts;
+    a.elem_verts = sp->d_elem_verts;
+    a.map = sp->d_map;
+    a.kappa = dk;
+    a.f = df;
+    a.use_kappa = pb->use_kappa;
+    a.mass_coef = pb->mass_coef;
+    a.rhs_order = pb->rhs_order;
+    a.E = pb->E;
+    a.nu = pb->nu;
+    a.K = *dK;
+    a.B = *dB;
+    if ((rc = launch_element(sp, pb, a, s)))
+        return rc;
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess)
+        return phg::hip_fail(e, "element kernel", __FILE__, __LINE__);
     return PHG_HIP_OK;
-fail:
-    phg::dfree(*dK);
-    phg::dfree(*dB);
-    phg::dfree(df);
-    phg::dfree(dk);
-    *dK = *dB = nullptr;
-    return rc;
 }
 
 // Element coefficients of the stiffness factorisation used by the
@@ -1622,6 +1712,7 @@ extern "C" int phg_hip_space_destroy(phg_hip_space *sp)
     phg::dfree(sp->d_elem_verts);
     phg::dfree(sp->d_map);
     phg::dfree(sp->d_bdry);
+    free_asm_workspace(sp->asm_ws);
     delete sp;
     return PHG_HIP_OK;
 }
@@ -1635,15 +1726,13 @@ extern "C" int phg_hip_element_matrices(phg_hip_space *sp,
     if ((rc = phg::require_ctx()) || (rc = check_problem(sp, pb)))
         return rc;
     double *dK, *dB;
-    if ((rc = element_pass(sp, pb, kappa, f, &dK, &dB, nullptr)))
+    if ((rc = element_pass(sp, pb, kappa, f, &dK, &dB)))
         return rc;
     int M = sp->nbas * pb->dim;
     if (K)
         rc = phg::download(K, dK, (size_t)sp->nelem * M * M * sizeof(double));
     if (!rc && b)
         rc = phg::download(b, dB, (size_t)sp->nelem * M * sizeof(double));
-    phg::dfree(dK);
-    phg::dfree(dB);
     return rc;
 }
 
@@ -1659,44 +1748,46 @@ extern "C" int phg_hip_assemble(phg_hip_space *sp, const phg_hip_problem *pb,
     hipStream_t s = phg::ctx()->stream;
     const int N = sp->nbas, dim = pb->dim, M = N * dim;
     const int64_t n = sp->ndof * dim, ne = sp->nelem;
+    // scratch (element matrices, incidence, row segments) from the space's
+    // workspace; the matrix owns its CSR, right-hand side and row flags
     double *dK = nullptr, *dB = nullptr, *dg = nullptr;
-    int32_t *cnt = nullptr, *inc = nullptr, *count = nullptr;
-    int64_t *inc_off = nullptr, *scr_off = nullptr, *rowptr64 = nullptr;
+    int32_t *count = nullptr;
+    int64_t *rowptr64 = nullptr;
     int32_t *scols = nullptr;
     double *svals = nullptr;
     uint8_t *bdry_row = nullptr;
-    int64_t ninc = 0, nscr = 0, nnz = 0;
+    AsmWorkspace *w = nullptr;
+    int64_t nnz = 0;
     phg_hip_mat *A = nullptr;
+    phg::PhaseTrace tr("phg_hip_assemble");
 
-    if ((rc = element_pass(sp, pb, kappa, f, &dK, &dB, nullptr)))
+    if ((rc = element_pass(sp, pb, kappa, f, &dK, &dB)))
         return rc;
+    tr.mark("element pass (uploads f, kappa)");
 
     // per-row Dirichlet flags (each node's flag on its dim rows) and values
     if ((rc = phg::dmalloc((void **)&bdry_row, (size_t)n)))
         goto done;
     hipLaunchKernelGGL(k_row_flags, dim3(nblk(n, 256)), dim3(256), 0, s, n, dim,
                        (const uint8_t *)sp->d_bdry, bdry_row);
-    if ((rc = phg::dmalloc((void **)&dg, (size_t)n * sizeof(double))) ||
+    if ((rc = ws_buffer(sp, WS_G, (size_t)n * sizeof(double), (void **)&dg)) ||
         (rc = phg::upload(dg, g, (size_t)n * sizeof(double))))
         goto done;
 
+    tr.mark("row flags, upload g");
     // incidence
-    if ((rc = build_incidence(sp, dim, &cnt, &inc_off, &inc)))
+    if ((rc = space_incidence(sp, dim, &w)))
         goto done;
-    (void)ninc;
-    // scratch segments: count * M entries per row (upper bound)
-    if ((rc = phg::dmalloc((void **)&count, (size_t)(n + 1) * sizeof(int32_t))) ||
-        (rc = phg::dmalloc((void **)&scr_off, (size_t)(n + 1) * sizeof(int64_t))))
-        goto done;
-    // count[r] = cnt[r] * M (cnt holds the incidence sizes after the fill)
-    hipLaunchKernelGGL(k_scale_i32, dim3(nblk(n, 256)), dim3(256), 0, s, n,
-                       (const int32_t *)cnt, (int32_t)M, count);
-    if ((rc = scan_i32(n, count, scr_off, &nscr, s)))
-        goto done;
-    if ((rc = phg::dmalloc((void **)&scols, (size_t)(nscr + 1) * sizeof(int32_t))) ||
-        (rc = phg::dmalloc((void **)&svals, (size_t)(nscr + 1) * sizeof(double))))
+    tr.mark("incidence");
+    // scratch segments: incidence * M entries per row (upper bound)
+    if ((rc = ws_buffer(sp, WS_COUNT, (size_t)(n + 1) * sizeof(int32_t), (void **)&count)) ||
+        (rc = ws_buffer(sp, WS_SCOLS, (size_t)(w->nscr + 1) * sizeof(int32_t),
+                        (void **)&scols)) ||
+        (rc = ws_buffer(sp, WS_SVALS, (size_t)(w->nscr + 1) * sizeof(double),
+                        (void **)&svals)))
         goto done;
 
+    tr.mark("scratch segments");
     A = new phg_hip_mat();
     A->n = n;
     A->dim = dim;
@@ -1718,34 +1809,36 @@ extern "C" int phg_hip_assemble(phg_hip_space *sp, const phg_hip_problem *pb,
                        (const int32_t *)sp->d_elem_verts,
                        (const int32_t *)sp->d_map, A->d_xyz);
     {
-        WalkArgs w;
-        w.n = n;
-        w.N = N;
-        w.dim = dim;
-        w.map = sp->d_map;
-        w.inc_off = inc_off;
-        w.inc = inc;
-        w.K = dK;
-        w.B = dB;
-        w.bdry = bdry_row;
-        w.g = dg;
-        w.scr_off = scr_off;
-        w.scols = scols;
-        w.svals = svals;
-        w.count = count;
-        w.rhs = A->d_rhs;
+        WalkArgs wk;
+        wk.n = n;
+        wk.N = N;
+        wk.dim = dim;
+        wk.map = sp->d_map;
+        wk.inc_off = w->inc_off;
+        wk.inc = w->inc;
+        wk.K = dK;
+        wk.B = dB;
+        wk.bdry = bdry_row;
+        wk.g = dg;
+        wk.scr_off = w->scr_off;
+        wk.scols = scols;
+        wk.svals = svals;
+        wk.count = count;
+        wk.rhs = A->d_rhs;
         // 64/G rows per wavefront, G the smallest of 16, 32, 64 holding
         // an element row
         const unsigned grid = (unsigned)std::min<int64_t>((n + 3) / 4, 1 << 16);
         if (M <= 16)
-            hipLaunchKernelGGL((k_row_walk_hash<16, 128>), dim3(grid), dim3(64), 0, s, w);
+            hipLaunchKernelGGL((k_row_walk_hash<16, 128>), dim3(grid), dim3(64), 0, s, wk);
         else if (M <= 32)
-            hipLaunchKernelGGL((k_row_walk_hash<32, 256>), dim3(grid), dim3(64), 0, s, w);
+            hipLaunchKernelGGL((k_row_walk_hash<32, 256>), dim3(grid), dim3(64), 0, s, wk);
         else   // M <= 60 for P1-P3, scalar or vector (check_problem)
-            hipLaunchKernelGGL((k_row_walk_hash<64, 512>), dim3(grid), dim3(64), 0, s, w);
+            hipLaunchKernelGGL((k_row_walk_hash<64, 512>), dim3(grid), dim3(64), 0, s, wk);
         PHG_HIP_CHECK(hipGetLastError());
     }
-    if ((rc = phg::dmalloc((void **)&rowptr64, (size_t)(n + 1) * sizeof(int64_t))) ||
+    tr.mark("row walk");
+    if ((rc = ws_buffer(sp, WS_ROWPTR, (size_t)(n + 1) * sizeof(int64_t),
+                        (void **)&rowptr64)) ||
         (rc = scan_i32(n, count, rowptr64, &nnz, s)))
         goto done;
     if (nnz >= (int64_t)INT32_MAX) {
@@ -1759,13 +1852,14 @@ extern "C" int phg_hip_assemble(phg_hip_space *sp, const phg_hip_problem *pb,
         (rc = phg::dmalloc((void **)&A->d_vals, (size_t)(nnz + 1) * sizeof(double))))
         goto done;
     hipLaunchKernelGGL(k_compact, dim3(nblk(n, RPB)), dim3(256), 0, s, n,
-                       (const int64_t *)scr_off, (const int32_t *)scols,
+                       (const int64_t *)w->scr_off, (const int32_t *)scols,
                        (const double *)svals, (const int64_t *)rowptr64,
                        A->d_cols, A->d_vals);
     hipLaunchKernelGGL(k_to_int32, dim3(nblk(n + 1, 256)), dim3(256), 0, s, n,
                        (const int64_t *)rowptr64, A->d_rowptr);
     PHG_HIP_CHECK(hipGetLastError());
     PHG_HIP_CHECK(hipStreamSynchronize(s));
+    tr.mark("row pointers, compaction");
     A->d_bdry_row = bdry_row;
     bdry_row = nullptr;
     *Aout = A;
@@ -1774,17 +1868,6 @@ extern "C" int phg_hip_assemble(phg_hip_space *sp, const phg_hip_problem *pb,
 done:
     if (A)
         phg_hip_mat_destroy(A);
-    phg::dfree(dK);
-    phg::dfree(dB);
-    phg::dfree(dg);
-    phg::dfree(cnt);
-    phg::dfree(inc);
-    phg::dfree(inc_off);
-    phg::dfree(count);
-    phg::dfree(scr_off);
-    phg::dfree(scols);
-    phg::dfree(svals);
-    phg::dfree(rowptr64);
     phg::dfree(bdry_row);
     return rc;
 }
@@ -1799,6 +1882,7 @@ extern "C" int phg_hip_mat_eliminate(phg_hip_mat *A)
     if (A->eliminated)
         return PHG_HIP_OK;
     hipStream_t s = phg::ctx()->stream;
+    phg::PhaseTrace tr("phg_hip_mat_eliminate");
     int64_t n = A->n, nnz = 0;
     int32_t *count = nullptr, *ncols = nullptr, *nrow32 = nullptr;
     int64_t *nrow = nullptr;
@@ -1842,9 +1926,15 @@ extern "C" int phg_hip_mat_eliminate(phg_hip_mat *A)
     A->eliminated = 1;
     phg::dfree(count);
     phg::dfree(nrow);
-    if ((rc = phg::build_row_blocks(A)))
-        return rc;
-    return phg::build_solver_copy(A);
+    tr.mark("boundary columns, lifting");
+    // the row blocks of the PHG-order CSR are split again at its first
+    // product
+    phg::dfree(A->d_rblk);
+    A->d_rblk = nullptr;
+    A->nrblk = 0;
+    rc = phg::build_solver_copy(A);
+    tr.mark("solver numbering and copy");
+    return rc;
 fail:
     phg::dfree(count);
     phg::dfree(nrow);
@@ -1998,8 +2088,7 @@ extern "C" int phg_hip_adjoint_logkappa(phg_hip_space *sp,
     hipStream_t s = phg::ctx()->stream;
     double *dk = nullptr, *Me = nullptr, *Jel = nullptr, *part = nullptr,
            *dgrad = nullptr;
-    int32_t *cnt = nullptr, *inc = nullptr;
-    int64_t *off = nullptr;
+    AsmWorkspace *wsp = nullptr;     // the space's incidence (scalar rows)
     phg_hip_vec *rhs = nullptr, *lam = nullptr;
     const int nparts = 256;
     std::vector<double> hp(nparts);
@@ -2010,7 +2099,7 @@ extern "C" int phg_hip_adjoint_logkappa(phg_hip_space *sp,
         (rc = phg::dmalloc((void **)&Jel, (size_t)ne * sizeof(double))) ||
         (rc = phg::dmalloc((void **)&dgrad, (size_t)ne * sizeof(double))) ||
         (rc = phg::dmalloc((void **)&part, (size_t)nparts * sizeof(double))) ||
-        (rc = build_incidence(sp, 1, &cnt, &off, &inc)) ||
+        (rc = space_incidence(sp, 1, &wsp)) ||
         (rc = phg_hip_vec_create(n, &rhs)))
         goto done;
     if (lambda == nullptr && (rc = phg_hip_vec_create(n, &lam)))
@@ -2038,7 +2127,7 @@ extern "C" int phg_hip_adjoint_logkappa(phg_hip_space *sp,
     hipLaunchKernelGGL(k_sum_blocks, dim3(nparts), dim3(256), 0, s, ne,
                        (const double *)Jel, part);
     hipLaunchKernelGGL(k_adj_rows, dim3(nblk(n, 256)), dim3(256), 0, s, n, N,
-                       (const int64_t *)off, (const int32_t *)inc,
+                       (const int64_t *)wsp->inc_off, (const int32_t *)wsp->inc,
                        (const uint8_t *)A->d_bdry_row, (const double *)Me,
                        rhs->d);
     PHG_HIP_CHECK(hipGetLastError());
@@ -2081,9 +2170,6 @@ done:
     phg::dfree(Jel);
     phg::dfree(dgrad);
     phg::dfree(part);
-    phg::dfree(cnt);
-    phg::dfree(off);
-    phg::dfree(inc);
     if (rhs)
         phg_hip_vec_destroy(rhs);
     if (lam)

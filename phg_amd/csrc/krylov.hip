@@ -34,6 +34,8 @@
 #include <functional>
 #include <memory>
 #include <type_traits>
+#include <atomic>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -3127,30 +3129,78 @@ int split_row_blocks_host(const int32_t *rp, int64_t n, int64_t nnz, int64_t cap
                           int64_t rcap, int32_t **d, int64_t *nb)
 {
     // greedy split into runs of consecutive rows holding at most cap entries
-    // and rcap rows; a longer row forms a block of its own
-    std::vector<int32_t> blk;
-    int rc;
-    blk.reserve((size_t)(nnz / cap + n / rcap + 2));
-    blk.push_back(0);
-    int64_t cur_nnz = 0, cur_rows = 0;
-    for (int64_t r = 0; r < n; r++) {
-        int64_t len = rp[(size_t)r + 1] - rp[(size_t)r];
-        if (len > cap) {
-            if (cur_rows > 0)
+    // and rcap rows; a longer row forms a block of its own.  The greedy runs
+    // on its own in fixed chunks of rows (a block never spans two chunks),
+    // so the chunks split on host threads and the result does not depend on
+    // their number; a row's sum in the kernels does not depend on the
+    // blocking at all.
+    constexpr int64_t CHUNK = 1 << 16;
+    const int64_t nch = std::max<int64_t>(1, (n + CHUNK - 1) / CHUNK);
+    std::vector<std::vector<int32_t>> part((size_t)nch);
+    auto split = [&](int64_t c) {
+        std::vector<int32_t> &blk = part[(size_t)c];
+        const int64_t r0 = c * CHUNK, r1 = std::min(n, r0 + CHUNK);
+        int64_t cur_nnz = 0, cur_rows = 0;
+        for (int64_t r = r0; r < r1; r++) {
+            int64_t len = rp[(size_t)r + 1] - rp[(size_t)r];
+            if (len > cap) {
+                if (cur_rows > 0)
+                    blk.push_back((int32_t)r);
+                blk.push_back((int32_t)(r + 1));
+                cur_nnz = cur_rows = 0;
+                continue;
+            }
+            if (cur_rows > 0 && (cur_nnz + len > cap || cur_rows == rcap)) {
                 blk.push_back((int32_t)r);
-            blk.push_back((int32_t)(r + 1));
-            cur_nnz = cur_rows = 0;
-            continue;
+                cur_nnz = cur_rows = 0;
+            }
+            cur_nnz += len;
+            cur_rows++;
         }
-        if (cur_rows > 0 && (cur_nnz + len > cap || cur_rows == rcap)) {
-            blk.push_back((int32_t)r);
-            cur_nnz = cur_rows = 0;
+        if (cur_rows > 0)
+            blk.push_back((int32_t)r1);
+    };
+    try {
+        const int nt = (int)std::min<int64_t>(nch, std::max(1u, std::thread::hardware_concurrency()));
+        std::vector<std::thread> th;
+        std::atomic<int64_t> next{0};
+        std::atomic<bool> failed{false};
+        auto work = [&]() {
+            try {
+                for (int64_t c; (c = next++) < nch;)
+                    split(c);
+            }
+            catch (...) {
+                failed = true;
+            }
+        };
+        for (int t = 1; t < std::min(nt, 16); t++) {
+            try {
+                th.emplace_back(work);
+            }
+            catch (...) {
+                break;
+            }
         }
-        cur_nnz += len;
-        cur_rows++;
+        work();
+        for (auto &x : th)
+            x.join();
+        if (failed)
+            throw std::bad_alloc();
     }
-    if (cur_rows > 0)
-        blk.push_back((int32_t)n);
+    catch (const std::bad_alloc &) {
+        return set_error(PHG_HIP_E_NOMEM, "row-block split: out of host memory");
+    }
+    std::vector<int32_t> blk;
+    size_t total = 1;
+    for (const auto &v : part)
+        total += v.size();
+    blk.reserve(total);
+    blk.push_back(0);
+    for (const auto &v : part)
+        blk.insert(blk.end(), v.begin(), v.end());
+    (void)nnz;
+    int rc;
     dfree(*d);
     *d = nullptr;
     *nb = (int64_t)blk.size() - 1;
@@ -3303,6 +3353,12 @@ void free_workspace(phg_hip_mat *A)
 int spmv64(const phg_hip_mat *A, const double *x, double *y, double alpha,
            double beta, hipStream_t s)
 {
+    // the row blocks of the PHG-order CSR are built on the first product
+    if (A->d_rblk == nullptr && A->n > 0) {
+        int rc = build_row_blocks(const_cast<phg_hip_mat *>(A));
+        if (rc)
+            return rc;
+    }
     if (A->nrblk == 0)
         return PHG_HIP_OK;
     hipLaunchKernelGGL(k_spmv64, dim3(grid_spmv<CsrView>(A, k_spmv64)), dim3(BS), 0, s,
@@ -3319,6 +3375,16 @@ namespace {
 int ensure_format(phg_hip_mat *A, int fmt, int need32)
 {
     int rc;
+    // the 4096-entry and wavefront row blocks of the solver copy are split
+    // when a layout first uses them
+    if ((fmt == FMT_B4K || fmt == FMT_B4K_PAR) && A->q_rblk1k == nullptr &&
+        (rc = phg::split_row_blocks(A->q_rowptr, A->n, A->nnz, phg::SPMV_LARGE_NNZ,
+                                    phg::SPMV_BLOCK_ROWS, &A->q_rblk1k, &A->q_nrblk1k)))
+        return rc;
+    if (fmt == FMT_WAVE && A->q_wblk == nullptr &&
+        (rc = phg::split_row_blocks(A->q_rowptr, A->n, A->nnz, phg::SPMV_WAVE_NNZ,
+                                    phg::SPMV_WAVE_ROWS, &A->q_wblk, &A->q_nwblk)))
+        return rc;
     if (fmt == FMT_SELL && A->d_svals == nullptr && (rc = phg::build_sell(A)))
         return rc;
     if (fmt == FMT_DICT || fmt == FMT_DICT_NP) {
@@ -3830,6 +3896,8 @@ extern "C" int phg_hip_mat_tune(phg_hip_mat *A, int32_t reps)
             A->tune_us[f][0] = A->tune_us[f][1] = -1.0;   // not representable
             continue;
         }
+        if ((rc = ensure_format(A, f, 1)))
+            return rc;
         rc = with_format(f, [&](auto tag) {
             using View = decltype(tag);
             using F = FmtData<View>;

@@ -53,6 +53,18 @@ int download(void *h, const void *d, size_t bytes);
 // (assemble.hip), taken by phg_hip_init with the device current
 int noalloc_init(int device);
 
+// Phase timer of the set-up paths (assembly, elimination, solver copy):
+// with PHG_HIP_SETUP_TRACE set in the environment, mark(what) waits for the
+// library stream and prints the time since the previous mark to stderr;
+// otherwise it does nothing.
+struct PhaseTrace {
+    explicit PhaseTrace(const char *scope);
+    void mark(const char *what);
+    const char *scope;
+    bool on;
+    double t_last;
+};
+
 }  // namespace phg
 
 // Element-block layout of the matrix-free operator (eop.hip, see EopData
@@ -76,8 +88,16 @@ struct EopLayout {
     ~EopLayout();
 };
 
+// Assembly workspace of a space (assemble.hip): the row incidence of the
+// last problem dimension assembled (a function of the element maps alone)
+// and the scratch buffers of phg_hip_assemble, kept between assemblies and
+// freed with the space.
+struct AsmWorkspace;
+void free_asm_workspace(AsmWorkspace *w);
+
 // opaque handles -----------------------------------------------------------
 struct phg_hip_space {
+    AsmWorkspace *asm_ws = nullptr;
     int32_t order, nbas;
     int64_t nvert, nelem, ndof;
     uint64_t uid = 0;                // unique per space of the process

@@ -140,6 +140,7 @@ int build_solver_copy(phg_hip_mat *A)
     hipStream_t s = ctx()->stream;
     const int64_t n = A->n;
     int rc;
+    PhaseTrace tr("build_solver_copy");
     free_solver_copy(A);
     if (!A->reorder || A->d_xyz == nullptr || A->ndof * A->dim != n) {
         // solve in PHG's numbering
@@ -203,6 +204,7 @@ int build_solver_copy(phg_hip_mat *A)
             cleanup();
             return rc ? rc : set_error(PHG_HIP_E_HIP, "radix sort failed");
         }
+        tr.mark("Morton keys, sort");
         if ((rc = dmalloc((void **)&A->q_perm, (size_t)n * 4)) ||
             (rc = dmalloc((void **)&iperm, (size_t)n * 4)) ||
             (rc = dmalloc((void **)&len, (size_t)(n + 1) * 4)) ||
@@ -245,18 +247,17 @@ int build_solver_copy(phg_hip_mat *A)
                            (const int64_t *)qrp64, A->q_rowptr);
         PHG_HIP_CHECK(hipGetLastError());
         PHG_HIP_CHECK(hipStreamSynchronize(s));
+        tr.mark("renumbered copy");
         cleanup();
+        tr.mark("frees");
     }
-    // the three row-block splits of the solver copy from one download
-    std::vector<int32_t> qrp((size_t)n + 1);
-    if ((rc = download(qrp.data(), A->q_rowptr, qrp.size() * sizeof(int32_t))) ||
-        (rc = split_row_blocks_host(qrp.data(), n, A->nnz, SPMV_BLOCK_NNZ,
-                                    SPMV_BLOCK_ROWS, &A->q_rblk, &A->q_nrblk)) ||
-        (rc = split_row_blocks_host(qrp.data(), n, A->nnz, SPMV_LARGE_NNZ,
-                                    SPMV_BLOCK_ROWS, &A->q_rblk1k, &A->q_nrblk1k)) ||
-        (rc = split_row_blocks_host(qrp.data(), n, A->nnz, SPMV_WAVE_NNZ,
-                                    SPMV_WAVE_ROWS, &A->q_wblk, &A->q_nwblk)))
+    // the 2048-entry row blocks of the solver copy (the FP64 residuals of
+    // the mixed solvers and the B2K layouts use them); the 4096-entry and
+    // wavefront blocks are split when a layout first needs them
+    if ((rc = split_row_blocks(A->q_rowptr, n, A->nnz, SPMV_BLOCK_NNZ, SPMV_BLOCK_ROWS,
+                               &A->q_rblk, &A->q_nrblk)))
         return rc;
+    tr.mark("row-block split");
     return PHG_HIP_OK;
 }
 

@@ -4,7 +4,13 @@
 
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <thread>
+#include <vector>
 
 namespace phg {
 
@@ -55,14 +61,84 @@ void dfree(void *p)
         (void)hipFree(p);
 }
 
+// Host arrays of several MB (load vectors, Dirichlet values, coefficients
+// as PHG hands them over) go through a pinned staging buffer of the
+// library: host threads copy a piece into it, then one DMA moves the piece;
+// a pageable copy of the same size runs at a fraction of the link's rate.
+constexpr size_t STAGE_BYTES = size_t(64) << 20;
+constexpr size_t STAGE_MIN = size_t(4) << 20;
+static void *g_stage = nullptr;
+
+static void parallel_memcpy(void *dst, const void *src, size_t bytes)
+{
+    const int nt = (int)std::min<size_t>(8, std::max<size_t>(1, bytes / (size_t(1) << 20)));
+    std::vector<std::thread> th;
+    auto piece = [&](int t) {
+        const size_t lo = bytes * t / nt, hi = bytes * (t + 1) / nt;
+        memcpy((char *)dst + lo, (const char *)src + lo, hi - lo);
+    };
+    for (int t = 1; t < nt; t++) {
+        try {
+            th.emplace_back(piece, t);
+        }
+        catch (...) {
+            piece(t);
+        }
+    }
+    piece(0);
+    for (auto &x : th)
+        x.join();
+}
+
 int upload(void *d, const void *h, size_t bytes)
 {
     if (bytes == 0)
         return PHG_HIP_OK;
-    PHG_HIP_CHECK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice,
-                                 g_ctx.stream));
-    PHG_HIP_CHECK(hipStreamSynchronize(g_ctx.stream));
+    if (bytes >= STAGE_MIN && g_stage == nullptr &&
+        hipHostMalloc(&g_stage, STAGE_BYTES, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        g_stage = nullptr;
+    }
+    if (bytes < STAGE_MIN || g_stage == nullptr) {
+        PHG_HIP_CHECK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice,
+                                     g_ctx.stream));
+        PHG_HIP_CHECK(hipStreamSynchronize(g_ctx.stream));
+        return PHG_HIP_OK;
+    }
+    for (size_t off = 0; off < bytes; off += STAGE_BYTES) {
+        const size_t len = std::min(STAGE_BYTES, bytes - off);
+        parallel_memcpy(g_stage, (const char *)h + off, len);
+        PHG_HIP_CHECK(hipMemcpyAsync((char *)d + off, g_stage, len,
+                                     hipMemcpyHostToDevice, g_ctx.stream));
+        PHG_HIP_CHECK(hipStreamSynchronize(g_ctx.stream));
+    }
     return PHG_HIP_OK;
+}
+
+static double now_ms()
+{
+    return std::chrono::duration<double, std::milli>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+PhaseTrace::PhaseTrace(const char *sc)
+    : scope(sc), on(getenv("PHG_HIP_SETUP_TRACE") != nullptr), t_last(0.0)
+{
+    if (on) {
+        (void)hipStreamSynchronize(g_ctx.stream);
+        t_last = now_ms();
+    }
+}
+
+void PhaseTrace::mark(const char *what)
+{
+    if (!on)
+        return;
+    (void)hipStreamSynchronize(g_ctx.stream);
+    const double t = now_ms();
+    fprintf(stderr, "%s: %-28s %8.2f ms\n", scope, what, t - t_last);
+    t_last = t;
 }
 
 int download(void *h, const void *d, size_t bytes)
@@ -140,6 +216,9 @@ extern "C" int phg_hip_finalize(void)
         (void)hipStreamDestroy(g_ctx.aux[i]);
     }
     (void)hipHostFree(g_ctx.host_mailbox);
+    if (g_stage != nullptr)
+        (void)hipHostFree(g_stage);
+    g_stage = nullptr;
     (void)hipStreamDestroy(g_ctx.stream);
     g_ctx = Context();
     g_ready = false;
