@@ -191,7 +191,7 @@ int phg_hip_bicgstab(const phg_hip_mat *A, const phg_hip_vec *b,
  * augmented by the corrections of the last augk cycles) and the Jacobi
  * PC.  restart and augk are -gmres_restart (PHG default 10) and
  * -lgmres_augk (default 2); restart <= 0 means maxit + 1 and augk <= 0
- * means 2 as in the reference, and restart + augk must stay below 40.
+ * means 2 as in the reference, and restart + augk must stay below 4096 (augk at most 64).
  * Stopping rule as phg_hip_pcg on the Givens residual estimate.  As in the
  * reference the residual reported is |r0| unless the tolerance was
  * reached, and b = 0 gives x = 0 and residual 1.  Returns the number of
@@ -199,6 +199,33 @@ int phg_hip_bicgstab(const phg_hip_mat *A, const phg_hip_vec *b,
 int phg_hip_gmres(const phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x,
                   int32_t restart, int32_t augk, int32_t maxit, double rtol,
                   double btol, double atol, double *residual);
+
+/* -gmres_type values, in the order of solver-phg.c:71-77 gmres_names */
+enum {
+    PHG_HIP_GMRES_L = 0,    /* solver-phg.c:783  left preconditioned */
+    PHG_HIP_GMRES_R = 1,    /* solver-phg.c:1051 right preconditioned */
+    PHG_HIP_FGMRES_R = 2,   /* solver-phg.c:1263 right, flexible */
+    PHG_HIP_LGMRES_L = 3,   /* solver-phg.c:1481 left, augmented */
+    PHG_HIP_LGMRES_R = 4,   /* solver-phg.c:1801 right, augmented */
+    PHG_HIP_FLGMRES_R = 5   /* solver-phg.c:2063 right, flexible, augmented */
+};
+
+/* solver-phg.c:2484 GMRES_Solve with any -gmres_type (gmres_funcs,
+ * solver-phg.c:74) and the Jacobi PC; phg_hip_gmres is gmres_type
+ * PHG_HIP_FLGMRES_R.  The plain variants ignore augk.  Left variants work
+ * on z = M^-1 (b - A x): when the Givens estimate |z| meets the tolerance
+ * (taken from the unpreconditioned |r0|, |b|) they compute b - A x; if that
+ * is still above it they restart and run the next cycle until the estimate
+ * has dropped by the ratio of the two (solver-phg.c:904-908, :979-1003),
+ * and they report the last true residual.  Right variants report the
+ * Givens estimate.  Both report |r0| when no check was made.  A zero
+ * diagonal is PHG_HIP_E_ZERO_DIAG where jacobi_pc would first run: at once
+ * for the left variants, in the first cycle for gmres_r and lgmres_r, at
+ * the first Arnoldi step for the flexible ones. */
+int phg_hip_gmres_type(const phg_hip_mat *A, const phg_hip_vec *b,
+                       phg_hip_vec *x, int32_t gmres_type, int32_t restart,
+                       int32_t augk, int32_t maxit, double rtol, double btol,
+                       double atol, double *residual);
 
 /* Mixed precision (no reference counterpart): FP64 iterative refinement
  * r = b - A x in FP64, correction from Jacobi-PCG in FP32 arithmetic on an

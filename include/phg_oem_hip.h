@@ -244,6 +244,12 @@ extern OEM_SOLVER phgSolverHipGMRES_;
 int phgSolverHipSetOptions(SOLVER *solver, int mixed, int restart, int augk,
                            int tune);
 
+/* The role of -gmres_type (solver-phg.c:458 GMRESRegisterOptions keyword
+ * list gmres_names, solver-phg.c:71) for a hip_gmres solver: one of
+ * "gmres_l", "gmres_r", "fgmres_r", "lgmres_l", "lgmres_r", "flgmres_r"
+ * (the default).  Returns 0, or -1 for another solver or an unknown name. */
+int phgSolverHipSetGMRESType(SOLVER *solver, const char *name);
+
 /* MV_FUNC (matvec.h:60) for a matrix-free MAT made by phgMatCreateHip:
  * y = A x (MAT_OP_N; MAT_OP_T for a symmetric device matrix; MAT_OP_D: the
  * diagonal) with x and y host vectors of PHG; the product runs on the GPU in

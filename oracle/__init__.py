@@ -75,8 +75,8 @@ _sig("orc_pcg", C.c_int, [_P(_Csr), _P(_dbl), _P(_dbl), C.c_int, _dbl, _dbl, _db
                           _P(_dbl)])
 _sig("orc_bicgstab", C.c_int, [_P(_Csr), _P(_dbl), _P(_dbl), C.c_int, _dbl, _dbl, _dbl,
                                _P(_dbl)])
-_sig("orc_flgmres", C.c_int, [_P(_Csr), _P(_dbl), _P(_dbl), C.c_int, C.c_int, C.c_int, _dbl,
-                              _dbl, _dbl, _P(_dbl)])
+_sig("orc_gmres", C.c_int, [C.c_int, _P(_Csr), _P(_dbl), _P(_dbl), C.c_int, C.c_int, C.c_int,
+                            _dbl, _dbl, _dbl, _P(_dbl)])
 _sig("orc_matvec", None, [_P(_Csr), _P(_dbl), _P(_dbl)])
 _sig("orc_csr_free", None, [_P(_Csr)])
 _sig("orc_mass_apply", _dbl, [C.c_int, _i64, _P(_dbl), _P(_i32), _P(_i32), _i64,
@@ -225,15 +225,19 @@ def bicgstab(A: Csr, b, x0=None, maxit=10000, rtol=1e-10, btol=1e-10, atol=0.0):
     return x, nits, res.value
 
 
+GMRES_TYPES = ("gmres_l", "gmres_r", "fgmres_r", "lgmres_l", "lgmres_r", "flgmres_r")
+
+
 def gmres(A: Csr, b, x0=None, restart=10, augk=2, maxit=10000, rtol=1e-10, btol=1e-10,
-          atol=0.0):
-    """solver-phg.c:2063 solver_FLGMRES_r (PHG's default GMRES, Jacobi PC)."""
+          atol=0.0, gmres_type="flgmres_r"):
+    """solver-phg.c:59-77 gmres_funcs with the Jacobi PC; gmres_type is one of
+    GMRES_TYPES (PHG's -gmres_type names, default flgmres_r)."""
     b = np.ascontiguousarray(b, dtype=np.float64)
     x = np.zeros(A.n) if x0 is None else np.array(x0, dtype=np.float64)
     res = _dbl()
-    p = A._p if A._p is not None else C.byref(A._s)
-    nits = lib.orc_flgmres(p, _ptr(b, _dbl), _ptr(x, _dbl), restart, augk, maxit, rtol,
-                           btol, atol, C.byref(res))
+    nits = lib.orc_gmres(GMRES_TYPES.index(gmres_type), _arg(A), _ptr(b, _dbl),
+                         _ptr(x, _dbl), restart, augk, maxit, rtol, btol, atol,
+                         C.byref(res))
     return x, nits, res.value
 
 

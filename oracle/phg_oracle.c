@@ -925,21 +925,53 @@ static void amatvec(const orc_csr *A, double alpha, const double *x,
     free(data);
 }
 
-int orc_flgmres(const orc_csr *A, const double *b, double *x, int restart,
-                int augk, int maxit, double rtol, double btol, double atol,
-                double *residual)
+/* The GMRES family of solver-phg.c:59-77 (gmres_names / gmres_funcs) with
+ * jacobi_pc (solver-phg.c:207), one restatement for the six variants:
+ *   type 0 gmres_l   solver-phg.c:783   left preconditioned
+ *   type 1 gmres_r   solver-phg.c:1051  right preconditioned
+ *   type 2 fgmres_r  solver-phg.c:1263  right, flexible (keeps M^-1 v_i)
+ *   type 3 lgmres_l  solver-phg.c:1481  left, augmented
+ *   type 4 lgmres_r  solver-phg.c:1801  right, augmented
+ *   type 5 flgmres_r solver-phg.c:2063  right, flexible, augmented
+ * The augmented ("L") variants add the corrections of the last augk
+ * cycles to the Krylov space (cycle c has restart + min(c, augk) steps).
+ * Left variants run on z = M^-1 (b - A x); when the Givens estimate |z|
+ * reaches the tolerance they check the true residual and, if it is still
+ * above it, continue the cycle until the estimate has dropped by the ratio
+ * of the two (false convergence, solver-phg.c:904-908, :979-1003); they
+ * report the last true residual.  Right variants report the Givens
+ * estimate once it reaches the tolerance.  Either reports |r0| when no
+ * tolerance check was made. */
+enum { GM_L = 0, GM_R, GM_FR, GM_LL, GM_LR, GM_FLR };
+
+int orc_gmres(int type, const orc_csr *A, const double *b, double *x, int restart,
+              int augk, int maxit, double rtol, double btol, double atol,
+              double *residual)
 {
-    /* solver-phg.c:2063-2340 solver_FLGMRES_r (right-preconditioned,
-     * flexible, augmented by the last augk corrections) with jacobi_pc */
     int64_t n = A->n;
     size_t sz = (size_t)n * sizeof(double);
-    int nits = 0, i, j, k, itr_out, mk, zi, m;
-    double res, r0, r_norm, b_norm, tol, tmp, h1, h2, x1 = 0.0;
-    double *diag = xmalloc(sz), *z = xmalloc(sz), *rv = xmalloc(sz);
+    const int left = type == GM_L || type == GM_LL;
+    const int flex = type == GM_FR || type == GM_FLR;
+    const int aug = type >= GM_LL;
+    int nits = 0, i, j, k, itr_out = 0, mk, m, rs;
+    int break_flag = 0, false_conv = 0;
+    double res, r_norm, b_norm, tol, tmp, h1, h2, x1 = 0.0, s0 = 1.0;
+    if (type < GM_L || type > GM_FLR)
+        return -2;
     if (restart <= 0)
         restart = maxit + 1;
-    if (augk <= 0)
+    if (!aug)
+        augk = 0;                   /* Unused(augk) in the plain variants */
+    else if (augk <= 0)
         augk = 2;
+    b_norm = norm2(n, b);
+    if (fabs(b_norm) < 10.0 * DBL_MIN) {
+        memset(x, 0, sz);
+        if (residual)
+            *residual = 1.0;        /* GMRES_Solve's initial value */
+        return 0;
+    }
+    double *diag = xmalloc(sz), *z = xmalloc(sz), *t = xmalloc(sz);
     for (int64_t q = 0; q < n; q++) {
         diag[q] = 0.0;
         for (int64_t jj = A->rowptr[q]; jj < A->rowptr[q + 1]; jj++)
@@ -948,69 +980,80 @@ int orc_flgmres(const orc_csr *A, const double *b, double *x, int restart,
                 break;
             }
     }
-    b_norm = norm2(n, b);
-    if (fabs(b_norm) < 10.0 * DBL_MIN) {
-        memset(x, 0, sz);
-        if (residual)
-            *residual = 1.0;        /* GMRES_Solve's initial value */
-        free(diag); free(z); free(rv);
-        return 0;
-    }
     mk = restart;
     m = restart + augk;             /* largest subspace */
     double **v = xmalloc((size_t)(m + 2) * sizeof(double *));
     double **flv = xmalloc((size_t)(m + 2) * sizeof(double *));
-    double **zp = xmalloc((size_t)augk * sizeof(double *));
+    double **zp = xmalloc((size_t)(augk + 1) * sizeof(double *));
     for (i = 0; i < m + 2; i++) {
         v[i] = xmalloc(sz);
-        flv[i] = xmalloc(sz);
+        flv[i] = flex ? xmalloc(sz) : NULL;
     }
     for (i = 0; i < augk; i++)
         zp[i] = xmalloc(sz);
     double *H = xmalloc((size_t)(m + 2) * (m + 2) * sizeof(double));
 #define HH(a, c) H[(size_t)(a) * (m + 2) + (c)]
+    /* y persists over cycles and starts at zero: the augmented update
+     * reads y[mk+1..mk+ak] even when the cycle stopped before step mk+ak
+     * (solver-phg.c:1679-1683), i.e. values of an earlier cycle */
     double *y = xmalloc((size_t)(m + 2) * 4 * sizeof(double));
+    memset(y, 0, (size_t)(m + 2) * 4 * sizeof(double));
     double *sv = y + m + 2, *c1 = y + 2 * (m + 2), *c2 = y + 3 * (m + 2);
 
-    memcpy(z, b, sz);
+    /* r (left: the vector t, right: z) = b - A x, skipped for x = 0 */
+    double *r0v = left ? t : z;
+    memcpy(r0v, b, sz);
     for (int64_t q = 0; q < n; q++)
         x1 += fabs(x[q]);
     if (x1 != 0.0)
-        amatvec(A, -1.0, x, 1.0, z);
-    r0 = r_norm = res = norm2(n, z);
-    tol = r0 * rtol;
+        amatvec(A, -1.0, x, 1.0, r0v);
+    r_norm = res = norm2(n, r0v);
+    tol = r_norm * rtol;
     if (tol < b_norm * btol)
         tol = b_norm * btol;
     if (tol < atol)
         tol = atol;
-    (void)r0;
+    if (left && jacobi(n, diag, t, z)) {
+        nits = -1;
+        goto done;
+    }
     sv[0] = 1.0;
-    itr_out = 0;
     while (nits < maxit) {
-        restart = itr_out <= augk ? mk + itr_out : mk + augk;
+        rs = itr_out <= augk ? mk + itr_out : mk + augk;
         sv[1] = r_norm = norm2(n, z);
         vaxpby(n, 1.0 / r_norm, z, 0.0, v[1]);
         i = 0;
         memset(H, 0, (size_t)(m + 2) * (m + 2) * sizeof(double));
-        while (i < restart && r_norm > tol && nits < maxit) {
+        while (i < rs && (r_norm > tol || false_conv) && nits < maxit) {
             i++, nits++;
-            double *v1 = v[i + 1];
-            if (jacobi(n, diag, i <= mk ? v[i] : zp[i - mk - 1], rv)) {
-                nits = -1;
-                goto done;
+            double *src = i <= mk ? v[i] : zp[i - mk - 1], *w = v[i + 1];
+            if (left) {                         /* w = M^-1 A src */
+                amatvec(A, 1.0, src, 0.0, t);
+                if (jacobi(n, diag, t, w)) {
+                    nits = -1;
+                    goto done;
+                }
             }
-            amatvec(A, 1.0, rv, 0.0, v1);
-            memcpy(flv[i], rv, sz);
+            else {                              /* w = A M^-1 src */
+                if (jacobi(n, diag, src, t)) {
+                    nits = -1;
+                    goto done;
+                }
+                amatvec(A, 1.0, t, 0.0, w);
+                if (flex)
+                    memcpy(flv[i], t, sz);
+            }
             for (k = 1; k <= i; k++) {          /* modified Gram-Schmidt */
-                HH(i, k) = dot(n, v1, v[k]);
-                vaxpby(n, -HH(i, k), v[k], 1.0, v1);
+                HH(i, k) = dot(n, w, v[k]);
+                vaxpby(n, -HH(i, k), v[k], 1.0, w);
             }
-            HH(i, i + 1) = norm2(n, v1);
+            HH(i, i + 1) = norm2(n, w);
             if (fabs(HH(i, i + 1)) < 10.0 * DBL_MIN) {
                 i--;                            /* happy breakdown */
+                break_flag = 1;
                 break;
             }
-            vaxpby(n, 0.0, NULL, 1.0 / HH(i, i + 1), v1);
+            vaxpby(n, 0.0, NULL, 1.0 / HH(i, i + 1), w);
             for (j = 1; j < i; j++) {           /* previous rotations */
                 h1 = c1[j] * HH(i, j) + c2[j] * HH(i, j + 1);
                 h2 = -c2[j] * HH(i, j) + c1[j] * HH(i, j + 1);
@@ -1026,7 +1069,11 @@ int orc_flgmres(const orc_csr *A, const double *b, double *x, int restart,
             sv[i] = c1[i] * sv[i];
             HH(i, i) = c1[i] * HH(i, i) + c2[i] * HH(i, i + 1);
             r_norm = fabs(sv[i + 1]);
-            if (r_norm <= tol)
+            if (left) {
+                if (false_conv && r_norm * s0 <= tol)
+                    false_conv = 0;
+            }
+            else if (r_norm <= tol)
                 break;
         }
         for (j = i; j > 0; j--) {               /* H y = s */
@@ -1034,13 +1081,18 @@ int orc_flgmres(const orc_csr *A, const double *b, double *x, int restart,
             for (k = j - 1; k > 0; k--)
                 sv[k] -= HH(j, k) * y[j];
         }
-        if (i > 0) {
-            vaxpby(n, y[1], flv[1], 0.0, z);
-            for (j = 2; j <= i; j++)
-                vaxpby(n, y[j], flv[j], 1.0, z);
+        /* flexible: x += sum y_j flv_j (x += z after a cycle of no step) */
+        if (flex) {
+            if (i > 0) {
+                vaxpby(n, y[1], flv[1], 0.0, z);
+                for (j = 2; j <= i; j++)
+                    vaxpby(n, y[j], flv[j], 1.0, z);
+            }
+            vaxpby(n, 1.0, z, 1.0, x);
         }
-        vaxpby(n, 1.0, z, 1.0, x);
-        if (i > 0) {
+        /* z = V y plus the augmentation part: the correction (left) or its
+         * preimage under M^-1 (right), kept as augmentation vector */
+        if ((!flex || aug) && i > 0) {
             vaxpby(n, y[1], v[1], 0.0, z);
             if (i <= mk) {
                 for (j = 2; j <= i; j++)
@@ -1054,14 +1106,43 @@ int orc_flgmres(const orc_csr *A, const double *b, double *x, int restart,
                     vaxpby(n, y[j + mk], zp[j - 1], 1.0, z);
             }
         }
-        zi = itr_out % augk;
-        memcpy(zp[zi], z, sz);
-        if (r_norm <= tol) {
-            res = r_norm;
-            break;
+        if (left)
+            vaxpby(n, 1.0, z, 1.0, x);
+        else if (!flex) {
+            if (jacobi(n, diag, z, t)) {
+                nits = -1;
+                goto done;
+            }
+            vaxpby(n, 1.0, t, 1.0, x);
         }
-        memcpy(z, b, sz);
-        amatvec(A, -1.0, x, 1.0, z);
+        if (aug)
+            memcpy(zp[itr_out % augk], z, sz);
+        if (!left) {
+            if (r_norm <= tol) {
+                res = r_norm;
+                break;
+            }
+            memcpy(z, b, sz);
+            amatvec(A, -1.0, x, 1.0, z);
+        }
+        else {
+            false_conv = 0;
+            memcpy(t, b, sz);
+            amatvec(A, -1.0, x, 1.0, t);
+            if (r_norm <= tol) {
+                res = norm2(n, t);
+                if (res <= tol || break_flag)
+                    break;
+                if (i < rs) {
+                    false_conv = 1;
+                    s0 = res / r_norm;
+                }
+            }
+            if (jacobi(n, diag, t, z)) {
+                nits = -1;
+                goto done;
+            }
+        }
         itr_out++;
     }
 done:
@@ -1075,7 +1156,7 @@ done:
     for (i = 0; i < augk; i++)
         free(zp[i]);
     free(v); free(flv); free(zp); free(H); free(y);
-    free(diag); free(z); free(rv);
+    free(diag); free(z); free(t);
     return nits;
 }
 

@@ -92,14 +92,15 @@ int orc_pcg(const orc_csr *A, const double *b, double *x, int maxit,
 int orc_bicgstab(const orc_csr *A, const double *b, double *x, int maxit,
                  double rtol, double btol, double atol, double *residual);
 
-/* solver-phg.c:2063 solver_FLGMRES_r (PHG's default GMRES: right
- * preconditioned, flexible, augmented by augk previous corrections) with
- * jacobi_pc; restart <= 0 means maxit + 1, augk <= 0 means 2.  The
- * residual reported is |r0| unless the tolerance was reached, as in the
- * reference.  Returns nits (<0: zero diagonal). */
-int orc_flgmres(const orc_csr *A, const double *b, double *x, int restart,
-                int augk, int maxit, double rtol, double btol, double atol,
-                double *residual);
+/* solver-phg.c:59-77 gmres_funcs with jacobi_pc: type 0 gmres_l, 1 gmres_r,
+ * 2 fgmres_r, 3 lgmres_l, 4 lgmres_r, 5 flgmres_r (PHG's default).
+ * restart <= 0 means maxit + 1; augk (augmented types only) <= 0 means 2.
+ * Right variants report the Givens estimate once it reached the tolerance,
+ * left ones the last true residual, both |r0| when no check was made.
+ * Returns nits (-1: zero diagonal, -2: bad type). */
+int orc_gmres(int type, const orc_csr *A, const double *b, double *x, int restart,
+              int augk, int maxit, double rtol, double btol, double atol,
+              double *residual);
 
 /* y = A x (matvec.c:887-930, row sums in storage order). */
 void orc_matvec(const orc_csr *A, const double *x, double *y);

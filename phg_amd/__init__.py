@@ -135,6 +135,8 @@ _sig(hip_lib, "phg_hip_bicgstab", C.c_int,
      [_vp, _vp, _vp, _i32, _dbl, _dbl, _dbl, _P(_dbl)])
 _sig(hip_lib, "phg_hip_gmres", C.c_int,
      [_vp, _vp, _vp, _i32, _i32, _i32, _dbl, _dbl, _dbl, _P(_dbl)])
+_sig(hip_lib, "phg_hip_gmres_type", C.c_int,
+     [_vp, _vp, _vp, _i32, _i32, _i32, _i32, _dbl, _dbl, _dbl, _P(_dbl)])
 _sig(hip_lib, "phg_hip_pcg_mixed", C.c_int,
      [_vp, _vp, _vp, _i32, _dbl, _dbl, _dbl, _dbl, _P(_i32), _P(_dbl)])
 _sig(hip_lib, "phg_hip_pcg_reliable", C.c_int,
@@ -641,13 +643,19 @@ def bicgstab(A: Mat, b: Vec, x: Vec, maxit=10000, rtol=1e-10, btol=1e-10, atol=0
     return nits, res.value
 
 
+GMRES_TYPES = ("gmres_l", "gmres_r", "fgmres_r", "lgmres_l", "lgmres_r", "flgmres_r")
+
+
 def gmres(A: Mat, b: Vec, x: Vec, restart=10, augk=2, maxit=10000, rtol=1e-10, btol=1e-10,
-          atol=0.0):
-    """FP64 Jacobi-FLGMRES (solver-phg.c:2063, PHG's default GMRES).
-    Returns (nits, residual)."""
+          atol=0.0, gmres_type="flgmres_r"):
+    """FP64 Jacobi-GMRES of the -gmres_type named (solver-phg.c:71-77; PHG's
+    default flgmres_r, solver-phg.c:2063).  Returns (nits, residual)."""
     res = _dbl()
-    nits = _check(hip_lib.phg_hip_gmres(A._h, b._h, x._h, restart, augk, maxit, rtol, btol,
-                                        atol, C.byref(res)), "gmres")
+    if gmres_type not in GMRES_TYPES:
+        raise ValueError(f"gmres_type must be one of {GMRES_TYPES}")
+    nits = _check(hip_lib.phg_hip_gmres_type(A._h, b._h, x._h, GMRES_TYPES.index(gmres_type),
+                                             restart, augk, maxit, rtol, btol, atol,
+                                             C.byref(res)), "gmres")
     return nits, res.value
 
 

@@ -1479,10 +1479,13 @@ __global__ void __launch_bounds__(BS) k_bi_true_res(View mv, const double *va,
 }
 
 //------------------------------------------------------------------- GMRES
-// solver-phg.c:2063 solver_FLGMRES_r (PHG's default GMRES) with jacobi_pc:
-// the vector work of one Arnoldi step runs on the device --
-//   k_gm_pc        flv_i = src / diag (src = v_i or an augmentation vector)
-//   k_gm_spmv      v_{i+1} = A flv_i, partials of (v_{i+1}, v_1)
+// The GMRES family of solver-phg.c:59-77 (gmres_l, gmres_r, fgmres_r,
+// lgmres_l, lgmres_r, flgmres_r) with jacobi_pc: the vector work of one
+// Arnoldi step runs on the device --
+//   k_gm_pc        flv_i = src / diag (right variants: src = v_i or an
+//                  augmentation vector)
+//   k_gm_spmv      v_{i+1} = A flv_i (right) or (A src) / diag (left),
+//                  partials of (v_{i+1}, v_1)
 //   k_gm_mgs       modified Gram-Schmidt step k: h_k from the partials,
 //                  v_{i+1} -= h_k v_k, partials of (v_{i+1}, v_{k+1}) or of
 //                  |v_{i+1}|^2 after the last k
@@ -1491,7 +1494,7 @@ __global__ void __launch_bounds__(BS) k_bi_true_res(View mv, const double *va,
 // -- and the host reads the column h_1..h_{i+1} and applies the Givens
 // rotations, least-squares solve and restart logic of the reference in
 // FP64.  Work-group 0 writes each h_k to hcol.
-constexpr int GM_MAXV = 40;          // largest subspace (restart + augk)
+constexpr int GM_MAXV = 4096;        // largest subspace (restart + augk) + 1
 
 __global__ void __launch_bounds__(BS) k_zero(int64_t n, double *x)
 {
@@ -1511,12 +1514,14 @@ __global__ void __launch_bounds__(BS) k_gm_pc(int64_t n, const double *src,
 template <typename View>
 __global__ void __launch_bounds__(BS) k_gm_spmv(View mv, const double *va,
                                                 const double *f, double *v,
-                                                const double *v1, double *part_out)
+                                                const double *v1, const double *ldiag,
+                                                double *part_out)
 {
     double a[1] = {0.0};
     spmv_apply<double>(mv, va, f, [&](int32_t row, double s, double) {
-        v[row] = s;
-        a[0] += s * v1[row];
+        const double w = ldiag ? s / ldiag[row] : s;
+        v[row] = w;
+        a[0] += w * v1[row];
     });
     write_partials<1>(a, part_out);
 }
@@ -1564,38 +1569,55 @@ __global__ void __launch_bounds__(BS) k_gm_scale(int64_t n, double *w,
 }
 
 // out = sum_j c_j src_j in the reference's order (c_1 src_1, then + c_j
-// src_j), then dst += out (add_to) or dst = out
-struct GmCombo {
-    const double *src[GM_MAXV];
-    double c[GM_MAXV];
-    int m;
-};
-__global__ void __launch_bounds__(BS) k_gm_combo(int64_t n, GmCombo cb, double *dst,
-                                                 int add_to)
+// src_j) over two runs of vectors stored n apart: src_j = b1 + j n for
+// j < m1, then b2 + (j - m1) n; c_j = coef[j], or cval for every j when coef
+// is null.  Then z = out (z non-null), x += out / pdiag (pdiag non-null) or
+// x += out (x non-null), zp = out (zp non-null).  Each element's sources
+// are read before any output is written, so an output may be a source.
+__global__ void __launch_bounds__(BS) k_gm_combo(int64_t n, const double *b1, int m1,
+                                                 const double *b2, int m2,
+                                                 const double *coef, double cval,
+                                                 double *z, double *x,
+                                                 const double *pdiag, double *zp)
 {
     for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * BS) {
-        double z = cb.c[0] * cb.src[0][i];
-        for (int j = 1; j < cb.m; j++)
-            z += cb.c[j] * cb.src[j][i];
-        dst[i] = add_to ? dst[i] + z : z;
+        double o = (coef ? coef[0] : cval) * b1[i];
+        for (int j = 1; j < m1; j++)
+            o += (coef ? coef[j] : cval) * b1[(size_t)j * n + i];
+        for (int j = 0; j < m2; j++)
+            o += (coef ? coef[m1 + j] : cval) * b2[(size_t)j * n + i];
+        if (z)
+            z[i] = o;
+        if (x)
+            x[i] = x[i] + (pdiag ? o / pdiag[i] : o);
+        if (zp)
+            zp[i] = o;
     }
 }
 
-// z = b - A x (or b when x is zero), partials of |z|^2 and the number of
-// zero diagonals
+// z = b - A x (or b when x is zero), partials of |z|^2 and of the number of
+// zero diagonals; with left set, z = (b - A x) / diag instead and a third
+// partial of |z|^2 (the first stays |b - A x|^2)
 template <typename View>
 __global__ void __launch_bounds__(BS) k_gm_resid(int64_t n, View mv, const double *va,
                                                  const double *diag, const double *b,
-                                                 const double *x, int xnz, double *z,
-                                                 double *part_out)
+                                                 const double *x, int xnz, int left,
+                                                 double *z, double *part_out)
 {
-    double a[2] = {0.0, 0.0};
-    auto acc = [&](int64_t row, double zi) {
-        z[row] = zi;
-        a[0] += zi * zi;
-        if (diag[row] == 0.0)
+    double a[3] = {0.0, 0.0, 0.0};
+    auto acc = [&](int64_t row, double ri) {
+        a[0] += ri * ri;
+        const double di = diag[row];
+        if (di == 0.0)
             a[1] += 1.0;
+        if (left) {
+            const double zi = ri / di;
+            z[row] = zi;
+            a[2] += zi * zi;
+        }
+        else
+            z[row] = ri;
     };
     if (xnz)
         spmv_apply<double>(mv, va, x,
@@ -1604,7 +1626,7 @@ __global__ void __launch_bounds__(BS) k_gm_resid(int64_t n, View mv, const doubl
         for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
              i += (int64_t)gridDim.x * BS)
             acc(i, b[i]);
-    write_partials<2>(a, part_out);
+    write_partials<3>(a, part_out);
 }
 
 //------------------------------------------ reliable-update mixed precision
@@ -2017,9 +2039,13 @@ struct Workspace {
     // BiCGstab: rh, ph, v, s, sh, t (one allocation) and its states
     double *bi = nullptr;
     BiState *bst = nullptr;
-    // GMRES basis, preconditioned and augmentation vectors (gm_cnt of n)
+    // GMRES basis, preconditioned and augmentation vectors (gm_cnt of n);
+    // combination coefficients (2 slots of gm_ccap, staged through pinned
+    // host memory) and a pinned slot for reading a column of H back
     double *gm = nullptr;
     int gm_cnt = 0;
+    double *gm_coef = nullptr, *gm_coef_h = nullptr;
+    int gm_ccap = 0;
     double *seg(int k) { return partials + (size_t)k * 4 * MAX_GRID; }
 };
 
@@ -2174,6 +2200,9 @@ void release_ws(Workspace *w)
     phg::dfree(w->bi);
     phg::dfree(w->bst);
     phg::dfree(w->gm);
+    phg::dfree(w->gm_coef);
+    if (w->gm_coef_h)
+        (void)hipHostFree(w->gm_coef_h);
     *w = Workspace();
 }
 
@@ -2477,22 +2506,32 @@ int host_partial_sum(const double *d, int np, int k, hipStream_t s, double *out)
 // IsZero (utils.h:451) on the host
 inline bool is_zero_host(double a) { return fabs(a) < 10.0 * 2.2250738585072014e-308; }
 
-// solver-phg.c:2063 solver_FLGMRES_r in the numbering of the format
+// The GMRES family of solver-phg.c:59-77 in the numbering of the format:
+// type 0 gmres_l (solver-phg.c:783), 1 gmres_r (:1051), 2 fgmres_r (:1263),
+// 3 lgmres_l (:1481), 4 lgmres_r (:1801), 5 flgmres_r (:2063).
 template <typename View>
-int gmres_impl(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x, int restart,
-               int augk, int32_t maxit, double rtol, double btol, double atol,
-               double *residual)
+int gmres_impl(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x, int type,
+               int restart, int augk, int32_t maxit, double rtol, double btol,
+               double atol, double *residual)
 {
     using F = FmtData<View>;
     hipStream_t s = phg::ctx()->stream;
+    if (type < PHG_HIP_GMRES_L || type > PHG_HIP_FLGMRES_R)
+        return phg::set_error(PHG_HIP_E_ARG, "phg_hip_gmres: unknown GMRES type %d", type);
+    const bool left = type == PHG_HIP_GMRES_L || type == PHG_HIP_LGMRES_L;
+    const bool flex = type == PHG_HIP_FGMRES_R || type == PHG_HIP_FLGMRES_R;
+    const bool aug = type >= PHG_HIP_LGMRES_L;
     if (restart <= 0)
         restart = maxit + 1;
-    if (augk <= 0)
+    if (!aug)
+        augk = 0;                    // Unused(augk) in the plain variants
+    else if (augk <= 0)
         augk = 2;
     const int mk = restart, m = restart + augk;
-    if (m + 1 > GM_MAXV || augk > 8)
+    if (m + 1 > GM_MAXV || augk > 64)
         return phg::set_error(PHG_HIP_E_ARG,
-                              "phg_hip_gmres: restart + augk must be below %d", GM_MAXV);
+                              "phg_hip_gmres: restart + augk must be below %d and augk "
+                              "at most 64", GM_MAXV);
     Workspace *ws = get_ws(A, 0);
     if (ws == nullptr)
         return phg::set_error(PHG_HIP_E_NOMEM, "solver workspace allocation failed");
@@ -2507,6 +2546,31 @@ int gmres_impl(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x, int restart
             return phg::set_error(PHG_HIP_E_NOMEM, "GMRES workspace allocation failed");
         ws->gm_cnt = cnt;
     }
+    if (ws->gm_ccap < m + 2) {
+        phg::dfree(ws->gm_coef);
+        if (ws->gm_coef_h)
+            (void)hipHostFree(ws->gm_coef_h);
+        ws->gm_coef = ws->gm_coef_h = nullptr;
+        ws->gm_ccap = 0;
+        if (phg::dmalloc((void **)&ws->gm_coef, 2 * (size_t)(m + 2) * sizeof(double)) ||
+            hipHostMalloc((void **)&ws->gm_coef_h, 3 * (size_t)(m + 2) * sizeof(double),
+                          hipHostMallocDefault) != hipSuccess) {
+            ws->gm_coef_h = nullptr;
+            return phg::set_error(PHG_HIP_E_NOMEM, "GMRES workspace allocation failed");
+        }
+        ws->gm_ccap = m + 2;
+    }
+    // coefficients of a combination into slot k; a slot is rewritten only
+    // after a later stream synchronisation (every cycle ends with one)
+    auto put_coef = [&](int k, const double *c, int cnt_c) -> const double * {
+        double *h = ws->gm_coef_h + (size_t)k * ws->gm_ccap;
+        double *dv = ws->gm_coef + (size_t)k * ws->gm_ccap;
+        std::copy(c, c + cnt_c, h);
+        if (hipMemcpyAsync(dv, h, (size_t)cnt_c * sizeof(double), hipMemcpyHostToDevice, s) !=
+            hipSuccess)
+            return nullptr;
+        return dv;
+    };
     auto V = [&](int j) { return ws->gm + (size_t)(j - 1) * n; };              // 1..m+1
     auto FL = [&](int j) { return ws->gm + (size_t)(m + j) * n; };             // 1..m
     auto ZP = [&](int j) { return ws->gm + (size_t)(2 * m + 1 + j) * n; };     // 0..augk-1
@@ -2532,6 +2596,7 @@ int gmres_impl(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x, int restart
         // longer columns live in the partials segment after the MGS ones
         hcol = ws->seg(SEG_IR);
     }
+    static_assert(GM_MAXV + 1 <= 4 * MAX_GRID, "a column of H fits a partials segment");
     int rc;
     double nb[2];
     hipLaunchKernelGGL(k_pcg_init_norms, dim3(g), dim3(BS), 0, s, n, bb,
@@ -2541,64 +2606,75 @@ int gmres_impl(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x, int restart
         (rc = host_partial_sum(pn, g, 1, s, &nb[1])))
         return rc;
     const double b_norm = sqrt(nb[0]);
+    std::vector<double> cbuf((size_t)(m + 2));
     std::vector<double> H((size_t)(m + 2) * (m + 2)), yv((size_t)(m + 2)),
         sv((size_t)(m + 2)), c1((size_t)(m + 2)), c2((size_t)(m + 2));
     auto Hm = [&](int a, int c) -> double & { return H[(size_t)a * (m + 2) + c]; };
     int nits = 0;
     double res = 1.0;                // GMRES_Solve's initial value
+    // r = b - A x (x != 0, solver-phg.c:2158: phgVecNorm1(x) != 0); the
+    // left variants keep z = M^-1 r, the right ones z = r
+    auto residual_pass = [&](int xnz, double *rr, double *zz2, double *nzero) {
+        hipLaunchKernelGGL((k_gm_resid<View>), dim3(gs), dim3(BS), 0, s, n, mv, va, d, bb,
+                           (const double *)xx, xnz, left ? 1 : 0, z, pr);
+        PHG_HIP_CHECK(hipGetLastError());
+        double r2;
+        int e;
+        if ((e = host_partial_sum(pr, gs, 0, s, &r2)) ||
+            (nzero && (e = host_partial_sum(pr, gs, 1, s, nzero))) ||
+            (e = host_partial_sum(pr, gs, left ? 2 : 0, s, zz2)))
+            return e;
+        *rr = sqrt(r2);
+        return PHG_HIP_OK;
+    };
     if (is_zero_host(b_norm)) {
         hipLaunchKernelGGL(k_zero, dim3(g), dim3(BS), 0, s, n, xx);
     }
     else {
-        // z = b - A x when x != 0 (solver-phg.c:2158: phgVecNorm1(x) != 0)
-        hipLaunchKernelGGL((k_gm_resid<View>), dim3(gs), dim3(BS), 0, s, n, mv, va, d,
-                           bb, (const double *)xx, nb[1] != 0.0 ? 1 : 0, z, pr);
-        PHG_HIP_CHECK(hipGetLastError());
-        double zz, nzero;
-        if ((rc = host_partial_sum(pr, gs, 0, s, &zz)) ||
-            (rc = host_partial_sum(pr, gs, 1, s, &nzero)))
+        double zz, nzero, r_norm;
+        if ((rc = residual_pass(nb[1] != 0.0 ? 1 : 0, &r_norm, &zz, &nzero)))
             return rc;
-        double r_norm = sqrt(zz);
         res = r_norm;
         double tol = r_norm * rtol;
         if (tol < b_norm * btol)
             tol = b_norm * btol;
         if (tol < atol)
             tol = atol;
-        if (nzero != 0.0 && maxit > 0 && r_norm > tol)
+        // jacobi_pc stops on a zero diagonal (solver-phg.c:230) the first
+        // time it runs: at once for the left variants (M z = r0), in the
+        // first cycle for the plain right ones (x += M^-1 V y), at the
+        // first Arnoldi step for the flexible ones
+        if (nzero != 0.0 &&
+            (left || (maxit > 0 && (!flex || r_norm > tol))))
             return phg::set_error(PHG_HIP_E_ZERO_DIAG,
                                   "zero diagonal in Jacobi preconditioner");
         sv[0] = 1.0;
+        double s0 = 1.0;
+        bool false_conv = false, break_flag = false;
         int itr_out = 0;
-        bool first_cycle = true;
         while (nits < maxit) {
             const int rs = itr_out <= augk ? mk + itr_out : mk + augk;
-            if (!first_cycle) {
-                if ((rc = host_partial_sum(pr, gs, 0, s, &zz)))
-                    return rc;
-            }
-            first_cycle = false;
             sv[1] = r_norm = sqrt(zz);
-            // v_1 = z / |z| (vaxpby(1 / r_norm, z, 0, v_1)); flv_1 = v_1 / diag
-            {
-                GmCombo cb{};
-                cb.src[0] = z;
-                cb.c[0] = 1.0 / r_norm;
-                cb.m = 1;
-                hipLaunchKernelGGL(k_gm_combo, dim3(g), dim3(BS), 0, s, n, cb, V(1), 0);
+            // v_1 = z / |z| (vaxpby(1 / r_norm, z, 0, v_1)); right: flv_1 = v_1 / diag
+            hipLaunchKernelGGL(k_gm_combo, dim3(g), dim3(BS), 0, s, n, (const double *)z, 1,
+                               (const double *)nullptr, 0, (const double *)nullptr,
+                               1.0 / r_norm, V(1), (double *)nullptr,
+                               (const double *)nullptr, (double *)nullptr);
+            if (!left)
                 hipLaunchKernelGGL(k_gm_pc, dim3(g), dim3(BS), 0, s, n,
                                    (const double *)V(1), d, FL(1));
-            }
             int i = 0;
             std::fill(H.begin(), H.end(), 0.0);
-            while (i < rs && r_norm > tol && nits < maxit) {
+            while (i < rs && (r_norm > tol || false_conv) && nits < maxit) {
                 i++, nits++;
-                if (i > mk)
-                    hipLaunchKernelGGL(k_gm_pc, dim3(g), dim3(BS), 0, s, n,
-                                       (const double *)ZP(i - mk - 1), d, FL(i));
+                const double *src = i <= mk ? V(i) : ZP(i - mk - 1);
+                if (!left && i > mk)
+                    hipLaunchKernelGGL(k_gm_pc, dim3(g), dim3(BS), 0, s, n, src, d, FL(i));
                 double *w = V(i + 1);
                 hipLaunchKernelGGL((k_gm_spmv<View>), dim3(gs), dim3(BS), 0, s, mv, va,
-                                   (const double *)FL(i), w, (const double *)V(1), pp[0]);
+                                   left ? src : (const double *)FL(i), w,
+                                   (const double *)V(1), left ? d : (const double *)nullptr,
+                                   pp[0]);
                 for (int k = 1; k <= i; k++)
                     hipLaunchKernelGGL(k_gm_mgs, dim3(g), dim3(BS), 0, s, n, w,
                                        (const double *)V(k),
@@ -2609,9 +2685,9 @@ int gmres_impl(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x, int restart
                 // augmentation vector
                 hipLaunchKernelGGL(k_gm_scale, dim3(g), dim3(BS), 0, s, n, w,
                                    (const double *)pp[i & 1], g, hcol, i + 1, d,
-                                   i + 1 <= mk ? FL(i + 1) : (double *)nullptr);
+                                   !left && i + 1 <= mk ? FL(i + 1) : (double *)nullptr);
                 PHG_HIP_CHECK(hipGetLastError());
-                double *hm = (double *)phg::ctx()->host_mailbox;
+                double *hm = ws->gm_coef_h + 2 * (size_t)ws->gm_ccap;
                 PHG_HIP_CHECK(hipMemcpyAsync(hm, hcol, (size_t)(i + 2) * sizeof(double),
                                              hipMemcpyDeviceToHost, s));
                 PHG_HIP_CHECK(hipStreamSynchronize(s));
@@ -2619,6 +2695,7 @@ int gmres_impl(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x, int restart
                     Hm(i, k) = hm[k];
                 if (is_zero_host(Hm(i, i + 1))) {
                     i--;                 // happy breakdown
+                    break_flag = true;
                     break;
                 }
                 for (int j = 1; j < i; j++) {
@@ -2636,7 +2713,13 @@ int gmres_impl(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x, int restart
                 sv[i] = c1[i] * sv[i];
                 Hm(i, i) = c1[i] * Hm(i, i) + c2[i] * Hm(i, i + 1);
                 r_norm = fabs(sv[i + 1]);
-                if (r_norm <= tol)
+                if (left) {
+                    // false convergence: run on until the estimate has
+                    // dropped by the ratio true / estimated residual
+                    if (false_conv && r_norm * s0 <= tol)
+                        false_conv = false;
+                }
+                else if (r_norm <= tol)
                     break;
             }
             for (int j = i; j > 0; j--) {
@@ -2644,59 +2727,76 @@ int gmres_impl(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x, int restart
                 for (int k = j - 1; k > 0; k--)
                     sv[k] -= Hm(j, k) * yv[j];
             }
-            // x += sum_j y_j flv_j (or += z when the cycle made no step)
-            GmCombo cb{};
-            if (i > 0) {
-                for (int j = 1; j <= i; j++) {
-                    cb.src[j - 1] = FL(j);
-                    cb.c[j - 1] = yv[j];
+            // flexible: x += sum_j y_j flv_j (x += z after a cycle of no step)
+            if (flex) {
+                if (i > 0) {
+                    const double *cd = put_coef(0, &yv[1], i);
+                    if (cd == nullptr)
+                        return phg::set_error(PHG_HIP_E_HIP, "GMRES coefficient copy failed");
+                    hipLaunchKernelGGL(k_gm_combo, dim3(g), dim3(BS), 0, s, n,
+                                       (const double *)FL(1), i, (const double *)nullptr, 0,
+                                       cd, 0.0, (double *)nullptr, xx,
+                                       (const double *)nullptr, (double *)nullptr);
                 }
-                cb.m = i;
+                else
+                    hipLaunchKernelGGL(k_gm_combo, dim3(g), dim3(BS), 0, s, n,
+                                       (const double *)z, 1, (const double *)nullptr, 0,
+                                       (const double *)nullptr, 1.0, (double *)nullptr, xx,
+                                       (const double *)nullptr, (double *)nullptr);
+            }
+            // z = V y plus the augmentation part (y_{mk+j} for j <= the
+            // number of augmentation vectors: values of an earlier cycle
+            // where this one stopped before step mk + j, as in the
+            // reference), kept as augmentation vector itr_out % augk; the
+            // left variants add it to x, the plain right ones M^-1 of it
+            if (!flex || aug) {
+                double *xo = flex ? (double *)nullptr : xx;
+                const double *pd = left ? (const double *)nullptr : d;
+                double *zo = aug ? ZP(itr_out % augk) : (double *)nullptr;
+                if (i > 0) {
+                    const int top = i <= mk ? i : mk;
+                    const int ak = i <= mk ? 0 : (itr_out < augk ? itr_out : augk);
+                    std::copy(&yv[1], &yv[1] + top, cbuf.begin());
+                    for (int j = 1; j <= ak; j++)
+                        cbuf[(size_t)(top + j - 1)] = yv[j + mk];
+                    const double *cd = put_coef(1, cbuf.data(), top + ak);
+                    if (cd == nullptr)
+                        return phg::set_error(PHG_HIP_E_HIP, "GMRES coefficient copy failed");
+                    hipLaunchKernelGGL(k_gm_combo, dim3(g), dim3(BS), 0, s, n,
+                                       (const double *)V(1), top, (const double *)ZP(0), ak,
+                                       cd, 0.0, z, xo, pd, zo);
+                }
+                else
+                    hipLaunchKernelGGL(k_gm_combo, dim3(g), dim3(BS), 0, s, n,
+                                       (const double *)z, 1, (const double *)nullptr, 0,
+                                       (const double *)nullptr, 1.0, z, xo, pd, zo);
+            }
+            PHG_HIP_CHECK(hipGetLastError());
+            if (!left) {
+                if (r_norm <= tol) {
+                    res = r_norm;
+                    break;
+                }
+                double rr;
+                if ((rc = residual_pass(1, &rr, &zz, nullptr)))
+                    return rc;
             }
             else {
-                cb.src[0] = z;
-                cb.c[0] = 1.0;
-                cb.m = 1;
-            }
-            hipLaunchKernelGGL(k_gm_combo, dim3(g), dim3(BS), 0, s, n, cb, xx, 1);
-            // the cycle's correction in the unpreconditioned space becomes
-            // augmentation vector itr_out % augk
-            const int zi = itr_out % augk;
-            GmCombo cz{};
-            if (i > 0) {
-                const int top = i <= mk ? i : mk;
-                for (int j = 1; j <= top; j++) {
-                    cz.src[j - 1] = V(j);
-                    cz.c[j - 1] = yv[j];
-                }
-                cz.m = top;
-                if (i > mk) {
-                    const int ak = itr_out < augk ? itr_out : augk;
-                    for (int j = 1; j <= ak; j++) {
-                        cz.src[cz.m] = ZP(j - 1);
-                        cz.c[cz.m] = yv[j + mk];
-                        cz.m++;
+                // true residual r = b - A x and z = M^-1 r for the next cycle
+                false_conv = false;
+                double rr;
+                if ((rc = residual_pass(1, &rr, &zz, nullptr)))
+                    return rc;
+                if (r_norm <= tol) {
+                    res = rr;
+                    if (res <= tol || break_flag)
+                        break;
+                    if (i < rs) {
+                        false_conv = true;
+                        s0 = res / r_norm;
                     }
                 }
             }
-            else {
-                cz.src[0] = z;
-                cz.c[0] = 1.0;
-                cz.m = 1;
-            }
-            // rows are read before they are written, so zp_zi may be a
-            // source; the result goes to z first, then to zp_zi
-            hipLaunchKernelGGL(k_gm_combo, dim3(g), dim3(BS), 0, s, n, cz, z, 0);
-            hipLaunchKernelGGL(k_copy, dim3(g), dim3(BS), 0, s, n, (const double *)z,
-                               ZP(zi));
-            PHG_HIP_CHECK(hipGetLastError());
-            if (r_norm <= tol) {
-                res = r_norm;
-                break;
-            }
-            hipLaunchKernelGGL((k_gm_resid<View>), dim3(gs), dim3(BS), 0, s, n, mv, va, d,
-                               bb, (const double *)xx, 1, z, pr);
-            PHG_HIP_CHECK(hipGetLastError());
             itr_out++;
         }
     }
@@ -3546,10 +3646,10 @@ extern "C" int phg_hip_bicgstab(const phg_hip_mat *A_, const phg_hip_vec *b,
     });
 }
 
-extern "C" int phg_hip_gmres(const phg_hip_mat *A_, const phg_hip_vec *b,
-                             phg_hip_vec *x, int32_t restart, int32_t augk,
-                             int32_t maxit, double rtol, double btol, double atol,
-                             double *residual)
+extern "C" int phg_hip_gmres_type(const phg_hip_mat *A_, const phg_hip_vec *b,
+                                  phg_hip_vec *x, int32_t gmres_type, int32_t restart,
+                                  int32_t augk, int32_t maxit, double rtol, double btol,
+                                  double atol, double *residual)
 {
     int rc;
     if ((rc = check_solve_args(A_, b, x)))
@@ -3559,8 +3659,18 @@ extern "C" int phg_hip_gmres(const phg_hip_mat *A_, const phg_hip_vec *b,
         return rc;
     return with_format(A->fmt64, [&](auto tag) {
         using View = decltype(tag);
-        return gmres_impl<View>(A, b, x, restart, augk, maxit, rtol, btol, atol, residual);
+        return gmres_impl<View>(A, b, x, gmres_type, restart, augk, maxit, rtol, btol, atol,
+                                residual);
     });
+}
+
+extern "C" int phg_hip_gmres(const phg_hip_mat *A, const phg_hip_vec *b,
+                             phg_hip_vec *x, int32_t restart, int32_t augk,
+                             int32_t maxit, double rtol, double btol, double atol,
+                             double *residual)
+{
+    return phg_hip_gmres_type(A, b, x, PHG_HIP_FLGMRES_R, restart, augk, maxit, rtol, btol,
+                              atol, residual);
 }
 
 extern "C" int phg_hip_pcg_mixed(const phg_hip_mat *A_, const phg_hip_vec *b,

@@ -20,6 +20,7 @@ enum { HIP_PCG = 0, HIP_BICGSTAB = 1, HIP_GMRES = 2 };
 typedef struct {
     int kind;                 /* HIP_PCG / HIP_BICGSTAB / HIP_GMRES */
     int mixed, restart, augk, tune;
+    int gmres_type;           /* PHG_HIP_GMRES_L .. PHG_HIP_FLGMRES_R */
     phg_hip_mat *A;           /* device system */
     int owns_A;               /* 0: borrowed from a matrix-free MAT */
     phg_hip_vec *b, *x;
@@ -61,6 +62,7 @@ static int init_kind(SOLVER *solver, int kind)
     d->kind = kind;
     d->restart = 10;          /* -gmres_restart default (solver-phg.c) */
     d->augk = 2;              /* -lgmres_augk default */
+    d->gmres_type = PHG_HIP_FLGMRES_R;  /* -gmres_type default */
     solver->oem_data = d;
     /* the library needs its stream before any matrix is uploaded */
     if (phg_hip_init(0) != PHG_HIP_OK)
@@ -227,8 +229,9 @@ static int HipSolve(SOLVER *solver, VEC *x, BOOLEAN destroy)
                                 solver->btol, solver->atol, &residual);
         break;
     default:
-        nits = phg_hip_gmres(d->A, d->b, d->x, d->restart, d->augk, solver->maxit,
-                             solver->rtol, solver->btol, solver->atol, &residual);
+        nits = phg_hip_gmres_type(d->A, d->b, d->x, d->gmres_type, d->restart, d->augk,
+                                  solver->maxit, solver->rtol, solver->btol, solver->atol,
+                                  &residual);
         break;
     }
     if (nits == PHG_HIP_E_ZERO_DIAG)
@@ -266,6 +269,21 @@ int phgSolverHipSetOptions(SOLVER *solver, int mixed, int restart, int augk, int
     d->augk = augk > 0 ? augk : 2;
     d->tune = tune;
     return 0;
+}
+
+int phgSolverHipSetGMRESType(SOLVER *solver, const char *name)
+{
+    /* the keywords of -gmres_type, solver-phg.c:71 gmres_names */
+    static const char *names[] = {"gmres_l", "gmres_r", "fgmres_r",
+                                  "lgmres_l", "lgmres_r", "flgmres_r"};
+    if (solver->oem_solver != &phgSolverHipGMRES_ || name == NULL)
+        return -1;
+    for (int i = 0; i < (int)(sizeof(names) / sizeof(names[0])); i++)
+        if (strcmp(name, names[i]) == 0) {
+            ((HIP_DATA *)solver->oem_data)->gmres_type = PHG_HIP_GMRES_L + i;
+            return 0;
+        }
+    return -1;
 }
 
 /*-------------------------------------------------------- MV_FUNC, PC_PROC */

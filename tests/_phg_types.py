@@ -92,6 +92,7 @@ def load():
            ("phgSolverSetMaxIt", None, [_SP, C.c_int]),
            ("phgSolverSetTol", None, [_SP, FLOAT]),
            ("phgSolverHipSetOptions", C.c_int, [_SP, C.c_int, C.c_int, C.c_int, C.c_int]),
+           ("phgSolverHipSetGMRESType", C.c_int, [_SP, C.c_char_p]),
            ("phgMatCreateHip", _P(MAT), [C.c_void_p, BOOLEAN]),
            ("phgMatDestroyHip", None, [_P(_P(MAT))]),
            ("phgHipMatVec", C.c_int, [C.c_int, _P(MAT), _P(VEC), _P(VEC)]),

@@ -180,6 +180,32 @@ def test_bicgstab_and_gmres_vtables(oem, name):
     oem.phgSolverDestroy(C.byref(s))
 
 
+@pytest.mark.parametrize("gtype", orc.GMRES_TYPES)
+def test_gmres_vtable_every_gmres_type(oem, gtype):
+    """hip_gmres with each -gmres_type keyword (solver-phg.c:71) against the
+    oracle's restatement of that variant; unknown keywords and other
+    solvers are refused."""
+    c, Ao = _system(3, 2)
+    M = PackedMat(Ao.rowptr, Ao.cols, Ao.vals, c["bdry"])
+    s = oem.phgSolverMat2Solver(C.addressof(T.vtable(oem, "phgSolverHipGMRES_")), M.ptr)
+    assert oem.phgSolverHipSetGMRESType(s, b"gmres") == -1
+    assert oem.phgSolverHipSetGMRESType(s, gtype.encode()) == 0
+    _vec_np(s.contents.rhs, M.n)[:] = Ao.rhs
+    x = oem.phgMapCreateVec(C.pointer(M.map), 1)
+    nits = oem.phgSolverVecSolve(s, 0, x)
+    xo, nits_o, res_o = orc.gmres(Ao, Ao.rhs, rtol=1e-10, btol=1e-10, gmres_type=gtype)
+    xv = _vec_np(x, M.n).copy()
+    print(f"{gtype}: {nits} its (oracle {nits_o}), residual {s.contents.residual:.3e} "
+          f"(oracle {res_o:.3e})", flush=True)
+    assert abs(nits - nits_o) <= max(2, nits_o // 20)
+    assert np.linalg.norm(xv - xo) / np.linalg.norm(xo) < 1e-8
+    oem.phgVecDestroy(C.byref(x))
+    oem.phgSolverDestroy(C.byref(s))
+    s = oem.phgSolverMat2Solver(C.addressof(T.vtable(oem, "phgSolverHipPCG_")), M.ptr)
+    assert oem.phgSolverHipSetGMRESType(s, gtype.encode()) == -1
+    oem.phgSolverDestroy(C.byref(s))
+
+
 def test_matrix_free_device_matrix(oem, hip):
     """The element loop lowered to the GPU (phg_hip_assemble + eliminate)
     wrapped as a matrix-free PHG MAT: SOLVER_HIP_PCG solves it without host
