@@ -231,8 +231,12 @@ def c4spmm(reps=20):
     best64 = min(shapes, key=lambda s: shapes[s][0])
     b32, b64 = pa.spmm_bytes(A, True), pa.spmm_bytes(A, False)
     n_, nnz = A.info()
+    # a plain copy of known size, the calibration of HBM counters
+    ncopy = 1 << 24
+    copy_us = pa.stream_copy(ncopy, reps)
     return {"config": "c4spmm", "workload": "vector P2 elasticity 40^3x6 SpMM, 16 columns",
-            "n": n_, "nnz": nnz, "shapes_us": {str(s): v for s, v in shapes.items()},
+            "n": n_, "nnz": nnz, "copy": {"bytes_each_way": 8 * ncopy, "us": copy_us},
+            "shapes_us": {str(s): v for s, v in shapes.items()},
             "fp32": {"shape": best32, "us": shapes[best32][1], "bytes": b32,
                      "GBps": round(b32 / shapes[best32][1] / 1e3, 1)},
             "fp64": {"shape": best64, "us": shapes[best64][0], "bytes": b64,

@@ -160,6 +160,33 @@ __device__ inline Cols<T, CPL> load_cols(const T *p)
     return r;
 }
 
+// value of lane k (0..3) of each quad of lanes, by DPP quad_perm
+template <int K>
+__device__ inline int quad_bcast_i(int v)
+{
+    return __builtin_amdgcn_mov_dpp(v, K | K << 2 | K << 4 | K << 6, 0xf, 0xf, false);
+}
+__device__ inline int quad_bcast_i(int v, int k)
+{
+    switch (k) {
+    case 0: return quad_bcast_i<0>(v);
+    case 1: return quad_bcast_i<1>(v);
+    case 2: return quad_bcast_i<2>(v);
+    default: return quad_bcast_i<3>(v);
+    }
+}
+__device__ inline float quad_bcast(float v, int k)
+{
+    return __int_as_float(quad_bcast_i(__float_as_int(v), k));
+}
+__device__ inline double quad_bcast(double v, int k)
+{
+    const int64_t b = __double_as_longlong(v);
+    const int lo = quad_bcast_i((int)(b & 0xffffffff), k);
+    const int hi = quad_bcast_i((int)(b >> 32), k);
+    return __longlong_as_double((int64_t)(uint32_t)lo | ((int64_t)hi << 32));
+}
+
 // block rows one work-group step covers for a shape
 template <int S, int CPL>
 constexpr int rows_per_group() { return (64 / (S * (KB / CPL))) * (TB / 64); }
@@ -228,15 +255,35 @@ __device__ inline void bsr_spmm(const BsrView &m, const T *__restrict__ va,
                                     xq[q].v[i] -= xr[q].v[i];
                         }
                         const T *a = va + (int64_t)(j0 + jj) * (B * B);
+                        if constexpr (NCG == 4 && B == 3) {
+                            // the four column-group lanes of a quad share
+                            // the block: lane cg loads entries cg, cg + 4
+                            // and 8 (cg = 0; the others repeat entry cg),
+                            // and the entries go round the quad by DPP, so
+                            // the block is fetched once, not four times
+                            const T ld[3] = {a[cg], a[cg + 4], a[cg == 0 ? 8 : cg]};
 #pragma unroll
-                        for (int p = 0; p < B; p++)
+                            for (int p = 0; p < B; p++)
 #pragma unroll
-                            for (int q = 0; q < B; q++) {
-                                const T apq = a[p * B + q];
+                                for (int q = 0; q < B; q++) {
+                                    const int e = p * B + q;
+                                    const T apq = quad_bcast(ld[e >> 2], e & 3);
 #pragma unroll
-                                for (int i = 0; i < CPL; i++)
-                                    acc[p][i] += apq * xq[q].v[i];
-                            }
+                                    for (int i = 0; i < CPL; i++)
+                                        acc[p][i] += apq * xq[q].v[i];
+                                }
+                        }
+                        else {
+#pragma unroll
+                            for (int p = 0; p < B; p++)
+#pragma unroll
+                                for (int q = 0; q < B; q++) {
+                                    const T apq = a[p * B + q];
+#pragma unroll
+                                    for (int i = 0; i < CPL; i++)
+                                        acc[p][i] += apq * xq[q].v[i];
+                                }
+                        }
                     }
                 }
             }
