@@ -353,11 +353,108 @@ static int read_line(FILE *f, char *buf, int n)
     return fgets(buf, n, f) != NULL;
 }
 
+/* refine.c:1829 permute_vertices: local vertices v0 and v1 of element e
+ * trade places together with their faces (face i lies opposite vertex i),
+ * neighbours and boundary types; each local edge takes the global edge of
+ * the vertex pair it now joins (the two edge pairs of the switch at
+ * refine.c:1862). */
+static void permute_vertices(phg_grid *g, int64_t e, int v0, int v1)
+{
+    if (v0 == v1)
+        return;
+    int32_t *v = g->elem_verts + 4 * e, *ed = g->elem_edges + 6 * e;
+    int32_t *fa = g->elem_faces + 4 * e, *nb = g->elem_neigh + 4 * e;
+    uint8_t *bt = g->elem_bound + 4 * e;
+    int sigma[4] = {0, 1, 2, 3}, t;
+    int32_t old[6], x;
+    sigma[v0] = v1;
+    sigma[v1] = v0;
+    x = v[v0]; v[v0] = v[v1]; v[v1] = x;
+    x = fa[v0]; fa[v0] = fa[v1]; fa[v1] = x;
+    x = nb[v0]; nb[v0] = nb[v1]; nb[v1] = x;
+    t = bt[v0]; bt[v0] = bt[v1]; bt[v1] = (uint8_t)t;
+    memcpy(old, ed, sizeof(old));
+    for (int k = 0; k < 6; k++) {
+        int a = sigma[phg_tet_edge_vertex[k][0]], b = sigma[phg_tet_edge_vertex[k][1]];
+        for (int m = 0; m < 6; m++)
+            if ((phg_tet_edge_vertex[m][0] == a && phg_tet_edge_vertex[m][1] == b) ||
+                (phg_tet_edge_vertex[m][0] == b && phg_tet_edge_vertex[m][1] == a))
+                ed[m] = old[k];
+    }
+}
+
+/* refine.c:1814 comp_edge: squared length, then global edge index */
+static int comp_edge(const double *len, const int32_t *ed, int i0, int i1)
+{
+    const int32_t e0 = ed[i0], e1 = ed[i1];
+    const double d = len[e0] - len[e1];
+    if (d == 0)
+        return e0 > e1 ? 1 : (e0 < e1 ? -1 : 0);
+    return d < 0 ? -1 : 1;
+}
+
+/* refine.c:1916 phgRefineInit with do_marking (Liu/Arnold initial marking),
+ * run by PHG on meshes imported without refinement types (utils.c:5617,
+ * after the edges and faces are numbered): per element the longest edge
+ * becomes edge 0-1, then the marked edges of faces 0 and 1 pick one of the
+ * permutations of refine.c:1987-2021.  Global numbers do not change; the
+ * elements' local order does. */
+static void refine_init_marking(phg_grid *g)
+{
+    double *len = xmalloc((size_t)(g->nedge > 0 ? g->nedge : 1) * sizeof(double));
+    for (int64_t i = 0; i < g->nedge; i++)
+        len[i] = -1.0;
+    for (int64_t e = 0; e < g->nelem; e++)
+        for (int k = 0; k < 6; k++) {
+            const int32_t id = g->elem_edges[6 * e + k];
+            if (len[id] >= 0)
+                continue;
+            const double *a = g->verts + 3 * g->elem_verts[4 * e + phg_tet_edge_vertex[k][0]];
+            const double *b = g->verts + 3 * g->elem_verts[4 * e + phg_tet_edge_vertex[k][1]];
+            const double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+            len[id] = dx * dx + dy * dy + dz * dz;
+        }
+    for (int64_t e = 0; e < g->nelem; e++) {
+        const int32_t *ed = g->elem_edges + 6 * e;
+        int k = 0, j;
+        for (j = 1; j < 6; j++)
+            if (comp_edge(len, ed, k, j) < 0)
+                k = j;
+        permute_vertices(g, e, 0, phg_tet_edge_vertex[k][0]);
+        permute_vertices(g, e, 1, phg_tet_edge_vertex[k][1]);
+        j = 3;                          /* marked edge of face 0 */
+        if (comp_edge(len, ed, j, 4) < 0)
+            j = 4;
+        if (comp_edge(len, ed, j, 5) < 0)
+            j = 5;
+        k = 1;                          /* marked edge of face 1 */
+        if (comp_edge(len, ed, k, 2) < 0)
+            k = 2;
+        if (comp_edge(len, ed, k, 5) < 0)
+            k = 5;
+        switch (j * 10 + k) {
+        case 32: case 42: case 52:
+            permute_vertices(g, e, 2, 3);
+            break;
+        case 35:
+            permute_vertices(g, e, 0, 1);
+            break;
+        case 45:
+            permute_vertices(g, e, 2, 3);
+            permute_vertices(g, e, 0, 1);
+            break;
+        default:                        /* 31, 41, 51, 55: no permutation */
+            break;
+        }
+    }
+    free(len);
+}
+
 phg_grid *phg_grid_read_albert(const char *path)
 {
     FILE *f = fopen(path, "r");
     char line[512];
-    int nv = -1, ne = -1;
+    int nv = -1, ne = -1, typed = 0;
     phg_grid *g = NULL;
 
     if (f == NULL)
@@ -387,6 +484,8 @@ phg_grid *phg_grid_read_albert(const char *path)
             int what = !strcmp(p, "vertex coordinates") ? 0 :
                        !strcmp(p, "element vertices") ? 1 :
                        !strcmp(p, "element boundaries") ? 2 : 3;
+            if (!strcmp(p, "element type"))
+                typed = 1;
             int64_t cnt = what == 0 ? nv : ne;
             for (int64_t i = 0; i < cnt; i++) {
                 if (!read_line(f, line, sizeof(line)))
@@ -418,6 +517,11 @@ phg_grid *phg_grid_read_albert(const char *path)
     if (g == NULL)
         return NULL;
     phg_grid_number(g);
+    /* albert.c:461 returns whether the file had element types; without them
+     * phgImport_ runs the initial marking (utils.c:5617 phgRefineInit(g,
+     * !flag)), with them the file's vertex order stands */
+    if (!typed)
+        refine_init_marking(g);
     return g;
 fail:
     fclose(f);

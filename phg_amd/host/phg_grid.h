@@ -16,7 +16,10 @@
  *     dof.c:978 phgDofMapEdgeData).
  * Meshes read from ALBERT files that carry an "element type" section are
  * imported without the vertex permutation of refine.c:1916 phgRefineInit,
- * which is what PHG does for such files.
+ * which is what PHG does for such files; files without it get that
+ * permutation (Liu/Arnold initial marking: longest edge first, then the
+ * marked edges of faces 0 and 1) after the numbering, as utils.c:5617 runs
+ * it.
  */
 #ifndef PHG_GRID_H
 #define PHG_GRID_H
