@@ -919,6 +919,8 @@ struct WalkArgs {
     double *svals;
     int32_t *count;         // [n] entries of each row
     double *rhs;            // [n]
+    int32_t *ovf;           // rows past the table: ovf[0] their number, then
+                            // the rows (null: replay them serially)
 };
 
 // one row replayed serially (matvec.c:3045 add_columns_to_row)
@@ -984,9 +986,12 @@ __device__ inline void walk_row_serial(const WalkArgs &w, int64_t r)
 // found in the row's table adds its value to that entry, a new one opens the
 // next entry, numbered by lane order.  Entries thus open in PHG's append
 // order and every entry sums its contributions in element order starting
-// from the first, as the row list does.  Rows with more than E distinct
-// columns, and Dirichlet rows, take the serial path.
-template <int G, int E>
+// from the first, as the row list does.  Dirichlet rows take the serial
+// path.  Rows with more than E distinct columns go to the list w.ovf, which
+// the LIST instance (one row per wavefront, a larger table) walks the same
+// way afterwards; a row that overflows that table too, or any row when there
+// is no list, is replayed serially.
+template <int G, int E, bool LIST = false>
 __global__ void __launch_bounds__(64) k_row_walk_hash(WalkArgs w)
 {
     constexpr int TS = 2 * E, NG = 64 / G;
@@ -998,9 +1003,10 @@ __global__ void __launch_bounds__(64) k_row_walk_hash(WalkArgs w)
     const uint64_t gm = (G == 64 ? ~0ull : (1ull << (G % 64)) - 1ull) << (grp * G % 64);
     const uint64_t below = (1ull << lane) - 1ull;
     const int M = w.N * w.dim;
-    for (int64_t r0 = (int64_t)blockIdx.x * NG; r0 < w.n; r0 += (int64_t)gridDim.x * NG) {
-        const int64_t r = r0 + grp;
-        const bool live = r < w.n;
+    const int64_t nrow = LIST ? (int64_t)w.ovf[0] : w.n;
+    for (int64_t r0 = (int64_t)blockIdx.x * NG; r0 < nrow; r0 += (int64_t)gridDim.x * NG) {
+        const bool live = r0 + grp < nrow;
+        const int64_t r = LIST ? (live ? (int64_t)w.ovf[1 + r0 + grp] : 0) : r0 + grp;
         const int64_t t0 = live ? w.inc_off[r] : 0;
         const int ninc = live ? (int)(w.inc_off[r + 1] - t0) : 0;
         const bool serial = live && w.bdry[r] != 0;
@@ -1074,8 +1080,12 @@ __global__ void __launch_bounds__(64) k_row_walk_hash(WalkArgs w)
                 w.rhs[r] = rhs;
             }
         }
-        else if (live && b == 0)
-            walk_row_serial(w, r);
+        else if (live && b == 0) {
+            if (!serial && !LIST && w.ovf != nullptr)
+                w.ovf[1 + atomicAdd(&w.ovf[0], 1)] = (int32_t)r;
+            else
+                walk_row_serial(w, r);
+        }
         __syncthreads();
     }
 }
@@ -1416,7 +1426,7 @@ fail:
 }  // namespace
 
 // scratch buffers of the assembly workspace
-enum { WS_K, WS_B, WS_F, WS_KAPPA, WS_G, WS_COUNT, WS_SCOLS, WS_SVALS, WS_ROWPTR, WS_N };
+enum { WS_K, WS_B, WS_F, WS_KAPPA, WS_G, WS_COUNT, WS_SCOLS, WS_SVALS, WS_ROWPTR, WS_OVF, WS_N };
 struct AsmWorkspace {
     int dim = 0;                      // problem dimension of the incidence (0: none)
     int32_t *cnt = nullptr, *inc = nullptr;
@@ -1825,15 +1835,22 @@ extern "C" int phg_hip_assemble(phg_hip_space *sp, const phg_hip_problem *pb,
         wk.svals = svals;
         wk.count = count;
         wk.rhs = A->d_rhs;
+        if ((rc = ws_buffer(sp, WS_OVF, (size_t)(n + 1) * sizeof(int32_t),
+                            (void **)&wk.ovf)))
+            goto done;
+        PHG_HIP_CHECK(hipMemsetAsync(wk.ovf, 0, sizeof(int32_t), s));
         // 64/G rows per wavefront, G the smallest of 16, 32, 64 holding
-        // an element row
+        // an element row; the rows past its table (high-valence vertices of
+        // refined P3 meshes: up to 343 columns) then one per wavefront with
+        // a table of 2048
         const unsigned grid = (unsigned)std::min<int64_t>((n + 3) / 4, 1 << 16);
         if (M <= 16)
             hipLaunchKernelGGL((k_row_walk_hash<16, 128>), dim3(grid), dim3(64), 0, s, wk);
         else if (M <= 32)
-            hipLaunchKernelGGL((k_row_walk_hash<32, 256>), dim3(grid), dim3(64), 0, s, wk);
+            hipLaunchKernelGGL((k_row_walk_hash<32, 128>), dim3(grid), dim3(64), 0, s, wk);
         else   // M <= 60 for P1-P3, scalar or vector (check_problem)
             hipLaunchKernelGGL((k_row_walk_hash<64, 512>), dim3(grid), dim3(64), 0, s, wk);
+        hipLaunchKernelGGL((k_row_walk_hash<64, 2048, true>), dim3(2048), dim3(64), 0, s, wk);
         PHG_HIP_CHECK(hipGetLastError());
     }
     tr.mark("row walk");
