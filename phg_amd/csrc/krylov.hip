@@ -3215,98 +3215,129 @@ int time_spmv(phg_hip_mat *A, const T *va, int reps, double *us)
 
 namespace phg {
 
+// Greedy split of the rows into runs of consecutive rows holding at most
+// cap entries and rcap rows; a longer row forms a block of its own.  The
+// greedy runs on its own in chunks of SPLIT_CHUNK rows (a block never spans
+// two chunks), one work-item per chunk: a counting pass, an exclusive scan
+// of the counts, and a pass writing each chunk's block ends at its offset.
+// A row's sum in the kernels does not depend on the blocking.
+constexpr int64_t SPLIT_CHUNK = 1024;
+
+template <typename Emit>
+__device__ inline void split_chunk(const int32_t *rp, int64_t r0, int64_t r1, int64_t cap,
+                                   int64_t rcap, Emit emit)
+{
+    int64_t cur_nnz = 0, cur_rows = 0;
+    for (int64_t r = r0; r < r1; r++) {
+        const int64_t len = rp[r + 1] - rp[r];
+        if (len > cap) {
+            if (cur_rows > 0)
+                emit((int32_t)r);
+            emit((int32_t)(r + 1));
+            cur_nnz = cur_rows = 0;
+            continue;
+        }
+        if (cur_rows > 0 && (cur_nnz + len > cap || cur_rows == rcap)) {
+            emit((int32_t)r);
+            cur_nnz = cur_rows = 0;
+        }
+        cur_nnz += len;
+        cur_rows++;
+    }
+    if (cur_rows > 0)
+        emit((int32_t)r1);
+}
+
+__global__ void k_split_count(const int32_t *rp, int64_t n, int64_t cap, int64_t rcap,
+                              int32_t *cnt)
+{
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t r0 = c * SPLIT_CHUNK;
+    if (r0 >= n)
+        return;
+    int32_t k = 0;
+    split_chunk(rp, r0, min(n, r0 + SPLIT_CHUNK), cap, rcap, [&](int32_t) { k++; });
+    cnt[c] = k;
+}
+
+// exclusive scan of cnt[0..m) into off[0..m], one work-group of BS
+__global__ void __launch_bounds__(BS) k_split_scan(const int32_t *cnt, int64_t m,
+                                                   int32_t *off)
+{
+    __shared__ int32_t sh[BS];
+    int32_t carry = 0;
+    for (int64_t base = 0; base < m; base += BS) {
+        const int64_t i = base + threadIdx.x;
+        int32_t v = i < m ? cnt[i] : 0;
+        sh[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < BS; o <<= 1) {
+            const int32_t t = threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
+            __syncthreads();
+            sh[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < m)
+            off[i] = carry + sh[threadIdx.x] - v;
+        const int32_t tot = sh[BS - 1];
+        __syncthreads();
+        carry += tot;
+    }
+    if (threadIdx.x == 0)
+        off[m] = carry;
+}
+
+__global__ void k_split_fill(const int32_t *rp, int64_t n, int64_t cap, int64_t rcap,
+                             const int32_t *off, int32_t *blk)
+{
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t r0 = c * SPLIT_CHUNK;
+    if (c == 0)
+        blk[0] = 0;
+    if (r0 >= n)
+        return;
+    int32_t *o = blk + 1 + off[c];
+    split_chunk(rp, r0, min(n, r0 + SPLIT_CHUNK), cap, rcap, [&](int32_t e) { *o++ = e; });
+}
+
 int split_row_blocks(const int32_t *d_rowptr, int64_t n, int64_t nnz,
                      int64_t cap, int64_t rcap, int32_t **d, int64_t *nb)
 {
-    std::vector<int32_t> rp((size_t)n + 1);
-    int rc;
-    if ((rc = download(rp.data(), d_rowptr, rp.size() * sizeof(int32_t))))
-        return rc;
-    return split_row_blocks_host(rp.data(), n, nnz, cap, rcap, d, nb);
-}
-
-int split_row_blocks_host(const int32_t *rp, int64_t n, int64_t nnz, int64_t cap,
-                          int64_t rcap, int32_t **d, int64_t *nb)
-{
-    // greedy split into runs of consecutive rows holding at most cap entries
-    // and rcap rows; a longer row forms a block of its own.  The greedy runs
-    // on its own in fixed chunks of rows (a block never spans two chunks),
-    // so the chunks split on host threads and the result does not depend on
-    // their number; a row's sum in the kernels does not depend on the
-    // blocking at all.
-    constexpr int64_t CHUNK = 1 << 16;
-    const int64_t nch = std::max<int64_t>(1, (n + CHUNK - 1) / CHUNK);
-    std::vector<std::vector<int32_t>> part((size_t)nch);
-    auto split = [&](int64_t c) {
-        std::vector<int32_t> &blk = part[(size_t)c];
-        const int64_t r0 = c * CHUNK, r1 = std::min(n, r0 + CHUNK);
-        int64_t cur_nnz = 0, cur_rows = 0;
-        for (int64_t r = r0; r < r1; r++) {
-            int64_t len = rp[(size_t)r + 1] - rp[(size_t)r];
-            if (len > cap) {
-                if (cur_rows > 0)
-                    blk.push_back((int32_t)r);
-                blk.push_back((int32_t)(r + 1));
-                cur_nnz = cur_rows = 0;
-                continue;
-            }
-            if (cur_rows > 0 && (cur_nnz + len > cap || cur_rows == rcap)) {
-                blk.push_back((int32_t)r);
-                cur_nnz = cur_rows = 0;
-            }
-            cur_nnz += len;
-            cur_rows++;
-        }
-        if (cur_rows > 0)
-            blk.push_back((int32_t)r1);
-    };
-    try {
-        const int nt = (int)std::min<int64_t>(nch, std::max(1u, std::thread::hardware_concurrency()));
-        std::vector<std::thread> th;
-        std::atomic<int64_t> next{0};
-        std::atomic<bool> failed{false};
-        auto work = [&]() {
-            try {
-                for (int64_t c; (c = next++) < nch;)
-                    split(c);
-            }
-            catch (...) {
-                failed = true;
-            }
-        };
-        for (int t = 1; t < std::min(nt, 16); t++) {
-            try {
-                th.emplace_back(work);
-            }
-            catch (...) {
-                break;
-            }
-        }
-        work();
-        for (auto &x : th)
-            x.join();
-        if (failed)
-            throw std::bad_alloc();
-    }
-    catch (const std::bad_alloc &) {
-        return set_error(PHG_HIP_E_NOMEM, "row-block split: out of host memory");
-    }
-    std::vector<int32_t> blk;
-    size_t total = 1;
-    for (const auto &v : part)
-        total += v.size();
-    blk.reserve(total);
-    blk.push_back(0);
-    for (const auto &v : part)
-        blk.insert(blk.end(), v.begin(), v.end());
     (void)nnz;
+    hipStream_t s = ctx()->stream;
+    const int64_t m = (n + SPLIT_CHUNK - 1) / SPLIT_CHUNK;
+    int32_t *cnt = nullptr, *off = nullptr, total = 0;
     int rc;
     dfree(*d);
     *d = nullptr;
-    *nb = (int64_t)blk.size() - 1;
-    if ((rc = dmalloc((void **)d, blk.size() * sizeof(int32_t))) ||
-        (rc = upload(*d, blk.data(), blk.size() * sizeof(int32_t))))
+    // at most two block ends per row, plus the leading 0
+    if ((rc = dmalloc((void **)&cnt, (size_t)(m + 1) * sizeof(int32_t))) ||
+        (rc = dmalloc((void **)&off, (size_t)(m + 1) * sizeof(int32_t))) ||
+        (rc = dmalloc((void **)d, (size_t)(2 * n + 2) * sizeof(int32_t)))) {
+        dfree(cnt);
+        dfree(off);
+        dfree(*d);
+        *d = nullptr;
         return rc;
+    }
+    const int g = (int)std::max<int64_t>(1, (m + 255) / 256);
+    hipLaunchKernelGGL(k_split_count, dim3(g), dim3(256), 0, s, d_rowptr, n, cap, rcap, cnt);
+    hipLaunchKernelGGL(k_split_scan, dim3(1), dim3(BS), 0, s, (const int32_t *)cnt, m, off);
+    hipLaunchKernelGGL(k_split_fill, dim3(g), dim3(256), 0, s, d_rowptr, n, cap, rcap,
+                       (const int32_t *)off, *d);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        rc = download(&total, off + m, sizeof(int32_t));
+    else
+        rc = hip_fail(e, "row-block split", __FILE__, __LINE__);
+    dfree(cnt);
+    dfree(off);
+    if (rc) {
+        dfree(*d);
+        *d = nullptr;
+        return rc;
+    }
+    *nb = total;
     return PHG_HIP_OK;
 }
 

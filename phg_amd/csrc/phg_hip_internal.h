@@ -271,9 +271,6 @@ int build_solver_copy(phg_hip_mat *A);
 void free_solver_copy(phg_hip_mat *A);
 int split_row_blocks(const int32_t *d_rowptr, int64_t n, int64_t nnz,
                      int64_t cap, int64_t rcap, int32_t **d_blk, int64_t *nblk);
-// the same split from a row-pointer array in host memory
-int split_row_blocks_host(const int32_t *rp, int64_t n, int64_t nnz, int64_t cap,
-                          int64_t rcap, int32_t **d, int64_t *nb);
 int build_sell(phg_hip_mat *A);
 void free_sell(phg_hip_mat *A);
 int ensure_fp32_copy(phg_hip_mat *A);
