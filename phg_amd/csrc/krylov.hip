@@ -673,6 +673,37 @@ struct XsView {
 };
 template <int PF> struct XsViewT : XsView {};
 
+// x line of a dictionary slot: the LW values from column d, 0 past n
+template <typename T> struct XLine { T v[1 << phg::XS_SHIFT]; };
+template <typename T>
+__device__ inline void load_xline(const T *__restrict__ x, int64_t n, int32_t d,
+                                  XLine<T> &xl)
+{
+    constexpr int LW = 1 << phg::XS_SHIFT;
+    if ((int64_t)d + LW <= n) {
+        if (sizeof(T) == 4) {
+#pragma unroll
+            for (int k = 0; k < LW / 4; k++) {
+                const float4 a = ((const float4 *)(x + d))[k];
+                xl.v[4 * k] = a.x; xl.v[4 * k + 1] = a.y;
+                xl.v[4 * k + 2] = a.z; xl.v[4 * k + 3] = a.w;
+            }
+        }
+        else {
+#pragma unroll
+            for (int k = 0; k < LW / 2; k++) {
+                const double2 a = ((const double2 *)(x + d))[k];
+                xl.v[2 * k] = a.x; xl.v[2 * k + 1] = a.y;
+            }
+        }
+    }
+    else {
+#pragma unroll
+        for (int k = 0; k < LW; k++)
+            xl.v[k] = (int64_t)d + k < n ? x[d + k] : (T)0;
+    }
+}
+
 template <int PF, typename T, typename Epi>
 __device__ inline void xs_spmv(const XsView &m, const T *__restrict__ va,
                                const T *__restrict__ x, Epi epi)
@@ -683,7 +714,6 @@ __device__ inline void xs_spmv(const XsView &m, const T *__restrict__ va,
     static_assert(phg::XS_SLOTS <= BS, "one work-item per line");
     __shared__ __align__(16) T prod[phg::XS_BLOCK_NNZ];
     __shared__ __align__(16) T xsh[phg::XS_SLOTS * LW];
-    __shared__ int32_t lines[phg::XS_SLOTS];
     __shared__ SweepMeta md;
     const DictView &dv = reinterpret_cast<const DictView &>(m);
     int64_t first, end, step;
@@ -692,17 +722,25 @@ __device__ inline void xs_spmv(const XsView &m, const T *__restrict__ va,
     for (int64_t c0 = first; c0 < end; c0 += step * DICT_META) {
         const int nb = sweep_meta(dv, c0, end, step, md);
         Stage cur, nxt;
+        // work-item t < nd holds the x line of dictionary slot t; with PF
+        // the next block's lines load while this block's products run
+        XLine<T> xl;
         sweep_fetch(dv, md, va, x, 0, cur);
+        if (tid < md.d1[0] - md.d0[0])
+            load_xline(x, m.n, cur.d, xl);
         for (int i = 0; i < nb; i++) {
-            if (!PF && i > 0)
-                sweep_fetch(dv, md, va, x, i, cur);
             const int nd = md.d1[i] - md.d0[i];
-            if (tid < phg::XS_SLOTS)
-                lines[tid] = cur.d;
-            lds_barrier();
-            for (int e = tid; e < nd * LW; e += BS) {
-                const int64_t col = (int64_t)lines[e >> phg::XS_SHIFT] + (e & (LW - 1));
-                xsh[e] = col < m.n ? x[col] : (T)0;
+            if (!PF && i > 0) {
+                sweep_fetch(dv, md, va, x, i, cur);
+                if (tid < nd)
+                    load_xline(x, m.n, cur.d, xl);
+            }
+            // the previous block's products (the last xsh reads) ended at
+            // its second barrier
+            if (tid < nd) {
+#pragma unroll
+                for (int k = 0; k < LW; k++)
+                    xsh[tid * LW + k] = xl.v[k];
             }
             if (PF && i + 1 < nb)
                 sweep_fetch(dv, md, va, x, i + 1, nxt);
@@ -723,6 +761,8 @@ __device__ inline void xs_spmv(const XsView &m, const T *__restrict__ va,
                     }
                 }
             }
+            if (PF && i + 1 < nb && tid < md.d1[i + 1] - md.d0[i + 1])
+                load_xline(x, m.n, nxt.d, xl);
             lds_barrier();
             if (tid < cur.q1 - cur.q0) {
                 T sum = ordered_sum(prod, cur.s0 - cur.f0, cur.s1 - cur.f0);
