@@ -38,6 +38,7 @@
 // element contributes (P2: 64 B against ~240 B).
 #include "phg_hip_internal.h"
 
+#include <hipcub/hipcub.hpp>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -124,39 +125,6 @@ struct NoInit : std::allocator<T> {
 };
 template <typename T>
 using HostVec = std::vector<T, NoInit<T>>;
-
-// sort of v (std::sort per chunk on the host threads, then pairwise merges)
-template <typename V>
-void host_sort(V &v)
-{
-    const int64_t n = (int64_t)v.size();
-    const int nt = host_threads(n);
-    std::vector<int64_t> cut((size_t)nt + 1);
-    for (int t = 0; t <= nt; t++)
-        cut[(size_t)t] = n * t / nt;
-    host_parallel(n, [&](int64_t lo, int64_t hi, int) { std::sort(v.begin() + lo, v.begin() + hi); });
-    V w((size_t)n);
-    for (int width = 1; width < nt; width *= 2) {
-        std::vector<std::thread> th;
-        for (int t = 0; t < nt; t += 2 * width) {
-            const int64_t a = cut[(size_t)t], m = cut[(size_t)std::min(t + width, nt)],
-                          b = cut[(size_t)std::min(t + 2 * width, nt)];
-            auto merge = [&v, &w, a, m, b] {
-                std::merge(v.begin() + a, v.begin() + m, v.begin() + m, v.begin() + b,
-                           w.begin() + a);
-            };
-            try {
-                th.emplace_back(merge);
-            }
-            catch (...) {
-                merge();
-            }
-        }
-        for (auto &x : th)
-            x.join();
-        v.swap(w);
-    }
-}
 
 template <int P> struct EopShape {
     static constexpr int NB = P == 1 ? 4 : (P == 2 ? 10 : 20);
@@ -689,7 +657,7 @@ void build_tiles_f32(int p, int nb, std::vector<float> &out)
             }
 }
 
-uint64_t spread3(uint64_t v)
+__host__ __device__ inline uint64_t spread3(uint64_t v)
 {
     v &= 0x1fffffull;
     v = (v | v << 32) & 0x1f00000000ffffull;
@@ -777,6 +745,31 @@ EopLayout::~EopLayout()
     phg::dfree(eperm);
 }
 
+// Morton key of each element's centroid (0.25 times the sum of its vertex
+// coordinates in vertex order), quantised to 21 bits per axis: the rounding
+// steps are spelled out so the key is the one the host formula gives
+__global__ void k_elem_morton(int64_t ne, const double *__restrict__ xyz,
+                              const int32_t *__restrict__ ev, double lo0, double lo1,
+                              double lo2, double sc, uint64_t *key, int32_t *idx)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne)
+        return;
+    const double lo[3] = {lo0, lo1, lo2};
+    uint64_t q[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        double m = 0.0;
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+            m = __dadd_rn(m, xyz[3 * (int64_t)ev[4 * e + v] + c]);
+        const double f = __dmul_rn(__dsub_rn(__dmul_rn(0.25, m), lo[c]), sc);
+        q[c] = (uint64_t)fmin(fmax(f, 0.0), 2097151.0);
+    }
+    key[e] = spread3(q[0]) | spread3(q[1]) << 1 | spread3(q[2]) << 2;
+    idx[e] = (int32_t)e;
+}
+
 //============================================================== C ABI ====
 
 extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
@@ -857,25 +850,37 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
             }
         double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
         const double sc = ext > 0 ? 2097151.0 / ext : 0.0;
-        // (key, element) pairs: sorting them is a stable sort by key
-        HostVec<std::pair<uint64_t, int32_t>> key((size_t)ne);
-        host_parallel(ne, [&](int64_t e0, int64_t e1, int) {
-            for (int64_t e = e0; e < e1; e++) {
-                uint64_t q[3];
-                for (int c = 0; c < 3; c++) {
-                    double m = 0.0;
-                    for (int v = 0; v < 4; v++)
-                        m += xyz[(size_t)(3 * ev[(size_t)(4 * e + v)] + c)];
-                    double f = (0.25 * m - lo[c]) * sc;
-                    q[c] = (uint64_t)std::min(std::max(f, 0.0), 2097151.0);
-                }
-                key[(size_t)e] = {spread3(q[0]) | spread3(q[1]) << 1 | spread3(q[2]) << 2,
-                                  (int32_t)e};
-            }
-        });
-        host_sort(key);
-        for (int64_t e = 0; e < ne; e++)
-            eperm[(size_t)e] = key[(size_t)e].second;
+        // (key, element) pairs radix-sorted on the device: a stable sort by
+        // key, as the elements start in index order
+        uint64_t *dk = nullptr, *dk2 = nullptr;
+        int32_t *di = nullptr, *di2 = nullptr;
+        void *tmp = nullptr;
+        size_t tmp_bytes = 0;
+        auto done_sort = [&]() {
+            dfree(dk); dfree(dk2); dfree(di); dfree(di2); dfree(tmp);
+        };
+        if ((rc = dmalloc((void **)&dk, (size_t)ne * 8)) ||
+            (rc = dmalloc((void **)&dk2, (size_t)ne * 8)) ||
+            (rc = dmalloc((void **)&di, (size_t)ne * 4)) ||
+            (rc = dmalloc((void **)&di2, (size_t)ne * 4))) {
+            done_sort();
+            return rc;
+        }
+        hipLaunchKernelGGL(k_elem_morton, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s,
+                           ne, (const double *)sp->d_verts, (const int32_t *)sp->d_elem_verts,
+                           lo[0], lo[1], lo[2], sc, dk, di);
+        if (hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, dk, dk2, di, di2, (int)ne,
+                                               0, 63, s) != hipSuccess ||
+            (rc = dmalloc(&tmp, tmp_bytes)) ||
+            hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, dk, dk2, di, di2, (int)ne, 0,
+                                               63, s) != hipSuccess) {
+            done_sort();
+            return rc ? rc : set_error(PHG_HIP_E_HIP, "element order: radix sort failed");
+        }
+        rc = download(eperm.data(), di2, (size_t)ne * 4);
+        done_sort();
+        if (rc)
+            return rc;
     }
     mark("element order");
     // element DOF map in the solver numbering, Dirichlet DOFs -1
