@@ -273,13 +273,16 @@ int phg_hip_pcg_multi(int32_t nmat, phg_hip_mat *const *A,
 
 /* Matrix-free FP32 operator for the inner iterations of
  * phg_hip_pcg_reliable / phg_hip_pcg_multi (no reference counterpart).
- * For the eliminated matrix A of a scalar stiffness problem (pb->dim 1, no
- * mass term; P1-P3 on the space sp, kappa as given to phg_hip_assemble) it
- * stores per element the six coefficients kappa_e vol_e grad lambda_a .
- * grad lambda_b (a <= b in 1..3), and applies K_e = sum_m c_m T^m (the stiffness of
- * quad.c:5260 phgQuadGradBasAGradBas, factored through fixed reference
- * tensors T^m) to x on the matrix cores (bf16 MFMA, operands split into
- * three bf16 parts so products keep FP32 precision).  Elements are taken in
+ * For the eliminated matrix A of a scalar problem (pb->dim 1; P1-P3 on the
+ * space sp, kappa as given to phg_hip_assemble, optional mass term
+ * pb->mass_coef) it stores per element the six coefficients
+ * kappa_e vol_e grad lambda_a . grad lambda_b (a <= b in 1..3), plus
+ * mass_coef vol_e with a mass term, and applies
+ * K_e = sum_m c_m T^m (the stiffness of quad.c:5260 phgQuadGradBasAGradBas
+ * and the mass of quad.c:5582 phgQuadBasDotBas, factored through fixed
+ * reference tensors T^m) to x on the matrix cores (P1 / P2: FP32 MFMA; P3:
+ * bf16 MFMA with operands split into three bf16 parts so products keep FP32
+ * precision).  Elements are taken in
  * blocks of consecutive elements (Morton order); each row's sum over its
  * elements runs in a fixed order (in LDS within a block, then over blocks
  * for rows shared between blocks), so results are deterministic.  Dirichlet

@@ -150,7 +150,7 @@ __host__ __device__ inline void face_verts(int k, int &a, int &b, int &c)
 }
 
 // lagrange.c:80 / :160 / :392 (full-block evaluation)
-__device__ inline void bas_values(int p, const double *lam, double *v)
+__host__ __device__ inline void bas_values(int p, const double *lam, double *v)
 {
     double L0 = lam[0], L1 = lam[1], L2 = lam[2], L3 = lam[3];
     if (p == 1) {
@@ -1570,8 +1570,8 @@ int element_pass(phg_hip_space *sp, const phg_hip_problem *pb,
 // a 4 x 4 Gram matrix of rank 3 would add).  Element i of the output is
 // mesh element eperm[i].
 __global__ void k_eop_coef(int64_t ne, const double *verts, const int32_t *ev,
-                           const double *kappa, const int32_t *eperm,
-                           float *coef)
+                           const double *kappa, const int32_t *eperm, int nc,
+                           double mass, float *coef)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ne)
@@ -1589,8 +1589,10 @@ __global__ void k_eop_coef(int64_t ne, const double *verts, const int32_t *ev,
     int m = 0;
     for (int a = 1; a < 4; a++)
         for (int b = a; b < 4; b++, m++)
-            coef[6 * i + m] = (float)(sc * (J[4 * a] * J[4 * b] + J[4 * a + 1] * J[4 * b + 1] +
-                                            J[4 * a + 2] * J[4 * b + 2]));
+            coef[nc * i + m] = (float)(sc * (J[4 * a] * J[4 * b] + J[4 * a + 1] * J[4 * b + 1] +
+                                             J[4 * a + 2] * J[4 * b + 2]));
+    if (nc > 6)
+        coef[nc * i + 6] = (float)(mass * volume(x));
 }
 
 }  // namespace
@@ -1602,7 +1604,9 @@ namespace phg {
 // 2(p-1) (exact for the integrand; quad.c rules, lagrange.c gradients), so
 // that sum_m c_m(e) T^m_ij = kappa_e vol_e sum_q w_q grad phi_i . grad phi_j,
 // the element stiffness of quad.c:5260 phgQuadGradBasAGradBas.
-// T is [6][nb][nb], pairs m in the order (1,1) (1,2) (1,3) (2,2) (2,3) (3,3).
+// T is [7][nb][nb], pairs m in the order (1,1) (1,2) (1,3) (2,2) (2,3) (3,3),
+// then T^6_ij = sum_q w_q phi_i phi_j on the rule of order 2p, so that
+// vol_e T^6 is the element mass matrix of quad.c:5582 phgQuadBasDotBas.
 void eop_reference_tensors(int p, double *T)
 {
     const int nb = p == 1 ? 4 : (p == 2 ? 10 : 20);
@@ -1632,16 +1636,29 @@ void eop_reference_tensors(int p, double *T)
                         t += S[(((size_t)(3 * b + a) * nb) + i) * nb + j];
                     T[((size_t)m * nb + i) * nb + j] = t;
                 }
+    make_rule(2 * p, q);
+    double *M = T + (size_t)6 * nb * nb;
+    for (int k = 0; k < nb * nb; k++)
+        M[k] = 0.0;
+    for (int k = 0; k < q.np; k++) {
+        double b[NB];
+        bas_values(p, q.pts[k], b);
+        for (int i = 0; i < nb; i++)
+            for (int j = 0; j < nb; j++)
+                M[(size_t)i * nb + j] += q.w[k] * (b[j] * b[i]);
+    }
 }
 
 int eop_coefficients(const phg_hip_space *sp, const double *d_kappa,
-                     const int32_t *d_eperm, float *d_coef, hipStream_t s)
+                     const int32_t *d_eperm, double mass_coef, float *d_coef,
+                     hipStream_t s)
 {
     if (sp->nelem == 0)
         return PHG_HIP_OK;
     hipLaunchKernelGGL(k_eop_coef, dim3(nblk(sp->nelem, 256)), dim3(256), 0, s,
                        sp->nelem, (const double *)sp->d_verts,
-                       (const int32_t *)sp->d_elem_verts, d_kappa, d_eperm, d_coef);
+                       (const int32_t *)sp->d_elem_verts, d_kappa, d_eperm,
+                       mass_coef != 0.0 ? 7 : 6, mass_coef, d_coef);
     PHG_HIP_CHECK(hipGetLastError());
     return PHG_HIP_OK;
 }

@@ -1,12 +1,14 @@
-// Matrix-free FP32 operator of a scalar stiffness matrix, element products on
-// the matrix cores, row sums in LDS.
+// Matrix-free FP32 operator of a scalar stiffness (+ mass) matrix, element
+// products on the matrix cores, row sums in LDS.
 //
 // For an affine tetrahedron the stiffness of quad.c:5260
 // (phgQuadGradBasAGradBas with a P0 coefficient) factors as
 //     K_e = sum_{m=0..5} c_m(e) T^m,
 //     c_m(e) = kappa_e vol_e (grad lambda_a . grad lambda_b),  m = (a <= b),
 // over the independent barycentric coordinates a, b = 1..3, with reference
-// tensors T^m fixed per order (assemble.hip computes both).  Applying the
+// tensors T^m fixed per order (assemble.hip computes both).  A mass term
+// a phgQuadBasDotBas (quad.c:5582) is the seventh product, c_6 = a vol_e and
+// T^6 the reference mass matrix, in slot 6 of the tiles.  Applying the
 // operator to x is then, per element,
 //     y_e[i] = sum_m c_m(e) (T^m x_e)[i],
 // and T^m x_e for 16 elements at a time is a matrix product: rows (i, m) of
@@ -16,7 +18,7 @@
 // for FP32 x) and accumulating the six products whose order is at most
 // 2^-16 relative (t0x0, t0x1, t1x0, t0x2, t1x1, t2x0) in FP32; the dropped
 // terms are below 2^-24.  One 16-row output tile holds the slots m (8 rows,
-// 6 used) of two local indices i = 2t, 2t + 1 for 16 elements; the A
+// 6 used, 7 with a mass term) of two local indices i = 2t, 2t + 1 for 16 elements; the A
 // fragments (T parts) of all tiles stay in LDS for the kernel, the B
 // fragments (x parts) are written to LDS per 16 elements.
 //
@@ -45,6 +47,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cmath>
 #include <chrono>
 #include <memory>
 #include <numeric>
@@ -59,7 +62,9 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int EB = 512;           // work-items per work-group (8 wavefronts)
 constexpr int EW = EB / 64;
-constexpr int NC = 6;             // coefficients per element
+constexpr int NREF = 7;           // reference tensors: six stiffness, one mass
+// coefficients per element: the six stiffness ones, then the mass term's
+template <bool MASS> constexpr int kNc = MASS ? 7 : 6;
 constexpr int FB = 256;           // work-items of the fix-up kernel
 constexpr uint32_t KD_NONE = 0xffffffffu;
 
@@ -171,12 +176,13 @@ struct BlockView {
     const uint16_t *rs;       // [nent + nblk] slot starts, block b at eb_row[b] + b
     const int32_t *ppos;      // [nent] partial position of a shared row
     const uint32_t *idx;      // [nelem][4][W] index chunks of the four lane groups
-    const float *coef;        // [nelem][6]
+    const float *coef;        // [nelem][6 or 7]
     const void *tiles;        // A fragments: bf16 (split path) or FP32
     float *part;              // partials of shared rows
     const int32_t *drows;     // [ndir] Dirichlet rows, spread evenly over the blocks
     int64_t ndir;
     const float *diag;        // FP32 diagonal (Dirichlet rows: y = diag x)
+    float msum[10];           // mass term: row sum of T^6 for the local indices of tile t
 };
 
 // work-group total of one double per work-item, to out[blockIdx.x]
@@ -235,7 +241,7 @@ template <int ET, int W> struct Round {
     float cf[ET][4];
 };
 
-template <int NB, int W, int ET>
+template <int NB, int W, int ET, bool MASS>
 __device__ inline void load_round(const BlockView &m, int32_t e0, int32_t ne, int w, int n,
                                   int g, int rd, Round<ET, W> &r)
 {
@@ -248,12 +254,13 @@ __device__ inline void load_round(const BlockView &m, int32_t e0, int32_t ne, in
 #pragma unroll
         for (int u = 0; u < W; u++)
             r.ch[et][u] = ld32(m.idx, base + u);
-        // coefficients m = 4 (g % 2) + q; for odd g the slots 6, 7 read the
-        // next element's (finite: the array has two entries of padding) and
-        // meet all-zero rows of the T tiles
+        // coefficients m = 4 (g % 2) + q; for odd g the slots past the
+        // element's last coefficient (7, and 6 without the mass term) read
+        // the next element's (finite: the array has two entries of padding)
+        // and meet all-zero rows of the T tiles
 #pragma unroll
         for (int q = 0; q < 4; q++)
-            r.cf[et][q] = ld32(m.coef, e * NC + 4 * (g & 1) + q);
+            r.cf[et][q] = ld32(m.coef, e * kNc<MASS> + 4 * (g & 1) + q);
     }
 }
 
@@ -265,26 +272,28 @@ __device__ inline uint32_t chunk_half(const uint32_t (&ch)[W], int h)
 }
 
 // x of a lane's local indices, minus x_0 of its element (local index 0,
-// held by lane n of the wavefront).  Every T^m annihilates constants, so
-// T^m x_e = T^m (x_e - x_0 1) exactly; in FP32 the products then carry
-// rounding errors relative to the variation of x over the element rather
-// than to x itself -- the local differences of a smooth x are exact
-// (Sterbenz) and the Krylov iterations converge as on the assembled FP32
-// matrix.  Dirichlet and absent local indices (x = 0, row KMAX of xs) and
-// local indices past NB (zero rows of T) take -x_0 alike.
+// held by lane n of the wavefront), and x_0.  Every stiffness tensor T^m
+// annihilates constants, so T^m x_e = T^m (x_e - x_0 1) exactly; in FP32 the
+// products then carry rounding errors relative to the variation of x over
+// the element rather than to x itself -- the local differences of a smooth x
+// are exact (Sterbenz) and the Krylov iterations converge as on the
+// assembled FP32 matrix.  The mass tensor does not: its product takes
+// x_0 (T^6 1)_i back after the contraction.  Dirichlet and absent local
+// indices (x = 0, row KMAX of xs) and local indices past NB (zero rows of T)
+// take -x_0 alike.
 template <int KMAX, int NJ, int ET, int W>
 __device__ inline void load_relative(const float *xs, const uint32_t (&ch)[ET][W], int n,
-                                     float (&xv)[ET][NJ])
+                                     float (&xv)[ET][NJ], float (&x0)[ET])
 {
 #pragma unroll
     for (int et = 0; et < ET; et++) {
 #pragma unroll
         for (int u = 0; u < NJ; u++)
             xv[et][u] = xs[min(chunk_half(ch[et], u), (uint32_t)KMAX)];
-        const float x0 = __shfl(xv[et][0], n, 64);
+        x0[et] = __shfl(xv[et][0], n, 64);
 #pragma unroll
         for (int u = 0; u < NJ; u++)
-            xv[et][u] -= x0;
+            xv[et][u] -= x0[et];
     }
 }
 
@@ -295,7 +304,7 @@ template <int P> constexpr int kWaves = P == 3 ? 2 : (P == 2 ? 6 : 4);
 // y = A x over the element blocks: interior and Dirichlet rows of y,
 // partials of shared rows, the work-group's part of (x, y) over all rows
 // in part_out
-template <int P>
+template <int P, bool MASS>
 __global__ void __launch_bounds__(EB, kWaves<P>)
 k_eop_block(BlockView m, const float *__restrict__ x, float *__restrict__ y,
             double *part_out, const int32_t *done, const int32_t *mode, int32_t want)
@@ -370,15 +379,33 @@ k_eop_block(BlockView m, const float *__restrict__ x, float *__restrict__ y,
         for (int i = 0; i < RPT; i++)
             xr[i] = x[rw[i]];
         Round<ET, W> ra, rb;
-        load_round<NB, W>(m, e0, ne, w, n, g, 0, ra);
+        load_round<NB, W, ET, MASS>(m, e0, ne, w, n, g, 0, ra);
         // one round: x from the block's rows, B fragments, products, slots
         auto compute = [&](int rd, Round<ET, W> &cur) {
             float yv[ET][NO];
+            // x of the lane's local indices (split path j = NJ g + u, FP32
+            // path j = 4 u + g; Dirichlet and none: row KMAX of xs, which
+            // holds 0), relative to x_0
+            float xv[ET][NJ], xz[ET];
+            load_relative<S::KMAX, NJ, ET, W>(xs, cur.ch, n, xv, xz);
+            // mass term: the lanes of slots 4..7 (odd g, c_6 their third
+            // coefficient) add c_6 x_0 (T^6 1)_i to their part of row i
+            float mc[ET];
+#pragma unroll
+            for (int et = 0; et < ET; et++)
+                mc[et] = (g & 1) ? cur.cf[et][2] * xz[et] : 0.0f;
+            // contract tile t's products with c_m, then add the two row
+            // groups of local index 2 t + g / 2 (lanes g and g ^ 1)
+            auto contract = [&](int et, int t, const f32x4 &a) {
+                float v = cur.cf[et][0] * a[0] + cur.cf[et][1] * a[1] + cur.cf[et][2] * a[2] +
+                          cur.cf[et][3] * a[3];
+                if constexpr (MASS)
+                    v += mc[et] * m.msum[t];
+                v += __shfl_xor(v, 16, 64);
+                if ((t & 1) == (g & 1))
+                    yv[et][t >> 1] = v;
+            };
             if constexpr (SPLIT) {
-                // x of the lane's local indices j = NJ g + u (Dirichlet and
-                // none: row KMAX of xs, which holds 0), relative to x_0
-                float xv[ET][NJ];
-                load_relative<S::KMAX, NJ, ET, W>(xs, cur.ch, n, xv);
                 // split into bf16 parts through the wavefront's LDS rows, one
                 // tile at a time, into B fragments
                 bf16x8 bf[ET][KS];
@@ -418,24 +445,13 @@ k_eop_block(BlockView m, const float *__restrict__ x, float *__restrict__ y,
                             acc[et] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bf[et][s2],
                                                                               acc[et], 0, 0, 0);
                     }
-                    // contract with c_m, then add the two row groups of local
-                    // index 2 t + g / 2 (lanes g and g ^ 1)
 #pragma unroll
-                    for (int et = 0; et < ET; et++) {
-                        float v = cur.cf[et][0] * acc[et][0] + cur.cf[et][1] * acc[et][1] +
-                                  cur.cf[et][2] * acc[et][2] + cur.cf[et][3] * acc[et][3];
-                        v += __shfl_xor(v, 16, 64);
-                        if ((t & 1) == (g & 1))
-                            yv[et][t >> 1] = v;
-                    }
+                    for (int et = 0; et < ET; et++)
+                        contract(et, t, acc[et]);
                 }
             }
             else {
-                // x of the lane's local indices j = 4 u + g (Dirichlet and
-                // none: row KMAX of xs, which holds 0), relative to x_0, are
-                // the B fragments
-                float xv[ET][NJ];
-                load_relative<S::KMAX, NJ, ET, W>(xs, cur.ch, n, xv);
+                // the relative x are the B fragments
 #pragma unroll
                 for (int t = 0; t < NT; t++) {
                     f32x4 acc[ET];
@@ -449,13 +465,8 @@ k_eop_block(BlockView m, const float *__restrict__ x, float *__restrict__ y,
                             acc[et] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[t][u], xv[et][u],
                                                                            acc[et], 0, 0, 0);
 #pragma unroll
-                    for (int et = 0; et < ET; et++) {
-                        float v = cur.cf[et][0] * acc[et][0] + cur.cf[et][1] * acc[et][1] +
-                                  cur.cf[et][2] * acc[et][2] + cur.cf[et][3] * acc[et][3];
-                        v += __shfl_xor(v, 16, 64);
-                        if ((t & 1) == (g & 1))
-                            yv[et][t >> 1] = v;
-                    }
+                    for (int et = 0; et < ET; et++)
+                        contract(et, t, acc[et]);
                 }
             }
             // element results into their slots; slot SMAX takes the
@@ -500,10 +511,10 @@ k_eop_block(BlockView m, const float *__restrict__ x, float *__restrict__ y,
             const int ntile = (ne + 15) / 16;
             const int nround = (ntile + EW * ET - 1) / (EW * ET);
             for (int rd = 0; rd < nround; rd += 2) {
-                load_round<NB, W>(m, e0, ne, w, n, g, rd + 1, rb);
+                load_round<NB, W, ET, MASS>(m, e0, ne, w, n, g, rd + 1, rb);
                 compute(rd, ra);
                 const bool last = rd + 2 >= nround;
-                load_round<NB, W>(m, last ? e0n : e0, last ? nen : ne, w, n, g,
+                load_round<NB, W, ET, MASS>(m, last ? e0n : e0, last ? nen : ne, w, n, g,
                                   last ? 0 : rd + 2, ra);
                 compute(rd + 1, rb);
             }
@@ -607,10 +618,12 @@ double bf16_value(uint16_t h)
     return (double)f;
 }
 
-// A fragments [nb / 2][ks][64][8] of the stacked split T (see above)
-void build_tiles(int p, int nb, int ks, std::vector<uint16_t> &out)
+// A fragments [nb / 2][ks][64][8] of the stacked split T (see above); slot
+// m = 6 holds the mass tensor when mass is set, else zeros
+void build_tiles(int p, int nb, int ks, bool mass, std::vector<uint16_t> &out)
 {
-    std::vector<double> T((size_t)NC * nb * nb);
+    const int nm = mass ? 7 : 6;
+    std::vector<double> T((size_t)NREF * nb * nb);
     phg::eop_reference_tensors(p, T.data());
     // three-part split of every T entry
     std::vector<uint16_t> part((size_t)3 * T.size());
@@ -632,7 +645,7 @@ void build_tiles(int p, int nb, int ks, std::vector<uint16_t> &out)
                     const int k = 32 * s + 8 * (l / 16) + u;
                     const int q = k / nb, j = k % nb;
                     uint16_t v = 0;
-                    if (m < NC && q < 6)
+                    if (m < nm && q < 6)
                         v = part[(size_t)kTPart[q] * T.size() +
                                  ((size_t)m * nb + i) * nb + j];
                     out[(((size_t)t * ks + s) * 64 + l) * 8 + u] = v;
@@ -641,10 +654,11 @@ void build_tiles(int p, int nb, int ks, std::vector<uint16_t> &out)
 
 // FP32 A values [nb / 2][nj][64] of the FP32 path: lane l of tile t and
 // K step u holds T^m[i][j], row R = l % 16 -> m = R % 8, i = 2 t + R / 8,
-// j = 4 u + l / 16 (0 for m >= 6 or j >= nb)
-void build_tiles_f32(int p, int nb, std::vector<float> &out)
+// j = 4 u + l / 16 (0 for j >= nb, for m = 7, and for m = 6 without mass)
+void build_tiles_f32(int p, int nb, bool mass, std::vector<float> &out)
 {
-    std::vector<double> T((size_t)NC * nb * nb);
+    const int nm = mass ? 7 : 6;
+    std::vector<double> T((size_t)NREF * nb * nb);
     phg::eop_reference_tensors(p, T.data());
     const int nt = nb / 2, nj = (nb + 3) / 4;
     out.assign((size_t)nt * nj * 64, 0.0f);
@@ -652,7 +666,7 @@ void build_tiles_f32(int p, int nb, std::vector<float> &out)
         for (int u = 0; u < nj; u++)
             for (int l = 0; l < 64; l++) {
                 const int R = l % 16, m = R % 8, i = 2 * t + R / 8, j = 4 * u + l / 16;
-                if (m < NC && j < nb)
+                if (m < nm && j < nb)
                     out[((size_t)t * nj + u) * 64 + l] = (float)T[((size_t)m * nb + i) * nb + j];
             }
 }
@@ -668,11 +682,12 @@ __host__ __device__ inline uint64_t spread3(uint64_t v)
     return v;
 }
 
-template <int P>
+template <int P, bool MASS>
 int grid_block()
 {
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_eop_block<P>,
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu,
+                                                     (const void *)k_eop_block<P, MASS>,
                                                      EB, 0) != hipSuccess ||
         per_cu <= 0) {
         (void)hipGetLastError();
@@ -687,11 +702,18 @@ void launch_block(const EopData *eo, const float *x, const float *diag, float *y
                   double *part_out, const int32_t *done, const int32_t *mode, int32_t want,
                   hipStream_t s)
 {
-    const BlockView m{eo->nblk, eo->eb_elem, eo->eb_row, eo->eb_nint, eo->rows, eo->rs,
-                      eo->ppos, eo->idx, eo->coef, (const void *)eo->tiles, eo->part,
-                      eo->fx_rows + eo->nfx, eo->ndir, diag};
-    hipLaunchKernelGGL(k_eop_block<P>, dim3(eo->grid_b), dim3(EB), 0, s, m, x, y,
-                       part_out, done, mode, want);
+    BlockView m{eo->nblk, eo->eb_elem, eo->eb_row, eo->eb_nint, eo->rows, eo->rs,
+                eo->ppos, eo->idx, eo->coef, (const void *)eo->tiles, eo->part,
+                eo->fx_rows + eo->nfx, eo->ndir, diag, {}};
+    if (eo->mass) {
+        for (int t = 0; t < eo->nb / 2; t++)
+            m.msum[t] = eo->msum[t];
+        hipLaunchKernelGGL((k_eop_block<P, true>), dim3(eo->grid_b), dim3(EB), 0, s, m, x, y,
+                           part_out, done, mode, want);
+    }
+    else
+        hipLaunchKernelGGL((k_eop_block<P, false>), dim3(eo->grid_b), dim3(EB), 0, s, m, x,
+                           y, part_out, done, mode, want);
 }
 
 }  // namespace
@@ -739,6 +761,7 @@ EopLayout::~EopLayout()
     phg::dfree(ppos);
     phg::dfree(idx);
     phg::dfree(tiles);
+    phg::dfree(tiles_mass);
     phg::dfree(fx_rows);
     phg::dfree(fx_off);
     phg::dfree(fxc);
@@ -783,12 +806,12 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
         return rc;
     if (A == nullptr || sp == nullptr || pb == nullptr)
         return set_error(PHG_HIP_E_ARG, "phg_hip_mat_element_operator: null argument");
-    if (!A->eliminated || A->dim != 1 || pb->dim != 1 || pb->mass_coef != 0.0 ||
-        A->n != sp->ndof || sp->order < 1 || sp->order > 3 ||
-        (pb->use_kappa && kappa == nullptr))
+    if (!A->eliminated || A->dim != 1 || pb->dim != 1 || A->n != sp->ndof ||
+        sp->order < 1 || sp->order > 3 || (pb->use_kappa && kappa == nullptr) ||
+        !std::isfinite(pb->mass_coef))
         return set_error(PHG_HIP_E_ARG,
                          "phg_hip_mat_element_operator: needs the eliminated matrix "
-                         "of a scalar stiffness problem (no mass term) on this space");
+                         "of a scalar problem (stiffness, optional mass term) on this space");
     // PHG_HIP_SETUP_TRACE=1 prints the time of each set-up phase to stderr
     static const bool trace = getenv("PHG_HIP_SETUP_TRACE") != nullptr;
     auto t_last = std::chrono::steady_clock::now();
@@ -1087,19 +1110,39 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
             fx_rows.push_back((int32_t)i);
     const int64_t ndir = (int64_t)fx_rows.size() - nfx;
     int ks = p == 1 ? EopShape<1>::KS : (p == 2 ? EopShape<2>::KS : EopShape<3>::KS);
-    // A fragments: bf16 parts (P3) or FP32 values (P1, P2), as bytes
-    std::vector<uint8_t> tiles;
-    if (p == 3) {
-        std::vector<uint16_t> tb;
-        build_tiles(p, nb, ks, tb);
-        tiles.resize(tb.size() * 2);
-        memcpy(tiles.data(), tb.data(), tiles.size());
+    // A fragments: bf16 parts (P3) or FP32 values (P1, P2), as bytes;
+    // without and with the mass tensor
+    std::vector<uint8_t> tiles[2];
+    for (int ms = 0; ms < 2; ms++) {
+        if (p == 3) {
+            std::vector<uint16_t> tb;
+            build_tiles(p, nb, ks, ms != 0, tb);
+            tiles[ms].resize(tb.size() * 2);
+            memcpy(tiles[ms].data(), tb.data(), tiles[ms].size());
+        }
+        else {
+            std::vector<float> tf;
+            build_tiles_f32(p, nb, ms != 0, tf);
+            tiles[ms].resize(tf.size() * 4);
+            memcpy(tiles[ms].data(), tf.data(), tiles[ms].size());
+        }
     }
-    else {
-        std::vector<float> tf;
-        build_tiles_f32(p, nb, tf);
-        tiles.resize(tf.size() * 4);
-        memcpy(tiles.data(), tf.data(), tiles.size());
+    // row sums of the mass tensor; the local indices 2 t, 2 t + 1 of a tile
+    // are of one kind (vertex, edge or face nodes: the kinds start at even
+    // local indices), so their sums agree and one value serves the tile
+    {
+        std::vector<double> T((size_t)NREF * nb * nb);
+        eop_reference_tensors(p, T.data());
+        for (int t = 0; t < nb / 2; t++) {
+            double r[2] = {0.0, 0.0};
+            for (int h = 0; h < 2; h++)
+                for (int j = 0; j < nb; j++)
+                    r[h] += T[((size_t)6 * nb + 2 * t + h) * nb + j];
+            if (std::fabs(r[0] - r[1]) > 1e-14)
+                return set_error(PHG_HIP_E_STATE,
+                                 "element operator set-up: mass row sums of tile %d differ", t);
+            L.msum[t] = (float)r[0];
+        }
     }
     L.p = p;
     L.nb = nb;
@@ -1125,7 +1168,8 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
         (rc = put((void **)&L.rs, rs.data(), rs.size() * 2)) ||
         (rc = put((void **)&L.ppos, ppos.data(), ppos.size() * 4)) ||
         (rc = put((void **)&L.idx, idx.data(), idx.size() * 4)) ||
-        (rc = put((void **)&L.tiles, tiles.data(), tiles.size())) ||
+        (rc = put((void **)&L.tiles, tiles[0].data(), tiles[0].size())) ||
+        (rc = put((void **)&L.tiles_mass, tiles[1].data(), tiles[1].size())) ||
         (rc = put((void **)&L.fx_rows, fx_rows.data(), fx_rows.size() * 4)) ||
         (rc = put((void **)&L.fx_off, fx_off.data(), fx_off.size() * 4)) ||
         (rc = put((void **)&L.eperm, eperm.data(), (size_t)ne * 4)))
@@ -1133,7 +1177,8 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
     mark("uploads");
     // grids: every block group resident; partial sums of both kernels fit
     // the solvers' partial-sum segment
-    const int gb = p == 1 ? grid_block<1>() : (p == 2 ? grid_block<2>() : grid_block<3>());
+    const int gb = p == 1 ? grid_block<1, false>()
+                          : (p == 2 ? grid_block<2, false>() : grid_block<3, false>());
     L.grid_b = (int)std::max<int64_t>(1, std::min<int64_t>({nblk, (int64_t)gb,
                                                              (int64_t)EOP_MAX_PART}));
     L.grid_f = (int)std::max<int64_t>(1, std::min<int64_t>((nfx + FB - 1) / FB, 4096));
@@ -1193,7 +1238,9 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
     eo->rs = lay->rs;
     eo->ppos = lay->ppos;
     eo->idx = lay->idx;
-    eo->tiles = lay->tiles;
+    eo->mass = pb->mass_coef != 0.0;
+    eo->tiles = eo->mass ? lay->tiles_mass : lay->tiles;
+    eo->msum = lay->msum;
     eo->npart = lay->npart;
     eo->fx_rows = lay->fx_rows;
     eo->fx_off = lay->fx_off;
@@ -1201,27 +1248,34 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
     eo->ndir = lay->ndir;
     eo->fxc = lay->fxc;
     eo->grid_b = lay->grid_b;
+    if (eo->mass) {
+        const int gm = lay->p == 1 ? grid_block<1, true>()
+                                   : (lay->p == 2 ? grid_block<2, true>() : grid_block<3, true>());
+        eo->grid_b = (int)std::min<int64_t>(eo->grid_b, std::max(gm, 1));
+    }
     eo->grid_f = lay->grid_f;
     eo->lay = lay;
     A->eop = eo;
     double *dk = nullptr;
+    const int nc = eo->mass ? kNc<true> : kNc<false>;
     auto fail = [&](int code) {
         dfree(dk);
         free_eop(A);
         return code;
     };
     if ((rc = dmalloc((void **)&eo->part, (size_t)std::max<int64_t>(lay->npart, 1) * 4)) ||
-        (rc = dmalloc((void **)&eo->coef, ((size_t)ne * NC + 2) * 4)))
+        (rc = dmalloc((void **)&eo->coef, ((size_t)ne * nc + 2) * 4)))
         return fail(rc);
     if (pb->use_kappa &&
         ((rc = dmalloc((void **)&dk, (size_t)ne * 8)) ||
          (rc = upload(dk, kappa, (size_t)ne * 8))))
         return fail(rc);
-    if ((rc = eop_coefficients(sp, pb->use_kappa ? dk : nullptr, lay->eperm, eo->coef, s)))
+    if ((rc = eop_coefficients(sp, pb->use_kappa ? dk : nullptr, lay->eperm, pb->mass_coef,
+                               eo->coef, s)))
         return fail(rc);
     // two entries of padding, read (and multiplied by zero) for the last
     // element
-    if (hipMemsetAsync(eo->coef + ne * NC, 0, 2 * sizeof(float), s) != hipSuccess)
+    if (hipMemsetAsync(eo->coef + ne * nc, 0, 2 * sizeof(float), s) != hipSuccess)
         return fail(hip_fail(hipGetLastError(), "element operator set-up", __FILE__, __LINE__));
     hipError_t e = hipStreamSynchronize(s);
     dfree(dk);

@@ -81,7 +81,9 @@ struct EopLayout {
     uint16_t *rs = nullptr;
     int32_t *ppos = nullptr;
     uint32_t *idx = nullptr;
-    uint8_t *tiles = nullptr;
+    uint8_t *tiles = nullptr;        // A fragments of the stiffness tensors
+    uint8_t *tiles_mass = nullptr;   // the same with the mass tensor in slot 6
+    float msum[10] = {};             // row sum of the mass tensor, local index 2 t (= 2 t + 1)
     int32_t *fx_rows = nullptr, *fx_off = nullptr;
     uint32_t *fxc = nullptr;
     int32_t *eperm = nullptr;        // [nelem] Morton position -> element
@@ -164,8 +166,10 @@ struct EopData {
     int32_t *ppos = nullptr;          // [nent] partial slot of a shared row
     uint32_t *idx = nullptr;          // [nelem][4][w] lane-group chunks of 16-bit row
                                       // indices and slots (0xffff: none)
-    float *coef = nullptr;            // [nelem][6] + 2 entries of padding
+    int mass = 0;                     // mass term: 7 coefficients per element
+    float *coef = nullptr;            // [nelem][6 or 7] + 2 entries of padding
     uint8_t *tiles = nullptr;         // A fragments (bf16 parts for P3, FP32 for P1 / P2)
+    const float *msum = nullptr;      // [nb / 2] mass row sums per tile (layout's)
     float *part = nullptr;            // [npart] partial sums of shared rows
     int64_t npart = 0;
     // fix-up: nfx shared rows (partial ranges fx_off) then ndir Dirichlet rows
@@ -313,9 +317,12 @@ __device__ inline void xcd_sweep(int64_t n, int64_t &first, int64_t &end,
 // matrix-free operator (eop.hip; reference data from assemble.hip)
 constexpr int EOP_MAX_PART = 2048;   // partial sums one apply writes, at most
 constexpr uint32_t EOP_NOT_SHARED = 0xffffffffu;
+// T: [7][nb][nb], the six stiffness tensors and the mass tensor
 void eop_reference_tensors(int p, double *T);
+// per element 6 coefficients (mass_coef 0) or 7 (the mass term last)
 int eop_coefficients(const phg_hip_space *sp, const double *d_kappa,
-                     const int32_t *d_eperm, float *d_coef, hipStream_t s);
+                     const int32_t *d_eperm, double mass_coef, float *d_coef,
+                     hipStream_t s);
 // launches y = A x of the operator in the solver numbering (diag: FP32
 // diagonal, for the Dirichlet rows) and writes the eo->grid_b partial sums
 // of (x, y) into part_out.  The rows shared between element blocks are set

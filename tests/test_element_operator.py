@@ -14,9 +14,9 @@ from tests._cases import elasticity_case, poisson_case
 pytestmark = pytest.mark.gpu
 
 
-def _system(pa, kind, p, n):
+def _system(pa, kind, p, n, mass=0.0):
     c = poisson_case(n, p, kind)
-    pb = pa.Problem.poisson(use_kappa=(kind == "kappa"))
+    pb = pa.Problem.poisson(use_kappa=(kind == "kappa"), mass_coef=mass)
     sp = pa.Space(c["grid"], p, bdry=c["bdry"])
     A = pa.assemble(sp, pb, kappa=c["kappa"], f=c["f"], g=c["g"])
     A.eliminate()
@@ -59,6 +59,99 @@ def test_apply_matches_assembled_matrix(hip, kind, p, n):
 # ranges whose ends fall inside a block, so each range ends with a short block
 MULTI_BLOCK_CASES = [("kappa", 1, 16), ("kappa", 2, 9), ("sin", 3, 6), ("kappa", 1, 32),
                      ("kappa", 1, 33)]
+
+
+# -div(kappa grad u) + a u = f (simplest.c:70 with a = mass_coef): the mass
+# tensor in slot 6 of the element products, P1-P3, and one mesh of many
+# blocks; a = 50 makes the mass term dominate the P2 kappa matrix
+MASS_CASES = [("kappa", 1, 6, 1.0), ("sin", 1, 16, 3.0), ("sin", 2, 3, 1.0),
+              ("kappa", 2, 4, 50.0), ("kappa", 2, 9, 0.5), ("sin", 3, 2, 1.0),
+              ("kappa", 3, 3, 10.0)]
+
+
+@pytest.mark.parametrize("kind,p,n,a", MASS_CASES)
+def test_apply_with_mass_term(hip, kind, p, n, a):
+    """Against the FP64 SpMV of the assembled matrix (stiffness plus a times
+    the mass matrix of quad.c:5582, whose entries the CSR parity tests pin
+    bit for bit to the oracle).  The products act on x - x_0 per element
+    and the mass term takes x_0 (T^6 1)_i back: the third trial adds a large
+    constant to x, which only the mass term sees -- a missing or misplaced
+    correction gives errors of order 1 there."""
+    pa = hip
+    c, pb, sp, A = _system(pa, kind, p, n, mass=a)
+    A.element_operator(sp, pb, kappa=c["kappa"])
+    rng = np.random.default_rng(30 + p)
+    bd = c["bdry"].astype(bool)
+    for trial in range(3):
+        xv = rng.standard_normal(A.n)
+        if trial == 1:
+            xv = np.where(bd, 0.0, 1.0)
+        elif trial == 2:
+            xv = xv + 1000.0
+        x, y64, y32 = pa.Vec(A.n, xv), pa.Vec(A.n), pa.Vec(A.n)
+        A.matvec(x, y64)
+        A.element_apply(x, y32)
+        ya, yb = y64.numpy(), y32.numpy()
+        rel = np.linalg.norm(yb - ya) / np.linalg.norm(ya)
+        assert rel < 2e-6, (trial, rel)
+        assert np.max(np.abs(yb - ya)) < 2e-5 * np.max(np.abs(ya)), trial
+    eb, _ = A.element_bytes()
+    lay = A.element_layout()
+    assert eb > 0 and lay["interior"] + lay["shared"] + lay["dirichlet"] == A.n
+
+
+@pytest.mark.parametrize("p,n", [(1, 8), (2, 4), (3, 3)])
+def test_reliable_solver_with_mass_term(hip, p, n):
+    """Mixed-precision PCG with FP32 iterations on the operator, against the
+    oracle's FP64 PCG on its own assembly of the same problem."""
+    pa = hip
+    a = 2.5
+    c, pb, sp, A = _system(pa, "kappa", p, n, mass=a)
+    Ao = orc.assemble(orc.problem(p, use_kappa=True, mass_coef=a), c["verts"],
+                      c["elem_verts"], c["map"], c["ndof"], c["bdry"], c["g"], c["f"],
+                      c["kappa"])
+    orc.eliminate(Ao, c["bdry"])
+    xo, _, _ = orc.pcg(Ao, Ao.rhs, rtol=1e-13, btol=1e-13, maxit=100000)
+    x_csr = pa.Vec(A.n)
+    n_csr, _, _ = pa.pcg_reliable(A, A.rhs(), x_csr, rtol=1e-10, btol=1e-10)
+    A.element_operator(sp, pb, kappa=c["kappa"])
+    x = pa.Vec(A.n)
+    nits, upd, _ = pa.pcg_reliable(A, A.rhs(), x, rtol=1e-10, btol=1e-10)
+    print("mass", p, n, "iterations csr", n_csr, "element operator", nits, flush=True)
+    assert upd >= 1
+    assert abs(nits - n_csr) <= max(3, n_csr // 10)
+    assert np.linalg.norm(x.numpy() - xo) / np.linalg.norm(xo) < 1e-8
+
+
+def test_layout_shared_with_and_without_mass(hip):
+    """A mass matrix on a space whose layout a stiffness matrix built takes
+    that layout and its own tiles and coefficients; both operators match
+    their matrices, and the stiffness one is unchanged by the other's
+    set-up."""
+    pa = hip
+    c = poisson_case(6, 2, "kappa")
+    sp = pa.Space(c["grid"], 2, bdry=c["bdry"])
+    outs = []
+    mats = []
+    for a in (0.0, 4.0):
+        pb = pa.Problem.poisson(use_kappa=True, mass_coef=a)
+        A = pa.assemble(sp, pb, kappa=c["kappa"], f=c["f"], g=c["g"])
+        A.eliminate()
+        A.element_operator(sp, pb, kappa=c["kappa"])
+        mats.append(A)
+    xv = np.random.default_rng(8).standard_normal(mats[0].n)
+    x = pa.Vec(mats[0].n, xv)
+    for A in mats:
+        y64, y32 = pa.Vec(A.n), pa.Vec(A.n)
+        A.matvec(x, y64)
+        A.element_apply(x, y32)
+        ya, yb = y64.numpy(), y32.numpy()
+        assert np.linalg.norm(yb - ya) / np.linalg.norm(ya) < 2e-6
+        outs.append(yb)
+    assert not np.allclose(outs[0], outs[1])
+    y = pa.Vec(mats[0].n)
+    mats[0].element_apply(x, y)
+    assert np.array_equal(y.numpy(), outs[0])
 
 
 @pytest.mark.parametrize("kind,p,n", MULTI_BLOCK_CASES)
@@ -177,9 +270,6 @@ def test_element_operator_rejects_other_problems(hip):
     with pytest.raises(pa.PhgHipError):
         A.element_operator(sp, pb)
     c, pb, sp, A = _system(pa, "sin", 2, 2)
-    mass = pa.Problem.poisson(use_kappa=False, mass_coef=1.0)
-    with pytest.raises(pa.PhgHipError):
-        A.element_operator(sp, mass)
     with pytest.raises(pa.PhgHipError):
         A.use_element_operator(True)    # not built on this matrix
 
