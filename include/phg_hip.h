@@ -269,6 +269,44 @@ int phg_hip_pcg_multi(int32_t nmat, phg_hip_mat *const *A,
                       double delta, int32_t use_graph, int32_t *nits,
                       double *residual);
 
+/* Re-assembly and solve of several problems in captured HIP graphs (no
+ * reference counterpart; the per-system steps are phg_hip_assemble +
+ * phg_hip_mat_eliminate on a fixed pattern, then phg_hip_pcg_reliable).
+ * System i is the eliminated matrix A[i] that phg_hip_assemble made on the
+ * space sp[i] for the problem pb[i].  kappa[i] (nelem entries; ignored
+ * unless pb[i]->use_kappa), f[i] (ndof * dim) and g[i] (n) are device
+ * vectors with the coefficients, the load and the Dirichlet values as
+ * phg_hip_assemble takes them; the batch keeps their addresses, and every
+ * phg_hip_batch_run assembles A[i] again from their current contents (the
+ * element kernel, PHG's row insertion and elimination into A[i]'s arrays,
+ * its solver copy, FP32 copies and layouts, and the element operator's
+ * coefficients when A[i] has one), then solves A[i] x[i] = rhs(A[i]) with
+ * the reliable-update PCG from the current x[i] (the initial guess; zero it
+ * for a cold start).  The pattern must not change: coefficients that make
+ * element-matrix entries exactly zero where they were not (or the reverse)
+ * make phg_hip_batch_run return PHG_HIP_E_ARG, and A[i] must then be
+ * assembled again.  create runs one full assembly pass with the vectors'
+ * current contents to check the pattern against A[i], and captures two
+ * graphs: refill + solver start + 16 iterations of every system (a stream
+ * per system), and 16 further iterations.  run launches the first and then
+ * the second until every system has converged on the device, with two
+ * launches in flight: the host reads each launch's solver states from pinned
+ * memory after it ends, while the next runs, and never waits on an idle
+ * GPU.  Returns the largest iteration count; nits, residual per system
+ * (optional).  While a batch exists its matrices must not be eliminated
+ * again, given a new element operator or a new FP32 format (run returns
+ * PHG_HIP_E_STATE); layouts the refill does not update (SELL, block copies)
+ * are dropped and rebuilt from the new values when next used.  nmat <= 16. */
+typedef struct phg_hip_batch phg_hip_batch;
+int phg_hip_batch_create(int32_t nmat, phg_hip_space *const *sp,
+                         const phg_hip_problem *const *pb, phg_hip_mat *const *A,
+                         const phg_hip_vec *const *kappa, const phg_hip_vec *const *f,
+                         const phg_hip_vec *const *g, phg_hip_vec *const *x,
+                         int32_t maxit, double rtol, double btol, double atol,
+                         double delta, phg_hip_batch **batch);
+int phg_hip_batch_run(phg_hip_batch *batch, int32_t *nits, double *residual);
+int phg_hip_batch_destroy(phg_hip_batch *batch);
+
 /*------------------------------------------------- matrix-free operator */
 
 /* Matrix-free FP32 operator for the inner iterations of

@@ -146,6 +146,11 @@ _sig(hip_lib, "phg_hip_pcg_block", C.c_int,
 _sig(hip_lib, "phg_hip_pcg_multi", C.c_int,
      [_i32, _P(_vp), _P(_vp), _P(_vp), _i32, _dbl, _dbl, _dbl, _dbl, _i32, _P(_i32),
       _P(_dbl)])
+_sig(hip_lib, "phg_hip_batch_create", C.c_int,
+     [_i32, _P(_vp), _P(_P(Problem)), _P(_vp), _P(_vp), _P(_vp), _P(_vp), _P(_vp), _i32,
+      _dbl, _dbl, _dbl, _dbl, _P(_vp)])
+_sig(hip_lib, "phg_hip_batch_run", C.c_int, [_vp, _P(_i32), _P(_dbl)])
+_sig(hip_lib, "phg_hip_batch_destroy", C.c_int, [_vp])
 _sig(hip_lib, "phg_hip_mat_element_operator", C.c_int, [_vp, _vp, _P(Problem), _P(_dbl)])
 _sig(hip_lib, "phg_hip_mat_use_element_operator", C.c_int, [_vp, _i32])
 _sig(hip_lib, "phg_hip_element_apply", C.c_int, [_vp, _vp, _vp])
@@ -564,6 +569,50 @@ def pcg_multi(As, bs, xs, maxit=100000, rtol=1e-10, btol=1e-10, atol=0.0, delta=
                                      int(use_graph), _ptr(nits, _i32), _ptr(res, _dbl)),
            "pcg_multi")
     return nits, res
+
+
+class Batch:
+    """Re-assembly + mixed-precision solve of several systems in captured HIP
+    graphs (phg_hip_batch_*).  systems: [(space, problem, A, kappa, f, g, x)]
+    with A the eliminated matrix assembled on space for problem and kappa
+    (or None), f, g, x device Vecs; run() assembles every A again from the
+    vectors' current contents and solves A x = rhs(A) from the current x.
+    Returns (nits[k], residual[k])."""
+
+    def __init__(self, systems, maxit=100000, rtol=1e-10, btol=1e-10, atol=0.0,
+                 delta=0.1):
+        k = len(systems)
+        if not 1 <= k <= 16:
+            raise ValueError("1 <= number of systems <= 16 required")
+        self._keep = systems
+        self._pbs = [pb for _, pb, _, _, _, _, _ in systems]
+        null = _vp()
+        spa = (_vp * k)(*[sp._h for sp, _, _, _, _, _, _ in systems])
+        pba = (_P(Problem) * k)(*[C.pointer(pb) for pb in self._pbs])
+        aa = (_vp * k)(*[A._h for _, _, A, _, _, _, _ in systems])
+        ka = (_vp * k)(*[(kp._h if kp is not None else null)
+                         for _, _, _, kp, _, _, _ in systems])
+        fa = (_vp * k)(*[f._h for _, _, _, _, f, _, _ in systems])
+        ga = (_vp * k)(*[g._h for _, _, _, _, _, g, _ in systems])
+        xa = (_vp * k)(*[x._h for _, _, _, _, _, _, x in systems])
+        h = _vp()
+        _check(hip_lib.phg_hip_batch_create(k, spa, pba, aa, ka, fa, ga, xa, maxit, rtol,
+                                            btol, atol, delta, C.byref(h)),
+               "phg_hip_batch_create")
+        self._h = h
+        self.k = k
+
+    def run(self):
+        nits = np.zeros(self.k, dtype=np.int32)
+        res = np.zeros(self.k)
+        _check(hip_lib.phg_hip_batch_run(self._h, _ptr(nits, _i32), _ptr(res, _dbl)),
+               "phg_hip_batch_run")
+        return nits, res
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            hip_lib.phg_hip_batch_destroy(self._h)
+            self._h = None
 
 
 def spmm_tune(A: "Mat", reps: int = 10) -> dict:

@@ -1353,6 +1353,51 @@ int launch_element(const phg_hip_space *sp, const phg_hip_problem *pb,
     return PHG_HIP_OK;
 }
 
+// PHG's row insertion over all rows (w.ovf[0] must be 0): 64/G rows per
+// wavefront, G the smallest of 16, 32, 64 holding an element row; the rows
+// past its table (high-valence vertices of refined P3 meshes: up to 343
+// columns) then one per wavefront with a table of 2048
+int launch_row_walk(const WalkArgs &wk, hipStream_t s)
+{
+    const int M = wk.N * wk.dim;
+    const unsigned grid = (unsigned)std::min<int64_t>((wk.n + 3) / 4, 1 << 16);
+    if (wk.n == 0)
+        return PHG_HIP_OK;
+    if (M <= 16)
+        hipLaunchKernelGGL((k_row_walk_hash<16, 128>), dim3(grid), dim3(64), 0, s, wk);
+    else if (M <= 32)
+        hipLaunchKernelGGL((k_row_walk_hash<32, 128>), dim3(grid), dim3(64), 0, s, wk);
+    else   // M <= 60 for P1-P3, scalar or vector (check_problem)
+        hipLaunchKernelGGL((k_row_walk_hash<64, 512>), dim3(grid), dim3(64), 0, s, wk);
+    hipLaunchKernelGGL((k_row_walk_hash<64, 2048, true>), dim3(2048), dim3(64), 0, s, wk);
+    PHG_HIP_CHECK(hipGetLastError());
+    return PHG_HIP_OK;
+}
+
+// scratch segment offsets of the row walk: scr_off = exclusive scan of
+// cnt[r] * M (cnt: incidence sizes), total in *nscr
+int scratch_offsets(int64_t n, const int32_t *cnt, int M, int64_t **scr_off, int64_t *nscr)
+{
+    hipStream_t s = phg::ctx()->stream;
+    int32_t *count = nullptr;
+    int rc;
+    *scr_off = nullptr;
+    if ((rc = phg::dmalloc((void **)&count, (size_t)(n + 1) * sizeof(int32_t))) ||
+        (rc = phg::dmalloc((void **)scr_off, (size_t)(n + 1) * sizeof(int64_t)))) {
+        phg::dfree(count);
+        return rc;
+    }
+    hipLaunchKernelGGL(k_scale_i32, dim3(nblk(n, 256)), dim3(256), 0, s, n, cnt, (int32_t)M,
+                       count);
+    rc = scan_i32(n, count, *scr_off, nscr, s);
+    phg::dfree(count);
+    if (rc) {
+        phg::dfree(*scr_off);
+        *scr_off = nullptr;
+    }
+    return rc;
+}
+
 int check_problem(const phg_hip_space *sp, const phg_hip_problem *pb)
 {
     if (sp == nullptr || pb == nullptr)
@@ -1483,7 +1528,6 @@ int space_incidence(phg_hip_space *sp, int dim, AsmWorkspace **out)
     *out = w;
     if (w->dim == dim)
         return PHG_HIP_OK;
-    hipStream_t s = phg::ctx()->stream;
     phg::dfree(w->cnt);
     phg::dfree(w->inc);
     phg::dfree(w->inc_off);
@@ -1492,22 +1536,8 @@ int space_incidence(phg_hip_space *sp, int dim, AsmWorkspace **out)
     w->inc_off = w->scr_off = nullptr;
     w->dim = 0;
     int rc;
-    if ((rc = build_incidence(sp, dim, &w->cnt, &w->inc_off, &w->inc)))
-        return rc;
-    const int64_t n = sp->ndof * dim;
-    const int M = sp->nbas * dim;
-    int32_t *count = nullptr;
-    if ((rc = phg::dmalloc((void **)&count, (size_t)(n + 1) * sizeof(int32_t))) ||
-        (rc = phg::dmalloc((void **)&w->scr_off, (size_t)(n + 1) * sizeof(int64_t)))) {
-        phg::dfree(count);
-        return rc;
-    }
-    // count[r] = cnt[r] * M (cnt holds the incidence sizes after the fill)
-    hipLaunchKernelGGL(k_scale_i32, dim3(nblk(n, 256)), dim3(256), 0, s, n,
-                       (const int32_t *)w->cnt, (int32_t)M, count);
-    rc = scan_i32(n, count, w->scr_off, &w->nscr, s);
-    phg::dfree(count);
-    if (rc)
+    if ((rc = build_incidence(sp, dim, &w->cnt, &w->inc_off, &w->inc)) ||
+        (rc = scratch_offsets(sp->ndof * dim, w->cnt, sp->nbas * dim, &w->scr_off, &w->nscr)))
         return rc;
     w->dim = dim;
     return PHG_HIP_OK;
@@ -1735,6 +1765,9 @@ extern "C" int phg_hip_space_destroy(phg_hip_space *sp)
 {
     if (sp == nullptr)
         return PHG_HIP_OK;
+    if (sp->batch_refs > 0)
+        return phg::set_error(PHG_HIP_E_STATE, "phg_hip_space_destroy: the space is in a "
+                                               "batch; destroy the batch first");
     phg::dfree(sp->d_verts);
     phg::dfree(sp->d_elem_verts);
     phg::dfree(sp->d_map);
@@ -1856,19 +1889,8 @@ extern "C" int phg_hip_assemble(phg_hip_space *sp, const phg_hip_problem *pb,
                             (void **)&wk.ovf)))
             goto done;
         PHG_HIP_CHECK(hipMemsetAsync(wk.ovf, 0, sizeof(int32_t), s));
-        // 64/G rows per wavefront, G the smallest of 16, 32, 64 holding
-        // an element row; the rows past its table (high-valence vertices of
-        // refined P3 meshes: up to 343 columns) then one per wavefront with
-        // a table of 2048
-        const unsigned grid = (unsigned)std::min<int64_t>((n + 3) / 4, 1 << 16);
-        if (M <= 16)
-            hipLaunchKernelGGL((k_row_walk_hash<16, 128>), dim3(grid), dim3(64), 0, s, wk);
-        else if (M <= 32)
-            hipLaunchKernelGGL((k_row_walk_hash<32, 128>), dim3(grid), dim3(64), 0, s, wk);
-        else   // M <= 60 for P1-P3, scalar or vector (check_problem)
-            hipLaunchKernelGGL((k_row_walk_hash<64, 512>), dim3(grid), dim3(64), 0, s, wk);
-        hipLaunchKernelGGL((k_row_walk_hash<64, 2048, true>), dim3(2048), dim3(64), 0, s, wk);
-        PHG_HIP_CHECK(hipGetLastError());
+        if ((rc = launch_row_walk(wk, s)))
+            goto done;
     }
     tr.mark("row walk");
     if ((rc = ws_buffer(sp, WS_ROWPTR, (size_t)(n + 1) * sizeof(int64_t),
@@ -2082,6 +2104,10 @@ extern "C" int phg_hip_mat_destroy(phg_hip_mat *A)
 {
     if (A == nullptr)
         return PHG_HIP_OK;
+    if (A->batch_refs > 0)
+        return phg::set_error(PHG_HIP_E_STATE,
+                              "phg_hip_mat_destroy: the matrix is in a batch; destroy the "
+                              "batch first");
     phg::dfree(A->d_rowptr);
     phg::dfree(A->d_cols);
     phg::dfree(A->d_vals);
@@ -2403,3 +2429,216 @@ fail:
     phg_hip_mat_destroy(A);
     return rc;
 }
+
+//================================================================ refill
+// Re-assembly of an eliminated matrix for new coefficient values (kappa, the
+// load, the Dirichlet values): the kernels of phg_hip_assemble and
+// phg_hip_mat_eliminate -- element pass, PHG's row insertion, scans,
+// compaction, elimination -- run into buffers the plan owns and into the
+// matrix's PHG-order arrays, which the plan re-allocates to a capacity that
+// bounds every pattern (each row's incidence times the element size).  The
+// pattern may change with the values (PHG drops exact zeros, matvec.c:5460,
+// and whether a cancelling sum rounds to zero depends on kappa), so the
+// scans run on the device and nothing waits for their totals:
+// refill_enqueue only launches kernels -- no allocation, no host
+// synchronisation, no memset -- so a HIP graph can capture it.  The new
+// entry count is left at nnz_dev.
+namespace {
+
+__global__ void k_refill_reset(int32_t *ovf)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        ovf[0] = 0;
+}
+
+// out[0..n] = exclusive scan of in[0..n-1] with the total at out[n] (the
+// kernels of scan_i32 on a caller-provided block-sum buffer of nb + 1)
+void scan_i32_enqueue(int64_t n, const int32_t *in, int64_t *out, int64_t *bsum,
+                      hipStream_t s)
+{
+    const int64_t nb = (n + 1 + 1023) / 1024;
+    hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(1024), 0, s, n, in, out, bsum);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, nb, bsum, bsum + nb);
+    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(1024), 0, s, n, out,
+                       (const int64_t *)bsum, (const int64_t *)(bsum + nb));
+}
+
+}  // namespace
+
+namespace phg {
+
+struct RefillPlan {
+    phg_hip_mat *A = nullptr;
+    const phg_hip_space *sp = nullptr;
+    phg_hip_problem pb;
+    ElemArgs ea;
+    WalkArgs wk;
+    int64_t n = 0, cap = 0;
+    // owned device buffers: element matrices and loads, incidence, row-walk
+    // scratch, the matrix before elimination (row pointers in both widths,
+    // columns, values, right-hand side), the eliminated row counts and
+    // pointers, and the scans' block sums
+    double *K = nullptr, *B = nullptr;
+    int32_t *cnt = nullptr, *inc = nullptr;
+    int64_t *inc_off = nullptr, *scr_off = nullptr;
+    int32_t *scols = nullptr, *count = nullptr, *ovf = nullptr, *ecount = nullptr;
+    double *svals = nullptr;
+    int64_t *rp64 = nullptr, *nrp64 = nullptr, *bsum = nullptr;
+    int32_t *rp32 = nullptr, *cols = nullptr;
+    double *vals = nullptr, *rhs = nullptr;
+    ~RefillPlan()
+    {
+        dfree(K); dfree(B); dfree(cnt); dfree(inc); dfree(inc_off); dfree(scr_off);
+        dfree(scols); dfree(count); dfree(ovf); dfree(ecount); dfree(svals); dfree(rp64);
+        dfree(nrp64); dfree(bsum); dfree(rp32); dfree(cols); dfree(vals); dfree(rhs);
+    }
+};
+
+void refill_plan_destroy(RefillPlan *p) { delete p; }
+
+const int64_t *refill_nnz(const RefillPlan *p) { return p->nrp64 + p->n; }
+
+int refill_enqueue(const RefillPlan *P, hipStream_t s)
+{
+    const int64_t n = P->n;
+    phg_hip_mat *A = P->A;
+    int rc;
+    if ((rc = launch_element(P->sp, &P->pb, P->ea, s)))
+        return rc;
+    hipLaunchKernelGGL(k_refill_reset, dim3(1), dim3(64), 0, s, P->ovf);
+    if ((rc = launch_row_walk(P->wk, s)))
+        return rc;
+    scan_i32_enqueue(n, P->count, P->rp64, P->bsum, s);
+    hipLaunchKernelGGL(k_compact, dim3(nblk(n, RPB)), dim3(256), 0, s, n,
+                       (const int64_t *)P->scr_off, (const int32_t *)P->scols,
+                       (const double *)P->svals, (const int64_t *)P->rp64, P->cols, P->vals);
+    hipLaunchKernelGGL(k_to_int32, dim3(nblk(n + 1, 256)), dim3(256), 0, s, n,
+                       (const int64_t *)P->rp64, P->rp32);
+    hipLaunchKernelGGL(k_elim_count, dim3(nblk(n, RPB)), dim3(256), 0, s, n,
+                       (const int32_t *)P->rp32, (const int32_t *)P->cols,
+                       (const uint8_t *)A->d_bdry_row, P->ecount);
+    scan_i32_enqueue(n, P->ecount, P->nrp64, P->bsum, s);
+    hipLaunchKernelGGL(k_elim_fill, dim3(nblk(n, RPB)), dim3(256), 0, s, n,
+                       (const int32_t *)P->rp32, (const int32_t *)P->cols,
+                       (const double *)P->vals, (const double *)P->rhs,
+                       (const uint8_t *)A->d_bdry_row, (const int64_t *)P->nrp64, A->d_cols,
+                       A->d_vals, A->d_rhs, A->d_diag);
+    hipLaunchKernelGGL(k_to_int32, dim3(nblk(n + 1, 256)), dim3(256), 0, s, n,
+                       (const int64_t *)P->nrp64, A->d_rowptr);
+    PHG_HIP_CHECK(hipGetLastError());
+    return PHG_HIP_OK;
+}
+
+int refill_plan_create(phg_hip_space *sp, const phg_hip_problem *pb, phg_hip_mat *A,
+                       const double *d_kappa, const double *d_f, const double *d_g,
+                       RefillPlan **out)
+{
+    int rc;
+    *out = nullptr;
+    if ((rc = check_problem(sp, pb)))
+        return rc;
+    if (A == nullptr || !A->eliminated || A->space_uid != sp->uid || A->dim != pb->dim ||
+        A->n != sp->ndof * pb->dim || A->d_bdry_row == nullptr)
+        return set_error(PHG_HIP_E_ARG, "refill: the matrix must be the eliminated matrix "
+                                        "phg_hip_assemble made on this space for this problem");
+    if ((pb->use_kappa && d_kappa == nullptr) || d_f == nullptr || d_g == nullptr)
+        return set_error(PHG_HIP_E_ARG, "refill: kappa (when used), f and g are required");
+    hipStream_t s = ctx()->stream;
+    const int N = sp->nbas, dim = pb->dim, M = N * dim;
+    const int64_t n = A->n, ne = sp->nelem;
+    std::unique_ptr<RefillPlan> P(new RefillPlan());
+    P->A = A;
+    P->sp = sp;
+    P->pb = *pb;
+    P->n = n;
+    int64_t nscr = 0;
+    const int64_t nb = (n + 1 + 1023) / 1024;
+    if ((rc = dmalloc((void **)&P->K, (size_t)ne * M * M * sizeof(double) + 8)) ||
+        (rc = dmalloc((void **)&P->B, (size_t)ne * M * sizeof(double) + 8)) ||
+        (rc = build_incidence(sp, dim, &P->cnt, &P->inc_off, &P->inc)) ||
+        (rc = scratch_offsets(n, P->cnt, M, &P->scr_off, &nscr)) ||
+        (rc = dmalloc((void **)&P->scols, (size_t)(nscr + 1) * sizeof(int32_t))) ||
+        (rc = dmalloc((void **)&P->svals, (size_t)(nscr + 1) * sizeof(double))) ||
+        (rc = dmalloc((void **)&P->count, (size_t)(n + 1) * sizeof(int32_t))) ||
+        (rc = dmalloc((void **)&P->ecount, (size_t)(n + 1) * sizeof(int32_t))) ||
+        (rc = dmalloc((void **)&P->ovf, (size_t)(n + 1) * sizeof(int32_t))) ||
+        (rc = dmalloc((void **)&P->rhs, (size_t)(n + 1) * sizeof(double))) ||
+        (rc = dmalloc((void **)&P->rp64, (size_t)(n + 1) * sizeof(int64_t))) ||
+        (rc = dmalloc((void **)&P->nrp64, (size_t)(n + 1) * sizeof(int64_t))) ||
+        (rc = dmalloc((void **)&P->bsum, (size_t)(nb + 1) * sizeof(int64_t))) ||
+        (rc = dmalloc((void **)&P->rp32, (size_t)(n + 1) * sizeof(int32_t))) ||
+        (rc = dmalloc((void **)&P->cols, (size_t)(nscr + 1) * sizeof(int32_t))) ||
+        (rc = dmalloc((void **)&P->vals, (size_t)(nscr + 1) * sizeof(double))))
+        return rc;
+    if (nscr >= (int64_t)INT32_MAX)
+        return set_error(PHG_HIP_E_ARG, "refill: %lld entries bound exceeds int32 CSR",
+                         (long long)nscr);
+    // the matrix's PHG-order columns and values at the capacity
+    if (A->nnz > nscr)
+        return set_error(PHG_HIP_E_STATE, "refill: the matrix exceeds its pattern bound");
+    int32_t *ncols = nullptr;
+    double *nvals = nullptr;
+    if ((rc = dmalloc((void **)&ncols, (size_t)(nscr + 1) * sizeof(int32_t))) ||
+        (rc = dmalloc((void **)&nvals, (size_t)(nscr + 1) * sizeof(double)))) {
+        dfree(ncols);
+        return rc;
+    }
+    if (hipMemcpyAsync(ncols, A->d_cols, (size_t)A->nnz * sizeof(int32_t),
+                       hipMemcpyDeviceToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(nvals, A->d_vals, (size_t)A->nnz * sizeof(double),
+                       hipMemcpyDeviceToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        dfree(ncols);
+        dfree(nvals);
+        return hip_fail(hipGetLastError(), "refill: matrix copy", __FILE__, __LINE__);
+    }
+    P->cap = nscr;
+    ElemArgs &a = P->ea;
+    a.nelem = ne;
+    a.verts = sp->d_verts;
+    a.elem_verts = sp->d_elem_verts;
+    a.map = sp->d_map;
+    a.kappa = pb->use_kappa ? d_kappa : nullptr;
+    a.f = d_f;
+    a.use_kappa = pb->use_kappa;
+    a.mass_coef = pb->mass_coef;
+    a.rhs_order = pb->rhs_order;
+    a.E = pb->E;
+    a.nu = pb->nu;
+    a.K = P->K;
+    a.B = P->B;
+    WalkArgs &w = P->wk;
+    w.n = n;
+    w.N = N;
+    w.dim = dim;
+    w.map = sp->d_map;
+    w.inc_off = P->inc_off;
+    w.inc = P->inc;
+    w.K = P->K;
+    w.B = P->B;
+    w.bdry = A->d_bdry_row;
+    w.g = d_g;
+    w.scr_off = P->scr_off;
+    w.scols = P->scols;
+    w.svals = P->svals;
+    w.count = P->count;
+    w.rhs = P->rhs;
+    w.ovf = P->ovf;
+    // hand the capacity arrays to the matrix (the solver copy aliases the
+    // PHG-order arrays when the matrix is not renumbered: it is dropped and
+    // rebuilt when next needed)
+    free_solver_copy(A);
+    free_sell(A);
+    A->q_stale = 1;
+    dfree(A->d_cols);
+    dfree(A->d_vals);
+    A->d_cols = ncols;
+    A->d_vals = nvals;
+    dfree(A->d_rblk);
+    A->d_rblk = nullptr;
+    A->nrblk = 0;
+    *out = P.release();
+    return PHG_HIP_OK;
+}
+
+}  // namespace phg

@@ -708,9 +708,11 @@ void free_bsr(phg_hip_mat *A)
 // the CSR itself (1x1 blocks) otherwise
 int build_bsr(phg_hip_mat *A)
 {
+    int rc;
+    if ((rc = ensure_solver_copy(A)))
+        return rc;
     if (A->b_dim != 0)
         return PHG_HIP_OK;
-    int rc;
     hipStream_t s = ctx()->stream;
     if ((rc = ensure_fp32_copy(A)))
         return rc;

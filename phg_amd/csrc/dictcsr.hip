@@ -60,28 +60,33 @@ int build_xstage(phg_hip_mat *A)
     return build_dict_layout(A, XS_SHIFT, XS_SLOTS, XS_BLOCK_NNZ, &A->xs);
 }
 
-int build_dict_layout(phg_hip_mat *A, int shift, int max_slots, int block_nnz,
-                      DictLayout *Lo)
+namespace {
+
+// The host plan of a layout over the solver CSR (rp, ci): block tables,
+// dictionary, 16-bit entry codes in their stored (transposed) order, and for
+// every stored entry the position of its value in the solver CSR (-1 for
+// padding), from which the builder takes the values.  Returns 1, or -1 when
+// the matrix is not representable.
+struct DictPlan {
+    std::vector<int32_t> brow, bent, bdict, dict, crp, src;
+    std::vector<uint16_t> idx;
+};
+
+int plan_dict(const std::vector<int32_t> &rp, const std::vector<int32_t> &ci, int64_t n,
+              int shift, int max_slots, int block_nnz, DictPlan &P)
 {
-    if (Lo->state != 0)
-        return PHG_HIP_OK;
     const int32_t lmask = (1 << shift) - 1;
-    const int64_t n = A->n, nnz = A->nnz;
-    int rc;
-    std::vector<int32_t> rp((size_t)n + 1), ci((size_t)nnz);
-    std::vector<double> va((size_t)nnz);
-    if ((rc = download(rp.data(), A->q_rowptr, rp.size() * sizeof(int32_t))) ||
-        (rc = download(ci.data(), A->q_cols, ci.size() * sizeof(int32_t))) ||
-        (rc = download(va.data(), A->q_vals, va.size() * sizeof(double))))
-        return rc;
+    const int64_t nnz = rp[(size_t)n];
     const int64_t nchunk = (n + lmask) >> shift;
     std::vector<int32_t> mark((size_t)nchunk, -1), slot((size_t)nchunk, 0);
-    std::vector<int32_t> brow, bent, bdict, dict, crp((size_t)n + 1);
-    std::vector<int32_t> fresh;
-    std::vector<uint16_t> idx, tidx;
-    std::vector<double> val, tval;
+    std::vector<int32_t> &brow = P.brow, &bent = P.bent, &bdict = P.bdict, &dict = P.dict;
+    std::vector<int32_t> &crp = P.crp, &src = P.src;
+    std::vector<uint16_t> &idx = P.idx;
+    std::vector<int32_t> fresh, tsrc;
+    std::vector<uint16_t> tidx;
+    crp.assign((size_t)n + 1, 0);
     idx.reserve((size_t)nnz + (size_t)nnz / 64 + 64);
-    val.reserve(idx.capacity());
+    src.reserve(idx.capacity());
     int64_t r = 0;
     int32_t stamp = 0;
     while (r < n) {
@@ -94,10 +99,8 @@ int build_dict_layout(phg_hip_mat *A, int shift, int max_slots, int block_nnz,
         while (r < n && nrows < DICT_BLOCK_ROWS) {
             const int64_t j0 = rp[(size_t)r], j1 = rp[(size_t)r + 1];
             const int64_t len = j1 - j0;
-            if (len > block_nnz) {
-                Lo->state = -1;
-                return PHG_HIP_OK;
-            }
+            if (len > block_nnz)
+                return -1;
             if ((int64_t)idx.size() - e0 + len > block_nnz)
                 break;
             // chunks of this row not yet in the block's dictionary
@@ -112,10 +115,8 @@ int build_dict_layout(phg_hip_mat *A, int shift, int max_slots, int block_nnz,
             if (nslot + (int)fresh.size() > max_slots) {
                 for (int32_t c : fresh)
                     mark[(size_t)c] = -1;
-                if (nrows == 0) {
-                    Lo->state = -1;
-                    return PHG_HIP_OK;
-                }
+                if (nrows == 0)
+                    return -1;
                 break;
             }
             for (int32_t c : fresh) {
@@ -127,7 +128,7 @@ int build_dict_layout(phg_hip_mat *A, int shift, int max_slots, int block_nnz,
             for (int64_t j = j0; j < j1; j++) {
                 int32_t c = ci[(size_t)j];
                 idx.push_back((uint16_t)(slot[(size_t)(c >> shift)] << shift | (c & lmask)));
-                val.push_back(va[(size_t)j]);
+                src.push_back((int32_t)j);
             }
             nrows++;
             r++;
@@ -136,7 +137,7 @@ int build_dict_layout(phg_hip_mat *A, int shift, int max_slots, int block_nnz,
             dict.push_back(0);       // padding entries read slot 0
         while (idx.size() % 8) {
             idx.push_back(0);
-            val.push_back(0.0);
+            src.push_back(-1);
         }
         // store each group of up to 512 entries transposed: memory position
         // 8 l + j holds entry j L + l of the group (L = group size / 8), so
@@ -147,45 +148,79 @@ int build_dict_layout(phg_hip_mat *A, int shift, int max_slots, int block_nnz,
             for (int64_t g0 = 0; g0 < m; g0 += DICT_GROUP) {
                 const int64_t gm = std::min<int64_t>(DICT_GROUP, m - g0), L = gm / 8;
                 tidx.assign(idx.begin() + e0 + g0, idx.begin() + e0 + g0 + gm);
-                tval.assign(val.begin() + e0 + g0, val.begin() + e0 + g0 + gm);
+                tsrc.assign(src.begin() + e0 + g0, src.begin() + e0 + g0 + gm);
                 for (int64_t l = 0; l < L; l++)
                     for (int j = 0; j < 8; j++) {
                         idx[(size_t)(e0 + g0 + 8 * l + j)] = tidx[(size_t)(j * L + l)];
-                        val[(size_t)(e0 + g0 + 8 * l + j)] = tval[(size_t)(j * L + l)];
+                        src[(size_t)(e0 + g0 + 8 * l + j)] = tsrc[(size_t)(j * L + l)];
                     }
             }
         }
         stamp++;
-        if (idx.size() > (size_t)INT32_MAX - block_nnz) {
-            Lo->state = -1;
-            return PHG_HIP_OK;
-        }
+        if (idx.size() > (size_t)INT32_MAX - block_nnz)
+            return -1;
     }
     crp[(size_t)n] = (int32_t)idx.size();
     brow.push_back((int32_t)n);
     bent.push_back((int32_t)idx.size());
     bdict.push_back((int32_t)dict.size());
-    std::vector<float> val32(val.size());
-    for (size_t i = 0; i < val.size(); i++)
+    return 1;
+}
+
+int download_pattern(const phg_hip_mat *A, std::vector<int32_t> &rp, std::vector<int32_t> &ci)
+{
+    int rc;
+    rp.resize((size_t)A->n + 1);
+    ci.resize((size_t)A->nnz);
+    if ((rc = download(rp.data(), A->q_rowptr, rp.size() * sizeof(int32_t))) ||
+        (rc = download(ci.data(), A->q_cols, ci.size() * sizeof(int32_t))))
+        return rc;
+    return PHG_HIP_OK;
+}
+
+}  // namespace
+
+int build_dict_layout(phg_hip_mat *A, int shift, int max_slots, int block_nnz,
+                      DictLayout *Lo)
+{
+    if (Lo->state != 0)
+        return PHG_HIP_OK;
+    const int64_t nnz = A->nnz;
+    int rc;
+    std::vector<int32_t> rp, ci;
+    std::vector<double> va((size_t)nnz);
+    if ((rc = download_pattern(A, rp, ci)) ||
+        (rc = download(va.data(), A->q_vals, va.size() * sizeof(double))))
+        return rc;
+    DictPlan P;
+    if (plan_dict(rp, ci, A->n, shift, max_slots, block_nnz, P) < 0) {
+        Lo->state = -1;
+        return PHG_HIP_OK;
+    }
+    const int64_t nb = (int64_t)P.brow.size() - 1, ne = (int64_t)P.idx.size();
+    std::vector<double> val((size_t)ne);
+    std::vector<float> val32((size_t)ne);
+    for (size_t i = 0; i < (size_t)ne; i++) {
+        val[i] = P.src[i] >= 0 ? va[(size_t)P.src[i]] : 0.0;
         val32[i] = (float)val[i];
-    const int64_t nb = (int64_t)brow.size() - 1, ne = (int64_t)idx.size();
+    }
     DictLayout L;
     L.shift = shift;
     L.max_slots = max_slots;
-    if ((rc = dmalloc((void **)&L.rowptr, crp.size() * 4)) ||
-        (rc = dmalloc((void **)&L.blk_row, brow.size() * 4)) ||
-        (rc = dmalloc((void **)&L.blk_ent, bent.size() * 4)) ||
-        (rc = dmalloc((void **)&L.blk_dict, bdict.size() * 4)) ||
-        (rc = dmalloc((void **)&L.dict, dict.size() * 4)) ||
+    if ((rc = dmalloc((void **)&L.rowptr, P.crp.size() * 4)) ||
+        (rc = dmalloc((void **)&L.blk_row, P.brow.size() * 4)) ||
+        (rc = dmalloc((void **)&L.blk_ent, P.bent.size() * 4)) ||
+        (rc = dmalloc((void **)&L.blk_dict, P.bdict.size() * 4)) ||
+        (rc = dmalloc((void **)&L.dict, P.dict.size() * 4)) ||
         (rc = dmalloc((void **)&L.idx, (size_t)ne * 2)) ||
         (rc = dmalloc((void **)&L.val32, (size_t)ne * 4)) ||
         (rc = dmalloc((void **)&L.val64, (size_t)ne * 8)) ||
-        (rc = upload(L.rowptr, crp.data(), crp.size() * 4)) ||
-        (rc = upload(L.blk_row, brow.data(), brow.size() * 4)) ||
-        (rc = upload(L.blk_ent, bent.data(), bent.size() * 4)) ||
-        (rc = upload(L.blk_dict, bdict.data(), bdict.size() * 4)) ||
-        (rc = upload(L.dict, dict.data(), dict.size() * 4)) ||
-        (rc = upload(L.idx, idx.data(), (size_t)ne * 2)) ||
+        (rc = upload(L.rowptr, P.crp.data(), P.crp.size() * 4)) ||
+        (rc = upload(L.blk_row, P.brow.data(), P.brow.size() * 4)) ||
+        (rc = upload(L.blk_ent, P.bent.data(), P.bent.size() * 4)) ||
+        (rc = upload(L.blk_dict, P.bdict.data(), P.bdict.size() * 4)) ||
+        (rc = upload(L.dict, P.dict.data(), P.dict.size() * 4)) ||
+        (rc = upload(L.idx, P.idx.data(), (size_t)ne * 2)) ||
         (rc = upload(L.val32, val32.data(), (size_t)ne * 4)) ||
         (rc = upload(L.val64, val.data(), (size_t)ne * 8))) {
         free_dict_layout(&L);
@@ -193,7 +228,7 @@ int build_dict_layout(phg_hip_mat *A, int shift, int max_slots, int block_nnz,
     }
     L.nblk = nb;
     L.nent = ne;
-    L.ndict = (int64_t)dict.size();
+    L.ndict = (int64_t)P.dict.size();
     L.state = 1;
     *Lo = L;
     return PHG_HIP_OK;

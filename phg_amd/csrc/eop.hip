@@ -823,7 +823,10 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
                 std::chrono::duration<double, std::milli>(t - t_last).count());
         t_last = t;
     };
-    if ((rc = ensure_fp32_copy(A)))
+    if (A->batch_refs > 0)
+        return set_error(PHG_HIP_E_STATE, "phg_hip_mat_element_operator: the matrix is in a "
+                                          "batch, whose graphs hold its operator");
+    if ((rc = ensure_solver_copy(A)) || (rc = ensure_fp32_copy(A)))
         return rc;
     free_eop(A);
     mark("fp32 copy");

@@ -326,6 +326,38 @@ struct SellView {
     int64_t nslice, n;
 };
 
+// PHG's own CSR in storage order, walked in the solver numbering: solver
+// row i is PHG row perm[i] and PHG column c is solver column iperm[c].  The
+// launch shape depends on n alone, so a graph that captured it stays valid
+// when a re-assembly changes the pattern (the batch path).  16 lanes per
+// row, strided partial sums combined by a fixed shuffle tree.
+struct PhgPermView {
+    const int32_t *rp, *ci, *perm, *iperm;
+    int64_t n;
+};
+constexpr int PPV_LANES = 16, PPV_ROWS = BS / PPV_LANES;
+
+template <typename T, typename Epi>
+__device__ inline void spmv_apply(const PhgPermView &m, const T *__restrict__ va,
+                                  const T *__restrict__ x, Epi epi)
+{
+    const int sl = threadIdx.x % PPV_LANES;
+    const int64_t step = (int64_t)gridDim.x * PPV_ROWS;
+    for (int64_t i0 = (int64_t)blockIdx.x * PPV_ROWS; i0 < m.n; i0 += step) {
+        const int64_t i = i0 + threadIdx.x / PPV_LANES;
+        T sum = 0;
+        if (i < m.n) {
+            const int32_t r = m.perm[i], j1 = m.rp[r + 1];
+            for (int32_t j = m.rp[r] + sl; j < j1; j += PPV_LANES)
+                sum += va[j] * x[m.iperm[m.ci[j]]];
+        }
+        for (int o = PPV_LANES / 2; o > 0; o >>= 1)
+            sum += __shfl_xor(sum, o, 64);
+        if (i < m.n && sl == 0)
+            epi((int32_t)i, sum, x[i]);
+    }
+}
+
 template <typename T, int CAP, int MODE, typename Epi>
 __device__ inline void spmv_apply(const CsrBlockView<CAP, MODE> &m,
                                   const T *va, const T *x, Epi epi)
@@ -2021,6 +2053,13 @@ __global__ void __launch_bounds__(BS) k_copy(int64_t n, const double *a,
         b[i] = a[i];
 }
 
+// *st = h (the solver state's start values, passed by value)
+__global__ void k_state_set(State *st, State h)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        *st = h;
+}
+
 __global__ void k_to_f32(int64_t n, const double *a, float *b)
 {
     for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
@@ -2152,6 +2191,14 @@ SellView view_of(const phg_hip_mat *A, SellView *)
 {
     return SellView{A->d_sptr, A->d_swidth, A->d_scols, A->nslice, A->n};
 }
+PhgPermView view_of(const phg_hip_mat *A, PhgPermView *)
+{
+    return PhgPermView{A->d_rowptr, A->d_cols, A->q_perm, A->q_iperm, A->n};
+}
+int64_t units_of(const phg_hip_mat *A, PhgPermView *)
+{
+    return (A->n + PPV_ROWS - 1) / PPV_ROWS;
+}
 template <int CAP, int MODE>
 int64_t units_of(const phg_hip_mat *A, CsrBlockView<CAP, MODE> *)
 {
@@ -2218,6 +2265,13 @@ template <int PF> struct FmtData<XsViewT<PF>> {
 template <int PF> struct FmtData<DictViewT<PF>> {
     static const double *v64(const phg_hip_mat *A) { return A->cd.val64; }
     static const float *v32(const phg_hip_mat *A) { return A->cd.val32; }
+    static const double *d64(const phg_hip_mat *A) { return A->q_diag; }
+    static const float *d32(const phg_hip_mat *A) { return A->q_diag32; }
+    static const int32_t *perm(const phg_hip_mat *A) { return A->q_perm; }
+};
+template <> struct FmtData<PhgPermView> {
+    static const double *v64(const phg_hip_mat *A) { return A->d_vals; }
+    static const float *v32(const phg_hip_mat *) { return nullptr; }
     static const double *d64(const phg_hip_mat *A) { return A->q_diag; }
     static const float *d32(const phg_hip_mat *A) { return A->q_diag32; }
     static const int32_t *perm(const phg_hip_mat *A) { return A->q_perm; }
@@ -3019,7 +3073,7 @@ struct RuPlan {
 template <typename View>
 int ru_prologue(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x,
                 int32_t maxit, double rtol, double btol, double atol,
-                double delta, hipStream_t s, RuPlan<View> *pl)
+                double delta, hipStream_t s, RuPlan<View> *pl, bool capture = false)
 {
     using F = FmtData<View>;
     Workspace *ws = get_ws(A, 1);
@@ -3031,9 +3085,10 @@ int ru_prologue(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x,
     pl->gs = grid_spmv<View>(A, k_ru_spmv<View>);
     pl->mv = view_of(A, (View *)nullptr);
     pl->gs32 = pl->gs;
-    pl->resid_in_update = F::perm(A) == A->q_perm;
     // the operator works in the solver numbering, as every layout but SELL
-    if (A->eop != nullptr && A->eop->on && pl->resid_in_update) {
+    const bool solver_numbering = F::perm(A) == A->q_perm;
+    pl->resid_in_update = solver_numbering && !std::is_same<View, PhgPermView>::value;
+    if (A->eop != nullptr && A->eop->on && solver_numbering) {
         pl->eo = A->eop;
         pl->gs32 = A->eop->grid_b;
     }
@@ -3045,9 +3100,15 @@ int ru_prologue(phg_hip_mat *A, const phg_hip_vec *b, phg_hip_vec *x,
     h.maxit = maxit;
     h.delta = delta;
     h.mode = RU_START;
-    PHG_HIP_CHECK(hipMemcpyAsync(ws->st, &h, sizeof(State),
-                                 hipMemcpyHostToDevice, s));
-    PHG_HIP_CHECK(hipStreamSynchronize(s));    // h lives on this stack
+    if (capture) {
+        // under graph capture: the state is set by a kernel, no host copy
+        hipLaunchKernelGGL(k_state_set, dim3(1), dim3(64), 0, s, ws->st, h);
+    }
+    else {
+        PHG_HIP_CHECK(hipMemcpyAsync(ws->st, &h, sizeof(State),
+                                     hipMemcpyHostToDevice, s));
+        PHG_HIP_CHECK(hipStreamSynchronize(s));    // h lives on this stack
+    }
     pl->bb = b->d;
     pl->xx = x->d;
     pl->perm = F::perm(A);
@@ -3546,6 +3607,8 @@ namespace {
 int ensure_format(phg_hip_mat *A, int fmt, int need32)
 {
     int rc;
+    if ((rc = phg::ensure_solver_copy(A)))
+        return rc;
     // the 4096-entry and wavefront row blocks of the solver copy are split
     // when a layout first uses them
     if ((fmt == FMT_B4K || fmt == FMT_B4K_PAR) && A->q_rblk1k == nullptr &&
@@ -3920,6 +3983,419 @@ extern "C" int phg_hip_pcg_multi(int32_t nmat, phg_hip_mat *const *Av,
     return mx;
 }
 
+//=============================================================== batches
+// Several problems assembled and solved in captured HIP graphs.  Per system
+// (a stream of the pool each) the first graph holds the whole assembly on
+// the device -- element pass, PHG's row insertion, scans, compaction and
+// elimination into the matrix's PHG-order arrays (assemble.hip refill plan),
+// the element operator's coefficients and the diagonal in the solver
+// numbering -- then the start of the reliable-update solver and CH of its
+// slots; the second graph holds CH further slots.  PHG drops exact zeros, so
+// the pattern may change with the coefficients: nothing in the graphs
+// depends on nnz.  The FP32 iterations run on the element operator (its
+// layout is a function of the mesh alone) and the FP64 residuals on PHG's
+// CSR through the solver numbering (PhgPermView).  The slots read their
+// system's state on the device and return at once once it is done: the
+// host launches the first graph and then the second, two launches in
+// flight, and reads the states each launch copies to pinned memory only
+// after that launch has ended, while the next one runs.
+struct phg_hip_batch {
+    struct Item {
+        phg_hip_mat *A = nullptr;       // the caller's matrix, re-assembled in place
+        phg_hip_space *sp = nullptr;
+        phg_hip_problem pb;
+        const double *kappa = nullptr;
+        phg::RefillPlan *plan = nullptr;
+        // the solver's view of A: A's PHG-order arrays (not owned) with the
+        // batch's solver numbering and its inverse, diagonals in that
+        // numbering, a workspace, and A's element operator (handed back to
+        // A when the batch is destroyed)
+        phg_hip_mat S;
+        phg_hip_vec b;                  // view of A's right-hand side
+        phg_hip_vec *x = nullptr;
+        phg_hip_vec *inputs[3] = {};    // kappa (or null), f, g
+        MultiItem run;
+        bool held = false;              // A, sp and the vectors count this batch
+    };
+    std::vector<Item> items;
+    hipGraphExec_t first = nullptr, next = nullptr;
+    void *pinned = nullptr;             // [nmat] State, then [nmat] int64 entry counts
+    int32_t maxit = 0;
+    // the matrix, space and vectors of an item count the batch (+1 / -1)
+    static void hold(Item &it, int d)
+    {
+        it.A->batch_refs += d;
+        it.sp->batch_refs += d;
+        it.x->batch_refs += d;
+        for (phg_hip_vec *v : it.inputs)
+            if (v)
+                v->batch_refs += d;
+    }
+    ~phg_hip_batch()
+    {
+        if (first)
+            (void)hipGraphExecDestroy(first);
+        if (next)
+            (void)hipGraphExecDestroy(next);
+        if (pinned)
+            (void)hipHostFree(pinned);
+        for (Item &it : items) {
+            if (it.held)
+                hold(it, -1);
+            phg::refill_plan_destroy(it.plan);
+            phg_hip_mat &S = it.S;
+            phg::free_workspace(&S);
+            // the operator's coefficients are those of the last assembly, as
+            // are A's values; its numbering is A's solver numbering, which a
+            // rebuild of A's solver copy reproduces
+            if (S.eop != nullptr && it.A != nullptr && it.A->eop == nullptr)
+                it.A->eop = S.eop;
+            else if (S.eop != nullptr)
+                phg::free_eop(&S);
+            S.eop = nullptr;
+            phg::dfree(S.q_perm);
+            phg::dfree(S.q_iperm);
+            phg::dfree(S.q_diag);
+            phg::dfree(S.q_diag32);
+            S.q_perm = S.q_iperm = nullptr;
+            S.q_diag = nullptr;
+            S.q_diag32 = nullptr;
+        }
+    }
+};
+
+namespace {
+
+constexpr int BATCH_CH = 16;   // slots per graph launch (even: the state parity holds)
+
+__global__ void k_perm_init(int64_t n, const int32_t *perm, int32_t *q, int32_t *iq)
+{
+    for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * BS) {
+        const int32_t r = perm ? perm[i] : (int32_t)i;
+        q[i] = r;
+        iq[r] = (int32_t)i;
+    }
+}
+
+// the diagonal in the solver numbering, FP64 and FP32
+__global__ void k_diag_gather(int64_t n, const int32_t *perm, const double *d, double *q,
+                              float *q32)
+{
+    for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * BS) {
+        const double v = d[perm[i]];
+        q[i] = v;
+        q32[i] = (float)v;
+    }
+}
+
+// captures one graph: per system on a pool stream, (assembly + solver start
+// when first) and BATCH_CH slots, then the copy of the state (and, when
+// first, of the entry count) to pinned memory
+int batch_capture(phg_hip_batch *B, bool first, double rtol, double btol, double atol,
+                  double delta, hipGraphExec_t *out)
+{
+    hipStream_t s = phg::ctx()->stream;
+    hipStream_t *pool = nullptr;
+    int npool = 0, rc;
+    if ((rc = phg::aux_streams(&pool, &npool)))
+        return rc;
+    const int nmat = (int)B->items.size();
+    State *hm = (State *)B->pinned;
+    int64_t *hn = (int64_t *)(hm + nmat);
+    hipEvent_t fork = nullptr, join[MULTI_MAX] = {};
+    hipGraph_t graph = nullptr;
+    auto cleanup = [&]() {
+        if (fork)
+            (void)hipEventDestroy(fork);
+        for (int i = 0; i < nmat; i++)
+            if (join[i])
+                (void)hipEventDestroy(join[i]);
+        if (graph)
+            (void)hipGraphDestroy(graph);
+    };
+    hipError_t e = hipEventCreateWithFlags(&fork, hipEventDisableTiming);
+    for (int i = 0; i < nmat && e == hipSuccess; i++)
+        e = hipEventCreateWithFlags(&join[i], hipEventDisableTiming);
+    if (e != hipSuccess) {
+        cleanup();
+        return phg::hip_fail(e, "batch events", __FILE__, __LINE__);
+    }
+    if ((e = hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed)) != hipSuccess) {
+        cleanup();
+        return phg::hip_fail(e, "batch capture", __FILE__, __LINE__);
+    }
+    rc = PHG_HIP_OK;
+    e = hipEventRecord(fork, s);
+    for (int i = 0; i < nmat && e == hipSuccess && rc == PHG_HIP_OK; i++) {
+        phg_hip_batch::Item &it = B->items[(size_t)i];
+        phg_hip_mat &S = it.S;
+        hipStream_t si = pool[i % npool];
+        if ((e = hipStreamWaitEvent(si, fork, 0)) != hipSuccess)
+            break;
+        if (first) {
+            if ((rc = phg::refill_enqueue(it.plan, si)))
+                break;
+            hipLaunchKernelGGL(k_diag_gather, dim3(grid_for(S.n)), dim3(BS), 0, si, S.n,
+                               (const int32_t *)S.q_perm, (const double *)it.A->d_diag,
+                               S.q_diag, S.q_diag32);
+            if ((rc = phg::eop_coefficients(it.sp, it.pb.use_kappa ? it.kappa : nullptr,
+                                            S.eop->lay->eperm, it.pb.mass_coef, S.eop->coef,
+                                            si)))
+                break;
+            auto pl = std::make_shared<RuPlan<PhgPermView>>();
+            if ((rc = ru_prologue<PhgPermView>(&S, &it.b, it.x, B->maxit, rtol, btol, atol,
+                                               delta, si, pl.get(), true)))
+                break;
+            phg_hip_mat *Sp = &S;
+            phg_hip_vec *x = it.x;
+            it.run.slot = [Sp, pl](int64_t k, hipStream_t st) {
+                ru_slot<PhgPermView>(Sp, *pl, k, st, false);
+            };
+            it.run.finish = [Sp, pl, x](hipStream_t st) { ru_finish<PhgPermView>(Sp, *pl, x, st); };
+            it.run.st = pl->ws->st;
+        }
+        for (int k = 0; k < BATCH_CH; k++)
+            it.run.slot(k, si);
+        if ((e = hipMemcpyAsync(hm + i, it.run.st + (BATCH_CH & 1), sizeof(State),
+                                hipMemcpyDeviceToHost, si)) != hipSuccess)
+            break;
+        if (first && (e = hipMemcpyAsync(hn + i, phg::refill_nnz(it.plan), sizeof(int64_t),
+                                         hipMemcpyDeviceToHost, si)) != hipSuccess)
+            break;
+        if ((e = hipEventRecord(join[i], si)) != hipSuccess ||
+            (e = hipStreamWaitEvent(s, join[i], 0)) != hipSuccess)
+            break;
+    }
+    if (e == hipSuccess)
+        e = hipGetLastError();
+    hipError_t e2 = hipStreamEndCapture(s, &graph);
+    if (rc == PHG_HIP_OK && e == hipSuccess && e2 == hipSuccess)
+        e = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+    cleanup();
+    if (rc)
+        return rc;
+    if (e != hipSuccess || e2 != hipSuccess)
+        return phg::hip_fail(e != hipSuccess ? e : e2, "batch graph capture", __FILE__,
+                             __LINE__);
+    return PHG_HIP_OK;
+}
+
+// the solver's view of item it's matrix (see phg_hip_batch::Item)
+int batch_item_setup(phg_hip_batch::Item &it, const phg_hip_vec *f, const phg_hip_vec *g)
+{
+    phg_hip_mat *A = it.A;
+    phg_hip_mat &S = it.S;
+    hipStream_t s = phg::ctx()->stream;
+    const int64_t n = A->n;
+    int rc;
+    if ((rc = phg::ensure_solver_copy(A)) ||
+        (rc = phg::dmalloc((void **)&S.q_perm, (size_t)n * 4 + 4)) ||
+        (rc = phg::dmalloc((void **)&S.q_iperm, (size_t)n * 4 + 4)) ||
+        (rc = phg::dmalloc((void **)&S.q_diag, (size_t)n * 8 + 8)) ||
+        (rc = phg::dmalloc((void **)&S.q_diag32, (size_t)n * 4 + 4)))
+        return rc;
+    hipLaunchKernelGGL(k_perm_init, dim3(grid_for(n)), dim3(BS), 0, s, n,
+                       (const int32_t *)A->q_perm, S.q_perm, S.q_iperm);
+    PHG_HIP_CHECK(hipGetLastError());
+    PHG_HIP_CHECK(hipStreamSynchronize(s));
+    S.eop = A->eop;
+    A->eop = nullptr;
+    S.eop->on = 1;
+    // the plan re-allocates A's columns and values and drops its solver copy
+    if ((rc = phg::refill_plan_create(it.sp, &it.pb, A, it.kappa, f->d, g->d, &it.plan)))
+        return rc;
+    S.n = n;
+    S.dim = A->dim;
+    S.ndof = A->ndof;
+    S.nnz = A->nnz;
+    S.eliminated = 1;
+    S.d_rowptr = A->d_rowptr;
+    S.d_cols = A->d_cols;
+    S.d_vals = A->d_vals;
+    S.d_rhs = A->d_rhs;
+    S.d_diag = A->d_diag;
+    if (get_ws(&S, 1) == nullptr)
+        return phg::set_error(PHG_HIP_E_NOMEM, "workspace allocation failed");
+    it.b.n = n;
+    it.b.d = A->d_rhs;
+    return PHG_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" int phg_hip_batch_create(int32_t nmat, phg_hip_space *const *sp,
+                                    const phg_hip_problem *const *pb,
+                                    phg_hip_mat *const *Av, const phg_hip_vec *const *kappa,
+                                    const phg_hip_vec *const *f, const phg_hip_vec *const *g,
+                                    phg_hip_vec *const *x, int32_t maxit, double rtol,
+                                    double btol, double atol, double delta,
+                                    phg_hip_batch **out)
+{
+    int rc;
+    if ((rc = phg::require_ctx()))
+        return rc;
+    if (out == nullptr || nmat < 1 || nmat > MULTI_MAX || sp == nullptr || pb == nullptr ||
+        Av == nullptr || f == nullptr || g == nullptr || x == nullptr)
+        return phg::set_error(PHG_HIP_E_ARG, "phg_hip_batch_create: 1 <= nmat <= %d and "
+                                             "non-null arrays required", MULTI_MAX);
+    if (!(delta > 0.0 && delta < 1.0))
+        return phg::set_error(PHG_HIP_E_ARG, "delta must lie in (0, 1)");
+    *out = nullptr;
+    for (int i = 0; i < nmat; i++) {
+        phg_hip_mat *A = Av[i];
+        if (sp[i] == nullptr || pb[i] == nullptr || A == nullptr || f[i] == nullptr ||
+            g[i] == nullptr || x[i] == nullptr)
+            return phg::set_error(PHG_HIP_E_ARG, "phg_hip_batch_create: null entry %d", i);
+        for (int j = 0; j < i; j++)
+            if (Av[j] == A)
+                return phg::set_error(PHG_HIP_E_ARG,
+                                      "phg_hip_batch_create: a matrix appears twice");
+        const phg_hip_problem &p = *pb[i];
+        const int64_t ne = sp[i]->nelem, nf = sp[i]->ndof * p.dim;
+        if ((p.use_kappa && (kappa == nullptr || kappa[i] == nullptr ||
+                             kappa[i]->n != ne)) ||
+            f[i]->n != nf || g[i]->n != A->n || x[i]->n != A->n)
+            return phg::set_error(PHG_HIP_E_ARG,
+                                  "phg_hip_batch_create: system %d: kappa needs nelem, f "
+                                  "ndof*dim, g and x n entries", i);
+        if (A->eop == nullptr || A->eop->nelem != ne ||
+            (p.mass_coef != 0.0) != (A->eop->mass != 0) || A->space_uid != sp[i]->uid)
+            return phg::set_error(PHG_HIP_E_ARG,
+                                  "phg_hip_batch_create: system %d needs the eliminated "
+                                  "matrix assembled on its space with its element operator "
+                                  "(phg_hip_mat_element_operator) for this problem", i);
+    }
+    std::unique_ptr<phg_hip_batch> B(new phg_hip_batch());
+    B->items.resize((size_t)nmat);
+    B->maxit = maxit;
+    for (int i = 0; i < nmat; i++) {
+        phg_hip_batch::Item &it = B->items[(size_t)i];
+        const phg_hip_problem &p = *pb[i];
+        it.A = Av[i];
+        it.sp = sp[i];
+        it.pb = p;
+        it.kappa = p.use_kappa ? kappa[i]->d : nullptr;
+        it.x = x[i];
+        it.inputs[0] = p.use_kappa ? const_cast<phg_hip_vec *>(kappa[i]) : nullptr;
+        it.inputs[1] = const_cast<phg_hip_vec *>(f[i]);
+        it.inputs[2] = const_cast<phg_hip_vec *>(g[i]);
+        if ((rc = batch_item_setup(it, f[i], g[i])))
+            return rc;
+    }
+    if (hipHostMalloc(&B->pinned, (size_t)nmat * (sizeof(State) + sizeof(int64_t)) + 64,
+                      hipHostMallocDefault) != hipSuccess)
+        return phg::set_error(PHG_HIP_E_NOMEM, "pinned mailbox allocation failed");
+    PHG_HIP_CHECK(hipStreamSynchronize(phg::ctx()->stream));
+    if ((rc = batch_capture(B.get(), true, rtol, btol, atol, delta, &B->first)) ||
+        (rc = batch_capture(B.get(), false, rtol, btol, atol, delta, &B->next)))
+        return rc;
+    for (auto &it : B->items) {
+        phg_hip_batch::hold(it, +1);
+        it.held = true;
+    }
+    *out = B.release();
+    return PHG_HIP_OK;
+}
+
+extern "C" int phg_hip_batch_run(phg_hip_batch *B, int32_t *nits, double *residual)
+{
+    int rc;
+    if ((rc = phg::require_ctx()))
+        return rc;
+    if (B == nullptr)
+        return phg::set_error(PHG_HIP_E_ARG, "null batch");
+    const int nmat = (int)B->items.size();
+    hipStream_t s = phg::ctx()->stream;
+    State *hm = (State *)B->pinned;
+    const int64_t *hn = (const int64_t *)(hm + nmat);
+    constexpr int D = 2;                 // graph launches in flight
+    hipEvent_t ev[D] = {};
+    auto cleanup = [&]() {
+        for (int k = 0; k < D; k++)
+            if (ev[k])
+                (void)hipEventDestroy(ev[k]);
+    };
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < D && e == hipSuccess; k++)
+        e = hipEventCreateWithFlags(&ev[k], hipEventDisableTiming);
+    int64_t launched = 0, ended = 0;
+    auto launch = [&]() -> hipError_t {
+        hipError_t e1 = hipGraphLaunch(launched == 0 ? B->first : B->next, s);
+        if (e1 == hipSuccess)
+            e1 = hipEventRecord(ev[launched % D], s);
+        launched++;
+        return e1;
+    };
+    const int64_t limit = (2 * (int64_t)B->maxit + 4096) / BATCH_CH + D;
+    bool stuck = false;
+    while (e == hipSuccess && launched < D)
+        e = launch();
+    while (e == hipSuccess) {
+        if ((e = hipEventSynchronize(ev[ended % D])) != hipSuccess)
+            break;
+        ended++;
+        bool all = true;
+        for (int i = 0; i < nmat; i++)
+            all = all && hm[i].done != RUNNING;
+        if (all)
+            break;
+        if (launched > limit) {
+            stuck = true;
+            break;
+        }
+        e = launch();
+    }
+    if (e == hipSuccess && !stuck) {
+        for (auto &it : B->items)
+            it.run.finish(s);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(s);
+    cleanup();
+    if (e != hipSuccess)
+        return phg::hip_fail(e, "batch run", __FILE__, __LINE__);
+    // the matrices: new entry counts; the row blocks of the PHG-order CSR and
+    // any solver copy built since the last run follow the new pattern when
+    // next needed
+    for (int i = 0; i < nmat; i++) {
+        phg_hip_mat *A = B->items[(size_t)i].A;
+        A->nnz = hn[i];
+        B->items[(size_t)i].S.nnz = hn[i];
+        phg::dfree(A->d_rblk);
+        A->d_rblk = nullptr;
+        A->nrblk = 0;
+        phg::free_solver_copy(A);
+        phg::free_sell(A);
+        A->q_stale = 1;
+    }
+    if (stuck)
+        return phg::set_error(PHG_HIP_E_HIP, "batched solve did not finish");
+    int32_t mx = 0;
+    for (int i = 0; i < nmat; i++) {
+        if (hm[i].done == ZERO_DIAG)
+            return phg::set_error(PHG_HIP_E_ZERO_DIAG, "zero diagonal in Jacobi preconditioner");
+        if (nits)
+            nits[i] = hm[i].nits;
+        if (residual)
+            residual[i] = hm[i].res;
+        mx = std::max(mx, hm[i].nits);
+    }
+    return mx;
+}
+
+extern "C" int phg_hip_batch_destroy(phg_hip_batch *B)
+{
+    if (B == nullptr)
+        return PHG_HIP_OK;
+    if (phg::ctx())
+        (void)hipStreamSynchronize(phg::ctx()->stream);
+    delete B;
+    return PHG_HIP_OK;
+}
+
 extern "C" int phg_hip_mat_set_format(phg_hip_mat *A, int32_t fmt64,
                                       int32_t fmt32)
 {
@@ -3958,6 +4434,9 @@ extern "C" int phg_hip_element_apply(const phg_hip_mat *A_, const phg_hip_vec *x
         return phg::set_error(PHG_HIP_E_ARG,
                               "phg_hip_element_apply: operator not built, size "
                               "mismatch or x aliases y");
+    // the operator's Dirichlet rows read the FP32 diagonal
+    if ((rc = phg::ensure_solver_copy(A)) || (rc = phg::ensure_fp32_copy(A)))
+        return rc;
     hipStream_t s = phg::ctx()->stream;
     Workspace *ws = get_ws(A, 1);
     if (ws == nullptr)
@@ -3984,6 +4463,8 @@ extern "C" int phg_hip_element_time(const phg_hip_mat *A_, int32_t reps,
     phg_hip_mat *A = const_cast<phg_hip_mat *>(A_);
     if (A == nullptr || A->eop == nullptr || elem_us == nullptr || gather_us == nullptr)
         return phg::set_error(PHG_HIP_E_ARG, "phg_hip_element_time: operator not built");
+    if ((rc = phg::ensure_solver_copy(A)) || (rc = phg::ensure_fp32_copy(A)))
+        return rc;
     if (reps <= 0)
         reps = 10;
     hipStream_t s = phg::ctx()->stream;

@@ -113,6 +113,7 @@ struct phg_hip_space {
     int32_t *d_map = nullptr;        // [nelem][nbas]
     uint8_t *d_bdry = nullptr;       // [ndof]
     double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};   // bounding box of the vertices
+    int batch_refs = 0;              // batches whose graphs read its arrays
     // host copies of the element maps, kept for the element operator's
     // layout builder
     std::vector<int32_t> h_elem_verts, h_map;
@@ -122,6 +123,7 @@ struct phg_hip_space {
 struct phg_hip_vec {
     int64_t n;
     double *d = nullptr;
+    int batch_refs = 0;              // batches whose graphs read or write it
 };
 
 // Blocks of rows; per block a dictionary of the distinct column groups
@@ -210,7 +212,11 @@ struct phg_hip_mat {
     // orders DOFs along a Morton curve of their nodes (q_perm: solver index ->
     // PHG index); the q_* arrays alias the PHG-order ones when q_perm is null
     int reorder = 1;                 // 1: derive the spatial numbering
+    // 1: the PHG-order arrays changed (a batch re-assembled the matrix) and
+    // the solver copy is rebuilt when next needed (ensure_solver_copy)
+    int q_stale = 0;
     int32_t *q_perm = nullptr;
+    int32_t *q_iperm = nullptr;      // PHG index -> solver index (batch views only)
     int q_owns = 0;
     int32_t *q_rowptr = nullptr, *q_cols = nullptr;
     double *q_vals = nullptr, *q_diag = nullptr;
@@ -256,6 +262,10 @@ struct phg_hip_mat {
     double spmm_us[8][2] = {};
     // matrix-free FP32 operator (eop.hip), null unless built
     EopData *eop = nullptr;
+    // batches (phg_hip_batch) whose graphs hold this matrix's buffers: while
+    // nonzero, calls that would free them (destroy, a new element operator)
+    // are refused
+    int batch_refs = 0;
 };
 
 namespace phg {
@@ -272,6 +282,9 @@ void free_workspace(phg_hip_mat *A);
 int build_row_blocks(phg_hip_mat *A);
 // solver copy (reorder.hip)
 int build_solver_copy(phg_hip_mat *A);
+// rebuilds the solver copy when q_stale is set (every entry point that reads
+// the q_* arrays or the element operator calls it first)
+int ensure_solver_copy(phg_hip_mat *A);
 void free_solver_copy(phg_hip_mat *A);
 int split_row_blocks(const int32_t *d_rowptr, int64_t n, int64_t nnz,
                      int64_t cap, int64_t rcap, int32_t **d_blk, int64_t *nblk);
@@ -291,6 +304,21 @@ int build_dict(phg_hip_mat *A);     // A->cd
 int build_xstage(phg_hip_mat *A);   // A->xs
 void free_dict_layout(DictLayout *L);
 void free_dict(phg_hip_mat *A);     // both
+// Re-assembly for new values (assemble.hip): a plan made once per (space,
+// problem, eliminated matrix) with device pointers to the kappa, load and
+// Dirichlet-value arrays.  It re-allocates the matrix's PHG-order columns and
+// values to a capacity bounding every pattern and drops its solver copy
+// (q_stale).  refill_enqueue then launches the element pass, row walk, scans,
+// compaction and elimination into the matrix's PHG-order arrays, with no
+// allocation, synchronisation or memset (capturable); the new entry count is
+// at the device address refill_nnz.
+struct RefillPlan;
+int refill_plan_create(phg_hip_space *sp, const phg_hip_problem *pb, phg_hip_mat *A,
+                       const double *d_kappa, const double *d_f, const double *d_g,
+                       RefillPlan **out);
+int refill_enqueue(const RefillPlan *p, hipStream_t s);
+const int64_t *refill_nnz(const RefillPlan *p);
+void refill_plan_destroy(RefillPlan *p);
 int setup_diagonal(phg_hip_mat *A);
 // Rows (or blocks) of one XCD swept together: [0, n) is split into 8 equal
 // regions, region k processed by the work-groups the dispatcher places on

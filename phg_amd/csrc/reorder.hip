@@ -121,18 +121,38 @@ void free_solver_copy(phg_hip_mat *A)
         dfree(A->q_diag);
     }
     dfree(A->q_perm);
+    dfree(A->q_iperm);
     dfree(A->q_vals32);
     dfree(A->q_diag32);
     dfree(A->q_rblk);
     dfree(A->q_rblk1k);
     dfree(A->q_wblk);
-    A->q_perm = nullptr;
+    A->q_perm = A->q_iperm = nullptr;
     A->q_rowptr = A->q_cols = nullptr;
     A->q_vals = A->q_diag = nullptr;
     A->q_vals32 = A->q_diag32 = nullptr;
     A->q_rblk = A->q_rblk1k = A->q_wblk = nullptr;
     A->q_nrblk = A->q_nrblk1k = A->q_nwblk = 0;
     A->q_owns = 0;
+}
+
+int ensure_solver_copy(phg_hip_mat *A)
+{
+    if (!A->q_stale)
+        return PHG_HIP_OK;
+    // the element operator is kept: its numbering is the node Morton order
+    // the rebuild derives again from the same coordinates, and its
+    // coefficients were set with the values that made the PHG-order arrays
+    EopData *eop = A->eop;
+    A->eop = nullptr;
+    int rc = build_solver_copy(A);
+    A->eop = eop;
+    if (rc == PHG_HIP_OK)
+        A->q_stale = 0;
+    // an operator reads the FP32 diagonal of its Dirichlet rows
+    if (rc == PHG_HIP_OK && eop != nullptr)
+        rc = ensure_fp32_copy(A);
+    return rc;
 }
 
 int build_solver_copy(phg_hip_mat *A)

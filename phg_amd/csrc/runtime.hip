@@ -261,6 +261,9 @@ extern "C" int phg_hip_vec_destroy(phg_hip_vec *v)
 {
     if (v == nullptr)
         return PHG_HIP_OK;
+    if (v->batch_refs > 0)
+        return phg::set_error(PHG_HIP_E_STATE, "phg_hip_vec_destroy: the vector is in a "
+                                               "batch; destroy the batch first");
     phg::dfree(v->d);
     delete v;
     return PHG_HIP_OK;
