@@ -9,6 +9,8 @@ in flight on a stream pool; a chunk of iterations of every system captured
 once in a HIP graph and replayed until all converge).
 
     python -m phg_amd.multimesh          # the 8-mesh P3 batch, one JSON line
+    python -m phg_amd.multimesh batch    # assembly + solve for new kappa: sequential
+                                         # against one phg_hip_batch_run
 """
 from __future__ import annotations
 
@@ -167,9 +169,91 @@ def bench(nmesh=8, n=16, p=3, rtol=1e-8, reps=2):
                 "kernel_level_largest": kernel_level}}
 
 
+def bench_batch(nmesh=8, n=16, p=3, rtol=1e-8, reps=3, seed=6, sigma=0.5):
+    """Assembly + solve with new coefficients: the sequential path (per mesh
+    phg_hip_assemble, phg_hip_mat_eliminate, the element operator's
+    coefficients on the space's layout, phg_hip_pcg_reliable) against one
+    phg_hip_batch_run (assembly and solve of all meshes in the captured
+    graphs).  Each repetition draws new kappa; both legs start from the same
+    host arrays."""
+    import phg_amd as pa
+    meshes = batch_meshes(nmesh, n)
+    pb = pa.Problem.poisson(use_kappa=True)
+    rng = np.random.default_rng(seed)
+    draws = [[np.exp(rng.normal(0.0, sigma, size=g.nelem)) for g, _, _ in meshes]
+             for _ in range(reps + 1)]
+    spaces = [pa.Space(g, p) for g, _, _ in meshes]
+    ones = [np.ones(g.ndof(p)) for g, _, _ in meshes]
+    zeros = [np.zeros(g.ndof(p)) for g, _, _ in meshes]
+
+    def sequential(ks):
+        out = []
+        for sp, k, f, z in zip(spaces, ks, ones, zeros):
+            A = pa.assemble(sp, pb, kappa=k, f=f, g=z)
+            A.eliminate()
+            A.element_operator(sp, pb, kappa=k)
+            x = pa.Vec(A.n)
+            nit, _, _ = pa.pcg_reliable(A, A.rhs(), x, rtol=rtol, btol=rtol)
+            out.append((A, x, nit))
+        pa.synchronize()
+        return out
+    sequential(draws[0])            # warm: spaces keep their workspaces and layouts
+    t = time.perf_counter()
+    for r in range(reps):
+        seq = sequential(draws[1 + r])
+    t_seq = (time.perf_counter() - t) / reps
+    x_seq = [x.numpy() for _, x, _ in seq]
+    its_seq = [int(i) for _, _, i in seq]
+    # the batch, made on matrices of the first draw
+    t0 = time.perf_counter()
+    kv = [pa.Vec(g.nelem, k) for (g, _, _), k in zip(meshes, draws[0])]
+    fv = [pa.Vec(len(f), f) for f in ones]
+    gv = [pa.Vec(len(z), z) for z in zeros]
+    xv = [pa.Vec(len(z), z) for z in zeros]
+    mats = []
+    for sp, k, f, z in zip(spaces, draws[0], ones, zeros):
+        A = pa.assemble(sp, pb, kappa=k, f=f, g=z)
+        A.eliminate()
+        A.element_operator(sp, pb, kappa=k)
+        mats.append(A)
+    B = pa.Batch([(sp, pb, A, k, f, g, x) for sp, A, k, f, g, x in
+                  zip(spaces, mats, kv, fv, gv, xv)], rtol=rtol, btol=rtol)
+    pa.synchronize()
+    t_make = time.perf_counter() - t0
+    B.run()
+    pa.synchronize()
+    t = time.perf_counter()
+    for r in range(reps):
+        for v, k in zip(kv, draws[1 + r]):
+            v.upload(k)
+        for x in xv:
+            x.axpby(0.0, x, 0.0)
+        its_b, _ = B.run()
+    pa.synchronize()
+    t_b = (time.perf_counter() - t) / reps
+    dev = max(float(np.linalg.norm(x.numpy() - a) / np.linalg.norm(a))
+              for a, x in zip(x_seq, xv))
+    same_csr = all(all(np.array_equal(u, v) for u, v in zip(A.download(), A1.download()))
+                   for A, (A1, _, _) in zip(mats, seq))
+    tot = sum(m.n for m in mats)
+    return {"sequential_assemble_solve_s": round(t_seq, 4),
+            "batch_assemble_solve_s": round(t_b, 4),
+            "speedup": round(t_seq / t_b, 3),
+            "dofs_per_s_batch": round(tot / t_b, 1),
+            "dofs_per_s_sequential": round(tot / t_seq, 1),
+            "batch_make_s": round(t_make, 3),
+            "its_sequential": its_seq, "its_batch": [int(i) for i in its_b],
+            "max_rel_diff_vs_sequential_solution": dev,
+            "csr_bitwise_equal_to_sequential_assembly": bool(same_csr),
+            "nnz": [int(m.nnz) for m in mats]}
+
+
 def main():
     import phg_amd as pa
     pa.init(0)
+    if len(sys.argv) > 1 and sys.argv[1] == "batch":
+        print(json.dumps(bench_batch()), flush=True)
+        return 0
     print(json.dumps(bench()), flush=True)
     return 0
 
