@@ -90,7 +90,7 @@ def _ptr(a, t):
 
 
 def nbas(p):
-    return {1: 4, 2: 10, 3: 20}[p]
+    return {1: 4, 2: 10, 3: 20, 4: 35}[p]
 
 
 def problem(p, dim=1, use_kappa=False, mass_coef=0.0, rhs_order=-1, E=2.5, nu=0.25,

@@ -22,14 +22,14 @@ static void *xmalloc(size_t n)
 /* quad.c:1799-2000 static 3D rules, permutations of
  * include/phg/quad-permu.h:83-120 in the same point order. */
 
-#define NQMAX 24
+#define NQMAX 46
 typedef struct {
     int np;
     double pts[NQMAX][4];
     double wts[NQMAX];
 } qrule;
 
-static qrule rules[7];
+static qrule rules[9];
 static int rules_ready = 0;
 
 static void add31(qrule *q, double a, double w)
@@ -40,6 +40,12 @@ static void add31(qrule *q, double a, double w)
         memcpy(q->pts[q->np], P[i], sizeof(P[i]));
         q->wts[q->np++] = w;
     }
+}
+
+static void add4(qrule *q, double w)
+{
+    q->pts[q->np][0] = q->pts[q->np][1] = q->pts[q->np][2] = q->pts[q->np][3] = .25;
+    q->wts[q->np++] = w;
 }
 
 static void add22(qrule *q, double a, double w)
@@ -106,6 +112,31 @@ static void init_rules(void)
           .05535718154365472209515327785372602);
     add211(&rules[6], .06366100187501752529923552760572698,
            .60300566479164914136743113906093969, 27. / 560.);
+    /* order 7 (quad.c:1990) */
+    add4(&rules[7], .09548528946413084886057843611722638);
+    add31(&rules[7], .31570114977820279942342999959331149,
+          .04232958120996702907628617079854674);
+    add22(&rules[7], .05048982259839636876305382298656247,
+          .03189692783285757993427482408294246);
+    add211(&rules[7], .18883383102600104773643110385458576,
+           .57517163758700002348324157702230752, .03720713072833462136961556119148112);
+    add211(&rules[7], .02126547254148324598883610149981994,
+           .81083024109854856111810537984823239, .00811077082990334156610343349109654);
+    /* order 8 (quad.c:2017) */
+    add31(&rules[8], .03967542307038990126507132953938949,
+          .00639714777990232132145142033517302);
+    add31(&rules[8], .31448780069809631378416056269714830,
+          .04019044802096617248816115847981783);
+    add31(&rules[8], .10198669306270330000000000000000000,
+          .02430797550477032117486910877192260);
+    add31(&rules[8], .18420369694919151227594641734890918,
+          .05485889241369744046692412399039144);
+    add22(&rules[8], .06343628775453989240514123870189827,
+          .03571961223409918246495096899661762);
+    add211(&rules[8], .02169016206772800480266248262493018,
+           .71993192203946593588943495335273478, .00718319069785253940945110521980376);
+    add211(&rules[8], .20448008063679571424133557487274534,
+           .58057719012880922417539817139062041, .01637218194531911754093813975611913);
     /* order 0 falls to the order-1 rule (quad.c:2807 init_table) */
     rules[0] = rules[1];
     rules_ready = 1;
@@ -116,7 +147,7 @@ int orc_quad3d(int order, const double **pts, const double **wts)
     init_rules();
     if (order < 0)
         order = 0;
-    if (order > 6)
+    if (order > 8)
         return -1;
     *pts = &rules[order].pts[0][0];
     *wts = rules[order].wts;
@@ -147,6 +178,48 @@ void orc_bas(int p, const double *lam, double *val)
         *(q++) = 4. * L1 * L3;
         *q = 4. * L2 * L3;
     }
+    else if (p == 4) {                    /* lagrange.c:757 full block */
+        double L0T4M1 = 4. * L0 - 1., L1T4M1 = 4. * L1 - 1., L2T4M1 = 4. * L2 - 1.,
+               L3T4M1 = 4. * L3 - 1.;
+        double t0, t1, t2, t3, d01, d02, d03, d12, d13, d23, d123, d023, d013, d012;
+        double *q = val;
+        *(q++) = L0T4M1 * ((2. / 3.) * L0 - (1. / 3.)) * (L0T4M1 - 2.) * L0;
+        *(q++) = L1T4M1 * ((2. / 3.) * L1 - (1. / 3.)) * (L1T4M1 - 2.) * L1;
+        *(q++) = L2T4M1 * ((2. / 3.) * L2 - (1. / 3.)) * (L2T4M1 - 2.) * L2;
+        *(q++) = L3T4M1 * ((2. / 3.) * L3 - (1. / 3.)) * (L3T4M1 - 2.) * L3;
+        t0 = L0T4M1 * ((32. / 3.) * L0 - (16. / 3.));
+        t1 = L1T4M1 * ((32. / 3.) * L1 - (16. / 3.));
+        t2 = L2T4M1 * ((32. / 3.) * L2 - (16. / 3.));
+        t3 = L3T4M1 * ((32. / 3.) * L3 - (16. / 3.));
+        d01 = L0 * L1;
+        d02 = L0 * L2;
+        d03 = L0 * L3;
+        d12 = L1 * L2;
+        d13 = L1 * L3;
+        d23 = L2 * L3;
+        /* edges 0-1, 0-2, 0-3, 1-2, 1-3, 2-3: near the first vertex, middle,
+         * near the second */
+        *(q++) = t0 * d01; *(q++) = 4. * L0T4M1 * L1T4M1 * d01; *(q++) = t1 * d01;
+        *(q++) = t0 * d02; *(q++) = 4. * L0T4M1 * L2T4M1 * d02; *(q++) = t2 * d02;
+        *(q++) = t0 * d03; *(q++) = 4. * L0T4M1 * L3T4M1 * d03; *(q++) = t3 * d03;
+        *(q++) = t1 * d12; *(q++) = 4. * L1T4M1 * L2T4M1 * d12; *(q++) = t2 * d12;
+        *(q++) = t1 * d13; *(q++) = 4. * L1T4M1 * L3T4M1 * d13; *(q++) = t3 * d13;
+        *(q++) = t2 * d23; *(q++) = 4. * L2T4M1 * L3T4M1 * d23; *(q++) = t3 * d23;
+        L0T4M1 *= 32.;
+        L1T4M1 *= 32.;
+        L2T4M1 *= 32.;
+        L3T4M1 *= 32.;
+        d123 = L1 * d23;
+        d023 = L0 * d23;
+        d013 = L0 * d13;
+        d012 = L0 * d12;
+        /* faces 0..3, one node near each face vertex in face-vertex order */
+        *(q++) = L1T4M1 * d123; *(q++) = L2T4M1 * d123; *(q++) = L3T4M1 * d123;
+        *(q++) = L0T4M1 * d023; *(q++) = L2T4M1 * d023; *(q++) = L3T4M1 * d023;
+        *(q++) = L0T4M1 * d013; *(q++) = L1T4M1 * d013; *(q++) = L3T4M1 * d013;
+        *(q++) = L0T4M1 * d012; *(q++) = L1T4M1 * d012; *(q++) = L2T4M1 * d012;
+        *q = 256. * L0 * d123;            /* element */
+    }
     else {                                /* lagrange.c:392 full block */
         double L03 = 3. * L0, L13 = 3. * L1, L23 = 3. * L2, L33 = 3. * L3, d;
         double *q = val;
@@ -173,9 +246,14 @@ void orc_bas(int p, const double *lam, double *val)
     }
 }
 
+int orc_nbas(int p)
+{
+    return p == 1 ? 4 : (p == 2 ? 10 : (p == 3 ? 20 : 35));
+}
+
 void orc_grad_lambda(int p, const double *lam, double *grad)
 {
-    int nb = p == 1 ? 4 : (p == 2 ? 10 : 20);
+    int nb = orc_nbas(p);
     memset(grad, 0, (size_t)nb * 4 * sizeof(double));
     if (p == 1) {                         /* lagrange.c:111 */
         for (int k = 0; k < 4; k++)
@@ -190,6 +268,48 @@ void orc_grad_lambda(int p, const double *lam, double *grad)
             grad[4 * (4 + k) + a] = 4. * lam[b];
             grad[4 * (4 + k) + b] = 4. * lam[a];
         }
+        return;
+    }
+    if (p == 4) {                         /* lagrange.c:1085 full block */
+        for (int k = 0; k < 4; k++) {
+            double L0 = lam[k];
+            grad[4 * k + k] = ((8. / 3.) * L0 - 1.) * (L0 * (16. * L0 - 12.) + 1.);
+        }
+        for (int k = 0; k < 6; k++) {
+            /* edge (a, b): near a (TEMP0 a, b), middle (TEMP1), near b
+             * (TEMP0 b, a) */
+            int ab[3][2] = {{EV[k][0], EV[k][1]}, {EV[k][0], EV[k][1]},
+                            {EV[k][1], EV[k][0]}};
+            for (int t = 0; t < 3; t++) {
+                int v0 = ab[t][0], v1 = ab[t][1];
+                double L0 = lam[v0], L1 = lam[v1], *g = grad + 4 * (4 + 3 * k + t);
+                if (t == 1) {
+                    g[v0] = (32. * L0 - 4.) * (4. * L1 - 1.) * L1;
+                    g[v1] = L0 * (4. * L0 - 1.) * (32. * L1 - 4.);
+                }
+                else {
+                    g[v0] = (L0 * ((384. / 3.) * L0 - 64.) + (16. / 3.)) * L1;
+                    g[v1] = L0 * ((64. / 3.) * L0 - (16. / 3.)) * (2. * L0 - 1.);
+                }
+            }
+        }
+        for (int f = 0; f < 4; f++) {
+            /* face (a, b, c): nodes (a, b, c), (b, c, a), (c, a, b) */
+            int a = FV[f][0], b = FV[f][1], c = FV[f][2];
+            int cyc[3][3] = {{a, b, c}, {b, c, a}, {c, a, b}};
+            for (int t = 0; t < 3; t++) {
+                int v1 = cyc[t][0], v2 = cyc[t][1], v3 = cyc[t][2];
+                double L1 = lam[v1], L2 = lam[v2], L3 = lam[v3];
+                double *g = grad + 4 * (22 + 3 * f + t);
+                g[v1] = (256. * L1 - 32.) * L2 * L3;
+                g[v2] = (128. * L1 - 32.) * L1 * L3;
+                g[v3] = (128. * L1 - 32.) * L1 * L2;
+            }
+        }
+        grad[4 * 34 + 0] = 256. * lam[1] * lam[2] * lam[3];
+        grad[4 * 34 + 1] = 256. * lam[0] * lam[2] * lam[3];
+        grad[4 * 34 + 2] = 256. * lam[0] * lam[1] * lam[3];
+        grad[4 * 34 + 3] = 256. * lam[0] * lam[1] * lam[2];
         return;
     }
     /* lagrange.c:538 */
@@ -300,12 +420,12 @@ static void xgrad(int p, int k, const double *G /* [4] lambda grad of k */,
                  G[2] * J[2 * 4 + c] + G[3] * J[3 * 4 + c];
 }
 
-#define NBMAX 20
+#define NBMAX 35
 
 void orc_element(const orc_problem *pb, const double x[4][3], double kappa,
                  const double *f_loc, double *K, double *b)
 {
-    int p = pb->p, N = p == 1 ? 4 : (p == 2 ? 10 : 20), dim = pb->dim;
+    int p = pb->p, N = orc_nbas(p), dim = pb->dim;
     double J[16], vol;
     const double *qp, *qw;
     int nq;
@@ -498,7 +618,7 @@ orc_csr *orc_assemble(const orc_problem *pb, int64_t nelem, const double *verts,
                       int64_t ndof, const uint8_t *bdry, const double *g,
                       const double *f, const double *kappa)
 {
-    int p = pb->p, N = p == 1 ? 4 : (p == 2 ? 10 : 20), dim = pb->dim;
+    int p = pb->p, N = orc_nbas(p), dim = pb->dim;
     int M = N * dim;
     int64_t n = ndof * dim;
     mrow *rows = calloc((size_t)n, sizeof(mrow));
@@ -1182,7 +1302,7 @@ double orc_mass_apply(int p, int64_t nelem, const double *verts,
                       const int32_t *elem_verts, const int32_t *map,
                       int64_t ndof, const double *e, double *Me)
 {
-    int N = p == 1 ? 4 : (p == 2 ? 10 : 20);
+    int N = orc_nbas(p);
     orc_problem mp = {p, 1, 1, 1.0, -1, 0.0, 0.0, 0};
     double *K = xmalloc((size_t)N * N * sizeof(double));
     double *B = xmalloc((size_t)N * sizeof(double));
@@ -1217,7 +1337,7 @@ void orc_adjoint_grad(int p, int64_t nelem, const double *verts,
                       const double *kappa, const double *u,
                       const double *lam, double *grad)
 {
-    int N = p == 1 ? 4 : (p == 2 ? 10 : 20);
+    int N = orc_nbas(p);
     orc_problem sp = {p, 1, 1, 0.0, -1, 0.0, 0.0, 0};
     double *K = xmalloc((size_t)N * N * sizeof(double));
     double *B = xmalloc((size_t)N * sizeof(double));

@@ -24,6 +24,7 @@ int orc_quad3d(int order, const double **pts, const double **wts);
 
 /* lagrange.c:80-731 P1/P2/P3 basis values (full-block path) and lambda
  * gradients [nbas][4]. */
+int orc_nbas(int p);
 void orc_bas(int p, const double *lam, double *val);
 void orc_grad_lambda(int p, const double *lam, double *grad);
 

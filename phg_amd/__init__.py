@@ -488,7 +488,7 @@ def _opt(a, dtype=np.float64):
 
 
 def element_matrices(sp: Space, pb: Problem, kappa=None, f=None):
-    M = {1: 4, 2: 10, 3: 20}[sp.order] * pb.dim
+    M = {1: 4, 2: 10, 3: 20, 4: 35}[sp.order] * pb.dim
     kap = _opt(kappa)
     f = _opt(f) if f is not None else np.zeros(sp.ndof * pb.dim)
     K = np.empty((sp.nelem, M, M), dtype=np.float64)

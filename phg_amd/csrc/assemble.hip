@@ -35,19 +35,32 @@
 
 namespace {
 
-constexpr int NQ = 24;   // largest rule used (order 6)
+constexpr int NQ = 24;   // largest rule used by P1-P3 (order 6)
 constexpr int NB = 20;   // P3
+constexpr int NQ8 = 46;  // order-8 rule (P4 mass and load)
 
 //--------------------------------------------------------------- quadrature
 // quad.c:1799-2000 static 3D rules with the orbit order of
 // include/phg/quad-permu.h:83-120; order 0 uses the order-1 rule
 // (quad.c:2807 init_table).
-struct Rule {
+template <int CAP>
+struct RuleT {
+    static constexpr int cap = CAP;
     int np;
-    double pts[NQ][4];
-    double w[NQ];
+    double pts[CAP][4];
+    double w[CAP];
 };
+using Rule = RuleT<NQ>;
+using Rule8 = RuleT<NQ8>;
 
+template <typename Rule>
+__host__ __device__ inline void r4(Rule &q, double w)
+{
+    q.pts[q.np][0] = q.pts[q.np][1] = q.pts[q.np][2] = q.pts[q.np][3] = .25;
+    q.w[q.np++] = w;
+}
+
+template <typename Rule>
 __host__ __device__ inline void r31(Rule &q, double a, double w)
 {
     double b = 1.0 - 3.0 * a;
@@ -59,6 +72,7 @@ __host__ __device__ inline void r31(Rule &q, double a, double w)
     }
 }
 
+template <typename Rule>
 __host__ __device__ inline void r22(Rule &q, double a, double w)
 {
     double h = 0.5 - a;
@@ -71,6 +85,7 @@ __host__ __device__ inline void r22(Rule &q, double a, double w)
     }
 }
 
+template <typename Rule>
 __host__ __device__ inline void r211(Rule &q, double a, double b, double w)
 {
     double c = 1.0 - a - a - b;
@@ -84,6 +99,8 @@ __host__ __device__ inline void r211(Rule &q, double a, double b, double w)
     }
 }
 
+// orders above 6 need a rule of capacity NQ8
+template <typename Rule>
 __host__ __device__ inline void make_rule(int order, Rule &q)
 {
     q.np = 0;
@@ -117,7 +134,7 @@ __host__ __device__ inline void make_rule(int order, Rule &q)
         r22(q, .04550370412564964949188052627933943,
             .04254602077708146643806942812025744);
     }
-    else {
+    else if (order == 6 || Rule::cap < NQ8) {
         r31(q, .21460287125915202928883921938628499,
             .03992275025816749209969062755747998);
         r31(q, .04067395853461135311557944895641006,
@@ -126,6 +143,33 @@ __host__ __device__ inline void make_rule(int order, Rule &q)
             .05535718154365472209515327785372602);
         r211(q, .06366100187501752529923552760572698,
              .60300566479164914136743113906093969, 27. / 560.);
+    }
+    else if (order == 7) {      // quad.c:1990
+        r4(q, .09548528946413084886057843611722638);
+        r31(q, .31570114977820279942342999959331149,
+            .04232958120996702907628617079854674);
+        r22(q, .05048982259839636876305382298656247,
+            .03189692783285757993427482408294246);
+        r211(q, .18883383102600104773643110385458576,
+             .57517163758700002348324157702230752, .03720713072833462136961556119148112);
+        r211(q, .02126547254148324598883610149981994,
+             .81083024109854856111810537984823239, .00811077082990334156610343349109654);
+    }
+    else {                      // quad.c:2017, order 8
+        r31(q, .03967542307038990126507132953938949,
+            .00639714777990232132145142033517302);
+        r31(q, .31448780069809631378416056269714830,
+            .04019044802096617248816115847981783);
+        r31(q, .10198669306270330000000000000000000,
+            .02430797550477032117486910877192260);
+        r31(q, .18420369694919151227594641734890918,
+            .05485889241369744046692412399039144);
+        r22(q, .06343628775453989240514123870189827,
+            .03571961223409918246495096899661762);
+        r211(q, .02169016206772800480266248262493018,
+             .71993192203946593588943495335273478, .00718319069785253940945110521980376);
+        r211(q, .20448008063679571424133557487274534,
+             .58057719012880922417539817139062041, .01637218194531911754093813975611913);
     }
 }
 
@@ -149,11 +193,45 @@ __host__ __device__ inline void face_verts(int k, int &a, int &b, int &c)
     c = t[k][2];
 }
 
-// lagrange.c:80 / :160 / :392 (full-block evaluation)
+// lagrange.c:80 / :160 / :392 / :757 (full-block evaluation)
 __host__ __device__ inline void bas_values(int p, const double *lam, double *v)
 {
     double L0 = lam[0], L1 = lam[1], L2 = lam[2], L3 = lam[3];
-    if (p == 1) {
+    if (p == 4) {
+        double L0T4M1 = 4. * L0 - 1., L1T4M1 = 4. * L1 - 1., L2T4M1 = 4. * L2 - 1.,
+               L3T4M1 = 4. * L3 - 1.;
+        v[0] = L0T4M1 * ((2. / 3.) * L0 - (1. / 3.)) * (L0T4M1 - 2.) * L0;
+        v[1] = L1T4M1 * ((2. / 3.) * L1 - (1. / 3.)) * (L1T4M1 - 2.) * L1;
+        v[2] = L2T4M1 * ((2. / 3.) * L2 - (1. / 3.)) * (L2T4M1 - 2.) * L2;
+        v[3] = L3T4M1 * ((2. / 3.) * L3 - (1. / 3.)) * (L3T4M1 - 2.) * L3;
+        const double t[4] = {L0T4M1 * ((32. / 3.) * L0 - (16. / 3.)),
+                             L1T4M1 * ((32. / 3.) * L1 - (16. / 3.)),
+                             L2T4M1 * ((32. / 3.) * L2 - (16. / 3.)),
+                             L3T4M1 * ((32. / 3.) * L3 - (16. / 3.))};
+        const double m[4] = {L0T4M1, L1T4M1, L2T4M1, L3T4M1};
+        // edges: near the first vertex, middle, near the second
+        for (int k = 0; k < 6; k++) {
+            int a, b;
+            edge_verts(k, a, b);
+            const double d = lam[a] * lam[b];
+            v[4 + 3 * k] = t[a] * d;
+            v[5 + 3 * k] = 4. * m[a] * m[b] * d;
+            v[6 + 3 * k] = t[b] * d;
+        }
+        const double m32[4] = {L0T4M1 * 32., L1T4M1 * 32., L2T4M1 * 32., L3T4M1 * 32.};
+        const double d23 = L2 * L3, d13 = L1 * L3, d12 = L1 * L2;
+        const double df[4] = {L1 * d23, L0 * d23, L0 * d13, L0 * d12};
+        // faces: one node near each face vertex in face-vertex order
+        for (int k = 0; k < 4; k++) {
+            int a, b, c;
+            face_verts(k, a, b, c);
+            v[22 + 3 * k] = m32[a] * df[k];
+            v[23 + 3 * k] = m32[b] * df[k];
+            v[24 + 3 * k] = m32[c] * df[k];
+        }
+        v[34] = 256. * L0 * df[0];
+    }
+    else if (p == 1) {
         v[0] = L0; v[1] = L1; v[2] = L2; v[3] = L3;
     }
     else if (p == 2) {
@@ -199,12 +277,55 @@ __host__ __device__ inline void bas_values(int p, const double *lam, double *v)
     }
 }
 
-// lagrange.c:111 / :248 / :538: gradients with respect to lambda, [nb][4]
+// lagrange.c:111 / :248 / :538 / :1085: gradients with respect to lambda,
+// [nb][4]
 __host__ __device__ inline void bas_grad_lambda(int p, const double *lam, double *g)
 {
-    int nb = p == 1 ? 4 : (p == 2 ? 10 : 20);
+    int nb = p == 1 ? 4 : (p == 2 ? 10 : (p == 3 ? 20 : 35));
     for (int i = 0; i < 4 * nb; i++)
         g[i] = 0.;
+    if (p == 4) {
+        for (int k = 0; k < 4; k++) {
+            const double L0 = lam[k];
+            g[4 * k + k] = ((8. / 3.) * L0 - 1.) * (L0 * (16. * L0 - 12.) + 1.);
+        }
+        for (int k = 0; k < 6; k++) {
+            int a, b;
+            edge_verts(k, a, b);
+            for (int t = 0; t < 3; t++) {
+                // TEMP0 (a, b), TEMP1 (a, b), TEMP0 (b, a)
+                const int v0 = t == 2 ? b : a, v1 = t == 2 ? a : b;
+                const double L0 = lam[v0], L1 = lam[v1];
+                double *gg = g + 4 * (4 + 3 * k + t);
+                if (t == 1) {
+                    gg[v0] = (32. * L0 - 4.) * (4. * L1 - 1.) * L1;
+                    gg[v1] = L0 * (4. * L0 - 1.) * (32. * L1 - 4.);
+                }
+                else {
+                    gg[v0] = (L0 * ((384. / 3.) * L0 - 64.) + (16. / 3.)) * L1;
+                    gg[v1] = L0 * ((64. / 3.) * L0 - (16. / 3.)) * (2. * L0 - 1.);
+                }
+            }
+        }
+        for (int k = 0; k < 4; k++) {
+            int a, b, c;
+            face_verts(k, a, b, c);
+            const int cyc[3][3] = {{a, b, c}, {b, c, a}, {c, a, b}};
+            for (int t = 0; t < 3; t++) {
+                const int v1 = cyc[t][0], v2 = cyc[t][1], v3 = cyc[t][2];
+                const double L1 = lam[v1], L2 = lam[v2], L3 = lam[v3];
+                double *gg = g + 4 * (22 + 3 * k + t);
+                gg[v1] = (256. * L1 - 32.) * L2 * L3;
+                gg[v2] = (128. * L1 - 32.) * L1 * L3;
+                gg[v3] = (128. * L1 - 32.) * L1 * L2;
+            }
+        }
+        g[4 * 34 + 0] = 256. * lam[1] * lam[2] * lam[3];
+        g[4 * 34 + 1] = 256. * lam[0] * lam[2] * lam[3];
+        g[4 * 34 + 2] = 256. * lam[0] * lam[1] * lam[3];
+        g[4 * 34 + 3] = 256. * lam[0] * lam[1] * lam[2];
+        return;
+    }
     if (p == 1) {
         for (int k = 0; k < 4; k++)
             g[4 * k + k] = 1.0;
@@ -381,28 +502,40 @@ struct ElemArgs {
     double *B;               // [nelem][M]
 };
 
-struct Tables {
-    int nq_s, nq_m, nq_r;
-    double ws[NQ], wm[NQ], wr[NQ];
-    double gs[NQ][NB][4];   // lambda gradients at the stiffness points
-    double bm[NQ][NB];      // basis values at the mass points
-    double br[NQ][NB];      // basis values at the load points
+// table sizes per order: P1-P3 share one layout (rules up to order 6, 20
+// bases); P4 has 35 bases, stiffness gradients on the order-6 rule and mass
+// and load values on rules up to order 8
+template <int P> struct Shape {
+    static constexpr int N = P == 1 ? 4 : (P == 2 ? 10 : (P == 3 ? 20 : 35));
+    static constexpr int NBT = P <= 3 ? NB : 35;
+    static constexpr int NQG = NQ;                     // stiffness points
+    static constexpr int NQT = P <= 3 ? NQ : NQ8;      // mass and load points
 };
 
 template <int P>
-__device__ void build_tables(Tables &T, int dim, int rhs_order, int mass, bool qc)
+struct TablesT {
+    int nq_s, nq_m, nq_r;
+    double ws[Shape<P>::NQG], wm[Shape<P>::NQT], wr[Shape<P>::NQT];
+    double gs[Shape<P>::NQG][Shape<P>::NBT][4];   // lambda gradients at the stiffness points
+    double bm[Shape<P>::NQT][Shape<P>::NBT];      // basis values at the mass points
+    double br[Shape<P>::NQT][Shape<P>::NBT];      // basis values at the load points
+};
+
+template <int P>
+__device__ void build_tables(TablesT<P> &T, int dim, int rhs_order, int mass, bool qc)
 {
-    constexpr int N = P == 1 ? 4 : (P == 2 ? 10 : 20);
+    constexpr int N = Shape<P>::N;
     // all work-items build the same rules; thread 0 publishes them
     if (threadIdx.x == 0) {
-        Rule q;
+        RuleT<Shape<P>::NQT> q;
         // stiffness (quad.c:5271; elastic.c:141); with the quadrature cache
-        // the element's one rule of order 2p (poisson.c:126)
+        // the element's one rule of order 2p (poisson.c:126; P1-P3 only, the
+        // host refuses it for P4)
         make_rule(qc ? 2 * P : 2 * (P - 1), q);
-        T.nq_s = q.np;
-        for (int i = 0; i < q.np; i++) {
+        T.nq_s = q.np < Shape<P>::NQG ? q.np : Shape<P>::NQG;
+        for (int i = 0; i < T.nq_s; i++) {
             T.ws[i] = q.w[i];
-            double g[NB * 4];
+            double g[Shape<P>::NBT * 4];
             bas_grad_lambda(P, q.pts[i], g);
             for (int k = 0; k < N; k++)
                 for (int c = 0; c < 4; c++)
@@ -433,9 +566,9 @@ __device__ void build_tables(Tables &T, int dim, int rhs_order, int mass, bool q
 template <int P, int DIM, bool QC = false>
 __global__ void __launch_bounds__(128) k_element(ElemArgs a)
 {
-    constexpr int N = P == 1 ? 4 : (P == 2 ? 10 : 20);
+    constexpr int N = Shape<P>::N;
     constexpr int M = N * DIM;
-    __shared__ Tables T;
+    __shared__ TablesT<P> T;
     build_tables<P>(T, DIM, a.rhs_order, a.mass_coef != 0.0, QC);
 
     int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -458,7 +591,8 @@ __global__ void __launch_bounds__(128) k_element(ElemArgs a)
     const int32_t *m = a.map + e * (int64_t)N;
 
     // points of the stiffness rule: order 2(P-1), or 2P with the cache
-    constexpr int NQS = QC ? (P == 1 ? 4 : (P == 2 ? 14 : 24)) : (P == 1 ? 1 : (P == 2 ? 4 : 14));
+    constexpr int NQS = QC ? (P == 1 ? 4 : (P == 2 ? 14 : 24))
+                           : (P == 1 ? 1 : (P == 2 ? 4 : (P == 3 ? 14 : 24)));
     if (DIM == 1) {
         const double kap = a.use_kappa ? a.kappa[e] : 0.0;
         const int nq = NQS;
@@ -506,7 +640,7 @@ __global__ void __launch_bounds__(128) k_element(ElemArgs a)
 #pragma unroll
         for (int k = 0; k < N; k++)
             fl[k] = a.f[m[k]];
-        double fq[NQ];
+        double fq[Shape<P>::NQT];
         for (int q = 0; q < T.nq_r; q++) {
             double s = 0.0;
 #pragma unroll
@@ -571,7 +705,7 @@ __global__ void __launch_bounds__(128) k_element(ElemArgs a)
             }
         }
         // load: vector path of phgQuadDofTimesBas, negated (elastic.c:212)
-        double fq[NQ][3];
+        double fq[Shape<P>::NQT][3];
         for (int q = 0; q < T.nq_r; q++) {
             double s0 = 0.0, s1 = 0.0, s2 = 0.0;
             for (int k = 0; k < N; k++) {
@@ -631,8 +765,8 @@ struct AdjArgs {
 template <int P>
 __global__ void __launch_bounds__(128) k_adj_mass(AdjArgs a)
 {
-    constexpr int N = P == 1 ? 4 : (P == 2 ? 10 : 20);
-    __shared__ Tables T;
+    constexpr int N = Shape<P>::N;
+    __shared__ TablesT<P> T;
     build_tables<P>(T, 1, -1, 1, false);
     int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= a.nelem)
@@ -652,7 +786,7 @@ __global__ void __launch_bounds__(128) k_adj_mass(AdjArgs a)
     for (int k = 0; k < N; k++)
         el[k] = a.u[m[k]] - a.ud[m[k]];
     // values of e_h at the mass points, then Me_i = vol sum_q w_q phi_i e_h
-    double eq[NQ];
+    double eq[Shape<P>::NQT];
     for (int q = 0; q < T.nq_m; q++) {
         double t = 0.0;
 #pragma unroll
@@ -689,9 +823,9 @@ __global__ void k_adj_rows(int64_t n, int N, const int64_t *off,
 template <int P>
 __global__ void __launch_bounds__(128) k_adj_grad(AdjArgs a)
 {
-    constexpr int N = P == 1 ? 4 : (P == 2 ? 10 : 20);
-    constexpr int NQS = P == 1 ? 1 : (P == 2 ? 4 : 14);
-    __shared__ Tables T;
+    constexpr int N = Shape<P>::N;
+    constexpr int NQS = P == 1 ? 1 : (P == 2 ? 4 : (P == 3 ? 14 : 24));
+    __shared__ TablesT<P> T;
     build_tables<P>(T, 1, -1, 0, false);
     int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= a.nelem)
@@ -742,9 +876,9 @@ __global__ void __launch_bounds__(128) k_dof_norms(int64_t nelem,
                                                    const double *v,
                                                    double *l2, double *h1)
 {
-    constexpr int N = P == 1 ? 4 : (P == 2 ? 10 : 20);
-    constexpr int NQS = P == 1 ? 1 : (P == 2 ? 4 : 14);
-    __shared__ Tables T;
+    constexpr int N = Shape<P>::N;
+    constexpr int NQS = P == 1 ? 1 : (P == 2 ? 4 : (P == 3 ? 14 : 24));
+    __shared__ TablesT<P> T;
     build_tables<P>(T, 1, -1, 1, false);
     int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nelem)
@@ -825,6 +959,22 @@ __global__ void k_dof_coords(int64_t nelem, int order, int N,
     double lam[4] = {0., 0., 0., 0.};
     if (b < 4)
         lam[b] = 1.0;
+    else if (order == 4) {
+        // lagrange.c:1671 P4_points: edge nodes from the first vertex, face
+        // node t near face vertex t, the centroid
+        if (b < 22) {
+            const int k = (b - 4) / 3, t = (b - 4) % 3;
+            lam[c_ev[k][0]] = 0.75 - 0.25 * t;
+            lam[c_ev[k][1]] = 0.25 + 0.25 * t;
+        }
+        else if (b < 34) {
+            const int f = (b - 22) / 3, t = (b - 22) % 3;
+            for (int i = 0; i < 3; i++)
+                lam[c_fv[f][i]] = i == t ? 0.5 : 0.25;
+        }
+        else
+            lam[0] = lam[1] = lam[2] = lam[3] = 0.25;
+    }
     else if (order == 2) {
         lam[c_ev[b - 4][0]] = 0.5;
         lam[c_ev[b - 4][1]] = 0.5;
@@ -1338,8 +1488,10 @@ int launch_element(const phg_hip_space *sp, const phg_hip_problem *pb,
             hipLaunchKernelGGL((k_element<1, 1>), grid, block, 0, s, a);
         else if (sp->order == 2)
             hipLaunchKernelGGL((k_element<2, 1>), grid, block, 0, s, a);
-        else
+        else if (sp->order == 3)
             hipLaunchKernelGGL((k_element<3, 1>), grid, block, 0, s, a);
+        else
+            hipLaunchKernelGGL((k_element<4, 1>), grid, block, 0, s, a);
     }
     else {
         if (sp->order == 1)
@@ -1408,11 +1560,16 @@ int check_problem(const phg_hip_space *sp, const phg_hip_problem *pb)
     if (pb->dim == 3 && (pb->use_kappa || pb->mass_coef != 0.0))
         return phg::set_error(PHG_HIP_E_ARG,
                               "kappa/mass terms are scalar-only");
-    if (pb->rhs_order > 6)
+    // rules up to order 6 for P1-P3, up to order 8 for P4 (scalar, without
+    // the quadrature cache)
+    const int maxq = sp->order == 4 ? 8 : 6;
+    if (sp->order == 4 && (pb->dim != 1 || pb->qcache))
+        return phg::set_error(PHG_HIP_E_ARG, "P4: scalar problems without qcache only");
+    if (pb->rhs_order > maxq)
         return phg::set_error(PHG_HIP_E_ARG,
-                              "load quadrature order %d > 6 unsupported",
-                              pb->rhs_order);
-    if (pb->mass_coef != 0.0 && 2 * sp->order > 6)
+                              "load quadrature order %d > %d unsupported",
+                              pb->rhs_order, maxq);
+    if (pb->mass_coef != 0.0 && 2 * sp->order > maxq)
         return phg::set_error(PHG_HIP_E_ARG, "mass order too high");
     if (pb->qcache && (pb->dim != 1 || pb->use_kappa || pb->mass_coef != 0.0 ||
                        (pb->rhs_order >= 0 && pb->rhs_order != 2 * sp->order)))
@@ -1706,11 +1863,11 @@ extern "C" int phg_hip_space_create(int32_t order, int64_t nvert,
     int rc;
     if ((rc = phg::require_ctx()))
         return rc;
-    if (out == nullptr || order < 1 || order > 3 || nvert <= 0 || nelem < 0 ||
+    if (out == nullptr || order < 1 || order > 4 || nvert <= 0 || nelem < 0 ||
         ndof <= 0 || verts == nullptr || elem_verts == nullptr ||
         map == nullptr || bdry == nullptr)
         return phg::set_error(PHG_HIP_E_ARG, "phg_hip_space_create: bad args");
-    int nb = order == 1 ? 4 : (order == 2 ? 10 : 20);
+    int nb = order == 1 ? 4 : (order == 2 ? 10 : (order == 3 ? 20 : 35));
     // host-side bounds checks so that no kernel indexes out of range
     for (int64_t i = 0; i < nelem * 4; i++)
         if (elem_verts[i] < 0 || elem_verts[i] >= nvert)
@@ -2181,8 +2338,10 @@ extern "C" int phg_hip_adjoint_logkappa(phg_hip_space *sp,
             hipLaunchKernelGGL(k_adj_mass<1>, g, b, 0, s, a);
         else if (sp->order == 2)
             hipLaunchKernelGGL(k_adj_mass<2>, g, b, 0, s, a);
-        else
+        else if (sp->order == 3)
             hipLaunchKernelGGL(k_adj_mass<3>, g, b, 0, s, a);
+        else
+            hipLaunchKernelGGL(k_adj_mass<4>, g, b, 0, s, a);
     }
     hipLaunchKernelGGL(k_sum_blocks, dim3(nparts), dim3(256), 0, s, ne,
                        (const double *)Jel, part);
@@ -2219,8 +2378,10 @@ extern "C" int phg_hip_adjoint_logkappa(phg_hip_space *sp,
             hipLaunchKernelGGL(k_adj_grad<1>, g, b, 0, s, a);
         else if (sp->order == 2)
             hipLaunchKernelGGL(k_adj_grad<2>, g, b, 0, s, a);
-        else
+        else if (sp->order == 3)
             hipLaunchKernelGGL(k_adj_grad<3>, g, b, 0, s, a);
+        else
+            hipLaunchKernelGGL(k_adj_grad<4>, g, b, 0, s, a);
     }
     PHG_HIP_CHECK(hipGetLastError());
     rc = phg::download(grad, dgrad, (size_t)ne * sizeof(double));
@@ -2288,8 +2449,13 @@ int dof_norms_run(const phg_hip_space *sp, const phg_hip_vec *v, double *dl,
                            (const double *)sp->d_verts,
                            (const int32_t *)sp->d_elem_verts,
                            (const int32_t *)sp->d_map, (const double *)v->d, dl, dh);
-    else
+    else if (sp->order == 3)
         hipLaunchKernelGGL(k_dof_norms<3>, g, b, 0, s, ne,
+                           (const double *)sp->d_verts,
+                           (const int32_t *)sp->d_elem_verts,
+                           (const int32_t *)sp->d_map, (const double *)v->d, dl, dh);
+    else
+        hipLaunchKernelGGL(k_dof_norms<4>, g, b, 0, s, ne,
                            (const double *)sp->d_verts,
                            (const int32_t *)sp->d_elem_verts,
                            (const int32_t *)sp->d_map, (const double *)v->d, dl, dh);

@@ -537,6 +537,7 @@ int phg_dof_nbas(int order)
     case 1: return 4;
     case 2: return 10;
     case 3: return 20;
+    case 4: return 35;
     }
     return -1;
 }
@@ -547,6 +548,7 @@ int64_t phg_dof_count(const phg_grid *g, int order)
     case 1: return g->nvert;
     case 2: return (int64_t)g->nvert + g->nedge;
     case 3: return (int64_t)g->nvert + 2 * (int64_t)g->nedge + g->nface;
+    case 4: return (int64_t)g->nvert + 3 * (int64_t)g->nedge + 3 * (int64_t)g->nface + g->nelem;
     }
     return -1;
 }
@@ -577,6 +579,32 @@ void phg_dof_map(const phg_grid *g, int order, int32_t *map)
             for (int k = 0; k < 4; k++)
                 m[16 + k] = (int32_t)(f0 + g->elem_faces[4 * e + k]);
         }
+        else if (order == 4) {
+            /* dof.c:4821 phgDofMapE2D_: vertex, edge, face, element blocks */
+            const int64_t f4 = (int64_t)g->nvert + 3 * (int64_t)g->nedge,
+                          e4 = f4 + 3 * (int64_t)g->nface;
+            for (int k = 0; k < 6; k++) {
+                /* dof.c:978 phgDofMapEdgeData: reversed when the edge runs
+                 * from the larger to the smaller global vertex */
+                int rev = v[phg_tet_edge_vertex[k][0]] > v[phg_tet_edge_vertex[k][1]];
+                for (int t = 0; t < 3; t++)
+                    m[4 + 3 * k + t] = (int32_t)(v0 + 3 * (int64_t)ed[k] + (rev ? 2 - t : t));
+            }
+            for (int k = 0; k < 4; k++) {
+                /* lagrange.c:1571 P4_map: the face's node t (near its
+                 * vertex t) goes to the slot of that vertex's rank among the
+                 * face's global vertex indices */
+                int32_t a = v[face_verts[k][0]], b = v[face_verts[k][1]],
+                        c = v[face_verts[k][2]];
+                int M[3] = {0, 0, 0};
+                if (a > b) M[0]++; else M[1]++;
+                if (a > c) M[0]++; else M[2]++;
+                if (b > c) M[1]++; else M[2]++;
+                for (int t = 0; t < 3; t++)
+                    m[22 + 3 * k + t] = (int32_t)(f4 + 3 * (int64_t)g->elem_faces[4 * e + k] + M[t]);
+            }
+            m[34] = (int32_t)(e4 + e);
+        }
     }
 }
 
@@ -591,6 +619,25 @@ static void node_lambda(int order, int b, double lam[4])
         return;
     }
     b -= 4;
+    if (order == 4) {
+        /* lagrange.c:1671 P4_points: edge nodes from the first vertex,
+         * face node t near face vertex t, the centroid */
+        if (b < 18) {
+            int k = b / 3, t = b % 3;
+            lam[phg_tet_edge_vertex[k][0]] = 0.75 - 0.25 * t;
+            lam[phg_tet_edge_vertex[k][1]] = 0.25 + 0.25 * t;
+            return;
+        }
+        b -= 18;
+        if (b < 12) {
+            int f = b / 3, t = b % 3;
+            for (int i = 0; i < 3; i++)
+                lam[fv[f][i]] = i == t ? 0.5 : 0.25;
+            return;
+        }
+        lam[0] = lam[1] = lam[2] = lam[3] = 0.25;
+        return;
+    }
     if (order == 2) {
         lam[phg_tet_edge_vertex[b][0]] = 0.5;
         lam[phg_tet_edge_vertex[b][1]] = 0.5;
@@ -659,6 +706,18 @@ void phg_dof_bdry(const phg_grid *g, int order, unsigned mask, uint8_t *flags)
                 (g->types_edge[i] & mask) != 0;
         for (int64_t i = 0; i < g->nface; i++)
             flags[f0 + i] = (g->types_face[i] & mask) != 0;
+    }
+    else if (order == 4) {
+        /* element-interior nodes are never boundary DOFs */
+        const int64_t f4 = v0 + 3 * (int64_t)g->nedge, e4 = f4 + 3 * (int64_t)g->nface;
+        for (int64_t i = 0; i < g->nedge; i++)
+            flags[v0 + 3 * i] = flags[v0 + 3 * i + 1] = flags[v0 + 3 * i + 2] =
+                (g->types_edge[i] & mask) != 0;
+        for (int64_t i = 0; i < g->nface; i++)
+            flags[f4 + 3 * i] = flags[f4 + 3 * i + 1] = flags[f4 + 3 * i + 2] =
+                (g->types_face[i] & mask) != 0;
+        for (int64_t i = 0; i < g->nelem; i++)
+            flags[e4 + i] = 0;
     }
     (void)nd;
 }

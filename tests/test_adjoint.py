@@ -57,7 +57,7 @@ class LogKappaObjective(torch.autograd.Function):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("p,n", [(1, 6), (2, 4), (3, 3)])
+@pytest.mark.parametrize("p,n", [(1, 6), (2, 4), (3, 3), (4, 2)])
 def test_adjoint_matches_cpu_adjoint(hip, p, n):
     pa = hip
     c = c3_case(n, p)

@@ -36,6 +36,7 @@ CASES = [
     ("sin", 2, dict(mass_coef=1.0)), ("sin", 3, dict(mass_coef=2.5)),
     ("sin", 1, dict(qcache=True)), ("sin", 2, dict(qcache=True)),
     ("sin", 3, dict(qcache=True)),
+    ("sin", 4, dict()), ("kappa", 4, dict(use_kappa=True)), ("sin", 4, dict(mass_coef=1.5)),
 ]
 
 
@@ -81,7 +82,7 @@ def test_elasticity_element_bitexact(hip):
 
 
 ASM_CASES = [("sin", 1, 4), ("sin", 2, 3), ("sin", 3, 2), ("kappa", 2, 3),
-             ("kappa", 3, 2), ("linear", 1, 5)]
+             ("kappa", 3, 2), ("linear", 1, 5), ("sin", 4, 2), ("kappa", 4, 2)]
 
 
 def _assemble_both(pa, c, pb):
@@ -137,7 +138,7 @@ def test_elasticity_csr_bitexact(hip, p):
 
 
 @pytest.mark.parametrize("kind,p,n", [("sin", 1, 8), ("sin", 2, 4), ("kappa", 2, 4),
-                                      ("kappa", 3, 3), ("linear", 2, 3)])
+                                      ("kappa", 3, 3), ("linear", 2, 3), ("kappa", 4, 3)])
 def test_pcg_matches_oracle(hip, kind, p, n):
     pa = hip
     c = poisson_case(n, p, kind)
@@ -326,7 +327,7 @@ def test_reliable_nonzero_guess_and_zero_rhs(hip):
     assert np.all(x2.numpy() == 0.0)
 
 
-@pytest.mark.parametrize("p", [1, 2, 3])
+@pytest.mark.parametrize("p", [1, 2, 3, 4])
 def test_dof_norms(hip, p):
     """phgDofNormL2 of a DOF and of its gradient: closed forms for v = x
     (1/3 and 1 on the unit cube) and the oracle's quadratic forms
@@ -428,7 +429,7 @@ def _fan_mesh(path, k):
         fh.write("\nelement type:\n" + "0\n" * len(el))
 
 
-@pytest.mark.parametrize("p", [1, 2, 3])
+@pytest.mark.parametrize("p", [1, 2, 3, 4])
 def test_high_valence_rows_bitexact(hip, tmp_path, p):
     """Rows with more distinct columns than the insertion kernel's hash table
     holds take its serial replay; the CSR stays bit for bit the oracle's."""
@@ -449,8 +450,8 @@ def test_high_valence_rows_bitexact(hip, tmp_path, p):
                       g.dof_map(p), nd, bd, gv, f, kappa)
     rp, ci, va, rhs = A.download()
     # the centre vertex's row exceeds the table (P1, P2: 16 lanes and 128
-    # entries per row; P3: 32 lanes, 256 entries)
-    assert np.diff(Ao.rowptr).max() > {1: 128, 2: 128, 3: 256}[p]
+    # entries per row; P3: 32 lanes, 128 entries; P4: 64 lanes, 512 entries)
+    assert np.diff(Ao.rowptr).max() > {1: 128, 2: 128, 3: 128, 4: 512}[p]
     assert np.array_equal(rp, Ao.rowptr)
     assert np.array_equal(ci, Ao.cols) and _bits_equal(va, Ao.vals)
     assert _bits_equal(rhs, Ao.rhs)

@@ -19,7 +19,7 @@ def _simplex_moment(alpha):
     return 6.0 * np.prod([math.factorial(a) for a in alpha]) / math.factorial(sum(alpha) + 3)
 
 
-@pytest.mark.parametrize("order", range(0, 7))
+@pytest.mark.parametrize("order", range(0, 9))
 def test_quad_weights_and_exactness(order):
     pts, w = orc.quad3d(order)
     assert abs(w.sum() - 1.0) < 1e-14
@@ -32,7 +32,7 @@ def test_quad_weights_and_exactness(order):
         assert abs(val - _simplex_moment(alpha)) < 1e-13, (order, alpha)
 
 
-@pytest.mark.parametrize("order", range(0, 7))
+@pytest.mark.parametrize("order", range(0, 9))
 def test_quad_test_c_closed_form(order):
     # test/quad_test.c:129-153: sum_i lambda_i^p -> 4*6/((p+1)(p+2)(p+3))
     pts, w = orc.quad3d(order)
@@ -42,7 +42,7 @@ def test_quad_test_c_closed_form(order):
 
 
 def test_quad_point_counts():
-    assert [orc.quad3d(k)[0].shape[0] for k in range(7)] == [1, 1, 4, 8, 14, 14, 24]
+    assert [orc.quad3d(k)[0].shape[0] for k in range(9)] == [1, 1, 4, 8, 14, 14, 24, 35, 46]
 
 # ----------------------------------------------------------------- bases
 
@@ -61,10 +61,20 @@ def _nodes(p):
                 l = np.zeros(4); l[a], l[b] = wa, wb; nodes.append(l)
         for f in F:
             l = np.zeros(4); l[list(f)] = 1 / 3; nodes.append(l)
+    if p == 4:
+        # edge nodes from the first vertex, face nodes near each face vertex
+        # in face-vertex order, the centroid (lagrange.c:1671 P4_points)
+        for a, b in E:
+            for wa in (.75, .5, .25):
+                l = np.zeros(4); l[a], l[b] = wa, 1 - wa; nodes.append(l)
+        for f in F:
+            for t in range(3):
+                l = np.zeros(4); l[list(f)] = .25; l[f[t]] = .5; nodes.append(l)
+        nodes.append(np.full(4, .25))
     return np.array(nodes)
 
 
-@pytest.mark.parametrize("p", [1, 2, 3])
+@pytest.mark.parametrize("p", [1, 2, 3, 4])
 def test_bases_nodal_and_partition_of_unity(p):
     N = _nodes(p)
     V = np.array([orc.bas(p, l) for l in N])
@@ -75,7 +85,7 @@ def test_bases_nodal_and_partition_of_unity(p):
         assert abs(orc.bas(p, l).sum() - 1.0) < 1e-14
 
 
-@pytest.mark.parametrize("p", [1, 2, 3])
+@pytest.mark.parametrize("p", [1, 2, 3, 4])
 def test_lambda_gradients_finite_difference(p):
     rng = np.random.default_rng(1)
     h = 1e-6
@@ -109,7 +119,7 @@ def _grad_lambda_x(x):
 
 
 def _brute_stiffness(p, x, kappa=1.0):
-    pts, w = orc.quad3d(6)
+    pts, w = orc.quad3d(max(6, 2 * p))
     GL = _grad_lambda_x(x)
     vol = abs(np.linalg.det(x[1:] - x[0])) / 6
     nb = orc.nbas(p)
@@ -121,7 +131,7 @@ def _brute_stiffness(p, x, kappa=1.0):
 
 
 def _brute_mass(p, x):
-    pts, w = orc.quad3d(6)
+    pts, w = orc.quad3d(max(6, 2 * p))
     vol = abs(np.linalg.det(x[1:] - x[0])) / 6
     V = np.array([orc.bas(p, pt) for pt in pts])
     return vol * (V.T * w) @ V
@@ -134,7 +144,7 @@ def _rand_tet(rng):
             return x
 
 
-@pytest.mark.parametrize("p", [1, 2, 3])
+@pytest.mark.parametrize("p", [1, 2, 3, 4])
 def test_stiffness_and_mass_brute_force(p):
     rng = np.random.default_rng(3)
     for _ in range(4):
@@ -165,13 +175,13 @@ def test_kuhn_p1_closed_form():
     assert np.allclose(Mb, vol / 20 * (np.ones((4, 4)) + np.eye(4)), atol=1e-15)
 
 
-@pytest.mark.parametrize("p", [1, 2, 3])
+@pytest.mark.parametrize("p", [1, 2, 3, 4])
 def test_load_f_one_is_basis_integral(p):
     rng = np.random.default_rng(4)
     x = _rand_tet(rng)
     nb = orc.nbas(p)
     _, b = orc.element(orc.problem(p), x, f_loc=np.ones(nb))
-    pts, w = orc.quad3d(6)
+    pts, w = orc.quad3d(max(6, 2 * p))
     vol = abs(np.linalg.det(x[1:] - x[0])) / 6
     ref = vol * sum(w[q] * orc.bas(p, pts[q]) for q in range(len(w)))
     assert np.allclose(b, ref, rtol=1e-12, atol=1e-14)
