@@ -53,6 +53,7 @@ class _GridStruct(C.Structure):
         ("elem_faces", _P(_i32)), ("elem_neigh", _P(_i32)),
         ("elem_bound", _P(C.c_uint8)), ("types_vert", _P(C.c_uint8)),
         ("types_edge", _P(C.c_uint8)), ("types_face", _P(C.c_uint8)),
+        ("elem_type", _P(C.c_uint8)), ("trav", _P(_i32)),
     ]
 
 
@@ -85,6 +86,7 @@ _sig(grid_lib, "phg_grid_read_albert", _P(_GridStruct), [_cp])
 _sig(grid_lib, "phg_grid_write_albert", C.c_int, [_P(_GridStruct), _cp])
 _sig(grid_lib, "phg_grid_refine_toward", _P(_GridStruct),
      [_P(_GridStruct), _P(_dbl), C.c_int, _dbl, _dbl, _P(C.c_int)])
+_sig(grid_lib, "phg_grid_bisect", _P(_GridStruct), [_P(_GridStruct), _P(_i32)])
 _sig(grid_lib, "phg_grid_free", None, [_P(_GridStruct)])
 _sig(grid_lib, "phg_dof_nbas", C.c_int, [C.c_int])
 _sig(grid_lib, "phg_dof_count", _i64, [_P(_GridStruct), C.c_int])
@@ -233,6 +235,33 @@ class Grid:
         bc = (C.c_int * 6)(*bcode)
         return Grid(grid_lib.phg_grid_refine_toward(self._p, _ptr(pt, _dbl), int(levels),
                                                     float(radius), float(shrink), bc))
+
+    def bisect(self, marks) -> "Grid":
+        """PHG's bisection (refine.c:2346 phgRefineMarkedElements): every
+        element e bisected at least marks[e] times, with PHG's closure,
+        vertex creation order and index inheritance (phg_grid_bisect)."""
+        m = np.ascontiguousarray(np.broadcast_to(marks, (self.nelem,)), dtype=np.int32)
+        p = grid_lib.phg_grid_bisect(self._p, _ptr(m, _i32))
+        if not p:
+            raise RuntimeError("phg_grid_bisect failed")
+        return Grid(p)
+
+    def refine_all(self, level: int) -> "Grid":
+        """refine.c:2541 phgRefineAllElements(g, level)."""
+        return self.bisect(np.full(self.nelem, int(level), dtype=np.int32))
+
+    def elem_type(self):
+        return np.ctypeslib.as_array(self.s.elem_type, (self.nelem,)).copy()
+
+    def traversal(self):
+        """Element indices in PHG's tree traversal order (identity for a flat
+        mesh)."""
+        if not self.s.trav:
+            return np.arange(self.nelem, dtype=np.int32)
+        return np.ctypeslib.as_array(self.s.trav, (self.nelem,)).copy()
+
+    def elem_bound(self):
+        return np.ctypeslib.as_array(self.s.elem_bound, (self.nelem, 4)).copy()
 
     def write_albert(self, path: str) -> None:
         if grid_lib.phg_grid_write_albert(self._p, path.encode()) != 0:

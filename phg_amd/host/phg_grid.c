@@ -102,6 +102,9 @@ static phg_grid *grid_alloc(int nvert, int nelem)
     g->elem_verts = xmalloc((size_t)nelem * 4 * sizeof(int32_t));
     g->elem_bound = xmalloc((size_t)nelem * 4);
     memset(g->elem_bound, 0, (size_t)nelem * 4);
+    g->elem_type = xmalloc((size_t)nelem + 1);
+    memset(g->elem_type, 0, (size_t)nelem + 1);
+    g->trav = NULL;
     return g;
 }
 
@@ -118,6 +121,8 @@ void phg_grid_free(phg_grid *g)
     free(g->types_vert);
     free(g->types_edge);
     free(g->types_face);
+    free(g->elem_type);
+    free(g->trav);
     free(g);
 }
 
@@ -289,7 +294,13 @@ void phg_grid_number(phg_grid *g)
         }
     }
 
-    /* boundary types OR-ed from faces onto vertices and edges */
+    phg_grid_types(g);
+}
+
+/* boundary types OR-ed from faces onto vertices and edges (utils.c:962) */
+void phg_grid_types(phg_grid *g)
+{
+    const int64_t ne = g->nelem;
     free(g->types_vert);
     free(g->types_edge);
     free(g->types_face);
@@ -330,21 +341,24 @@ int phg_grid_write_albert(const phg_grid *g, const char *path)
     for (int64_t i = 0; i < g->nvert; i++)
         fprintf(f, "%.17g %.17g %.17g\n", g->verts[3 * i], g->verts[3 * i + 1],
                 g->verts[3 * i + 2]);
+    /* elements in traversal order, as albert.c:537 phgExportALBERT writes
+     * them */
     fprintf(f, "\nelement vertices:\n");
-    for (int64_t e = 0; e < g->nelem; e++) {
+    for (int64_t t = 0; t < g->nelem; t++) {
+        const int64_t e = g->trav ? g->trav[t] : t;
         const int32_t *v = g->elem_verts + 4 * e;
         fprintf(f, "%d %d %d %d\n", v[0], v[1], v[2], v[3]);
     }
     fprintf(f, "\nelement boundaries:\n");
-    for (int64_t e = 0; e < g->nelem; e++) {
+    for (int64_t t = 0; t < g->nelem; t++) {
+        const int64_t e = g->trav ? g->trav[t] : t;
         const uint8_t *b = g->elem_bound + 4 * e;
         fprintf(f, "%d %d %d %d\n", bdry_to_albert(b[0]), bdry_to_albert(b[1]),
                 bdry_to_albert(b[2]), bdry_to_albert(b[3]));
     }
-    /* the type code only steers later bisection, not the numbering */
     fprintf(f, "\nelement type:\n");
-    for (int64_t e = 0; e < g->nelem; e++)
-        fprintf(f, "0\n");
+    for (int64_t t = 0; t < g->nelem; t++)
+        fprintf(f, "%d\n", g->elem_type[g->trav ? g->trav[t] : t]);
     return fclose(f) == 0 ? 0 : -1;
 }
 
@@ -432,18 +446,40 @@ static void refine_init_marking(phg_grid *g)
             k = 2;
         if (comp_edge(len, ed, k, 5) < 0)
             k = 5;
+        /* types as ALBERT codes: D 0, F 1, M 3, O 4 */
         switch (j * 10 + k) {
-        case 32: case 42: case 52:
+        case 31:
+            g->elem_type[e] = 1;
+            break;
+        case 32:
+            g->elem_type[e] = 0;
             permute_vertices(g, e, 2, 3);
             break;
         case 35:
+            g->elem_type[e] = 3;
             permute_vertices(g, e, 0, 1);
             break;
+        case 41:
+            g->elem_type[e] = 0;
+            break;
+        case 42:
+            g->elem_type[e] = 1;
+            permute_vertices(g, e, 2, 3);
+            break;
         case 45:
+            g->elem_type[e] = 3;
             permute_vertices(g, e, 2, 3);
             permute_vertices(g, e, 0, 1);
             break;
-        default:                        /* 31, 41, 51, 55: no permutation */
+        case 51:
+            g->elem_type[e] = 3;
+            break;
+        case 52:
+            g->elem_type[e] = 3;
+            permute_vertices(g, e, 2, 3);
+            break;
+        default:                        /* 55 */
+            g->elem_type[e] = 4;
             break;
         }
     }
@@ -483,7 +519,8 @@ phg_grid *phg_grid_read_albert(const char *path)
                 g = grid_alloc(nv, ne);
             int what = !strcmp(p, "vertex coordinates") ? 0 :
                        !strcmp(p, "element vertices") ? 1 :
-                       !strcmp(p, "element boundaries") ? 2 : 3;
+                       !strcmp(p, "element boundaries") ? 2 :
+                       !strcmp(p, "element type") ? 4 : 3;
             if (!strcmp(p, "element type"))
                 typed = 1;
             int64_t cnt = what == 0 ? nv : ne;
@@ -509,6 +546,12 @@ phg_grid *phg_grid_read_albert(const char *path)
                         goto fail;
                     for (int a = 0; a < 4; a++)
                         g->elem_bound[4 * i + a] = bdry_from_albert(b[a]);
+                }
+                else if (what == 4) {
+                    int t;
+                    if (sscanf(line, "%d", &t) != 1 || t < 0 || t > 4)
+                        goto fail;
+                    g->elem_type[i] = (uint8_t)t;
                 }
             }
         }

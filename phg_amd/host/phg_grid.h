@@ -46,6 +46,13 @@ typedef struct phg_grid {
     uint8_t *types_vert;    /* [nvert] */
     uint8_t *types_edge;    /* [nedge] */
     uint8_t *types_face;    /* [nface] */
+    /* bisection type of each element, ALBERT code (albert.c:36): 0 DIAGONAL,
+     * 1 FACE, 2 EDGE, 3 MIXED, 4 OPPOSITE */
+    uint8_t *elem_type;     /* [nelem] */
+    /* element index at each position of PHG's tree traversal (leaves in
+     * depth-first order, utils.c:2519 phgTraverseElements); NULL: index order
+     * (a flat, imported mesh) */
+    int32_t *trav;          /* [nelem] or NULL */
 } phg_grid;
 
 /* Local edge -> vertex table (elem-info.c:62 phgTetEdgeVertex). */
@@ -74,8 +81,25 @@ phg_grid *phg_grid_refine_toward(const phg_grid *g, const double pt[3],
                                  int levels, double radius, double shrink,
                                  const int bcode[6]);
 
+/* PHG's bisection, refine.c:2346 phgRefineMarkedElements in a serial run:
+ * every element e is bisected at least marks[e] times (by element index),
+ * with PHG's conforming closure (refine.c:1661 refine_path), vertex creation
+ * order (refine.c:1230 new_vertex), child vertex order and types
+ * (refine.c:749 divide_simplex), and index inheritance (refine.c:2176
+ * update_indices): the child holding the refinement edge's smaller vertex
+ * keeps its parent's element index and the halves of the parent's edges and
+ * faces, the other child takes the next new element index, and new edges and
+ * faces are numbered in tree traversal order (utils.c:4282, :4499).  g needs
+ * its numbering and element types (any grid made by this layer has them).
+ * Returns the refined grid (elements by index, with its traversal order) or
+ * NULL on error. */
+phg_grid *phg_grid_bisect(const phg_grid *g, const int32_t *marks);
+
 /* Computes neighbours, edge and face numbering and boundary types. */
 void phg_grid_number(phg_grid *g);
+/* Boundary types of vertices, edges and faces from the elements' face types
+ * and neighbours (utils.c:962 types_callback). */
+void phg_grid_types(phg_grid *g);
 void phg_grid_free(phg_grid *g);
 
 /* Lagrange Pk (k = 1, 2, 3) scalar DOF layout. */

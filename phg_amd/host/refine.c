@@ -365,6 +365,10 @@ phg_grid *phg_grid_refine_toward(const phg_grid *g, const double pt[3],
     s.xyz = NULL;
     o->elem_verts = xalloc((size_t)nlive * 4 * sizeof(int32_t));
     o->elem_bound = xalloc((size_t)nlive * 4);
+    /* its ALBERT file carries types (all 0), so PHG imports it unpermuted */
+    o->elem_type = calloc((size_t)nlive + 1, 1);
+    if (!o->elem_type)
+        abort();
     int64_t k = 0;
     static const int fv[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
     for (int64_t e = 0; e < s.ne; e++) {
