@@ -87,6 +87,9 @@ _sig(grid_lib, "phg_grid_write_albert", C.c_int, [_P(_GridStruct), _cp])
 _sig(grid_lib, "phg_grid_refine_toward", _P(_GridStruct),
      [_P(_GridStruct), _P(_dbl), C.c_int, _dbl, _dbl, _P(C.c_int)])
 _sig(grid_lib, "phg_grid_bisect", _P(_GridStruct), [_P(_GridStruct), _P(_i32)])
+_sig(grid_lib, "phg_grid_mark", C.c_int,
+     [_P(_GridStruct), C.c_int, _P(_dbl), _dbl, _dbl, C.c_int, _P(_i32)])
+MARK_NONE, MARK_ALL, MARK_MAX, MARK_GERS, MARK_EQDIST = range(5)
 _sig(grid_lib, "phg_grid_free", None, [_P(_GridStruct)])
 _sig(grid_lib, "phg_dof_nbas", C.c_int, [C.c_int])
 _sig(grid_lib, "phg_dof_count", _i64, [_P(_GridStruct), C.c_int])
@@ -245,6 +248,19 @@ class Grid:
         if not p:
             raise RuntimeError("phg_grid_bisect failed")
         return Grid(p)
+
+    def mark(self, strategy: int, err=None, theta: float = 0.5, tol: float = 1.0,
+             depth: int = 1) -> np.ndarray:
+        """mark.c:912 phgMarkElements with an element-wise indicator; returns
+        the marks by element index (phg_grid_mark)."""
+        m = np.zeros(self.nelem, dtype=np.int32)
+        e = None if err is None else np.ascontiguousarray(err, dtype=np.float64)
+        if e is not None and e.size != self.nelem:
+            raise ValueError("one indicator per element")
+        if grid_lib.phg_grid_mark(self._p, int(strategy), _ptr(e, _dbl), float(theta),
+                                  float(tol), int(depth), _ptr(m, _i32)) != 0:
+            raise ValueError("phg_grid_mark: bad arguments")
+        return m
 
     def refine_all(self, level: int) -> "Grid":
         """refine.c:2541 phgRefineAllElements(g, level)."""

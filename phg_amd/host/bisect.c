@@ -27,6 +27,7 @@
  * when its boundary type is PHG_BT_INTERIOR (HasLocalNeighbour). */
 #include "phg_grid.h"
 
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -819,4 +820,115 @@ done:
     free(B->stack);
     pm_free(&B->mid);
     return o;
+}
+
+/*------------------------------------------------------------- marking
+ * mark.c:912 phgMarkElements for an element-wise (P0) error indicator,
+ * refinement strategies only (this build does not coarsen).  Sums and
+ * norms run over the elements in index order, as ForAllElements does. */
+
+/* mark.c:48 part_sum, element data, refine: the sum of the indicators
+ * above mx * factor */
+static double part_sum(int64_t n, const double *err, double mx, double factor)
+{
+    const double tmp = mx * factor;
+    double sum = 0.0;
+    for (int64_t i = 0; i < n; i++)
+        if (err[i] > tmp)
+            sum += err[i];
+    return sum;
+}
+
+/* mark.c:190 mark_cal_factor, refine branch: bisection on the factor until
+ * the partial sum repeats an end value, meets eta, or the interval is below
+ * 1e-4; *first stands for the function's static call counter */
+static double cal_factor(int64_t n, const double *err, double mx, double eta, double low,
+                         double up, int *first, double *low_sum, double *up_sum)
+{
+    for (;;) {
+        const double sm = (low + up) * 0.5;
+        double c2;
+        if (!*first) {
+            c2 = part_sum(n, err, mx, sm);
+        }
+        else {
+            *low_sum = part_sum(n, err, mx, low);
+            c2 = part_sum(n, err, mx, sm);
+            *up_sum = part_sum(n, err, mx, up);
+            *first = 0;
+        }
+        if (*low_sum == c2 || c2 == *up_sum || c2 == eta || up - low < 0.0001)
+            return sm;
+        if (c2 > eta) {
+            *low_sum = c2;
+            low = sm;
+        }
+        else {
+            *up_sum = c2;
+            up = sm;
+        }
+    }
+}
+
+int phg_grid_mark(const phg_grid *g, int strategy, const double *err, double theta,
+                  double tol, int depth, int32_t *marks)
+{
+    if (g == NULL || marks == NULL || depth <= 0 ||
+        (strategy != PHG_MARK_ALL && strategy != PHG_MARK_NONE && err == NULL))
+        return -1;
+    const int64_t n = g->nelem;
+    for (int64_t i = 0; i < n; i++)        /* mark.c:330 mark_init */
+        marks[i] = 0;
+    switch (strategy) {
+    case PHG_MARK_ALL:
+        for (int64_t i = 0; i < n; i++)
+            marks[i] = depth;
+        return 0;
+    case PHG_MARK_NONE:
+        return 0;
+    case PHG_MARK_MAX: {                    /* mark.c:684 mark_refine_max_elem */
+        double mx = 0.0;
+        for (int64_t i = 0; i < n; i++)
+            if (fabs(err[i]) > mx)
+                mx = fabs(err[i]);
+        const double tmp = mx * theta;
+        if (tmp <= 0.)
+            return 0;
+        for (int64_t i = 0; i < n; i++)
+            if (err[i] > tmp)
+                marks[i] = depth;
+        return 0;
+    }
+    case PHG_MARK_GERS: {                   /* mark.c:406 mark_refine_gers_elem */
+        if (theta < 0. || theta > 1.)
+            return -1;
+        double pe = 0.0, l1 = 0.0;
+        for (int64_t i = 0; i < n; i++) {
+            if (fabs(err[i]) > pe)
+                pe = fabs(err[i]);
+            l1 += fabs(err[i]);
+        }
+        if (pe <= 0.0)
+            return 0;
+        const double flag = l1 * theta;
+        int first = 1;
+        double ls = 0.0, us = 0.0;
+        const double gama = cal_factor(n, err, pe, flag, 0.0, 1.0, &first, &ls, &us);
+        pe *= gama;
+        for (int64_t i = 0; i < n; i++)
+            if (err[i] > pe)
+                marks[i] = depth;
+        return 0;
+    }
+    case PHG_MARK_EQDIST: {                 /* mark.c:646 mark_refine_eqdist */
+        if (!(tol > 0.) || theta < 0.)
+            return -1;
+        const double flag = theta * tol;
+        for (int64_t i = 0; i < n; i++)
+            if (err[i] > flag)
+                marks[i] = depth;
+        return 0;
+    }
+    }
+    return -1;
 }

@@ -95,6 +95,18 @@ phg_grid *phg_grid_refine_toward(const phg_grid *g, const double pt[3],
  * NULL on error. */
 phg_grid *phg_grid_bisect(const phg_grid *g, const int32_t *marks);
 
+/* mark.c:912 phgMarkElements with an element-wise error indicator err
+ * (indexed by element), refinement strategies: ALL marks every element,
+ * MAX those above theta * max|err| (mark.c:684), GERS the Dorfler bulk found
+ * by mark.c:190's bisection on the factor with theta the bulk fraction
+ * (mark.c:406), EQDIST those above theta * tol (mark.c:646).  marks[e] =
+ * depth for marked elements, 0 otherwise.  Returns 0, or -1 on bad
+ * arguments. */
+enum { PHG_MARK_NONE = 0, PHG_MARK_ALL = 1, PHG_MARK_MAX = 2, PHG_MARK_GERS = 3,
+       PHG_MARK_EQDIST = 4 };
+int phg_grid_mark(const phg_grid *g, int strategy, const double *err, double theta,
+                  double tol, int depth, int32_t *marks);
+
 /* Computes neighbours, edge and face numbering and boundary types. */
 void phg_grid_number(phg_grid *g);
 /* Boundary types of vertices, edges and faces from the elements' face types

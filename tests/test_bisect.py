@@ -202,3 +202,46 @@ def test_marks_and_failures():
     marks[5] = 1
     h = g.bisect(marks)
     assert h.nelem > g.nelem + 1
+
+
+def test_marking_strategies_known_answers():
+    """mark.c:912 phgMarkElements on cube.dat's six elements, answers worked
+    out by hand from mark.c's steps."""
+    g = _cube()
+    # MAX (mark.c:684): theta * max = 0.5 * 6 = 3, mark err > 3
+    err = np.array([1.0, 2.0, 3.0, 4.0, 5.0, 6.0])
+    assert list(g.mark(pa.MARK_MAX, err, theta=0.5, depth=2)) == [0, 0, 0, 2, 2, 2]
+    # EQDIST (mark.c:646): theta * tol = 0.5 * 4 = 2, mark err > 2
+    assert list(g.mark(pa.MARK_EQDIST, err, theta=0.5, tol=4.0)) == [0, 0, 1, 1, 1, 1]
+    assert list(g.mark(pa.MARK_ALL, depth=3)) == [3] * 6
+    # GERS (mark.c:406, :190): err = [1 2 3 4 0 0], theta 0.5: eta = 0.5 * 10
+    # = 5, max 4.  Factor 0.5: S(0) = 10, S(.5) = 3+4 = 7, S(1) = 0; 7 > 5 so
+    # [.5, 1]; S(.75) = 4 <= 5 so [.5, .75]; S(.625) = 7 = S(low): stop at
+    # .625, mark err > 2.5
+    err = np.array([1.0, 2.0, 3.0, 4.0, 0.0, 0.0])
+    assert list(g.mark(pa.MARK_GERS, err, theta=0.5)) == [0, 0, 1, 1, 0, 0]
+    # err = [10 9 1 1 1 1], theta 0.4: eta = 9.2, max 10.  S(.5) = 19 > 9.2
+    # so [.5, 1]; S(.75) = 19 = S(low): stop, mark err > 7.5 -- both large
+    # elements, although 10 alone exceeds eta
+    err = np.array([10.0, 9.0, 1.0, 1.0, 1.0, 1.0])
+    assert list(g.mark(pa.MARK_GERS, err, theta=0.4)) == [1, 1, 0, 0, 0, 0]
+    with pytest.raises(ValueError):
+        g.mark(pa.MARK_MAX, None)
+
+
+def test_mark_then_bisect_refines_marked_elements():
+    g = _cube().refine_all(3)
+    rng = np.random.default_rng(8)
+    err = rng.uniform(0.0, 1.0, g.nelem)
+    m = g.mark(pa.MARK_GERS, err, theta=0.3, depth=2)
+    h = g.bisect(m)
+    # every marked element is cut into at least 4 leaves
+    px = g.verts()[g.elem_verts()]
+    cx = h.verts()[h.elem_verts()].mean(axis=1)
+    inside = np.zeros(g.nelem, dtype=int)
+    for i in range(g.nelem):
+        X = np.hstack([px[i], np.ones((4, 1))])
+        lam = np.linalg.solve(X.T, np.vstack([cx.T, np.ones(h.nelem)]))
+        inside[i] = (lam.min(axis=0) > -1e-12).sum()
+    assert (inside[m > 0] >= 4).all() and (inside >= 1).all()
+    assert inside.sum() == h.nelem
