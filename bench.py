@@ -1,9 +1,10 @@
 """Benchmark of the MI355X finite-element hot path.
 
 Workload (default): P2 variable-coefficient Poisson  -div(kappa grad u) = 1,
-u = 0 on the boundary of the unit cube, on a 64^3 x 6 Kuhn mesh (1,572,864
-tetrahedra, 2,146,689 DOFs), kappa_e = exp(g_e) with g_e ~ N(0, 0.5^2) i.i.d.
-per element (seeded).  FP64 assembly on the GPU, then mixed-precision
+u = 0 on the boundary of the unit cube, on the 64^3 x 6 mesh that PHG's
+refinement makes of test/cube.dat in 18 bisections (1,572,864 tetrahedra,
+2,146,689 DOFs, PHG's numbering), kappa_e = exp(g_e) with g_e ~ N(0, 0.5^2)
+i.i.d. per element in element-index order (seeded).  FP64 assembly on the GPU, then mixed-precision
 Jacobi-PCG: FP32 SpMV and vectors, with the residual replaced by the FP64
 residual b - A x (x accumulated in FP64) at reliable-update points
 (--mixed reliable, the default), or restarted FP32 PCG solves under FP64
@@ -78,7 +79,8 @@ def measured_traffic(kernel_key, nnz):
 def build_problem(n, p, sigma, seed):
     import phg_amd as pa
     t0 = time.perf_counter()
-    g = pa.Grid.kuhn(n)
+    # the mesh as PHG builds it: test/cube.dat bisected 3 log2(n) times
+    g = pa.Grid.phg_cube(n) if n & (n - 1) == 0 else pa.Grid.kuhn(n)
     nd = g.ndof(p)
     bd = g.dof_bdry(p)
     rng = np.random.default_rng(seed)
@@ -394,7 +396,9 @@ def main():
         "dtype": "f32+f64",
         "data": "synthetic",
         "config": {
-            "workload": (f"P2 -div(kappa grad u)=1 on {args.n}^3x6 Kuhn mesh "
+            "workload": (f"P2 -div(kappa grad u)=1 on {args.n}^3x6 mesh "
+                         + ("(test/cube.dat + PHG bisections) " if args.n & (args.n - 1) == 0
+                            else "(structured Kuhn) ") +
                          f"({g.nelem} tets, {nd} DOFs), kappa=exp(N(0,{args.sigma}^2)) "
                          f"seed {args.seed}; " + (
                              f"FP32 Jacobi-PCG with reliable FP64 residual updates "

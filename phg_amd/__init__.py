@@ -84,8 +84,6 @@ def _sig(lib, name, res, args):
 _sig(grid_lib, "phg_grid_new_kuhn", _P(_GridStruct), [_i32, _P(C.c_int)])
 _sig(grid_lib, "phg_grid_read_albert", _P(_GridStruct), [_cp])
 _sig(grid_lib, "phg_grid_write_albert", C.c_int, [_P(_GridStruct), _cp])
-_sig(grid_lib, "phg_grid_refine_toward", _P(_GridStruct),
-     [_P(_GridStruct), _P(_dbl), C.c_int, _dbl, _dbl, _P(C.c_int)])
 _sig(grid_lib, "phg_grid_bisect", _P(_GridStruct), [_P(_GridStruct), _P(_i32)])
 _sig(grid_lib, "phg_grid_mark", C.c_int,
      [_P(_GridStruct), C.c_int, _P(_dbl), _dbl, _dbl, C.c_int, _P(_i32)])
@@ -230,14 +228,36 @@ class Grid:
     def read_albert(cls, path: str):
         return cls(grid_lib.phg_grid_read_albert(path.encode()))
 
-    def refine_toward(self, point, levels: int, radius: float = 0.45, shrink: float = 1.0,
-                      bcode=(1, 1, 1, 1, 1, 1)) -> "Grid":
-        """Locally refined copy: `levels` bisection sweeps (conforming closure)
-        of the elements within radius * shrink^l of `point` (Kuhn meshes)."""
-        pt = np.ascontiguousarray(point, dtype=np.float64)
-        bc = (C.c_int * 6)(*bcode)
-        return Grid(grid_lib.phg_grid_refine_toward(self._p, _ptr(pt, _dbl), int(levels),
-                                                    float(radius), float(shrink), bc))
+    @classmethod
+    def phg_cube(cls, n: int, bcode=(1, 1, 1, 1, 1, 1)):
+        """The n^3 x 6 mesh as PHG builds it: test/cube.dat (6 Kuhn
+        tetrahedra, which kuhn(1) reproduces with its boundary codes and
+        types) bisected 3 log2(n) times by one phgRefineAllElements call, so
+        vertices, elements, edges and faces carry PHG's numbering.  n a power
+        of 2."""
+        lev = int(n).bit_length() - 1
+        if n < 1 or (1 << lev) != n:
+            raise ValueError("phg_cube: n must be a power of 2")
+        g = cls.kuhn(1, bcode)
+        return g.refine_all(3 * lev) if lev > 0 else g
+
+    def refine_toward(self, point, levels: int, radius: float = 0.45,
+                      shrink: float = 1.0) -> "Grid":
+        """Local refinement toward `point` with PHG's bisection: sweep l marks
+        (depth 1) every element whose centroid lies within radius * shrink^l
+        of the point or which contains it, and refines them with PHG's
+        conforming closure (phg_grid_bisect)."""
+        pt = np.asarray(point, dtype=np.float64)
+        g = self
+        for lev in range(int(levels)):
+            x = g.verts()[g.elem_verts()]
+            near = np.linalg.norm(x.mean(axis=1) - pt, axis=1) < radius * shrink ** lev
+            X = np.concatenate([x, np.ones((g.nelem, 4, 1))], axis=2)
+            rhs = np.broadcast_to(np.append(pt, 1.0)[:, None], (g.nelem, 4, 1))
+            lam = np.linalg.solve(np.transpose(X, (0, 2, 1)), rhs)[..., 0]
+            inside = lam.min(axis=1) >= -1e-12
+            g = g.bisect((near | inside).astype(np.int32))
+        return g
 
     def bisect(self, marks) -> "Grid":
         """PHG's bisection (refine.c:2346 phgRefineMarkedElements): every

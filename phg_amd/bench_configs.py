@@ -2,8 +2,8 @@
 
     python -m phg_amd.bench_configs c1 c2asm c3 c4 c5
 
-c1  P1 Poisson -Lap u = f, u = sin(pi x) sin(pi y) sin(pi z), 32^3 x 6 Kuhn
-    mesh (35,937 DOFs): assembly + FP64 Jacobi-PCG to rtol 1e-10, error
+c1  P1 Poisson -Lap u = f, u = sin(pi x) sin(pi y) sin(pi z), 32^3 x 6 mesh
+    (test/cube.dat + 15 PHG bisections, 35,937 DOFs): assembly + FP64 Jacobi-PCG to rtol 1e-10, error
     norms of u_h - I_h u.
 c2asm  assembly + Dirichlet elimination of the headline P2 kappa-Poisson
     problem (64^3 x 6, 2,146,689 DOFs).
@@ -49,7 +49,7 @@ def _timed(fn, reps=3, warm=1):
 def c1():
     import phg_amd as pa
     n, p = 32, 1
-    g = pa.Grid.kuhn(n)
+    g = pa.Grid.phg_cube(n)          # test/cube.dat + 15 bisections
     xyz = g.dof_coords(p)
     u_ex = np.sin(np.pi * xyz[:, 0]) * np.sin(np.pi * xyz[:, 1]) * np.sin(np.pi * xyz[:, 2])
     f = 3 * np.pi ** 2 * u_ex
@@ -78,11 +78,11 @@ def c1():
 
 def c2asm():
     """Assembly and Dirichlet elimination (with the solver numbering) of the
-    P2 kappa-Poisson problem on the 64^3 x 6 Kuhn mesh, inputs on the host as
+    P2 kappa-Poisson problem on the 64^3 x 6 mesh (test/cube.dat + 18 bisections), inputs on the host as
     PHG hands them over."""
     import phg_amd as pa
     n, p = 64, 2
-    g = pa.Grid.kuhn(n)
+    g = pa.Grid.phg_cube(n)          # test/cube.dat + 18 bisections
     nd = g.ndof(p)
     bd = g.dof_bdry(p)
     kappa = np.exp(np.random.default_rng(2024).normal(0.0, 0.5, size=g.nelem))
@@ -248,7 +248,7 @@ def solvers(n=32, p=2, rtol=1e-8):
     x0 = 0, rtol = btol): PCG, BiCGstab and the default GMRES (flgmres_r,
     restart 10, augk 2)."""
     import phg_amd as pa
-    g = pa.Grid.kuhn(n)
+    g = pa.Grid.phg_cube(n) if n & (n - 1) == 0 else pa.Grid.kuhn(n)
     nd = g.ndof(p)
     rng = np.random.default_rng(2024)
     kappa = np.exp(rng.normal(0.0, 0.5, size=g.nelem))

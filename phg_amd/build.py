@@ -72,8 +72,7 @@ def build(verbose: bool = False) -> None:
 
     tmp = os.path.join(LIB, "libphg_grid.so.tmp")
     _run(["gcc", "-O2", "-fPIC", "-shared", "-std=c11", "-Wall",
-          os.path.join(HOST, "phg_grid.c"), os.path.join(HOST, "refine.c"),
-          os.path.join(HOST, "bisect.c"),
+          os.path.join(HOST, "phg_grid.c"), os.path.join(HOST, "bisect.c"),
           "-lm", "-o", tmp])
     os.replace(tmp, os.path.join(LIB, "libphg_grid.so"))
 

@@ -71,16 +71,6 @@ phg_grid *phg_grid_new_kuhn(int n, const int bcode[6]);
 int phg_grid_write_albert(const phg_grid *g, const char *path);
 phg_grid *phg_grid_read_albert(const char *path);
 
-/* Local refinement toward a point (refine.c): `levels` sweeps; sweep l
- * bisects (with conforming closure) every element whose centroid lies
- * within radius * shrink^l of pt or which contains pt.  Newest-vertex
- * bisection in Maubach's ordering of the Kuhn tetrahedra of g (a mesh from
- * phg_grid_new_kuhn).  bcode as for phg_grid_new_kuhn.  The refined mesh is
- * numbered before returning. */
-phg_grid *phg_grid_refine_toward(const phg_grid *g, const double pt[3],
-                                 int levels, double radius, double shrink,
-                                 const int bcode[6]);
-
 /* PHG's bisection, refine.c:2346 phgRefineMarkedElements in a serial run:
  * every element e is bisected at least marks[e] times (by element index),
  * with PHG's conforming closure (refine.c:1661 refine_path), vertex creation

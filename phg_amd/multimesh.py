@@ -1,8 +1,9 @@
 """Batches of independent finite-element problems on unevenly refined meshes.
 
-Each mesh starts from the n^3 x 6 Kuhn mesh of the unit cube and is refined
-toward a seeded random corner of the cube with `levels` bisection sweeps
-(host, phg_grid_refine_toward); the problem on it is -div(kappa grad u) = 1,
+Each mesh starts from the n^3 x 6 mesh of the unit cube as PHG builds it
+(test/cube.dat bisected 3 log2(n) times) and is refined toward a seeded
+random corner of the cube with `levels` sweeps of PHG's bisection (host,
+phg_grid_bisect); the problem on it is -div(kappa grad u) = 1,
 u = 0 on the boundary, with a lognormal P0 coefficient.  The systems are
 assembled on the GPU and solved together by phg_hip_pcg_multi (all systems
 in flight on a stream pool; a chunk of iterations of every system captured
@@ -28,7 +29,7 @@ def batch_meshes(nmesh=8, n=16, seed=5, lmin=2, lmax=8, radius=0.4):
     from `seed`; the level counts are spread over [lmin, lmax]."""
     import phg_amd as pa
     rng = np.random.default_rng(seed)
-    base = pa.Grid.kuhn(n)
+    base = pa.Grid.phg_cube(n) if n & (n - 1) == 0 else pa.Grid.kuhn(n)
     levels = np.linspace(lmin, lmax, nmesh).round().astype(int)
     rng.shuffle(levels)
     out = []
@@ -145,7 +146,8 @@ def bench(nmesh=8, n=16, p=3, rtol=1e-8, reps=2):
                     "block_speedup_vs_csr": round(t_csr / t_el, 3)}
     tot = sum(ndofs)
     return {"config": "c5",
-            "workload": (f"{nmesh} P{p} kappa-Poisson problems on {n}^3x6 Kuhn meshes refined "
+            "workload": (f"{nmesh} P{p} kappa-Poisson problems on {n}^3x6 meshes "
+                         f"(test/cube.dat + PHG bisections) refined with PHG's bisection "
                          f"toward seeded corners (levels {[m[2] for m in meshes]}), "
                          f"mixed-precision PCG to rtol {rtol}"),
             "dofs": ndofs, "total_dofs": tot, "nnz": [A.info()[1] for A in As],

@@ -23,14 +23,16 @@ def lognormal_kappa(nelem, sigma, seed):
     return np.exp(rng.normal(0.0, sigma, size=nelem))
 
 
-def poisson_case(n, p, kind="sin", sigma=0.5, seed=7):
-    """Returns dict(grid, map, ndof, bdry, coords, f, g, kappa) for a Kuhn n^3 mesh.
+def poisson_case(n, p, kind="sin", sigma=0.5, seed=7, mesh="kuhn"):
+    """Returns dict(grid, map, ndof, bdry, coords, f, g, kappa) for an n^3 x 6
+    mesh: the structured generator's (mesh="kuhn") or PHG's (mesh="phg":
+    test/cube.dat bisected 3 log2(n) times, the workloads' mesh).
 
     kind: 'sin'   -Lap u = f with u = sin sin sin (C1 family),
           'kappa' -div(kappa grad u) = 1, u = 0 (C2 family),
           'linear' u = x+y+z, f = 0 (examples/functions.h TEST_CASE 1).
     """
-    g = pa.Grid.kuhn(n, POISSON_ALL)
+    g = pa.Grid.phg_cube(n, POISSON_ALL) if mesh == "phg" else pa.Grid.kuhn(n, POISSON_ALL)
     xyz = g.dof_coords(p)
     nd = g.ndof(p)
     case = dict(grid=g, p=p, map=g.dof_map(p), ndof=nd, bdry=g.dof_bdry(p),

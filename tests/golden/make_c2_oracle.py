@@ -1,5 +1,6 @@
 """Writes c2_64_oracle.npz: the oracle's FP64 solution of the P2
-kappa-Poisson workload on the 64^3 x 6 Kuhn mesh, at a fixed sample of DOFs.
+kappa-Poisson workload on the 64^3 x 6 mesh (test/cube.dat bisected 18 times
+by PHG's refinement), at a fixed sample of DOFs.
 
 Problem (the benchmark's headline system): -div(kappa grad u) = 1, u = 0 on
 the boundary of the unit cube, 1,572,864 tetrahedra, 2,146,689 DOFs,
@@ -30,7 +31,7 @@ sys.path.insert(0, ROOT)
 def main():
     import oracle as orc
     from tests._cases import poisson_case
-    c = poisson_case(64, 2, "kappa", sigma=0.5, seed=2024)
+    c = poisson_case(64, 2, "kappa", sigma=0.5, seed=2024, mesh="phg")
     t0 = time.perf_counter()
     A = orc.assemble(orc.problem(2, use_kappa=True), c["verts"], c["elem_verts"], c["map"],
                      c["ndof"], c["bdry"], c["g"], c["f"], c["kappa"])

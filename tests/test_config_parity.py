@@ -56,7 +56,7 @@ def _both(pa, c, use_kappa):
 
 def test_c1_32_csr_and_pcg(hip):
     pa = hip
-    c = poisson_case(32, 1, "sin")
+    c = poisson_case(32, 1, "sin", mesh="phg")
     assert c["ndof"] == 35937 and c["grid"].nelem == 196608
     sp, pb, A, Ao = _both(pa, c, False)
     _assert_csr_equal(A, Ao)
@@ -75,7 +75,7 @@ def test_c1_32_csr_and_pcg(hip):
 
 def test_c2_64_csr_bitexact_and_solutions(hip):
     pa = hip
-    c = poisson_case(64, 2, "kappa", seed=2024)
+    c = poisson_case(64, 2, "kappa", seed=2024, mesh="phg")
     assert c["ndof"] == 2146689 and c["grid"].nelem == 1572864
     sp, pb, A, Ao = _both(pa, c, True)
     _assert_csr_equal(A, Ao)
