@@ -1836,6 +1836,24 @@ void eop_reference_tensors(int p, double *T)
     }
 }
 
+int eop_rule_gradients(int p, double *w, double *D)
+{
+    const int nb = p == 1 ? 4 : (p == 2 ? 10 : 20);
+    Rule q;
+    make_rule(2 * (p - 1), q);
+    if (q.np > NPMAX_EOP)
+        return -1;
+    double g[4 * NB];
+    for (int k = 0; k < q.np; k++) {
+        w[k] = q.w[k];
+        bas_grad_lambda(p, q.pts[k], g);
+        for (int i = 0; i < nb; i++)
+            for (int a = 1; a < 4; a++)
+                D[((size_t)k * nb + i) * 3 + a - 1] = g[4 * i + a] - g[4 * i];
+    }
+    return q.np;
+}
+
 int eop_coefficients(const phg_hip_space *sp, const double *d_kappa,
                      const int32_t *d_eperm, double mass_coef, float *d_coef,
                      hipStream_t s)

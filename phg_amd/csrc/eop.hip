@@ -22,6 +22,18 @@
 // fragments (T parts) of all tiles stay in LDS for the kernel, the B
 // fragments (x parts) are written to LDS per 16 elements.
 //
+// P2 takes the products through the points of the rule the tensors are
+// summed on (order 2, four points q, weights w_q):
+//     U_e[q][b] = sum_j D_jb(q) x_e[j],   V_e[q][a] = w_q sum_b c_ab(e) U_e[q][b],
+//     y_e[i]    = sum_{q,a} D_ia(q) V_e[q][a],
+// D_ia(q) the gradient of phi_i in the independent barycentric coordinates
+// at point q and c_ab the symmetric metric of the c_m -- the same sum in
+// another order.  Both products are v_mfma_f32_16x16x4f32 of one 16 x 12
+// tile (rows (q, a) against the local indices j, then rows i against (q, a)),
+// six MFMA per 16 elements against fifteen for the stacked tensors, and the
+// 3 x 3 metric between them is applied per lane.  With a mass term the third
+// product c_6 T^6 x_e goes into the same accumulator.
+//
 // Element blocks.  The elements, in Morton order of their centroids, are cut
 // into blocks of at most EBLK elements touching at most KMAX rows.  A
 // work-group takes a block at a time:
@@ -167,6 +179,21 @@ template <int P> constexpr int kEt = P == 1 ? 2 : 1;
 // 4 (l / 16) + r of column l % 16.
 template <int P> constexpr bool kSplit = P == 3;
 
+// P2: products through the rule's points (see above).  Lane l = 16 g + n of
+// the first product holds U rows 4 g + r = (q = g, a = r), r < 3, of element
+// n -- the point and all three directions the metric needs -- and hands
+// V[q = g][a = s] to K step s of the second product as its B fragment.  The
+// output rows 4 g + r are the local indices kOutRow2(4 g + r): three per lane
+// group for g = 0, 1, two for g = 2, 3.
+template <int P> constexpr bool kPts = P == 2;
+constexpr int kNp2 = 4;               // points of the order-2 rule
+
+__host__ __device__ constexpr int kOutRow2(int row)
+{
+    return row / 4 < 2 ? (row % 4 < 3 ? 3 * (row / 4) + row % 4 : -1)
+                       : (row % 4 < 2 ? 6 + 2 * (row / 4 - 2) + row % 4 : -1);
+}
+
 struct BlockView {
     int64_t nblk;
     const int32_t *eb_elem;   // [nblk + 1] element ranges
@@ -182,7 +209,7 @@ struct BlockView {
     const int32_t *drows;     // [ndir] Dirichlet rows, spread evenly over the blocks
     int64_t ndir;
     const float *diag;        // FP32 diagonal (Dirichlet rows: y = diag x)
-    float msum[10];           // mass term: row sum of T^6 for the local indices of tile t
+    float msum[16];           // mass term: row sums of T^6 (P1 / P3 per tile t, P2 per row)
 };
 
 // work-group total of one double per work-item, to out[blockIdx.x]
@@ -228,7 +255,8 @@ __device__ inline T ld32(const T *base, uint32_t idx)
 // positions of j are filled through LDS) or j = 4 u + g (FP32 path: K step
 // u), u < NJ, and stores the results of the local
 // indices i = 2 t + g / 2 with t = g mod 2 (the D rows it holds after the
-// contraction), t = 2 o + g % 2 (o < NO).  Its chunk of the element's index
+// contraction), t = 2 o + g % 2 (o < NO) -- on the P2 point-product path
+// the local indices kOutRow2(4 g + o) instead.  Its chunk of the element's index
 // record holds, as 16-bit values, the block-local row of each j and the
 // block-local slot of each i (0xffff: Dirichlet / none).
 //
@@ -236,31 +264,46 @@ __device__ inline T ld32(const T *base, uint32_t idx)
 // into the block) and hold raw values; elements past the block are masked
 // where the results are stored, so the loads of a round stay in flight
 // while the previous round computes.
-template <int ET, int W> struct Round {
+template <int ET, int W, int NCF> struct Round {
     uint32_t ch[ET][W];
-    float cf[ET][4];
+    float cf[ET][NCF];
 };
 
-template <int NB, int W, int ET, bool MASS>
+// coefficients a lane holds: four (the tiles of its row group), or all of
+// its element's on the point-product path
+template <int P, bool MASS> constexpr int kNcf = kPts<P> ? kNc<MASS> : 4;
+
+template <int P, int W, int ET, bool MASS>
 __device__ inline void load_round(const BlockView &m, int32_t e0, int32_t ne, int w, int n,
-                                  int g, int rd, Round<ET, W> &r)
+                                  int g, int rd, Round<ET, W, kNcf<P, MASS>> &r)
 {
     // 32-bit offsets: nelem * nb < 2^30 is checked at set-up
 #pragma unroll
     for (int et = 0; et < ET; et++) {
         const int el = ((rd * EW + w) * ET + et) * 16 + n;
         const uint32_t e = (uint32_t)(e0 + min(el, ne - 1));
-        const uint32_t base = (e * 4 + (uint32_t)g) * W;
+        // P2 lane groups 2, 3 hold two dwords (the third load repeats the
+        // second; compute() spreads them into the three-dword form)
+        constexpr bool P2 = kPts<P>;
+        const uint32_t base = e * (uint32_t)phg::eop_record_dwords(P) +
+                              (uint32_t)(P2 ? (g < 2 ? 3 * g : 2 * g + 2) : g * W);
 #pragma unroll
         for (int u = 0; u < W; u++)
-            r.ch[et][u] = ld32(m.idx, base + u);
+            r.ch[et][u] = ld32(m.idx, base + (uint32_t)(P2 && g >= 2 ? min(u, 1) : u));
         // coefficients m = 4 (g % 2) + q; for odd g the slots past the
         // element's last coefficient (7, and 6 without the mass term) read
         // the next element's (finite: the array has two entries of padding)
-        // and meet all-zero rows of the T tiles
+        // and meet all-zero rows of the T tiles.  Point-product path: all.
+        if constexpr (kPts<P>) {
 #pragma unroll
-        for (int q = 0; q < 4; q++)
-            r.cf[et][q] = ld32(m.coef, e * kNc<MASS> + 4 * (g & 1) + q);
+            for (int q = 0; q < kNc<MASS>; q++)
+                r.cf[et][q] = ld32(m.coef, e * kNc<MASS> + q);
+        }
+        else {
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                r.cf[et][q] = ld32(m.coef, e * kNc<MASS> + 4 * (g & 1) + q);
+        }
     }
 }
 
@@ -315,14 +358,19 @@ k_eop_block(BlockView m, const float *__restrict__ x, float *__restrict__ y,
     constexpr int RPT = (S::KMAX + EB - 1) / EB;   // touched rows per work-item
     if (done != nullptr && (*done != 0 || *mode != want))
         return;
-    constexpr bool SPLIT = kSplit<P>;
+    constexpr bool SPLIT = kSplit<P>, PTS = kPts<P>;
+    constexpr int NCF = kNcf<P, MASS>;
+    // point-product A values: first product (NJ steps), second (3 steps),
+    // mass tensor (NJ steps)
+    constexpr int NAP = 2 * NJ + 3;
     __shared__ uint4 at[SPLIT ? NT * KS * 64 : 1];
     __shared__ __align__(16) __bf16 bt[SPLIT ? EW : 1][16][KP];
     __shared__ float xs[S::KMAX + 1];
     __shared__ float slot[S::SMAX + 1];
     const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
     const int n = l & 15, g = l >> 4;
-    float af[SPLIT ? 1 : NT][SPLIT ? 1 : NJ];
+    float af[SPLIT || PTS ? 1 : NT][SPLIT || PTS ? 1 : NJ];
+    float ap[PTS ? NAP : 1];
     if constexpr (SPLIT) {
         for (int t = tid; t < NT * KS * 64; t += EB)
             at[t] = ((const uint4 *)m.tiles)[t];
@@ -330,6 +378,11 @@ k_eop_block(BlockView m, const float *__restrict__ x, float *__restrict__ y,
         constexpr int PADW = KP - 6 * NB;
         for (int t = l; t < 16 * PADW; t += 64)
             bt[w][t / PADW][6 * NB + t % PADW] = (__bf16)0.0f;
+    }
+    else if constexpr (PTS) {
+#pragma unroll
+        for (int k = 0; k < (MASS ? NAP : NJ + 3); k++)
+            ap[k] = ((const float *)m.tiles)[k * 64 + l];
     }
     else {
 #pragma unroll
@@ -378,22 +431,31 @@ k_eop_block(BlockView m, const float *__restrict__ x, float *__restrict__ y,
 #pragma unroll
         for (int i = 0; i < RPT; i++)
             xr[i] = x[rw[i]];
-        Round<ET, W> ra, rb;
-        load_round<NB, W, ET, MASS>(m, e0, ne, w, n, g, 0, ra);
+        Round<ET, W, NCF> ra, rb;
+        load_round<P, W, ET, MASS>(m, e0, ne, w, n, g, 0, ra);
         // one round: x from the block's rows, B fragments, products, slots
-        auto compute = [&](int rd, Round<ET, W> &cur) {
+        auto compute = [&](int rd, Round<ET, W, NCF> &cur) {
             float yv[ET][NO];
             // x of the lane's local indices (split path j = NJ g + u, FP32
             // path j = 4 u + g; Dirichlet and none: row KMAX of xs, which
             // holds 0), relative to x_0
             float xv[ET][NJ], xz[ET];
+            if constexpr (PTS) {
+                // lane groups 2, 3: [j0 j1][i0 i1] -> [j0 j1][none i0][i1 none]
+#pragma unroll
+                for (int et = 0; et < ET; et++) {
+                    const uint32_t c1 = cur.ch[et][1];
+                    cur.ch[et][1] = g >= 2 ? 0xffffu | c1 << 16 : c1;
+                    cur.ch[et][2] = g >= 2 ? c1 >> 16 | 0xffff0000u : cur.ch[et][2];
+                }
+            }
             load_relative<S::KMAX, NJ, ET, W>(xs, cur.ch, n, xv, xz);
             // mass term: the lanes of slots 4..7 (odd g, c_6 their third
             // coefficient) add c_6 x_0 (T^6 1)_i to their part of row i
             float mc[ET];
 #pragma unroll
             for (int et = 0; et < ET; et++)
-                mc[et] = (g & 1) ? cur.cf[et][2] * xz[et] : 0.0f;
+                mc[et] = PTS ? 0.0f : ((g & 1) ? cur.cf[et][2] * xz[et] : 0.0f);
             // contract tile t's products with c_m, then add the two row
             // groups of local index 2 t + g / 2 (lanes g and g ^ 1)
             auto contract = [&](int et, int t, const f32x4 &a) {
@@ -405,7 +467,44 @@ k_eop_block(BlockView m, const float *__restrict__ x, float *__restrict__ y,
                 if ((t & 1) == (g & 1))
                     yv[et][t >> 1] = v;
             };
-            if constexpr (SPLIT) {
+            if constexpr (PTS) {
+#pragma unroll
+                for (int et = 0; et < ET; et++) {
+                    const float *c = cur.cf[et];
+                    // U rows (q = g, a = r) of element n
+                    f32x4 U = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                    for (int u = 0; u < NJ; u++)
+                        U = __builtin_amdgcn_mfma_f32_16x16x4f32(ap[u], xv[et][u], U, 0, 0, 0);
+                    // the metric at the lane's point; w_q is in the second
+                    // product's A values
+                    const float v[3] = {c[0] * U[0] + c[1] * U[1] + c[2] * U[2],
+                                        c[1] * U[0] + c[3] * U[1] + c[4] * U[2],
+                                        c[2] * U[0] + c[4] * U[1] + c[5] * U[2]};
+                    f32x4 Y = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                    for (int s2 = 0; s2 < 3; s2++)
+                        Y = __builtin_amdgcn_mfma_f32_16x16x4f32(ap[NJ + s2], v[s2], Y, 0, 0, 0);
+                    if constexpr (MASS) {
+                        // c_6 T^6 (x_e - x_0) in the same accumulator, then
+                        // c_6 x_0 (T^6 1)_i
+#pragma unroll
+                        for (int u = 0; u < NJ; u++)
+                            Y = __builtin_amdgcn_mfma_f32_16x16x4f32(ap[NJ + 3 + u],
+                                                                     c[6] * xv[et][u], Y, 0, 0, 0);
+                        const float c0 = c[6] * xz[et];
+#pragma unroll
+                        for (int o = 0; o < NO; o++)
+                            yv[et][o] = Y[o] + c0 * m.msum[4 * g + o];
+                    }
+                    else {
+#pragma unroll
+                        for (int o = 0; o < NO; o++)
+                            yv[et][o] = Y[o];
+                    }
+                }
+            }
+            else if constexpr (SPLIT) {
                 // split into bf16 parts through the wavefront's LDS rows, one
                 // tile at a time, into B fragments
                 bf16x8 bf[ET][KS];
@@ -502,6 +601,19 @@ k_eop_block(BlockView m, const float *__restrict__ x, float *__restrict__ y,
             int32_t rwn[RPT];
             uint32_t qrn[RPT];
             row_loads(bq, k0n, nkn, rwn, qrn);
+            // loads this block needs after its rounds, issued now so that no
+            // dependent load waits at the block's end: the partial positions
+            // of its rows and its first EB Dirichlet rows (their x and
+            // diagonal follow after the first round)
+            const int kc = max(nk - 1, 0);
+            int32_t opp[RPT];
+#pragma unroll
+            for (int i = 0; i < RPT; i++)
+                opp[i] = m.ppos[k0 + min(tid + EB * i, kc)];
+            const int64_t d0 = m.ndir * b / m.nblk, d1 = m.ndir * (b + 1) / m.nblk;
+            const bool dhas = d0 + tid < d1;
+            const int32_t dr = dhas ? m.drows[d0 + tid] : 0;
+            float dxv = 0.0f, ddv = 0.0f;
             // rounds in pairs, the loads of round rd + 1 issued before round
             // rd computes; the round data never moves between registers (a
             // move would wait for the loads in flight).  Loads are
@@ -511,23 +623,22 @@ k_eop_block(BlockView m, const float *__restrict__ x, float *__restrict__ y,
             const int ntile = (ne + 15) / 16;
             const int nround = (ntile + EW * ET - 1) / (EW * ET);
             for (int rd = 0; rd < nround; rd += 2) {
-                load_round<NB, W, ET, MASS>(m, e0, ne, w, n, g, rd + 1, rb);
+                load_round<P, W, ET, MASS>(m, e0, ne, w, n, g, rd + 1, rb);
                 compute(rd, ra);
+                if (rd == 0 && dhas) {
+                    dxv = x[dr];
+                    ddv = m.diag[dr];
+                }
                 const bool last = rd + 2 >= nround;
-                load_round<NB, W, ET, MASS>(m, last ? e0n : e0, last ? nen : ne, w, n, g,
+                load_round<P, W, ET, MASS>(m, last ? e0n : e0, last ? nen : ne, w, n, g,
                                   last ? 0 : rd + 2, ra);
                 compute(rd + 1, rb);
             }
-            // the next block's x, this block's output rows and partial slots
+            // the next block's x
             float xrn[RPT];
-            int32_t orow[RPT], opp[RPT];
-            const int kc = max(nk - 1, 0);
 #pragma unroll
-            for (int i = 0; i < RPT; i++) {
+            for (int i = 0; i < RPT; i++)
                 xrn[i] = x[rwn[i]];
-                orow[i] = m.rows[k0 + min(tid + EB * i, kc)];
-                opp[i] = m.ppos[k0 + min(tid + EB * i, kc)];
-            }
             __syncthreads();
             // row sums, slots in element order
 #pragma unroll
@@ -549,14 +660,18 @@ k_eop_block(BlockView m, const float *__restrict__ x, float *__restrict__ y,
                 if (k < nk)
                     dot += (double)xs[k] * (double)sum;
                 if (k < nint)
-                    y[orow[i]] = sum;
+                    y[rw[i]] = sum;
                 else if (k < nk)
                     m.part[opp[i]] = sum;
             }
             // this block's share of the Dirichlet rows: y = diag x
             {
-                const int64_t d0 = m.ndir * b / m.nblk, d1 = m.ndir * (b + 1) / m.nblk;
-                for (int64_t t = d0 + tid; t < d1; t += EB) {
+                if (dhas) {
+                    const float v = ddv * dxv;
+                    y[dr] = v;
+                    dot += (double)dxv * (double)v;
+                }
+                for (int64_t t = d0 + EB + tid; t < d1; t += EB) {
                     const int32_t r = m.drows[t];
                     const float xv = x[r], v = m.diag[r] * xv;
                     y[r] = v;
@@ -573,6 +688,7 @@ k_eop_block(BlockView m, const float *__restrict__ x, float *__restrict__ y,
             ne = nen;
 #pragma unroll
             for (int i = 0; i < RPT; i++) {
+                rw[i] = rwn[i];
                 qr[i] = qrn[i];
                 xr[i] = xrn[i];
             }
@@ -671,6 +787,48 @@ void build_tiles_f32(int p, int nb, bool mass, std::vector<float> &out)
             }
 }
 
+// FP32 A values [2 nj + 3][64] of the P2 point-product path (lane l):
+//   K step u < nj of the first product:  row R = l % 16 -> (q, a) = (R / 4,
+//     R % 4), column j = 4 u + l / 16: D_ja(q) (0 for a = 3, j >= nb);
+//   K step s < 3 of the second: row R -> local index i = kOutRow2(R),
+//     column (q = l / 16, a = s): w_q D_ia(q) (0 where i < 0);
+//   K step u < nj of the mass product: T^6[kOutRow2(R)][4 u + l / 16].
+// Returns false unless the rule has kNp2 points.
+bool build_tiles_pts(int p, int nb, std::vector<float> &out, float *msum)
+{
+    double w[phg::NPMAX_EOP], D[(size_t)phg::NPMAX_EOP * 20 * 3];
+    if (phg::eop_rule_gradients(p, w, D) != kNp2)
+        return false;
+    std::vector<double> T((size_t)NREF * nb * nb);
+    phg::eop_reference_tensors(p, T.data());
+    const double *M = T.data() + (size_t)6 * nb * nb;
+    const int nj = (nb + 3) / 4;
+    out.assign((size_t)(2 * nj + 3) * 64, 0.0f);
+    for (int l = 0; l < 64; l++) {
+        const int R = l % 16, i = kOutRow2(R);
+        for (int u = 0; u < nj; u++) {
+            const int q = R / 4, a = R % 4, j = 4 * u + l / 16;
+            if (a < 3 && j < nb)
+                out[(size_t)u * 64 + l] = (float)D[((size_t)q * nb + j) * 3 + a];
+            if (i >= 0 && j < nb)
+                out[(size_t)(nj + 3 + u) * 64 + l] = (float)M[(size_t)i * nb + j];
+        }
+        for (int s = 0; s < 3; s++) {
+            const int q = l / 16;
+            if (i >= 0)
+                out[(size_t)(nj + s) * 64 + l] = (float)(w[q] * D[((size_t)q * nb + i) * 3 + s]);
+        }
+    }
+    for (int R = 0; R < 16; R++) {
+        const int i = kOutRow2(R);
+        double r = 0.0;
+        for (int j = 0; i >= 0 && j < nb; j++)
+            r += M[(size_t)i * nb + j];
+        msum[R] = (float)r;
+    }
+    return true;
+}
+
 __host__ __device__ inline uint64_t spread3(uint64_t v)
 {
     v &= 0x1fffffull;
@@ -706,7 +864,7 @@ void launch_block(const EopData *eo, const float *x, const float *diag, float *y
                 eo->ppos, eo->idx, eo->coef, (const void *)eo->tiles, eo->part,
                 eo->fx_rows + eo->nfx, eo->ndir, diag, {}};
     if (eo->mass) {
-        for (int t = 0; t < eo->nb / 2; t++)
+        for (int t = 0; t < 16; t++)
             m.msum[t] = eo->msum[t];
         hipLaunchKernelGGL((k_eop_block<P, true>), dim3(eo->grid_b), dim3(EB), 0, s, m, x, y,
                            part_out, done, mode, want);
@@ -1077,10 +1235,12 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
         }
     }
     mark("block tables");
-    // lane-group index chunks (see load_round): x rows of j = NJ g + u,
-    // result slots of i = 2 t + g / 2, t = 2 o + g % 2
+    // lane-group index chunks (see load_round): x rows of j = NJ g + u
+    // (P3) or 4 u + g, result slots of i = 2 t + g / 2, t = 2 o + g % 2 (P1,
+    // P3) or kOutRow2(4 g + o) (P2)
     const int nj = (nb + 3) / 4, no = (nb / 2 + 1) / 2, wc = (nj + no + 1) / 2;
-    HostVec<uint32_t> idx((size_t)ne * 4 * wc);
+    const int rec = eop_record_dwords(p);
+    HostVec<uint32_t> idx((size_t)ne * rec);
     host_parallel(ne, [&](int64_t lo, int64_t hi, int) {
         for (int64_t e = lo; e < hi; e++)
             for (int g = 0; g < 4; g++) {
@@ -1093,12 +1253,26 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
                         h[u] = kd[(size_t)(e * nb + j)] & 0xffffu;
                 }
                 for (int o = 0; o < no; o++) {
+                    if (p == 2) {
+                        const int i = kOutRow2(4 * g + o);
+                        if (i >= 0)
+                            h[nj + o] = kd[(size_t)(e * nb + i)] >> 16;
+                        continue;
+                    }
                     const int t = 2 * o + (g & 1);
                     if (t < nb / 2)
                         h[nj + o] = kd[(size_t)(e * nb + 2 * t + (g >> 1))] >> 16;
                 }
+                if (p == 2 && g >= 2) {
+                    // j 4 u + g is past nb for u = 2, i past the lane's two
+                    uint32_t *d = &idx[(size_t)(e * rec + 2 * g + 2)];
+                    d[0] = h[0] | h[1] << 16;
+                    d[1] = h[nj] | h[nj + 1] << 16;
+                    continue;
+                }
+                const int off = p == 2 ? 3 * g : g * wc;
                 for (int u = 0; u < wc; u++)
-                    idx[(size_t)((e * 4 + g) * wc + u)] = h[2 * u] | h[2 * u + 1] << 16;
+                    idx[(size_t)(e * rec + off + u)] = h[2 * u] | h[2 * u + 1] << 16;
             }
     });
     mark("index chunks");
@@ -1123,6 +1297,14 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
             tiles[ms].resize(tb.size() * 2);
             memcpy(tiles[ms].data(), tb.data(), tiles[ms].size());
         }
+        else if (p == 2) {
+            std::vector<float> tf;
+            if (!build_tiles_pts(p, nb, tf, L.msum))
+                return set_error(PHG_HIP_E_STATE,
+                                 "element operator set-up: the P2 rule has not %d points", kNp2);
+            tiles[ms].resize(tf.size() * 4);
+            memcpy(tiles[ms].data(), tf.data(), tiles[ms].size());
+        }
         else {
             std::vector<float> tf;
             build_tiles_f32(p, nb, ms != 0, tf);
@@ -1133,7 +1315,8 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
     // row sums of the mass tensor; the local indices 2 t, 2 t + 1 of a tile
     // are of one kind (vertex, edge or face nodes: the kinds start at even
     // local indices), so their sums agree and one value serves the tile
-    {
+    // (P2: per output row, set with the tiles)
+    if (p != 2) {
         std::vector<double> T((size_t)NREF * nb * nb);
         eop_reference_tensors(p, T.data());
         for (int t = 0; t < nb / 2; t++) {

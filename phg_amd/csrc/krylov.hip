@@ -4517,8 +4517,8 @@ extern "C" int phg_hip_element_bytes(const phg_hip_mat *A, int64_t *elem_bytes,
     // touched-row entry its row (and partial slot when shared), slot start
     // and x; interior rows of y and the partials written; block tables;
     // Dirichlet rows (row, x, diagonal read, y written)
-    const int64_t nj = (eo->nb + 3) / 4, no = (eo->nb / 2 + 1) / 2, wc = (nj + no + 1) / 2;
-    *elem_bytes = eo->nelem * (16 * wc + 4 * (eo->mass ? 7 : 6)) + eo->nent * (4 + 2 + 4) +
+    *elem_bytes = eo->nelem * (4 * phg::eop_record_dwords(eo->p) + 4 * (eo->mass ? 7 : 6)) +
+                  eo->nent * (4 + 2 + 4) +
                   eo->nblk * (2 + 3 * 4) + nsh * (4 + 4) + eo->nint * 4 + eo->ndir * 16;
     // fix-up: rows, partial ranges and partials of the shared rows, y
     *gather_bytes = eo->nfx * (4 + 4 + 4) + 4 + eo->npart * 4;

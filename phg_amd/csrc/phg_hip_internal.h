@@ -83,7 +83,9 @@ struct EopLayout {
     uint32_t *idx = nullptr;
     uint8_t *tiles = nullptr;        // A fragments of the stiffness tensors
     uint8_t *tiles_mass = nullptr;   // the same with the mass tensor in slot 6
-    float msum[10] = {};             // row sum of the mass tensor, local index 2 t (= 2 t + 1)
+    // mass tensor row sums: P1 / P3 per output tile (local index 2 t = 2 t + 1),
+    // P2 per output row of the point-product path
+    float msum[16] = {};
     int32_t *fx_rows = nullptr, *fx_off = nullptr;
     uint32_t *fxc = nullptr;
     int32_t *eperm = nullptr;        // [nelem] Morton position -> element
@@ -171,7 +173,7 @@ struct EopData {
     int mass = 0;                     // mass term: 7 coefficients per element
     float *coef = nullptr;            // [nelem][6 or 7] + 2 entries of padding
     uint8_t *tiles = nullptr;         // A fragments (bf16 parts for P3, FP32 for P1 / P2)
-    const float *msum = nullptr;      // [nb / 2] mass row sums per tile (layout's)
+    const float *msum = nullptr;      // [16] mass row sums (layout's)
     float *part = nullptr;            // [npart] partial sums of shared rows
     int64_t npart = 0;
     // fix-up: nfx shared rows (partial ranges fx_off) then ndir Dirichlet rows
@@ -347,6 +349,19 @@ constexpr int EOP_MAX_PART = 2048;   // partial sums one apply writes, at most
 constexpr uint32_t EOP_NOT_SHARED = 0xffffffffu;
 // T: [7][nb][nb], the six stiffness tensors and the mass tensor
 void eop_reference_tensors(int p, double *T);
+// the points of the rule those stiffness tensors are summed on: returns
+// their number np (at most NPMAX_EOP) with w[np] the weights and
+// D[np][nb][3] = d phi_i / d lambda_a - d phi_i / d lambda_0, a = 1..3
+constexpr int NPMAX_EOP = 16;
+// dwords of one element's index record (the four lane groups' chunks):
+// 4 (nj + no + 1) / 2 with nj = (nb + 3) / 4, no = (nb / 2 + 1) / 2, except
+// P2, whose lane groups 2 and 3 have one unused x entry and one unused
+// result entry each and take two dwords (3 + 3 + 2 + 2)
+constexpr int eop_record_dwords(int p)
+{
+    return p == 2 ? 10 : 4 * ((((p == 1 ? 4 : 20) + 3) / 4 + ((p == 1 ? 4 : 20) / 2 + 1) / 2 + 1) / 2);
+}
+int eop_rule_gradients(int p, double *w, double *D);
 // per element 6 coefficients (mass_coef 0) or 7 (the mass term last)
 int eop_coefficients(const phg_hip_space *sp, const double *d_kappa,
                      const int32_t *d_eperm, double mass_coef, float *d_coef,
