@@ -6,7 +6,7 @@
     rocprofv3 --pmc WRITE_SIZE -d OUT/pw -o run -- python3 -u REPO/bench.py \
         --steps 1 --warmup 0 --cpu-its 0 --no-profile
     python3 -m phg_amd.pmc_traffic OUT/pf/run_results.db OUT/pw/run_results.db \
-        --nnz 57343939 --copy-bytes 365530520 > profiles/hbm_traffic.json
+        --nnz 57377401 --copy-bytes 365731296 > profiles/hbm_traffic.json
 
 The counters are scaled by what they read for bench.py's copy kernel
 (`k_copy`, `--copy-bytes` read and as many written per launch; the largest
@@ -30,10 +30,12 @@ sys.path.insert(0, ROOT)
 
 # bench.py's roofline keys: kernel-name prefix -> (key, sources)
 KERNELS = {
-    "void (anonymous namespace)::k_eop_block<2>": (
-        "k_eop_block<2>", ["phg_amd/csrc/eop.hip"]),
+    "void (anonymous namespace)::k_eop_block<2, false>": (
+        "k_eop_block<2>", ["phg_amd/csrc/eop.hip", "phg_amd/csrc/phg_hip_internal.h"]),
     "void (anonymous namespace)::k_ru_spmv<(anonymous namespace)::XsViewT<1> >": (
         "k_ru_spmv<csr_xstage_pf>", ["phg_amd/csrc/krylov.hip", "phg_amd/csrc/dictcsr.hip"]),
+    "void (anonymous namespace)::k_ru_spmv<(anonymous namespace)::XsViewT<0> >": (
+        "k_ru_spmv<csr_xstage>", ["phg_amd/csrc/krylov.hip", "phg_amd/csrc/dictcsr.hip"]),
     "void (anonymous namespace)::k_pcg_spmv<double, (anonymous namespace)::XsViewT<1> >": (
         "k_pcg_spmv<double,csr_xstage_pf>",
         ["phg_amd/csrc/krylov.hip", "phg_amd/csrc/dictcsr.hip"]),
