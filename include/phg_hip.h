@@ -346,8 +346,10 @@ int phg_hip_element_bytes(const phg_hip_mat *A, int64_t *elem_bytes,
 int phg_hip_element_time(const phg_hip_mat *A, int32_t reps, double *elem_us,
                          double *gather_us);
 /* Block layout of the operator: out[0] blocks, [1] touched-row entries,
- * [2] interior rows, [3] shared rows, [4] partial sums, [5] Dirichlet rows. */
-int phg_hip_element_layout(const phg_hip_mat *A, int64_t out[6]);
+ * [2] interior rows, [3] shared rows, [4] partial sums, [5] Dirichlet rows,
+ * [6] 1 when the layout was built on the device (blocks of a fixed number of
+ * elements), 0 when by the host's split. */
+int phg_hip_element_layout(const phg_hip_mat *A, int64_t out[7]);
 
 /*------------------------------------------------ many right-hand sides */
 

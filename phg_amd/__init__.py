@@ -529,11 +529,13 @@ class Mat:
 
     def element_layout(self):
         """Block layout of the matrix-free operator: blocks, touched-row
-        entries, interior rows, shared rows, partial sums, Dirichlet rows."""
-        out = (_i64 * 6)()
+        entries, interior rows, shared rows, partial sums, Dirichlet rows,
+        and whether it was built on the device (1) or by the host (0)."""
+        out = (_i64 * 7)()
         _check(hip_lib.phg_hip_element_layout(self._h, out), "element_layout")
         return {k: int(v) for k, v in zip(
-            ("blocks", "entries", "interior", "shared", "partials", "dirichlet"), out)}
+            ("blocks", "entries", "interior", "shared", "partials", "dirichlet", "device"),
+            out)}
 
     def element_time(self, reps: int = 20):
         """(element kernel us, row gather us) per launch."""

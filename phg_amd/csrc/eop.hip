@@ -951,6 +951,591 @@ __global__ void k_elem_morton(int64_t ne, const double *__restrict__ xyz,
     idx[e] = (int32_t)e;
 }
 
+namespace {
+
+//==================================================== layout on the device ====
+// The element-operator layout built on the device for blocks of EBLK
+// consecutive elements (in Morton order), when every such block touches at
+// most KMAX rows -- the blocks the host's greedy split makes on such meshes,
+// without its cuts.  Block tables, slot order and partial order follow the
+// host builder's rules: touched rows of a block in first-touch order, then
+// interior rows before shared ones, each group by decreasing slot count
+// (ties by first touch); the slots of a row in element order; the partials
+// of a shared row in block order.
+constexpr int LB = 512;          // work-items of the layout kernels
+
+__device__ inline uint32_t row_hash(int32_t r, int hs)
+{
+    return ((uint32_t)r * 2654435761u) & (uint32_t)(hs - 1);
+}
+
+// exclusive prefix sum of one int per work-item over the work-group (in
+// work-item order); *total gets the sum
+__device__ inline int group_scan_excl(int v, int *total)
+{
+    __shared__ int ws[LB / 64];
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (l >= o)
+            x += y;
+    }
+    if (l == 63)
+        ws[w] = x;
+    __syncthreads();
+    int base = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < LB / 64; k++) {
+        base += k < w ? ws[k] : 0;
+        tot += ws[k];
+    }
+    __syncthreads();
+    *total = tot;
+    return base + x - v;
+}
+
+__global__ void k_lay_iperm(int64_t n, const int32_t *perm, int32_t *iperm)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        iperm[perm[i]] = (int32_t)i;
+}
+
+// element map in the solver numbering, elements in Morton order, Dirichlet
+// DOFs -1 (brow: Dirichlet flags in PHG's numbering)
+__global__ void k_lay_emap(int64_t ne, int nb, const int32_t *eperm, const int32_t *map,
+                           const int32_t *iperm, const uint8_t *brow, int32_t *emap)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ne * nb)
+        return;
+    const int64_t e2 = t / nb, a = t % nb;
+    const int32_t d = map[(int64_t)eperm[e2] * nb + a];
+    emap[t] = brow[d] ? -1 : (iperm ? iperm[d] : d);
+}
+
+// per block: touched rows in first-touch order (rows_ft[b][k]), slots per
+// touched row (ns_g[b][k]), the touched-row number of every entry (kft,
+// KD_NONE for Dirichlet DOFs), blocks per row (cnt, atomically), rows per
+// block (nk_g).  ovf is set when a block touches more than kmax rows.
+template <int HS>
+__global__ void __launch_bounds__(LB) k_lay_touch(int64_t ne, int nb, int eblk, int kmax,
+                                                  const int32_t *__restrict__ emap, int32_t *cnt,
+                                                  int32_t *rows_ft, int32_t *ns_g,
+                                                  int32_t *nk_g, uint32_t *kft, int32_t *ovf)
+{
+    __shared__ int32_t hkey[HS], hmin[HS], hrank[HS];
+    __shared__ int32_t nsc[HS / 2], rft[HS / 2];
+    const int tid = threadIdx.x;
+    const int64_t b = blockIdx.x, e0 = b * eblk;
+    const int nel = (int)min((int64_t)eblk, ne - e0), nt = nel * nb;
+    const int32_t *em = emap + e0 * nb;
+    for (int i = tid; i < HS; i += LB) {
+        hkey[i] = -1;
+        hmin[i] = 0x7fffffff;
+        hrank[i] = -1;
+    }
+    for (int i = tid; i < HS / 2; i += LB)
+        nsc[i] = 0;
+    __syncthreads();
+    // insert every row, keep its first position
+    bool bad = false;
+    for (int t = tid; t < nt && !bad; t += LB) {
+        const int32_t r = em[t];
+        if (r < 0)
+            continue;
+        uint32_t h = row_hash(r, HS);
+        int probes = 0;
+        while (true) {
+            const int32_t k = hkey[h];
+            if (k == r)
+                break;
+            if (k == -1) {
+                const int32_t old = atomicCAS(&hkey[h], -1, r);
+                if (old == -1 || old == r)
+                    break;
+            }
+            h = (h + 1) & (HS - 1);
+            if (++probes == HS) {
+                bad = true;
+                break;
+            }
+        }
+        if (!bad)
+            atomicMin(&hmin[h], t);
+    }
+    if (__syncthreads_or(bad)) {
+        if (tid == 0)
+            *ovf = 1;
+        return;
+    }
+    auto find = [&](int32_t r) {
+        uint32_t h = row_hash(r, HS);
+        for (int probes = 0; hkey[h] != r && probes < HS; probes++)
+            h = (h + 1) & (HS - 1);
+        return h;
+    };
+    // first touches in position order: each work-item a contiguous range
+    const int cw = (nt + LB - 1) / LB, t0 = min(nt, tid * cw), t1 = min(nt, t0 + cw);
+    int mine = 0;
+    for (int t = t0; t < t1; t++) {
+        const int32_t r = em[t];
+        if (r >= 0 && hmin[find(r)] == t)
+            mine++;
+    }
+    int nk = 0;
+    int k = group_scan_excl(mine, &nk);
+    if (nk > kmax) {
+        if (tid == 0)
+            *ovf = 1;
+        return;
+    }
+    for (int t = t0; t < t1; t++) {
+        const int32_t r = em[t];
+        if (r >= 0) {
+            const uint32_t h = find(r);
+            if (hmin[h] == t) {
+                hrank[h] = k;
+                rft[k] = r;
+                k++;
+            }
+        }
+    }
+    __syncthreads();
+    for (int t = tid; t < nt; t += LB) {
+        const int32_t r = em[t];
+        if (r < 0) {
+            kft[e0 * nb + t] = KD_NONE;
+            continue;
+        }
+        const int32_t kr = hrank[find(r)];
+        kft[e0 * nb + t] = (uint32_t)kr;
+        atomicAdd(&nsc[kr], 1);
+    }
+    __syncthreads();
+    for (int i = tid; i < nk; i += LB) {
+        rows_ft[b * kmax + i] = rft[i];
+        ns_g[b * kmax + i] = nsc[i];
+        atomicAdd(&cnt[rft[i]], 1);
+    }
+    if (tid == 0)
+        nk_g[b] = nk;
+}
+
+// per block: the order of its touched rows (interior first, then shared,
+// each by decreasing slot count, ties by first touch), rows, slot starts
+// rs, interior count, the entries' row and slot (kd: row | slot << 16, the
+// slots of a row in element order), and the sort keys of the partial order
+// (shared rows: their row; interior: INT_MAX) with the entry index.
+template <int KP>
+__global__ void __launch_bounds__(LB) k_lay_order(int64_t ne, int nb, int eblk, int kmax,
+                                                  const int32_t *__restrict__ cnt,
+                                                  const int32_t *__restrict__ rows_ft,
+                                                  const int32_t *__restrict__ ns_g,
+                                                  const int32_t *__restrict__ nk_g,
+                                                  const int32_t *__restrict__ eb_row,
+                                                  uint32_t *kd, int32_t *rows, uint16_t *rs,
+                                                  int32_t *eb_nint, int32_t *pkey,
+                                                  int32_t *pval, int32_t *nint_tot)
+{
+    __shared__ uint64_t key[KP];
+    __shared__ int32_t newpos[KP], sc[KP + 1], cmin[KP], ccnt[KP];
+    constexpr int MAXE = 32;          // entries per work-item (nt <= LB * MAXE)
+    const int tid = threadIdx.x;
+    const int64_t b = blockIdx.x, e0 = b * eblk;
+    const int nel = (int)min((int64_t)eblk, ne - e0), nt = nel * nb;
+    const int nk = nk_g[b];
+    const int64_t k0 = eb_row[b];
+    for (int k = tid; k < KP; k += LB) {
+        uint64_t v = ~0ull;
+        if (k < nk) {
+            const int32_t r = rows_ft[b * kmax + k];
+            const uint64_t shared = cnt[r] > 1 ? 1 : 0;
+            v = shared << 40 | (uint64_t)(0xfffff - ns_g[b * kmax + k]) << 20 | (uint64_t)k;
+        }
+        key[k] = v;
+        cmin[k] = 0x7fffffff;
+        ccnt[k] = 0;
+    }
+    __syncthreads();
+    // bitonic sort of the KP keys (unique below the padding)
+    for (int sz = 2; sz <= KP; sz <<= 1)
+        for (int st = sz >> 1; st > 0; st >>= 1) {
+            for (int i = tid; i < KP / 2; i += LB) {
+                const int lo = 2 * i - (i & (st - 1)), hi = lo + st;
+                const bool up = (lo & sz) == 0;
+                const uint64_t a = key[lo], c = key[hi];
+                if ((a > c) == up) {
+                    key[lo] = c;
+                    key[hi] = a;
+                }
+            }
+            __syncthreads();
+        }
+    // sorted position j of first-touch row k; slots per sorted row
+    const int per = KP / LB;          // consecutive sorted rows per work-item
+    int mine = 0, ni = 0;
+    for (int j = tid * per; j < tid * per + per; j++)
+        if (j < nk) {
+            const uint64_t v = key[j];
+            const int k = (int)(v & 0xfffff);
+            newpos[k] = j;
+            const int32_t r = rows_ft[b * kmax + k];
+            rows[k0 + j] = r;
+            const bool shared = (v >> 40) != 0;
+            pkey[k0 + j] = shared ? r : 0x7fffffff;
+            pval[k0 + j] = (int32_t)(k0 + j);
+            ni += shared ? 0 : 1;
+            mine += 0xfffff - (int)((v >> 20) & 0xfffff);
+        }
+    int tot = 0;
+    int s = group_scan_excl(mine, &tot);
+    for (int j = tid * per; j < tid * per + per; j++)
+        if (j < nk) {
+            sc[j] = s;
+            rs[k0 + b + j] = (uint16_t)s;
+            s += 0xfffff - (int)((key[j] >> 20) & 0xfffff);
+        }
+    int nitot = 0;
+    (void)group_scan_excl(ni, &nitot);
+    if (tid == 0) {
+        rs[k0 + b + nk] = (uint16_t)tot;
+        eb_nint[b] = nitot;
+        atomicAdd(nint_tot, nitot);
+    }
+    __syncthreads();
+    // slots of each row in element order: round after round, the first
+    // entry still waiting takes the row's next slot
+    int kl[MAXE];
+    uint32_t act = 0;
+#pragma unroll
+    for (int i = 0; i < MAXE; i++) {
+        const int t = tid + LB * i;
+        kl[i] = -1;
+        if (t < nt) {
+            const uint32_t f = kd[e0 * nb + t];
+            if (f == KD_NONE)
+                continue;
+            kl[i] = newpos[f];
+            act |= 1u << i;
+        }
+    }
+    while (__syncthreads_or(act != 0)) {
+#pragma unroll
+        for (int i = 0; i < MAXE; i++)
+            if (act >> i & 1u)
+                atomicMin(&cmin[kl[i]], tid + LB * i);
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < MAXE; i++) {
+            const int t = tid + LB * i;
+            if ((act >> i & 1u) && cmin[kl[i]] == t) {
+                kd[e0 * nb + t] = (uint32_t)kl[i] | (uint32_t)(sc[kl[i]] + ccnt[kl[i]]) << 16;
+                ccnt[kl[i]]++;
+                cmin[kl[i]] = 0x7fffffff;
+                act &= ~(1u << i);
+            }
+        }
+    }
+}
+
+// partial positions: pkey / pval sorted by row (stably, so in block order);
+// the first npart are the shared entries.  Starts of rows give fx_rows and
+// fx_off (through the scan of flag), ppos[entry] = its sorted position.
+__global__ void k_lay_flags(int64_t npart, const int32_t *pkey, int32_t *flag)
+{
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < npart)
+        flag[j] = (j == 0 || pkey[j] != pkey[j - 1]) ? 1 : 0;
+    else if (j == npart)
+        flag[j] = 0;
+}
+
+__global__ void k_lay_partials(int64_t npart, const int32_t *pkey, const int32_t *pval,
+                               const int32_t *flag, const int32_t *fpos, int32_t *ppos,
+                               int32_t *fx_rows, int32_t *fx_off, int32_t *maxcnt)
+{
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= npart)
+        return;
+    ppos[pval[j]] = (int32_t)j;
+    if (flag[j]) {
+        const int32_t u = fpos[j];
+        fx_rows[u] = pkey[j];
+        fx_off[u] = (int32_t)j;
+        int64_t e = j + 1;
+        while (e < npart && !flag[e])
+            e++;
+        atomicMax(maxcnt, (int32_t)(e - j));
+    }
+}
+
+// Dirichlet rows in the solver numbering, in row order
+__global__ void k_lay_dflags(int64_t n, const int32_t *perm, const uint8_t *brow, uint8_t *f)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        f[i] = brow[perm ? perm[i] : i] ? 1 : 0;
+}
+
+// lane-group index chunks (see load_round), from kd
+__global__ void k_lay_chunks(int64_t ne, int p, int nb, const uint32_t *kd, uint32_t *idx)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ne * 4)
+        return;
+    const int64_t e = t / 4;
+    const int g = (int)(t % 4);
+    const int nj = (nb + 3) / 4, no = (nb / 2 + 1) / 2, wc = (nj + no + 1) / 2;
+    const int rec = phg::eop_record_dwords(p);
+    uint32_t h[10];
+    for (int u = 0; u < 10; u++)
+        h[u] = 0xffffu;
+    for (int u = 0; u < nj; u++) {
+        const int j = p == 3 ? nj * g + u : 4 * u + g;
+        if (j < nb)
+            h[u] = kd[e * nb + j] & 0xffffu;
+    }
+    for (int o = 0; o < no; o++) {
+        int i = -1;
+        if (p == 2)
+            i = kOutRow2(4 * g + o);
+        else {
+            const int tt = 2 * o + (g & 1);
+            if (tt < nb / 2)
+                i = 2 * tt + (g >> 1);
+        }
+        if (i >= 0)
+            h[nj + o] = kd[e * nb + i] >> 16;
+    }
+    if (p == 2 && g >= 2) {
+        uint32_t *d = idx + e * rec + 2 * g + 2;
+        d[0] = h[0] | h[1] << 16;
+        d[1] = h[nj] | h[nj + 1] << 16;
+        return;
+    }
+    const int off = p == 2 ? 3 * g : g * wc;
+    for (int u = 0; u < wc; u++)
+        idx[e * rec + off + u] = h[2 * u] | h[2 * u + 1] << 16;
+}
+
+// per-row codes of the shared rows: partial start | count << 24
+__global__ void k_lay_codes(int64_t nfx, const int32_t *fx_rows, const int32_t *fx_off,
+                            uint32_t *code)
+{
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u < nfx)
+        code[fx_rows[u]] = (uint32_t)fx_off[u] | (uint32_t)(fx_off[u + 1] - fx_off[u]) << 24;
+}
+
+__global__ void k_lay_fill_u32(int64_t n, uint32_t v, uint32_t *a)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        a[i] = v;
+}
+
+// device temporaries of the layout build, freed on every exit
+struct DevTemps {
+    std::vector<void *> p;
+    template <typename T>
+    int get(T **d, size_t count)
+    {
+        *d = nullptr;
+        const int rc = phg::dmalloc((void **)d, std::max<size_t>(count * sizeof(T), 4));
+        if (rc == 0)
+            p.push_back((void *)*d);
+        return rc;
+    }
+    ~DevTemps()
+    {
+        for (void *q : p)
+            phg::dfree(q);
+    }
+};
+
+template <typename T>
+int read_one(T *h, const T *d)
+{
+    return phg::download(h, d, sizeof(T));
+}
+
+// The layout of the blocks of EBLK consecutive elements in Morton order
+// (d_eperm: that order, taken over by the layout), built on the device.
+// Returns PHG_HIP_OK, 1 when some block touches more than KMAX rows (the
+// host's split builds the layout then; nothing is kept), or an error.
+template <int P>
+int build_layout_device(EopLayout &L, const phg_hip_space *sp, const phg_hip_mat *A,
+                        int32_t *d_eperm, int codes, hipStream_t s)
+{
+    using S = EopShape<P>;
+    constexpr int NB = S::NB, EBLK = S::EBLK, KMAX = S::KMAX;
+    constexpr int HS = P == 1 ? 2048 : (P == 2 ? 4096 : 8192);
+    constexpr int KP = P == 1 ? 1024 : (P == 2 ? 2048 : 4096);
+    static_assert(2 * KMAX <= HS && KMAX <= KP && NB * EBLK <= 32 * LB, "layout kernel sizes");
+    using namespace phg;
+    int rc;
+    const int64_t ne = sp->nelem, n = A->n;
+    const int64_t nblk = (ne + EBLK - 1) / EBLK;
+    const int nbk = (int)nblk;
+    auto grid = [](int64_t m) { return dim3((unsigned)std::max<int64_t>(1, (m + 255) / 256)); };
+    DevTemps tmp;
+    int32_t *iperm = nullptr, *emap = nullptr, *cnt = nullptr, *rows_ft = nullptr,
+            *ns_g = nullptr, *nk_g = nullptr, *stats = nullptr;
+    uint32_t *kd = nullptr;
+    if ((A->q_perm && (rc = tmp.get(&iperm, (size_t)n))) ||
+        (rc = tmp.get(&emap, (size_t)(ne * NB))) || (rc = tmp.get(&cnt, (size_t)n)) ||
+        (rc = tmp.get(&rows_ft, (size_t)(nblk * KMAX))) ||
+        (rc = tmp.get(&ns_g, (size_t)(nblk * KMAX))) ||
+        (rc = tmp.get(&nk_g, (size_t)(nblk + 1))) || (rc = tmp.get(&stats, 8)) ||
+        (rc = tmp.get(&kd, (size_t)(ne * NB))))
+        return rc;
+    PHG_HIP_CHECK(hipMemsetAsync(cnt, 0, (size_t)n * 4, s));
+    PHG_HIP_CHECK(hipMemsetAsync(nk_g, 0, (size_t)(nblk + 1) * 4, s));
+    PHG_HIP_CHECK(hipMemsetAsync(stats, 0, 8 * 4, s));
+    if (iperm)
+        hipLaunchKernelGGL(k_lay_iperm, grid(n), dim3(256), 0, s, n, (const int32_t *)A->q_perm,
+                           iperm);
+    hipLaunchKernelGGL(k_lay_emap, grid(ne * NB), dim3(256), 0, s, ne, NB,
+                       (const int32_t *)d_eperm, (const int32_t *)sp->d_map,
+                       (const int32_t *)iperm, (const uint8_t *)A->d_bdry_row, emap);
+    hipLaunchKernelGGL((k_lay_touch<HS>), dim3((unsigned)nblk), dim3(LB), 0, s, ne, NB, EBLK,
+                       KMAX, (const int32_t *)emap, cnt, rows_ft, ns_g, nk_g, kd, stats);
+    PHG_HIP_CHECK(hipGetLastError());
+    int32_t ovf = 0;
+    if ((rc = read_one(&ovf, stats)))
+        return rc;
+    if (ovf)
+        return 1;
+    // block row ranges
+    void *cub = nullptr;
+    size_t cub_bytes = 0, need = 0;
+    auto cub_reserve = [&](size_t bytes) -> int {
+        if (bytes <= cub_bytes)
+            return PHG_HIP_OK;
+        cub_bytes = bytes;
+        return tmp.get((uint8_t **)&cub, bytes);
+    };
+    PHG_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, need, nk_g, nk_g, nbk + 1, s));
+    if ((rc = cub_reserve(need)) || (rc = dmalloc((void **)&L.eb_row, (size_t)(nblk + 1) * 4)))
+        return rc;
+    PHG_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(cub, need, nk_g, L.eb_row, nbk + 1, s));
+    int32_t nent32 = 0;
+    if ((rc = read_one(&nent32, L.eb_row + nblk)))
+        return rc;
+    const int64_t nent = nent32;
+    int32_t *pkey = nullptr, *pval = nullptr, *pkey2 = nullptr, *pval2 = nullptr;
+    if ((rc = dmalloc((void **)&L.rows, (size_t)(nent + 1) * 4)) ||
+        (rc = dmalloc((void **)&L.rs, (size_t)(nent + nblk + 1) * 2)) ||
+        (rc = dmalloc((void **)&L.eb_nint, (size_t)nblk * 4)) ||
+        (rc = dmalloc((void **)&L.ppos, (size_t)(nent + 1) * 4)) ||
+        (rc = tmp.get(&pkey, (size_t)nent + 1)) || (rc = tmp.get(&pval, (size_t)nent + 1)) ||
+        (rc = tmp.get(&pkey2, (size_t)nent + 1)) || (rc = tmp.get(&pval2, (size_t)nent)))
+        return rc;
+    hipLaunchKernelGGL((k_lay_order<KP>), dim3((unsigned)nblk), dim3(LB), 0, s, ne, NB, EBLK,
+                       KMAX, (const int32_t *)cnt, (const int32_t *)rows_ft,
+                       (const int32_t *)ns_g, (const int32_t *)nk_g,
+                       (const int32_t *)L.eb_row, kd, L.rows, L.rs, L.eb_nint, pkey, pval,
+                       stats + 1);
+    PHG_HIP_CHECK(hipGetLastError());
+    // partials of the shared rows in block order
+    PHG_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, need, pkey, pkey2, pval, pval2,
+                                                     (int)nent, 0, 32, s));
+    if ((rc = cub_reserve(need)))
+        return rc;
+    PHG_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(cub, need, pkey, pkey2, pval, pval2,
+                                                     (int)nent, 0, 32, s));
+    int32_t nint32 = 0;
+    if ((rc = read_one(&nint32, stats + 1)))
+        return rc;
+    const int64_t nint = nint32, npart = nent - nint;
+    int32_t *flag = pkey, *fpos = pval;     // the unsorted pairs are spent
+    hipLaunchKernelGGL(k_lay_flags, grid(npart + 1), dim3(256), 0, s, npart,
+                       (const int32_t *)pkey2, flag);
+    PHG_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, need, flag, fpos, (int)npart + 1, s));
+    if ((rc = cub_reserve(need)))
+        return rc;
+    PHG_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(cub, need, flag, fpos, (int)npart + 1, s));
+    int32_t nfx32 = 0;
+    if ((rc = read_one(&nfx32, fpos + npart)))
+        return rc;
+    const int64_t nfx = nfx32;
+    // Dirichlet rows after the shared rows, in row order
+    uint8_t *dflag = nullptr;
+    int32_t *dsel = nullptr;
+    if ((rc = tmp.get(&dflag, (size_t)n)) || (rc = tmp.get(&dsel, (size_t)n)))
+        return rc;
+    hipLaunchKernelGGL(k_lay_dflags, grid(n), dim3(256), 0, s, n, (const int32_t *)A->q_perm,
+                       (const uint8_t *)A->d_bdry_row, dflag);
+    hipcub::CountingInputIterator<int32_t> ci(0);
+    PHG_HIP_CHECK(hipcub::DeviceSelect::Flagged(nullptr, need, ci, dflag, dsel, stats + 2,
+                                                (int)n, s));
+    if ((rc = cub_reserve(need)))
+        return rc;
+    PHG_HIP_CHECK(hipcub::DeviceSelect::Flagged(cub, need, ci, dflag, dsel, stats + 2, (int)n,
+                                                s));
+    int32_t ndir32 = 0;
+    if ((rc = read_one(&ndir32, stats + 2)))
+        return rc;
+    const int64_t ndir = ndir32;
+    if ((rc = dmalloc((void **)&L.fx_rows, (size_t)(nfx + ndir) * 4)) ||
+        (rc = dmalloc((void **)&L.fx_off, (size_t)(nfx + 1) * 4)))
+        return rc;
+    hipLaunchKernelGGL(k_lay_fill_u32, grid(nent + 1), dim3(256), 0, s, nent + 1, 0xffffffffu,
+                       (uint32_t *)L.ppos);
+    hipLaunchKernelGGL(k_lay_partials, grid(npart), dim3(256), 0, s, npart,
+                       (const int32_t *)pkey2, (const int32_t *)pval2, (const int32_t *)flag,
+                       (const int32_t *)fpos, L.ppos, L.fx_rows, L.fx_off, stats + 3);
+    PHG_HIP_CHECK(hipGetLastError());
+    const int32_t np32 = (int32_t)npart, zero = 0;
+    if ((rc = upload(L.fx_off + nfx, &np32, 4)) || (rc = upload(L.rows + nent, &zero, 4)) ||
+        (rc = upload(L.ppos + nent, &zero, 4)))
+        return rc;
+    PHG_HIP_CHECK(hipMemsetAsync(L.rs + nent + nblk, 0, 2, s));
+    if (ndir > 0)
+        PHG_HIP_CHECK(hipMemcpyAsync(L.fx_rows + nfx, dsel, (size_t)ndir * 4,
+                                     hipMemcpyDeviceToDevice, s));
+    // element ranges, index chunks
+    std::vector<int32_t> eb_elem((size_t)nblk + 1);
+    for (int64_t b = 0; b <= nblk; b++)
+        eb_elem[(size_t)b] = (int32_t)std::min<int64_t>(b * EBLK, ne);
+    const int rec = eop_record_dwords(P);
+    if ((rc = dmalloc((void **)&L.eb_elem, (size_t)(nblk + 1) * 4)) ||
+        (rc = upload(L.eb_elem, eb_elem.data(), (size_t)(nblk + 1) * 4)) ||
+        (rc = dmalloc((void **)&L.idx, (size_t)ne * rec * 4)))
+        return rc;
+    hipLaunchKernelGGL(k_lay_chunks, grid(ne * 4), dim3(256), 0, s, ne, P, NB,
+                       (const uint32_t *)kd, L.idx);
+    PHG_HIP_CHECK(hipGetLastError());
+    // per-row codes when every partial start and count fits
+    int32_t maxcnt = 0;
+    if ((rc = read_one(&maxcnt, stats + 3)))
+        return rc;
+    if (codes && npart < ((int64_t)1 << 24) - 1 && maxcnt < 256) {
+        if ((rc = dmalloc((void **)&L.fxc, (size_t)n * 4)))
+            return rc;
+        hipLaunchKernelGGL(k_lay_fill_u32, grid(n), dim3(256), 0, s, n, EOP_NOT_SHARED, L.fxc);
+        hipLaunchKernelGGL(k_lay_codes, grid(nfx), dim3(256), 0, s, nfx,
+                           (const int32_t *)L.fx_rows, (const int32_t *)L.fx_off, L.fxc);
+    }
+    PHG_HIP_CHECK(hipGetLastError());
+    PHG_HIP_CHECK(hipStreamSynchronize(s));
+    L.nelem = ne;
+    L.n = n;
+    L.nblk = nblk;
+    L.nent = nent;
+    L.nint = nint;
+    L.nfx = nfx;
+    L.ndir = ndir;
+    L.npart = npart;
+    L.eperm = d_eperm;            // the layout holds the order from here on
+    return PHG_HIP_OK;
+}
+
+}  // namespace
+
 //============================================================== C ABI ====
 
 extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
@@ -1004,27 +1589,80 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
     // the space keeps host copies of its element maps
     const std::vector<int32_t> &map = sp->h_map, &ev = sp->h_elem_verts;
     const std::vector<double> &xyz = sp->h_verts;
-    std::vector<int32_t> perm;
-    std::vector<uint8_t> brow((size_t)n);
     if (map.size() != (size_t)ne * nb || ev.size() != (size_t)ne * 4 ||
         xyz.size() != (size_t)sp->nvert * 3)
         return set_error(PHG_HIP_E_STATE, "phg_hip_mat_element_operator: space without host maps");
-    if ((rc = download(brow.data(), A->d_bdry_row, (size_t)n)))
-        return rc;
-    // solver numbering: solver index -> PHG index (identity without q_perm)
-    std::vector<int32_t> iperm((size_t)n);
-    if (A->q_perm) {
-        perm.resize((size_t)n);
-        if ((rc = download(perm.data(), A->q_perm, (size_t)n * 4)))
-            return rc;
-        for (int64_t i = 0; i < n; i++)
-            iperm[(size_t)perm[(size_t)i]] = (int32_t)i;
+    auto put = [&](void **d, const void *h, size_t bytes) {
+        int r2 = dmalloc(d, std::max<size_t>(bytes, 4));
+        if (r2 == 0 && bytes > 0)
+            r2 = upload(*d, h, bytes);
+        return r2;
+    };
+    // both builders: A fragments, mass row sums, launch grids
+    auto finish = [&]() -> int {
+    int ks = p == 1 ? EopShape<1>::KS : (p == 2 ? EopShape<2>::KS : EopShape<3>::KS);
+    // A fragments: bf16 parts (P3) or FP32 values (P1, P2), as bytes;
+    // without and with the mass tensor
+    std::vector<uint8_t> tiles[2];
+    for (int ms = 0; ms < 2; ms++) {
+        if (p == 3) {
+            std::vector<uint16_t> tb;
+            build_tiles(p, nb, ks, ms != 0, tb);
+            tiles[ms].resize(tb.size() * 2);
+            memcpy(tiles[ms].data(), tb.data(), tiles[ms].size());
+        }
+        else if (p == 2) {
+            std::vector<float> tf;
+            if (!build_tiles_pts(p, nb, tf, L.msum))
+                return set_error(PHG_HIP_E_STATE,
+                                 "element operator set-up: the P2 rule has not %d points", kNp2);
+            tiles[ms].resize(tf.size() * 4);
+            memcpy(tiles[ms].data(), tf.data(), tiles[ms].size());
+        }
+        else {
+            std::vector<float> tf;
+            build_tiles_f32(p, nb, ms != 0, tf);
+            tiles[ms].resize(tf.size() * 4);
+            memcpy(tiles[ms].data(), tf.data(), tiles[ms].size());
+        }
     }
-    else
-        std::iota(iperm.begin(), iperm.end(), 0);
-    mark("downloads");
+    // row sums of the mass tensor; the local indices 2 t, 2 t + 1 of a tile
+    // are of one kind (vertex, edge or face nodes: the kinds start at even
+    // local indices), so their sums agree and one value serves the tile
+    // (P2: per output row, set with the tiles)
+    if (p != 2) {
+        std::vector<double> T((size_t)NREF * nb * nb);
+        eop_reference_tensors(p, T.data());
+        for (int t = 0; t < nb / 2; t++) {
+            double r[2] = {0.0, 0.0};
+            for (int h = 0; h < 2; h++)
+                for (int j = 0; j < nb; j++)
+                    r[h] += T[((size_t)6 * nb + 2 * t + h) * nb + j];
+            if (std::fabs(r[0] - r[1]) > 1e-14)
+                return set_error(PHG_HIP_E_STATE,
+                                 "element operator set-up: mass row sums of tile %d differ", t);
+            L.msum[t] = (float)r[0];
+        }
+    }
+    L.p = p;
+    L.nb = nb;
+    L.ks = ks;
+    if ((rc = put((void **)&L.tiles, tiles[0].data(), tiles[0].size())) ||
+        (rc = put((void **)&L.tiles_mass, tiles[1].data(), tiles[1].size())))
+        return rc;
+    // grids: every block group resident; partial sums of both kernels fit
+    // the solvers' partial-sum segment
+    const int gb = p == 1 ? grid_block<1, false>()
+                          : (p == 2 ? grid_block<2, false>() : grid_block<3, false>());
+    L.grid_b = (int)std::max<int64_t>(1, std::min<int64_t>({L.nblk, (int64_t)gb,
+                                                             (int64_t)EOP_MAX_PART}));
+    L.grid_f = (int)std::max<int64_t>(1, std::min<int64_t>((L.nfx + FB - 1) / FB, 4096));
+    mark("tiles, grids");
+    return PHG_HIP_OK;
+    };
     // elements in Morton order of their centroids
     std::vector<int32_t> eperm((size_t)ne);
+    bool on_device = false;
     {
         double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
         for (int64_t v = 0; v < sp->nvert; v++)
@@ -1061,12 +1699,50 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
             done_sort();
             return rc ? rc : set_error(PHG_HIP_E_HIP, "element order: radix sort failed");
         }
-        rc = download(eperm.data(), di2, (size_t)ne * 4);
+        // blocks of EBLK consecutive elements built on the device when each
+        // touches at most KMAX rows (PHG_HIP_EOP_HOST_LAYOUT: the host split)
+        if (getenv("PHG_HIP_EOP_HOST_LAYOUT") == nullptr) {
+            rc = p == 1 ? build_layout_device<1>(L, sp, A, di2, codes, s)
+                        : (p == 2 ? build_layout_device<2>(L, sp, A, di2, codes, s)
+                                  : build_layout_device<3>(L, sp, A, di2, codes, s));
+            if (rc == PHG_HIP_OK) {
+                di2 = nullptr;        // the layout holds the order
+                on_device = true;
+            }
+            else if (rc != 1) {
+                done_sort();
+                return rc;
+            }
+            rc = PHG_HIP_OK;
+        }
+        if (!on_device)
+            rc = download(eperm.data(), di2, (size_t)ne * 4);
         done_sort();
         if (rc)
             return rc;
     }
+    if (on_device) {
+        L.on_device = 1;
+        mark("device layout");
+        return finish();
+    }
     mark("element order");
+    std::vector<int32_t> perm;
+    std::vector<uint8_t> brow((size_t)n);
+    if ((rc = download(brow.data(), A->d_bdry_row, (size_t)n)))
+        return rc;
+    // solver numbering: solver index -> PHG index (identity without q_perm)
+    std::vector<int32_t> iperm((size_t)n);
+    if (A->q_perm) {
+        perm.resize((size_t)n);
+        if ((rc = download(perm.data(), A->q_perm, (size_t)n * 4)))
+            return rc;
+        for (int64_t i = 0; i < n; i++)
+            iperm[(size_t)perm[(size_t)i]] = (int32_t)i;
+    }
+    else
+        std::iota(iperm.begin(), iperm.end(), 0);
+    mark("downloads");
     // element DOF map in the solver numbering, Dirichlet DOFs -1
     HostVec<int32_t> emap((size_t)ne * nb);
     std::vector<uint8_t> bq((size_t)n);
@@ -1286,53 +1962,6 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
         if (bq[(size_t)i])
             fx_rows.push_back((int32_t)i);
     const int64_t ndir = (int64_t)fx_rows.size() - nfx;
-    int ks = p == 1 ? EopShape<1>::KS : (p == 2 ? EopShape<2>::KS : EopShape<3>::KS);
-    // A fragments: bf16 parts (P3) or FP32 values (P1, P2), as bytes;
-    // without and with the mass tensor
-    std::vector<uint8_t> tiles[2];
-    for (int ms = 0; ms < 2; ms++) {
-        if (p == 3) {
-            std::vector<uint16_t> tb;
-            build_tiles(p, nb, ks, ms != 0, tb);
-            tiles[ms].resize(tb.size() * 2);
-            memcpy(tiles[ms].data(), tb.data(), tiles[ms].size());
-        }
-        else if (p == 2) {
-            std::vector<float> tf;
-            if (!build_tiles_pts(p, nb, tf, L.msum))
-                return set_error(PHG_HIP_E_STATE,
-                                 "element operator set-up: the P2 rule has not %d points", kNp2);
-            tiles[ms].resize(tf.size() * 4);
-            memcpy(tiles[ms].data(), tf.data(), tiles[ms].size());
-        }
-        else {
-            std::vector<float> tf;
-            build_tiles_f32(p, nb, ms != 0, tf);
-            tiles[ms].resize(tf.size() * 4);
-            memcpy(tiles[ms].data(), tf.data(), tiles[ms].size());
-        }
-    }
-    // row sums of the mass tensor; the local indices 2 t, 2 t + 1 of a tile
-    // are of one kind (vertex, edge or face nodes: the kinds start at even
-    // local indices), so their sums agree and one value serves the tile
-    // (P2: per output row, set with the tiles)
-    if (p != 2) {
-        std::vector<double> T((size_t)NREF * nb * nb);
-        eop_reference_tensors(p, T.data());
-        for (int t = 0; t < nb / 2; t++) {
-            double r[2] = {0.0, 0.0};
-            for (int h = 0; h < 2; h++)
-                for (int j = 0; j < nb; j++)
-                    r[h] += T[((size_t)6 * nb + 2 * t + h) * nb + j];
-            if (std::fabs(r[0] - r[1]) > 1e-14)
-                return set_error(PHG_HIP_E_STATE,
-                                 "element operator set-up: mass row sums of tile %d differ", t);
-            L.msum[t] = (float)r[0];
-        }
-    }
-    L.p = p;
-    L.nb = nb;
-    L.ks = ks;
     L.nelem = ne;
     L.n = n;
     L.nblk = nblk;
@@ -1341,12 +1970,6 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
     L.nfx = nfx;
     L.ndir = ndir;
     L.npart = npart;
-    auto put = [&](void **d, const void *h, size_t bytes) {
-        int r2 = dmalloc(d, std::max<size_t>(bytes, 4));
-        if (r2 == 0 && bytes > 0)
-            r2 = upload(*d, h, bytes);
-        return r2;
-    };
     if ((rc = put((void **)&L.eb_elem, eb_elem.data(), eb_elem.size() * 4)) ||
         (rc = put((void **)&L.eb_row, eb_row.data(), eb_row.size() * 4)) ||
         (rc = put((void **)&L.eb_nint, eb_nint.data(), eb_nint.size() * 4)) ||
@@ -1354,20 +1977,11 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
         (rc = put((void **)&L.rs, rs.data(), rs.size() * 2)) ||
         (rc = put((void **)&L.ppos, ppos.data(), ppos.size() * 4)) ||
         (rc = put((void **)&L.idx, idx.data(), idx.size() * 4)) ||
-        (rc = put((void **)&L.tiles, tiles[0].data(), tiles[0].size())) ||
-        (rc = put((void **)&L.tiles_mass, tiles[1].data(), tiles[1].size())) ||
         (rc = put((void **)&L.fx_rows, fx_rows.data(), fx_rows.size() * 4)) ||
         (rc = put((void **)&L.fx_off, fx_off.data(), fx_off.size() * 4)) ||
         (rc = put((void **)&L.eperm, eperm.data(), (size_t)ne * 4)))
         return rc;
     mark("uploads");
-    // grids: every block group resident; partial sums of both kernels fit
-    // the solvers' partial-sum segment
-    const int gb = p == 1 ? grid_block<1, false>()
-                          : (p == 2 ? grid_block<2, false>() : grid_block<3, false>());
-    L.grid_b = (int)std::max<int64_t>(1, std::min<int64_t>({nblk, (int64_t)gb,
-                                                             (int64_t)EOP_MAX_PART}));
-    L.grid_f = (int)std::max<int64_t>(1, std::min<int64_t>((nfx + FB - 1) / FB, 4096));
     // per-row codes: partial start | count << 24 (when every start and count
     // fits)
     bool fits = codes && npart < ((int64_t)1 << 24) - 1;
@@ -1383,7 +1997,7 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
             return rc;
     }
     mark("shared-row codes");
-    return PHG_HIP_OK;
+    return finish();
     };
 
     std::shared_ptr<EopLayout> lay;

@@ -4525,7 +4525,7 @@ extern "C" int phg_hip_element_bytes(const phg_hip_mat *A, int64_t *elem_bytes,
     return PHG_HIP_OK;
 }
 
-extern "C" int phg_hip_element_layout(const phg_hip_mat *A, int64_t out[6])
+extern "C" int phg_hip_element_layout(const phg_hip_mat *A, int64_t out[7])
 {
     if (A == nullptr || A->eop == nullptr || out == nullptr)
         return phg::set_error(PHG_HIP_E_ARG, "phg_hip_element_layout: operator not built");
@@ -4536,6 +4536,7 @@ extern "C" int phg_hip_element_layout(const phg_hip_mat *A, int64_t out[6])
     out[3] = eo->nfx;
     out[4] = eo->npart;
     out[5] = eo->ndir;
+    out[6] = eo->lay && eo->lay->on_device ? 1 : 0;
     return PHG_HIP_OK;
 }
 

@@ -89,6 +89,7 @@ struct EopLayout {
     int32_t *fx_rows = nullptr, *fx_off = nullptr;
     uint32_t *fxc = nullptr;
     int32_t *eperm = nullptr;        // [nelem] Morton position -> element
+    int on_device = 0;               // built on the device (fixed element blocks)
     ~EopLayout();
 };
 

@@ -337,3 +337,52 @@ def test_without_shared_row_codes(hip, monkeypatch, graph):
     assert na == nb and n1a == n1b
     assert all(np.array_equal(u, v) for u, v in zip(xa, xb))
     assert np.array_equal(x1a, x1b)
+
+
+# (order, cubes per side, refined): meshes from test/cube.dat bisected, and
+# one locally refined P3 mesh of the batch workload
+LAYOUT_CASES = [(1, 16, False), (2, 16, False), (2, 32, False), (3, 8, False), (3, 8, True)]
+
+
+@pytest.mark.parametrize("p,n,refined", LAYOUT_CASES)
+def test_device_layout_matches_host_layout(hip, monkeypatch, p, n, refined):
+    """The layout built on the device (blocks of a fixed number of elements
+    in Morton order) against the host's greedy split.  Where the host cuts
+    the same blocks -- no block reaches the row limit -- both layouts hold
+    the same tables, so the operator is the same bit for bit; elsewhere the
+    device falls back to the host split, or its own blocks give an operator
+    within FP32 accuracy of the FP64 matrix."""
+    pa = hip
+    g = batch_meshes(nmesh=1, n=n, seed=4, lmin=3, lmax=3)[0][0] if refined \
+        else pa.Grid.phg_cube(n)
+    bd = g.dof_bdry(p)
+    kappa = np.exp(np.random.default_rng(3).normal(0.0, 0.5, g.nelem))
+    pb = pa.Problem.poisson(use_kappa=True)
+    ys, lays = [], []
+    for host in (False, True):
+        if host:
+            monkeypatch.setenv("PHG_HIP_EOP_HOST_LAYOUT", "1")
+        sp = pa.Space(g, p, bdry=bd)
+        A = pa.assemble(sp, pb, kappa=kappa, f=np.ones(g.ndof(p)), g=np.zeros(g.ndof(p)))
+        A.eliminate()
+        A.element_operator(sp, pb, kappa=kappa)
+        lay = A.element_layout()
+        if host:
+            assert lay["device"] == 0
+        xv = np.random.default_rng(5).standard_normal(A.n)
+        x, y64, y = pa.Vec(A.n, xv), pa.Vec(A.n), pa.Vec(A.n)
+        A.matvec(x, y64)
+        A.element_apply(x, y)
+        a, b = y64.numpy(), y.numpy()
+        assert np.linalg.norm(b - a) / np.linalg.norm(a) < 2e-6, (host, lay)
+        ys.append(b)
+        lays.append(lay)
+        del A, sp
+    monkeypatch.delenv("PHG_HIP_EOP_HOST_LAYOUT")
+    dev, hst = lays
+    same = {k: v for k, v in dev.items() if k != "device"} == \
+        {k: v for k, v in hst.items() if k != "device"}
+    if not refined:
+        assert dev["device"] == 1, dev
+    if dev["device"] == 1 and same:
+        assert np.array_equal(ys[0], ys[1])
