@@ -1366,9 +1366,9 @@ int read_one(T *h, const T *d)
 // (d_eperm: that order, taken over by the layout), built on the device.
 // Returns PHG_HIP_OK, 1 when some block touches more than KMAX rows (the
 // host's split builds the layout then; nothing is kept), or an error.
-template <int P>
+template <int P, typename Mark>
 int build_layout_device(EopLayout &L, const phg_hip_space *sp, const phg_hip_mat *A,
-                        int32_t *d_eperm, int codes, hipStream_t s)
+                        int32_t *d_eperm, int codes, hipStream_t s, Mark &&mark)
 {
     using S = EopShape<P>;
     constexpr int NB = S::NB, EBLK = S::EBLK, KMAX = S::KMAX;
@@ -1407,6 +1407,7 @@ int build_layout_device(EopLayout &L, const phg_hip_space *sp, const phg_hip_mat
     int32_t ovf = 0;
     if ((rc = read_one(&ovf, stats)))
         return rc;
+    mark("layout: element map, touched rows");
     if (ovf)
         return 1;
     // block row ranges
@@ -1450,6 +1451,7 @@ int build_layout_device(EopLayout &L, const phg_hip_space *sp, const phg_hip_mat
     int32_t nint32 = 0;
     if ((rc = read_one(&nint32, stats + 1)))
         return rc;
+    mark("layout: row order, slots, sort");
     const int64_t nint = nint32, npart = nent - nint;
     int32_t *flag = pkey, *fpos = pval;     // the unsorted pairs are spent
     hipLaunchKernelGGL(k_lay_flags, grid(npart + 1), dim3(256), 0, s, npart,
@@ -1509,6 +1511,7 @@ int build_layout_device(EopLayout &L, const phg_hip_space *sp, const phg_hip_mat
     hipLaunchKernelGGL(k_lay_chunks, grid(ne * 4), dim3(256), 0, s, ne, P, NB,
                        (const uint32_t *)kd, L.idx);
     PHG_HIP_CHECK(hipGetLastError());
+    mark("layout: partials, Dirichlet rows");
     // per-row codes when every partial start and count fits
     int32_t maxcnt = 0;
     if ((rc = read_one(&maxcnt, stats + 3)))
@@ -1522,6 +1525,7 @@ int build_layout_device(EopLayout &L, const phg_hip_space *sp, const phg_hip_mat
     }
     PHG_HIP_CHECK(hipGetLastError());
     PHG_HIP_CHECK(hipStreamSynchronize(s));
+    mark("layout: chunks, codes");
     L.nelem = ne;
     L.n = n;
     L.nblk = nblk;
@@ -1699,12 +1703,13 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
             done_sort();
             return rc ? rc : set_error(PHG_HIP_E_HIP, "element order: radix sort failed");
         }
+        mark("Morton keys, sort");
         // blocks of EBLK consecutive elements built on the device when each
         // touches at most KMAX rows (PHG_HIP_EOP_HOST_LAYOUT: the host split)
         if (getenv("PHG_HIP_EOP_HOST_LAYOUT") == nullptr) {
-            rc = p == 1 ? build_layout_device<1>(L, sp, A, di2, codes, s)
-                        : (p == 2 ? build_layout_device<2>(L, sp, A, di2, codes, s)
-                                  : build_layout_device<3>(L, sp, A, di2, codes, s));
+            rc = p == 1 ? build_layout_device<1>(L, sp, A, di2, codes, s, mark)
+                        : (p == 2 ? build_layout_device<2>(L, sp, A, di2, codes, s, mark)
+                                  : build_layout_device<3>(L, sp, A, di2, codes, s, mark));
             if (rc == PHG_HIP_OK) {
                 di2 = nullptr;        // the layout holds the order
                 on_device = true;
@@ -1726,7 +1731,7 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
         mark("device layout");
         return finish();
     }
-    mark("element order");
+    mark("element order download");
     std::vector<int32_t> perm;
     std::vector<uint8_t> brow((size_t)n);
     if ((rc = download(brow.data(), A->d_bdry_row, (size_t)n)))
