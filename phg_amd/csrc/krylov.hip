@@ -119,6 +119,54 @@ __device__ inline void sum_partials(const double *partials, int np,
     __syncthreads();
 }
 
+// sum_partials with f() called between the loads of the partials and their
+// reduction: loads that f issues are in flight while the partials are summed,
+// and since loads complete in order, waiting for the partials does not wait
+// for them.  Same summation order as sum_partials (np <= MAX_GRID).
+template <int NV, typename F>
+__device__ inline void sum_partials_then(const double *partials, int np, double (&out)[NV],
+                                         F &&f)
+{
+    constexpr int U = MAX_GRID / BS;
+    double pv[U][NV];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const int i = threadIdx.x + u * BS;
+#pragma unroll
+        for (int k = 0; k < NV; k++)
+            pv[u][k] = i < np ? partials[(int64_t)k * np + i] : 0.0;
+    }
+    f();
+    __shared__ double sh[NV][BS / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    double acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++)
+        acc[k] = 0.0;
+#pragma unroll
+    for (int u = 0; u < U; u++)
+#pragma unroll
+        for (int k = 0; k < NV; k++)
+            if ((int)threadIdx.x + u * BS < np)
+                acc[k] += pv[u][k];
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        double s = wave_sum(acc[k]);
+        if (lane == 0)
+            sh[k][wid] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < BS / 64; w++)
+            s += sh[k][w];
+        out[k] = s;
+    }
+    __syncthreads();
+}
+
 // solver state (two copies, see above)
 struct State {
     double rz, rz_prev, res, tol, rb, r0;
@@ -1777,10 +1825,21 @@ __global__ void __launch_bounds__(BS) k_ru_p(int64_t n, const float *r32,
             *sout = s;
         return;
     }
+    // this work-item's first group of 4 rows of an iteration slot is loaded
+    // while the partials are summed (an update slot leaves them unused)
+    const int64_t n4 = n >> 2, q0 = (int64_t)blockIdx.x * BS + threadIdx.x;
+    const bool has0 = q0 < n4;
+    Q4<float> r0, d0, p0;
     double t[3];
-    sum_partials<3>(part_in,
-                    s.mode == RU_ITER ? np_iter : (s.mode == RU_UPDATE ? np_upd : np_start),
-                    t);
+    sum_partials_then<3>(part_in,
+                         s.mode == RU_ITER ? np_iter : (s.mode == RU_UPDATE ? np_upd : np_start),
+                         t, [&]() {
+                             if (has0) {
+                                 r0 = ld4(r32, q0);
+                                 d0 = ld4(diag32, q0);
+                                 p0 = ld4((const float *)p32, q0);
+                             }
+                         });
     const double res = sqrt(t[0]), rz = t[1];
     State o = s;
     o.res = res;
@@ -1863,21 +1922,24 @@ __global__ void __launch_bounds__(BS) k_ru_p(int64_t n, const float *r32,
         return;
     }
     const float tb = (float)beta;
-    for_each4(n, [&](int64_t q) {
-        const Q4<float> rv = ld4(r32, q), dv = ld4(diag32, q);
-        Q4<float> pv;
-        if (!first)
-            pv = ld4((const float *)p32, q);
+    auto quad = [&](int64_t q, const Q4<float> &rv, const Q4<float> &dv, Q4<float> pv) {
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const float z = rv.v[k] / dv.v[k];
             pv.v[k] = first ? z : tb * pv.v[k] + z;
         }
         st4(p32, q, pv);
-    }, [&](int64_t i) {
+    };
+    if (has0)
+        quad(q0, r0, d0, p0);
+    for (int64_t q = q0 + (int64_t)gridDim.x * BS; q < n4; q += (int64_t)gridDim.x * BS)
+        quad(q, ld4(r32, q), ld4(diag32, q),
+             first ? Q4<float>{{0.0f, 0.0f, 0.0f, 0.0f}} : ld4((const float *)p32, q));
+    if (blockIdx.x == 0 && (int64_t)threadIdx.x < n - 4 * n4) {
+        const int64_t i = 4 * n4 + threadIdx.x;
         float z = r32[i] / diag32[i];
         p32[i] = first ? z : tb * p32[i] + z;
-    });
+    }
 }
 
 template <typename View>
@@ -1954,8 +2016,24 @@ __global__ void __launch_bounds__(BS) k_ru_update(int64_t n, float *x32,
     }
     if (st->mode != RU_ITER)
         return;
+    // this work-item's first group of 4 rows is loaded while the partials
+    // of (p, Ap) are summed (none of it depends on alpha)
+    const int64_t n4 = n >> 2, q0 = (int64_t)blockIdx.x * BS + threadIdx.x;
+    const bool has0 = q0 < n4;
+    Q4<float> x0, p0, r0, a0, d0;
+    uint4 c0 = make_uint4(0u, 0u, 0u, 0u);
     double pap[1];
-    sum_partials<1>(part_in, np_in, pap);
+    sum_partials_then<1>(part_in, np_in, pap, [&]() {
+        if (has0) {
+            x0 = ld4((const float *)x32, q0);
+            p0 = ld4(p, q0);
+            r0 = ld4((const float *)r, q0);
+            a0 = ld4(Ap, q0);
+            d0 = ld4(diag, q0);
+            if (fxc)
+                c0 = ((const uint4 *)fxc)[q0];
+        }
+    });
     if (pap[0] == 0.0) {
         if (blockIdx.x == 0 && threadIdx.x == 0)
             st->done = BREAKDOWN;
@@ -1984,11 +2062,10 @@ __global__ void __launch_bounds__(BS) k_ru_update(int64_t n, float *x32,
         v[0] += (double)ri * (double)ri;
         v[1] += (double)ri * (double)z;
     };
-    for_each4(n, [&](int64_t q) {
-        Q4<float> xv = ld4(x32, q), pv = ld4(p, q), rv = ld4((const float *)r, q),
-                  av = ld4(Ap, q), dv = ld4(diag, q);
+    // for_each4's order: the work-item's groups of 4, then the tail
+    auto quad = [&](int64_t q, Q4<float> xv, const Q4<float> &pv, Q4<float> rv, Q4<float> av,
+                    const Q4<float> &dv, const uint4 &cv) {
         if (fxc) {
-            const uint4 cv = ((const uint4 *)fxc)[q];
             av.v[0] = shared(cv.x, av.v[0]);
             av.v[1] = shared(cv.y, av.v[1]);
             av.v[2] = shared(cv.z, av.v[2]);
@@ -2005,7 +2082,14 @@ __global__ void __launch_bounds__(BS) k_ru_update(int64_t n, float *x32,
         }
         st4(x32, q, xv);
         st4(r, q, rv);
-    }, one);
+    };
+    if (has0)
+        quad(q0, x0, p0, r0, a0, d0, c0);
+    for (int64_t q = q0 + (int64_t)gridDim.x * BS; q < n4; q += (int64_t)gridDim.x * BS)
+        quad(q, ld4((const float *)x32, q), ld4(p, q), ld4((const float *)r, q), ld4(Ap, q),
+             ld4(diag, q), fxc ? ((const uint4 *)fxc)[q] : make_uint4(0u, 0u, 0u, 0u));
+    if (blockIdx.x == 0 && (int64_t)threadIdx.x < n - 4 * n4)
+        one(4 * n4 + threadIdx.x);
     write_partials<3>(v, part_out);
 }
 
