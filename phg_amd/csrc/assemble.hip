@@ -1145,8 +1145,10 @@ template <int G, int E, bool LIST = false>
 __global__ void __launch_bounds__(64) k_row_walk_hash(WalkArgs w)
 {
     constexpr int TS = 2 * E, NG = 64 / G;
+    constexpr int CC = 64;        // incident-element codes of a row kept in LDS
     __shared__ int32_t key[NG][TS];
     __shared__ int32_t ent[NG][TS];
+    __shared__ int32_t icode[NG][CC];
     __shared__ int32_t ecol[NG][E];
     __shared__ double eval[NG][E];
     const int lane = threadIdx.x, grp = lane / G, b = lane % G;
@@ -1166,27 +1168,48 @@ __global__ void __launch_bounds__(64) k_row_walk_hash(WalkArgs w)
             smax = max(smax, __shfl_xor(smax, o, 64));
         for (int k = b; k < TS; k += G)
             key[grp][k] = -1;
+        // the row's incident-element codes, so that a step's loads depend on
+        // an LDS read only
+        for (int k = b; k < min(steps, CC); k += G)
+            icode[grp][k] = w.inc[t0 + k];
         __syncthreads();
         int32_t nc = 0;
         bool over = false;
         double rhs = 0.0;
+        // loads of a step: element row value, its column (loaded whatever the
+        // value, so it does not wait for it) and the element's load entry;
+        // step t + 1's are issued before step t's insertion
+        int32_t cr = -1;
+        double vr = 0.0, br = 0.0;
+        auto fetch = [&](int t) {
+            if (t < steps) {
+                const int32_t code = t < CC ? icode[grp][t] : w.inc[t0 + t];
+                if (b < M) {
+                    const int64_t e = code / M;
+                    vr = w.K[(int64_t)code * M + b];  // [e][a][b], code = e M + a
+                    cr = (int32_t)row_of(w.map, e, b, w.N, w.dim);
+                }
+                if (b == 0)
+                    br = w.B[code];
+            }
+        };
+        fetch(0);
         for (int t = 0; t < smax; t++) {
             int32_t c = -1;
             double v = 0.0;
             if (t < steps) {
-                const int32_t code = w.inc[t0 + t];
                 if (b < M) {
-                    const int64_t e = code / M;
-                    v = w.K[(int64_t)code * M + b];   // [e][a][b], code = e M + a
+                    v = vr;
                     if (v != 0.0)                     // matvec.c:5460
-                        c = (int32_t)row_of(w.map, e, b, w.N, w.dim);
+                        c = cr;
                 }
                 if (b == 0) {
-                    const double bval = w.B[code];
+                    const double bval = br;
                     if (bval != 0.0)                  // matvec.c:5708
                         rhs += bval;
                 }
             }
+            fetch(t + 1);
             int slot = -1;
             bool fresh = false;
             if (c >= 0 && !over) {
