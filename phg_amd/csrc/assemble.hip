@@ -640,27 +640,29 @@ __global__ void __launch_bounds__(128) k_element(ElemArgs a)
 #pragma unroll
         for (int k = 0; k < N; k++)
             fl[k] = a.f[m[k]];
-        double fq[Shape<P>::NQT];
+        // points outer, local indices inner: each B[i] still sums its
+        // terms in point order from 0, and f_h at a point stays in a
+        // register instead of a per-point array
+        double d[N];
+#pragma unroll
+        for (int i = 0; i < N; i++)
+            d[i] = 0.0;
         for (int q = 0; q < T.nq_r; q++) {
-            double s = 0.0;
+            double fq = 0.0;
 #pragma unroll
             for (int k = 0; k < N; k++)
-                s += T.br[q][k] * fl[k];
-            fq[q] = s;
-        }
-        for (int i = 0; i < N; i++) {
-            double d = 0.0;
-            if (QC) {
-                // phgQCIntegrate_(Q_BAS i, f): (phi_i f) w per point
-                for (int q = 0; q < T.nq_r; q++)
-                    d += T.br[q][i] * fq[q] * T.wr[q];
+                fq += T.br[q][k] * fl[k];
+#pragma unroll
+            for (int i = 0; i < N; i++) {
+                if (QC)
+                    d[i] += T.br[q][i] * fq * T.wr[q];   // phgQCIntegrate_: (phi_i f) w
+                else
+                    d[i] += T.br[q][i] * T.wr[q] * fq;
             }
-            else {
-                for (int q = 0; q < T.nq_r; q++)
-                    d += T.br[q][i] * T.wr[q] * fq[q];
-            }
-            B[i] = d * vol;
         }
+#pragma unroll
+        for (int i = 0; i < N; i++)
+            B[i] = d[i] * vol;
         return;
     }
 
