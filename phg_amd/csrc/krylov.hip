@@ -1117,8 +1117,19 @@ __global__ void __launch_bounds__(BS) k_pcg_p(int64_t n, const T *r,
             *sout = s;
         return;
     }
-    double t[3];
-    sum_partials<3>(part_in, np_in, t);    // (r,r), (r,z), #zero diagonals
+    // the work-item's first group of 4 rows is loaded while the partials
+    // are summed
+    const int64_t n4 = n >> 2, q0 = (int64_t)blockIdx.x * BS + threadIdx.x;
+    const bool has0 = q0 < n4;
+    Q4<T> r0, d0, p0;
+    double t[3];                             // (r,r), (r,z), #zero diagonals
+    sum_partials_then<3>(part_in, np_in, t, [&]() {
+        if (has0) {
+            r0 = ld4(r, q0);
+            d0 = ld4(diag, q0);
+            p0 = ld4((const T *)p, q0);
+        }
+    });
     const double res = sqrt(t[0]), rz = t[1];
     State o = s;
     o.res = res;
@@ -1176,21 +1187,24 @@ __global__ void __launch_bounds__(BS) k_pcg_p(int64_t n, const T *r,
         return;
     const bool first = s.nits == 0;
     const T tb = (T)beta;
-    for_each4(n, [&](int64_t q) {
-        const Q4<T> rv = ld4(r, q), dv = ld4(diag, q);
-        Q4<T> pv;
-        if (!first)
-            pv = ld4((const T *)p, q);
+    auto quad = [&](int64_t q, const Q4<T> &rv, const Q4<T> &dv, Q4<T> pv) {
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const T z = rv.v[k] / dv.v[k];
             pv.v[k] = first ? z : tb * pv.v[k] + z;
         }
         st4(p, q, pv);
-    }, [&](int64_t i) {
+    };
+    if (has0)
+        quad(q0, r0, d0, p0);
+    for (int64_t q = q0 + (int64_t)gridDim.x * BS; q < n4; q += (int64_t)gridDim.x * BS)
+        quad(q, ld4(r, q), ld4(diag, q), first ? Q4<T>{{T(0), T(0), T(0), T(0)}}
+                                               : ld4((const T *)p, q));
+    if (blockIdx.x == 0 && (int64_t)threadIdx.x < n - 4 * n4) {
+        const int64_t i = 4 * n4 + threadIdx.x;
         T z = r[i] / diag[i];
         p[i] = first ? z : tb * p[i] + z;
-    });
+    }
 }
 
 template <typename T, typename View>
@@ -1217,8 +1231,21 @@ __global__ void __launch_bounds__(BS) k_pcg_update(int64_t n, T *x,
 {
     if (st->done)
         return;
+    // the work-item's first group of 4 rows is loaded while the partials
+    // of (p, Ap) are summed
+    const int64_t n4 = n >> 2, q0 = (int64_t)blockIdx.x * BS + threadIdx.x;
+    const bool has0 = q0 < n4;
+    Q4<T> x0, p0, r0, a0, d0;
     double pap[1];
-    sum_partials<1>(part_in, np_in, pap);
+    sum_partials_then<1>(part_in, np_in, pap, [&]() {
+        if (has0) {
+            x0 = ld4((const T *)x, q0);
+            p0 = ld4(p, q0);
+            r0 = ld4((const T *)r, q0);
+            a0 = ld4(Ap, q0);
+            d0 = ld4(diag, q0);
+        }
+    });
     if (pap[0] == 0.0) {                     // solver-phg.c:561
         if (blockIdx.x == 0 && threadIdx.x == 0)
             st->done = BREAKDOWN;
@@ -1226,9 +1253,8 @@ __global__ void __launch_bounds__(BS) k_pcg_update(int64_t n, T *x,
     }
     const T alpha = (T)(st->rz / pap[0]);
     double v[3] = {0.0, 0.0, 0.0};
-    for_each4(n, [&](int64_t q) {
-        Q4<T> xv = ld4((const T *)x, q), pv = ld4(p, q), rv = ld4((const T *)r, q),
-              av = ld4(Ap, q), dv = ld4(diag, q);
+    auto quad = [&](int64_t q, Q4<T> xv, const Q4<T> &pv, Q4<T> rv, const Q4<T> &av,
+                    const Q4<T> &dv) {
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             xv.v[k] += alpha * pv.v[k];
@@ -1240,14 +1266,21 @@ __global__ void __launch_bounds__(BS) k_pcg_update(int64_t n, T *x,
         }
         st4(x, q, xv);
         st4(r, q, rv);
-    }, [&](int64_t i) {
+    };
+    if (has0)
+        quad(q0, x0, p0, r0, a0, d0);
+    for (int64_t q = q0 + (int64_t)gridDim.x * BS; q < n4; q += (int64_t)gridDim.x * BS)
+        quad(q, ld4((const T *)x, q), ld4(p, q), ld4((const T *)r, q), ld4(Ap, q),
+             ld4(diag, q));
+    if (blockIdx.x == 0 && (int64_t)threadIdx.x < n - 4 * n4) {
+        const int64_t i = 4 * n4 + threadIdx.x;
         x[i] += alpha * p[i];
         T ri = r[i] - alpha * Ap[i];
         r[i] = ri;
         T z = ri / diag[i];
         v[0] += (double)ri * (double)ri;
         v[1] += (double)ri * (double)z;
-    });
+    }
     write_partials<3>(v, part_out);
 }
 
