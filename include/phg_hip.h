@@ -429,7 +429,7 @@ int phg_hip_adjoint_logkappa(phg_hip_space *sp, const phg_hip_problem *pb,
 
 /* No reference counterpart (PHG has one CSR kernel, matvec.c:887).  The
  * solvers run their SpMV on one of these layouts of the eliminated matrix,
- * all in the solver numbering (DOFs along a Morton curve of their nodes):
+ * all in the solver numbering (DOFs along a Hilbert curve of their nodes):
  *   B2K, B4K   work-group CSR stream: row blocks of at most 2048 / 4096
  *              entries are loaded coalesced and staged in LDS as products;
  *              one work-item sums a row in storage order;

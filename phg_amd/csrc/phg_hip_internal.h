@@ -212,7 +212,7 @@ struct phg_hip_mat {
     uint64_t space_uid = 0;          // uid of the space phg_hip_assemble used (0: none)
 
     // Solver copy: the eliminated matrix under a symmetric renumbering that
-    // orders DOFs along a Morton curve of their nodes (q_perm: solver index ->
+    // orders DOFs along a Hilbert curve of their nodes (q_perm: solver index ->
     // PHG index); the q_* arrays alias the PHG-order ones when q_perm is null
     int reorder = 1;                 // 1: derive the spatial numbering
     // 1: the PHG-order arrays changed (a batch re-assembled the matrix) and

@@ -1,9 +1,9 @@
 // Solver numbering: the eliminated matrix renumbered so that DOFs follow a
-// Morton (Z-order) curve of their nodes.
+// Hilbert curve of their nodes.
 //
 // PHG numbers DOFs vertex block first, then edges, then faces
 // (dof.c:4821 phgDofMapE2D_), so a vertex row gathers x entries from the far
-// end of the vector and every XCD's L2 sees most of x.  In the Morton
+// end of the vector and every XCD's L2 sees most of x.  In the curve
 // numbering a row's columns lie close to the row itself, and consecutive row
 // blocks -- which the SpMV hands to the same XCD -- touch a compact window of
 // x.  The renumbering is symmetric (rows and columns), keeps each row's
@@ -32,6 +32,39 @@ __device__ inline uint64_t spread3(uint64_t v)
     return v;
 }
 
+// Hilbert index of 21-bit coordinates: J. Skilling's transform of the axes
+// to the transposed Hilbert integer (AIP Conf. Proc. 707, 381 (2004),
+// "Programming the Hilbert curve"), whose bits interleaved give the index.
+// Consecutive indices are face neighbours, unlike the Morton curve's jumps.
+__device__ inline uint64_t hilbert3(uint32_t x0, uint32_t x1, uint32_t x2)
+{
+    uint32_t X[3] = {x0, x1, x2};
+    constexpr uint32_t M = 1u << 20;
+    for (uint32_t Q = M; Q > 1; Q >>= 1) {
+        const uint32_t P = Q - 1;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            if (X[i] & Q)
+                X[0] ^= P;
+            else {
+                const uint32_t t = (X[0] ^ X[i]) & P;
+                X[0] ^= t;
+                X[i] ^= t;
+            }
+        }
+    }
+    X[1] ^= X[0];
+    X[2] ^= X[1];
+    uint32_t t = 0;
+    for (uint32_t Q = M; Q > 1; Q >>= 1)
+        if (X[2] & Q)
+            t ^= Q - 1;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+        X[i] ^= t;
+    return spread3(X[0]) << 2 | spread3(X[1]) << 1 | spread3(X[2]);
+}
+
 __global__ void k_morton(int64_t n, const double *xyz, double x0, double y0,
                          double z0, double sx, double sy, double sz,
                          uint64_t *key, int32_t *idx)
@@ -45,7 +78,7 @@ __global__ void k_morton(int64_t n, const double *xyz, double x0, double y0,
     uint64_t qx = (uint64_t)fmin(fmax(fx * m, 0.0), m);
     uint64_t qy = (uint64_t)fmin(fmax(fy * m, 0.0), m);
     uint64_t qz = (uint64_t)fmin(fmax(fz * m, 0.0), m);
-    key[i] = spread3(qx) | spread3(qy) << 1 | spread3(qz) << 2;
+    key[i] = hilbert3((uint32_t)qx, (uint32_t)qy, (uint32_t)qz);
     idx[i] = (int32_t)i;
 }
 
@@ -140,7 +173,7 @@ int ensure_solver_copy(phg_hip_mat *A)
 {
     if (!A->q_stale)
         return PHG_HIP_OK;
-    // the element operator is kept: its numbering is the node Morton order
+    // the element operator is kept: its numbering is the node curve order
     // the rebuild derives again from the same coordinates, and its
     // coefficients were set with the values that made the PHG-order arrays
     EopData *eop = A->eop;
@@ -224,7 +257,7 @@ int build_solver_copy(phg_hip_mat *A)
             cleanup();
             return rc ? rc : set_error(PHG_HIP_E_HIP, "radix sort failed");
         }
-        tr.mark("Morton keys, sort");
+        tr.mark("curve keys, sort");
         if ((rc = dmalloc((void **)&A->q_perm, (size_t)n * 4)) ||
             (rc = dmalloc((void **)&iperm, (size_t)n * 4)) ||
             (rc = dmalloc((void **)&len, (size_t)(n + 1) * 4)) ||
