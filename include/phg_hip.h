@@ -321,7 +321,7 @@ int phg_hip_batch_destroy(phg_hip_batch *batch);
  * reference tensors T^m) to x on the matrix cores (P1 / P2: FP32 MFMA; P3:
  * bf16 MFMA with operands split into three bf16 parts so products keep FP32
  * precision).  Elements are taken in
- * blocks of consecutive elements (Morton order); each row's sum over its
+ * blocks of consecutive elements (Hilbert-curve order of their centroids); each row's sum over its
  * elements runs in a fixed order (in LDS within a block, then over blocks
  * for rows shared between blocks), so results are deterministic.  Dirichlet
  * rows and columns are treated as

@@ -34,7 +34,7 @@
 // 3 x 3 metric between them is applied per lane.  With a mass term the third
 // product c_6 T^6 x_e goes into the same accumulator.
 //
-// Element blocks.  The elements, in Morton order of their centroids, are cut
+// Element blocks.  The elements, in Hilbert-curve order of their centroids, are cut
 // into blocks of at most EBLK elements touching at most KMAX rows.  A
 // work-group takes a block at a time:
 //   1. x of the block's touched rows into LDS (one gather per row);
@@ -829,17 +829,6 @@ bool build_tiles_pts(int p, int nb, std::vector<float> &out, float *msum)
     return true;
 }
 
-__host__ __device__ inline uint64_t spread3(uint64_t v)
-{
-    v &= 0x1fffffull;
-    v = (v | v << 32) & 0x1f00000000ffffull;
-    v = (v | v << 16) & 0x1f0000ff0000ffull;
-    v = (v | v << 8) & 0x100f00f00f00f00full;
-    v = (v | v << 4) & 0x10c30c30c30c30c3ull;
-    v = (v | v << 2) & 0x1249249249249249ull;
-    return v;
-}
-
 template <int P, bool MASS>
 int grid_block()
 {
@@ -926,9 +915,9 @@ EopLayout::~EopLayout()
     phg::dfree(eperm);
 }
 
-// Morton key of each element's centroid (0.25 times the sum of its vertex
+// Hilbert key of each element's centroid (0.25 times the sum of its vertex
 // coordinates in vertex order), quantised to 21 bits per axis: the rounding
-// steps are spelled out so the key is the one the host formula gives
+// steps are spelled out so the key does not depend on contraction
 __global__ void k_elem_morton(int64_t ne, const double *__restrict__ xyz,
                               const int32_t *__restrict__ ev, double lo0, double lo1,
                               double lo2, double sc, uint64_t *key, int32_t *idx)
@@ -947,7 +936,7 @@ __global__ void k_elem_morton(int64_t ne, const double *__restrict__ xyz,
         const double f = __dmul_rn(__dsub_rn(__dmul_rn(0.25, m), lo[c]), sc);
         q[c] = (uint64_t)fmin(fmax(f, 0.0), 2097151.0);
     }
-    key[e] = spread3(q[0]) | spread3(q[1]) << 1 | spread3(q[2]) << 2;
+    key[e] = phg::hilbert3((uint32_t)q[0], (uint32_t)q[1], (uint32_t)q[2]);
     idx[e] = (int32_t)e;
 }
 
@@ -955,7 +944,7 @@ namespace {
 
 //==================================================== layout on the device ====
 // The element-operator layout built on the device for blocks of EBLK
-// consecutive elements (in Morton order), when every such block touches at
+// consecutive elements (in Hilbert-curve order), when every such block touches at
 // most KMAX rows -- the blocks the host's greedy split makes on such meshes,
 // without its cuts.  Block tables, slot order and partial order follow the
 // host builder's rules: touched rows of a block in first-touch order, then
@@ -1003,7 +992,7 @@ __global__ void k_lay_iperm(int64_t n, const int32_t *perm, int32_t *iperm)
         iperm[perm[i]] = (int32_t)i;
 }
 
-// element map in the solver numbering, elements in Morton order, Dirichlet
+// element map in the solver numbering, elements in curve order, Dirichlet
 // DOFs -1 (brow: Dirichlet flags in PHG's numbering)
 __global__ void k_lay_emap(int64_t ne, int nb, const int32_t *eperm, const int32_t *map,
                            const int32_t *iperm, const uint8_t *brow, int32_t *emap)
@@ -1362,7 +1351,7 @@ int read_one(T *h, const T *d)
     return phg::download(h, d, sizeof(T));
 }
 
-// The layout of the blocks of EBLK consecutive elements in Morton order
+// The layout of the blocks of EBLK consecutive elements in curve order
 // (d_eperm: that order, taken over by the layout), built on the device.
 // Returns PHG_HIP_OK, 1 when some block touches more than KMAX rows (the
 // host's split builds the layout then; nothing is kept), or an error.
@@ -1664,7 +1653,7 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
     mark("tiles, grids");
     return PHG_HIP_OK;
     };
-    // elements in Morton order of their centroids
+    // elements in Hilbert-curve order of their centroids
     std::vector<int32_t> eperm((size_t)ne);
     bool on_device = false;
     {
@@ -1703,7 +1692,7 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
             done_sort();
             return rc ? rc : set_error(PHG_HIP_E_HIP, "element order: radix sort failed");
         }
-        mark("Morton keys, sort");
+        mark("curve keys, sort");
         // blocks of EBLK consecutive elements built on the device when each
         // touches at most KMAX rows (PHG_HIP_EOP_HOST_LAYOUT: the host split)
         if (getenv("PHG_HIP_EOP_HOST_LAYOUT") == nullptr) {

@@ -88,7 +88,7 @@ struct EopLayout {
     float msum[16] = {};
     int32_t *fx_rows = nullptr, *fx_off = nullptr;
     uint32_t *fxc = nullptr;
-    int32_t *eperm = nullptr;        // [nelem] Morton position -> element
+    int32_t *eperm = nullptr;        // [nelem] curve position -> element
     int on_device = 0;               // built on the device (fixed element blocks)
     ~EopLayout();
 };
@@ -154,14 +154,14 @@ struct DictLayout {
 // coordinates, T^m the reference tensors), the element products on bf16
 // MFMA with FP32 operands
 // split into three bf16 parts, then an ordered per-row sum over the row's
-// incident (element, local index) pairs.  Elements are in Morton order of
+// incident (element, local index) pairs.  Elements are in Hilbert-curve order of
 // their centroids; Dirichlet DOFs have map -1 (their columns are the ones
 // elimination removes) and their rows are diag * x.
 struct EopData {
     int p = 0, nb = 0, ks = 0;        // order, basis size, K steps of 32
     int on = 0;                       // used by the FP32 solver slots
     int64_t nelem = 0, n = 0;
-    // element blocks (consecutive elements in Morton order)
+    // element blocks (consecutive elements in curve order)
     int64_t nblk = 0, nent = 0, nint = 0;   // blocks, touched-row entries, interior rows
     int32_t *eb_elem = nullptr;       // [nblk+1] element ranges
     int32_t *eb_row = nullptr;        // [nblk+1] touched-row entry ranges
@@ -345,6 +345,53 @@ __device__ inline void xcd_sweep(int64_t n, int64_t &first, int64_t &end,
     first = n * x / kXcd + j;
     end = n * (x + 1) / kXcd;
 }
+// space-filling curve keys of 21-bit coordinates (solver numbering, element
+// order of the matrix-free operator)
+__host__ __device__ inline uint64_t curve_spread3(uint64_t v)
+{
+    // 21 bits -> every third bit of 63
+    v &= 0x1fffffull;
+    v = (v | v << 32) & 0x1f00000000ffffull;
+    v = (v | v << 16) & 0x1f0000ff0000ffull;
+    v = (v | v << 8) & 0x100f00f00f00f00full;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ull;
+    v = (v | v << 2) & 0x1249249249249249ull;
+    return v;
+}
+
+// Hilbert index of 21-bit coordinates: J. Skilling's transform of the axes
+// to the transposed Hilbert integer (AIP Conf. Proc. 707, 381 (2004),
+// "Programming the Hilbert curve"), whose bits interleaved give the index.
+// Consecutive indices are face neighbours, unlike the Morton curve's jumps.
+__host__ __device__ inline uint64_t hilbert3(uint32_t x0, uint32_t x1, uint32_t x2)
+{
+    uint32_t X[3] = {x0, x1, x2};
+    constexpr uint32_t M = 1u << 20;
+    for (uint32_t Q = M; Q > 1; Q >>= 1) {
+        const uint32_t P = Q - 1;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            if (X[i] & Q)
+                X[0] ^= P;
+            else {
+                const uint32_t t = (X[0] ^ X[i]) & P;
+                X[0] ^= t;
+                X[i] ^= t;
+            }
+        }
+    }
+    X[1] ^= X[0];
+    X[2] ^= X[1];
+    uint32_t t = 0;
+    for (uint32_t Q = M; Q > 1; Q >>= 1)
+        if (X[2] & Q)
+            t ^= Q - 1;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+        X[i] ^= t;
+    return curve_spread3(X[0]) << 2 | curve_spread3(X[1]) << 1 | curve_spread3(X[2]);
+}
+
 // matrix-free operator (eop.hip; reference data from assemble.hip)
 constexpr int EOP_MAX_PART = 2048;   // partial sums one apply writes, at most
 constexpr uint32_t EOP_NOT_SHARED = 0xffffffffu;

@@ -20,51 +20,6 @@ namespace {
 
 constexpr int TB = 256;
 
-__device__ inline uint64_t spread3(uint64_t v)
-{
-    // 21 bits -> every third bit of 63
-    v &= 0x1fffffull;
-    v = (v | v << 32) & 0x1f00000000ffffull;
-    v = (v | v << 16) & 0x1f0000ff0000ffull;
-    v = (v | v << 8) & 0x100f00f00f00f00full;
-    v = (v | v << 4) & 0x10c30c30c30c30c3ull;
-    v = (v | v << 2) & 0x1249249249249249ull;
-    return v;
-}
-
-// Hilbert index of 21-bit coordinates: J. Skilling's transform of the axes
-// to the transposed Hilbert integer (AIP Conf. Proc. 707, 381 (2004),
-// "Programming the Hilbert curve"), whose bits interleaved give the index.
-// Consecutive indices are face neighbours, unlike the Morton curve's jumps.
-__device__ inline uint64_t hilbert3(uint32_t x0, uint32_t x1, uint32_t x2)
-{
-    uint32_t X[3] = {x0, x1, x2};
-    constexpr uint32_t M = 1u << 20;
-    for (uint32_t Q = M; Q > 1; Q >>= 1) {
-        const uint32_t P = Q - 1;
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            if (X[i] & Q)
-                X[0] ^= P;
-            else {
-                const uint32_t t = (X[0] ^ X[i]) & P;
-                X[0] ^= t;
-                X[i] ^= t;
-            }
-        }
-    }
-    X[1] ^= X[0];
-    X[2] ^= X[1];
-    uint32_t t = 0;
-    for (uint32_t Q = M; Q > 1; Q >>= 1)
-        if (X[2] & Q)
-            t ^= Q - 1;
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-        X[i] ^= t;
-    return spread3(X[0]) << 2 | spread3(X[1]) << 1 | spread3(X[2]);
-}
-
 __global__ void k_morton(int64_t n, const double *xyz, double x0, double y0,
                          double z0, double sx, double sy, double sz,
                          uint64_t *key, int32_t *idx)
@@ -78,7 +33,7 @@ __global__ void k_morton(int64_t n, const double *xyz, double x0, double y0,
     uint64_t qx = (uint64_t)fmin(fmax(fx * m, 0.0), m);
     uint64_t qy = (uint64_t)fmin(fmax(fy * m, 0.0), m);
     uint64_t qz = (uint64_t)fmin(fmax(fz * m, 0.0), m);
-    key[i] = hilbert3((uint32_t)qx, (uint32_t)qy, (uint32_t)qz);
+    key[i] = phg::hilbert3((uint32_t)qx, (uint32_t)qy, (uint32_t)qz);
     idx[i] = (int32_t)i;
 }
 
