@@ -70,12 +70,11 @@ int phg_hip_synchronize(void);
 
 /*---------------------------------------------------------- mesh / space */
 
-/* Uploads the mesh and the Lagrange Pk space (order 1..3):
+/* Uploads the mesh and the Lagrange Pk space (order 1..4):
  * verts[nvert][3] (utils.c:5470 phgImport_ result), elem_verts[nelem][4],
  * map[nelem][nbas] = dof.c:4821 phgDofMapE2D_ of every local basis,
  * bdry[ndof] = 1 for Dirichlet DOFs (dof-utils.c:1269 nodal path).
- * The space also keeps host copies of verts, elem_verts and map (for the
- * element operator's layout builder); the caller's arrays are not kept. */
+ * Everything lives on the device; the caller's arrays are not kept. */
 int phg_hip_space_create(int32_t order, int64_t nvert, const double *verts,
                          int64_t nelem, const int32_t *elem_verts,
                          const int32_t *map, int64_t ndof, const uint8_t *bdry,

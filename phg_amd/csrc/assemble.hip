@@ -1928,15 +1928,6 @@ extern "C" int phg_hip_space_create(int32_t order, int64_t nvert,
     sp->nvert = nvert;
     sp->nelem = nelem;
     sp->ndof = ndof;
-    try {
-        sp->h_verts.assign(verts, verts + nvert * 3);
-        sp->h_elem_verts.assign(elem_verts, elem_verts + nelem * 4);
-        sp->h_map.assign(map, map + nelem * nb);
-    }
-    catch (const std::bad_alloc &) {
-        delete sp;
-        return phg::set_error(PHG_HIP_E_NOMEM, "phg_hip_space_create: out of host memory");
-    }
     // bounding box of the vertices: every Lagrange node is a convex
     // combination of its element's vertices, so it bounds the nodes too
     for (int c = 0; c < 3; c++)

@@ -1579,12 +1579,6 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
     // space's layout; only the coefficients depend on kappa.
     const bool same_space = A->space_uid != 0 && A->space_uid == sp->uid;
     auto build_layout = [&](EopLayout &L) -> int {
-    // the space keeps host copies of its element maps
-    const std::vector<int32_t> &map = sp->h_map, &ev = sp->h_elem_verts;
-    const std::vector<double> &xyz = sp->h_verts;
-    if (map.size() != (size_t)ne * nb || ev.size() != (size_t)ne * 4 ||
-        xyz.size() != (size_t)sp->nvert * 3)
-        return set_error(PHG_HIP_E_STATE, "phg_hip_mat_element_operator: space without host maps");
     auto put = [&](void **d, const void *h, size_t bytes) {
         int r2 = dmalloc(d, std::max<size_t>(bytes, 4));
         if (r2 == 0 && bytes > 0)
@@ -1657,12 +1651,8 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
     std::vector<int32_t> eperm((size_t)ne);
     bool on_device = false;
     {
-        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-        for (int64_t v = 0; v < sp->nvert; v++)
-            for (int c = 0; c < 3; c++) {
-                lo[c] = std::min(lo[c], xyz[(size_t)(3 * v + c)]);
-                hi[c] = std::max(hi[c], xyz[(size_t)(3 * v + c)]);
-            }
+        // the space's bounding box of its vertices
+        const double *lo = sp->lo, *hi = sp->hi;
         double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
         const double sc = ext > 0 ? 2097151.0 / ext : 0.0;
         // (key, element) pairs radix-sorted on the device: a stable sort by
@@ -1736,6 +1726,10 @@ extern "C" int phg_hip_mat_element_operator(phg_hip_mat *A,
     }
     else
         std::iota(iperm.begin(), iperm.end(), 0);
+    // the element maps (PHG numbering), for the host split only
+    HostVec<int32_t> map((size_t)ne * nb);
+    if ((rc = download(map.data(), sp->d_map, (size_t)ne * nb * 4)))
+        return rc;
     mark("downloads");
     // element DOF map in the solver numbering, Dirichlet DOFs -1
     HostVec<int32_t> emap((size_t)ne * nb);

@@ -117,10 +117,6 @@ struct phg_hip_space {
     uint8_t *d_bdry = nullptr;       // [ndof]
     double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};   // bounding box of the vertices
     int batch_refs = 0;              // batches whose graphs read its arrays
-    // host copies of the element maps, kept for the element operator's
-    // layout builder
-    std::vector<int32_t> h_elem_verts, h_map;
-    std::vector<double> h_verts;
 };
 
 struct phg_hip_vec {
